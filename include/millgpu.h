@@ -1,0 +1,129 @@
+/* millgpu — MI355X-native per-substep slurry-plus-media engine (C ABI).
+ *
+ * Drop-in boundary for the reference's mill_core::Simulation step API
+ * (crates/mill-core/src/lib.rs:82-566) and its WASM binding
+ * (crates/mill-wasm/src/lib.rs:15-204). Params cross as the same serde JSON; arrays cross as
+ * caller-owned buffers laid out like the binding's Float32Array copies ([x0, y0, x1, y1, ...]).
+ *
+ * Conventions: every fallible entry point returns 0 on success and a negative MG_E* code on
+ * failure, writing a NUL-terminated message into (err, errlen) when err is non-null. No C++
+ * exception crosses this boundary. A handle is driven by one thread at a time; distinct
+ * handles are independent. The library owns all device memory. */
+#ifndef MILLGPU_H
+#define MILLGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MG_OK 0
+#define MG_E_PARAMS -1   /* JSON parse or Params::validate failure (params.rs:115-530 messages) */
+#define MG_E_DEVICE -2   /* HIP runtime error or no GPU */
+#define MG_E_CAPACITY -3 /* a per-drum scratch capacity was exceeded (see mg_sim_error_bits) */
+#define MG_E_ARG -4      /* invalid argument (null handle, short buffer, bad index) */
+
+typedef struct mg_sim mg_sim;
+typedef struct mg_batch mg_batch;
+
+/* EnergyBudget (lib.rs:61-78), f64 throughout. */
+typedef struct mg_energy_budget {
+  double elapsed_s, shaft_work_j, dem_wall_work_j, fluid_wall_work_j, ball_contact_dissipation_j,
+      fluid_wall_slip_j, fluid_viscous_j, fluid_clamp_removed_j, interface_created_j,
+      delta_mechanical_j, unattributed_j, fluid_wall_angular_impulse;
+} mg_energy_budget;
+
+/* ---- single drum: Simulation::new / with_initial_balls / set_params (lib.rs:126-274) ---- */
+/* Replaces Simulation::new (lib.rs:126) and the wasm constructor (mill-wasm/src/lib.rs:24-34):
+ * params_json == NULL means Params::default(). */
+int mg_sim_create(const char* params_json, mg_sim** out, char* err, size_t errlen);
+/* Replaces Simulation::with_initial_balls (lib.rs:134): xy holds n_balls (x, y) pairs. */
+int mg_sim_create_with_balls(const char* params_json, const float* xy, size_t n_balls, mg_sim** out,
+                             char* err, size_t errlen);
+void mg_sim_destroy(mg_sim* sim);
+/* Replaces Simulation::set_params (lib.rs:270): hot-applies without reseeding. */
+int mg_sim_set_params(mg_sim* sim, const char* params_json, char* err, size_t errlen);
+
+/* ---- stepping (lib.rs:426-509) ---- */
+int mg_sim_step(mg_sim* sim, float dt);             /* Simulation::step (lib.rs:456) */
+int mg_sim_step_fixed(mg_sim* sim);                 /* Simulation::step_fixed (lib.rs:438) */
+int mg_sim_step_fixed_n(mg_sim* sim, uint32_t n);   /* n x step_fixed, one host synchronisation */
+void mg_sim_reset_frame_stats(mg_sim* sim);          /* Simulation::reset_frame_stats (lib.rs:449) */
+float mg_sim_fixed_sub_dt(const mg_sim* sim);        /* Simulation::fixed_sub_dt (lib.rs:426) */
+uint32_t mg_sim_effective_substeps(const mg_sim* sim); /* Params::effective_substeps (params.rs:705) */
+
+/* ---- state copies (Simulation::balls / fluid, lib.rs:287-292; wasm copies :103-138) ---- */
+size_t mg_sim_ball_count(const mg_sim* sim);
+size_t mg_sim_fluid_count(const mg_sim* sim);
+/* Any output pointer may be NULL. x2n/v2n hold 2*count floats. */
+int mg_sim_copy_balls(mg_sim* sim, float* x2n, float* v2n, float* theta, float* omega);
+int mg_sim_copy_fluid(mg_sim* sim, float* x2n, float* v2n, float* dye);
+/* Overwrites the dynamic state (parity harness / state injection). guess2n may be NULL. */
+int mg_sim_set_state(mg_sim* sim, const float* ball_x2n, const float* ball_v2n, const float* theta,
+                     const float* omega, const float* fluid_x2n, const float* fluid_v2n,
+                     const float* guess2n, int has_guess);
+int mg_sim_copy_guess(mg_sim* sim, float* guess2n, int* has_guess);
+/* Sets the drum clock (drum_angle, sim_time) used by the next substep. */
+int mg_sim_set_clock(mg_sim* sim, float drum_angle, double sim_time);
+
+/* ---- read-only getters (lib.rs:276-349) ---- */
+float mg_sim_drum_angle(const mg_sim* sim);
+double mg_sim_sim_time(const mg_sim* sim);
+uint32_t mg_sim_viscosity_iterations(mg_sim* sim);
+float mg_sim_mean_shear_rate_per_s(mg_sim* sim);
+float mg_sim_power_draw_w(mg_sim* sim);
+float mg_sim_torque_nm(mg_sim* sim);
+float mg_sim_collision_rate_per_s(mg_sim* sim);
+float mg_sim_dissipated_power_w(mg_sim* sim);
+float mg_sim_max_substep_displacement_over_diameter(mg_sim* sim);
+uint32_t mg_sim_coupling_clamp_hits(mg_sim* sim);
+/* Last-substep solver statistics as doubles (order documented in milldynamics3_amd/sim.py). */
+int mg_sim_stats(mg_sim* sim, double* out, size_t cap);
+/* StressAccum state: count 4, se 4, hist 28, above 6, cum_wall 1, cum_above 6 (stress.rs:83-99). */
+int mg_sim_stress_state(mg_sim* sim, double* out49);
+/* Impact records of the last substep: 6 doubles each (kind, ball, partner|-1, vn, slip, e). */
+int mg_sim_impacts(mg_sim* sim, double* out, size_t cap_records, size_t* n_records);
+/* Simulation::metrics (lib.rs:370-412) as JSON (Metrics keys this engine computes). */
+int mg_sim_metrics_json(mg_sim* sim, char* buf, size_t cap);
+/* metrics::mixing_index (metrics.rs:506-576); returns 0 and sets *ok = 0 for None. */
+int mg_sim_mixing_index(mg_sim* sim, float* out, int* ok);
+int mg_sim_energy_budget(mg_sim* sim, mg_energy_budget* out);   /* lib.rs:211-229 */
+int mg_sim_reset_energy_budget(mg_sim* sim);                    /* lib.rs:205-208 */
+int mg_sim_set_stress_debounce_factor(mg_sim* sim, double factor); /* lib.rs:244 */
+
+/* ---- additions for this engine ---- */
+/* Whole-state snapshot (state + hidden CG warm start + EMAs + stress accumulator + energy budget). */
+int mg_sim_save_state(mg_sim* sim, void* buf, size_t cap, size_t* needed);
+int mg_sim_load_state(mg_sim* sim, const void* buf, size_t len);
+/* Debug exports of the last substep's step-2 fluid grid (grid.rs layout) and neighbour CSR
+ * (pbf.rs:235-288). meta: dense, min_cx, min_cy, width, height, n_cell_start, n_indices. */
+int mg_sim_debug_grid(mg_sim* sim, int64_t* meta7, uint32_t* cell_start, size_t cs_cap, uint32_t* indices,
+                      size_t idx_cap);
+int mg_sim_debug_neighbors(mg_sim* sim, uint32_t* offsets, size_t off_cap, uint32_t* nbrs, size_t nb_cap,
+                           size_t* n_edges);
+/* Forces the density pass count / CG iteration count of following substeps (0 = adaptive). */
+int mg_sim_force_iterations(mg_sim* sim, uint32_t density_passes, uint32_t cg_iterations);
+uint32_t mg_sim_error_bits(mg_sim* sim);
+
+/* ---- batched drums: one persistent launch per substep over all drums ---- */
+int mg_batch_create(const char* const* params_json, uint32_t n_drums, mg_batch** out, char* err, size_t errlen);
+void mg_batch_destroy(mg_batch* batch);
+uint32_t mg_batch_size(const mg_batch* batch);
+/* Advances every drum by `substeps` of its own fixed substep (step_fixed each). use_graph != 0
+ * replays a captured hipGraph of the launches. */
+int mg_batch_step(mg_batch* batch, uint32_t substeps, int use_graph);
+/* Borrowed per-drum view (valid until the batch is destroyed; do not destroy it). */
+mg_sim* mg_batch_drum(mg_batch* batch, uint32_t index);
+/* Waits for the batch's stream; returns MG_E_CAPACITY if any drum reported an overflow. */
+int mg_batch_sync(mg_batch* batch, char* err, size_t errlen);
+/* Device time (ms) of the last mg_batch_step, measured with HIP events on the batch stream. */
+float mg_batch_last_step_ms(const mg_batch* batch);
+
+const char* mg_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MILLGPU_H */

@@ -1,0 +1,369 @@
+"""ctypes wrapper around ``oracle/_build/libmgoracle.so`` (built by ``oracle/Makefile``).
+
+Test infrastructure only. ``default_params`` restates the serde defaults of the reference's
+``Params`` (crates/mill-core/src/params.rs:62-73, :200-228, :351-376, :419-429, :486-501, plus the
+``#[serde(default = ...)]`` fields at :162-175 and :339) so the oracle can be driven from the same
+JSON the product's C-ABI accepts.
+"""
+from __future__ import annotations
+
+import copy
+import ctypes as C
+import json
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libmgoracle.so")
+_lib = None
+
+MODES = {"ref": 0, "mixed": 1, "fp64": 2}
+
+
+def oracle_lib():
+    """Loads (building first if needed) the oracle shared library."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        subprocess.check_call(["make", "-s", "-C", _HERE])
+    lib = C.CDLL(_LIB_PATH)
+    P = C.c_void_p
+    dp = C.POINTER(C.c_double)
+    lib.orc_create.restype = P
+    lib.orc_create.argtypes = [dp, C.c_uint64, C.c_int, C.POINTER(C.c_float), C.c_size_t]
+    lib.orc_destroy.argtypes = [P]
+    for fn in ("orc_sizes", "orc_scalars", "orc_energy_budget", "orc_stress", "orc_fluid_bins"):
+        getattr(lib, fn).argtypes = [P, dp]
+    lib.orc_step_fixed.argtypes = [P]
+    lib.orc_step.argtypes = [P, C.c_double]
+    lib.orc_advance.argtypes = [P, C.c_double]
+    lib.orc_set_debug.argtypes = [P, C.c_int]
+    lib.orc_force_iters.argtypes = [P, C.c_int, C.c_uint32, C.c_uint32]
+    lib.orc_get_state.argtypes = [P] + [dp] * 8 + [C.POINTER(C.c_int)]
+    lib.orc_set_state.argtypes = [P] + [dp] * 7 + [C.c_int]
+    lib.orc_set_scalars.argtypes = [P, C.c_double, C.c_double]
+    lib.orc_impacts.restype = C.c_int
+    lib.orc_impacts.argtypes = [P, dp, C.c_int]
+    lib.orc_mixing_index.restype = C.c_int
+    lib.orc_mixing_index.argtypes = [P, dp]
+    u32p = C.POINTER(C.c_uint32)
+    i64p = C.POINTER(C.c_int64)
+    lib.orc_dbg_grid.restype = C.c_int
+    lib.orc_dbg_grid.argtypes = [P, i64p, u32p, C.c_int, u32p, C.c_int]
+    lib.orc_dbg_nbrs.restype = C.c_int
+    lib.orc_dbg_nbrs.argtypes = [P, u32p, C.c_int, u32p, C.c_int, i64p]
+    fp = C.POINTER(C.c_float)
+    lib.orc_grid_build.restype = C.c_int
+    lib.orc_grid_build.argtypes = [fp, C.c_size_t, C.c_float, i64p, u32p, C.c_int, u32p, C.c_int]
+    lib.orc_grid_pairs.restype = C.c_int64
+    lib.orc_grid_pairs.argtypes = [fp, C.c_size_t, C.c_float, u32p, C.c_int64]
+    lib.orc_neighbor_lists.restype = C.c_int
+    lib.orc_neighbor_lists.argtypes = [fp, C.c_size_t, C.c_float, u32p, u32p, C.c_int64, i64p]
+    lib.orc_rng.argtypes = [C.c_uint64, C.POINTER(C.c_uint64), fp, C.c_int]
+    lib.orc_sdf_world.argtypes = [dp, C.c_double, C.c_double, C.c_double, dp]
+    lib.orc_toi_swept.restype = C.c_int
+    lib.orc_toi_swept.argtypes = [dp, dp, dp, C.c_double, C.c_double, C.c_double, dp]
+    lib.orc_squeeze_coefficient.restype = C.c_double
+    lib.orc_squeeze_coefficient.argtypes = [C.c_double] * 4
+    lib.orc_resolved_ratio.restype = C.c_double
+    lib.orc_resolved_ratio.argtypes = [C.c_double]
+    _lib = lib
+    return lib
+
+
+_DEFAULTS = {
+    "mill": {"diameter_m": 0.063, "speed_mode": "rpm", "speed_value": 100.0,
+             "direction": "counter_clockwise"},
+    "media": {"ball_diameter_m": 0.002, "fill_fraction": 0.50, "packing_fraction_2d": 0.82,
+              "density_kg_m3": 6000.0, "restitution_ball_ball": 0.7, "restitution_ball_wall": 0.5,
+              "friction_ball_ball": 0.25, "friction_ball_wall": 0.35,
+              "friction_ball_ball_static": 0.25, "friction_ball_wall_static": 0.35,
+              "friction_velocity_scale_m_s": 0.02, "rolling_friction": 0.01},
+    "slurry": {"enabled": True, "fill_fraction": 0.35, "density_kg_m3": 1000.0,
+               "viscosity_pa_s": 0.001, "rheology": "newtonian", "yield_stress_pa": 0.0,
+               "wall_no_slip": 1.0, "ball_no_slip": 1.0, "wettability": 0.6,
+               "surface_tension_n_m": 0.072, "dye_pattern": "left_right"},
+    "lifters": {"count": 0, "height_m": 0.020, "base_width_m": 0.030, "top_width_m": 0.020,
+                "phase_deg": 0.0},
+    "simulation": {"resolution": 40, "substeps": 8, "pbf_iterations": 3, "dem_iterations": 2,
+                   "time_scale": 1.0, "frame_budget_ms": 12.0, "max_balls": 600, "seed": 1,
+                   "coarse_graining_mode": "off", "coarse_graining_k": 1.0},
+}
+
+
+def default_params() -> dict:
+    return copy.deepcopy(_DEFAULTS)
+
+
+def _merge(base: dict, over: dict) -> dict:
+    out = copy.deepcopy(base)
+    for k, v in (over or {}).items():
+        if isinstance(v, dict) and isinstance(out.get(k), dict):
+            out[k] = _merge(out[k], v)
+        else:
+            out[k] = v
+    return out
+
+
+def flatten_params(params) -> tuple[np.ndarray, int]:
+    """Flattens a Params dict (or JSON string) to the oracle's 41-double layout plus the seed."""
+    if isinstance(params, str):
+        params = json.loads(params)
+    p = _merge(_DEFAULTS, params or {})
+    m, me, s, li, si = p["mill"], p["media"], p["slurry"], p["lifters"], p["simulation"]
+    enum = {
+        "rpm": 0, "percent_critical": 1, "counter_clockwise": 0, "clockwise": 1,
+        "newtonian": 0, "bingham": 1, "left_right": 0, "top_bottom": 1, "none": 2,
+        "auto": 0, "off": 1, "manual": 2,
+    }
+    flat = [
+        m["diameter_m"], enum[m["speed_mode"]], m["speed_value"], enum[m["direction"]],
+        me["ball_diameter_m"], me["fill_fraction"], me["packing_fraction_2d"], me["density_kg_m3"],
+        me["restitution_ball_ball"], me["restitution_ball_wall"], me["friction_ball_ball"],
+        me["friction_ball_wall"], me["friction_ball_ball_static"], me["friction_ball_wall_static"],
+        me["friction_velocity_scale_m_s"], me["rolling_friction"],
+        1.0 if s["enabled"] else 0.0, s["fill_fraction"], s["density_kg_m3"], s["viscosity_pa_s"],
+        enum[s["rheology"]], s["yield_stress_pa"], s["wall_no_slip"], s["ball_no_slip"],
+        s["wettability"], s["surface_tension_n_m"], enum[s["dye_pattern"]],
+        li["count"], li["height_m"], li["base_width_m"], li["top_width_m"], li["phase_deg"],
+        si["resolution"], si["substeps"], si["pbf_iterations"], si["dem_iterations"],
+        si["time_scale"], si["frame_budget_ms"], si["max_balls"],
+        enum[si["coarse_graining_mode"]], si["coarse_graining_k"],
+    ]
+    # f32 fields are rounded to f32 exactly as serde parses them into f32.
+    arr = np.array(flat, dtype=np.float64)
+    return arr, int(si["seed"])
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class OracleSim:
+    """The oracle's Simulation (lib.rs:82-566 restated). ``mode``: 'ref' | 'mixed' | 'fp64'."""
+
+    def __init__(self, params=None, mode: str = "mixed", initial_balls=None):
+        lib = oracle_lib()
+        self._lib = lib
+        flat, seed = flatten_params(params)
+        self._flat = flat
+        init_ptr, n_init = None, 0
+        if initial_balls is not None:
+            ib = np.ascontiguousarray(np.asarray(initial_balls, dtype=np.float32).reshape(-1))
+            self._ib = ib
+            init_ptr, n_init = ib.ctypes.data_as(C.POINTER(C.c_float)), ib.size // 2
+        self._h = lib.orc_create(_dp(flat), seed, MODES[mode], init_ptr, n_init)
+        self.mode = mode
+        sz = np.zeros(16)
+        lib.orc_sizes(self._h, _dp(sz))
+        self.n_balls, self.n_fluid = int(sz[0]), int(sz[1])
+        self.effective_substeps, self.effective_resolution = int(sz[2]), int(sz[3])
+        self.true_ball_count = int(sz[4])
+        self.fixed_sub_dt = float(sz[5])
+        self.h, self.fluid_mass = float(sz[6]), float(sz[7])
+        self.ball_radius, self.ball_mass, self.ball_inertia = float(sz[8]), float(sz[9]), float(sz[10])
+        self.omega = float(sz[11])
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._lib.orc_destroy(self._h)
+            self._h = None
+
+    def step_fixed(self, n: int = 1):
+        for _ in range(n):
+            self._lib.orc_step_fixed(self._h)
+
+    def step(self, dt: float):
+        self._lib.orc_step(self._h, dt)
+
+    def advance(self, sub_dt: float):
+        self._lib.orc_advance(self._h, sub_dt)
+
+    def set_debug(self, on=True):
+        self._lib.orc_set_debug(self._h, 1 if on else 0)
+
+    def force_iterations(self, density: int, cg: int):
+        self._lib.orc_force_iters(self._h, 1, density, cg)
+
+    def release_iterations(self):
+        self._lib.orc_force_iters(self._h, 0, 0, 0)
+
+    def state(self) -> dict:
+        nb, nf = self.n_balls, self.n_fluid
+        bx, bv = np.zeros(2 * nb), np.zeros(2 * nb)
+        bth, bom = np.zeros(nb), np.zeros(nb)
+        fx, fv, dye, g = np.zeros(2 * nf), np.zeros(2 * nf), np.zeros(nf), np.zeros(2 * nf)
+        hg = C.c_int(0)
+        self._lib.orc_get_state(self._h, _dp(bx), _dp(bv), _dp(bth), _dp(bom), _dp(fx), _dp(fv),
+                                _dp(dye), _dp(g), C.byref(hg))
+        return {"ball_x": bx.reshape(-1, 2), "ball_v": bv.reshape(-1, 2), "ball_theta": bth,
+                "ball_omega": bom, "fluid_x": fx.reshape(-1, 2), "fluid_v": fv.reshape(-1, 2),
+                "dye": dye, "guess": g.reshape(-1, 2), "has_guess": bool(hg.value)}
+
+    def set_state(self, st: dict):
+        f = lambda k: np.ascontiguousarray(np.asarray(st[k], dtype=np.float64).reshape(-1))
+        arrs = [f("ball_x"), f("ball_v"), f("ball_theta"), f("ball_omega"), f("fluid_x"),
+                f("fluid_v"), f("guess")]
+        self._keep = arrs
+        self._lib.orc_set_state(self._h, *[_dp(a) for a in arrs], 1 if st.get("has_guess") else 0)
+
+    def set_scalars(self, drum_angle: float, sim_time: float):
+        self._lib.orc_set_scalars(self._h, drum_angle, sim_time)
+
+    SCALAR_KEYS = ["drum_angle", "sim_time", "power_draw_w", "torque_nm", "collision_rate_per_s",
+                   "dissipated_power_w", "max_substep_displacement_over_diameter",
+                   "coupling_clamp_hits", "viscosity_iterations", "density_iterations",
+                   "mean_shear_rate_per_s", "fluid_wall_work_j", "dem_wall_work_j",
+                   "dem_dissipated_energy_j", "dem_coupling_work_j", "collision_count",
+                   "n_groups", "n_links", "links_in_range", "n_ball_pairs", "ke_delta_noslip_j",
+                   "ke_delta_drag_j", "ke_delta_buoyancy_j", "ke_delta_viscosity_j",
+                   "ke_delta_clamp_j", "wall_impulse_x", "wall_impulse_y", "wall_angular_impulse"]
+
+    def scalars(self) -> dict:
+        o = np.zeros(32)
+        self._lib.orc_scalars(self._h, _dp(o))
+        return dict(zip(self.SCALAR_KEYS, o.tolist()))
+
+    ENERGY_KEYS = ["elapsed_s", "shaft_work_j", "dem_wall_work_j", "fluid_wall_work_j",
+                   "ball_contact_dissipation_j", "fluid_wall_slip_j", "fluid_viscous_j",
+                   "fluid_clamp_removed_j", "interface_created_j", "delta_mechanical_j",
+                   "unattributed_j", "fluid_wall_angular_impulse"]
+
+    def energy_budget(self) -> dict:
+        o = np.zeros(12)
+        self._lib.orc_energy_budget(self._h, _dp(o))
+        return dict(zip(self.ENERGY_KEYS, o.tolist()))
+
+    def stress(self) -> np.ndarray:
+        o = np.zeros(49)
+        self._lib.orc_stress(self._h, _dp(o))
+        return o
+
+    def impacts(self) -> np.ndarray:
+        cap = 4 * (self.n_balls + 8) + 64
+        o = np.zeros(6 * cap)
+        n = self._lib.orc_impacts(self._h, _dp(o), cap)
+        return o[: 6 * min(n, cap)].reshape(-1, 6)
+
+    def wall_bins(self) -> np.ndarray:
+        o = np.zeros(32)
+        self._lib.orc_fluid_bins(self._h, _dp(o))
+        return o
+
+    def mixing_index(self):
+        o = np.zeros(1)
+        ok = self._lib.orc_mixing_index(self._h, _dp(o))
+        return float(o[0]) if ok else None
+
+    def debug_grid(self):
+        meta = np.zeros(8, dtype=np.int64)
+        cs = np.zeros(4 * self.n_fluid + 1024, dtype=np.uint32)
+        idx = np.zeros(self.n_fluid + 1, dtype=np.uint32)
+        u32 = lambda a: a.ctypes.data_as(C.POINTER(C.c_uint32))
+        ok = self._lib.orc_dbg_grid(self._h, meta.ctypes.data_as(C.POINTER(C.c_int64)), u32(cs),
+                                    cs.size, u32(idx), idx.size)
+        assert ok
+        return {"dense": bool(meta[0]), "min_cx": int(meta[1]), "min_cy": int(meta[2]),
+                "width": int(meta[3]), "height": int(meta[4]),
+                "cell_start": cs[: meta[5]].copy(), "indices": idx[: meta[6]].copy()}
+
+    def debug_neighbors(self):
+        off = np.zeros(self.n_fluid + 2, dtype=np.uint32)
+        nb = np.zeros(64 * self.n_fluid + 64, dtype=np.uint32)
+        sizes = np.zeros(2, dtype=np.int64)
+        u32 = lambda a: a.ctypes.data_as(C.POINTER(C.c_uint32))
+        ok = self._lib.orc_dbg_nbrs(self._h, u32(off), off.size, u32(nb), nb.size,
+                                    sizes.ctypes.data_as(C.POINTER(C.c_int64)))
+        assert ok
+        return off[: sizes[0]].copy(), nb[: sizes[1]].copy()
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _u32(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+def grid_build(points, cell: float):
+    """UniformGrid::build (grid.rs:101-153) over float points; returns the CSR layout."""
+    lib = oracle_lib()
+    pts = np.ascontiguousarray(np.asarray(points, dtype=np.float32).reshape(-1))
+    n = pts.size // 2
+    meta = np.zeros(8, dtype=np.int64)
+    cs = np.zeros(2 * n + 1024, dtype=np.uint32)
+    idx = np.zeros(n + 1, dtype=np.uint32)
+    ok = lib.orc_grid_build(_fp(pts), n, cell, meta.ctypes.data_as(C.POINTER(C.c_int64)), _u32(cs),
+                            cs.size, _u32(idx), idx.size)
+    assert ok
+    return {"dense": bool(meta[0]), "min_cx": int(meta[1]), "min_cy": int(meta[2]),
+            "width": int(meta[3]), "height": int(meta[4]),
+            "cell_start": cs[: meta[5]].copy(), "indices": idx[: meta[6]].copy()}
+
+
+def grid_pairs(points, cell: float) -> np.ndarray:
+    """Candidate pairs in emission order (grid.rs:291-362)."""
+    lib = oracle_lib()
+    pts = np.ascontiguousarray(np.asarray(points, dtype=np.float32).reshape(-1))
+    n = pts.size // 2
+    k = lib.orc_grid_pairs(_fp(pts), n, cell, None, 0)
+    out = np.zeros(2 * max(k, 1), dtype=np.uint32)
+    lib.orc_grid_pairs(_fp(pts), n, cell, _u32(out), k)
+    return out[: 2 * k].reshape(-1, 2)
+
+
+def neighbor_lists(points, h: float):
+    """build_neighbor_lists (pbf.rs:262-288)."""
+    lib = oracle_lib()
+    pts = np.ascontiguousarray(np.asarray(points, dtype=np.float32).reshape(-1))
+    n = pts.size // 2
+    off = np.zeros(n + 1, dtype=np.uint32)
+    cap = 64 * n + 64
+    nb = np.zeros(cap, dtype=np.uint32)
+    ne = np.zeros(1, dtype=np.int64)
+    ok = lib.orc_neighbor_lists(_fp(pts), n, h, _u32(off), _u32(nb), cap,
+                                ne.ctypes.data_as(C.POINTER(C.c_int64)))
+    assert ok
+    return off, nb[: ne[0]].copy()
+
+
+def rng_stream(seed: int, n: int):
+    lib = oracle_lib()
+    u = np.zeros(n, dtype=np.uint64)
+    f = np.zeros(n, dtype=np.float32)
+    lib.orc_rng(seed, u.ctypes.data_as(C.POINTER(C.c_uint64)), _fp(f), n)
+    return u, f
+
+
+def drum_array(radius, omega=0.0, lifters=None):
+    li = lifters or {}
+    return np.array([radius, omega, li.get("count", 0), li.get("height_m", 0.02),
+                     li.get("base_width_m", 0.03), li.get("top_width_m", 0.02),
+                     li.get("phase_deg", 0.0)], dtype=np.float64)
+
+
+def sdf_world(drum, p, angle):
+    lib = oracle_lib()
+    out = np.zeros(3)
+    lib.orc_sdf_world(_dp(drum), float(p[0]), float(p[1]), float(angle), _dp(out))
+    return float(out[0]), out[1:].copy()
+
+
+def toi_swept(drum, x0, x1, a0, a1, radius):
+    lib = oracle_lib()
+    t = np.zeros(1)
+    ok = lib.orc_toi_swept(_dp(drum), _dp(np.asarray(x0, float)), _dp(np.asarray(x1, float)),
+                           a0, a1, radius, _dp(t))
+    return float(t[0]) if ok else None
+
+
+def squeeze_coefficient(k1, k2, h, nu):
+    return oracle_lib().orc_squeeze_coefficient(k1, k2, h, nu)
+
+
+def resolved_ratio(hc):
+    return oracle_lib().orc_resolved_ratio(hc)
