@@ -24,9 +24,14 @@ extern "C" {
 #define MG_E_DEVICE -2   /* HIP runtime error or no GPU */
 #define MG_E_CAPACITY -3 /* a per-drum scratch capacity was exceeded (see mg_sim_error_bits) */
 #define MG_E_ARG -4      /* invalid argument (null handle, short buffer, bad index) */
+#define MG_E_DISABLED -5 /* the drum substep engine is switched off in this build (see mg_drum_engine_enabled) */
 
 typedef struct mg_sim mg_sim;
 typedef struct mg_batch mg_batch;
+
+/* 1 if this build runs drum substeps on the GPU, 0 if the step entry points return
+ * MG_E_DISABLED. No reference counterpart: a build-level switch of this library. */
+int mg_drum_engine_enabled(void);
 
 /* EnergyBudget (lib.rs:61-78), f64 throughout. */
 typedef struct mg_energy_budget {

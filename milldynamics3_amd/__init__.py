@@ -4,6 +4,6 @@
 C ABI declared in ``include/millgpu.h``; ``Batch`` advances many drums with one persistent
 launch per substep. The compute path is ``libmillgpu.so`` (HIP, gfx950); there is no CPU path.
 """
-from .sim import Batch, Simulation, DEFAULT_PARAMS, params_json, lib  # noqa: F401
+from .sim import Batch, Simulation, DEFAULT_PARAMS, params_json, lib, drum_engine_enabled  # noqa: F401
 
-__all__ = ["Simulation", "Batch", "DEFAULT_PARAMS", "params_json", "lib"]
+__all__ = ["Simulation", "Batch", "DEFAULT_PARAMS", "params_json", "lib", "drum_engine_enabled"]

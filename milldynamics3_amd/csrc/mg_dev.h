@@ -2,15 +2,16 @@
 // evaluation order, workgroup reductions and scans with a fixed (deterministic) combination
 // order, and the uniform-grid build/query used by every neighbour stage.
 //
-// This header is compiled only into mg_step.hip with -ffp-contract=off, so a*b+c is never fused
-// (the reference's Rust f32 code never fuses either) and f32 division / sqrt are IEEE-rounded.
+// Every helper is always inlined (the stage kernels make no device-function calls and keep no
+// per-thread arrays), and the file is compiled with -ffp-contract=off so a*b+c is never fused
+// (the reference's Rust f32 code never fuses either); f32 division and sqrt are IEEE-rounded.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include "mg_layout.h"
 
-#define DEV __device__ __forceinline__
+#define DEV __device__ __attribute__((always_inline)) inline
 
 // ---------------------------------------------------------------------------------------------
 // f32 2D vector operations (glam::Vec2 semantics: component-wise, left to right)
@@ -95,7 +96,7 @@ DEV int tid() { return (int)threadIdx.x; }
 DEV int nthr() { return (int)blockDim.x; }
 DEV void bsync() { __syncthreads(); }
 
-__device__ inline double block_sum_d(double v) {
+DEV double block_sum_d(double v) {
   __shared__ double s_part[MG_BLOCK / 64];
   __shared__ double s_out;
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
@@ -112,7 +113,7 @@ __device__ inline double block_sum_d(double v) {
   bsync();
   return r;
 }
-__device__ inline uint32_t block_sum_u(uint32_t v) {
+DEV uint32_t block_sum_u(uint32_t v) {
   __shared__ uint32_t s_part[MG_BLOCK / 64];
   __shared__ uint32_t s_out;
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
@@ -130,7 +131,7 @@ __device__ inline uint32_t block_sum_u(uint32_t v) {
   return r;
 }
 // f32::max fold from 0.0 (exact regardless of order).
-__device__ inline float block_max_f(float v) {
+DEV float block_max_f(float v) {
   __shared__ float s_part[MG_BLOCK / 64];
   __shared__ float s_out;
   for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_down(v, o, 64));
@@ -149,7 +150,7 @@ __device__ inline float block_max_f(float v) {
 }
 
 // In-place exclusive prefix sum over a[0..n) (global memory), one contiguous chunk per thread.
-__device__ inline void block_scan_excl(uint32_t* a, int n) {
+DEV void block_scan_excl(uint32_t* a, int n) {
   __shared__ uint32_t s_part[MG_BLOCK];
   int T = nthr();
   int chunk = (n + T - 1) / T;
@@ -175,16 +176,18 @@ __device__ inline void block_scan_excl(uint32_t* a, int n) {
 
 // Groups record indices 0..nrec-1 by key[r] in 0..K-1 and, within one key, ascending by ord[r].
 // Output: seg[0..K] offsets, perm[0..nrec) record indices. cursor: K scratch entries.
-__device__ inline void ordered_csr(int nrec, const uint32_t* key, const uint32_t* ord, int K, uint32_t* seg,
+DEV void ordered_csr(int nrec, const uint32_t* key, const uint32_t* ord, int K, uint32_t* seg,
                                    uint32_t* perm, uint32_t* cursor) {
   for (int k = tid(); k <= K; k += nthr()) seg[k] = 0;
   bsync();
-  for (int r = tid(); r < nrec; r += nthr()) atomicAdd(&seg[key[r]], 1u);
+  for (int r = tid(); r < nrec; r += nthr())
+    if (key[r] < (uint32_t)K) atomicAdd(&seg[key[r]], 1u);
   bsync();
   block_scan_excl(seg, K + 1);
   for (int k = tid(); k < K; k += nthr()) cursor[k] = seg[k];
   bsync();
   for (int r = tid(); r < nrec; r += nthr()) {
+    if (key[r] >= (uint32_t)K) continue;
     uint32_t p = atomicAdd(&cursor[key[r]], 1u);
     perm[p] = (uint32_t)r;
   }
@@ -208,16 +211,7 @@ __device__ inline void ordered_csr(int nrec, const uint32_t* key, const uint32_t
 // Uniform grid (grid.rs:101-399): counting-sort CSR, Dense layout when the bounding box has at
 // most 2n + 512 cells, otherwise the Sparse layout (occupied cells only, lexicographic keys).
 // ---------------------------------------------------------------------------------------------
-struct GridView {
-  const uint32_t* start;
-  const uint32_t* idx;
-  const uint64_t* skeys;  // sparse keys (sorted), or null for dense
-  int32_t min_cx, min_cy;
-  int64_t W, H;
-  int32_t ncells;  // dense: W*H; sparse: number of occupied cells; 0: empty grid
-  int32_t dense;
-  float cell;
-};
+
 
 DEV uint64_t pack_key(int32_t cx, int32_t cy) {
   return ((uint64_t)((uint32_t)cx ^ 0x80000000u) << 32) | (uint64_t)((uint32_t)cy ^ 0x80000000u);
@@ -253,27 +247,22 @@ DEV void cell_of(float2 p, float c, int32_t& cx, int32_t& cy) {  // grid.rs:280-
   cy = sat_i32(floorf(p.y / c));
 }
 
-// for_each_near (grid.rs:371-399): the up-to-9 distinct cells around p, dy-major order.
-// Fills ranges in visiting order; returns the number of non-empty ranges.
-DEV int grid_near(const GridView& g, float2 p, uint32_t* rs, uint32_t* re) {
-  if (!vfinite(p) || g.ncells == 0) return 0;
-  int32_t cx, cy;
-  cell_of(p, g.cell, cx, cy);
-  int32_t kx[9], ky[9];
-  int nk = 0;
-  for (int dy = -1; dy <= 1; dy++)
-    for (int dx = -1; dx <= 1; dx++) {
-      int32_t ax = sadd_i32(cx, dx), ay = sadd_i32(cy, dy);
-      bool dup = false;
-      for (int q = 0; q < nk; q++) dup |= (kx[q] == ax && ky[q] == ay);
-      if (!dup) { kx[nk] = ax; ky[nk] = ay; nk++; }
-    }
-  int nr = 0;
-  for (int q = 0; q < nk; q++) {
-    uint32_t s, e;
-    if (grid_range(g, kx[q], ky[q], s, e)) { rs[nr] = s; re[nr] = e; nr++; }
+// for_each_near (grid.rs:371-399): the up-to-9 distinct cells around a point, visited dy-major
+// (dy = -1..1 outer, dx = -1..1 inner). Cell q (0..8) is offset (q % 3 - 1, q / 3 - 1); a cell
+// that saturates onto an earlier one at the i32 boundary is skipped, as the reference's dedupe
+// does. No per-thread arrays: callers loop q = 0..8 and call near_cell for each.
+DEV bool near_dup(int32_t cx, int32_t cy, int q) {
+  int dx = q % 3 - 1, dy = q / 3 - 1;
+  int32_t ax = sadd_i32(cx, dx), ay = sadd_i32(cy, dy);
+  for (int p = 0; p < q; p++) {
+    int ex = p % 3 - 1, ey = p / 3 - 1;
+    if (sadd_i32(cx, ex) == ax && sadd_i32(cy, ey) == ay) return true;
   }
-  return nr;
+  return false;
+}
+DEV bool near_cell(const GridView& g, int32_t cx, int32_t cy, int q, uint32_t& s, uint32_t& e) {
+  if (near_dup(cx, cy, q)) return false;
+  return grid_range(g, sadd_i32(cx, q % 3 - 1), sadd_i32(cy, q / 3 - 1), s, e);
 }
 
 struct GridScratch {
@@ -289,7 +278,7 @@ struct GridScratch {
 };
 
 // Block-wide grid build over pts[0..n). Every thread must call it. Returns the view.
-__device__ inline GridView grid_build(const float2* pts, int n, float cell, const GridScratch& G, uint32_t* err) {
+DEV GridView grid_build(const float2* pts, int n, float cell, const GridScratch& G, uint32_t* err) {
   __shared__ int32_t s_mnx, s_mny, s_mxx, s_mxy;
   __shared__ uint32_t s_cnt;
   if (tid() == 0) { s_mnx = INT32_MAX; s_mny = INT32_MAX; s_mxx = INT32_MIN; s_mxy = INT32_MIN; s_cnt = 0; }

@@ -72,6 +72,27 @@ struct MgImpact {
   float vn, slip, e;
 };
 
+// View of a built uniform grid (dense or sparse CSR); persisted between stage kernels.
+struct GridView {
+  const uint32_t* start;
+  const uint32_t* idx;
+  const uint64_t* skeys;  // sparse keys (sorted), or null for dense
+  int32_t min_cx, min_cy;
+  int64_t W, H;
+  int32_t ncells;  // dense: W*H; sparse: number of occupied cells; 0: empty grid
+  int32_t dense;
+  float cell;
+};
+
+// Per-drum values carried from one stage kernel of a substep to the next.
+struct MgStage {
+  GridView gball, gA, gB, gC;
+  uint32_t ngroups, nlinks, in_range, npairs;
+  double ke_mark, e_pred, cw;
+  float maxd, maxs;
+  double hist[28];  // stress histogram of the current substep's impacts
+};
+
 struct MgDrum {
   // ---- sizes and capacities
   int32_t nf, nb;
@@ -153,6 +174,7 @@ struct MgDrum {
   double* tbl_time;
   // accumulators
   MgScalars* sc;
+  MgStage* st;
   // per-substep schedule (array of MgSubstepIn, indexed by the launch's substep slot)
   const MgSubstepIn* sched;
 };

@@ -69,6 +69,8 @@ def lib():
     fp = C.POINTER(C.c_float)
     dp = C.POINTER(C.c_double)
     u32p = C.POINTER(C.c_uint32)
+    L.mg_drum_engine_enabled.argtypes = []
+    L.mg_drum_engine_enabled.restype = C.c_int
     L.mg_sim_create.argtypes = [ch, C.POINTER(P), ch, sz]
     L.mg_sim_create_with_balls.argtypes = [ch, fp, sz, C.POINTER(P), ch, sz]
     L.mg_sim_destroy.argtypes = [P]
@@ -141,6 +143,9 @@ def _check(rc, err=None, what="call"):
         msg = err.value.decode() if err is not None and err.value else f"{what} failed ({rc})"
         if rc == -1:
             raise ValueError(msg)
+        if rc == -5:
+            raise RuntimeError(f"{what}: the drum substep engine is switched off in this build "
+                               "(an earlier form faulted on the GPU; cause not yet shown)")
         raise RuntimeError(msg)
 
 
@@ -410,3 +415,8 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def drum_engine_enabled() -> bool:
+    """True if this build runs drum substeps on the GPU (``mg_drum_engine_enabled``)."""
+    return bool(lib().mg_drum_engine_enabled())
