@@ -1,0 +1,176 @@
+#!/usr/bin/env python3
+"""Benchmark driver: prints one JSON line.
+
+Headline metric (drum-substeps/s, batched FP32 vs plain): the drum substep engine is switched off
+in this build (an earlier form faulted on the GPU and the cause has not been shown), so the
+headline value is reported as null with the reason, never estimated.
+
+Measured on the GPU: the kernel-isolation workload of the same fluid hot path, a periodic box of
+side x side PBF particles (default 1024^2 = 1,048,576): per step the grid build + cell reorder,
+four density passes, velocity update, density recompute and the implicit viscosity CG to 1e-6
+relative residual, FP32 state with FP64 CG reductions, state resident in HBM. Reported as
+particle-steps/s and as algorithmic HBM GB/s from the canonical per-particle byte counts
+(B = 226 + 32 n_d + 84 n_cg bytes per particle per step).
+
+roofline: the dominant kernel by total time, its algorithmic bytes per launch divided by its
+average launch time from HIP events on the box's stream over the timed steps.
+cpu_baseline: the C++ restatement of the reference (oracle/, 1 core) on a bounded sample: C2 drum
+substeps (headline unit) and, inside kernel_isolation, box steps.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "drum-substeps/s (batched FP32 vs plain) at ≤1% power err; p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+# canonical algorithmic bytes per particle (FP32 state) per launch of each timed kernel class:
+# grid build + reorder (predict 40, bin 16, scan 2, reorder 80), density pass A (R x, W lambda),
+# pass B (R x, lambda; W x), velocity + density recompute (12 + v update folded in: R x, x0; W v, rho),
+# CG set-up 52, operator apply (R x, rho, p; W Ap) 28, vector update 56 (split 32 + 24 here)
+CLASS_NAMES = ["grid+reorder", "density_A", "density_B", "velocity+density", "cg_setup", "cg_apply",
+               "cg_update", "cg_direction"]
+CLASS_BYTES = [138.0, 12.0, 20.0, 28.0, 52.0, 28.0, 32.0, 24.0]
+C2_PARAMS = {"mill": {"diameter_m": 0.1397, "speed_mode": "percent_critical", "speed_value": 75.0},
+             "media": {"density_kg_m3": 7800.0},
+             "simulation": {"resolution": 100, "max_balls": 2000}}
+
+
+def cpu_count_used() -> int:
+    return 1
+
+
+def cpu_baseline_drum(seconds: float) -> dict:
+    from oracle import oracle as orc
+    s = orc.OracleSim(C2_PARAMS, "ref")
+    s.step_fixed(1)
+    n = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        s.step_fixed(1)
+        n += 1
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "drum-substeps/s", "cores": cpu_count_used(), "kind": "port",
+            "sample": f"{n} substeps of C2 (2000 discs, 12697 fluid, D=139.7 mm) on the C++ restatement "
+                      f"of the reference (f32), 1 core, {dt:.1f} s"}
+
+
+def cpu_baseline_box(cfg: dict, seconds: float) -> dict:
+    from oracle import oracle as orc
+    s = orc.C3Oracle(cfg, "mixed")
+    n = 0
+    t0 = time.perf_counter()
+    while n == 0 or time.perf_counter() - t0 < seconds:
+        s.step()
+        n += 1
+    dt = time.perf_counter() - t0
+    npart = cfg["side"] ** 2
+    return {"value": n * npart / dt, "unit": "particle-steps/s", "cores": cpu_count_used(), "kind": "port",
+            "sample": f"{n} steps of the {cfg['side']}^2 periodic box on the C++ restatement (f32, f64 CG dots), "
+                      f"1 core, {dt:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--side", type=int, default=1024)
+    ap.add_argument("--mu", type=float, default=1.0e-3)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(drums and boxes shard as independent units, one process per GPU)", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    from milldynamics3_amd import drum_engine_enabled
+    from milldynamics3_amd.box import Box
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        return 1
+    torch.cuda.set_device(0)
+
+    cfg = {"side": args.side, "mu": args.mu}
+    box = Box(cfg)
+    n = box.n
+    for _ in range(args.warmup):
+        box.step()
+    box._L.mg_box_set_timing(box._h, 1)
+    torch.cuda.synchronize()
+    n_cg = 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        box.step()
+        n_cg += box.stats()["cg_iterations"]
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    import ctypes as C
+    ms = np.zeros(8)
+    launches = np.zeros(8, dtype=np.uint64)
+    box._L.mg_box_kernel_times.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.c_int]
+    box._L.mg_box_kernel_times(box._h, ms.ctypes.data_as(C.POINTER(C.c_double)),
+                               launches.ctypes.data_as(C.POINTER(C.c_uint64)), 8)
+    ms_per_step = 1000.0 * wall / args.steps
+    n_d = 4
+    avg_cg = n_cg / args.steps
+    bytes_per_step = n * (226.0 + 32.0 * n_d + 84.0 * avg_cg)
+    gbs = bytes_per_step / (ms_per_step * 1e-3) / 1e9
+    k = int(np.argmax(ms[1:])) + 1  # dominant kernel (grid build is several launches together)
+    avg_launch_ms = ms[k] / max(int(launches[k]), 1)
+    achieved = CLASS_BYTES[k] * n / (avg_launch_ms * 1e-3) / 1e9
+    kernels = {CLASS_NAMES[i]: {"total_ms": round(float(ms[i]), 4), "launches": int(launches[i]),
+                                "avg_us": round(1000.0 * float(ms[i]) / max(int(launches[i]), 1), 2)}
+               for i in range(8)}
+    roofline = {"bound": "hbm", "kernel": CLASS_NAMES[k], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "bytes_per_launch": CLASS_BYTES[k] * n}
+    iso = {"metric": "particle-steps/s", "value": round(n * args.steps / wall, 1), "ms_per_step": round(ms_per_step, 4),
+           "hbm_gbs_algorithmic": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+           "cg_iterations_per_step": avg_cg, "density_passes": n_d,
+           "config": {"workload": f"periodic PBF box {args.side}x{args.side}", "particles": n, "mu": args.mu,
+                      "dtype": "f32 state, f64 CG reductions"},
+           "kernels": kernels}
+    headline_cpu = None
+    if not args.no_cpu:
+        headline_cpu = cpu_baseline_drum(args.cpu_seconds)
+        iso["cpu_baseline"] = cpu_baseline_box(cfg, args.cpu_seconds)
+    line = {
+        "metric": METRIC,
+        "value": None,
+        "unit": "drum-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "C2 single large drum (2000 discs, resolution 100): not measured",
+                   "status": "drum substep engine switched off in this build (earlier form faulted on the "
+                             "GPU; cause not shown); measured workload: kernel_isolation",
+                   "drum_engine_enabled": bool(drum_engine_enabled())},
+        "roofline": roofline,
+        "cpu_baseline": headline_cpu,
+        "kernel_isolation": iso,
+    }
+    print(json.dumps(line), flush=True)
+    box.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

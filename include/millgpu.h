@@ -128,6 +128,50 @@ float mg_batch_last_step_ms(const mg_batch* batch);
 
 const char* mg_version(void);
 
+
+/* ---- Periodic-box PBF workload: kernel isolation of the fluid hot path ---------------------
+ * No reference counterpart (the reference has no periodic domain). One step applies the
+ * reference's fluid arithmetic to a periodic box with minimum-image pair vectors: predict
+ * (pbf.rs:1225-1235), grid + neighbour pairs (grid.rs:101-399, pbf.rs:262-288), `passes` density
+ * passes (pbf.rs:1257-1334), v = (x - x0)/dt (pbf.rs:1656-1674), density recompute
+ * (pbf.rs:2136-2144) and the implicit viscosity CG (pbf.rs:312-480) to tol * |b|. */
+typedef struct mg_box mg_box;
+typedef struct mg_box_config {
+  uint32_t side;   /* side x side particles on a square lattice */
+  float dx;        /* lattice spacing (m); kernel radius h = 2 dx; box side L = side * dx */
+  float rest;      /* rest density */
+  float mu;        /* viscosity (Pa s); 0 skips the CG */
+  float dt;        /* step (s) */
+  uint32_t passes; /* density passes per step */
+  float tol;       /* the CG stops at tol * |b| */
+  uint32_t max_cg; /* CG iteration cap */
+  uint64_t seed;   /* Rng seed of the initial state (rng.rs) */
+} mg_box_config;
+
+/* h, L, cell width, particle mass, cells per side. */
+int mg_box_geometry(const mg_box_config* cfg, double* out5);
+int mg_box_create(const mg_box_config* cfg, mg_box** out, char* err, size_t errlen);
+void mg_box_destroy(mg_box* box);
+size_t mg_box_count(const mg_box* box);
+/* Runs `steps` steps and waits for them; mg_box_last_step_ms is their wall time on the stream. */
+int mg_box_step(mg_box* box, uint32_t steps);
+float mg_box_last_step_ms(const mg_box* box);
+/* Positions and velocities in the original particle order (2n floats each; either may be null). */
+int mg_box_copy_state(mg_box* box, float* x2n, float* v2n);
+int mg_box_set_state(mg_box* box, const float* x2n, const float* v2n);
+/* Density at the final positions of the last step, original order. */
+int mg_box_density(mg_box* box, float* out_n);
+/* CG iterations, final residual norm and |b| of the last step. */
+int mg_box_stats(mg_box* box, double* out3);
+/* Neighbour CSR of the last step in traversal order (original ids); returns the edge count,
+ * copying the entries only when they fit `cap`. */
+int64_t mg_box_neighbors(mg_box* box, uint32_t* off_n1, uint32_t* nb, int64_t cap);
+/* Per-kernel timing with HIP events on the box's stream (resets the totals). Classes: 0 grid build
+ * and reorder (all its launches together), 1 density pass A, 2 density pass B, 3 velocity +
+ * density recompute, 4 CG set-up, 5 CG operator apply, 6 CG vector update, 7 CG direction update. */
+int mg_box_set_timing(mg_box* box, int on);
+int mg_box_kernel_times(mg_box* box, double* ms, uint64_t* launches, int nclass);
+
 #ifdef __cplusplus
 }
 #endif

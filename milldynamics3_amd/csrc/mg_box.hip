@@ -1,0 +1,921 @@
+// Periodic-box PBF workload on the GPU (kernel isolation of the fluid hot path).
+//
+// One step applies the reference's fluid arithmetic to a box of side L with minimum-image pair
+// vectors and wrapped positions:
+//   bx_predict            x0 = x, x += v dt, cell key, cell histogram      pbf.rs:1225-1235, grid.rs:101-226
+//   bx_scan_*             exclusive scan of the histogram -> cell starts   grid.rs:160-200
+//   bx_scatter/bx_cellsort/bx_gather
+//                         stable counting sort: cell order, ascending particle id inside a
+//                         cell, and the particle arrays physically reordered by cell so every
+//                         neighbour gather below reads contiguous memory
+//   bx_lambda, bx_dp      density constraint passes (rho, C, lambda; dp, x)  pbf.rs:1257-1334
+//   bx_vel_density        v = (x - x0)/dt, density at the final positions    pbf.rs:1656-1674, 2136-2144
+//   bx_cg_*               implicit viscosity (I + dt L) v = v, warm start v pbf.rs:312-480
+// Pairs are the 3x3-cell candidates with |d|^2 <= h^2 at the predicted positions (the reference's
+// neighbour lists, pbf.rs:262-288), visited per particle in the reference's order: cells
+// (-1,-1), (-1,0), (0,-1), own, (1,0), (1,1), (0,1), (-1,1), (1,-1), ascending id within a cell.
+// The design is matrix-free: no per-edge array is stored; the Morris weights are recomputed in
+// every Laplacian application from x and rho. CG dot products and norms are FP64, reduced in a
+// fixed order (per-block tree, then one block over the partials), so results are deterministic.
+//
+// CDNA4 mapping: 256-thread workgroups (4 wave64s), one particle per lane; the blockIdx -> tile map
+// gives each of the 8 XCDs one contiguous run of cell-sorted particles so the halo rows of
+// neighbouring tiles are read from the same XCD's L2.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/millgpu.h"
+#include "mg_common.h"
+
+#define BX_BLOCK 256
+#define BX_SCAN_ITEMS 16
+#define BX_SCAN_CHUNK (BX_BLOCK * BX_SCAN_ITEMS)
+#define BX_CG_CHUNK 4
+// kernel classes of the optional per-kernel timing (mg_box_kernel_times)
+#define BX_NCLASS 8
+enum { BXT_GRID = 0, BXT_LAMBDA, BXT_DP, BXT_VEL, BXT_CGINIT, BXT_SPMV, BXT_UPDATE, BXT_P };
+#define BXK extern "C" __global__ void __launch_bounds__(BX_BLOCK)
+#define BDEV __device__ __attribute__((always_inline)) inline
+
+// Geometry and constants of one box, passed by value to every kernel.
+struct BoxGeo {
+  uint32_t n, ncx, nblk, xcd_chunk;
+  float h, h2, L, halfL, cw, mass, rest, dt, k6, ks, w0, m_over_rest, mu2m, eta2;
+};
+
+// Device scalars of the CG loop (pbf.rs:423-480).
+struct BoxScal {
+  double bnorm, target, rs_old, rs_new, pap, res;
+  float alpha, beta;
+  uint32_t it, done, max_cg, pad;
+};
+
+// ---------------------------------------------------------------------------------------------
+// device helpers (f32, no contraction: compiled with -ffp-contract=off)
+// ---------------------------------------------------------------------------------------------
+BDEV float2 f2(float x, float y) { return make_float2(x, y); }
+BDEV float2 add2(float2 a, float2 b) { return f2(a.x + b.x, a.y + b.y); }
+BDEV float2 sub2(float2 a, float2 b) { return f2(a.x - b.x, a.y - b.y); }
+BDEV float2 mul2(float2 a, float s) { return f2(a.x * s, a.y * s); }  // Vec2 * f32
+BDEV float2 smul2(float s, float2 a) { return f2(s * a.x, s * a.y); } // f32 * Vec2
+BDEV float2 div2(float2 a, float s) { return f2(a.x / s, a.y / s); }
+BDEV float dot2(float2 a, float2 b) { return a.x * b.x + a.y * b.y; }
+BDEV float len2(float2 a) { return a.x * a.x + a.y * a.y; }
+BDEV float wrapf(float a, float L) {  // f32::rem_euclid
+  float r = fmodf(a, L);
+  return r < 0.0f ? r + fabsf(L) : r;
+}
+BDEV float mi1(float d, const BoxGeo& g) {
+  if (d > g.halfL) return d - g.L;
+  if (d < -g.halfL) return d + g.L;
+  return d;
+}
+BDEV float2 mi2(float2 d, const BoxGeo& g) { return f2(mi1(d.x, g), mi1(d.y, g)); }
+BDEV uint32_t cell1(float a, const BoxGeo& g) {
+  float q = floorf(a / g.cw);
+  if (!(q >= 0.0f)) return 0u;
+  if (q >= (float)(g.ncx - 1u)) return g.ncx - 1u;
+  return (uint32_t)q;
+}
+BDEV float poly6(float r2, const BoxGeo& g) {  // pbf.rs:119-126
+  if (r2 >= g.h2 || r2 < 0.0f) return 0.0f;
+  float t = g.h2 - r2;
+  return g.k6 * t * t * t;
+}
+BDEV float2 spiky(float2 d, float r, const BoxGeo& g) {  // pbf.rs:131-137
+  if (r <= 1e-9f || r >= g.h) return f2(0.0f, 0.0f);
+  float coeff = g.ks * (g.h - r) * (g.h - r);
+  return mul2(div2(d, r), coeff);
+}
+// XCD-aware tile index: hardware dispatches consecutive workgroups round-robin over the 8 XCDs;
+// remap so XCD x processes tiles [x*chunk, (x+1)*chunk).
+BDEV uint32_t tile_id(const BoxGeo& g) {
+  const uint32_t b = blockIdx.x;
+  if (g.xcd_chunk == 0u) return b;  // tile count not a multiple of 8: identity map
+  return (b & 7u) * g.xcd_chunk + (b >> 3);
+}
+// Candidate cell o (0..8) of cell (cx, cy) in the reference's per-particle order, wrapped.
+BDEV uint32_t nbr_cell(uint32_t cx, uint32_t cy, int o, uint32_t ncx) {
+  const int ox = o == 0 ? -1 : (o == 1 ? -1 : (o == 2 ? 0 : (o == 3 ? 0 : (o == 4 ? 1 : (o == 5 ? 1 : (o == 6 ? 0 : (o == 7 ? -1 : 1)))))));
+  const int oy = o == 0 ? -1 : (o == 1 ? 0 : (o == 2 ? -1 : (o == 3 ? 0 : (o == 4 ? 0 : (o == 5 ? 1 : (o == 6 ? 1 : (o == 7 ? 1 : -1)))))));
+  uint32_t ax = (uint32_t)((int)cx + ox + (int)ncx) % ncx;
+  uint32_t ay = (uint32_t)((int)cy + oy + (int)ncx) % ncx;
+  return ay * ncx + ax;
+}
+// Fixed-order workgroup sum of one double per lane; lane 0 of the block gets the total.
+BDEV double block_sum(double v) {
+  __shared__ double s_w[BX_BLOCK / 64];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) s_w[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < BX_BLOCK / 64; k++) r += s_w[k];
+  __syncthreads();
+  return r;
+}
+
+// ---------------------------------------------------------------------------------------------
+// grid build + reorder
+// ---------------------------------------------------------------------------------------------
+BXK bx_fill_u32(uint32_t* __restrict__ a, uint32_t n, uint32_t v) {
+  uint32_t i = blockIdx.x * BX_BLOCK + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+BXK bx_predict(BoxGeo g, float2* __restrict__ X, const float2* __restrict__ V, float2* __restrict__ X0,
+               uint32_t* __restrict__ key, uint32_t* __restrict__ count) {
+  uint32_t i = blockIdx.x * BX_BLOCK + threadIdx.x;
+  if (i >= g.n) return;
+  float2 x = X[i], v = V[i];
+  X0[i] = x;
+  x = f2(wrapf(x.x + v.x * g.dt, g.L), wrapf(x.y + v.y * g.dt, g.L));
+  X[i] = x;
+  uint32_t k = cell1(x.y, g) * g.ncx + cell1(x.x, g);
+  key[i] = k;
+  atomicAdd(&count[k], 1u);
+}
+
+// Exclusive scan of a[0..m) in three launches: chunk totals, scan of the totals, chunk scan.
+BDEV uint32_t block_excl_scan(uint32_t v, uint32_t& total) {
+  __shared__ uint32_t s_w[BX_BLOCK / 64];
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t y = __shfl_up(x, o, 64);
+    if (l >= o) x += y;
+  }
+  if (l == 63) s_w[w] = x;
+  __syncthreads();
+  uint32_t before = 0, tot = 0;
+  for (int k = 0; k < BX_BLOCK / 64; k++) {
+    uint32_t s = s_w[k];
+    if (k < w) before += s;
+    tot += s;
+  }
+  __syncthreads();
+  total = tot;
+  return before + x - v;
+}
+BXK bx_scan_totals(const uint32_t* __restrict__ a, uint32_t m, uint32_t* __restrict__ bsum) {
+  const uint32_t base = blockIdx.x * BX_SCAN_CHUNK + threadIdx.x * BX_SCAN_ITEMS;
+  uint32_t s = 0;
+  for (int k = 0; k < BX_SCAN_ITEMS; k++) s += base + k < m ? a[base + k] : 0u;
+  uint32_t tot;
+  block_excl_scan(s, tot);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+BXK bx_scan_top(uint32_t* __restrict__ bsum, uint32_t nb) {  // one block, nb <= BX_SCAN_CHUNK
+  const uint32_t base = threadIdx.x * BX_SCAN_ITEMS;
+  uint32_t s = 0;
+  for (int k = 0; k < BX_SCAN_ITEMS; k++) s += base + k < nb ? bsum[base + k] : 0u;
+  uint32_t tot;
+  uint32_t off = block_excl_scan(s, tot);
+  for (int k = 0; k < BX_SCAN_ITEMS; k++)
+    if (base + k < nb) {
+      uint32_t t = bsum[base + k];
+      bsum[base + k] = off;
+      off += t;
+    }
+}
+BXK bx_scan_apply(uint32_t* __restrict__ a, uint32_t m, const uint32_t* __restrict__ bsum) {
+  const uint32_t base = blockIdx.x * BX_SCAN_CHUNK + threadIdx.x * BX_SCAN_ITEMS;
+  uint32_t s = 0;
+  for (int k = 0; k < BX_SCAN_ITEMS; k++) s += base + k < m ? a[base + k] : 0u;
+  uint32_t tot;
+  uint32_t off = block_excl_scan(s, tot) + bsum[blockIdx.x];
+  for (int k = 0; k < BX_SCAN_ITEMS; k++)
+    if (base + k < m) {
+      uint32_t t = a[base + k];
+      a[base + k] = off;
+      off += t;
+    }
+}
+BXK bx_scatter(BoxGeo g, const uint32_t* __restrict__ key, const uint32_t* __restrict__ start,
+               uint32_t* __restrict__ cursor, uint32_t* __restrict__ slot) {
+  uint32_t i = blockIdx.x * BX_BLOCK + threadIdx.x;
+  if (i >= g.n) return;
+  uint32_t k = key[i];
+  slot[start[k] + atomicAdd(&cursor[k], 1u)] = i;
+}
+// One lane per cell: order the cell's slots by particle id (cells hold a handful of particles).
+BXK bx_cellsort(uint32_t ncell, const uint32_t* __restrict__ start, uint32_t* __restrict__ slot,
+                const uint32_t* __restrict__ id) {
+  uint32_t c = blockIdx.x * BX_BLOCK + threadIdx.x;
+  if (c >= ncell) return;
+  const uint32_t s = start[c], e = start[c + 1];
+  for (uint32_t a = s + 1; a < e; a++) {
+    uint32_t v = slot[a], iv = id[v], b = a;
+    while (b > s && id[slot[b - 1]] > iv) {
+      slot[b] = slot[b - 1];
+      b--;
+    }
+    slot[b] = v;
+  }
+}
+BXK bx_gather(BoxGeo g, const uint32_t* __restrict__ slot, const float2* __restrict__ X, const float2* __restrict__ X0,
+              const float2* __restrict__ V, const uint32_t* __restrict__ id, float2* __restrict__ Xb,
+              float2* __restrict__ XPb, float2* __restrict__ X0b, float2* __restrict__ Vb, uint32_t* __restrict__ idb) {
+  uint32_t q = blockIdx.x * BX_BLOCK + threadIdx.x;
+  if (q >= g.n) return;
+  uint32_t i = slot[q];
+  float2 x = X[i];
+  Xb[q] = x;
+  XPb[q] = x;
+  X0b[q] = X0[i];
+  Vb[q] = V[i];
+  idb[q] = id[i];
+}
+
+// ---------------------------------------------------------------------------------------------
+// density passes (pbf.rs:1257-1334)
+// ---------------------------------------------------------------------------------------------
+BXK bx_lambda(BoxGeo g, const float2* __restrict__ XP, const float2* __restrict__ X, const uint32_t* __restrict__ start,
+              float* __restrict__ LAM) {
+  const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
+  if (i >= g.n) return;
+  const float2 xpi = XP[i], xi = X[i];
+  const uint32_t cx = cell1(xpi.x, g), cy = cell1(xpi.y, g);
+  float rho = g.w0;
+  float2 gs = f2(0.0f, 0.0f);
+  float sgs = 0.0f;
+  for (int o = 0; o < 9; o++) {
+    const uint32_t k = nbr_cell(cx, cy, o, g.ncx);
+    const uint32_t s = start[k], e = start[k + 1];
+    for (uint32_t j = s; j < e; j++) {
+      if (j == i) continue;
+      if (len2(mi2(sub2(xpi, XP[j]), g)) > g.h2) continue;
+      const float2 d = mi2(sub2(xi, X[j]), g);
+      const float r2 = len2(d);
+      rho += g.mass * poly6(r2, g);
+      const float r = sqrtf(r2);
+      const float2 gr = mul2(spiky(d, r, g), g.m_over_rest);
+      gs = add2(gs, gr);
+      sgs += len2(gr);
+    }
+  }
+  const float c = fmaxf(rho / g.rest - 1.0f, 0.0f);
+  sgs += len2(gs);
+  LAM[i] = -c / (sgs + 200.0f);
+}
+BXK bx_dp(BoxGeo g, const float2* __restrict__ XP, const float2* __restrict__ X, const uint32_t* __restrict__ start,
+          const float* __restrict__ LAM, float2* __restrict__ Xout) {
+  const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
+  if (i >= g.n) return;
+  const float2 xpi = XP[i], xi = X[i];
+  const float li = LAM[i];
+  const uint32_t cx = cell1(xpi.x, g), cy = cell1(xpi.y, g);
+  float2 s = f2(0.0f, 0.0f);
+  for (int o = 0; o < 9; o++) {
+    const uint32_t k = nbr_cell(cx, cy, o, g.ncx);
+    const uint32_t cs = start[k], ce = start[k + 1];
+    for (uint32_t j = cs; j < ce; j++) {
+      if (j == i) continue;
+      if (len2(mi2(sub2(xpi, XP[j]), g)) > g.h2) continue;
+      const float2 d = mi2(sub2(xi, X[j]), g);
+      const float r = sqrtf(len2(d));
+      s = add2(s, smul2(li + LAM[j], spiky(d, r, g)));
+    }
+  }
+  const float2 dp = mul2(s, g.m_over_rest);
+  Xout[i] = f2(wrapf(xi.x + dp.x, g.L), wrapf(xi.y + dp.y, g.L));
+}
+
+// ---------------------------------------------------------------------------------------------
+// velocity, density recompute, implicit viscosity
+// ---------------------------------------------------------------------------------------------
+BXK bx_vel_density(BoxGeo g, const float2* __restrict__ XP, const float2* __restrict__ X, const float2* __restrict__ X0,
+                   const uint32_t* __restrict__ start, float2* __restrict__ V, float* __restrict__ DENS,
+                   double* __restrict__ part) {
+  const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
+  double vv = 0.0;
+  if (i < g.n) {
+    const float2 xpi = XP[i], xi = X[i];
+    const float2 v = div2(mi2(sub2(xi, X0[i]), g), g.dt);
+    V[i] = v;
+    vv = (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+    const uint32_t cx = cell1(xpi.x, g), cy = cell1(xpi.y, g);
+    float rho = g.w0;
+    for (int o = 0; o < 9; o++) {
+      const uint32_t k = nbr_cell(cx, cy, o, g.ncx);
+      const uint32_t s = start[k], e = start[k + 1];
+      for (uint32_t j = s; j < e; j++) {
+        if (j == i) continue;
+        if (len2(mi2(sub2(xpi, XP[j]), g)) > g.h2) continue;
+        rho += g.mass * poly6(len2(mi2(sub2(xi, X[j]), g)), g);
+      }
+    }
+    DENS[i] = rho;
+  }
+  const double t = block_sum(vv);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// (L v)_i = Σ_j 4 c_ij e_ij (e_ij · (v_i - v_j)) with the Morris weight c_ij (pbf.rs:312-377).
+BDEV float2 lap_row(const BoxGeo& g, uint32_t i, const float2* __restrict__ XP, const float2* __restrict__ X,
+                    const float* __restrict__ DENS, const uint32_t* __restrict__ start, const float2* __restrict__ in) {
+  const float2 xpi = XP[i], xi = X[i], ini = in[i];
+  const float rho_i = fmaxf(DENS[i], 1e-6f);
+  const uint32_t cx = cell1(xpi.x, g), cy = cell1(xpi.y, g);
+  float2 s = f2(0.0f, 0.0f);
+  for (int o = 0; o < 9; o++) {
+    const uint32_t k = nbr_cell(cx, cy, o, g.ncx);
+    const uint32_t cs = start[k], ce = start[k + 1];
+    for (uint32_t j = cs; j < ce; j++) {
+      if (j == i) continue;
+      if (len2(mi2(sub2(xpi, XP[j]), g)) > g.h2) continue;
+      const float rho_j = fmaxf(DENS[j], 1e-6f);
+      const float2 d = mi2(sub2(xi, X[j]), g);
+      const float r = sqrtf(len2(d));
+      const float2 gr = spiky(d, r, g);
+      const float num = (g.mu2m / (rho_i * rho_j)) * fabsf(dot2(d, gr));
+      const float w = num / (len2(d) + g.eta2);
+      const float2 e = r > 1e-9f ? div2(d, r) : f2(1.0f, 0.0f);
+      s = add2(s, smul2(4.0f * w * dot2(e, sub2(ini, in[j])), e));
+    }
+  }
+  return s;
+}
+
+BXK bx_cg_init(BoxGeo g, const float2* __restrict__ XP, const float2* __restrict__ X, const float* __restrict__ DENS,
+               const uint32_t* __restrict__ start, const float2* __restrict__ V, float2* __restrict__ R,
+               float2* __restrict__ P, double* __restrict__ part, const BoxScal* __restrict__ sc) {
+  if (sc->done) return;
+  const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
+  double rr = 0.0;
+  if (i < g.n) {
+    const float2 v = V[i];
+    const float2 lv = lap_row(g, i, XP, X, DENS, start, V);
+    const float2 r = sub2(v, add2(v, mul2(lv, g.dt)));
+    R[i] = r;
+    P[i] = r;
+    rr = (double)r.x * (double)r.x + (double)r.y * (double)r.y;
+  }
+  const double t = block_sum(rr);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+BXK bx_cg_spmv(BoxGeo g, const float2* __restrict__ XP, const float2* __restrict__ X, const float* __restrict__ DENS,
+               const uint32_t* __restrict__ start, const float2* __restrict__ P, float2* __restrict__ AP,
+               double* __restrict__ part, const BoxScal* __restrict__ sc) {
+  if (sc->done) return;
+  const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
+  double pap = 0.0;
+  if (i < g.n) {
+    const float2 p = P[i];
+    const float2 ap = add2(p, mul2(lap_row(g, i, XP, X, DENS, start, P), g.dt));
+    AP[i] = ap;
+    pap = (double)p.x * (double)ap.x + (double)p.y * (double)ap.y;
+  }
+  const double t = block_sum(pap);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+BXK bx_cg_update(BoxGeo g, float2* __restrict__ V, float2* __restrict__ R, const float2* __restrict__ P,
+                 const float2* __restrict__ AP, double* __restrict__ part, const BoxScal* __restrict__ sc) {
+  if (sc->done) return;
+  const uint32_t i = blockIdx.x * BX_BLOCK + threadIdx.x;
+  const float a = sc->alpha;
+  double rr = 0.0;
+  if (i < g.n) {
+    V[i] = add2(V[i], mul2(P[i], a));
+    const float2 r = sub2(R[i], mul2(AP[i], a));
+    R[i] = r;
+    rr = (double)r.x * (double)r.x + (double)r.y * (double)r.y;
+  }
+  const double t = block_sum(rr);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+BXK bx_cg_p(BoxGeo g, const float2* __restrict__ R, float2* __restrict__ P, const BoxScal* __restrict__ sc) {
+  if (sc->done) return;
+  const uint32_t i = blockIdx.x * BX_BLOCK + threadIdx.x;
+  if (i < g.n) P[i] = add2(R[i], mul2(P[i], sc->beta));
+}
+// Single-block reductions of the per-block partials and the CG scalar logic.
+BDEV double sum_parts(const double* __restrict__ part, uint32_t nb) {
+  double s = 0.0;
+  for (uint32_t k = threadIdx.x; k < nb; k += BX_BLOCK) s += part[k];
+  return block_sum(s);
+}
+// stage 0: |b| and target; stage 1: initial residual; stage 2: p.Ap -> alpha; stage 3: r.r -> beta
+BXK bx_cg_scalar(const double* __restrict__ part, uint32_t nb, BoxScal* __restrict__ sc, int stage, float tol) {
+  if (sc->done) return;
+  const double s = sum_parts(part, nb);
+  if (threadIdx.x != 0) return;
+  if (stage == 0) {
+    const double bn = sqrt(s);
+    sc->bnorm = bn;
+    sc->res = 0.0;
+    sc->it = 0;
+    sc->target = (double)tol * bn;
+    if (!(bn > 0.0) || !isfinite(bn)) sc->done = 1u;
+  } else if (stage == 1) {
+    sc->rs_old = s;
+    sc->res = sqrt(s);
+    if (sqrt(s) <= sc->target) sc->done = 1u;
+  } else if (stage == 2) {
+    sc->it += 1u;
+    sc->pap = s;
+    if (!(s > 0.0) || !isfinite(s)) {
+      sc->done = 1u;
+    } else {
+      sc->alpha = (float)(sc->rs_old / s);
+    }
+  } else {
+    sc->rs_new = s;
+    sc->res = sqrt(s);
+    if (sqrt(s) <= sc->target || sc->it >= sc->max_cg) {
+      sc->done = 1u;
+    } else {
+      sc->beta = (float)(s / sc->rs_old);
+      sc->rs_old = s;
+    }
+  }
+}
+BXK bx_cg_reset(BoxScal* __restrict__ sc, uint32_t max_cg, uint32_t skip) {
+  if (threadIdx.x != 0) return;
+  sc->done = skip;
+  sc->it = 0;
+  sc->max_cg = max_cg;
+  sc->res = 0.0;
+  sc->bnorm = 0.0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// debug export: neighbour lists in the traversal order, rows and entries as original ids
+// ---------------------------------------------------------------------------------------------
+BXK bx_nbr_count(BoxGeo g, const float2* __restrict__ XP, const uint32_t* __restrict__ start,
+                 const uint32_t* __restrict__ id, uint32_t* __restrict__ cnt) {
+  const uint32_t i = blockIdx.x * BX_BLOCK + threadIdx.x;
+  if (i >= g.n) return;
+  const float2 xpi = XP[i];
+  const uint32_t cx = cell1(xpi.x, g), cy = cell1(xpi.y, g);
+  uint32_t c = 0;
+  for (int o = 0; o < 9; o++) {
+    const uint32_t k = nbr_cell(cx, cy, o, g.ncx);
+    for (uint32_t j = start[k]; j < start[k + 1]; j++)
+      if (j != i && len2(mi2(sub2(xpi, XP[j]), g)) <= g.h2) c++;
+  }
+  cnt[id[i]] = c;
+}
+BXK bx_nbr_write(BoxGeo g, const float2* __restrict__ XP, const uint32_t* __restrict__ start,
+                 const uint32_t* __restrict__ id, const uint32_t* __restrict__ off, uint32_t* __restrict__ nb, uint32_t cap) {
+  const uint32_t i = blockIdx.x * BX_BLOCK + threadIdx.x;
+  if (i >= g.n) return;
+  const float2 xpi = XP[i];
+  const uint32_t cx = cell1(xpi.x, g), cy = cell1(xpi.y, g);
+  uint32_t w = off[id[i]];
+  for (int o = 0; o < 9; o++) {
+    const uint32_t k = nbr_cell(cx, cy, o, g.ncx);
+    for (uint32_t j = start[k]; j < start[k + 1]; j++)
+      if (j != i && len2(mi2(sub2(xpi, XP[j]), g)) <= g.h2) {
+        if (w < cap) nb[w] = id[j];
+        w++;
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// host runtime
+// ---------------------------------------------------------------------------------------------
+struct mg_box {
+  mg_box_config cfg;
+  BoxGeo g;
+  uint32_t ncell = 0, nscan = 0;
+  float2 *X = nullptr, *XP = nullptr, *X0 = nullptr, *V = nullptr, *X2 = nullptr;
+  float2 *Xb = nullptr, *X0b = nullptr, *Vb = nullptr;
+  uint32_t *id = nullptr, *idb = nullptr, *key = nullptr, *slot = nullptr, *start = nullptr, *cursor = nullptr,
+           *bsum = nullptr;
+  float *LAM = nullptr, *DENS = nullptr;
+  float2 *R = nullptr, *P = nullptr, *AP = nullptr;
+  double* part = nullptr;
+  BoxScal* sc = nullptr;
+  BoxScal* h_sc = nullptr;  // pinned host mirror
+  void* arena = nullptr;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool have_neighbors = false;
+  float last_ms = 0.0f;
+  std::string err;
+  // optional per-kernel timing: event pairs around launches, accumulated after each sync
+  bool timing = false;
+  struct Pend { int cls; hipEvent_t a, b; };
+  std::vector<Pend> pend;
+  std::vector<hipEvent_t> pool;
+  double k_ms[BX_NCLASS] = {};
+  uint64_t k_n[BX_NCLASS] = {};
+};
+
+namespace {
+
+void box_set_err(char* err, size_t errlen, const std::string& m) {
+  if (err && errlen) {
+    size_t n = std::min(errlen - 1, m.size());
+    std::memcpy(err, m.data(), n);
+    err[n] = 0;
+  }
+}
+
+// Geometry of the workload (same f32 expressions as the seeding description in the header).
+bool box_geometry(const mg_box_config& c, BoxGeo& g, std::string& err) {
+  if (c.side < 8 || c.side > 8192) { err = "side must be in [8, 8192]"; return false; }
+  if (!(c.dx > 0.0f) || !(c.rest > 0.0f) || !(c.dt > 0.0f) || !(c.mu >= 0.0f) || !(c.tol > 0.0f)) {
+    err = "dx, rest, dt and tol must be positive and mu non-negative";
+    return false;
+  }
+  g.n = c.side * c.side;
+  g.h = 2.0f * c.dx;
+  g.h2 = g.h * g.h;
+  g.L = (float)c.side * c.dx;
+  g.halfL = 0.5f * g.L;
+  uint32_t n = (uint32_t)std::floor((double)g.L / (double)g.h);
+  while (n > 1 && g.L / (float)n < g.h) n--;
+  g.ncx = std::max(n, 1u);
+  if (g.ncx < 3) { err = "box too small: fewer than 3 cells per side"; return false; }
+  g.cw = g.L / (float)g.ncx;
+  g.k6 = 4.0f / (mg::PI_F * mg::powi8(g.h));
+  g.ks = -30.0f / (mg::PI_F * mg::powi5(g.h));
+  float lw = 0.0f;
+  for (int q = -2; q <= 2; q++)
+    for (int p = -2; p <= 2; p++) {
+      float rx = (float)p * c.dx, ry = (float)q * c.dx;
+      float r2 = rx * rx + ry * ry;
+      if (r2 >= g.h * g.h || r2 < 0.0f) continue;
+      float t = g.h * g.h - r2;
+      lw += g.k6 * t * t * t;
+    }
+  g.mass = 1.02f * c.rest / lw;
+  g.rest = c.rest;
+  g.dt = c.dt;
+  {
+    float t = g.h * g.h - 0.0f;
+    g.w0 = g.mass * (g.k6 * t * t * t);
+  }
+  g.m_over_rest = g.mass / c.rest;
+  g.mu2m = g.mass * (c.mu + c.mu);
+  g.eta2 = 0.01f * g.h * g.h;
+  g.nblk = (g.n + BX_BLOCK - 1) / BX_BLOCK;
+  g.xcd_chunk = (g.nblk % 8u == 0u) ? g.nblk / 8u : 0u;
+  return true;
+}
+
+float wrap_host(float a, float L) {
+  float r = std::fmod(a, L);
+  return r < 0.0f ? r + std::fabs(L) : r;
+}
+
+// Lattice + jitter + Box-Muller velocities from one Rng(seed) stream in particle order.
+void box_seed(const mg_box_config& c, const BoxGeo& g, std::vector<float>& x, std::vector<float>& v) {
+  const size_t n = (size_t)c.side * c.side;
+  x.assign(2 * n, 0.0f);
+  v.assign(2 * n, 0.0f);
+  mg::Rng rng(c.seed);
+  const float jit = 0.1f * c.dx;
+  const double two_pi = 6.283185307179586476925;
+  for (uint32_t iy = 0; iy < c.side; iy++)
+    for (uint32_t ix = 0; ix < c.side; ix++) {
+      size_t i = (size_t)iy * c.side + ix;
+      float jx = (rng.next_f32() * 2.0f - 1.0f) * jit;
+      float jy = (rng.next_f32() * 2.0f - 1.0f) * jit;
+      double u1 = rng.next_f32(), u2 = rng.next_f32(), u3 = rng.next_f32(), u4 = rng.next_f32();
+      double g1 = std::sqrt(-2.0 * std::log(std::max(u1, 1e-12))) * std::cos(two_pi * u2);
+      double g2 = std::sqrt(-2.0 * std::log(std::max(u3, 1e-12))) * std::cos(two_pi * u4);
+      x[2 * i] = wrap_host(((float)ix + 0.5f) * c.dx + jx, g.L);
+      x[2 * i + 1] = wrap_host(((float)iy + 0.5f) * c.dx + jy, g.L);
+      v[2 * i] = (float)(0.01 * g1);
+      v[2 * i + 1] = (float)(0.01 * g2);
+    }
+}
+
+template <class T> size_t carve(size_t off, T*& p, size_t count) {
+  p = reinterpret_cast<T*>(off);
+  return off + ((count * sizeof(T) + 255) / 256) * 256 + 256;
+}
+
+inline dim3 blocks_for(uint32_t n) { return dim3((n + BX_BLOCK - 1) / BX_BLOCK); }
+
+hipEvent_t take_event(mg_box* B) {
+  if (!B->pool.empty()) {
+    hipEvent_t e = B->pool.back();
+    B->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+// Brackets the launches between tic and toc with an event pair of class cls (when timing is on).
+struct Tic {
+  mg_box* B;
+  int cls;
+  hipEvent_t a = nullptr;
+  Tic(mg_box* b, int c) : B(b), cls(c) {
+    if (B->timing && (a = take_event(B))) (void)hipEventRecord(a, B->stream);
+  }
+  ~Tic() {
+    if (!a) return;
+    hipEvent_t b = take_event(B);
+    if (!b) { B->pool.push_back(a); return; }
+    (void)hipEventRecord(b, B->stream);
+    B->pend.push_back({cls, a, b});
+  }
+};
+// After a stream sync: fold the completed event pairs into the per-class totals.
+void collect_times(mg_box* B) {
+  for (auto& p : B->pend) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      B->k_ms[p.cls] += ms;
+      B->k_n[p.cls] += 1;
+    }
+    B->pool.push_back(p.a);
+    B->pool.push_back(p.b);
+  }
+  B->pend.clear();
+}
+
+// Enqueues one step up to and including the CG set-up; returns false on a launch error.
+bool enqueue_front(mg_box* B) {
+  const BoxGeo& g = B->g;
+  hipStream_t st = B->stream;
+  const dim3 blk(BX_BLOCK), gp = blocks_for(g.n);
+  {
+  Tic t(B, BXT_GRID);
+  hipLaunchKernelGGL(bx_fill_u32, blocks_for(B->ncell + 1), blk, 0, st, B->start, B->ncell + 1, 0u);
+  hipLaunchKernelGGL(bx_fill_u32, blocks_for(B->ncell), blk, 0, st, B->cursor, B->ncell, 0u);
+  hipLaunchKernelGGL(bx_predict, gp, blk, 0, st, g, B->X, B->V, B->X0, B->key, B->start);
+  hipLaunchKernelGGL(bx_scan_totals, dim3(B->nscan), blk, 0, st, B->start, B->ncell + 1, B->bsum);
+  hipLaunchKernelGGL(bx_scan_top, dim3(1), blk, 0, st, B->bsum, B->nscan);
+  hipLaunchKernelGGL(bx_scan_apply, dim3(B->nscan), blk, 0, st, B->start, B->ncell + 1, B->bsum);
+  hipLaunchKernelGGL(bx_scatter, gp, blk, 0, st, g, B->key, B->start, B->cursor, B->slot);
+  hipLaunchKernelGGL(bx_cellsort, blocks_for(B->ncell), blk, 0, st, B->ncell, B->start, B->slot, B->id);
+  hipLaunchKernelGGL(bx_gather, gp, blk, 0, st, g, B->slot, B->X, B->X0, B->V, B->id, B->Xb, B->XP, B->X0b, B->Vb,
+                     B->idb);
+  }
+  std::swap(B->X, B->Xb);
+  std::swap(B->X0, B->X0b);
+  std::swap(B->V, B->Vb);
+  std::swap(B->id, B->idb);
+  for (uint32_t p = 0; p < B->cfg.passes; p++) {
+    {
+      Tic t(B, BXT_LAMBDA);
+      hipLaunchKernelGGL(bx_lambda, gp, blk, 0, st, g, B->XP, B->X, B->start, B->LAM);
+    }
+    {
+      Tic t(B, BXT_DP);
+      hipLaunchKernelGGL(bx_dp, gp, blk, 0, st, g, B->XP, B->X, B->start, B->LAM, B->X2);
+    }
+    std::swap(B->X, B->X2);
+  }
+  {
+    Tic t(B, BXT_VEL);
+    hipLaunchKernelGGL(bx_vel_density, gp, blk, 0, st, g, B->XP, B->X, B->X0, B->start, B->V, B->DENS, B->part);
+  }
+  const uint32_t skip = B->cfg.mu > 0.0f ? 0u : 1u;
+  hipLaunchKernelGGL(bx_cg_reset, dim3(1), dim3(64), 0, st, B->sc, B->cfg.max_cg, skip);
+  hipLaunchKernelGGL(bx_cg_scalar, dim3(1), blk, 0, st, B->part, g.nblk, B->sc, 0, B->cfg.tol);
+  {
+    Tic t(B, BXT_CGINIT);
+    hipLaunchKernelGGL(bx_cg_init, gp, blk, 0, st, g, B->XP, B->X, B->DENS, B->start, B->V, B->R, B->P, B->part, B->sc);
+  }
+  hipLaunchKernelGGL(bx_cg_scalar, dim3(1), blk, 0, st, B->part, g.nblk, B->sc, 1, B->cfg.tol);
+  B->have_neighbors = true;
+  return hipGetLastError() == hipSuccess;
+}
+
+void enqueue_cg_chunk(mg_box* B, uint32_t iters) {
+  const BoxGeo& g = B->g;
+  hipStream_t st = B->stream;
+  const dim3 blk(BX_BLOCK), gp = blocks_for(g.n);
+  for (uint32_t k = 0; k < iters; k++) {
+    {
+      Tic t(B, BXT_SPMV);
+      hipLaunchKernelGGL(bx_cg_spmv, gp, blk, 0, st, g, B->XP, B->X, B->DENS, B->start, B->P, B->AP, B->part, B->sc);
+    }
+    hipLaunchKernelGGL(bx_cg_scalar, dim3(1), blk, 0, st, B->part, g.nblk, B->sc, 2, B->cfg.tol);
+    {
+      Tic t(B, BXT_UPDATE);
+      hipLaunchKernelGGL(bx_cg_update, gp, blk, 0, st, g, B->V, B->R, B->P, B->AP, B->part, B->sc);
+    }
+    hipLaunchKernelGGL(bx_cg_scalar, dim3(1), blk, 0, st, B->part, g.nblk, B->sc, 3, B->cfg.tol);
+    {
+      Tic t(B, BXT_P);
+      hipLaunchKernelGGL(bx_cg_p, gp, blk, 0, st, g, B->R, B->P, B->sc);
+    }
+  }
+}
+
+// One full step: front half asynchronously, then CG chunks until the device reports done.
+int run_step(mg_box* B) {
+  if (!enqueue_front(B)) { B->err = "kernel launch failed"; return MG_E_DEVICE; }
+  uint32_t launched = 0;
+  for (;;) {
+    hipMemcpyAsync(B->h_sc, B->sc, sizeof(BoxScal), hipMemcpyDeviceToHost, B->stream);
+    if (hipStreamSynchronize(B->stream) != hipSuccess) { B->err = "HIP stream error"; return MG_E_DEVICE; }
+    collect_times(B);
+    if (B->h_sc->done || launched >= B->cfg.max_cg) break;
+    uint32_t k = std::min<uint32_t>(BX_CG_CHUNK, B->cfg.max_cg - launched);
+    enqueue_cg_chunk(B, k);
+    launched += k;
+    if (hipGetLastError() != hipSuccess) { B->err = "kernel launch failed"; return MG_E_DEVICE; }
+  }
+  return MG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mg_box_geometry(const mg_box_config* cfg, double* out5) {
+  if (!cfg || !out5) return MG_E_ARG;
+  BoxGeo g;
+  std::string e;
+  if (!box_geometry(*cfg, g, e)) return MG_E_PARAMS;
+  out5[0] = g.h; out5[1] = g.L; out5[2] = g.cw; out5[3] = g.mass; out5[4] = g.ncx;
+  return MG_OK;
+}
+
+int mg_box_create(const mg_box_config* cfg, mg_box** out, char* err, size_t errlen) {
+  if (!cfg || !out) return MG_E_ARG;
+  *out = nullptr;
+  auto* B = new mg_box();
+  B->cfg = *cfg;
+  std::string e;
+  if (!box_geometry(*cfg, B->g, e)) { box_set_err(err, errlen, e); delete B; return MG_E_PARAMS; }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { box_set_err(err, errlen, "no HIP device"); delete B; return MG_E_DEVICE; }
+  const uint32_t n = B->g.n;
+  B->ncell = B->g.ncx * B->g.ncx;
+  B->nscan = (B->ncell + 1 + BX_SCAN_CHUNK - 1) / BX_SCAN_CHUNK;
+  if (B->nscan > BX_SCAN_CHUNK) { box_set_err(err, errlen, "too many cells"); delete B; return MG_E_PARAMS; }
+  size_t off = 0;
+  off = carve(off, B->X, n); off = carve(off, B->XP, n); off = carve(off, B->X0, n); off = carve(off, B->V, n);
+  off = carve(off, B->X2, n); off = carve(off, B->Xb, n); off = carve(off, B->X0b, n); off = carve(off, B->Vb, n);
+  off = carve(off, B->id, n); off = carve(off, B->idb, n); off = carve(off, B->key, n); off = carve(off, B->slot, n);
+  off = carve(off, B->start, (size_t)B->ncell + 1); off = carve(off, B->cursor, (size_t)B->ncell);
+  off = carve(off, B->bsum, (size_t)B->nscan);
+  off = carve(off, B->LAM, n); off = carve(off, B->DENS, n);
+  off = carve(off, B->R, n); off = carve(off, B->P, n); off = carve(off, B->AP, n);
+  off = carve(off, B->part, (size_t)B->g.nblk);
+  off = carve(off, B->sc, 1);
+  if (hipMalloc(&B->arena, off) != hipSuccess) { box_set_err(err, errlen, "hipMalloc failed"); delete B; return MG_E_DEVICE; }
+  const uintptr_t base = (uintptr_t)B->arena;
+  auto fix = [&](auto*& p) { p = reinterpret_cast<std::remove_reference_t<decltype(p)>>(base + (uintptr_t)p); };
+  fix(B->X); fix(B->XP); fix(B->X0); fix(B->V); fix(B->X2); fix(B->Xb); fix(B->X0b); fix(B->Vb);
+  fix(B->id); fix(B->idb); fix(B->key); fix(B->slot); fix(B->start); fix(B->cursor); fix(B->bsum);
+  fix(B->LAM); fix(B->DENS); fix(B->R); fix(B->P); fix(B->AP); fix(B->part); fix(B->sc);
+  bool ok = hipMemset(B->arena, 0, off) == hipSuccess;
+  ok = ok && hipHostMalloc((void**)&B->h_sc, sizeof(BoxScal), 0) == hipSuccess;
+  ok = ok && hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) == hipSuccess;
+  ok = ok && hipEventCreate(&B->ev0) == hipSuccess && hipEventCreate(&B->ev1) == hipSuccess;
+  if (ok) {
+    std::vector<float> x, v;
+    box_seed(*cfg, B->g, x, v);
+    std::vector<uint32_t> ids(n);
+    for (uint32_t i = 0; i < n; i++) ids[i] = i;
+    ok = hipMemcpy(B->X, x.data(), x.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(B->V, v.data(), v.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(B->id, ids.data(), ids.size() * 4, hipMemcpyHostToDevice) == hipSuccess;
+  }
+  if (!ok) { box_set_err(err, errlen, "HIP error while creating the box"); mg_box_destroy(B); return MG_E_DEVICE; }
+  *out = B;
+  return MG_OK;
+}
+
+void mg_box_destroy(mg_box* B) {
+  if (!B) return;
+  if (B->stream) (void)hipStreamSynchronize(B->stream);
+  if (B->arena) (void)hipFree(B->arena);
+  if (B->h_sc) (void)hipHostFree(B->h_sc);
+  if (B->ev0) (void)hipEventDestroy(B->ev0);
+  if (B->ev1) (void)hipEventDestroy(B->ev1);
+  collect_times(B);
+  for (hipEvent_t e : B->pool) (void)hipEventDestroy(e);
+  if (B->stream) (void)hipStreamDestroy(B->stream);
+  delete B;
+}
+
+size_t mg_box_count(const mg_box* B) { return B ? (size_t)B->g.n : 0; }
+
+int mg_box_step(mg_box* B, uint32_t steps) {
+  if (!B) return MG_E_ARG;
+  hipEventRecord(B->ev0, B->stream);
+  for (uint32_t s = 0; s < steps; s++) {
+    int rc = run_step(B);
+    if (rc) return rc;
+  }
+  hipEventRecord(B->ev1, B->stream);
+  if (hipEventSynchronize(B->ev1) != hipSuccess) return MG_E_DEVICE;
+  hipEventElapsedTime(&B->last_ms, B->ev0, B->ev1);
+  return MG_OK;
+}
+
+float mg_box_last_step_ms(const mg_box* B) { return B ? B->last_ms : 0.0f; }
+
+int mg_box_set_timing(mg_box* B, int on) {
+  if (!B) return MG_E_ARG;
+  B->timing = on != 0;
+  for (int k = 0; k < BX_NCLASS; k++) { B->k_ms[k] = 0.0; B->k_n[k] = 0; }
+  return MG_OK;
+}
+
+int mg_box_kernel_times(mg_box* B, double* ms, uint64_t* launches, int nclass) {
+  if (!B || !ms || !launches) return MG_E_ARG;
+  for (int k = 0; k < nclass && k < BX_NCLASS; k++) { ms[k] = B->k_ms[k]; launches[k] = B->k_n[k]; }
+  return MG_OK;
+}
+
+int mg_box_copy_state(mg_box* B, float* x2n, float* v2n) {
+  if (!B) return MG_E_ARG;
+  const uint32_t n = B->g.n;
+  std::vector<float> x(2 * (size_t)n), v(2 * (size_t)n);
+  std::vector<uint32_t> id(n);
+  bool ok = hipStreamSynchronize(B->stream) == hipSuccess;
+  ok = ok && hipMemcpy(x.data(), B->X, x.size() * 4, hipMemcpyDeviceToHost) == hipSuccess;
+  ok = ok && hipMemcpy(v.data(), B->V, v.size() * 4, hipMemcpyDeviceToHost) == hipSuccess;
+  ok = ok && hipMemcpy(id.data(), B->id, id.size() * 4, hipMemcpyDeviceToHost) == hipSuccess;
+  if (!ok) return MG_E_DEVICE;
+  for (uint32_t q = 0; q < n; q++) {
+    uint32_t i = id[q];
+    if (i >= n) return MG_E_DEVICE;
+    if (x2n) { x2n[2 * i] = x[2 * q]; x2n[2 * i + 1] = x[2 * q + 1]; }
+    if (v2n) { v2n[2 * i] = v[2 * q]; v2n[2 * i + 1] = v[2 * q + 1]; }
+  }
+  return MG_OK;
+}
+
+int mg_box_set_state(mg_box* B, const float* x2n, const float* v2n) {
+  if (!B || !x2n || !v2n) return MG_E_ARG;
+  const uint32_t n = B->g.n;
+  std::vector<uint32_t> ids(n);
+  for (uint32_t i = 0; i < n; i++) ids[i] = i;
+  bool ok = hipStreamSynchronize(B->stream) == hipSuccess;
+  ok = ok && hipMemcpy(B->X, x2n, (size_t)n * 8, hipMemcpyHostToDevice) == hipSuccess;
+  ok = ok && hipMemcpy(B->V, v2n, (size_t)n * 8, hipMemcpyHostToDevice) == hipSuccess;
+  ok = ok && hipMemcpy(B->id, ids.data(), (size_t)n * 4, hipMemcpyHostToDevice) == hipSuccess;
+  B->have_neighbors = false;
+  return ok ? MG_OK : MG_E_DEVICE;
+}
+
+int mg_box_density(mg_box* B, float* out) {
+  if (!B || !out) return MG_E_ARG;
+  const uint32_t n = B->g.n;
+  std::vector<float> d(n);
+  std::vector<uint32_t> id(n);
+  bool ok = hipStreamSynchronize(B->stream) == hipSuccess;
+  ok = ok && hipMemcpy(d.data(), B->DENS, (size_t)n * 4, hipMemcpyDeviceToHost) == hipSuccess;
+  ok = ok && hipMemcpy(id.data(), B->id, (size_t)n * 4, hipMemcpyDeviceToHost) == hipSuccess;
+  if (!ok) return MG_E_DEVICE;
+  for (uint32_t q = 0; q < n; q++) {
+    if (id[q] >= n) return MG_E_DEVICE;
+    out[id[q]] = d[q];
+  }
+  return MG_OK;
+}
+
+int mg_box_stats(mg_box* B, double* out3) {
+  if (!B || !out3) return MG_E_ARG;
+  if (hipStreamSynchronize(B->stream) != hipSuccess) return MG_E_DEVICE;
+  BoxScal s{};
+  if (hipMemcpy(&s, B->sc, sizeof s, hipMemcpyDeviceToHost) != hipSuccess) return MG_E_DEVICE;
+  out3[0] = s.it; out3[1] = s.res; out3[2] = s.bnorm;
+  return MG_OK;
+}
+
+int64_t mg_box_neighbors(mg_box* B, uint32_t* off, uint32_t* nb, int64_t cap) {
+  if (!B || !off) return MG_E_ARG;
+  if (!B->have_neighbors) return MG_E_ARG;
+  const uint32_t n = B->g.n;
+  const BoxGeo& g = B->g;
+  uint32_t* d_cnt = nullptr;
+  uint32_t* d_nb = nullptr;
+  const uint32_t dcap = (uint32_t)std::min<int64_t>(std::max<int64_t>(cap, 1), (int64_t)64 * n + 64);
+  if (hipMalloc(&d_cnt, ((size_t)n + 1) * 4) != hipSuccess) return MG_E_DEVICE;
+  if (hipMalloc(&d_nb, (size_t)dcap * 4) != hipSuccess) { (void)hipFree(d_cnt); return MG_E_DEVICE; }
+  hipLaunchKernelGGL(bx_nbr_count, blocks_for(n), dim3(BX_BLOCK), 0, B->stream, g, B->XP, B->start, B->id, d_cnt);
+  std::vector<uint32_t> cnt(n);
+  bool ok = hipMemcpyAsync(cnt.data(), d_cnt, (size_t)n * 4, hipMemcpyDeviceToHost, B->stream) == hipSuccess &&
+            hipStreamSynchronize(B->stream) == hipSuccess;
+  int64_t total = 0;
+  if (ok) {
+    off[0] = 0;
+    for (uint32_t i = 0; i < n; i++) { total += cnt[i]; off[i + 1] = (uint32_t)total; }
+    ok = hipMemcpy(d_cnt, off, (size_t)n * 4, hipMemcpyHostToDevice) == hipSuccess;
+  }
+  if (ok) {
+    hipLaunchKernelGGL(bx_nbr_write, blocks_for(n), dim3(BX_BLOCK), 0, B->stream, g, B->XP, B->start, B->id, d_cnt, d_nb, dcap);
+    ok = hipStreamSynchronize(B->stream) == hipSuccess;
+  }
+  if (ok && nb && total <= cap) ok = hipMemcpy(nb, d_nb, (size_t)total * 4, hipMemcpyDeviceToHost) == hipSuccess;
+  (void)hipFree(d_cnt);
+  (void)hipFree(d_nb);
+  return ok ? total : (int64_t)MG_E_DEVICE;
+}
+
+}  // extern "C"

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../include/millgpu.h"
+#include "mg_common.h"
 #include "mg_layout.h"
 #include "mg_params.h"
 
@@ -32,8 +33,6 @@ namespace {
 // MG_E_DISABLED instead of launching; creation, state copies and the other host paths still work.
 constexpr bool kDrumEngineEnabled = false;
 
-const float PI_F = 3.14159265358979323846f;
-const float TAU_F = 6.28318530717958647692f;
 
 void set_err(char* err, size_t errlen, const std::string& m) {
   if (err && errlen) {
@@ -42,22 +41,6 @@ void set_err(char* err, size_t errlen, const std::string& m) {
     err[n] = 0;
   }
 }
-
-// rng.rs:9-47
-struct Rng {
-  uint64_t s;
-  explicit Rng(uint64_t seed) : s(seed == 0 ? 0x9E3779B97F4A7C15ull : seed) {}
-  uint64_t next_u64() {
-    uint64_t x = s;
-    x ^= x << 13;
-    x ^= x >> 7;
-    x ^= x << 17;
-    s = x;
-    return x * 0x2545F4914F6CDD1Dull;
-  }
-  float next_f32() { return (float)(next_u64() >> 40) / (float)(1u << 24); }
-  float range(float lo, float hi) { return lo + next_f32() * (hi - lo); }
-};
 
 int32_t as_i32(float x) {
   if (std::isnan(x)) return 0;
@@ -70,9 +53,6 @@ uint64_t as_u64(float x) {
   if (x >= 1.8446744073709552e19f) return UINT64_MAX;
   return (uint64_t)x;
 }
-float powi8(float x) { float a = x * x; a = a * a; return a * a; }
-float powi5(float x) { float a = x * x; return x * (a * a); }
-float powi6(float x) { float a = x * x; return a * (a * a); }
 double powi3d(double x) { return x * (x * x); }
 
 struct SqueezeFilm {  // lubrication.rs:44-71

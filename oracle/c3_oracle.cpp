@@ -1,0 +1,394 @@
+// CPU oracle for the kernel-isolation workload: a periodic box of PBF particles.
+//
+// TEST INFRASTRUCTURE ONLY (included by mg_oracle.cpp; same rules as that file).
+//
+// The reference has no periodic domain, so this workload has no reference trace of its own. It
+// applies the reference's fluid arithmetic, unchanged, to a box of side L with minimum-image pair
+// vectors and wrapped positions:
+//   predict                 x0 = x; x += v dt (gravity 0)             pbf.rs:1225-1235
+//   neighbour lists         |d|^2 <= h^2, mutual, reference order      pbf.rs:262-288 (order as the
+//                           grid.rs:291-362 emission gives it per particle, see c3_neighbors)
+//   density passes          rho, one-sided C, lambda, dp, Jacobi       pbf.rs:1257-1334
+//   velocity                v = (x - x0) / dt                          pbf.rs:1656-1674
+//   density recompute       at the final positions                     pbf.rs:2136-2144
+//   Morris weights          c_ij, unit directions                      pbf.rs:312-344
+//   implicit viscosity CG   (I + dt L) v = v, warm start v             pbf.rs:359-480
+// with exactly `passes` density passes (the reference's adaptive count is not used here) and the
+// CG stopping at `tol`·|b| after at most `max_cg` iterations (the reference uses 1e-3 and 50).
+//
+// Workload definition (synthetic; documented beside the GPU path):
+//   side x side square lattice, spacing dx, kernel radius h = 2 dx, L = side·dx;
+//   particle i = iy·side + ix at ((ix + 0.5) dx + jx, (iy + 0.5) dx + jy), jitter U(-0.1dx, 0.1dx);
+//   velocity per component 0.01·N(0, 1) by Box-Muller (cosine branch), computed in f64;
+//   one Rng(seed) stream in particle order: jx, jy, u1, u2, u3, u4;
+//   mass = 1.02·rest / Σ_lattice W (the unjittered lattice is 2 % over-dense).
+
+namespace orc {
+
+struct C3Config {
+  uint32_t side = 1024;
+  float dx = 1.0e-3f;
+  float rest = 1000.0f;
+  float mu = 1.0e-3f;
+  float dt = 1.0f / 1440.0f;
+  uint32_t passes = 4;
+  float tol = 1.0e-6f;
+  uint32_t max_cg = 500;
+  uint64_t seed = 1;
+};
+
+// Box geometry shared by the seeding and the step (all f32, as the GPU path computes it).
+struct C3Box {
+  float dx, h, L, halfL, cw, mass;
+  uint32_t ncx;
+  static C3Box make(const C3Config& c) {
+    C3Box b;
+    b.dx = c.dx;
+    b.h = 2.0f * c.dx;
+    b.L = (float)c.side * c.dx;
+    b.halfL = 0.5f * b.L;
+    // cells of width cw = L / ncx >= h, so every pair within h lies in adjacent cells
+    uint32_t n = (uint32_t)std::floor((double)b.L / (double)b.h);
+    while (n > 1 && b.L / (float)n < b.h) n--;
+    b.ncx = std::max(n, 1u);
+    b.cw = b.L / (float)b.ncx;
+    float lw = 0.0f;  // Σ over the unjittered lattice around one site, row-major offsets
+    for (int q = -2; q <= 2; q++)
+      for (int p = -2; p <= 2; p++) {
+        float rx = (float)p * c.dx, ry = (float)q * c.dx;
+        lw += poly6<float>(rx * rx + ry * ry, b.h);
+      }
+    b.mass = 1.02f * c.rest / lw;
+    return b;
+  }
+  // rem_euclid (f32), as the reference wraps angles
+  float wrap(float a) const { return rem_euclid<float>(a, L); }
+  double wrap(double a) const { return rem_euclid<double>(a, (double)L); }
+  template <class R> R mi(R d) const {  // minimum image of one component
+    if (d > (R)halfL) return d - (R)L;
+    if (d < -(R)halfL) return d + (R)L;
+    return d;
+  }
+  template <class R> uint32_t cell1(R a) const {
+    R q = std::floor(a / (R)cw);
+    int64_t c = q < (R)0 ? 0 : (int64_t)q;
+    return (uint32_t)std::min<int64_t>(c, (int64_t)ncx - 1);
+  }
+};
+
+// Initial state in f32 (every precision policy starts from the same f32 values).
+static void c3_seed(const C3Config& c, const C3Box& b, std::vector<float>& x, std::vector<float>& v) {
+  const size_t n = (size_t)c.side * c.side;
+  x.assign(2 * n, 0.0f);
+  v.assign(2 * n, 0.0f);
+  Rng rng(c.seed);
+  const float jit = 0.1f * c.dx;
+  const double two_pi = 6.283185307179586476925;
+  for (uint32_t iy = 0; iy < c.side; iy++)
+    for (uint32_t ix = 0; ix < c.side; ix++) {
+      size_t i = (size_t)iy * c.side + ix;
+      float jx = (rng.next_f32() * 2.0f - 1.0f) * jit;
+      float jy = (rng.next_f32() * 2.0f - 1.0f) * jit;
+      double u1 = rng.next_f32(), u2 = rng.next_f32(), u3 = rng.next_f32(), u4 = rng.next_f32();
+      double g1 = std::sqrt(-2.0 * std::log(std::max(u1, 1e-12))) * std::cos(two_pi * u2);
+      double g2 = std::sqrt(-2.0 * std::log(std::max(u3, 1e-12))) * std::cos(two_pi * u4);
+      x[2 * i] = b.wrap(((float)ix + 0.5f) * c.dx + jx);
+      x[2 * i + 1] = b.wrap(((float)iy + 0.5f) * c.dx + jy);
+      v[2 * i] = (float)(0.01 * g1);
+      v[2 * i + 1] = (float)(0.01 * g2);
+    }
+}
+
+// Per-particle neighbour order of the reference (pbf.rs:262-288 over grid.rs:291-362): a point in
+// cell c receives its pairs from the cells (-1,-1), (-1,0), (0,-1), then c itself (ascending
+// index, skipping the point), then (1,0), (1,1), (0,1), (-1,1) and last (1,-1). With periodic
+// cells the offsets wrap; ncx >= 3 keeps the nine cells distinct.
+static const int kC3Ox[9] = {-1, -1, 0, 0, 1, 1, 0, -1, 1};
+static const int kC3Oy[9] = {-1, 0, -1, 0, 0, 1, 1, 1, -1};
+
+template <class R>
+static Nbrs c3_neighbors(const std::vector<V2<R>>& x, const C3Box& b) {
+  const size_t n = x.size();
+  const uint32_t nc = b.ncx;
+  // counting sort by cell, ascending index inside a cell
+  std::vector<uint32_t> start((size_t)nc * nc + 1, 0), idx(n);
+  std::vector<uint32_t> key(n);
+  for (size_t i = 0; i < n; i++) {
+    key[i] = b.cell1(x[i].y) * nc + b.cell1(x[i].x);
+    start[key[i] + 1]++;
+  }
+  for (size_t k = 1; k < start.size(); k++) start[k] += start[k - 1];
+  std::vector<uint32_t> cur(start.begin(), start.end() - 1);
+  for (size_t i = 0; i < n; i++) idx[cur[key[i]]++] = (uint32_t)i;
+  Nbrs L;
+  L.off.assign(n + 1, 0);
+  const R h2 = (R)b.h * (R)b.h;
+  std::vector<uint32_t> tmp;
+  for (size_t i = 0; i < n; i++) {
+    int cx = (int)b.cell1(x[i].x), cy = (int)b.cell1(x[i].y);
+    for (int o = 0; o < 9; o++) {
+      uint32_t ax = (uint32_t)((cx + kC3Ox[o] + (int)nc) % (int)nc);
+      uint32_t ay = (uint32_t)((cy + kC3Oy[o] + (int)nc) % (int)nc);
+      uint32_t k = ay * nc + ax;
+      for (uint32_t a = start[k]; a < start[k + 1]; a++) {
+        uint32_t j = idx[a];
+        if (j == (uint32_t)i) continue;
+        V2<R> d(b.mi(x[i].x - x[j].x), b.mi(x[i].y - x[j].y));
+        if (d.len2() <= h2) L.nb.push_back(j);
+      }
+    }
+    L.off[i + 1] = (uint32_t)L.nb.size();
+  }
+  return L;
+}
+
+template <class R, class A> struct C3Sim {
+  C3Config cfg;
+  C3Box box;
+  std::vector<V2<R>> x, v;
+  Nbrs nbr;                 // lists of the last step (at the predicted positions)
+  std::vector<R> dens;      // density at the final positions of the last step
+  uint32_t last_cg = 0;
+  A last_res = 0, last_bnorm = 0;
+
+  void init(const C3Config& c) {
+    cfg = c;
+    box = C3Box::make(c);
+    std::vector<float> xf, vf;
+    c3_seed(c, box, xf, vf);
+    size_t n = xf.size() / 2;
+    x.resize(n);
+    v.resize(n);
+    for (size_t i = 0; i < n; i++) {
+      x[i] = V2<R>((R)xf[2 * i], (R)xf[2 * i + 1]);
+      v[i] = V2<R>((R)vf[2 * i], (R)vf[2 * i + 1]);
+    }
+  }
+  V2<R> mi(V2<R> d) const { return V2<R>(box.mi(d.x), box.mi(d.y)); }
+  V2<R> wrap(V2<R> p) const { return V2<R>(box.wrap(p.x), box.wrap(p.y)); }
+  static A adot(V2<R> a, V2<R> b) { return (A)a.x * (A)b.x + (A)a.y * (A)b.y; }
+  static A dotv(const std::vector<V2<R>>& a, const std::vector<V2<R>>& b) {  // pbf.rs:292-294
+    A s = 0;
+    for (size_t i = 0; i < a.size(); i++) s += adot(a[i], b[i]);
+    return s;
+  }
+
+  void step() {
+    const size_t n = x.size();
+    const R dt = (R)cfg.dt, h = (R)box.h, rest = (R)cfg.rest, mass = (R)box.mass, mu = (R)cfg.mu;
+    // predict (pbf.rs:1225-1235), gravity 0, positions wrapped into the box
+    std::vector<V2<R>> x0 = x;
+    for (size_t i = 0; i < n; i++) x[i] = wrap(x[i] + v[i] * dt);
+    // neighbour lists at the predicted positions (pbf.rs:1238-1239)
+    nbr = c3_neighbors(x, box);
+    // density passes (pbf.rs:1257-1334), exactly cfg.passes
+    std::vector<R> density(n, 0), lambda(n, 0);
+    std::vector<V2<R>> dp(n), gcache(nbr.nb.size());
+    for (uint32_t pass = 0; pass < cfg.passes; pass++) {
+      for (size_t i = 0; i < n; i++) {
+        R rho = mass * poly6((R)0, h);
+        V2<R> gs;
+        R sgs = 0;
+        for (uint32_t k = nbr.off[i]; k < nbr.off[i + 1]; k++) {
+          uint32_t j = nbr.nb[k];
+          V2<R> d = mi(x[i] - x[j]);
+          R r2 = d.len2();
+          rho += mass * poly6(r2, h);
+          R r = d.len();
+          V2<R> rg = spiky_grad(d, r, h);
+          gcache[k] = rg;
+          V2<R> g = rg * (mass / rest);
+          gs += g;
+          sgs += g.len2();
+        }
+        density[i] = rho;
+        R c = rmax(density[i] / rest - F(1.0), (R)0);
+        sgs += gs.len2();
+        lambda[i] = -c / (sgs + F(200.0));
+      }
+      for (size_t i = 0; i < n; i++) {
+        V2<R> s;
+        for (uint32_t k = nbr.off[i]; k < nbr.off[i + 1]; k++) s += (lambda[i] + lambda[nbr.nb[k]]) * gcache[k];
+        dp[i] = s * (mass / rest);
+      }
+      for (size_t i = 0; i < n; i++) x[i] = wrap(x[i] + dp[i]);
+    }
+    // velocity (pbf.rs:1656-1674) with the minimum-image displacement
+    for (size_t i = 0; i < n; i++) v[i] = mi(x[i] - x0[i]) / dt;
+    // density recompute (pbf.rs:2136-2144)
+    for (size_t i = 0; i < n; i++) {
+      R rho = mass * poly6((R)0, h);
+      for (uint32_t k = nbr.off[i]; k < nbr.off[i + 1]; k++) rho += mass * poly6(mi(x[i] - x[nbr.nb[k]]).len2(), h);
+      density[i] = rho;
+    }
+    dens = density;
+    last_cg = 0;
+    if (!(mu > (R)0)) return;
+    // Morris weights (pbf.rs:312-344), Newtonian
+    const R eta2 = F(0.01) * h * h;
+    std::vector<R> W(nbr.nb.size());
+    std::vector<V2<R>> D(nbr.nb.size());
+    for (size_t i = 0; i < n; i++) {
+      R rho_i = rmax(density[i], F(1e-6));
+      for (uint32_t k = nbr.off[i]; k < nbr.off[i + 1]; k++) {
+        uint32_t j = nbr.nb[k];
+        R rho_j = rmax(density[j], F(1e-6));
+        V2<R> d = mi(x[i] - x[j]);
+        R r = d.len();
+        V2<R> g = spiky_grad(d, r, h);
+        R num = (mass * (mu + mu) / (rho_i * rho_j)) * std::fabs(d.dot(g));
+        W[k] = num / (d.len2() + eta2);
+        D[k] = r > F(1e-9) ? d / r : V2<R>(1, 0);
+      }
+    }
+    auto lap = [&](const std::vector<V2<R>>& vv, std::vector<V2<R>>& out) {  // pbf.rs:359-377
+      for (size_t i = 0; i < n; i++) {
+        V2<R> s;
+        for (uint32_t k = nbr.off[i]; k < nbr.off[i + 1]; k++) {
+          V2<R> e = D[k];
+          s += (F(4.0) * W[k] * e.dot(vv[i] - vv[nbr.nb[k]])) * e;
+        }
+        out[i] = s;
+      }
+    };
+    // solve_implicit_viscosity (pbf.rs:423-480) with tol and max_cg from the workload
+    A bn = std::sqrt(dotv(v, v));
+    last_bnorm = bn;
+    last_res = 0;
+    if (bn <= (A)0 || !std::isfinite(bn)) return;
+    A target = (A)cfg.tol * bn;
+    std::vector<V2<R>> lv(n), r(n), p(n), ap(n);
+    lap(v, lv);
+    for (size_t i = 0; i < n; i++) r[i] = v[i] - (v[i] + lv[i] * dt);
+    p = r;
+    A rs_old = dotv(r, r);
+    last_res = std::sqrt(rs_old);
+    if (std::sqrt(rs_old) <= target) return;
+    for (uint32_t it = 1; it <= cfg.max_cg; it++) {
+      last_cg = it;
+      lap(p, lv);
+      for (size_t i = 0; i < n; i++) ap[i] = p[i] + lv[i] * dt;
+      A pap = dotv(p, ap);
+      if (pap <= (A)0 || !std::isfinite(pap)) return;
+      R alpha = (R)(rs_old / pap);
+      for (size_t i = 0; i < n; i++) { v[i] += p[i] * alpha; r[i] -= ap[i] * alpha; }
+      A rs_new = dotv(r, r);
+      last_res = std::sqrt(rs_new);
+      if (std::sqrt(rs_new) <= target) return;
+      R bta = (R)(rs_new / rs_old);
+      for (size_t i = 0; i < n; i++) p[i] = r[i] + p[i] * bta;
+      rs_old = rs_new;
+    }
+  }
+};
+
+}  // namespace orc
+
+// ---- C API (ctypes) for the kernel-isolation oracle. mode: 1 = mixed (f32, f64 dots), 2 = fp64.
+struct C3Handle {
+  int mode;
+  void* p;
+};
+static orc::C3Config c3_cfg_from(const double* c) {
+  orc::C3Config k;
+  k.side = (uint32_t)c[0]; k.dx = (float)c[1]; k.rest = (float)c[2]; k.mu = (float)c[3]; k.dt = (float)c[4];
+  k.passes = (uint32_t)c[5]; k.tol = (float)c[6]; k.max_cg = (uint32_t)c[7]; k.seed = (uint64_t)c[8];
+  return k;
+}
+#define C3DISPATCH(h, ...)                                                                   \
+  do {                                                                                       \
+    if ((h)->mode == 2) { auto* s = (orc::C3Sim<double, double>*)(h)->p; __VA_ARGS__; }      \
+    else if ((h)->mode == 0) { auto* s = (orc::C3Sim<float, float>*)(h)->p; __VA_ARGS__; }   \
+    else { auto* s = (orc::C3Sim<float, double>*)(h)->p; __VA_ARGS__; }                      \
+  } while (0)
+
+extern "C" {
+// cfg: side, dx, rest, mu, dt, passes, tol, max_cg, seed. mode: 0 = f32/f32, 1 = f32/f64, 2 = f64.
+void* c3_create(const double* cfg, int mode) {
+  auto* h = new C3Handle{mode, nullptr};
+  orc::C3Config c = c3_cfg_from(cfg);
+  if (mode == 2) { auto* s = new orc::C3Sim<double, double>(); s->init(c); h->p = s; }
+  else if (mode == 0) { auto* s = new orc::C3Sim<float, float>(); s->init(c); h->p = s; }
+  else { auto* s = new orc::C3Sim<float, double>(); s->init(c); h->p = s; }
+  return h;
+}
+void c3_destroy(void* hv) {
+  auto* h = (C3Handle*)hv;
+  if (!h) return;
+  C3DISPATCH(h, delete s);
+  delete h;
+}
+// out: h, L, cw, mass, ncx
+void c3_box(const double* cfg, double* out) {
+  orc::C3Box b = orc::C3Box::make(c3_cfg_from(cfg));
+  out[0] = b.h; out[1] = b.L; out[2] = b.cw; out[3] = b.mass; out[4] = b.ncx;
+}
+void c3_step(void* hv, int n) {
+  auto* h = (C3Handle*)hv;
+  for (int k = 0; k < n; k++) C3DISPATCH(h, s->step());
+}
+size_t c3_count(void* hv) {
+  auto* h = (C3Handle*)hv;
+  size_t n = 0;
+  C3DISPATCH(h, n = s->x.size());
+  return n;
+}
+void c3_state(void* hv, double* x2n, double* v2n) {
+  auto* h = (C3Handle*)hv;
+  C3DISPATCH(h, {
+    for (size_t i = 0; i < s->x.size(); i++) {
+      x2n[2 * i] = (double)s->x[i].x; x2n[2 * i + 1] = (double)s->x[i].y;
+      v2n[2 * i] = (double)s->v[i].x; v2n[2 * i + 1] = (double)s->v[i].y;
+    }
+  });
+}
+void c3_set_state(void* hv, const double* x2n, const double* v2n) {
+  auto* h = (C3Handle*)hv;
+  C3DISPATCH(h, {
+    using RR = decltype(s->x[0].x);
+    for (size_t i = 0; i < s->x.size(); i++) {
+      s->x[i].x = (RR)x2n[2 * i]; s->x[i].y = (RR)x2n[2 * i + 1];
+      s->v[i].x = (RR)v2n[2 * i]; s->v[i].y = (RR)v2n[2 * i + 1];
+    }
+  });
+}
+// out: cg iterations, final residual norm, |b|
+void c3_stats(void* hv, double* out) {
+  auto* h = (C3Handle*)hv;
+  C3DISPATCH(h, { out[0] = s->last_cg; out[1] = (double)s->last_res; out[2] = (double)s->last_bnorm; });
+}
+void c3_density(void* hv, double* out) {
+  auto* h = (C3Handle*)hv;
+  C3DISPATCH(h, { for (size_t i = 0; i < s->dens.size(); i++) out[i] = (double)s->dens[i]; });
+}
+// Neighbour CSR of the last step; returns the edge count (copies only when it fits nb_cap).
+int64_t c3_neighbors(void* hv, uint32_t* off, uint32_t* nb, int64_t nb_cap) {
+  auto* h = (C3Handle*)hv;
+  int64_t e = 0;
+  C3DISPATCH(h, {
+    e = (int64_t)s->nbr.nb.size();
+    std::memcpy(off, s->nbr.off.data(), s->nbr.off.size() * 4);
+    if (e <= nb_cap) std::memcpy(nb, s->nbr.nb.data(), (size_t)e * 4);
+  });
+  return e;
+}
+// Neighbour CSR over caller points in a box (cfg gives side/dx; L and cells follow from them).
+int64_t c3_neighbors_of(const double* cfg, const float* pts, size_t n, uint32_t* off, uint32_t* nb, int64_t nb_cap) {
+  orc::C3Box b = orc::C3Box::make(c3_cfg_from(cfg));
+  std::vector<orc::V2<float>> p(n);
+  for (size_t i = 0; i < n; i++) p[i] = orc::V2<float>(pts[2 * i], pts[2 * i + 1]);
+  orc::Nbrs L = orc::c3_neighbors(p, b);
+  std::memcpy(off, L.off.data(), L.off.size() * 4);
+  if ((int64_t)L.nb.size() <= nb_cap) std::memcpy(nb, L.nb.data(), L.nb.size() * 4);
+  return (int64_t)L.nb.size();
+}
+// Initial state of the workload (f32), for checking the product's seeding.
+void c3_seed_state(const double* cfg, float* x2n, float* v2n) {
+  orc::C3Config c = c3_cfg_from(cfg);
+  orc::C3Box b = orc::C3Box::make(c);
+  std::vector<float> x, v;
+  orc::c3_seed(c, b, x, v);
+  std::memcpy(x2n, x.data(), x.size() * 4);
+  std::memcpy(v2n, v.data(), v.size() * 4);
+}
+}

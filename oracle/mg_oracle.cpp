@@ -2109,3 +2109,6 @@ int orc_toi_swept(const double* drum, const double* x0, const double* x1, double
 double orc_squeeze_coefficient(double k1, double k2, double h, double nu) { return SqueezeFilm(k1, k2, nu).coefficient(h); }
 double orc_resolved_ratio(double hc) { return resolved_ratio(hc); }
 }
+
+// Kernel-isolation workload (periodic box): see c3_oracle.cpp.
+#include "c3_oracle.cpp"

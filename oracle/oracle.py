@@ -367,3 +367,124 @@ def squeeze_coefficient(k1, k2, h, nu):
 
 def resolved_ratio(hc):
     return oracle_lib().orc_resolved_ratio(hc)
+
+
+# ---- kernel-isolation workload (periodic box), c3_oracle.cpp --------------------------------
+C3_DEFAULTS = {"side": 1024, "dx": 1.0e-3, "rest": 1000.0, "mu": 1.0e-3, "dt": 1.0 / 1440.0,
+               "passes": 4, "tol": 1.0e-6, "max_cg": 500, "seed": 1}
+_C3_KEYS = ["side", "dx", "rest", "mu", "dt", "passes", "tol", "max_cg", "seed"]
+C3_MODES = {"ref": 0, "mixed": 1, "fp64": 2}
+
+
+def c3_config(overrides=None) -> dict:
+    c = dict(C3_DEFAULTS)
+    c.update(overrides or {})
+    return c
+
+
+def _c3_arr(cfg) -> np.ndarray:
+    c = c3_config(cfg)
+    return np.array([float(c[k]) for k in _C3_KEYS], dtype=np.float64)
+
+
+def _c3_lib():
+    L = oracle_lib()
+    if getattr(L, "_c3_ready", False):
+        return L
+    P = C.c_void_p
+    dp = C.POINTER(C.c_double)
+    fp = C.POINTER(C.c_float)
+    u32p = C.POINTER(C.c_uint32)
+    L.c3_create.restype = P
+    L.c3_create.argtypes = [dp, C.c_int]
+    L.c3_destroy.argtypes = [P]
+    L.c3_box.argtypes = [dp, dp]
+    L.c3_step.argtypes = [P, C.c_int]
+    L.c3_count.restype = C.c_size_t
+    L.c3_count.argtypes = [P]
+    L.c3_state.argtypes = [P, dp, dp]
+    L.c3_set_state.argtypes = [P, dp, dp]
+    L.c3_stats.argtypes = [P, dp]
+    L.c3_density.argtypes = [P, dp]
+    L.c3_neighbors.restype = C.c_int64
+    L.c3_neighbors.argtypes = [P, u32p, u32p, C.c_int64]
+    L.c3_neighbors_of.restype = C.c_int64
+    L.c3_neighbors_of.argtypes = [dp, fp, C.c_size_t, u32p, u32p, C.c_int64]
+    L.c3_seed_state.argtypes = [dp, fp, fp]
+    L._c3_ready = True
+    return L
+
+
+def c3_box(cfg=None) -> dict:
+    """Box geometry: kernel radius h, side L, cell width cw, particle mass, cells per side."""
+    out = np.zeros(5)
+    _c3_lib().c3_box(_dp(_c3_arr(cfg)), _dp(out))
+    return {"h": out[0], "L": out[1], "cw": out[2], "mass": out[3], "ncx": int(out[4])}
+
+
+def c3_seed_state(cfg=None):
+    c = c3_config(cfg)
+    n = int(c["side"]) ** 2
+    x = np.zeros(2 * n, dtype=np.float32)
+    v = np.zeros(2 * n, dtype=np.float32)
+    _c3_lib().c3_seed_state(_dp(_c3_arr(c)), _fp(x), _fp(v))
+    return x.reshape(n, 2), v.reshape(n, 2)
+
+
+def c3_neighbors_of(cfg, points):
+    """Neighbour CSR (offsets, neighbours) of caller points in the workload's periodic box."""
+    pts = np.ascontiguousarray(points, dtype=np.float32).reshape(-1)
+    n = pts.size // 2
+    off = np.zeros(n + 1, dtype=np.uint32)
+    cap = 64 * n + 64
+    nb = np.zeros(cap, dtype=np.uint32)
+    e = _c3_lib().c3_neighbors_of(_dp(_c3_arr(cfg)), _fp(pts), n, _u32(off), _u32(nb), cap)
+    assert e <= cap
+    return off, nb[:e].copy()
+
+
+class C3Oracle:
+    """The periodic-box kernel-isolation workload on the CPU (test infrastructure only)."""
+
+    def __init__(self, cfg=None, mode: str = "mixed"):
+        self.cfg = c3_config(cfg)
+        self._L = _c3_lib()
+        self._h = self._L.c3_create(_dp(_c3_arr(self.cfg)), C3_MODES[mode])
+        self.n = int(self._L.c3_count(self._h))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._L.c3_destroy(self._h)
+            self._h = None
+
+    def step(self, n: int = 1):
+        self._L.c3_step(self._h, n)
+
+    def state(self):
+        x = np.zeros(2 * self.n)
+        v = np.zeros(2 * self.n)
+        self._L.c3_state(self._h, _dp(x), _dp(v))
+        return x.reshape(self.n, 2), v.reshape(self.n, 2)
+
+    def set_state(self, x, v):
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+        self._L.c3_set_state(self._h, _dp(x), _dp(v))
+
+    def stats(self) -> dict:
+        o = np.zeros(3)
+        self._L.c3_stats(self._h, _dp(o))
+        return {"cg_iterations": int(o[0]), "residual": o[1], "b_norm": o[2]}
+
+    def density(self) -> np.ndarray:
+        o = np.zeros(self.n)
+        self._L.c3_density(self._h, _dp(o))
+        return o
+
+    def neighbors(self):
+        off = np.zeros(self.n + 1, dtype=np.uint32)
+        cap = 64 * self.n + 64
+        nb = np.zeros(cap, dtype=np.uint32)
+        e = self._L.c3_neighbors(self._h, _u32(off), _u32(nb), cap)
+        assert e <= cap
+        return off, nb[:e].copy()

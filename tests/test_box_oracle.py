@@ -1,0 +1,134 @@
+"""Pins of the periodic-box oracle (oracle/c3_oracle.cpp) without a GPU.
+
+The periodic box has no reference trace, so its oracle is pinned by what fixes it independently:
+the reference grid's own neighbour order on inputs that never wrap, brute force on small inputs,
+the lattice closed form of the mass, conservation laws of the density and viscosity operators, the
+CG's own stopping rule, and the FP64 policy.
+"""
+import numpy as np
+import pytest
+
+
+def _brute_neighbors(pts, L, h):
+    n = len(pts)
+    d = pts[:, None, :].astype(np.float32) - pts[None, :, :].astype(np.float32)
+    half = np.float32(0.5) * np.float32(L)
+    d = np.where(d > half, d - np.float32(L), np.where(d < -half, d + np.float32(L), d)).astype(np.float32)
+    r2 = (d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]).astype(np.float32)
+    m = r2 <= np.float32(h) * np.float32(h)
+    np.fill_diagonal(m, False)
+    return [set(np.nonzero(m[i])[0].tolist()) for i in range(n)]
+
+
+def test_neighbors_match_brute_force_with_wrap(orc):
+    cfg = {"side": 24}
+    box = orc.c3_box(cfg)
+    rng = np.random.default_rng(5)
+    pts = (rng.random((600, 2)) * box["L"]).astype(np.float32)
+    pts[:40] = np.float32(box["L"]) * np.float32(0.999)  # pile some points at the wrap seam
+    pts[40:80, 0] = np.float32(0.0)
+    off, nb = orc.c3_neighbors_of(cfg, pts)
+    brute = _brute_neighbors(pts, box["L"], box["h"])
+    for i in range(len(pts)):
+        row = nb[off[i]:off[i + 1]].tolist()
+        assert len(row) == len(set(row))
+        assert set(row) == brute[i]
+
+
+def test_neighbor_order_is_reference_order_without_wrap(orc):
+    # h = 0.25 and L = 8 are exact, so the box cells are the reference grid's cells floor(x/h);
+    # points far from the seam never wrap, and the reference emission order (grid.rs:291-362 via
+    # pbf.rs:262-288, restated by orc_neighbor_lists) must come out entry for entry.
+    cfg = {"side": 64, "dx": 0.125}
+    box = orc.c3_box(cfg)
+    assert box["h"] == 0.25 and box["L"] == 8.0 and box["cw"] == 0.25
+    rng = np.random.default_rng(11)
+    pts = (2.0 + rng.random((900, 2)) * 4.0).astype(np.float32)
+    off, nb = orc.c3_neighbors_of(cfg, pts)
+    roff, rnb = orc.neighbor_lists(pts, box["h"])
+    assert np.array_equal(off, roff)
+    assert np.array_equal(nb, rnb)
+
+
+def test_mass_from_lattice_closed_form(orc):
+    box = orc.c3_box({"side": 32})
+    dx = 1.0e-3
+    h = 2 * dx
+    # Poly6 4/(pi h^8) (h^2 - r^2)^3 over the 9 lattice sites with r < h (r = 0, dx, sqrt(2) dx)
+    w = 4.0 / (np.pi * h ** 8)
+    s = w * (h ** 6 + 4 * (h * h - dx * dx) ** 3 + 4 * (h * h - 2 * dx * dx) ** 3)
+    assert box["mass"] == pytest.approx(1.02 * 1000.0 / s, rel=1e-5)
+    assert box["ncx"] == 16
+
+
+def test_seed_lattice_jitter_and_velocity_statistics(orc):
+    cfg = {"side": 128}
+    x, v = orc.c3_seed_state(cfg)
+    x2, v2 = orc.c3_seed_state(cfg)
+    assert np.array_equal(x, x2) and np.array_equal(v, v2)
+    dx = 1.0e-3
+    ix = np.tile(np.arange(128), 128)
+    iy = np.repeat(np.arange(128), 128)
+    jx = x[:, 0] - (ix + 0.5) * dx
+    jy = x[:, 1] - (iy + 0.5) * dx
+    assert np.abs(jx).max() <= 0.1 * dx * (1 + 1e-4)
+    assert np.abs(jy).max() <= 0.1 * dx * (1 + 1e-4)
+    assert abs(v.mean()) < 3e-4
+    assert v.std() == pytest.approx(0.01, rel=0.03)
+
+
+def test_unjittered_density_is_two_percent_over_rest(orc):
+    # The first pass's rho on the exact lattice is 1.02 rho0 for every particle; with zero
+    # velocity and jitter-free positions (set_state), the first lambda sees C = 0.02 everywhere,
+    # so one pass moves nothing (every dp cancels by symmetry).
+    cfg = {"side": 32, "passes": 1, "mu": 0.0}
+    s = orc.C3Oracle(cfg, "fp64")
+    dx = float(np.float32(1.0e-3))  # the box is built from the f32 spacing
+    ix = np.tile(np.arange(32), 32)
+    iy = np.repeat(np.arange(32), 32)
+    x = np.stack([(ix + 0.5) * dx, (iy + 0.5) * dx], 1)
+    s.set_state(x, np.zeros_like(x))
+    s.step()
+    xn, vn = s.state()
+    assert np.abs(xn - x).max() < 1e-15
+    assert np.allclose(s.density(), 1020.0, rtol=1e-6)
+
+
+@pytest.mark.parametrize("mu", [1.0e-3, 1.0])
+def test_step_conserves_momentum_and_cg_meets_tolerance(orc, mu):
+    cfg = {"side": 48, "mu": mu}
+    s = orc.C3Oracle(cfg, "fp64")
+    x0, v0 = s.state()
+    s.step()
+    x1, v1 = s.state()
+    st = s.stats()
+    # density passes and the viscous operator are antisymmetric in (i, j): total momentum stays
+    assert np.abs(v1.sum(0) - v0.sum(0)).max() < 1e-9 * np.abs(v0).sum()
+    assert st["residual"] <= 1e-6 * st["b_norm"] * (1 + 1e-12)
+    assert st["cg_iterations"] >= 1
+    if mu >= 1.0:
+        assert st["cg_iterations"] > 5
+
+
+def test_viscosity_dissipates_energy(orc):
+    cfg = {"side": 48, "passes": 4}
+    a = orc.C3Oracle(dict(cfg, mu=0.0), "fp64")
+    b = orc.C3Oracle(dict(cfg, mu=1.0), "fp64")
+    a.step()
+    b.step()
+    _, va = a.state()
+    _, vb = b.state()
+    assert (vb ** 2).sum() < (va ** 2).sum()
+
+
+def test_mixed_policy_tracks_fp64(orc):
+    cfg = {"side": 48, "mu": 1.0}
+    m = orc.C3Oracle(cfg, "mixed")
+    d = orc.C3Oracle(cfg, "fp64")
+    m.step()
+    d.step()
+    xm, vm = m.state()
+    xd, vd = d.state()
+    L = orc.c3_box(cfg)["L"]
+    assert np.abs(xm - xd).max() / L < 1e-6
+    assert np.abs(vm - vd).max() / np.abs(vd).max() < 1e-3
