@@ -37,9 +37,26 @@ HBM_PEAK_GBS = 8000.0
 CLASS_NAMES = ["grid+reorder", "density_A", "density_B", "velocity+density", "cg_setup", "cg_apply",
                "cg_update", "cg_direction"]
 CLASS_BYTES = [138.0, 12.0, 20.0, 28.0, 52.0, 28.0, 32.0, 24.0]
+CLASS_KERNELS = [None, "bx_lambda", "bx_dp", "bx_vel_density", "bx_cg_init", "bx_cg_spmv", "bx_cg_update", "bx_cg_p"]
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "box_1024_pmc_traffic.md")
 C2_PARAMS = {"mill": {"diameter_m": 0.1397, "speed_mode": "percent_critical", "speed_value": 75.0},
              "media": {"density_kg_m3": 7800.0},
              "simulation": {"resolution": 100, "max_balls": 2000}}
+
+
+def measured_traffic(kernel, side):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
+    (calibrated against a plain copy), or None when no summary covers this workload."""
+    if kernel is None or side != 1024 or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    for line in open(TRAFFIC_PROFILE):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 5 and cells[0] == kernel:
+            try:
+                return float(cells[4]) * 1024 * 1024
+            except ValueError:
+                return None
+    return None
 
 
 def cpu_count_used() -> int:
@@ -85,6 +102,8 @@ def main() -> int:
     ap.add_argument("--mu", type=float, default=1.0e-3)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="also run one plain device copy of this size (calibrates PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -102,6 +121,12 @@ def main() -> int:
         return 1
     torch.cuda.set_device(0)
 
+    if args.copy_mib > 0:
+        src = torch.ones(args.copy_mib * 262144, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     cfg = {"side": args.side, "mu": args.mu}
     box = Box(cfg)
     n = box.n
@@ -134,7 +159,8 @@ def main() -> int:
                                 "avg_us": round(1000.0 * float(ms[i]) / max(int(launches[i]), 1), 2)}
                for i in range(8)}
     roofline = {"bound": "hbm", "kernel": CLASS_NAMES[k], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(CLASS_KERNELS[k], args.side),
                 "bytes_per_launch": CLASS_BYTES[k] * n}
     iso = {"metric": "particle-steps/s", "value": round(n * args.steps / wall, 1), "ms_per_step": round(ms_per_step, 4),
            "hbm_gbs_algorithmic": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
