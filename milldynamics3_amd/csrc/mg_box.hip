@@ -38,6 +38,9 @@
 #define BX_SCAN_ITEMS 16
 #define BX_SCAN_CHUNK (BX_BLOCK * BX_SCAN_ITEMS)
 #define BX_CG_CHUNK 4
+#ifndef BX_KMAX
+#define BX_KMAX 32  // pairs kept per particle in the per-step pair table (more: the kernel rescans)
+#endif
 // kernel classes of the optional per-kernel timing (mg_box_kernel_times)
 #define BX_NCLASS 8
 enum { BXT_GRID = 0, BXT_LAMBDA, BXT_DP, BXT_VEL, BXT_CGINIT, BXT_SPMV, BXT_UPDATE, BXT_P };
@@ -54,7 +57,8 @@ struct BoxGeo {
 struct BoxScal {
   double bnorm, target, rs_old, rs_new, pap, res;
   float alpha, beta;
-  uint32_t it, done, max_cg, pad;
+  uint32_t it, done, max_cg;
+  float tol;
 };
 
 // ---------------------------------------------------------------------------------------------
@@ -123,6 +127,93 @@ BDEV double block_sum(double v) {
   return r;
 }
 
+
+// Per-step pair table: the neighbours of sorted particle i at the predicted positions, in the
+// reference's per-particle order, column-major (nb[k * n + i]) so lane-adjacent particles read
+// adjacent words. cnt[i] is the true count; a particle with more than BX_KMAX pairs is handled by
+// rescanning its 3x3 cells (same order, same membership test).
+struct PairView {
+  const float2* __restrict__ XP;
+  const uint32_t* __restrict__ start;
+  const uint32_t* __restrict__ nb;
+  const uint32_t* __restrict__ cnt;
+};
+template <class F>
+BDEV void scan_pairs(const BoxGeo& g, const float2* __restrict__ XP, const uint32_t* __restrict__ start, uint32_t i,
+                     float2 xpi, F&& f) {
+  const uint32_t cx = cell1(xpi.x, g), cy = cell1(xpi.y, g);
+  for (int o = 0; o < 9; o++) {
+    const uint32_t k = nbr_cell(cx, cy, o, g.ncx);
+    const uint32_t s = start[k], e = start[k + 1];
+    for (uint32_t j = s; j < e; j++) {
+      if (j == i) continue;
+      if (len2(mi2(sub2(xpi, XP[j]), g)) > g.h2) continue;
+      f(j);
+    }
+  }
+}
+template <class F>
+BDEV void for_pairs(const BoxGeo& g, const PairView& pv, uint32_t i, F&& f) {
+  const uint32_t c = pv.cnt[i];
+  if (c <= BX_KMAX) {
+    for (uint32_t k = 0; k < c; k++) f(pv.nb[(size_t)k * g.n + i]);
+  } else {
+    scan_pairs(g, pv.XP, pv.start, i, pv.XP[i], f);
+  }
+}
+
+// Single-block reductions of the per-block partials and the CG scalar logic.
+BDEV double sum_parts(const double* __restrict__ part, uint32_t nb) {
+  double s = 0.0;
+  for (uint32_t k = threadIdx.x; k < nb; k += BX_BLOCK) s += part[k];
+  return block_sum(s);
+}
+// CG scalar logic after a reduction (pbf.rs:423-480). stage 0: |b| and the target; stage 1: the
+// initial residual; stage 2: p.Ap -> alpha; stage 3: r.r -> convergence test, beta.
+BDEV void cg_scalar(BoxScal* __restrict__ sc, int stage, double s) {
+  if (stage == 0) {
+    const double bn = sqrt(s);
+    sc->bnorm = bn;
+    sc->res = 0.0;
+    sc->it = 0;
+    sc->target = (double)sc->tol * bn;
+    if (!(bn > 0.0) || !isfinite(bn)) sc->done = 1u;
+  } else if (stage == 1) {
+    sc->rs_old = s;
+    sc->res = sqrt(s);
+    if (sqrt(s) <= sc->target) sc->done = 1u;
+  } else if (stage == 2) {
+    sc->it += 1u;
+    sc->pap = s;
+    if (!(s > 0.0) || !isfinite(s)) {
+      sc->done = 1u;
+    } else {
+      sc->alpha = (float)(sc->rs_old / s);
+    }
+  } else {
+    sc->rs_new = s;
+    sc->res = sqrt(s);
+    if (sqrt(s) <= sc->target || sc->it >= sc->max_cg) {
+      sc->done = 1u;
+    } else {
+      sc->beta = (float)(s / sc->rs_old);
+      sc->rs_old = s;
+    }
+  }
+}
+// Every block stores its fixed-order partial; one single-block kernel then sums them in block
+// order. (A last-arriving-block reduction needs a device-scope release fence per block, which on
+// this multi-XCD part writes back the L2 and cost more than the extra launch.)
+BDEV void store_partial(double v, double* __restrict__ part) {
+  const double t = block_sum(v);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+BXK bx_cg_scalar(const double* __restrict__ part, uint32_t nb, BoxScal* __restrict__ sc, int stage) {
+  if (sc->done) return;
+  const double s = sum_parts(part, nb);
+  if (threadIdx.x == 0) cg_scalar(sc, stage, s);
+}
+
 // ---------------------------------------------------------------------------------------------
 // grid build + reorder
 // ---------------------------------------------------------------------------------------------
@@ -131,17 +222,37 @@ BXK bx_fill_u32(uint32_t* __restrict__ a, uint32_t n, uint32_t v) {
   if (i < n) a[i] = v;
 }
 
+// Lanes of one wave holding the same key form runs (the particles arrive in the previous step's
+// cell order, so most of a wave's keys repeat); one atomic per run instead of one per particle.
+// Returns the run's length on its head lane and 0 elsewhere; head = lane of the run's first entry.
+BDEV uint32_t wave_run(uint32_t key, uint32_t& head) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t prev = __shfl_up(key, 1, 64);
+  const bool is_head = lane == 0 || prev != key;
+  const uint64_t heads = __ballot(is_head);
+  const uint64_t upto = heads & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+  head = 63u - (uint32_t)__clzll(upto);
+  if (!is_head) return 0u;
+  const uint64_t above = lane == 63 ? 0ull : (heads & (~0ull << (lane + 1)));
+  const uint32_t next = above ? (uint32_t)(__ffsll((unsigned long long)above) - 1) : 64u;
+  return next - (uint32_t)lane;
+}
+
 BXK bx_predict(BoxGeo g, float2* __restrict__ X, const float2* __restrict__ V, float2* __restrict__ X0,
                uint32_t* __restrict__ key, uint32_t* __restrict__ count) {
-  uint32_t i = blockIdx.x * BX_BLOCK + threadIdx.x;
-  if (i >= g.n) return;
-  float2 x = X[i], v = V[i];
-  X0[i] = x;
-  x = f2(wrapf(x.x + v.x * g.dt, g.L), wrapf(x.y + v.y * g.dt, g.L));
-  X[i] = x;
-  uint32_t k = cell1(x.y, g) * g.ncx + cell1(x.x, g);
-  key[i] = k;
-  atomicAdd(&count[k], 1u);
+  const uint32_t i = blockIdx.x * BX_BLOCK + threadIdx.x;
+  uint32_t k = 0xFFFFFFFFu;
+  if (i < g.n) {
+    float2 x = X[i], v = V[i];
+    X0[i] = x;
+    x = f2(wrapf(x.x + v.x * g.dt, g.L), wrapf(x.y + v.y * g.dt, g.L));
+    X[i] = x;
+    k = cell1(x.y, g) * g.ncx + cell1(x.x, g);
+    key[i] = k;
+  }
+  uint32_t head;
+  const uint32_t len = wave_run(k, head);
+  if (len && k != 0xFFFFFFFFu) atomicAdd(&count[k], len);
 }
 
 // Exclusive scan of a[0..m) in three launches: chunk totals, scan of the totals, chunk scan.
@@ -201,10 +312,14 @@ BXK bx_scan_apply(uint32_t* __restrict__ a, uint32_t m, const uint32_t* __restri
 }
 BXK bx_scatter(BoxGeo g, const uint32_t* __restrict__ key, const uint32_t* __restrict__ start,
                uint32_t* __restrict__ cursor, uint32_t* __restrict__ slot) {
-  uint32_t i = blockIdx.x * BX_BLOCK + threadIdx.x;
-  if (i >= g.n) return;
-  uint32_t k = key[i];
-  slot[start[k] + atomicAdd(&cursor[k], 1u)] = i;
+  const uint32_t i = blockIdx.x * BX_BLOCK + threadIdx.x;
+  const uint32_t k = i < g.n ? key[i] : 0xFFFFFFFFu;
+  uint32_t head;
+  const uint32_t len = wave_run(k, head);
+  uint32_t base = 0;
+  if (len && k != 0xFFFFFFFFu) base = start[k] + atomicAdd(&cursor[k], len);
+  base = __shfl(base, (int)head, 64);
+  if (i < g.n) slot[base + ((threadIdx.x & 63u) - head)] = i;
 }
 // One lane per cell: order the cell's slots by particle id (cells hold a handful of particles).
 BXK bx_cellsort(uint32_t ncell, const uint32_t* __restrict__ start, uint32_t* __restrict__ slot,
@@ -235,56 +350,53 @@ BXK bx_gather(BoxGeo g, const uint32_t* __restrict__ slot, const float2* __restr
   idb[q] = id[i];
 }
 
+BXK bx_pairs(BoxGeo g, const float2* __restrict__ XP, const uint32_t* __restrict__ start, uint32_t* __restrict__ nb,
+              uint32_t* __restrict__ cnt) {
+  const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
+  if (i >= g.n) return;
+  uint32_t c = 0;
+  scan_pairs(g, XP, start, i, XP[i], [&](uint32_t j) __attribute__((always_inline)) {
+    if (c < BX_KMAX) nb[(size_t)c * g.n + i] = j;
+    c++;
+  });
+  cnt[i] = c;
+}
+
 // ---------------------------------------------------------------------------------------------
 // density passes (pbf.rs:1257-1334)
 // ---------------------------------------------------------------------------------------------
-BXK bx_lambda(BoxGeo g, const float2* __restrict__ XP, const float2* __restrict__ X, const uint32_t* __restrict__ start,
-              float* __restrict__ LAM) {
+BXK bx_lambda(BoxGeo g, PairView pv, const float2* __restrict__ X, float* __restrict__ LAM) {
   const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
   if (i >= g.n) return;
-  const float2 xpi = XP[i], xi = X[i];
-  const uint32_t cx = cell1(xpi.x, g), cy = cell1(xpi.y, g);
+  const float2 xi = X[i];
   float rho = g.w0;
   float2 gs = f2(0.0f, 0.0f);
   float sgs = 0.0f;
-  for (int o = 0; o < 9; o++) {
-    const uint32_t k = nbr_cell(cx, cy, o, g.ncx);
-    const uint32_t s = start[k], e = start[k + 1];
-    for (uint32_t j = s; j < e; j++) {
-      if (j == i) continue;
-      if (len2(mi2(sub2(xpi, XP[j]), g)) > g.h2) continue;
-      const float2 d = mi2(sub2(xi, X[j]), g);
-      const float r2 = len2(d);
-      rho += g.mass * poly6(r2, g);
-      const float r = sqrtf(r2);
-      const float2 gr = mul2(spiky(d, r, g), g.m_over_rest);
-      gs = add2(gs, gr);
-      sgs += len2(gr);
-    }
-  }
+  for_pairs(g, pv, i, [&](uint32_t j) __attribute__((always_inline)) {
+    const float2 d = mi2(sub2(xi, X[j]), g);
+    const float r2 = len2(d);
+    rho += g.mass * poly6(r2, g);
+    const float r = sqrtf(r2);
+    const float2 gr = mul2(spiky(d, r, g), g.m_over_rest);
+    gs = add2(gs, gr);
+    sgs += len2(gr);
+  });
   const float c = fmaxf(rho / g.rest - 1.0f, 0.0f);
   sgs += len2(gs);
   LAM[i] = -c / (sgs + 200.0f);
 }
-BXK bx_dp(BoxGeo g, const float2* __restrict__ XP, const float2* __restrict__ X, const uint32_t* __restrict__ start,
-          const float* __restrict__ LAM, float2* __restrict__ Xout) {
+BXK bx_dp(BoxGeo g, PairView pv, const float2* __restrict__ X, const float* __restrict__ LAM,
+          float2* __restrict__ Xout) {
   const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
   if (i >= g.n) return;
-  const float2 xpi = XP[i], xi = X[i];
+  const float2 xi = X[i];
   const float li = LAM[i];
-  const uint32_t cx = cell1(xpi.x, g), cy = cell1(xpi.y, g);
   float2 s = f2(0.0f, 0.0f);
-  for (int o = 0; o < 9; o++) {
-    const uint32_t k = nbr_cell(cx, cy, o, g.ncx);
-    const uint32_t cs = start[k], ce = start[k + 1];
-    for (uint32_t j = cs; j < ce; j++) {
-      if (j == i) continue;
-      if (len2(mi2(sub2(xpi, XP[j]), g)) > g.h2) continue;
-      const float2 d = mi2(sub2(xi, X[j]), g);
-      const float r = sqrtf(len2(d));
-      s = add2(s, smul2(li + LAM[j], spiky(d, r, g)));
-    }
-  }
+  for_pairs(g, pv, i, [&](uint32_t j) __attribute__((always_inline)) {
+    const float2 d = mi2(sub2(xi, X[j]), g);
+    const float r = sqrtf(len2(d));
+    s = add2(s, smul2(li + LAM[j], spiky(d, r, g)));
+  });
   const float2 dp = mul2(s, g.m_over_rest);
   Xout[i] = f2(wrapf(xi.x + dp.x, g.L), wrapf(xi.y + dp.y, g.L));
 }
@@ -292,93 +404,111 @@ BXK bx_dp(BoxGeo g, const float2* __restrict__ XP, const float2* __restrict__ X,
 // ---------------------------------------------------------------------------------------------
 // velocity, density recompute, implicit viscosity
 // ---------------------------------------------------------------------------------------------
-BXK bx_vel_density(BoxGeo g, const float2* __restrict__ XP, const float2* __restrict__ X, const float2* __restrict__ X0,
-                   const uint32_t* __restrict__ start, float2* __restrict__ V, float* __restrict__ DENS,
-                   double* __restrict__ part) {
+BXK bx_vel_density(BoxGeo g, PairView pv, const float2* __restrict__ X, const float2* __restrict__ X0,
+                   float2* __restrict__ V, float* __restrict__ DENS, double* __restrict__ part,
+                   BoxScal* __restrict__ sc) {
   const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
   double vv = 0.0;
   if (i < g.n) {
-    const float2 xpi = XP[i], xi = X[i];
+    const float2 xi = X[i];
     const float2 v = div2(mi2(sub2(xi, X0[i]), g), g.dt);
     V[i] = v;
     vv = (double)v.x * (double)v.x + (double)v.y * (double)v.y;
-    const uint32_t cx = cell1(xpi.x, g), cy = cell1(xpi.y, g);
     float rho = g.w0;
-    for (int o = 0; o < 9; o++) {
-      const uint32_t k = nbr_cell(cx, cy, o, g.ncx);
-      const uint32_t s = start[k], e = start[k + 1];
-      for (uint32_t j = s; j < e; j++) {
-        if (j == i) continue;
-        if (len2(mi2(sub2(xpi, XP[j]), g)) > g.h2) continue;
-        rho += g.mass * poly6(len2(mi2(sub2(xi, X[j]), g)), g);
-      }
-    }
+    for_pairs(g, pv, i, [&](uint32_t j) __attribute__((always_inline)) {
+      rho += g.mass * poly6(len2(mi2(sub2(xi, X[j]), g)), g);
+    });
     DENS[i] = rho;
   }
-  const double t = block_sum(vv);
-  if (threadIdx.x == 0) part[blockIdx.x] = t;
+  if (sc->done) return;  // viscosity off: no CG this step
+  store_partial(vv, part);
 }
 
 // (L v)_i = Σ_j 4 c_ij e_ij (e_ij · (v_i - v_j)) with the Morris weight c_ij (pbf.rs:312-377).
-BDEV float2 lap_row(const BoxGeo& g, uint32_t i, const float2* __restrict__ XP, const float2* __restrict__ X,
-                    const float* __restrict__ DENS, const uint32_t* __restrict__ start, const float2* __restrict__ in) {
-  const float2 xpi = XP[i], xi = X[i], ini = in[i];
+// Morris weight and unit direction of pair (i, j) (pbf.rs:312-344).
+BDEV void morris(const BoxGeo& g, float2 xi, float rho_i, float2 xj, float dens_j, float& w, float2& e) {
+  const float rho_j = fmaxf(dens_j, 1e-6f);
+  const float2 d = mi2(sub2(xi, xj), g);
+  const float r = sqrtf(len2(d));
+  const float2 gr = spiky(d, r, g);
+  const float num = (g.mu2m / (rho_i * rho_j)) * fabsf(dot2(d, gr));
+  w = num / (len2(d) + g.eta2);
+  e = r > 1e-9f ? div2(d, r) : f2(1.0f, 0.0f);
+}
+// (L v)_i with the weights computed here; when `store` the weights of a tabled particle are kept
+// (column-major, like the pair table) for the operator applications of this step's CG.
+BDEV float2 lap_row(const BoxGeo& g, uint32_t i, const PairView& pv, const float2* __restrict__ X,
+                    const float* __restrict__ DENS, const float2* __restrict__ in, bool store,
+                    float* __restrict__ CW, float2* __restrict__ CE) {
+  const float2 xi = X[i], ini = in[i];
   const float rho_i = fmaxf(DENS[i], 1e-6f);
-  const uint32_t cx = cell1(xpi.x, g), cy = cell1(xpi.y, g);
+  const bool keep = store && pv.cnt[i] <= BX_KMAX;
   float2 s = f2(0.0f, 0.0f);
-  for (int o = 0; o < 9; o++) {
-    const uint32_t k = nbr_cell(cx, cy, o, g.ncx);
-    const uint32_t cs = start[k], ce = start[k + 1];
-    for (uint32_t j = cs; j < ce; j++) {
-      if (j == i) continue;
-      if (len2(mi2(sub2(xpi, XP[j]), g)) > g.h2) continue;
-      const float rho_j = fmaxf(DENS[j], 1e-6f);
-      const float2 d = mi2(sub2(xi, X[j]), g);
-      const float r = sqrtf(len2(d));
-      const float2 gr = spiky(d, r, g);
-      const float num = (g.mu2m / (rho_i * rho_j)) * fabsf(dot2(d, gr));
-      const float w = num / (len2(d) + g.eta2);
-      const float2 e = r > 1e-9f ? div2(d, r) : f2(1.0f, 0.0f);
-      s = add2(s, smul2(4.0f * w * dot2(e, sub2(ini, in[j])), e));
+  uint32_t k = 0;
+  for_pairs(g, pv, i, [&](uint32_t j) __attribute__((always_inline)) {
+    float w;
+    float2 e;
+    morris(g, xi, rho_i, X[j], DENS[j], w, e);
+    if (keep) {
+      CW[(size_t)k * g.n + i] = w;
+      CE[(size_t)k * g.n + i] = e;
     }
+    k++;
+    s = add2(s, smul2(4.0f * w * dot2(e, sub2(ini, in[j])), e));
+  });
+  return s;
+}
+// (L v)_i from the kept weights (same values, same order as lap_row).
+BDEV float2 lap_row_kept(const BoxGeo& g, uint32_t i, const PairView& pv, const float2* __restrict__ X,
+                         const float* __restrict__ DENS, const float2* __restrict__ in,
+                         const float* __restrict__ CW, const float2* __restrict__ CE) {
+  const uint32_t c = pv.cnt[i];
+  if (c > BX_KMAX) return lap_row(g, i, pv, X, DENS, in, false, nullptr, nullptr);
+  const float2 ini = in[i];
+  float2 s = f2(0.0f, 0.0f);
+  for (uint32_t k = 0; k < c; k++) {
+    const size_t q = (size_t)k * g.n + i;
+    const uint32_t j = pv.nb[q];
+    const float w = CW[q];
+    const float2 e = CE[q];
+    s = add2(s, smul2(4.0f * w * dot2(e, sub2(ini, in[j])), e));
   }
   return s;
 }
 
-BXK bx_cg_init(BoxGeo g, const float2* __restrict__ XP, const float2* __restrict__ X, const float* __restrict__ DENS,
-               const uint32_t* __restrict__ start, const float2* __restrict__ V, float2* __restrict__ R,
-               float2* __restrict__ P, double* __restrict__ part, const BoxScal* __restrict__ sc) {
+BXK bx_cg_init(BoxGeo g, PairView pv, const float2* __restrict__ X, const float* __restrict__ DENS,
+               const float2* __restrict__ V, float2* __restrict__ R, float2* __restrict__ P,
+               float* __restrict__ CW, float2* __restrict__ CE, double* __restrict__ part,
+               const BoxScal* __restrict__ sc) {
   if (sc->done) return;
   const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
   double rr = 0.0;
   if (i < g.n) {
     const float2 v = V[i];
-    const float2 lv = lap_row(g, i, XP, X, DENS, start, V);
+    const float2 lv = lap_row(g, i, pv, X, DENS, V, true, CW, CE);
     const float2 r = sub2(v, add2(v, mul2(lv, g.dt)));
     R[i] = r;
     P[i] = r;
     rr = (double)r.x * (double)r.x + (double)r.y * (double)r.y;
   }
-  const double t = block_sum(rr);
-  if (threadIdx.x == 0) part[blockIdx.x] = t;
+  store_partial(rr, part);
 }
-BXK bx_cg_spmv(BoxGeo g, const float2* __restrict__ XP, const float2* __restrict__ X, const float* __restrict__ DENS,
-               const uint32_t* __restrict__ start, const float2* __restrict__ P, float2* __restrict__ AP,
-               double* __restrict__ part, const BoxScal* __restrict__ sc) {
+BXK bx_cg_spmv(BoxGeo g, PairView pv, const float2* __restrict__ X, const float* __restrict__ DENS,
+               const float* __restrict__ CW, const float2* __restrict__ CE, const float2* __restrict__ P,
+               float2* __restrict__ AP, double* __restrict__ part, const BoxScal* __restrict__ sc) {
   if (sc->done) return;
   const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
   double pap = 0.0;
   if (i < g.n) {
     const float2 p = P[i];
-    const float2 ap = add2(p, mul2(lap_row(g, i, XP, X, DENS, start, P), g.dt));
+    const float2 ap = add2(p, mul2(lap_row_kept(g, i, pv, X, DENS, P, CW, CE), g.dt));
     AP[i] = ap;
     pap = (double)p.x * (double)ap.x + (double)p.y * (double)ap.y;
   }
-  const double t = block_sum(pap);
-  if (threadIdx.x == 0) part[blockIdx.x] = t;
+  store_partial(pap, part);
 }
 BXK bx_cg_update(BoxGeo g, float2* __restrict__ V, float2* __restrict__ R, const float2* __restrict__ P,
-                 const float2* __restrict__ AP, double* __restrict__ part, const BoxScal* __restrict__ sc) {
+                 const float2* __restrict__ AP, double* __restrict__ part, BoxScal* __restrict__ sc) {
   if (sc->done) return;
   const uint32_t i = blockIdx.x * BX_BLOCK + threadIdx.x;
   const float a = sc->alpha;
@@ -389,58 +519,17 @@ BXK bx_cg_update(BoxGeo g, float2* __restrict__ V, float2* __restrict__ R, const
     R[i] = r;
     rr = (double)r.x * (double)r.x + (double)r.y * (double)r.y;
   }
-  const double t = block_sum(rr);
-  if (threadIdx.x == 0) part[blockIdx.x] = t;
+  store_partial(rr, part);
 }
 BXK bx_cg_p(BoxGeo g, const float2* __restrict__ R, float2* __restrict__ P, const BoxScal* __restrict__ sc) {
   if (sc->done) return;
   const uint32_t i = blockIdx.x * BX_BLOCK + threadIdx.x;
   if (i < g.n) P[i] = add2(R[i], mul2(P[i], sc->beta));
 }
-// Single-block reductions of the per-block partials and the CG scalar logic.
-BDEV double sum_parts(const double* __restrict__ part, uint32_t nb) {
-  double s = 0.0;
-  for (uint32_t k = threadIdx.x; k < nb; k += BX_BLOCK) s += part[k];
-  return block_sum(s);
-}
-// stage 0: |b| and target; stage 1: initial residual; stage 2: p.Ap -> alpha; stage 3: r.r -> beta
-BXK bx_cg_scalar(const double* __restrict__ part, uint32_t nb, BoxScal* __restrict__ sc, int stage, float tol) {
-  if (sc->done) return;
-  const double s = sum_parts(part, nb);
-  if (threadIdx.x != 0) return;
-  if (stage == 0) {
-    const double bn = sqrt(s);
-    sc->bnorm = bn;
-    sc->res = 0.0;
-    sc->it = 0;
-    sc->target = (double)tol * bn;
-    if (!(bn > 0.0) || !isfinite(bn)) sc->done = 1u;
-  } else if (stage == 1) {
-    sc->rs_old = s;
-    sc->res = sqrt(s);
-    if (sqrt(s) <= sc->target) sc->done = 1u;
-  } else if (stage == 2) {
-    sc->it += 1u;
-    sc->pap = s;
-    if (!(s > 0.0) || !isfinite(s)) {
-      sc->done = 1u;
-    } else {
-      sc->alpha = (float)(sc->rs_old / s);
-    }
-  } else {
-    sc->rs_new = s;
-    sc->res = sqrt(s);
-    if (sqrt(s) <= sc->target || sc->it >= sc->max_cg) {
-      sc->done = 1u;
-    } else {
-      sc->beta = (float)(s / sc->rs_old);
-      sc->rs_old = s;
-    }
-  }
-}
-BXK bx_cg_reset(BoxScal* __restrict__ sc, uint32_t max_cg, uint32_t skip) {
+BXK bx_cg_reset(BoxScal* __restrict__ sc, uint32_t max_cg, uint32_t skip, float tol) {
   if (threadIdx.x != 0) return;
   sc->done = skip;
+  sc->tol = tol;
   sc->it = 0;
   sc->max_cg = max_cg;
   sc->res = 0.0;
@@ -450,35 +539,22 @@ BXK bx_cg_reset(BoxScal* __restrict__ sc, uint32_t max_cg, uint32_t skip) {
 // ---------------------------------------------------------------------------------------------
 // debug export: neighbour lists in the traversal order, rows and entries as original ids
 // ---------------------------------------------------------------------------------------------
-BXK bx_nbr_count(BoxGeo g, const float2* __restrict__ XP, const uint32_t* __restrict__ start,
-                 const uint32_t* __restrict__ id, uint32_t* __restrict__ cnt) {
+BXK bx_nbr_count(BoxGeo g, PairView pv, const uint32_t* __restrict__ id, uint32_t* __restrict__ cnt) {
   const uint32_t i = blockIdx.x * BX_BLOCK + threadIdx.x;
   if (i >= g.n) return;
-  const float2 xpi = XP[i];
-  const uint32_t cx = cell1(xpi.x, g), cy = cell1(xpi.y, g);
   uint32_t c = 0;
-  for (int o = 0; o < 9; o++) {
-    const uint32_t k = nbr_cell(cx, cy, o, g.ncx);
-    for (uint32_t j = start[k]; j < start[k + 1]; j++)
-      if (j != i && len2(mi2(sub2(xpi, XP[j]), g)) <= g.h2) c++;
-  }
+  for_pairs(g, pv, i, [&](uint32_t) __attribute__((always_inline)) { c++; });
   cnt[id[i]] = c;
 }
-BXK bx_nbr_write(BoxGeo g, const float2* __restrict__ XP, const uint32_t* __restrict__ start,
-                 const uint32_t* __restrict__ id, const uint32_t* __restrict__ off, uint32_t* __restrict__ nb, uint32_t cap) {
+BXK bx_nbr_write(BoxGeo g, PairView pv, const uint32_t* __restrict__ id, const uint32_t* __restrict__ off,
+                 uint32_t* __restrict__ nb, uint32_t cap) {
   const uint32_t i = blockIdx.x * BX_BLOCK + threadIdx.x;
   if (i >= g.n) return;
-  const float2 xpi = XP[i];
-  const uint32_t cx = cell1(xpi.x, g), cy = cell1(xpi.y, g);
   uint32_t w = off[id[i]];
-  for (int o = 0; o < 9; o++) {
-    const uint32_t k = nbr_cell(cx, cy, o, g.ncx);
-    for (uint32_t j = start[k]; j < start[k + 1]; j++)
-      if (j != i && len2(mi2(sub2(xpi, XP[j]), g)) <= g.h2) {
-        if (w < cap) nb[w] = id[j];
-        w++;
-      }
-  }
+  for_pairs(g, pv, i, [&](uint32_t j) __attribute__((always_inline)) {
+    if (w < cap) nb[w] = id[j];
+    w++;
+  });
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -494,6 +570,9 @@ struct mg_box {
            *bsum = nullptr;
   float *LAM = nullptr, *DENS = nullptr;
   float2 *R = nullptr, *P = nullptr, *AP = nullptr;
+  uint32_t *pnb = nullptr, *pcnt = nullptr;  // per-step pair table
+  float* cw = nullptr;                         // per-step Morris weights of tabled pairs
+  float2* ce = nullptr;                        // and their unit directions
   double* part = nullptr;
   BoxScal* sc = nullptr;
   BoxScal* h_sc = nullptr;  // pinned host mirror
@@ -501,6 +580,7 @@ struct mg_box {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool have_neighbors = false;
+  uint32_t last_it = 0;  // CG iterations of the previous step (sizes the first chunk)
   float last_ms = 0.0f;
   std::string err;
   // optional per-kernel timing: event pairs around launches, accumulated after each sync
@@ -662,29 +742,35 @@ bool enqueue_front(mg_box* B) {
   std::swap(B->X0, B->X0b);
   std::swap(B->V, B->Vb);
   std::swap(B->id, B->idb);
+  const PairView pv{B->XP, B->start, B->pnb, B->pcnt};
+  {
+    Tic t(B, BXT_GRID);
+    hipLaunchKernelGGL(bx_pairs, gp, blk, 0, st, g, B->XP, B->start, B->pnb, B->pcnt);
+  }
   for (uint32_t p = 0; p < B->cfg.passes; p++) {
     {
       Tic t(B, BXT_LAMBDA);
-      hipLaunchKernelGGL(bx_lambda, gp, blk, 0, st, g, B->XP, B->X, B->start, B->LAM);
+      hipLaunchKernelGGL(bx_lambda, gp, blk, 0, st, g, pv, B->X, B->LAM);
     }
     {
       Tic t(B, BXT_DP);
-      hipLaunchKernelGGL(bx_dp, gp, blk, 0, st, g, B->XP, B->X, B->start, B->LAM, B->X2);
+      hipLaunchKernelGGL(bx_dp, gp, blk, 0, st, g, pv, B->X, B->LAM, B->X2);
     }
     std::swap(B->X, B->X2);
   }
+  const uint32_t skip = B->cfg.mu > 0.0f ? 0u : 1u;
+  hipLaunchKernelGGL(bx_cg_reset, dim3(1), dim3(64), 0, st, B->sc, B->cfg.max_cg, skip, B->cfg.tol);
   {
     Tic t(B, BXT_VEL);
-    hipLaunchKernelGGL(bx_vel_density, gp, blk, 0, st, g, B->XP, B->X, B->X0, B->start, B->V, B->DENS, B->part);
+    hipLaunchKernelGGL(bx_vel_density, gp, blk, 0, st, g, pv, B->X, B->X0, B->V, B->DENS, B->part, B->sc);
   }
-  const uint32_t skip = B->cfg.mu > 0.0f ? 0u : 1u;
-  hipLaunchKernelGGL(bx_cg_reset, dim3(1), dim3(64), 0, st, B->sc, B->cfg.max_cg, skip);
-  hipLaunchKernelGGL(bx_cg_scalar, dim3(1), blk, 0, st, B->part, g.nblk, B->sc, 0, B->cfg.tol);
+  hipLaunchKernelGGL(bx_cg_scalar, dim3(1), blk, 0, st, B->part, g.nblk, B->sc, 0);
   {
     Tic t(B, BXT_CGINIT);
-    hipLaunchKernelGGL(bx_cg_init, gp, blk, 0, st, g, B->XP, B->X, B->DENS, B->start, B->V, B->R, B->P, B->part, B->sc);
+    hipLaunchKernelGGL(bx_cg_init, gp, blk, 0, st, g, pv, B->X, B->DENS, B->V, B->R, B->P, B->cw, B->ce, B->part,
+                       B->sc);
   }
-  hipLaunchKernelGGL(bx_cg_scalar, dim3(1), blk, 0, st, B->part, g.nblk, B->sc, 1, B->cfg.tol);
+  hipLaunchKernelGGL(bx_cg_scalar, dim3(1), blk, 0, st, B->part, g.nblk, B->sc, 1);
   B->have_neighbors = true;
   return hipGetLastError() == hipSuccess;
 }
@@ -692,18 +778,19 @@ bool enqueue_front(mg_box* B) {
 void enqueue_cg_chunk(mg_box* B, uint32_t iters) {
   const BoxGeo& g = B->g;
   hipStream_t st = B->stream;
+  const PairView pv{B->XP, B->start, B->pnb, B->pcnt};
   const dim3 blk(BX_BLOCK), gp = blocks_for(g.n);
   for (uint32_t k = 0; k < iters; k++) {
     {
       Tic t(B, BXT_SPMV);
-      hipLaunchKernelGGL(bx_cg_spmv, gp, blk, 0, st, g, B->XP, B->X, B->DENS, B->start, B->P, B->AP, B->part, B->sc);
+      hipLaunchKernelGGL(bx_cg_spmv, gp, blk, 0, st, g, pv, B->X, B->DENS, B->cw, B->ce, B->P, B->AP, B->part, B->sc);
     }
-    hipLaunchKernelGGL(bx_cg_scalar, dim3(1), blk, 0, st, B->part, g.nblk, B->sc, 2, B->cfg.tol);
+    hipLaunchKernelGGL(bx_cg_scalar, dim3(1), blk, 0, st, B->part, g.nblk, B->sc, 2);
     {
       Tic t(B, BXT_UPDATE);
       hipLaunchKernelGGL(bx_cg_update, gp, blk, 0, st, g, B->V, B->R, B->P, B->AP, B->part, B->sc);
     }
-    hipLaunchKernelGGL(bx_cg_scalar, dim3(1), blk, 0, st, B->part, g.nblk, B->sc, 3, B->cfg.tol);
+    hipLaunchKernelGGL(bx_cg_scalar, dim3(1), blk, 0, st, B->part, g.nblk, B->sc, 3);
     {
       Tic t(B, BXT_P);
       hipLaunchKernelGGL(bx_cg_p, gp, blk, 0, st, g, B->R, B->P, B->sc);
@@ -711,10 +798,15 @@ void enqueue_cg_chunk(mg_box* B, uint32_t iters) {
   }
 }
 
-// One full step: front half asynchronously, then CG chunks until the device reports done.
+// One full step: the front half and a first CG chunk sized by the previous step's iteration count
+// are enqueued without a host wait; further chunks follow only if the device has not converged.
 int run_step(mg_box* B) {
   if (!enqueue_front(B)) { B->err = "kernel launch failed"; return MG_E_DEVICE; }
   uint32_t launched = 0;
+  if (B->cfg.mu > 0.0f && B->cfg.max_cg > 0) {
+    launched = std::min<uint32_t>(std::max<uint32_t>(B->last_it, 1u), B->cfg.max_cg);
+    enqueue_cg_chunk(B, launched);
+  }
   for (;;) {
     hipMemcpyAsync(B->h_sc, B->sc, sizeof(BoxScal), hipMemcpyDeviceToHost, B->stream);
     if (hipStreamSynchronize(B->stream) != hipSuccess) { B->err = "HIP stream error"; return MG_E_DEVICE; }
@@ -725,6 +817,7 @@ int run_step(mg_box* B) {
     launched += k;
     if (hipGetLastError() != hipSuccess) { B->err = "kernel launch failed"; return MG_E_DEVICE; }
   }
+  B->last_it = B->h_sc->it;
   return MG_OK;
 }
 
@@ -763,13 +856,15 @@ int mg_box_create(const mg_box_config* cfg, mg_box** out, char* err, size_t errl
   off = carve(off, B->LAM, n); off = carve(off, B->DENS, n);
   off = carve(off, B->R, n); off = carve(off, B->P, n); off = carve(off, B->AP, n);
   off = carve(off, B->part, (size_t)B->g.nblk);
+  off = carve(off, B->pnb, (size_t)BX_KMAX * n); off = carve(off, B->pcnt, n);
+  off = carve(off, B->cw, (size_t)BX_KMAX * n); off = carve(off, B->ce, (size_t)BX_KMAX * n);
   off = carve(off, B->sc, 1);
   if (hipMalloc(&B->arena, off) != hipSuccess) { box_set_err(err, errlen, "hipMalloc failed"); delete B; return MG_E_DEVICE; }
   const uintptr_t base = (uintptr_t)B->arena;
   auto fix = [&](auto*& p) { p = reinterpret_cast<std::remove_reference_t<decltype(p)>>(base + (uintptr_t)p); };
   fix(B->X); fix(B->XP); fix(B->X0); fix(B->V); fix(B->X2); fix(B->Xb); fix(B->X0b); fix(B->Vb);
   fix(B->id); fix(B->idb); fix(B->key); fix(B->slot); fix(B->start); fix(B->cursor); fix(B->bsum);
-  fix(B->LAM); fix(B->DENS); fix(B->R); fix(B->P); fix(B->AP); fix(B->part); fix(B->sc);
+  fix(B->LAM); fix(B->DENS); fix(B->R); fix(B->P); fix(B->AP); fix(B->part); fix(B->sc); fix(B->pnb); fix(B->pcnt); fix(B->cw); fix(B->ce);
   bool ok = hipMemset(B->arena, 0, off) == hipSuccess;
   ok = ok && hipHostMalloc((void**)&B->h_sc, sizeof(BoxScal), 0) == hipSuccess;
   ok = ok && hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) == hipSuccess;
@@ -898,7 +993,8 @@ int64_t mg_box_neighbors(mg_box* B, uint32_t* off, uint32_t* nb, int64_t cap) {
   const uint32_t dcap = (uint32_t)std::min<int64_t>(std::max<int64_t>(cap, 1), (int64_t)64 * n + 64);
   if (hipMalloc(&d_cnt, ((size_t)n + 1) * 4) != hipSuccess) return MG_E_DEVICE;
   if (hipMalloc(&d_nb, (size_t)dcap * 4) != hipSuccess) { (void)hipFree(d_cnt); return MG_E_DEVICE; }
-  hipLaunchKernelGGL(bx_nbr_count, blocks_for(n), dim3(BX_BLOCK), 0, B->stream, g, B->XP, B->start, B->id, d_cnt);
+  const PairView pv{B->XP, B->start, B->pnb, B->pcnt};
+  hipLaunchKernelGGL(bx_nbr_count, blocks_for(n), dim3(BX_BLOCK), 0, B->stream, g, pv, B->id, d_cnt);
   std::vector<uint32_t> cnt(n);
   bool ok = hipMemcpyAsync(cnt.data(), d_cnt, (size_t)n * 4, hipMemcpyDeviceToHost, B->stream) == hipSuccess &&
             hipStreamSynchronize(B->stream) == hipSuccess;
@@ -909,7 +1005,7 @@ int64_t mg_box_neighbors(mg_box* B, uint32_t* off, uint32_t* nb, int64_t cap) {
     ok = hipMemcpy(d_cnt, off, (size_t)n * 4, hipMemcpyHostToDevice) == hipSuccess;
   }
   if (ok) {
-    hipLaunchKernelGGL(bx_nbr_write, blocks_for(n), dim3(BX_BLOCK), 0, B->stream, g, B->XP, B->start, B->id, d_cnt, d_nb, dcap);
+    hipLaunchKernelGGL(bx_nbr_write, blocks_for(n), dim3(BX_BLOCK), 0, B->stream, g, pv, B->id, d_cnt, d_nb, dcap);
     ok = hipStreamSynchronize(B->stream) == hipSuccess;
   }
   if (ok && nb && total <= cap) ok = hipMemcpy(nb, d_nb, (size_t)total * 4, hipMemcpyDeviceToHost) == hipSuccess;
