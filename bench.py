@@ -32,12 +32,13 @@ METRIC = "drum-substeps/s (batched FP32 vs plain) at ≤1% power err; p50/p99 la
 HBM_PEAK_GBS = 8000.0
 # canonical algorithmic bytes per particle (FP32 state) per launch of each timed kernel class:
 # grid build + reorder (predict 40, bin 16, scan 2, reorder 80), density pass A (R x, W lambda),
-# pass B (R x, lambda; W x), velocity + density recompute (12 + v update folded in: R x, x0; W v, rho),
-# CG set-up 52, operator apply (R x, rho, p; W Ap) 28, vector update 56 (split 32 + 24 here)
-CLASS_NAMES = ["grid+reorder", "density_A", "density_B", "velocity+density", "cg_setup", "cg_apply",
-               "cg_update", "cg_direction"]
-CLASS_BYTES = [138.0, 12.0, 20.0, 28.0, 52.0, 28.0, 32.0, 24.0]
-CLASS_KERNELS = [None, "bx_lambda", "bx_dp", "bx_vel_density", "bx_cg_init", "bx_cg_spmv", "bx_cg_update", "bx_cg_p"]
+# pass B (R x, lambda; W x), velocity + density recompute (R x, x0; W v, rho), CG set-up 52, and the
+# 84 bytes of a CG iteration split over its two kernels: direction + operator apply (R x, rho, r,
+# p; W p, Ap = 44) and vector update (R v, r, Ap; W v, r = 40)
+CLASS_NAMES = ["grid+reorder+pairs", "density_A", "density_B", "velocity+density", "cg_setup", "cg_dir_apply",
+               "cg_update"]
+CLASS_BYTES = [138.0, 12.0, 20.0, 28.0, 52.0, 44.0, 40.0]
+CLASS_KERNELS = [None, "bx_lambda", "bx_dp", "bx_vel_density", "bx_cg_init", "bx_cg_dir", "bx_cg_upd"]
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "box_1024_pmc_traffic.md")
 C2_PARAMS = {"mill": {"diameter_m": 0.1397, "speed_mode": "percent_critical", "speed_value": 75.0},
              "media": {"density_kg_m3": 7800.0},
@@ -132,42 +133,49 @@ def main() -> int:
     n = box.n
     for _ in range(args.warmup):
         box.step()
-    box._L.mg_box_set_timing(box._h, 1)
-    torch.cuda.synchronize()
-    n_cg = 0
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
+    # 1) short pass with every kernel class timed: per-kernel table and the dominant kernel
+    box.set_timing(0xFF)
+    probe = max(1, min(args.steps, 5))
+    for _ in range(probe):
         box.step()
-        n_cg += box.stats()["cg_iterations"]
+    ms_all, launches_all = box.kernel_times()
+    k = int(np.argmax(ms_all[1:7])) + 1  # grid build is several launches together: not a single kernel
+    # 2) the timed region: K steps, no per-kernel events
+    box.set_timing(0)
+    torch.cuda.synchronize()
+    cg0 = box.total_cg_iterations
+    t0 = time.perf_counter()
+    box.step(args.steps)  # K steps in one call (the library waits for each step's CG to converge)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    import ctypes as C
-    ms = np.zeros(8)
-    launches = np.zeros(8, dtype=np.uint64)
-    box._L.mg_box_kernel_times.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.c_int]
-    box._L.mg_box_kernel_times(box._h, ms.ctypes.data_as(C.POINTER(C.c_double)),
-                               launches.ctypes.data_as(C.POINTER(C.c_uint64)), 8)
+    n_cg = box.total_cg_iterations - cg0
+    # 3) the same K steps again with HIP events around the dominant kernel only (its average launch)
+    box.set_timing(1 << k)
+    for _ in range(args.steps):
+        box.step()
+    ms_k, launches_k = box.kernel_times()
+    box.set_timing(0)
     ms_per_step = 1000.0 * wall / args.steps
     n_d = 4
     avg_cg = n_cg / args.steps
     bytes_per_step = n * (226.0 + 32.0 * n_d + 84.0 * avg_cg)
     gbs = bytes_per_step / (ms_per_step * 1e-3) / 1e9
-    k = int(np.argmax(ms[1:])) + 1  # dominant kernel (grid build is several launches together)
-    avg_launch_ms = ms[k] / max(int(launches[k]), 1)
+    avg_launch_ms = ms_k[k] / max(int(launches_k[k]), 1)
     achieved = CLASS_BYTES[k] * n / (avg_launch_ms * 1e-3) / 1e9
-    kernels = {CLASS_NAMES[i]: {"total_ms": round(float(ms[i]), 4), "launches": int(launches[i]),
-                                "avg_us": round(1000.0 * float(ms[i]) / max(int(launches[i]), 1), 2)}
-               for i in range(8)}
-    roofline = {"bound": "hbm", "kernel": CLASS_NAMES[k], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+    kernels = {CLASS_NAMES[i]: {"total_ms": round(float(ms_all[i]), 4), "launches": int(launches_all[i]),
+                                "avg_us": round(1000.0 * float(ms_all[i]) / max(int(launches_all[i]), 1), 2)}
+               for i in range(7)}
+    roofline = {"bound": "hbm", "kernel": CLASS_KERNELS[k], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": measured_traffic(CLASS_KERNELS[k], args.side),
-                "bytes_per_launch": CLASS_BYTES[k] * n}
+                "bytes_per_launch": CLASS_BYTES[k] * n, "avg_launch_us": round(1000.0 * avg_launch_ms, 2),
+                "launches": int(launches_k[k])}
     iso = {"metric": "particle-steps/s", "value": round(n * args.steps / wall, 1), "ms_per_step": round(ms_per_step, 4),
            "hbm_gbs_algorithmic": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
            "cg_iterations_per_step": avg_cg, "density_passes": n_d,
            "config": {"workload": f"periodic PBF box {args.side}x{args.side}", "particles": n, "mu": args.mu,
                       "dtype": "f32 state, f64 CG reductions"},
-           "kernels": kernels}
+           "kernels_probe": kernels}
     headline_cpu = None
     if not args.no_cpu:
         headline_cpu = cpu_baseline_drum(args.cpu_seconds)

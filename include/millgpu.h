@@ -163,13 +163,16 @@ int mg_box_set_state(mg_box* box, const float* x2n, const float* v2n);
 int mg_box_density(mg_box* box, float* out_n);
 /* CG iterations, final residual norm and |b| of the last step. */
 int mg_box_stats(mg_box* box, double* out3);
+/* CG iterations summed over every step this box has run. */
+uint64_t mg_box_total_cg_iterations(const mg_box* box);
 /* Neighbour CSR of the last step in traversal order (original ids); returns the edge count,
  * copying the entries only when they fit `cap`. */
 int64_t mg_box_neighbors(mg_box* box, uint32_t* off_n1, uint32_t* nb, int64_t cap);
-/* Per-kernel timing with HIP events on the box's stream (resets the totals). Classes: 0 grid build
- * and reorder (all its launches together), 1 density pass A, 2 density pass B, 3 velocity +
- * density recompute, 4 CG set-up, 5 CG operator apply, 6 CG vector update, 7 CG direction update. */
-int mg_box_set_timing(mg_box* box, int on);
+/* Per-kernel timing with HIP events on the box's stream for the classes whose bit is set in
+ * class_mask (0 = off; resets the totals). Classes: 0 grid build, reorder and pair table (all
+ * their launches together), 1 density pass A, 2 density pass B, 3 velocity + density recompute,
+ * 4 CG set-up, 5 CG direction + operator apply, 6 CG vector update, 7 unused. */
+int mg_box_set_timing(mg_box* box, uint32_t class_mask);
 int mg_box_kernel_times(mg_box* box, double* ms, uint64_t* launches, int nclass);
 
 #ifdef __cplusplus

@@ -62,6 +62,10 @@ def _lib():
     L.mg_box_stats.argtypes = [P, dp]
     L.mg_box_neighbors.argtypes = [P, u32p, u32p, C.c_int64]
     L.mg_box_neighbors.restype = C.c_int64
+    L.mg_box_set_timing.argtypes = [P, C.c_uint32]
+    L.mg_box_total_cg_iterations.argtypes = [P]
+    L.mg_box_total_cg_iterations.restype = C.c_uint64
+    L.mg_box_kernel_times.argtypes = [P, dp, C.POINTER(C.c_uint64), C.c_int]
     _ready.add(id(L))
     return L
 
@@ -97,8 +101,24 @@ class Box:
         _sim._check(self._L.mg_box_step(self._h, n), what="mg_box_step")
 
     @property
+    def total_cg_iterations(self) -> int:
+        return int(self._L.mg_box_total_cg_iterations(self._h))
+
+    @property
     def last_ms(self) -> float:
         return float(self._L.mg_box_last_step_ms(self._h))
+
+    def set_timing(self, class_mask: int):
+        """HIP-event timing of the kernel classes in class_mask (see include/millgpu.h)."""
+        _sim._check(self._L.mg_box_set_timing(self._h, class_mask), what="set_timing")
+
+    def kernel_times(self):
+        """(total ms, launches) per kernel class since the last set_timing."""
+        ms = np.zeros(8)
+        n = np.zeros(8, dtype=np.uint64)
+        _sim._check(self._L.mg_box_kernel_times(self._h, ms.ctypes.data_as(C.POINTER(C.c_double)),
+                                                n.ctypes.data_as(C.POINTER(C.c_uint64)), 8), what="kernel_times")
+        return ms, n
 
     def state(self):
         x = np.zeros(2 * self.n, dtype=np.float32)

@@ -4,23 +4,25 @@
 // vectors and wrapped positions:
 //   bx_predict            x0 = x, x += v dt, cell key, cell histogram      pbf.rs:1225-1235, grid.rs:101-226
 //   bx_scan_*             exclusive scan of the histogram -> cell starts   grid.rs:160-200
-//   bx_scatter/bx_cellsort/bx_gather
+//   bx_scatter, bx_cellsort, bx_gather
 //                         stable counting sort: cell order, ascending particle id inside a
 //                         cell, and the particle arrays physically reordered by cell so every
 //                         neighbour gather below reads contiguous memory
+//   bx_pairs              the step's pair table: 3x3-cell candidates with |d|^2 <= h^2 at the
+//                         predicted positions (the reference's neighbour lists, pbf.rs:262-288)
 //   bx_lambda, bx_dp      density constraint passes (rho, C, lambda; dp, x)  pbf.rs:1257-1334
 //   bx_vel_density        v = (x - x0)/dt, density at the final positions    pbf.rs:1656-1674, 2136-2144
-//   bx_cg_*               implicit viscosity (I + dt L) v = v, warm start v pbf.rs:312-480
-// Pairs are the 3x3-cell candidates with |d|^2 <= h^2 at the predicted positions (the reference's
-// neighbour lists, pbf.rs:262-288), visited per particle in the reference's order: cells
-// (-1,-1), (-1,0), (0,-1), own, (1,0), (1,1), (0,1), (-1,1), (1,-1), ascending id within a cell.
-// The design is matrix-free: no per-edge array is stored; the Morris weights are recomputed in
-// every Laplacian application from x and rho. CG dot products and norms are FP64, reduced in a
-// fixed order (per-block tree, then one block over the partials), so results are deterministic.
+//   bx_cg_init            r = b - A v with the Morris weights kept for the step   pbf.rs:312-407
+//   bx_cg_dir, bx_cg_upd  one CG iteration (direction + operator; vector update)  pbf.rs:423-480
+// Pairs are visited per particle in the reference's order: cells (-1,-1), (-1,0), (0,-1), own,
+// (1,0), (1,1), (0,1), (-1,1), (1,-1), ascending id within a cell; every sum over a particle's
+// neighbours therefore rounds exactly as the reference's does. CG dot products and norms are FP64
+// partial sums per workgroup (fixed tree), summed in block order by every consumer workgroup, so
+// the scalars are identical everywhere and results are run-to-run deterministic.
 //
-// CDNA4 mapping: 256-thread workgroups (4 wave64s), one particle per lane; the blockIdx -> tile map
-// gives each of the 8 XCDs one contiguous run of cell-sorted particles so the halo rows of
-// neighbouring tiles are read from the same XCD's L2.
+// CDNA4 mapping: 256-thread workgroups (4 wave64s; 1024 for the kernels producing FP64 partials),
+// one particle per lane; the blockIdx -> tile map gives each of the 8 XCDs one contiguous run of
+// cell-sorted particles so the halo rows of neighbouring tiles are read from the same XCD's L2.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -35,6 +37,7 @@
 #include "mg_common.h"
 
 #define BX_BLOCK 256
+#define BX_RBLOCK 1024  // workgroup size of the kernels that produce FP64 partial sums
 #define BX_SCAN_ITEMS 16
 #define BX_SCAN_CHUNK (BX_BLOCK * BX_SCAN_ITEMS)
 #define BX_CG_CHUNK 4
@@ -43,20 +46,20 @@
 #endif
 // kernel classes of the optional per-kernel timing (mg_box_kernel_times)
 #define BX_NCLASS 8
-enum { BXT_GRID = 0, BXT_LAMBDA, BXT_DP, BXT_VEL, BXT_CGINIT, BXT_SPMV, BXT_UPDATE, BXT_P };
+enum { BXT_GRID = 0, BXT_LAMBDA, BXT_DP, BXT_VEL, BXT_CGINIT, BXT_SPMV, BXT_UPDATE, BXT_SPARE };
 #define BXK extern "C" __global__ void __launch_bounds__(BX_BLOCK)
+#define BXKR extern "C" __global__ void __launch_bounds__(BX_RBLOCK)
 #define BDEV __device__ __attribute__((always_inline)) inline
 
 // Geometry and constants of one box, passed by value to every kernel.
 struct BoxGeo {
-  uint32_t n, ncx, nblk, xcd_chunk;
+  uint32_t n, ncx, nblk, xcd_chunk, nred, xcd_chunk_red;
   float h, h2, L, halfL, cw, mass, rest, dt, k6, ks, w0, m_over_rest, mu2m, eta2;
 };
 
 // Device scalars of the CG loop (pbf.rs:423-480).
 struct BoxScal {
-  double bnorm, target, rs_old, rs_new, pap, res;
-  float alpha, beta;
+  double bnorm, target, res;
   uint32_t it, done, max_cg;
   float tol;
 };
@@ -76,17 +79,31 @@ BDEV float wrapf(float a, float L) {  // f32::rem_euclid
   float r = fmodf(a, L);
   return r < 0.0f ? r + fabsf(L) : r;
 }
-BDEV float mi1(float d, const BoxGeo& g) {
+BDEV float mi1(float d, const BoxGeo& g) {  // minimum image
   if (d > g.halfL) return d - g.L;
   if (d < -g.halfL) return d + g.L;
   return d;
 }
 BDEV float2 mi2(float2 d, const BoxGeo& g) { return f2(mi1(d.x, g), mi1(d.y, g)); }
+// Pair vector a - b. W = false for a particle whose 3x3 cells do not wrap (interior): every pair
+// vector of such a particle is shorter than L/2 in each component, where the minimum image is the
+// identity, so both forms give the same bits.
+template <bool W>
+BDEV float2 rel(float2 a, float2 b, const BoxGeo& g) {
+  return W ? mi2(sub2(a, b), g) : sub2(a, b);
+}
 BDEV uint32_t cell1(float a, const BoxGeo& g) {
   float q = floorf(a / g.cw);
   if (!(q >= 0.0f)) return 0u;
   if (q >= (float)(g.ncx - 1u)) return g.ncx - 1u;
   return (uint32_t)q;
+}
+// True when the 3x3 cells around the particle's (predicted-position) cell do not wrap; needs at
+// least 4 cells per side so that two cells span at most L/2.
+BDEV bool interior(float2 xp, const BoxGeo& g) {
+  if (g.ncx < 4u) return false;
+  const uint32_t cx = cell1(xp.x, g), cy = cell1(xp.y, g);
+  return cx >= 1u && cx + 2u <= g.ncx && cy >= 1u && cy + 2u <= g.ncx;
 }
 BDEV float poly6(float r2, const BoxGeo& g) {  // pbf.rs:119-126
   if (r2 >= g.h2 || r2 < 0.0f) return 0.0f;
@@ -105,6 +122,11 @@ BDEV uint32_t tile_id(const BoxGeo& g) {
   if (g.xcd_chunk == 0u) return b;  // tile count not a multiple of 8: identity map
   return (b & 7u) * g.xcd_chunk + (b >> 3);
 }
+BDEV uint32_t tile_id_red(const BoxGeo& g) {
+  const uint32_t b = blockIdx.x;
+  if (g.xcd_chunk_red == 0u) return b;
+  return (b & 7u) * g.xcd_chunk_red + (b >> 3);
+}
 // Candidate cell o (0..8) of cell (cx, cy) in the reference's per-particle order, wrapped.
 BDEV uint32_t nbr_cell(uint32_t cx, uint32_t cy, int o, uint32_t ncx) {
   const int ox = o == 0 ? -1 : (o == 1 ? -1 : (o == 2 ? 0 : (o == 3 ? 0 : (o == 4 ? 1 : (o == 5 ? 1 : (o == 6 ? 0 : (o == 7 ? -1 : 1)))))));
@@ -113,20 +135,34 @@ BDEV uint32_t nbr_cell(uint32_t cx, uint32_t cy, int o, uint32_t ncx) {
   uint32_t ay = (uint32_t)((int)cy + oy + (int)ncx) % ncx;
   return ay * ncx + ax;
 }
-// Fixed-order workgroup sum of one double per lane; lane 0 of the block gets the total.
+// Fixed-order workgroup sum of one double per lane (wave shuffles, then the waves in order);
+// every lane receives the total.
+template <int NT>
 BDEV double block_sum(double v) {
-  __shared__ double s_w[BX_BLOCK / 64];
+  __shared__ double s_w[NT / 64];
+  __shared__ double s_tot;
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   if (l == 0) s_w[w] = v;
   __syncthreads();
-  double r = 0.0;
-  if (threadIdx.x == 0)
-    for (int k = 0; k < BX_BLOCK / 64; k++) r += s_w[k];
+  if (threadIdx.x == 0) {
+    double r = 0.0;
+    for (int k = 0; k < NT / 64; k++) r += s_w[k];
+    s_tot = r;
+  }
+  __syncthreads();
+  const double r = s_tot;
   __syncthreads();
   return r;
 }
-
+// Sum of the per-block partials in a fixed order. Every block that calls it computes the same
+// value bit for bit, so the CG scalars can be evaluated redundantly in each consumer block.
+template <int NT>
+BDEV double sum_parts(const double* __restrict__ part, uint32_t nb) {
+  double s = 0.0;
+  for (uint32_t k = threadIdx.x; k < nb; k += NT) s += part[k];
+  return block_sum<NT>(s);
+}
 
 // Per-step pair table: the neighbours of sorted particle i at the predicted positions, in the
 // reference's per-particle order, column-major (nb[k * n + i]) so lane-adjacent particles read
@@ -138,16 +174,23 @@ struct PairView {
   const uint32_t* __restrict__ nb;
   const uint32_t* __restrict__ cnt;
 };
-template <class F>
+template <bool W = true, class F>
 BDEV void scan_pairs(const BoxGeo& g, const float2* __restrict__ XP, const uint32_t* __restrict__ start, uint32_t i,
                      float2 xpi, F&& f) {
   const uint32_t cx = cell1(xpi.x, g), cy = cell1(xpi.y, g);
+  // the nine cell ranges are loaded up front (independent loads), then walked in order
+  uint32_t cs[9], ce[9];
+#pragma unroll
   for (int o = 0; o < 9; o++) {
     const uint32_t k = nbr_cell(cx, cy, o, g.ncx);
-    const uint32_t s = start[k], e = start[k + 1];
-    for (uint32_t j = s; j < e; j++) {
+    cs[o] = start[k];
+    ce[o] = start[k + 1];
+  }
+#pragma unroll
+  for (int o = 0; o < 9; o++) {
+    for (uint32_t j = cs[o]; j < ce[o]; j++) {
       if (j == i) continue;
-      if (len2(mi2(sub2(xpi, XP[j]), g)) > g.h2) continue;
+      if (len2(rel<W>(xpi, XP[j], g)) > g.h2) continue;
       f(j);
     }
   }
@@ -156,70 +199,61 @@ template <class F>
 BDEV void for_pairs(const BoxGeo& g, const PairView& pv, uint32_t i, F&& f) {
   const uint32_t c = pv.cnt[i];
   if (c <= BX_KMAX) {
-    for (uint32_t k = 0; k < c; k++) f(pv.nb[(size_t)k * g.n + i]);
+    // four table entries at a time: their loads (and the gathers they feed) are issued together
+    const uint32_t* __restrict__ col = pv.nb + i;
+    const size_t n = g.n;
+    uint32_t k = 0;
+    for (; k + 4u <= c; k += 4u) {
+      const uint32_t j0 = col[(size_t)k * n], j1 = col[(size_t)(k + 1u) * n];
+      const uint32_t j2 = col[(size_t)(k + 2u) * n], j3 = col[(size_t)(k + 3u) * n];
+      f(j0);
+      f(j1);
+      f(j2);
+      f(j3);
+    }
+    for (; k < c; k++) f(col[(size_t)k * n]);
   } else {
     scan_pairs(g, pv.XP, pv.start, i, pv.XP[i], f);
   }
 }
-
-// Single-block reductions of the per-block partials and the CG scalar logic.
-BDEV double sum_parts(const double* __restrict__ part, uint32_t nb) {
-  double s = 0.0;
-  for (uint32_t k = threadIdx.x; k < nb; k += BX_BLOCK) s += part[k];
-  return block_sum(s);
-}
-// CG scalar logic after a reduction (pbf.rs:423-480). stage 0: |b| and the target; stage 1: the
-// initial residual; stage 2: p.Ap -> alpha; stage 3: r.r -> convergence test, beta.
-BDEV void cg_scalar(BoxScal* __restrict__ sc, int stage, double s) {
-  if (stage == 0) {
-    const double bn = sqrt(s);
-    sc->bnorm = bn;
-    sc->res = 0.0;
-    sc->it = 0;
-    sc->target = (double)sc->tol * bn;
-    if (!(bn > 0.0) || !isfinite(bn)) sc->done = 1u;
-  } else if (stage == 1) {
-    sc->rs_old = s;
-    sc->res = sqrt(s);
-    if (sqrt(s) <= sc->target) sc->done = 1u;
-  } else if (stage == 2) {
-    sc->it += 1u;
-    sc->pap = s;
-    if (!(s > 0.0) || !isfinite(s)) {
-      sc->done = 1u;
-    } else {
-      sc->alpha = (float)(sc->rs_old / s);
+// As for_pairs, with the per-neighbour gathers split out: load(j) returns what use(j, v) needs,
+// and the loads of four table entries are all issued before the first use (memory-level
+// parallelism; without it every neighbour costs two dependent round trips).
+template <class L, class F>
+BDEV void for_pairs_g(const BoxGeo& g, const PairView& pv, uint32_t i, L&& load, F&& use) {
+  const uint32_t c = pv.cnt[i];
+  if (c <= BX_KMAX) {
+    const uint32_t* __restrict__ col = pv.nb + i;
+    const size_t n = g.n;
+    uint32_t k = 0;
+    for (; k + 4u <= c; k += 4u) {
+      const uint32_t j0 = col[(size_t)k * n], j1 = col[(size_t)(k + 1u) * n];
+      const uint32_t j2 = col[(size_t)(k + 2u) * n], j3 = col[(size_t)(k + 3u) * n];
+      const auto a0 = load(j0);
+      const auto a1 = load(j1);
+      const auto a2 = load(j2);
+      const auto a3 = load(j3);
+      use(j0, a0);
+      use(j1, a1);
+      use(j2, a2);
+      use(j3, a3);
+    }
+    for (; k < c; k++) {
+      const uint32_t j = col[(size_t)k * n];
+      use(j, load(j));
     }
   } else {
-    sc->rs_new = s;
-    sc->res = sqrt(s);
-    if (sqrt(s) <= sc->target || sc->it >= sc->max_cg) {
-      sc->done = 1u;
-    } else {
-      sc->beta = (float)(s / sc->rs_old);
-      sc->rs_old = s;
-    }
+    scan_pairs(g, pv.XP, pv.start, i, pv.XP[i], [&](uint32_t j) __attribute__((always_inline)) { use(j, load(j)); });
   }
-}
-// Every block stores its fixed-order partial; one single-block kernel then sums them in block
-// order. (A last-arriving-block reduction needs a device-scope release fence per block, which on
-// this multi-XCD part writes back the L2 and cost more than the extra launch.)
-BDEV void store_partial(double v, double* __restrict__ part) {
-  const double t = block_sum(v);
-  if (threadIdx.x == 0) part[blockIdx.x] = t;
-}
-BXK bx_cg_scalar(const double* __restrict__ part, uint32_t nb, BoxScal* __restrict__ sc, int stage) {
-  if (sc->done) return;
-  const double s = sum_parts(part, nb);
-  if (threadIdx.x == 0) cg_scalar(sc, stage, s);
 }
 
 // ---------------------------------------------------------------------------------------------
 // grid build + reorder
 // ---------------------------------------------------------------------------------------------
-BXK bx_fill_u32(uint32_t* __restrict__ a, uint32_t n, uint32_t v) {
-  uint32_t i = blockIdx.x * BX_BLOCK + threadIdx.x;
-  if (i < n) a[i] = v;
+BXK bx_zero2(uint32_t* __restrict__ a, uint32_t na, uint32_t* __restrict__ b, uint32_t nb) {
+  const uint32_t i = blockIdx.x * BX_BLOCK + threadIdx.x;
+  if (i < na) a[i] = 0u;
+  if (i < nb) b[i] = 0u;
 }
 
 // Lanes of one wave holding the same key form runs (the particles arrive in the previous step's
@@ -321,10 +355,21 @@ BXK bx_scatter(BoxGeo g, const uint32_t* __restrict__ key, const uint32_t* __res
   base = __shfl(base, (int)head, 64);
   if (i < g.n) slot[base + ((threadIdx.x & 63u) - head)] = i;
 }
-// One lane per cell: order the cell's slots by particle id (cells hold a handful of particles).
+// One lane per cell: order the cell's slots by particle id (a cell holds a handful of particles).
+// Lane 0 of the grid also resets this step's CG scalars (nothing reads them before the CG).
 BXK bx_cellsort(uint32_t ncell, const uint32_t* __restrict__ start, uint32_t* __restrict__ slot,
-                const uint32_t* __restrict__ id) {
-  uint32_t c = blockIdx.x * BX_BLOCK + threadIdx.x;
+                const uint32_t* __restrict__ id, BoxScal* __restrict__ sc, uint32_t cg_off, uint32_t max_cg,
+                float tol) {
+  const uint32_t c = blockIdx.x * BX_BLOCK + threadIdx.x;
+  if (c == 0) {
+    sc->done = cg_off;
+    sc->it = 0u;
+    sc->max_cg = max_cg;
+    sc->tol = tol;
+    sc->res = 0.0;
+    sc->bnorm = 0.0;
+    sc->target = 0.0;
+  }
   if (c >= ncell) return;
   const uint32_t s = start[c], e = start[c + 1];
   for (uint32_t a = s + 1; a < e; a++) {
@@ -336,13 +381,15 @@ BXK bx_cellsort(uint32_t ncell, const uint32_t* __restrict__ start, uint32_t* __
     slot[b] = v;
   }
 }
+// Moves every particle to its sorted position (one lane per destination: coalesced writes); x is
+// also written to the predicted copy XP.
 BXK bx_gather(BoxGeo g, const uint32_t* __restrict__ slot, const float2* __restrict__ X, const float2* __restrict__ X0,
               const float2* __restrict__ V, const uint32_t* __restrict__ id, float2* __restrict__ Xb,
               float2* __restrict__ XPb, float2* __restrict__ X0b, float2* __restrict__ Vb, uint32_t* __restrict__ idb) {
-  uint32_t q = blockIdx.x * BX_BLOCK + threadIdx.x;
+  const uint32_t q = blockIdx.x * BX_BLOCK + threadIdx.x;
   if (q >= g.n) return;
-  uint32_t i = slot[q];
-  float2 x = X[i];
+  const uint32_t i = slot[q];
+  const float2 x = X[i];
   Xb[q] = x;
   XPb[q] = x;
   X0b[q] = X0[i];
@@ -355,25 +402,29 @@ BXK bx_pairs(BoxGeo g, const float2* __restrict__ XP, const uint32_t* __restrict
   const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
   if (i >= g.n) return;
   uint32_t c = 0;
-  scan_pairs(g, XP, start, i, XP[i], [&](uint32_t j) __attribute__((always_inline)) {
+  auto put = [&](uint32_t j) __attribute__((always_inline)) {
     if (c < BX_KMAX) nb[(size_t)c * g.n + i] = j;
     c++;
-  });
+  };
+  const float2 xpi = XP[i];
+  if (interior(xpi, g)) scan_pairs<false>(g, XP, start, i, xpi, put);
+  else scan_pairs<true>(g, XP, start, i, xpi, put);
   cnt[i] = c;
 }
 
 // ---------------------------------------------------------------------------------------------
 // density passes (pbf.rs:1257-1334)
 // ---------------------------------------------------------------------------------------------
-BXK bx_lambda(BoxGeo g, PairView pv, const float2* __restrict__ X, float* __restrict__ LAM) {
-  const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
-  if (i >= g.n) return;
+template <bool W>
+BDEV void lambda_row(const BoxGeo& g, const PairView& pv, uint32_t i, const float2* __restrict__ X,
+                     float* __restrict__ LAM) {
   const float2 xi = X[i];
   float rho = g.w0;
   float2 gs = f2(0.0f, 0.0f);
   float sgs = 0.0f;
-  for_pairs(g, pv, i, [&](uint32_t j) __attribute__((always_inline)) {
-    const float2 d = mi2(sub2(xi, X[j]), g);
+  for_pairs_g(g, pv, i, [&](uint32_t j) __attribute__((always_inline)) { return X[j]; },
+              [&](uint32_t, float2 xj) __attribute__((always_inline)) {
+    const float2 d = rel<W>(xi, xj, g);
     const float r2 = len2(d);
     rho += g.mass * poly6(r2, g);
     const float r = sqrtf(r2);
@@ -385,29 +436,42 @@ BXK bx_lambda(BoxGeo g, PairView pv, const float2* __restrict__ X, float* __rest
   sgs += len2(gs);
   LAM[i] = -c / (sgs + 200.0f);
 }
+BXK bx_lambda(BoxGeo g, PairView pv, const float2* __restrict__ X, float* __restrict__ LAM) {
+  const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
+  if (i >= g.n) return;
+  if (interior(pv.XP[i], g)) lambda_row<false>(g, pv, i, X, LAM);
+  else lambda_row<true>(g, pv, i, X, LAM);
+}
+template <bool W>
+BDEV void dp_row(const BoxGeo& g, const PairView& pv, uint32_t i, const float2* __restrict__ X,
+                 const float* __restrict__ LAM, float2* __restrict__ Xout) {
+  const float2 xi = X[i];
+  const float li = LAM[i];
+  float2 s = f2(0.0f, 0.0f);
+  struct Nb { float2 x; float l; };
+  for_pairs_g(g, pv, i, [&](uint32_t j) __attribute__((always_inline)) { return Nb{X[j], LAM[j]}; },
+              [&](uint32_t, const Nb& a) __attribute__((always_inline)) {
+    const float2 d = rel<W>(xi, a.x, g);
+    const float r = sqrtf(len2(d));
+    s = add2(s, smul2(li + a.l, spiky(d, r, g)));
+  });
+  const float2 dp = mul2(s, g.m_over_rest);
+  Xout[i] = f2(wrapf(xi.x + dp.x, g.L), wrapf(xi.y + dp.y, g.L));
+}
 BXK bx_dp(BoxGeo g, PairView pv, const float2* __restrict__ X, const float* __restrict__ LAM,
           float2* __restrict__ Xout) {
   const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
   if (i >= g.n) return;
-  const float2 xi = X[i];
-  const float li = LAM[i];
-  float2 s = f2(0.0f, 0.0f);
-  for_pairs(g, pv, i, [&](uint32_t j) __attribute__((always_inline)) {
-    const float2 d = mi2(sub2(xi, X[j]), g);
-    const float r = sqrtf(len2(d));
-    s = add2(s, smul2(li + LAM[j], spiky(d, r, g)));
-  });
-  const float2 dp = mul2(s, g.m_over_rest);
-  Xout[i] = f2(wrapf(xi.x + dp.x, g.L), wrapf(xi.y + dp.y, g.L));
+  if (interior(pv.XP[i], g)) dp_row<false>(g, pv, i, X, LAM, Xout);
+  else dp_row<true>(g, pv, i, X, LAM, Xout);
 }
 
 // ---------------------------------------------------------------------------------------------
 // velocity, density recompute, implicit viscosity
 // ---------------------------------------------------------------------------------------------
-BXK bx_vel_density(BoxGeo g, PairView pv, const float2* __restrict__ X, const float2* __restrict__ X0,
-                   float2* __restrict__ V, float* __restrict__ DENS, double* __restrict__ part,
-                   BoxScal* __restrict__ sc) {
-  const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
+BXKR bx_vel_density(BoxGeo g, PairView pv, const float2* __restrict__ X, const float2* __restrict__ X0,
+                    float2* __restrict__ V, float* __restrict__ DENS, double* __restrict__ part_b) {
+  const uint32_t i = tile_id_red(g) * BX_RBLOCK + threadIdx.x;
   double vv = 0.0;
   if (i < g.n) {
     const float2 xi = X[i];
@@ -415,28 +479,39 @@ BXK bx_vel_density(BoxGeo g, PairView pv, const float2* __restrict__ X, const fl
     V[i] = v;
     vv = (double)v.x * (double)v.x + (double)v.y * (double)v.y;
     float rho = g.w0;
-    for_pairs(g, pv, i, [&](uint32_t j) __attribute__((always_inline)) {
-      rho += g.mass * poly6(len2(mi2(sub2(xi, X[j]), g)), g);
-    });
+    auto load = [&](uint32_t j) __attribute__((always_inline)) { return X[j]; };
+    if (interior(pv.XP[i], g)) {
+      for_pairs_g(g, pv, i, load, [&](uint32_t, float2 xj) __attribute__((always_inline)) {
+        rho += g.mass * poly6(len2(rel<false>(xi, xj, g)), g);
+      });
+    } else {
+      for_pairs_g(g, pv, i, load, [&](uint32_t, float2 xj) __attribute__((always_inline)) {
+        rho += g.mass * poly6(len2(rel<true>(xi, xj, g)), g);
+      });
+    }
     DENS[i] = rho;
   }
-  if (sc->done) return;  // viscosity off: no CG this step
-  store_partial(vv, part);
+  const double t = block_sum<BX_RBLOCK>(vv);
+  if (threadIdx.x == 0) part_b[blockIdx.x] = t;
 }
 
-// (L v)_i = Σ_j 4 c_ij e_ij (e_ij · (v_i - v_j)) with the Morris weight c_ij (pbf.rs:312-377).
 // Morris weight and unit direction of pair (i, j) (pbf.rs:312-344).
+// The quotient d / r is formed once: the kernel gradient (d / r) * coeff and the direction d / r
+// use the same rounded value, exactly as two separate divisions would give.
+template <bool W = true>
 BDEV void morris(const BoxGeo& g, float2 xi, float rho_i, float2 xj, float dens_j, float& w, float2& e) {
   const float rho_j = fmaxf(dens_j, 1e-6f);
-  const float2 d = mi2(sub2(xi, xj), g);
+  const float2 d = rel<W>(xi, xj, g);
   const float r = sqrtf(len2(d));
-  const float2 gr = spiky(d, r, g);
+  const float2 dr = div2(d, r);
+  const float2 gr = (r <= 1e-9f || r >= g.h) ? f2(0.0f, 0.0f) : mul2(dr, g.ks * (g.h - r) * (g.h - r));
   const float num = (g.mu2m / (rho_i * rho_j)) * fabsf(dot2(d, gr));
   w = num / (len2(d) + g.eta2);
-  e = r > 1e-9f ? div2(d, r) : f2(1.0f, 0.0f);
+  e = r > 1e-9f ? dr : f2(1.0f, 0.0f);
 }
 // (L v)_i with the weights computed here; when `store` the weights of a tabled particle are kept
 // (column-major, like the pair table) for the operator applications of this step's CG.
+template <bool W = true>
 BDEV float2 lap_row(const BoxGeo& g, uint32_t i, const PairView& pv, const float2* __restrict__ X,
                     const float* __restrict__ DENS, const float2* __restrict__ in, bool store,
                     float* __restrict__ CW, float2* __restrict__ CE) {
@@ -445,16 +520,18 @@ BDEV float2 lap_row(const BoxGeo& g, uint32_t i, const PairView& pv, const float
   const bool keep = store && pv.cnt[i] <= BX_KMAX;
   float2 s = f2(0.0f, 0.0f);
   uint32_t k = 0;
-  for_pairs(g, pv, i, [&](uint32_t j) __attribute__((always_inline)) {
+  struct Nb { float2 x; float d; float2 v; };
+  for_pairs_g(g, pv, i, [&](uint32_t j) __attribute__((always_inline)) { return Nb{X[j], DENS[j], in[j]}; },
+              [&](uint32_t, const Nb& a) __attribute__((always_inline)) {
     float w;
     float2 e;
-    morris(g, xi, rho_i, X[j], DENS[j], w, e);
+    morris<W>(g, xi, rho_i, a.x, a.d, w, e);
     if (keep) {
       CW[(size_t)k * g.n + i] = w;
       CE[(size_t)k * g.n + i] = e;
     }
     k++;
-    s = add2(s, smul2(4.0f * w * dot2(e, sub2(ini, in[j])), e));
+    s = add2(s, smul2(4.0f * w * dot2(e, sub2(ini, a.v)), e));
   });
   return s;
 }
@@ -476,42 +553,134 @@ BDEV float2 lap_row_kept(const BoxGeo& g, uint32_t i, const PairView& pv, const 
   return s;
 }
 
-BXK bx_cg_init(BoxGeo g, PairView pv, const float2* __restrict__ X, const float* __restrict__ DENS,
-               const float2* __restrict__ V, float2* __restrict__ R, float2* __restrict__ P,
-               float* __restrict__ CW, float2* __restrict__ CE, double* __restrict__ part,
-               const BoxScal* __restrict__ sc) {
+BXKR bx_cg_init(BoxGeo g, PairView pv, const float2* __restrict__ X, const float* __restrict__ DENS,
+                const float2* __restrict__ V, float2* __restrict__ R, float2* __restrict__ P,
+                float* __restrict__ CW, float2* __restrict__ CE, double* __restrict__ part_rr0,
+                const BoxScal* __restrict__ sc) {
   if (sc->done) return;
-  const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
+  const uint32_t i = tile_id_red(g) * BX_RBLOCK + threadIdx.x;
   double rr = 0.0;
   if (i < g.n) {
     const float2 v = V[i];
-    const float2 lv = lap_row(g, i, pv, X, DENS, V, true, CW, CE);
+    const float2 lv = interior(pv.XP[i], g) ? lap_row<false>(g, i, pv, X, DENS, V, true, CW, CE)
+                                            : lap_row<true>(g, i, pv, X, DENS, V, true, CW, CE);
     const float2 r = sub2(v, add2(v, mul2(lv, g.dt)));
     R[i] = r;
     P[i] = r;
     rr = (double)r.x * (double)r.x + (double)r.y * (double)r.y;
   }
-  store_partial(rr, part);
+  const double t = block_sum<BX_RBLOCK>(rr);
+  if (threadIdx.x == 0) part_rr0[blockIdx.x] = t;
 }
-BXK bx_cg_spmv(BoxGeo g, PairView pv, const float2* __restrict__ X, const float* __restrict__ DENS,
-               const float* __restrict__ CW, const float2* __restrict__ CE, const float2* __restrict__ P,
-               float2* __restrict__ AP, double* __restrict__ part, const BoxScal* __restrict__ sc) {
+
+// CG iteration t (pbf.rs:423-480), first half. Every block evaluates the scalars from the partial
+// sums of the previous kernels (same fixed order everywhere): for t = 1 the target tol·|b| and the
+// initial residual; for t > 1 the convergence test on r(t-1).r(t-1), the iteration cap, and
+// beta = rs(t-1)/rs(t-2). The direction p = r + beta p_old is formed on the fly for the particle
+// and its neighbours (no separate direction kernel), then Ap = p + dt L p with the kept weights.
+BXKR bx_cg_dir(BoxGeo g, PairView pv, const float2* __restrict__ X, const float* __restrict__ DENS,
+               const float* __restrict__ CW, const float2* __restrict__ CE, const float2* __restrict__ R,
+               const float2* __restrict__ Pold, float2* __restrict__ Pnew, float2* __restrict__ AP,
+               const double* __restrict__ part_b, const double* __restrict__ part_rr_prev,
+               const double* __restrict__ part_rr_prev2, double* __restrict__ part_pap, BoxScal* __restrict__ sc,
+               uint32_t t) {
   if (sc->done) return;
-  const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
+  double target;
+  bool stop = false;
+  float beta = 0.0f;
+  if (t == 1u) {
+    const double bn = sqrt(sum_parts<BX_RBLOCK>(part_b, g.nred));
+    target = (double)sc->tol * bn;
+    const double rs0 = sum_parts<BX_RBLOCK>(part_rr_prev, g.nred);
+    if (!(bn > 0.0) || !isfinite(bn) || sqrt(rs0) <= target) stop = true;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc->bnorm = bn;
+      sc->target = target;
+      sc->res = (bn > 0.0 && isfinite(bn)) ? sqrt(rs0) : 0.0;
+    }
+  } else {
+    target = sc->target;
+    const double rs1 = sum_parts<BX_RBLOCK>(part_rr_prev, g.nred);
+    const double rs2 = sum_parts<BX_RBLOCK>(part_rr_prev2, g.nred);
+    if (sqrt(rs1) <= target || t - 1u >= sc->max_cg) stop = true;
+    beta = (float)(rs1 / rs2);
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc->res = sqrt(rs1);
+  }
+  if (stop) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc->it = t - 1u;
+      sc->done = 1u;
+    }
+    return;
+  }
+  const uint32_t i = tile_id_red(g) * BX_RBLOCK + threadIdx.x;
   double pap = 0.0;
   if (i < g.n) {
-    const float2 p = P[i];
-    const float2 ap = add2(p, mul2(lap_row_kept(g, i, pv, X, DENS, P, CW, CE), g.dt));
+    const bool first = t == 1u;
+    const float2 p = first ? Pold[i] : add2(R[i], mul2(Pold[i], beta));
+    float2 s = f2(0.0f, 0.0f);
+    const uint32_t c = pv.cnt[i];
+    if (c <= BX_KMAX) {
+      auto pdir = [&](uint32_t j) __attribute__((always_inline)) {
+        return first ? Pold[j] : add2(R[j], mul2(Pold[j], beta));
+      };
+      auto term = [&](size_t q, float2 pj) __attribute__((always_inline)) {
+        s = add2(s, smul2(4.0f * CW[q] * dot2(CE[q], sub2(p, pj)), CE[q]));
+      };
+      const size_t n = g.n;
+      uint32_t k = 0;
+      for (; k + 4u <= c; k += 4u) {
+        const size_t q0 = (size_t)k * n + i, q1 = q0 + n, q2 = q1 + n, q3 = q2 + n;
+        const uint32_t j0 = pv.nb[q0], j1 = pv.nb[q1], j2 = pv.nb[q2], j3 = pv.nb[q3];
+        const float2 p0 = pdir(j0), p1 = pdir(j1), p2 = pdir(j2), p3 = pdir(j3);
+        term(q0, p0);
+        term(q1, p1);
+        term(q2, p2);
+        term(q3, p3);
+      }
+      for (; k < c; k++) {
+        const size_t q = (size_t)k * n + i;
+        term(q, pdir(pv.nb[q]));
+      }
+    } else {
+      const float2 xi = X[i];
+      const float rho_i = fmaxf(DENS[i], 1e-6f);
+      scan_pairs(g, pv.XP, pv.start, i, pv.XP[i], [&](uint32_t j) __attribute__((always_inline)) {
+        const float2 pj = first ? Pold[j] : add2(R[j], mul2(Pold[j], beta));
+        float w;
+        float2 e;
+        morris(g, xi, rho_i, X[j], DENS[j], w, e);
+        s = add2(s, smul2(4.0f * w * dot2(e, sub2(p, pj)), e));
+      });
+    }
+    const float2 ap = add2(p, mul2(s, g.dt));
+    Pnew[i] = p;
     AP[i] = ap;
     pap = (double)p.x * (double)ap.x + (double)p.y * (double)ap.y;
   }
-  store_partial(pap, part);
+  const double tt = block_sum<BX_RBLOCK>(pap);
+  if (threadIdx.x == 0) part_pap[blockIdx.x] = tt;
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc->it = t;
 }
-BXK bx_cg_update(BoxGeo g, float2* __restrict__ V, float2* __restrict__ R, const float2* __restrict__ P,
-                 const float2* __restrict__ AP, double* __restrict__ part, BoxScal* __restrict__ sc) {
+
+// CG iteration t, second half: alpha = rs(t-1)/p.Ap (stop if p.Ap <= 0), v += alpha p,
+// r -= alpha Ap, partial sums of r.r for the next iteration's test.
+BXKR bx_cg_upd(BoxGeo g, float2* __restrict__ V, float2* __restrict__ R, const float2* __restrict__ P,
+               const float2* __restrict__ AP, const double* __restrict__ part_pap,
+               const double* __restrict__ part_rr_prev, double* __restrict__ part_rr_out,
+               BoxScal* __restrict__ sc, uint32_t t) {
   if (sc->done) return;
-  const uint32_t i = blockIdx.x * BX_BLOCK + threadIdx.x;
-  const float a = sc->alpha;
+  const double pap = sum_parts<BX_RBLOCK>(part_pap, g.nred);
+  if (!(pap > 0.0) || !isfinite(pap)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc->it = t;
+      sc->done = 1u;
+    }
+    return;
+  }
+  const double rs_old = sum_parts<BX_RBLOCK>(part_rr_prev, g.nred);
+  const float a = (float)(rs_old / pap);
+  const uint32_t i = blockIdx.x * BX_RBLOCK + threadIdx.x;
   double rr = 0.0;
   if (i < g.n) {
     V[i] = add2(V[i], mul2(P[i], a));
@@ -519,21 +688,21 @@ BXK bx_cg_update(BoxGeo g, float2* __restrict__ V, float2* __restrict__ R, const
     R[i] = r;
     rr = (double)r.x * (double)r.x + (double)r.y * (double)r.y;
   }
-  store_partial(rr, part);
+  const double tt = block_sum<BX_RBLOCK>(rr);
+  if (threadIdx.x == 0) part_rr_out[blockIdx.x] = tt;
 }
-BXK bx_cg_p(BoxGeo g, const float2* __restrict__ R, float2* __restrict__ P, const BoxScal* __restrict__ sc) {
+
+// End of a chunk of iterations ending with iteration t: the convergence test and the cap on
+// r(t).r(t), so the host sees `done` without another iteration being launched.
+BXKR bx_cg_check(BoxGeo g, const double* __restrict__ part_rr_last, BoxScal* __restrict__ sc, uint32_t t) {
   if (sc->done) return;
-  const uint32_t i = blockIdx.x * BX_BLOCK + threadIdx.x;
-  if (i < g.n) P[i] = add2(R[i], mul2(P[i], sc->beta));
-}
-BXK bx_cg_reset(BoxScal* __restrict__ sc, uint32_t max_cg, uint32_t skip, float tol) {
+  const double rs = sum_parts<BX_RBLOCK>(part_rr_last, g.nred);
   if (threadIdx.x != 0) return;
-  sc->done = skip;
-  sc->tol = tol;
-  sc->it = 0;
-  sc->max_cg = max_cg;
-  sc->res = 0.0;
-  sc->bnorm = 0.0;
+  sc->res = sqrt(rs);
+  if (sqrt(rs) <= sc->target || t >= sc->max_cg) {
+    sc->it = t;
+    sc->done = 1u;
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -573,7 +742,11 @@ struct mg_box {
   uint32_t *pnb = nullptr, *pcnt = nullptr;  // per-step pair table
   float* cw = nullptr;                         // per-step Morris weights of tabled pairs
   float2* ce = nullptr;                        // and their unit directions
-  double* part = nullptr;
+  double* part_b = nullptr;    // |b|^2 partials (one per reduction block)
+  double* part_rr = nullptr;   // r.r partials, two generations: [0, nred) and [nred, 2 nred)
+  double* part_pap = nullptr;  // p.Ap partials
+  float2* P2 = nullptr;        // second direction buffer (the CG alternates P and P2)
+  uint32_t cg_t = 0;           // next CG iteration number of the current step
   BoxScal* sc = nullptr;
   BoxScal* h_sc = nullptr;  // pinned host mirror
   void* arena = nullptr;
@@ -581,10 +754,11 @@ struct mg_box {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool have_neighbors = false;
   uint32_t last_it = 0;  // CG iterations of the previous step (sizes the first chunk)
+  uint64_t total_it = 0;  // CG iterations summed over every step run
   float last_ms = 0.0f;
   std::string err;
   // optional per-kernel timing: event pairs around launches, accumulated after each sync
-  bool timing = false;
+  uint32_t timing = 0;  // bit c set: time launches of kernel class c
   struct Pend { int cls; hipEvent_t a, b; };
   std::vector<Pend> pend;
   std::vector<hipEvent_t> pool;
@@ -642,6 +816,8 @@ bool box_geometry(const mg_box_config& c, BoxGeo& g, std::string& err) {
   g.eta2 = 0.01f * g.h * g.h;
   g.nblk = (g.n + BX_BLOCK - 1) / BX_BLOCK;
   g.xcd_chunk = (g.nblk % 8u == 0u) ? g.nblk / 8u : 0u;
+  g.nred = (g.n + BX_RBLOCK - 1) / BX_RBLOCK;
+  g.xcd_chunk_red = (g.nred % 8u == 0u) ? g.nred / 8u : 0u;
   return true;
 }
 
@@ -696,7 +872,7 @@ struct Tic {
   int cls;
   hipEvent_t a = nullptr;
   Tic(mg_box* b, int c) : B(b), cls(c) {
-    if (B->timing && (a = take_event(B))) (void)hipEventRecord(a, B->stream);
+    if (((B->timing >> cls) & 1u) && (a = take_event(B))) (void)hipEventRecord(a, B->stream);
   }
   ~Tic() {
     if (!a) return;
@@ -723,18 +899,19 @@ void collect_times(mg_box* B) {
 // Enqueues one step up to and including the CG set-up; returns false on a launch error.
 bool enqueue_front(mg_box* B) {
   const BoxGeo& g = B->g;
+  const uint32_t cg_off = (B->cfg.mu > 0.0f && B->cfg.max_cg > 0) ? 0u : 1u;
   hipStream_t st = B->stream;
   const dim3 blk(BX_BLOCK), gp = blocks_for(g.n);
   {
   Tic t(B, BXT_GRID);
-  hipLaunchKernelGGL(bx_fill_u32, blocks_for(B->ncell + 1), blk, 0, st, B->start, B->ncell + 1, 0u);
-  hipLaunchKernelGGL(bx_fill_u32, blocks_for(B->ncell), blk, 0, st, B->cursor, B->ncell, 0u);
+  hipLaunchKernelGGL(bx_zero2, blocks_for(B->ncell + 1), blk, 0, st, B->start, B->ncell + 1, B->cursor, B->ncell);
   hipLaunchKernelGGL(bx_predict, gp, blk, 0, st, g, B->X, B->V, B->X0, B->key, B->start);
   hipLaunchKernelGGL(bx_scan_totals, dim3(B->nscan), blk, 0, st, B->start, B->ncell + 1, B->bsum);
   hipLaunchKernelGGL(bx_scan_top, dim3(1), blk, 0, st, B->bsum, B->nscan);
   hipLaunchKernelGGL(bx_scan_apply, dim3(B->nscan), blk, 0, st, B->start, B->ncell + 1, B->bsum);
   hipLaunchKernelGGL(bx_scatter, gp, blk, 0, st, g, B->key, B->start, B->cursor, B->slot);
-  hipLaunchKernelGGL(bx_cellsort, blocks_for(B->ncell), blk, 0, st, B->ncell, B->start, B->slot, B->id);
+  hipLaunchKernelGGL(bx_cellsort, blocks_for(B->ncell), blk, 0, st, B->ncell, B->start, B->slot, B->id, B->sc, cg_off,
+                     B->cfg.max_cg, B->cfg.tol);
   hipLaunchKernelGGL(bx_gather, gp, blk, 0, st, g, B->slot, B->X, B->X0, B->V, B->id, B->Xb, B->XP, B->X0b, B->Vb,
                      B->idb);
   }
@@ -758,44 +935,49 @@ bool enqueue_front(mg_box* B) {
     }
     std::swap(B->X, B->X2);
   }
-  const uint32_t skip = B->cfg.mu > 0.0f ? 0u : 1u;
-  hipLaunchKernelGGL(bx_cg_reset, dim3(1), dim3(64), 0, st, B->sc, B->cfg.max_cg, skip, B->cfg.tol);
+  const dim3 rblk(BX_RBLOCK), gr(g.nred);
   {
     Tic t(B, BXT_VEL);
-    hipLaunchKernelGGL(bx_vel_density, gp, blk, 0, st, g, pv, B->X, B->X0, B->V, B->DENS, B->part, B->sc);
+    hipLaunchKernelGGL(bx_vel_density, gr, rblk, 0, st, g, pv, B->X, B->X0, B->V, B->DENS, B->part_b);
   }
-  hipLaunchKernelGGL(bx_cg_scalar, dim3(1), blk, 0, st, B->part, g.nblk, B->sc, 0);
-  {
+  if (!cg_off) {
     Tic t(B, BXT_CGINIT);
-    hipLaunchKernelGGL(bx_cg_init, gp, blk, 0, st, g, pv, B->X, B->DENS, B->V, B->R, B->P, B->cw, B->ce, B->part,
-                       B->sc);
+    hipLaunchKernelGGL(bx_cg_init, gr, rblk, 0, st, g, pv, B->X, B->DENS, B->V, B->R, B->P, B->cw, B->ce,
+                       B->part_rr, B->sc);
   }
-  hipLaunchKernelGGL(bx_cg_scalar, dim3(1), blk, 0, st, B->part, g.nblk, B->sc, 1);
+  B->cg_t = 1;
   B->have_neighbors = true;
   return hipGetLastError() == hipSuccess;
 }
 
+// Enqueues CG iterations cg_t .. cg_t + iters - 1 and the end-of-chunk convergence check.
 void enqueue_cg_chunk(mg_box* B, uint32_t iters) {
   const BoxGeo& g = B->g;
   hipStream_t st = B->stream;
   const PairView pv{B->XP, B->start, B->pnb, B->pcnt};
-  const dim3 blk(BX_BLOCK), gp = blocks_for(g.n);
+  const dim3 rblk(BX_RBLOCK), gr(g.nred);
+  const size_t nr = g.nred;
   for (uint32_t k = 0; k < iters; k++) {
+    const uint32_t t = B->cg_t++;
+    // iteration t reads the direction of t-1 and writes the other buffer; r.r of iteration u lives
+    // in generation u & 1 (the initial residual is generation 0)
+    float2* pold = (t & 1u) ? B->P : B->P2;
+    float2* pnew = (t & 1u) ? B->P2 : B->P;
+    const double* rr_prev = B->part_rr + ((t - 1u) & 1u) * nr;
+    const double* rr_prev2 = B->part_rr + (t & 1u) * nr;
+    double* rr_out = B->part_rr + (t & 1u) * nr;
     {
-      Tic t(B, BXT_SPMV);
-      hipLaunchKernelGGL(bx_cg_spmv, gp, blk, 0, st, g, pv, B->X, B->DENS, B->cw, B->ce, B->P, B->AP, B->part, B->sc);
+      Tic tt(B, BXT_SPMV);
+      hipLaunchKernelGGL(bx_cg_dir, gr, rblk, 0, st, g, pv, B->X, B->DENS, B->cw, B->ce, B->R, pold, pnew, B->AP,
+                         B->part_b, rr_prev, rr_prev2, B->part_pap, B->sc, t);
     }
-    hipLaunchKernelGGL(bx_cg_scalar, dim3(1), blk, 0, st, B->part, g.nblk, B->sc, 2);
     {
-      Tic t(B, BXT_UPDATE);
-      hipLaunchKernelGGL(bx_cg_update, gp, blk, 0, st, g, B->V, B->R, B->P, B->AP, B->part, B->sc);
-    }
-    hipLaunchKernelGGL(bx_cg_scalar, dim3(1), blk, 0, st, B->part, g.nblk, B->sc, 3);
-    {
-      Tic t(B, BXT_P);
-      hipLaunchKernelGGL(bx_cg_p, gp, blk, 0, st, g, B->R, B->P, B->sc);
+      Tic tt(B, BXT_UPDATE);
+      hipLaunchKernelGGL(bx_cg_upd, gr, rblk, 0, st, g, B->V, B->R, pnew, B->AP, B->part_pap, rr_prev, rr_out, B->sc, t);
     }
   }
+  const uint32_t last = B->cg_t - 1u;
+  hipLaunchKernelGGL(bx_cg_check, dim3(1), rblk, 0, st, g, B->part_rr + (last & 1u) * nr, B->sc, last);
 }
 
 // One full step: the front half and a first CG chunk sized by the previous step's iteration count
@@ -806,6 +988,8 @@ int run_step(mg_box* B) {
   if (B->cfg.mu > 0.0f && B->cfg.max_cg > 0) {
     launched = std::min<uint32_t>(std::max<uint32_t>(B->last_it, 1u), B->cfg.max_cg);
     enqueue_cg_chunk(B, launched);
+  } else {
+    launched = B->cfg.max_cg;  // no CG this step
   }
   for (;;) {
     hipMemcpyAsync(B->h_sc, B->sc, sizeof(BoxScal), hipMemcpyDeviceToHost, B->stream);
@@ -818,6 +1002,7 @@ int run_step(mg_box* B) {
     if (hipGetLastError() != hipSuccess) { B->err = "kernel launch failed"; return MG_E_DEVICE; }
   }
   B->last_it = B->h_sc->it;
+  B->total_it += B->h_sc->it;
   return MG_OK;
 }
 
@@ -855,7 +1040,8 @@ int mg_box_create(const mg_box_config* cfg, mg_box** out, char* err, size_t errl
   off = carve(off, B->bsum, (size_t)B->nscan);
   off = carve(off, B->LAM, n); off = carve(off, B->DENS, n);
   off = carve(off, B->R, n); off = carve(off, B->P, n); off = carve(off, B->AP, n);
-  off = carve(off, B->part, (size_t)B->g.nblk);
+  off = carve(off, B->part_b, (size_t)B->g.nred); off = carve(off, B->part_rr, 2 * (size_t)B->g.nred);
+  off = carve(off, B->part_pap, (size_t)B->g.nred); off = carve(off, B->P2, n);
   off = carve(off, B->pnb, (size_t)BX_KMAX * n); off = carve(off, B->pcnt, n);
   off = carve(off, B->cw, (size_t)BX_KMAX * n); off = carve(off, B->ce, (size_t)BX_KMAX * n);
   off = carve(off, B->sc, 1);
@@ -864,7 +1050,7 @@ int mg_box_create(const mg_box_config* cfg, mg_box** out, char* err, size_t errl
   auto fix = [&](auto*& p) { p = reinterpret_cast<std::remove_reference_t<decltype(p)>>(base + (uintptr_t)p); };
   fix(B->X); fix(B->XP); fix(B->X0); fix(B->V); fix(B->X2); fix(B->Xb); fix(B->X0b); fix(B->Vb);
   fix(B->id); fix(B->idb); fix(B->key); fix(B->slot); fix(B->start); fix(B->cursor); fix(B->bsum);
-  fix(B->LAM); fix(B->DENS); fix(B->R); fix(B->P); fix(B->AP); fix(B->part); fix(B->sc); fix(B->pnb); fix(B->pcnt); fix(B->cw); fix(B->ce);
+  fix(B->LAM); fix(B->DENS); fix(B->R); fix(B->P); fix(B->AP); fix(B->part_b); fix(B->part_rr); fix(B->part_pap); fix(B->P2); fix(B->sc); fix(B->pnb); fix(B->pcnt); fix(B->cw); fix(B->ce);
   bool ok = hipMemset(B->arena, 0, off) == hipSuccess;
   ok = ok && hipHostMalloc((void**)&B->h_sc, sizeof(BoxScal), 0) == hipSuccess;
   ok = ok && hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) == hipSuccess;
@@ -913,9 +1099,9 @@ int mg_box_step(mg_box* B, uint32_t steps) {
 
 float mg_box_last_step_ms(const mg_box* B) { return B ? B->last_ms : 0.0f; }
 
-int mg_box_set_timing(mg_box* B, int on) {
+int mg_box_set_timing(mg_box* B, uint32_t class_mask) {
   if (!B) return MG_E_ARG;
-  B->timing = on != 0;
+  B->timing = class_mask;
   for (int k = 0; k < BX_NCLASS; k++) { B->k_ms[k] = 0.0; B->k_n[k] = 0; }
   return MG_OK;
 }
@@ -981,6 +1167,9 @@ int mg_box_stats(mg_box* B, double* out3) {
   if (hipMemcpy(&s, B->sc, sizeof s, hipMemcpyDeviceToHost) != hipSuccess) return MG_E_DEVICE;
   out3[0] = s.it; out3[1] = s.res; out3[2] = s.bnorm;
   return MG_OK;
+}
+
+uint64_t mg_box_total_cg_iterations(const mg_box* B) { return B ? B->total_it : 0;
 }
 
 int64_t mg_box_neighbors(mg_box* B, uint32_t* off, uint32_t* nb, int64_t cap) {

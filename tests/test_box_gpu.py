@@ -45,7 +45,8 @@ def test_initial_state_matches_oracle_seed(box_mod, orc):
     assert g == og
 
 
-@pytest.mark.parametrize("side,mu,steps", [(64, 1.0e-3, 3), (40, 1.0, 2), (48, 0.0, 2), (200, 1.0e-3, 2)])
+@pytest.mark.parametrize("side,mu,steps", [(64, 1.0e-3, 3), (40, 1.0, 2), (48, 0.0, 2), (200, 1.0e-3, 2),
+                                             (8, 1.0e-3, 3), (10, 1.0, 2)])
 def test_steps_match_oracle(box_mod, orc, side, mu, steps):
     cfg = {"side": side, "mu": mu}
     print(f"box parity side={side} mu={mu}", flush=True)
@@ -85,3 +86,15 @@ def test_full_size_million_particles(box_mod, orc):
     # momentum: f32 positions quantise v = (x - x0)/dt at ulp(L)/dt per particle
     ulp = float(np.spacing(np.float32(1.024))) * 1440.0
     assert np.abs(v1.sum(0) - v0.sum(0)).max() < 8 * ulp * np.sqrt(b.n)
+
+
+def test_capped_cg_matches_oracle(box_mod, orc):
+    # the CG stops at max_cg without converging; both sides must stop at the same iterate
+    cfg = {"side": 48, "mu": 1.0, "max_cg": 3}
+    b = box_mod.Box(cfg)
+    o = orc.C3Oracle(cfg, "mixed")
+    for k in range(2):
+        b.step()
+        o.step()
+        _compare(b, o, f"max_cg=3 step={k}")
+    assert b.stats()["cg_iterations"] == 3
