@@ -41,6 +41,13 @@
 #define BX_SCAN_ITEMS 16
 #define BX_SCAN_CHUNK (BX_BLOCK * BX_SCAN_ITEMS)
 #define BX_CG_CHUNK 4
+// Loop-shape switches (both forms give identical results; the defaults are the measured faster ones).
+#ifndef BX_BATCH
+#define BX_BATCH 1     // issue the gathers of four table entries before using them
+#endif
+#ifndef BX_INTERIOR
+#define BX_INTERIOR 0  // skip the minimum image for particles whose 3x3 cells do not wrap
+#endif
 #ifndef BX_KMAX
 #define BX_KMAX 32  // pairs kept per particle in the per-step pair table (more: the kernel rescans)
 #endif
@@ -101,7 +108,7 @@ BDEV uint32_t cell1(float a, const BoxGeo& g) {
 // True when the 3x3 cells around the particle's (predicted-position) cell do not wrap; needs at
 // least 4 cells per side so that two cells span at most L/2.
 BDEV bool interior(float2 xp, const BoxGeo& g) {
-  if (g.ncx < 4u) return false;
+  if (!BX_INTERIOR || g.ncx < 4u) return false;
   const uint32_t cx = cell1(xp.x, g), cy = cell1(xp.y, g);
   return cx >= 1u && cx + 2u <= g.ncx && cy >= 1u && cy + 2u <= g.ncx;
 }
@@ -226,7 +233,7 @@ BDEV void for_pairs_g(const BoxGeo& g, const PairView& pv, uint32_t i, L&& load,
     const uint32_t* __restrict__ col = pv.nb + i;
     const size_t n = g.n;
     uint32_t k = 0;
-    for (; k + 4u <= c; k += 4u) {
+    for (; BX_BATCH && k + 4u <= c; k += 4u) {
       const uint32_t j0 = col[(size_t)k * n], j1 = col[(size_t)(k + 1u) * n];
       const uint32_t j2 = col[(size_t)(k + 2u) * n], j3 = col[(size_t)(k + 3u) * n];
       const auto a0 = load(j0);
