@@ -41,6 +41,8 @@
 #define BX_SCAN_ITEMS 16
 #define BX_SCAN_CHUNK (BX_BLOCK * BX_SCAN_ITEMS)
 #define BX_CG_CHUNK 4
+#define BX_WINDOW 16  // steps enqueued between host waits in a multi-step call
+#define BX_SPARE_IT 1  // CG iterations launched beyond the predicted count in a window
 // Loop-shape switches (both forms give identical results; the defaults are the measured faster ones).
 #ifndef BX_BATCH
 #define BX_BATCH 1     // issue the gathers of four table entries before using them
@@ -713,6 +715,29 @@ BXKR bx_cg_check(BoxGeo g, const double* __restrict__ part_rr_last, BoxScal* __r
 }
 
 // ---------------------------------------------------------------------------------------------
+// multi-step windows: state checkpoints and the per-step CG verdict
+// ---------------------------------------------------------------------------------------------
+// Copies the step state (x, v, id): live -> checkpoint slot, or checkpoint slot -> live.
+BXK bx_checkpoint(BoxGeo g, const float2* __restrict__ X, const float2* __restrict__ V, const uint32_t* __restrict__ id,
+                  float2* __restrict__ cx, float2* __restrict__ cv, uint32_t* __restrict__ cid) {
+  const uint32_t i = blockIdx.x * BX_BLOCK + threadIdx.x;
+  if (i >= g.n) return;
+  cx[i] = X[i];
+  cv[i] = V[i];
+  cid[i] = id[i];
+}
+// After step s of a window: records its CG iteration count and marks the first step whose CG was
+// still running when its launched iterations ran out (that step is re-run from its checkpoint).
+BXK bx_verify(const BoxScal* __restrict__ sc, uint32_t* __restrict__ verdict, uint32_t s) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  verdict[1 + s] = sc->it;
+  if (!sc->done && verdict[0] > s) verdict[0] = s;
+}
+BXK bx_verdict_reset(uint32_t* __restrict__ verdict) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) verdict[0] = 0xFFFFFFFFu;
+}
+
+// ---------------------------------------------------------------------------------------------
 // debug export: neighbour lists in the traversal order, rows and entries as original ids
 // ---------------------------------------------------------------------------------------------
 BXK bx_nbr_count(BoxGeo g, PairView pv, const uint32_t* __restrict__ id, uint32_t* __restrict__ cnt) {
@@ -762,6 +787,12 @@ struct mg_box {
   bool have_neighbors = false;
   uint32_t last_it = 0;  // CG iterations of the previous step (sizes the first chunk)
   uint64_t total_it = 0;  // CG iterations summed over every step run
+  // multi-step windows: checkpoint ring (state at the start of each step of the window), the
+  // device verdict [first overrun step, iterations of step 0..W-1] and its pinned host copy
+  float2 *ck_x = nullptr, *ck_v = nullptr;
+  uint32_t* ck_id = nullptr;
+  uint32_t* verdict = nullptr;
+  uint32_t* h_verdict = nullptr;
   float last_ms = 0.0f;
   std::string err;
   // optional per-kernel timing: event pairs around launches, accumulated after each sync
@@ -1013,6 +1044,51 @@ int run_step(mg_box* B) {
   return MG_OK;
 }
 
+// Runs `steps` steps with one host wait per window of up to BX_WINDOW steps. Each step of a window
+// is enqueued with the state checkpointed first and its CG given the last verified iteration count
+// plus BX_SPARE_IT iterations (launches after convergence exit at once). If some step's CG had not
+// converged when its iterations ran out, the window is cut there: that step's checkpoint is
+// restored and the step is re-run with host-driven chunks (run_step) before the next window. The
+// results are those of running every step with run_step.
+int run_window(mg_box* B, uint32_t steps) {
+  const BoxGeo& g = B->g;
+  const bool cg_on = B->cfg.mu > 0.0f && B->cfg.max_cg > 0;
+  uint32_t done = 0;
+  while (done < steps) {
+    const uint32_t W = std::min<uint32_t>(steps - done, BX_WINDOW);
+    const uint32_t pred = std::min<uint32_t>(std::max<uint32_t>(B->last_it, 1u) + BX_SPARE_IT, B->cfg.max_cg);
+    hipLaunchKernelGGL(bx_verdict_reset, dim3(1), dim3(64), 0, B->stream, B->verdict);
+    for (uint32_t s = 0; s < W; s++) {
+      const size_t o = (size_t)s * g.n;
+      hipLaunchKernelGGL(bx_checkpoint, blocks_for(g.n), dim3(BX_BLOCK), 0, B->stream, g, B->X, B->V, B->id,
+                         B->ck_x + o, B->ck_v + o, B->ck_id + o);
+      if (!enqueue_front(B)) { B->err = "kernel launch failed"; return MG_E_DEVICE; }
+      if (cg_on) enqueue_cg_chunk(B, pred);
+      hipLaunchKernelGGL(bx_verify, dim3(1), dim3(64), 0, B->stream, B->sc, B->verdict, s);
+    }
+    hipMemcpyAsync(B->h_verdict, B->verdict, (W + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, B->stream);
+    if (hipStreamSynchronize(B->stream) != hipSuccess) { B->err = "HIP stream error"; return MG_E_DEVICE; }
+    if (hipGetLastError() != hipSuccess) { B->err = "kernel launch failed"; return MG_E_DEVICE; }
+    collect_times(B);
+    const uint32_t bad = B->h_verdict[0];
+    const uint32_t good = bad < W ? bad : W;
+    for (uint32_t s = 0; s < good; s++) B->total_it += B->h_verdict[1 + s];
+    if (good > 0) B->last_it = B->h_verdict[good];
+    done += good;
+    if (bad < W) {
+      // restore the state at the start of the overrun step and run it with host-driven chunks
+      const size_t o = (size_t)bad * g.n;
+      hipLaunchKernelGGL(bx_checkpoint, blocks_for(g.n), dim3(BX_BLOCK), 0, B->stream, g, B->ck_x + o, B->ck_v + o,
+                         B->ck_id + o, B->X, B->V, B->id);
+      B->last_it = std::max<uint32_t>(B->last_it, pred);
+      const int rc = run_step(B);
+      if (rc) return rc;
+      done += 1;
+    }
+  }
+  return MG_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1051,15 +1127,18 @@ int mg_box_create(const mg_box_config* cfg, mg_box** out, char* err, size_t errl
   off = carve(off, B->part_pap, (size_t)B->g.nred); off = carve(off, B->P2, n);
   off = carve(off, B->pnb, (size_t)BX_KMAX * n); off = carve(off, B->pcnt, n);
   off = carve(off, B->cw, (size_t)BX_KMAX * n); off = carve(off, B->ce, (size_t)BX_KMAX * n);
+  off = carve(off, B->ck_x, (size_t)BX_WINDOW * n); off = carve(off, B->ck_v, (size_t)BX_WINDOW * n);
+  off = carve(off, B->ck_id, (size_t)BX_WINDOW * n); off = carve(off, B->verdict, BX_WINDOW + 1);
   off = carve(off, B->sc, 1);
   if (hipMalloc(&B->arena, off) != hipSuccess) { box_set_err(err, errlen, "hipMalloc failed"); delete B; return MG_E_DEVICE; }
   const uintptr_t base = (uintptr_t)B->arena;
   auto fix = [&](auto*& p) { p = reinterpret_cast<std::remove_reference_t<decltype(p)>>(base + (uintptr_t)p); };
   fix(B->X); fix(B->XP); fix(B->X0); fix(B->V); fix(B->X2); fix(B->Xb); fix(B->X0b); fix(B->Vb);
   fix(B->id); fix(B->idb); fix(B->key); fix(B->slot); fix(B->start); fix(B->cursor); fix(B->bsum);
-  fix(B->LAM); fix(B->DENS); fix(B->R); fix(B->P); fix(B->AP); fix(B->part_b); fix(B->part_rr); fix(B->part_pap); fix(B->P2); fix(B->sc); fix(B->pnb); fix(B->pcnt); fix(B->cw); fix(B->ce);
+  fix(B->LAM); fix(B->DENS); fix(B->R); fix(B->P); fix(B->AP); fix(B->part_b); fix(B->part_rr); fix(B->part_pap); fix(B->P2); fix(B->sc); fix(B->pnb); fix(B->pcnt); fix(B->cw); fix(B->ce); fix(B->ck_x); fix(B->ck_v); fix(B->ck_id); fix(B->verdict);
   bool ok = hipMemset(B->arena, 0, off) == hipSuccess;
   ok = ok && hipHostMalloc((void**)&B->h_sc, sizeof(BoxScal), 0) == hipSuccess;
+  ok = ok && hipHostMalloc((void**)&B->h_verdict, (BX_WINDOW + 1) * sizeof(uint32_t), 0) == hipSuccess;
   ok = ok && hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) == hipSuccess;
   ok = ok && hipEventCreate(&B->ev0) == hipSuccess && hipEventCreate(&B->ev1) == hipSuccess;
   if (ok) {
@@ -1081,6 +1160,7 @@ void mg_box_destroy(mg_box* B) {
   if (B->stream) (void)hipStreamSynchronize(B->stream);
   if (B->arena) (void)hipFree(B->arena);
   if (B->h_sc) (void)hipHostFree(B->h_sc);
+  if (B->h_verdict) (void)hipHostFree(B->h_verdict);
   if (B->ev0) (void)hipEventDestroy(B->ev0);
   if (B->ev1) (void)hipEventDestroy(B->ev1);
   collect_times(B);
@@ -1094,10 +1174,8 @@ size_t mg_box_count(const mg_box* B) { return B ? (size_t)B->g.n : 0; }
 int mg_box_step(mg_box* B, uint32_t steps) {
   if (!B) return MG_E_ARG;
   hipEventRecord(B->ev0, B->stream);
-  for (uint32_t s = 0; s < steps; s++) {
-    int rc = run_step(B);
-    if (rc) return rc;
-  }
+  const int rc = run_window(B, steps);
+  if (rc) return rc;
   hipEventRecord(B->ev1, B->stream);
   if (hipEventSynchronize(B->ev1) != hipSuccess) return MG_E_DEVICE;
   hipEventElapsedTime(&B->last_ms, B->ev0, B->ev1);

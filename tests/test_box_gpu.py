@@ -98,3 +98,20 @@ def test_capped_cg_matches_oracle(box_mod, orc):
         o.step()
         _compare(b, o, f"max_cg=3 step={k}")
     assert b.stats()["cg_iterations"] == 3
+
+
+@pytest.mark.parametrize("side,mu,steps", [(64, 1.0e-3, 20), (64, 1.0, 5), (48, 0.0, 17)])
+def test_multi_step_windows_match_oracle(box_mod, orc, side, mu, steps):
+    # K steps in one call run as windows with checkpoints; a step whose CG outruns the predicted
+    # iteration count is re-run from its checkpoint (mu = 1 forces that on the first window)
+    cfg = {"side": side, "mu": mu}
+    b = box_mod.Box(cfg)
+    o = orc.C3Oracle(cfg, "mixed")
+    b.step(steps)
+    o.step(steps)
+    x, v = b.state()
+    xo, vo = o.state()
+    assert np.array_equal(x.astype(np.float64), xo)
+    assert np.array_equal(b.density().astype(np.float64), o.density())
+    assert np.abs(v - vo).max() / max(np.abs(vo).max(), 1e-30) <= V_TOL
+    assert b.stats()["cg_iterations"] == o.stats()["cg_iterations"]
