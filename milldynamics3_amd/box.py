@@ -65,6 +65,7 @@ def _lib():
     L.mg_box_set_timing.argtypes = [P, C.c_uint32]
     L.mg_box_total_cg_iterations.argtypes = [P]
     L.mg_box_total_cg_iterations.restype = C.c_uint64
+    L.mg_box_set_cg_prediction.argtypes = [P, C.c_uint32]
     L.mg_box_kernel_times.argtypes = [P, dp, C.POINTER(C.c_uint64), C.c_int]
     _ready.add(id(L))
     return L
@@ -103,6 +104,10 @@ class Box:
     @property
     def total_cg_iterations(self) -> int:
         return int(self._L.mg_box_total_cg_iterations(self._h))
+
+    def set_cg_prediction(self, iterations: int):
+        """CG iteration count the next multi-step window assumes (see include/millgpu.h)."""
+        _sim._check(self._L.mg_box_set_cg_prediction(self._h, iterations), what="set_cg_prediction")
 
     @property
     def last_ms(self) -> float:

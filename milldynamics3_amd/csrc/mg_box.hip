@@ -703,14 +703,23 @@ BXKR bx_cg_upd(BoxGeo g, float2* __restrict__ V, float2* __restrict__ R, const f
 
 // End of a chunk of iterations ending with iteration t: the convergence test and the cap on
 // r(t).r(t), so the host sees `done` without another iteration being launched.
-BXKR bx_cg_check(BoxGeo g, const double* __restrict__ part_rr_last, BoxScal* __restrict__ sc, uint32_t t) {
-  if (sc->done) return;
-  const double rs = sum_parts<BX_RBLOCK>(part_rr_last, g.nred);
-  if (threadIdx.x != 0) return;
-  sc->res = sqrt(rs);
-  if (sqrt(rs) <= sc->target || t >= sc->max_cg) {
-    sc->it = t;
-    sc->done = 1u;
+// In a multi-step window it also records the step's verdict: verdict[1 + s] = iterations, and
+// verdict[0] = the first step s whose CG was still running when its launched iterations ran out.
+BXKR bx_cg_check(BoxGeo g, const double* __restrict__ part_rr_last, BoxScal* __restrict__ sc, uint32_t t,
+                 uint32_t* __restrict__ verdict, uint32_t s) {
+  if (!sc->done) {
+    const double rs = sum_parts<BX_RBLOCK>(part_rr_last, g.nred);
+    if (threadIdx.x == 0) {
+      sc->res = sqrt(rs);
+      if (sqrt(rs) <= sc->target || t >= sc->max_cg) {
+        sc->it = t;
+        sc->done = 1u;
+      }
+    }
+  }
+  if (threadIdx.x == 0 && verdict) {
+    verdict[1 + s] = sc->it;
+    if (!sc->done && verdict[0] > s) verdict[0] = s;
   }
 }
 
@@ -725,13 +734,6 @@ BXK bx_checkpoint(BoxGeo g, const float2* __restrict__ X, const float2* __restri
   cx[i] = X[i];
   cv[i] = V[i];
   cid[i] = id[i];
-}
-// After step s of a window: records its CG iteration count and marks the first step whose CG was
-// still running when its launched iterations ran out (that step is re-run from its checkpoint).
-BXK bx_verify(const BoxScal* __restrict__ sc, uint32_t* __restrict__ verdict, uint32_t s) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  verdict[1 + s] = sc->it;
-  if (!sc->done && verdict[0] > s) verdict[0] = s;
 }
 BXK bx_verdict_reset(uint32_t* __restrict__ verdict) {
   if (threadIdx.x == 0 && blockIdx.x == 0) verdict[0] = 0xFFFFFFFFu;
@@ -787,8 +789,9 @@ struct mg_box {
   bool have_neighbors = false;
   uint32_t last_it = 0;  // CG iterations of the previous step (sizes the first chunk)
   uint64_t total_it = 0;  // CG iterations summed over every step run
-  // multi-step windows: checkpoint ring (state at the start of each step of the window), the
-  // device verdict [first overrun step, iterations of step 0..W-1] and its pinned host copy
+  // multi-step windows: checkpoint of the state at the window start, the device verdict
+  // [first overrun step, iterations of step 0..W-1] and its pinned host copy
+  bool cg_stable = false;  // the last window's steps all took the same CG iteration count
   float2 *ck_x = nullptr, *ck_v = nullptr;
   uint32_t* ck_id = nullptr;
   uint32_t* verdict = nullptr;
@@ -988,8 +991,9 @@ bool enqueue_front(mg_box* B) {
   return hipGetLastError() == hipSuccess;
 }
 
-// Enqueues CG iterations cg_t .. cg_t + iters - 1 and the end-of-chunk convergence check.
-void enqueue_cg_chunk(mg_box* B, uint32_t iters) {
+// Enqueues CG iterations cg_t .. cg_t + iters - 1 and the end-of-chunk convergence check (which
+// also records the window verdict of step `s` when `verdict` is given).
+void enqueue_cg_chunk(mg_box* B, uint32_t iters, uint32_t* verdict = nullptr, uint32_t s = 0) {
   const BoxGeo& g = B->g;
   hipStream_t st = B->stream;
   const PairView pv{B->XP, B->start, B->pnb, B->pcnt};
@@ -1015,7 +1019,7 @@ void enqueue_cg_chunk(mg_box* B, uint32_t iters) {
     }
   }
   const uint32_t last = B->cg_t - 1u;
-  hipLaunchKernelGGL(bx_cg_check, dim3(1), rblk, 0, st, g, B->part_rr + (last & 1u) * nr, B->sc, last);
+  hipLaunchKernelGGL(bx_cg_check, dim3(1), rblk, 0, st, g, B->part_rr + (last & 1u) * nr, B->sc, last, verdict, s);
 }
 
 // One full step: the front half and a first CG chunk sized by the previous step's iteration count
@@ -1044,42 +1048,53 @@ int run_step(mg_box* B) {
   return MG_OK;
 }
 
-// Runs `steps` steps with one host wait per window of up to BX_WINDOW steps. Each step of a window
-// is enqueued with the state checkpointed first and its CG given the last verified iteration count
-// plus BX_SPARE_IT iterations (launches after convergence exit at once). If some step's CG had not
-// converged when its iterations ran out, the window is cut there: that step's checkpoint is
-// restored and the step is re-run with host-driven chunks (run_step) before the next window. The
-// results are those of running every step with run_step.
+// Runs `steps` steps with one host wait per window of up to BX_WINDOW steps. The state at the
+// start of a window is checkpointed; each step's CG is given the last verified iteration count
+// (plus BX_SPARE_IT while the counts are changing; launches after convergence exit at once). If
+// the CG of step b of the window had not converged when its iterations ran out, the checkpoint is
+// restored, steps 0 .. b-1 are replayed (deterministically, as they ran), and step b runs with
+// host-driven chunks (run_step). The results are those of running every step with run_step.
 int run_window(mg_box* B, uint32_t steps) {
   const BoxGeo& g = B->g;
   const bool cg_on = B->cfg.mu > 0.0f && B->cfg.max_cg > 0;
+  const dim3 gp = blocks_for(g.n), blk(BX_BLOCK);
   uint32_t done = 0;
   while (done < steps) {
     const uint32_t W = std::min<uint32_t>(steps - done, BX_WINDOW);
-    const uint32_t pred = std::min<uint32_t>(std::max<uint32_t>(B->last_it, 1u) + BX_SPARE_IT, B->cfg.max_cg);
-    hipLaunchKernelGGL(bx_verdict_reset, dim3(1), dim3(64), 0, B->stream, B->verdict);
-    for (uint32_t s = 0; s < W; s++) {
-      const size_t o = (size_t)s * g.n;
-      hipLaunchKernelGGL(bx_checkpoint, blocks_for(g.n), dim3(BX_BLOCK), 0, B->stream, g, B->X, B->V, B->id,
-                         B->ck_x + o, B->ck_v + o, B->ck_id + o);
-      if (!enqueue_front(B)) { B->err = "kernel launch failed"; return MG_E_DEVICE; }
-      if (cg_on) enqueue_cg_chunk(B, pred);
-      hipLaunchKernelGGL(bx_verify, dim3(1), dim3(64), 0, B->stream, B->sc, B->verdict, s);
+    const uint32_t spare = B->cg_stable ? 0u : BX_SPARE_IT;
+    const uint32_t pred = std::min<uint32_t>(std::max<uint32_t>(B->last_it, 1u) + spare, B->cfg.max_cg);
+    if (cg_on) {
+      hipLaunchKernelGGL(bx_checkpoint, gp, blk, 0, B->stream, g, B->X, B->V, B->id, B->ck_x, B->ck_v, B->ck_id);
+      hipLaunchKernelGGL(bx_verdict_reset, dim3(1), dim3(64), 0, B->stream, B->verdict);
     }
-    hipMemcpyAsync(B->h_verdict, B->verdict, (W + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, B->stream);
+    for (uint32_t s = 0; s < W; s++) {
+      if (!enqueue_front(B)) { B->err = "kernel launch failed"; return MG_E_DEVICE; }
+      if (cg_on) enqueue_cg_chunk(B, pred, B->verdict, s);
+    }
+    if (cg_on) hipMemcpyAsync(B->h_verdict, B->verdict, (W + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, B->stream);
     if (hipStreamSynchronize(B->stream) != hipSuccess) { B->err = "HIP stream error"; return MG_E_DEVICE; }
     if (hipGetLastError() != hipSuccess) { B->err = "kernel launch failed"; return MG_E_DEVICE; }
     collect_times(B);
+    if (!cg_on) {
+      done += W;
+      continue;
+    }
     const uint32_t bad = B->h_verdict[0];
     const uint32_t good = bad < W ? bad : W;
-    for (uint32_t s = 0; s < good; s++) B->total_it += B->h_verdict[1 + s];
+    bool same = true;
+    for (uint32_t s = 0; s < good; s++) {
+      B->total_it += B->h_verdict[1 + s];
+      same = same && B->h_verdict[1 + s] == B->h_verdict[1];
+    }
     if (good > 0) B->last_it = B->h_verdict[good];
+    B->cg_stable = bad >= W && same;
     done += good;
     if (bad < W) {
-      // restore the state at the start of the overrun step and run it with host-driven chunks
-      const size_t o = (size_t)bad * g.n;
-      hipLaunchKernelGGL(bx_checkpoint, blocks_for(g.n), dim3(BX_BLOCK), 0, B->stream, g, B->ck_x + o, B->ck_v + o,
-                         B->ck_id + o, B->X, B->V, B->id);
+      hipLaunchKernelGGL(bx_checkpoint, gp, blk, 0, B->stream, g, B->ck_x, B->ck_v, B->ck_id, B->X, B->V, B->id);
+      for (uint32_t s = 0; s < bad; s++) {  // replay the steps that had converged
+        if (!enqueue_front(B)) { B->err = "kernel launch failed"; return MG_E_DEVICE; }
+        enqueue_cg_chunk(B, pred);
+      }
       B->last_it = std::max<uint32_t>(B->last_it, pred);
       const int rc = run_step(B);
       if (rc) return rc;
@@ -1127,8 +1142,8 @@ int mg_box_create(const mg_box_config* cfg, mg_box** out, char* err, size_t errl
   off = carve(off, B->part_pap, (size_t)B->g.nred); off = carve(off, B->P2, n);
   off = carve(off, B->pnb, (size_t)BX_KMAX * n); off = carve(off, B->pcnt, n);
   off = carve(off, B->cw, (size_t)BX_KMAX * n); off = carve(off, B->ce, (size_t)BX_KMAX * n);
-  off = carve(off, B->ck_x, (size_t)BX_WINDOW * n); off = carve(off, B->ck_v, (size_t)BX_WINDOW * n);
-  off = carve(off, B->ck_id, (size_t)BX_WINDOW * n); off = carve(off, B->verdict, BX_WINDOW + 1);
+  off = carve(off, B->ck_x, n); off = carve(off, B->ck_v, n);
+  off = carve(off, B->ck_id, n); off = carve(off, B->verdict, BX_WINDOW + 1);
   off = carve(off, B->sc, 1);
   if (hipMalloc(&B->arena, off) != hipSuccess) { box_set_err(err, errlen, "hipMalloc failed"); delete B; return MG_E_DEVICE; }
   const uintptr_t base = (uintptr_t)B->arena;
@@ -1254,7 +1269,13 @@ int mg_box_stats(mg_box* B, double* out3) {
   return MG_OK;
 }
 
-uint64_t mg_box_total_cg_iterations(const mg_box* B) { return B ? B->total_it : 0;
+uint64_t mg_box_total_cg_iterations(const mg_box* B) { return B ? B->total_it : 0; }
+
+int mg_box_set_cg_prediction(mg_box* B, uint32_t iterations) {
+  if (!B) return MG_E_ARG;
+  B->last_it = iterations;
+  B->cg_stable = true;
+  return MG_OK;
 }
 
 int64_t mg_box_neighbors(mg_box* B, uint32_t* off, uint32_t* nb, int64_t cap) {

@@ -115,3 +115,22 @@ def test_multi_step_windows_match_oracle(box_mod, orc, side, mu, steps):
     assert np.array_equal(b.density().astype(np.float64), o.density())
     assert np.abs(v - vo).max() / max(np.abs(vo).max(), 1e-30) <= V_TOL
     assert b.stats()["cg_iterations"] == o.stats()["cg_iterations"]
+
+
+def test_window_rerun_after_low_prediction_matches_oracle(box_mod, orc):
+    # at mu = 3 the CG needs 31, 32, 31, 32, 32, ... iterations; after two steps a prediction of 31
+    # makes the second step of the next window overrun, so the window is cut, its first step is
+    # replayed from the checkpoint and the overrun step re-run: results must not change
+    cfg = {"side": 24, "mu": 3.0}
+    b = box_mod.Box(cfg)
+    o = orc.C3Oracle(cfg, "mixed")
+    b.step(2)
+    o.step(2)
+    b.set_cg_prediction(31)
+    b.step(6)
+    o.step(6)
+    x, v = b.state()
+    xo, vo = o.state()
+    assert np.array_equal(x.astype(np.float64), xo)
+    assert np.abs(v - vo).max() / max(np.abs(vo).max(), 1e-30) <= V_TOL
+    assert b.stats()["cg_iterations"] == o.stats()["cg_iterations"]
