@@ -165,10 +165,11 @@ int mg_box_density(mg_box* box, float* out_n);
 int mg_box_stats(mg_box* box, double* out3);
 /* CG iterations summed over every step this box has run. */
 uint64_t mg_box_total_cg_iterations(const mg_box* box);
-/* Multi-step calls run windows of steps between host waits, each step's CG given the iteration
- * count the previous window saw; a step that needs more is re-run from the window's checkpoint
- * (same results). This sets the count the next window assumes (a small value exercises the
- * re-run path). */
+/* Multi-step calls run windows of up to 16 steps between host waits, each step's CG given the
+ * largest iteration count the previous window saw (plus a margin while the counts change); a step
+ * that needs more stops the rest of its window and is re-run from its checkpoint (same results),
+ * and the next window is half as long. This sets the count the next window assumes and marks the
+ * counts as steady (a small value exercises the re-run path). */
 int mg_box_set_cg_prediction(mg_box* box, uint32_t iterations);
 /* Neighbour CSR of the last step in traversal order (original ids); returns the edge count,
  * copying the entries only when they fit `cap`. */

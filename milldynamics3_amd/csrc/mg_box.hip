@@ -60,10 +60,13 @@ enum { BXT_GRID = 0, BXT_LAMBDA, BXT_DP, BXT_VEL, BXT_CGINIT, BXT_SPMV, BXT_UPDA
 #define BXKR extern "C" __global__ void __launch_bounds__(BX_RBLOCK)
 #define BDEV __device__ __attribute__((always_inline)) inline
 
-// Geometry and constants of one box, passed by value to every kernel.
+// Geometry and constants of one box, passed by value to every kernel. `halt` is the window stop
+// word: once a step of a multi-step window has run out of CG iterations before converging, the
+// window's remaining kernels return at once (the host restores that step's checkpoint).
 struct BoxGeo {
   uint32_t n, ncx, nblk, xcd_chunk, nred, xcd_chunk_red;
   float h, h2, L, halfL, cw, mass, rest, dt, k6, ks, w0, m_over_rest, mu2m, eta2;
+  const uint32_t* halt;
 };
 
 // Device scalars of the CG loop (pbf.rs:423-480).
@@ -259,7 +262,9 @@ BDEV void for_pairs_g(const BoxGeo& g, const PairView& pv, uint32_t i, L&& load,
 // ---------------------------------------------------------------------------------------------
 // grid build + reorder
 // ---------------------------------------------------------------------------------------------
-BXK bx_zero2(uint32_t* __restrict__ a, uint32_t na, uint32_t* __restrict__ b, uint32_t nb) {
+BXK bx_zero2(const uint32_t* __restrict__ halt, uint32_t* __restrict__ a, uint32_t na, uint32_t* __restrict__ b,
+             uint32_t nb) {
+  if (*halt) return;
   const uint32_t i = blockIdx.x * BX_BLOCK + threadIdx.x;
   if (i < na) a[i] = 0u;
   if (i < nb) b[i] = 0u;
@@ -283,6 +288,7 @@ BDEV uint32_t wave_run(uint32_t key, uint32_t& head) {
 
 BXK bx_predict(BoxGeo g, float2* __restrict__ X, const float2* __restrict__ V, float2* __restrict__ X0,
                uint32_t* __restrict__ key, uint32_t* __restrict__ count) {
+  if (*g.halt) return;
   const uint32_t i = blockIdx.x * BX_BLOCK + threadIdx.x;
   uint32_t k = 0xFFFFFFFFu;
   if (i < g.n) {
@@ -319,7 +325,9 @@ BDEV uint32_t block_excl_scan(uint32_t v, uint32_t& total) {
   total = tot;
   return before + x - v;
 }
-BXK bx_scan_totals(const uint32_t* __restrict__ a, uint32_t m, uint32_t* __restrict__ bsum) {
+BXK bx_scan_totals(const uint32_t* __restrict__ halt, const uint32_t* __restrict__ a, uint32_t m,
+                   uint32_t* __restrict__ bsum) {
+  if (*halt) return;
   const uint32_t base = blockIdx.x * BX_SCAN_CHUNK + threadIdx.x * BX_SCAN_ITEMS;
   uint32_t s = 0;
   for (int k = 0; k < BX_SCAN_ITEMS; k++) s += base + k < m ? a[base + k] : 0u;
@@ -327,7 +335,8 @@ BXK bx_scan_totals(const uint32_t* __restrict__ a, uint32_t m, uint32_t* __restr
   block_excl_scan(s, tot);
   if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
-BXK bx_scan_top(uint32_t* __restrict__ bsum, uint32_t nb) {  // one block, nb <= BX_SCAN_CHUNK
+BXK bx_scan_top(const uint32_t* __restrict__ halt, uint32_t* __restrict__ bsum, uint32_t nb) {  // one block
+  if (*halt) return;
   const uint32_t base = threadIdx.x * BX_SCAN_ITEMS;
   uint32_t s = 0;
   for (int k = 0; k < BX_SCAN_ITEMS; k++) s += base + k < nb ? bsum[base + k] : 0u;
@@ -340,7 +349,9 @@ BXK bx_scan_top(uint32_t* __restrict__ bsum, uint32_t nb) {  // one block, nb <=
       off += t;
     }
 }
-BXK bx_scan_apply(uint32_t* __restrict__ a, uint32_t m, const uint32_t* __restrict__ bsum) {
+BXK bx_scan_apply(const uint32_t* __restrict__ halt, uint32_t* __restrict__ a, uint32_t m,
+                  const uint32_t* __restrict__ bsum) {
+  if (*halt) return;
   const uint32_t base = blockIdx.x * BX_SCAN_CHUNK + threadIdx.x * BX_SCAN_ITEMS;
   uint32_t s = 0;
   for (int k = 0; k < BX_SCAN_ITEMS; k++) s += base + k < m ? a[base + k] : 0u;
@@ -355,6 +366,7 @@ BXK bx_scan_apply(uint32_t* __restrict__ a, uint32_t m, const uint32_t* __restri
 }
 BXK bx_scatter(BoxGeo g, const uint32_t* __restrict__ key, const uint32_t* __restrict__ start,
                uint32_t* __restrict__ cursor, uint32_t* __restrict__ slot) {
+  if (*g.halt) return;
   const uint32_t i = blockIdx.x * BX_BLOCK + threadIdx.x;
   const uint32_t k = i < g.n ? key[i] : 0xFFFFFFFFu;
   uint32_t head;
@@ -366,9 +378,10 @@ BXK bx_scatter(BoxGeo g, const uint32_t* __restrict__ key, const uint32_t* __res
 }
 // One lane per cell: order the cell's slots by particle id (a cell holds a handful of particles).
 // Lane 0 of the grid also resets this step's CG scalars (nothing reads them before the CG).
-BXK bx_cellsort(uint32_t ncell, const uint32_t* __restrict__ start, uint32_t* __restrict__ slot,
-                const uint32_t* __restrict__ id, BoxScal* __restrict__ sc, uint32_t cg_off, uint32_t max_cg,
-                float tol) {
+BXK bx_cellsort(const uint32_t* __restrict__ halt, uint32_t ncell, const uint32_t* __restrict__ start,
+                uint32_t* __restrict__ slot, const uint32_t* __restrict__ id, BoxScal* __restrict__ sc,
+                uint32_t cg_off, uint32_t max_cg, float tol) {
+  if (*halt) return;
   const uint32_t c = blockIdx.x * BX_BLOCK + threadIdx.x;
   if (c == 0) {
     sc->done = cg_off;
@@ -395,6 +408,7 @@ BXK bx_cellsort(uint32_t ncell, const uint32_t* __restrict__ start, uint32_t* __
 BXK bx_gather(BoxGeo g, const uint32_t* __restrict__ slot, const float2* __restrict__ X, const float2* __restrict__ X0,
               const float2* __restrict__ V, const uint32_t* __restrict__ id, float2* __restrict__ Xb,
               float2* __restrict__ XPb, float2* __restrict__ X0b, float2* __restrict__ Vb, uint32_t* __restrict__ idb) {
+  if (*g.halt) return;
   const uint32_t q = blockIdx.x * BX_BLOCK + threadIdx.x;
   if (q >= g.n) return;
   const uint32_t i = slot[q];
@@ -408,6 +422,7 @@ BXK bx_gather(BoxGeo g, const uint32_t* __restrict__ slot, const float2* __restr
 
 BXK bx_pairs(BoxGeo g, const float2* __restrict__ XP, const uint32_t* __restrict__ start, uint32_t* __restrict__ nb,
               uint32_t* __restrict__ cnt) {
+  if (*g.halt) return;
   const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
   if (i >= g.n) return;
   uint32_t c = 0;
@@ -446,6 +461,7 @@ BDEV void lambda_row(const BoxGeo& g, const PairView& pv, uint32_t i, const floa
   LAM[i] = -c / (sgs + 200.0f);
 }
 BXK bx_lambda(BoxGeo g, PairView pv, const float2* __restrict__ X, float* __restrict__ LAM) {
+  if (*g.halt) return;
   const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
   if (i >= g.n) return;
   if (interior(pv.XP[i], g)) lambda_row<false>(g, pv, i, X, LAM);
@@ -469,6 +485,7 @@ BDEV void dp_row(const BoxGeo& g, const PairView& pv, uint32_t i, const float2* 
 }
 BXK bx_dp(BoxGeo g, PairView pv, const float2* __restrict__ X, const float* __restrict__ LAM,
           float2* __restrict__ Xout) {
+  if (*g.halt) return;
   const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
   if (i >= g.n) return;
   if (interior(pv.XP[i], g)) dp_row<false>(g, pv, i, X, LAM, Xout);
@@ -480,6 +497,7 @@ BXK bx_dp(BoxGeo g, PairView pv, const float2* __restrict__ X, const float* __re
 // ---------------------------------------------------------------------------------------------
 BXKR bx_vel_density(BoxGeo g, PairView pv, const float2* __restrict__ X, const float2* __restrict__ X0,
                     float2* __restrict__ V, float* __restrict__ DENS, double* __restrict__ part_b) {
+  if (*g.halt) return;
   const uint32_t i = tile_id_red(g) * BX_RBLOCK + threadIdx.x;
   double vv = 0.0;
   if (i < g.n) {
@@ -566,7 +584,7 @@ BXKR bx_cg_init(BoxGeo g, PairView pv, const float2* __restrict__ X, const float
                 const float2* __restrict__ V, float2* __restrict__ R, float2* __restrict__ P,
                 float* __restrict__ CW, float2* __restrict__ CE, double* __restrict__ part_rr0,
                 const BoxScal* __restrict__ sc) {
-  if (sc->done) return;
+  if (*g.halt | sc->done) return;  // both words loaded at once
   const uint32_t i = tile_id_red(g) * BX_RBLOCK + threadIdx.x;
   double rr = 0.0;
   if (i < g.n) {
@@ -593,7 +611,7 @@ BXKR bx_cg_dir(BoxGeo g, PairView pv, const float2* __restrict__ X, const float*
                const double* __restrict__ part_b, const double* __restrict__ part_rr_prev,
                const double* __restrict__ part_rr_prev2, double* __restrict__ part_pap, BoxScal* __restrict__ sc,
                uint32_t t) {
-  if (sc->done) return;
+  if (*g.halt | sc->done) return;  // both words loaded at once
   double target;
   bool stop = false;
   float beta = 0.0f;
@@ -678,7 +696,7 @@ BXKR bx_cg_upd(BoxGeo g, float2* __restrict__ V, float2* __restrict__ R, const f
                const float2* __restrict__ AP, const double* __restrict__ part_pap,
                const double* __restrict__ part_rr_prev, double* __restrict__ part_rr_out,
                BoxScal* __restrict__ sc, uint32_t t) {
-  if (sc->done) return;
+  if (*g.halt | sc->done) return;  // both words loaded at once
   const double pap = sum_parts<BX_RBLOCK>(part_pap, g.nred);
   if (!(pap > 0.0) || !isfinite(pap)) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -704,9 +722,11 @@ BXKR bx_cg_upd(BoxGeo g, float2* __restrict__ V, float2* __restrict__ R, const f
 // End of a chunk of iterations ending with iteration t: the convergence test and the cap on
 // r(t).r(t), so the host sees `done` without another iteration being launched.
 // In a multi-step window it also records the step's verdict: verdict[1 + s] = iterations, and
-// verdict[0] = the first step s whose CG was still running when its launched iterations ran out.
+// verdict[0] = the first step s whose CG was still running when its launched iterations ran out;
+// that step also sets the stop word, so the rest of the window does no work.
 BXKR bx_cg_check(BoxGeo g, const double* __restrict__ part_rr_last, BoxScal* __restrict__ sc, uint32_t t,
-                 uint32_t* __restrict__ verdict, uint32_t s) {
+                 uint32_t* __restrict__ verdict, uint32_t s, uint32_t* __restrict__ halt) {
+  if (*g.halt) return;
   if (!sc->done) {
     const double rs = sum_parts<BX_RBLOCK>(part_rr_last, g.nred);
     if (threadIdx.x == 0) {
@@ -719,7 +739,10 @@ BXKR bx_cg_check(BoxGeo g, const double* __restrict__ part_rr_last, BoxScal* __r
   }
   if (threadIdx.x == 0 && verdict) {
     verdict[1 + s] = sc->it;
-    if (!sc->done && verdict[0] > s) verdict[0] = s;
+    if (!sc->done) {
+      verdict[0] = s;
+      *halt = 1u;
+    }
   }
 }
 
@@ -729,14 +752,18 @@ BXKR bx_cg_check(BoxGeo g, const double* __restrict__ part_rr_last, BoxScal* __r
 // Copies the step state (x, v, id): live -> checkpoint slot, or checkpoint slot -> live.
 BXK bx_checkpoint(BoxGeo g, const float2* __restrict__ X, const float2* __restrict__ V, const uint32_t* __restrict__ id,
                   float2* __restrict__ cx, float2* __restrict__ cv, uint32_t* __restrict__ cid) {
+  if (*g.halt) return;
   const uint32_t i = blockIdx.x * BX_BLOCK + threadIdx.x;
   if (i >= g.n) return;
   cx[i] = X[i];
   cv[i] = V[i];
   cid[i] = id[i];
 }
-BXK bx_verdict_reset(uint32_t* __restrict__ verdict) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) verdict[0] = 0xFFFFFFFFu;
+BXK bx_verdict_reset(uint32_t* __restrict__ verdict, uint32_t* __restrict__ halt) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    verdict[0] = 0xFFFFFFFFu;
+    *halt = 0u;
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -789,13 +816,15 @@ struct mg_box {
   bool have_neighbors = false;
   uint32_t last_it = 0;  // CG iterations of the previous step (sizes the first chunk)
   uint64_t total_it = 0;  // CG iterations summed over every step run
-  // multi-step windows: checkpoint of the state at the window start, the device verdict
-  // [first overrun step, iterations of step 0..W-1] and its pinned host copy
+  // multi-step windows: state checkpoints (slot s = start of step s of the window), the device
+  // verdict [first overrun step, iterations of step 0..W-1] and its pinned host copy
   bool cg_stable = false;  // the last window's steps all took the same CG iteration count
   float2 *ck_x = nullptr, *ck_v = nullptr;
   uint32_t* ck_id = nullptr;
   uint32_t* verdict = nullptr;
   uint32_t* h_verdict = nullptr;
+  uint32_t* halt = nullptr;  // window stop word (BoxGeo::halt)
+  uint32_t win = BX_WINDOW;  // length of the next window: halved after an overrun, doubled after a clean one
   float last_ms = 0.0f;
   std::string err;
   // optional per-kernel timing: event pairs around launches, accumulated after each sync
@@ -855,6 +884,7 @@ bool box_geometry(const mg_box_config& c, BoxGeo& g, std::string& err) {
   g.m_over_rest = g.mass / c.rest;
   g.mu2m = g.mass * (c.mu + c.mu);
   g.eta2 = 0.01f * g.h * g.h;
+  g.halt = nullptr;
   g.nblk = (g.n + BX_BLOCK - 1) / BX_BLOCK;
   g.xcd_chunk = (g.nblk % 8u == 0u) ? g.nblk / 8u : 0u;
   g.nred = (g.n + BX_RBLOCK - 1) / BX_RBLOCK;
@@ -945,13 +975,14 @@ bool enqueue_front(mg_box* B) {
   const dim3 blk(BX_BLOCK), gp = blocks_for(g.n);
   {
   Tic t(B, BXT_GRID);
-  hipLaunchKernelGGL(bx_zero2, blocks_for(B->ncell + 1), blk, 0, st, B->start, B->ncell + 1, B->cursor, B->ncell);
+  hipLaunchKernelGGL(bx_zero2, blocks_for(B->ncell + 1), blk, 0, st, B->halt, B->start, B->ncell + 1, B->cursor,
+                     B->ncell);
   hipLaunchKernelGGL(bx_predict, gp, blk, 0, st, g, B->X, B->V, B->X0, B->key, B->start);
-  hipLaunchKernelGGL(bx_scan_totals, dim3(B->nscan), blk, 0, st, B->start, B->ncell + 1, B->bsum);
-  hipLaunchKernelGGL(bx_scan_top, dim3(1), blk, 0, st, B->bsum, B->nscan);
-  hipLaunchKernelGGL(bx_scan_apply, dim3(B->nscan), blk, 0, st, B->start, B->ncell + 1, B->bsum);
+  hipLaunchKernelGGL(bx_scan_totals, dim3(B->nscan), blk, 0, st, B->halt, B->start, B->ncell + 1, B->bsum);
+  hipLaunchKernelGGL(bx_scan_top, dim3(1), blk, 0, st, B->halt, B->bsum, B->nscan);
+  hipLaunchKernelGGL(bx_scan_apply, dim3(B->nscan), blk, 0, st, B->halt, B->start, B->ncell + 1, B->bsum);
   hipLaunchKernelGGL(bx_scatter, gp, blk, 0, st, g, B->key, B->start, B->cursor, B->slot);
-  hipLaunchKernelGGL(bx_cellsort, blocks_for(B->ncell), blk, 0, st, B->ncell, B->start, B->slot, B->id, B->sc, cg_off,
+  hipLaunchKernelGGL(bx_cellsort, blocks_for(B->ncell), blk, 0, st, B->halt, B->ncell, B->start, B->slot, B->id, B->sc, cg_off,
                      B->cfg.max_cg, B->cfg.tol);
   hipLaunchKernelGGL(bx_gather, gp, blk, 0, st, g, B->slot, B->X, B->X0, B->V, B->id, B->Xb, B->XP, B->X0b, B->Vb,
                      B->idb);
@@ -1019,16 +1050,17 @@ void enqueue_cg_chunk(mg_box* B, uint32_t iters, uint32_t* verdict = nullptr, ui
     }
   }
   const uint32_t last = B->cg_t - 1u;
-  hipLaunchKernelGGL(bx_cg_check, dim3(1), rblk, 0, st, g, B->part_rr + (last & 1u) * nr, B->sc, last, verdict, s);
+  hipLaunchKernelGGL(bx_cg_check, dim3(1), rblk, 0, st, g, B->part_rr + (last & 1u) * nr, B->sc, last, verdict, s,
+                     B->halt);
 }
 
-// One full step: the front half and a first CG chunk sized by the previous step's iteration count
-// are enqueued without a host wait; further chunks follow only if the device has not converged.
-int run_step(mg_box* B) {
+// One full step: the front half and a first CG chunk of `first` iterations are enqueued without a
+// host wait; further chunks follow only if the device has not converged.
+int run_step(mg_box* B, uint32_t first) {
   if (!enqueue_front(B)) { B->err = "kernel launch failed"; return MG_E_DEVICE; }
   uint32_t launched = 0;
   if (B->cfg.mu > 0.0f && B->cfg.max_cg > 0) {
-    launched = std::min<uint32_t>(std::max<uint32_t>(B->last_it, 1u), B->cfg.max_cg);
+    launched = std::min<uint32_t>(std::max<uint32_t>(first, 1u), B->cfg.max_cg);
     enqueue_cg_chunk(B, launched);
   } else {
     launched = B->cfg.max_cg;  // no CG this step
@@ -1048,26 +1080,38 @@ int run_step(mg_box* B) {
   return MG_OK;
 }
 
-// Runs `steps` steps with one host wait per window of up to BX_WINDOW steps. The state at the
-// start of a window is checkpointed; each step's CG is given the last verified iteration count
-// (plus BX_SPARE_IT while the counts are changing; launches after convergence exit at once). If
-// the CG of step b of the window had not converged when its iterations ran out, the checkpoint is
-// restored, steps 0 .. b-1 are replayed (deterministically, as they ran), and step b runs with
-// host-driven chunks (run_step). The results are those of running every step with run_step.
+// Runs `steps` steps with one host wait per window of up to BX_WINDOW steps. Each step's CG is
+// launched with the largest iteration count of the previous window plus a margin while the counts
+// are changing (launches after convergence return at once). If step b of a window had not converged
+// when its iterations ran out, it sets the stop word (the rest of the window does no work), the
+// state checkpointed at the start of step b is restored and step b is run with host-driven chunks
+// (run_step); the next window is then half as long. The results are those of running every step
+// with run_step: iterations past convergence change nothing.
 int run_window(mg_box* B, uint32_t steps) {
   const BoxGeo& g = B->g;
   const bool cg_on = B->cfg.mu > 0.0f && B->cfg.max_cg > 0;
   const dim3 gp = blocks_for(g.n), blk(BX_BLOCK);
   uint32_t done = 0;
   while (done < steps) {
-    const uint32_t W = std::min<uint32_t>(steps - done, BX_WINDOW);
-    const uint32_t spare = B->cg_stable ? 0u : BX_SPARE_IT;
-    const uint32_t pred = std::min<uint32_t>(std::max<uint32_t>(B->last_it, 1u) + spare, B->cfg.max_cg);
-    if (cg_on) {
-      hipLaunchKernelGGL(bx_checkpoint, gp, blk, 0, B->stream, g, B->X, B->V, B->id, B->ck_x, B->ck_v, B->ck_id);
-      hipLaunchKernelGGL(bx_verdict_reset, dim3(1), dim3(64), 0, B->stream, B->verdict);
+    const uint32_t W = std::min<uint32_t>(steps - done, cg_on ? B->win : BX_WINDOW);
+    const uint32_t m = std::max<uint32_t>(B->last_it, 1u);
+    const uint32_t spare = B->cg_stable ? 0u : BX_SPARE_IT + m / 8u;
+    const uint32_t pred = std::min<uint32_t>(m + spare, B->cfg.max_cg);
+    if (cg_on && W == 1) {  // a single step: no checkpoint, chunks driven from the host
+      const int rc = run_step(B, pred);
+      if (rc) return rc;
+      B->win = B->h_sc->it <= pred ? 2u : 1u;
+      B->cg_stable = false;
+      done += 1;
+      continue;
     }
+    if (cg_on) hipLaunchKernelGGL(bx_verdict_reset, dim3(1), dim3(64), 0, B->stream, B->verdict, B->halt);
     for (uint32_t s = 0; s < W; s++) {
+      if (cg_on) {
+        const size_t o = (size_t)s * g.n;
+        hipLaunchKernelGGL(bx_checkpoint, gp, blk, 0, B->stream, g, B->X, B->V, B->id, B->ck_x + o, B->ck_v + o,
+                           B->ck_id + o);
+      }
       if (!enqueue_front(B)) { B->err = "kernel launch failed"; return MG_E_DEVICE; }
       if (cg_on) enqueue_cg_chunk(B, pred, B->verdict, s);
     }
@@ -1082,24 +1126,27 @@ int run_window(mg_box* B, uint32_t steps) {
     const uint32_t bad = B->h_verdict[0];
     const uint32_t good = bad < W ? bad : W;
     bool same = true;
+    uint32_t most = 0;
     for (uint32_t s = 0; s < good; s++) {
       B->total_it += B->h_verdict[1 + s];
       same = same && B->h_verdict[1 + s] == B->h_verdict[1];
+      most = std::max<uint32_t>(most, B->h_verdict[1 + s]);
     }
-    if (good > 0) B->last_it = B->h_verdict[good];
+    if (good > 0) B->last_it = most;
     B->cg_stable = bad >= W && same;
     done += good;
-    if (bad < W) {
-      hipLaunchKernelGGL(bx_checkpoint, gp, blk, 0, B->stream, g, B->ck_x, B->ck_v, B->ck_id, B->X, B->V, B->id);
-      for (uint32_t s = 0; s < bad; s++) {  // replay the steps that had converged
-        if (!enqueue_front(B)) { B->err = "kernel launch failed"; return MG_E_DEVICE; }
-        enqueue_cg_chunk(B, pred);
-      }
-      B->last_it = std::max<uint32_t>(B->last_it, pred);
-      const int rc = run_step(B);
-      if (rc) return rc;
-      done += 1;
+    if (bad >= W) {
+      B->win = std::min<uint32_t>(2u * B->win, BX_WINDOW);
+      continue;
     }
+    B->win = std::max<uint32_t>(B->win / 2u, 1u);
+    const size_t o = (size_t)bad * g.n;
+    hipLaunchKernelGGL(bx_verdict_reset, dim3(1), dim3(64), 0, B->stream, B->verdict, B->halt);
+    hipLaunchKernelGGL(bx_checkpoint, gp, blk, 0, B->stream, g, B->ck_x + o, B->ck_v + o, B->ck_id + o, B->X, B->V,
+                       B->id);
+    const int rc = run_step(B, pred + BX_CG_CHUNK);
+    if (rc) return rc;
+    done += 1;
   }
   return MG_OK;
 }
@@ -1142,15 +1189,16 @@ int mg_box_create(const mg_box_config* cfg, mg_box** out, char* err, size_t errl
   off = carve(off, B->part_pap, (size_t)B->g.nred); off = carve(off, B->P2, n);
   off = carve(off, B->pnb, (size_t)BX_KMAX * n); off = carve(off, B->pcnt, n);
   off = carve(off, B->cw, (size_t)BX_KMAX * n); off = carve(off, B->ce, (size_t)BX_KMAX * n);
-  off = carve(off, B->ck_x, n); off = carve(off, B->ck_v, n);
-  off = carve(off, B->ck_id, n); off = carve(off, B->verdict, BX_WINDOW + 1);
-  off = carve(off, B->sc, 1);
+  off = carve(off, B->ck_x, (size_t)BX_WINDOW * n); off = carve(off, B->ck_v, (size_t)BX_WINDOW * n);
+  off = carve(off, B->ck_id, (size_t)BX_WINDOW * n); off = carve(off, B->verdict, BX_WINDOW + 1);
+  off = carve(off, B->halt, 1); off = carve(off, B->sc, 1);
   if (hipMalloc(&B->arena, off) != hipSuccess) { box_set_err(err, errlen, "hipMalloc failed"); delete B; return MG_E_DEVICE; }
   const uintptr_t base = (uintptr_t)B->arena;
   auto fix = [&](auto*& p) { p = reinterpret_cast<std::remove_reference_t<decltype(p)>>(base + (uintptr_t)p); };
   fix(B->X); fix(B->XP); fix(B->X0); fix(B->V); fix(B->X2); fix(B->Xb); fix(B->X0b); fix(B->Vb);
   fix(B->id); fix(B->idb); fix(B->key); fix(B->slot); fix(B->start); fix(B->cursor); fix(B->bsum);
-  fix(B->LAM); fix(B->DENS); fix(B->R); fix(B->P); fix(B->AP); fix(B->part_b); fix(B->part_rr); fix(B->part_pap); fix(B->P2); fix(B->sc); fix(B->pnb); fix(B->pcnt); fix(B->cw); fix(B->ce); fix(B->ck_x); fix(B->ck_v); fix(B->ck_id); fix(B->verdict);
+  fix(B->LAM); fix(B->DENS); fix(B->R); fix(B->P); fix(B->AP); fix(B->part_b); fix(B->part_rr); fix(B->part_pap); fix(B->P2); fix(B->sc); fix(B->pnb); fix(B->pcnt); fix(B->cw); fix(B->ce); fix(B->ck_x); fix(B->ck_v); fix(B->ck_id); fix(B->verdict); fix(B->halt);
+  B->g.halt = B->halt;
   bool ok = hipMemset(B->arena, 0, off) == hipSuccess;
   ok = ok && hipHostMalloc((void**)&B->h_sc, sizeof(BoxScal), 0) == hipSuccess;
   ok = ok && hipHostMalloc((void**)&B->h_verdict, (BX_WINDOW + 1) * sizeof(uint32_t), 0) == hipSuccess;

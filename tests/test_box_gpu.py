@@ -100,10 +100,12 @@ def test_capped_cg_matches_oracle(box_mod, orc):
     assert b.stats()["cg_iterations"] == 3
 
 
-@pytest.mark.parametrize("side,mu,steps", [(64, 1.0e-3, 20), (64, 1.0, 5), (48, 0.0, 17)])
+@pytest.mark.parametrize("side,mu,steps", [(64, 1.0e-3, 20), (64, 1.0, 5), (48, 0.0, 17), (24, 3.0, 37)])
 def test_multi_step_windows_match_oracle(box_mod, orc, side, mu, steps):
     # K steps in one call run as windows with checkpoints; a step whose CG outruns the predicted
-    # iteration count is re-run from its checkpoint (mu = 1 forces that on the first window)
+    # iteration count stops the rest of its window and is re-run from its checkpoint, and the next
+    # window is shorter (mu = 1 forces that on the first window; at mu = 3 the counts vary by one
+    # from step to step over 37 steps)
     cfg = {"side": side, "mu": mu}
     b = box_mod.Box(cfg)
     o = orc.C3Oracle(cfg, "mixed")
@@ -119,8 +121,8 @@ def test_multi_step_windows_match_oracle(box_mod, orc, side, mu, steps):
 
 def test_window_rerun_after_low_prediction_matches_oracle(box_mod, orc):
     # at mu = 3 the CG needs 31, 32, 31, 32, 32, ... iterations; after two steps a prediction of 31
-    # makes the second step of the next window overrun, so the window is cut, its first step is
-    # replayed from the checkpoint and the overrun step re-run: results must not change
+    # makes the second step of the next window overrun, so the rest of the window is stopped and the
+    # overrun step is re-run from its checkpoint: results must not change
     cfg = {"side": 24, "mu": 3.0}
     b = box_mod.Box(cfg)
     o = orc.C3Oracle(cfg, "mixed")
