@@ -175,9 +175,10 @@ int mg_box_set_cg_prediction(mg_box* box, uint32_t iterations);
  * copying the entries only when they fit `cap`. */
 int64_t mg_box_neighbors(mg_box* box, uint32_t* off_n1, uint32_t* nb, int64_t cap);
 /* Per-kernel timing with HIP events on the box's stream for the classes whose bit is set in
- * class_mask (0 = off; resets the totals). Classes: 0 grid build, reorder and pair table (all
- * their launches together), 1 density pass A, 2 density pass B, 3 velocity + density recompute,
- * 4 CG set-up, 5 CG direction + operator apply, 6 CG vector update, 7 unused. */
+ * class_mask (0 = off; resets the totals). Classes: 0 grid build and reorder (all their launches
+ * together; also the pair table when a step has no density pass), 1 density pass A (passes 2..),
+ * 2 density pass B, 3 velocity + density recompute, 4 CG set-up, 5 CG direction + operator
+ * apply, 6 CG vector update, 7 pair table + first density pass A. */
 int mg_box_set_timing(mg_box* box, uint32_t class_mask);
 int mg_box_kernel_times(mg_box* box, double* ms, uint64_t* launches, int nclass);
 

@@ -53,9 +53,12 @@
 #ifndef BX_KMAX
 #define BX_KMAX 32  // pairs kept per particle in the per-step pair table (more: the kernel rescans)
 #endif
+#ifndef BX_NB_RANGE
+#define BX_NB_RANGE 32767  // largest |j - i| a pair-table entry holds (int16; beyond: the kernel rescans)
+#endif
 // kernel classes of the optional per-kernel timing (mg_box_kernel_times)
 #define BX_NCLASS 8
-enum { BXT_GRID = 0, BXT_LAMBDA, BXT_DP, BXT_VEL, BXT_CGINIT, BXT_SPMV, BXT_UPDATE, BXT_SPARE };
+enum { BXT_GRID = 0, BXT_LAMBDA, BXT_DP, BXT_VEL, BXT_CGINIT, BXT_SPMV, BXT_UPDATE, BXT_PAIRLAM };
 #define BXK extern "C" __global__ void __launch_bounds__(BX_BLOCK)
 #define BXKR extern "C" __global__ void __launch_bounds__(BX_RBLOCK)
 #define BDEV __device__ __attribute__((always_inline)) inline
@@ -178,17 +181,36 @@ BDEV double sum_parts(const double* __restrict__ part, uint32_t nb) {
 
 // Per-step pair table: the neighbours of sorted particle i at the predicted positions, in the
 // reference's per-particle order, column-major (nb[k * n + i]) so lane-adjacent particles read
-// adjacent words. cnt[i] is the true count; a particle with more than BX_KMAX pairs is handled by
+// adjacent entries. An entry is the signed 16-bit offset j - i taken modulo n into [-n/2, n/2]
+// (after the cell sort a neighbour sits within about two cell rows of i, or as far on the other
+// side of the periodic seam). cnt[i] is the true count; a particle with more than BX_KMAX pairs, or
+// with a neighbour whose offset does not fit 16 bits, has cnt[i] > BX_KMAX and is handled by
 // rescanning its 3x3 cells (same order, same membership test).
 struct PairView {
   const float2* __restrict__ XP;
   const uint32_t* __restrict__ start;
-  const uint32_t* __restrict__ nb;
+  const int16_t* __restrict__ nb;
   const uint32_t* __restrict__ cnt;
 };
+BDEV uint32_t nb_dec(int32_t d, uint32_t i, uint32_t n) {  // table entry -> particle index
+  int32_t j = (int32_t)i + d;
+  j += j < 0 ? (int32_t)n : 0;
+  j -= j >= (int32_t)n ? (int32_t)n : 0;
+  return (uint32_t)j;
+}
+BDEV bool nb_enc(uint32_t j, uint32_t i, uint32_t n, int16_t& e) {  // false: does not fit
+  int32_t d = (int32_t)j - (int32_t)i;
+  const int32_t half = (int32_t)(n >> 1);
+  if (d > half) d -= (int32_t)n;
+  else if (d < -half) d += (int32_t)n;
+  if (d < -BX_NB_RANGE || d > BX_NB_RANGE) return false;
+  e = (int16_t)d;
+  return true;
+}
+// f(j, d, r2) for every pair, with d = xpi - XP[j] (minimum image) and r2 = |d|^2 as the test used.
 template <bool W = true, class F>
-BDEV void scan_pairs(const BoxGeo& g, const float2* __restrict__ XP, const uint32_t* __restrict__ start, uint32_t i,
-                     float2 xpi, F&& f) {
+BDEV void scan_pairs_d(const BoxGeo& g, const float2* __restrict__ XP, const uint32_t* __restrict__ start, uint32_t i,
+                       float2 xpi, F&& f) {
   const uint32_t cx = cell1(xpi.x, g), cy = cell1(xpi.y, g);
   // the nine cell ranges are loaded up front (independent loads), then walked in order
   uint32_t cs[9], ce[9];
@@ -202,28 +224,35 @@ BDEV void scan_pairs(const BoxGeo& g, const float2* __restrict__ XP, const uint3
   for (int o = 0; o < 9; o++) {
     for (uint32_t j = cs[o]; j < ce[o]; j++) {
       if (j == i) continue;
-      if (len2(rel<W>(xpi, XP[j], g)) > g.h2) continue;
-      f(j);
+      const float2 d = rel<W>(xpi, XP[j], g);
+      const float r2 = len2(d);
+      if (r2 > g.h2) continue;
+      f(j, d, r2);
     }
   }
+}
+template <bool W = true, class F>
+BDEV void scan_pairs(const BoxGeo& g, const float2* __restrict__ XP, const uint32_t* __restrict__ start, uint32_t i,
+                     float2 xpi, F&& f) {
+  scan_pairs_d<W>(g, XP, start, i, xpi, [&](uint32_t j, float2, float) __attribute__((always_inline)) { f(j); });
 }
 template <class F>
 BDEV void for_pairs(const BoxGeo& g, const PairView& pv, uint32_t i, F&& f) {
   const uint32_t c = pv.cnt[i];
   if (c <= BX_KMAX) {
     // four table entries at a time: their loads (and the gathers they feed) are issued together
-    const uint32_t* __restrict__ col = pv.nb + i;
+    const int16_t* __restrict__ col = pv.nb + i;
     const size_t n = g.n;
     uint32_t k = 0;
     for (; k + 4u <= c; k += 4u) {
-      const uint32_t j0 = col[(size_t)k * n], j1 = col[(size_t)(k + 1u) * n];
-      const uint32_t j2 = col[(size_t)(k + 2u) * n], j3 = col[(size_t)(k + 3u) * n];
+      const uint32_t j0 = nb_dec(col[(size_t)k * n], i, g.n), j1 = nb_dec(col[(size_t)(k + 1u) * n], i, g.n);
+      const uint32_t j2 = nb_dec(col[(size_t)(k + 2u) * n], i, g.n), j3 = nb_dec(col[(size_t)(k + 3u) * n], i, g.n);
       f(j0);
       f(j1);
       f(j2);
       f(j3);
     }
-    for (; k < c; k++) f(col[(size_t)k * n]);
+    for (; k < c; k++) f(nb_dec(col[(size_t)k * n], i, g.n));
   } else {
     scan_pairs(g, pv.XP, pv.start, i, pv.XP[i], f);
   }
@@ -235,12 +264,12 @@ template <class L, class F>
 BDEV void for_pairs_g(const BoxGeo& g, const PairView& pv, uint32_t i, L&& load, F&& use) {
   const uint32_t c = pv.cnt[i];
   if (c <= BX_KMAX) {
-    const uint32_t* __restrict__ col = pv.nb + i;
+    const int16_t* __restrict__ col = pv.nb + i;
     const size_t n = g.n;
     uint32_t k = 0;
     for (; BX_BATCH && k + 4u <= c; k += 4u) {
-      const uint32_t j0 = col[(size_t)k * n], j1 = col[(size_t)(k + 1u) * n];
-      const uint32_t j2 = col[(size_t)(k + 2u) * n], j3 = col[(size_t)(k + 3u) * n];
+      const uint32_t j0 = nb_dec(col[(size_t)k * n], i, g.n), j1 = nb_dec(col[(size_t)(k + 1u) * n], i, g.n);
+      const uint32_t j2 = nb_dec(col[(size_t)(k + 2u) * n], i, g.n), j3 = nb_dec(col[(size_t)(k + 3u) * n], i, g.n);
       const auto a0 = load(j0);
       const auto a1 = load(j1);
       const auto a2 = load(j2);
@@ -251,7 +280,7 @@ BDEV void for_pairs_g(const BoxGeo& g, const PairView& pv, uint32_t i, L&& load,
       use(j3, a3);
     }
     for (; k < c; k++) {
-      const uint32_t j = col[(size_t)k * n];
+      const uint32_t j = nb_dec(col[(size_t)k * n], i, g.n);
       use(j, load(j));
     }
   } else {
@@ -420,45 +449,64 @@ BXK bx_gather(BoxGeo g, const uint32_t* __restrict__ slot, const float2* __restr
   idb[q] = id[i];
 }
 
-BXK bx_pairs(BoxGeo g, const float2* __restrict__ XP, const uint32_t* __restrict__ start, uint32_t* __restrict__ nb,
+// Records the pair table of particle i while f(j, d, r2) sees every pair.
+template <bool W, class F>
+BDEV void table_row(const BoxGeo& g, const float2* __restrict__ XP, const uint32_t* __restrict__ start,
+                    int16_t* __restrict__ nb, uint32_t* __restrict__ cnt, uint32_t i, float2 xpi, F&& f) {
+  uint32_t c = 0;
+  bool fits = true;
+  scan_pairs_d<W>(g, XP, start, i, xpi, [&](uint32_t j, float2 d, float r2) __attribute__((always_inline)) {
+    int16_t e = 0;
+    fits = nb_enc(j, i, g.n, e) && fits;
+    if (c < BX_KMAX) nb[(size_t)c * g.n + i] = e;
+    c++;
+    f(j, d, r2);
+  });
+  cnt[i] = fits ? c : (c > BX_KMAX ? c : BX_KMAX + 1u);
+}
+BXK bx_pairs(BoxGeo g, const float2* __restrict__ XP, const uint32_t* __restrict__ start, int16_t* __restrict__ nb,
               uint32_t* __restrict__ cnt) {
   if (*g.halt) return;
   const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
   if (i >= g.n) return;
-  uint32_t c = 0;
-  auto put = [&](uint32_t j) __attribute__((always_inline)) {
-    if (c < BX_KMAX) nb[(size_t)c * g.n + i] = j;
-    c++;
-  };
+  auto none = [](uint32_t, float2, float) __attribute__((always_inline)) {};
   const float2 xpi = XP[i];
-  if (interior(xpi, g)) scan_pairs<false>(g, XP, start, i, xpi, put);
-  else scan_pairs<true>(g, XP, start, i, xpi, put);
-  cnt[i] = c;
+  if (interior(xpi, g)) table_row<false>(g, XP, start, nb, cnt, i, xpi, none);
+  else table_row<true>(g, XP, start, nb, cnt, i, xpi, none);
 }
 
 // ---------------------------------------------------------------------------------------------
 // density passes (pbf.rs:1257-1334)
 // ---------------------------------------------------------------------------------------------
+// Density pass A for one particle: rho, C and lambda from the pair vectors d (r2 = |d|^2).
+struct LamAcc {
+  float rho;
+  float2 gs;
+  float sgs;
+};
+BDEV void lambda_term(const BoxGeo& g, float2 d, float r2, LamAcc& a) {
+  a.rho += g.mass * poly6(r2, g);
+  const float r = sqrtf(r2);
+  const float2 gr = mul2(spiky(d, r, g), g.m_over_rest);
+  a.gs = add2(a.gs, gr);
+  a.sgs += len2(gr);
+}
+BDEV float lambda_of(const BoxGeo& g, LamAcc a) {
+  const float c = fmaxf(a.rho / g.rest - 1.0f, 0.0f);
+  a.sgs += len2(a.gs);
+  return -c / (a.sgs + 200.0f);
+}
 template <bool W>
 BDEV void lambda_row(const BoxGeo& g, const PairView& pv, uint32_t i, const float2* __restrict__ X,
                      float* __restrict__ LAM) {
   const float2 xi = X[i];
-  float rho = g.w0;
-  float2 gs = f2(0.0f, 0.0f);
-  float sgs = 0.0f;
+  LamAcc a{g.w0, f2(0.0f, 0.0f), 0.0f};
   for_pairs_g(g, pv, i, [&](uint32_t j) __attribute__((always_inline)) { return X[j]; },
               [&](uint32_t, float2 xj) __attribute__((always_inline)) {
     const float2 d = rel<W>(xi, xj, g);
-    const float r2 = len2(d);
-    rho += g.mass * poly6(r2, g);
-    const float r = sqrtf(r2);
-    const float2 gr = mul2(spiky(d, r, g), g.m_over_rest);
-    gs = add2(gs, gr);
-    sgs += len2(gr);
+    lambda_term(g, d, len2(d), a);
   });
-  const float c = fmaxf(rho / g.rest - 1.0f, 0.0f);
-  sgs += len2(gs);
-  LAM[i] = -c / (sgs + 200.0f);
+  LAM[i] = lambda_of(g, a);
 }
 BXK bx_lambda(BoxGeo g, PairView pv, const float2* __restrict__ X, float* __restrict__ LAM) {
   if (*g.halt) return;
@@ -466,6 +514,21 @@ BXK bx_lambda(BoxGeo g, PairView pv, const float2* __restrict__ X, float* __rest
   if (i >= g.n) return;
   if (interior(pv.XP[i], g)) lambda_row<false>(g, pv, i, X, LAM);
   else lambda_row<true>(g, pv, i, X, LAM);
+}
+// First density pass A of a step, fused with the pair-table build: right after the reorder the
+// positions X equal the predicted ones XP, so the membership test's pair vectors are the ones
+// pass A needs (same values, same order as bx_pairs followed by bx_lambda).
+BXK bx_pairs_lambda(BoxGeo g, const float2* __restrict__ XP, const uint32_t* __restrict__ start,
+                    int16_t* __restrict__ nb, uint32_t* __restrict__ cnt, float* __restrict__ LAM) {
+  if (*g.halt) return;
+  const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
+  if (i >= g.n) return;
+  LamAcc a{g.w0, f2(0.0f, 0.0f), 0.0f};
+  auto acc = [&](uint32_t, float2 d, float r2) __attribute__((always_inline)) { lambda_term(g, d, r2, a); };
+  const float2 xpi = XP[i];
+  if (interior(xpi, g)) table_row<false>(g, XP, start, nb, cnt, i, xpi, acc);
+  else table_row<true>(g, XP, start, nb, cnt, i, xpi, acc);
+  LAM[i] = lambda_of(g, a);
 }
 template <bool W>
 BDEV void dp_row(const BoxGeo& g, const PairView& pv, uint32_t i, const float2* __restrict__ X,
@@ -572,7 +635,7 @@ BDEV float2 lap_row_kept(const BoxGeo& g, uint32_t i, const PairView& pv, const 
   float2 s = f2(0.0f, 0.0f);
   for (uint32_t k = 0; k < c; k++) {
     const size_t q = (size_t)k * g.n + i;
-    const uint32_t j = pv.nb[q];
+    const uint32_t j = nb_dec(pv.nb[q], i, g.n);
     const float w = CW[q];
     const float2 e = CE[q];
     s = add2(s, smul2(4.0f * w * dot2(e, sub2(ini, in[j])), e));
@@ -658,7 +721,8 @@ BXKR bx_cg_dir(BoxGeo g, PairView pv, const float2* __restrict__ X, const float*
       uint32_t k = 0;
       for (; k + 4u <= c; k += 4u) {
         const size_t q0 = (size_t)k * n + i, q1 = q0 + n, q2 = q1 + n, q3 = q2 + n;
-        const uint32_t j0 = pv.nb[q0], j1 = pv.nb[q1], j2 = pv.nb[q2], j3 = pv.nb[q3];
+        const uint32_t j0 = nb_dec(pv.nb[q0], i, g.n), j1 = nb_dec(pv.nb[q1], i, g.n);
+        const uint32_t j2 = nb_dec(pv.nb[q2], i, g.n), j3 = nb_dec(pv.nb[q3], i, g.n);
         const float2 p0 = pdir(j0), p1 = pdir(j1), p2 = pdir(j2), p3 = pdir(j3);
         term(q0, p0);
         term(q1, p1);
@@ -667,7 +731,7 @@ BXKR bx_cg_dir(BoxGeo g, PairView pv, const float2* __restrict__ X, const float*
       }
       for (; k < c; k++) {
         const size_t q = (size_t)k * n + i;
-        term(q, pdir(pv.nb[q]));
+        term(q, pdir(nb_dec(pv.nb[q], i, g.n)));
       }
     } else {
       const float2 xi = X[i];
@@ -747,18 +811,8 @@ BXKR bx_cg_check(BoxGeo g, const double* __restrict__ part_rr_last, BoxScal* __r
 }
 
 // ---------------------------------------------------------------------------------------------
-// multi-step windows: state checkpoints and the per-step CG verdict
+// multi-step windows: the per-step CG verdict and the stop word
 // ---------------------------------------------------------------------------------------------
-// Copies the step state (x, v, id): live -> checkpoint slot, or checkpoint slot -> live.
-BXK bx_checkpoint(BoxGeo g, const float2* __restrict__ X, const float2* __restrict__ V, const uint32_t* __restrict__ id,
-                  float2* __restrict__ cx, float2* __restrict__ cv, uint32_t* __restrict__ cid) {
-  if (*g.halt) return;
-  const uint32_t i = blockIdx.x * BX_BLOCK + threadIdx.x;
-  if (i >= g.n) return;
-  cx[i] = X[i];
-  cv[i] = V[i];
-  cid[i] = id[i];
-}
 BXK bx_verdict_reset(uint32_t* __restrict__ verdict, uint32_t* __restrict__ halt) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     verdict[0] = 0xFFFFFFFFu;
@@ -796,11 +850,13 @@ struct mg_box {
   uint32_t ncell = 0, nscan = 0;
   float2 *X = nullptr, *XP = nullptr, *X0 = nullptr, *V = nullptr, *X2 = nullptr;
   float2 *Xb = nullptr, *X0b = nullptr, *Vb = nullptr;
+  float2* VN = nullptr;  // the step's new velocity (velocity update + CG); swapped with V at step end
   uint32_t *id = nullptr, *idb = nullptr, *key = nullptr, *slot = nullptr, *start = nullptr, *cursor = nullptr,
            *bsum = nullptr;
   float *LAM = nullptr, *DENS = nullptr;
   float2 *R = nullptr, *P = nullptr, *AP = nullptr;
-  uint32_t *pnb = nullptr, *pcnt = nullptr;  // per-step pair table
+  int16_t* pnb = nullptr;    // per-step pair table (16-bit relative offsets)
+  uint32_t* pcnt = nullptr;  // and the per-particle pair counts
   float* cw = nullptr;                         // per-step Morris weights of tabled pairs
   float2* ce = nullptr;                        // and their unit directions
   double* part_b = nullptr;    // |b|^2 partials (one per reduction block)
@@ -816,11 +872,11 @@ struct mg_box {
   bool have_neighbors = false;
   uint32_t last_it = 0;  // CG iterations of the previous step (sizes the first chunk)
   uint64_t total_it = 0;  // CG iterations summed over every step run
-  // multi-step windows: state checkpoints (slot s = start of step s of the window), the device
-  // verdict [first overrun step, iterations of step 0..W-1] and its pinned host copy
+  // multi-step windows: the device verdict [first overrun step, iterations of step 0..W-1] and its
+  // pinned host copy; where each step's start state lives (see run_window)
   bool cg_stable = false;  // the last window's steps all took the same CG iteration count
-  float2 *ck_x = nullptr, *ck_v = nullptr;
-  uint32_t* ck_id = nullptr;
+  struct Start { float2* x; float2* v; uint32_t* id; };
+  Start starts[BX_WINDOW];
   uint32_t* verdict = nullptr;
   uint32_t* h_verdict = nullptr;
   uint32_t* halt = nullptr;  // window stop word (BoxGeo::halt)
@@ -967,7 +1023,9 @@ void collect_times(mg_box* B) {
   B->pend.clear();
 }
 
-// Enqueues one step up to and including the CG set-up; returns false on a launch error.
+// Enqueues one step up to and including the CG set-up; returns false on a launch error. When it
+// returns, X0, V and id hold the step's start state (positions, velocities, ids) in the step's cell
+// order: nothing later in the step writes them (the new velocity goes to VN).
 bool enqueue_front(mg_box* B) {
   const BoxGeo& g = B->g;
   const uint32_t cg_off = (B->cfg.mu > 0.0f && B->cfg.max_cg > 0) ? 0u : 1u;
@@ -992,12 +1050,15 @@ bool enqueue_front(mg_box* B) {
   std::swap(B->V, B->Vb);
   std::swap(B->id, B->idb);
   const PairView pv{B->XP, B->start, B->pnb, B->pcnt};
-  {
+  if (B->cfg.passes == 0) {
     Tic t(B, BXT_GRID);
     hipLaunchKernelGGL(bx_pairs, gp, blk, 0, st, g, B->XP, B->start, B->pnb, B->pcnt);
   }
   for (uint32_t p = 0; p < B->cfg.passes; p++) {
-    {
+    if (p == 0) {  // the first pass A also builds the pair table
+      Tic t(B, BXT_PAIRLAM);
+      hipLaunchKernelGGL(bx_pairs_lambda, gp, blk, 0, st, g, B->XP, B->start, B->pnb, B->pcnt, B->LAM);
+    } else {
       Tic t(B, BXT_LAMBDA);
       hipLaunchKernelGGL(bx_lambda, gp, blk, 0, st, g, pv, B->X, B->LAM);
     }
@@ -1010,11 +1071,11 @@ bool enqueue_front(mg_box* B) {
   const dim3 rblk(BX_RBLOCK), gr(g.nred);
   {
     Tic t(B, BXT_VEL);
-    hipLaunchKernelGGL(bx_vel_density, gr, rblk, 0, st, g, pv, B->X, B->X0, B->V, B->DENS, B->part_b);
+    hipLaunchKernelGGL(bx_vel_density, gr, rblk, 0, st, g, pv, B->X, B->X0, B->VN, B->DENS, B->part_b);
   }
   if (!cg_off) {
     Tic t(B, BXT_CGINIT);
-    hipLaunchKernelGGL(bx_cg_init, gr, rblk, 0, st, g, pv, B->X, B->DENS, B->V, B->R, B->P, B->cw, B->ce,
+    hipLaunchKernelGGL(bx_cg_init, gr, rblk, 0, st, g, pv, B->X, B->DENS, B->VN, B->R, B->P, B->cw, B->ce,
                        B->part_rr, B->sc);
   }
   B->cg_t = 1;
@@ -1046,7 +1107,7 @@ void enqueue_cg_chunk(mg_box* B, uint32_t iters, uint32_t* verdict = nullptr, ui
     }
     {
       Tic tt(B, BXT_UPDATE);
-      hipLaunchKernelGGL(bx_cg_upd, gr, rblk, 0, st, g, B->V, B->R, pnew, B->AP, B->part_pap, rr_prev, rr_out, B->sc, t);
+      hipLaunchKernelGGL(bx_cg_upd, gr, rblk, 0, st, g, B->VN, B->R, pnew, B->AP, B->part_pap, rr_prev, rr_out, B->sc, t);
     }
   }
   const uint32_t last = B->cg_t - 1u;
@@ -1075,29 +1136,40 @@ int run_step(mg_box* B, uint32_t first) {
     launched += k;
     if (hipGetLastError() != hipSuccess) { B->err = "kernel launch failed"; return MG_E_DEVICE; }
   }
+  std::swap(B->V, B->VN);
   B->last_it = B->h_sc->it;
   B->total_it += B->h_sc->it;
   return MG_OK;
 }
 
+// Makes *names[0] point at `keep`, which one of the other names holds; they swap.
+template <class T, size_t N>
+void take_buffer(T* keep, T** const (&names)[N]) {
+  for (size_t k = 1; k < N; k++)
+    if (*names[k] == keep) {
+      std::swap(*names[0], *names[k]);
+      return;
+    }
+}
+
 // Runs `steps` steps with one host wait per window of up to BX_WINDOW steps. Each step's CG is
 // launched with the largest iteration count of the previous window plus a margin while the counts
 // are changing (launches after convergence return at once). If step b of a window had not converged
-// when its iterations ran out, it sets the stop word (the rest of the window does no work), the
-// state checkpointed at the start of step b is restored and step b is run with host-driven chunks
+// when its iterations ran out, it sets the stop word: the rest of the window does no work. Step b's
+// start state is still in the buffers that held it during step b (its X0, V and id, in its cell
+// order: a step's result does not depend on the order of its input, which the cell sort removes),
+// so those buffers become the live state again and step b is re-run with host-driven chunks
 // (run_step); the next window is then half as long. The results are those of running every step
 // with run_step: iterations past convergence change nothing.
 int run_window(mg_box* B, uint32_t steps) {
-  const BoxGeo& g = B->g;
   const bool cg_on = B->cfg.mu > 0.0f && B->cfg.max_cg > 0;
-  const dim3 gp = blocks_for(g.n), blk(BX_BLOCK);
   uint32_t done = 0;
   while (done < steps) {
     const uint32_t W = std::min<uint32_t>(steps - done, cg_on ? B->win : BX_WINDOW);
     const uint32_t m = std::max<uint32_t>(B->last_it, 1u);
     const uint32_t spare = B->cg_stable ? 0u : BX_SPARE_IT + m / 8u;
     const uint32_t pred = std::min<uint32_t>(m + spare, B->cfg.max_cg);
-    if (cg_on && W == 1) {  // a single step: no checkpoint, chunks driven from the host
+    if (cg_on && W == 1) {  // a single step: chunks driven from the host
       const int rc = run_step(B, pred);
       if (rc) return rc;
       B->win = B->h_sc->it <= pred ? 2u : 1u;
@@ -1107,13 +1179,10 @@ int run_window(mg_box* B, uint32_t steps) {
     }
     if (cg_on) hipLaunchKernelGGL(bx_verdict_reset, dim3(1), dim3(64), 0, B->stream, B->verdict, B->halt);
     for (uint32_t s = 0; s < W; s++) {
-      if (cg_on) {
-        const size_t o = (size_t)s * g.n;
-        hipLaunchKernelGGL(bx_checkpoint, gp, blk, 0, B->stream, g, B->X, B->V, B->id, B->ck_x + o, B->ck_v + o,
-                           B->ck_id + o);
-      }
       if (!enqueue_front(B)) { B->err = "kernel launch failed"; return MG_E_DEVICE; }
+      B->starts[s] = {B->X0, B->V, B->id};
       if (cg_on) enqueue_cg_chunk(B, pred, B->verdict, s);
+      std::swap(B->V, B->VN);
     }
     if (cg_on) hipMemcpyAsync(B->h_verdict, B->verdict, (W + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, B->stream);
     if (hipStreamSynchronize(B->stream) != hipSuccess) { B->err = "HIP stream error"; return MG_E_DEVICE; }
@@ -1140,10 +1209,14 @@ int run_window(mg_box* B, uint32_t steps) {
       continue;
     }
     B->win = std::max<uint32_t>(B->win / 2u, 1u);
-    const size_t o = (size_t)bad * g.n;
+    const mg_box::Start& st = B->starts[bad];
+    float2** const pos[] = {&B->X, &B->Xb, &B->X0, &B->X0b, &B->X2};
+    float2** const vel[] = {&B->V, &B->Vb, &B->VN};
+    uint32_t** const ids[] = {&B->id, &B->idb};
+    take_buffer(st.x, pos);
+    take_buffer(st.v, vel);
+    take_buffer(st.id, ids);
     hipLaunchKernelGGL(bx_verdict_reset, dim3(1), dim3(64), 0, B->stream, B->verdict, B->halt);
-    hipLaunchKernelGGL(bx_checkpoint, gp, blk, 0, B->stream, g, B->ck_x + o, B->ck_v + o, B->ck_id + o, B->X, B->V,
-                       B->id);
     const int rc = run_step(B, pred + BX_CG_CHUNK);
     if (rc) return rc;
     done += 1;
@@ -1180,6 +1253,7 @@ int mg_box_create(const mg_box_config* cfg, mg_box** out, char* err, size_t errl
   size_t off = 0;
   off = carve(off, B->X, n); off = carve(off, B->XP, n); off = carve(off, B->X0, n); off = carve(off, B->V, n);
   off = carve(off, B->X2, n); off = carve(off, B->Xb, n); off = carve(off, B->X0b, n); off = carve(off, B->Vb, n);
+  off = carve(off, B->VN, n);
   off = carve(off, B->id, n); off = carve(off, B->idb, n); off = carve(off, B->key, n); off = carve(off, B->slot, n);
   off = carve(off, B->start, (size_t)B->ncell + 1); off = carve(off, B->cursor, (size_t)B->ncell);
   off = carve(off, B->bsum, (size_t)B->nscan);
@@ -1189,15 +1263,14 @@ int mg_box_create(const mg_box_config* cfg, mg_box** out, char* err, size_t errl
   off = carve(off, B->part_pap, (size_t)B->g.nred); off = carve(off, B->P2, n);
   off = carve(off, B->pnb, (size_t)BX_KMAX * n); off = carve(off, B->pcnt, n);
   off = carve(off, B->cw, (size_t)BX_KMAX * n); off = carve(off, B->ce, (size_t)BX_KMAX * n);
-  off = carve(off, B->ck_x, (size_t)BX_WINDOW * n); off = carve(off, B->ck_v, (size_t)BX_WINDOW * n);
-  off = carve(off, B->ck_id, (size_t)BX_WINDOW * n); off = carve(off, B->verdict, BX_WINDOW + 1);
+  off = carve(off, B->verdict, BX_WINDOW + 1);
   off = carve(off, B->halt, 1); off = carve(off, B->sc, 1);
   if (hipMalloc(&B->arena, off) != hipSuccess) { box_set_err(err, errlen, "hipMalloc failed"); delete B; return MG_E_DEVICE; }
   const uintptr_t base = (uintptr_t)B->arena;
   auto fix = [&](auto*& p) { p = reinterpret_cast<std::remove_reference_t<decltype(p)>>(base + (uintptr_t)p); };
-  fix(B->X); fix(B->XP); fix(B->X0); fix(B->V); fix(B->X2); fix(B->Xb); fix(B->X0b); fix(B->Vb);
+  fix(B->X); fix(B->XP); fix(B->X0); fix(B->V); fix(B->X2); fix(B->Xb); fix(B->X0b); fix(B->Vb); fix(B->VN);
   fix(B->id); fix(B->idb); fix(B->key); fix(B->slot); fix(B->start); fix(B->cursor); fix(B->bsum);
-  fix(B->LAM); fix(B->DENS); fix(B->R); fix(B->P); fix(B->AP); fix(B->part_b); fix(B->part_rr); fix(B->part_pap); fix(B->P2); fix(B->sc); fix(B->pnb); fix(B->pcnt); fix(B->cw); fix(B->ce); fix(B->ck_x); fix(B->ck_v); fix(B->ck_id); fix(B->verdict); fix(B->halt);
+  fix(B->LAM); fix(B->DENS); fix(B->R); fix(B->P); fix(B->AP); fix(B->part_b); fix(B->part_rr); fix(B->part_pap); fix(B->P2); fix(B->sc); fix(B->pnb); fix(B->pcnt); fix(B->cw); fix(B->ce); fix(B->verdict); fix(B->halt);
   B->g.halt = B->halt;
   bool ok = hipMemset(B->arena, 0, off) == hipSuccess;
   ok = ok && hipHostMalloc((void**)&B->h_sc, sizeof(BoxScal), 0) == hipSuccess;
