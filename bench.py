@@ -39,8 +39,8 @@ HBM_PEAK_GBS = 8000.0
 CLASS_NAMES = ["grid+reorder", "density_A", "density_B", "velocity+density", "cg_setup", "cg_dir_apply",
                "cg_update", "pairs+density_A1"]
 CLASS_BYTES = [138.0, 12.0, 20.0, 28.0, 52.0, 44.0, 40.0, 12.0]
-CLASS_KERNELS = [None, "bx_lambda", "bx_dp", "bx_vel_density", "bx_cg_init", "bx_cg_dir", "bx_cg_upd",
-                 "bx_pairs_lambda"]
+CLASS_KERNELS = [None, "bx_lambda<float>", "bx_dp<float>", "bx_vel_density<float>", "bx_cg_init<float>",
+                 "bx_cg_dir<float>", "bx_cg_upd<float>", "bx_pairs_lambda<float>"]
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "box_1024_pmc_traffic.md")
 C2_PARAMS = {"mill": {"diameter_m": 0.1397, "speed_mode": "percent_critical", "speed_value": 75.0},
              "media": {"density_kg_m3": 7800.0},
@@ -96,6 +96,29 @@ def cpu_baseline_box(cfg: dict, seconds: float) -> dict:
                       f"1 core, {dt:.1f} s"}
 
 
+def bench_fp64(args, cfg) -> dict:
+    """The same workload with FP64 state and arithmetic: K steps in one call after the warm-up."""
+    import torch
+    from milldynamics3_amd.box import Box
+    box = Box(dict(cfg, fp64=1))
+    n = box.n
+    box.step(args.warmup)
+    torch.cuda.synchronize()
+    cg0 = box.total_cg_iterations
+    t0 = time.perf_counter()
+    box.step(args.steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    avg_cg = (box.total_cg_iterations - cg0) / args.steps
+    box.close()
+    ms = 1000.0 * wall / args.steps
+    # FP64 canonical bytes per particle: 434 + 64 n_d + 168 n_cg (every stored value twice as wide)
+    gbs = n * (434.0 + 64.0 * 4 + 168.0 * avg_cg) / (ms * 1e-3) / 1e9
+    return {"metric": "particle-steps/s", "value": round(n * args.steps / wall, 1), "ms_per_step": round(ms, 4),
+            "hbm_gbs_algorithmic": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+            "cg_iterations_per_step": avg_cg, "dtype": "f64 state and arithmetic"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,6 +128,7 @@ def main() -> int:
     ap.add_argument("--mu", type=float, default=1.0e-3)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-state box measurement")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one plain device copy of this size (calibrates PMC byte counters)")
     args = ap.parse_args()
@@ -178,6 +202,8 @@ def main() -> int:
            "config": {"workload": f"periodic PBF box {args.side}x{args.side}", "particles": n, "mu": args.mu,
                       "dtype": "f32 state, f64 CG reductions"},
            "kernels_probe": kernels}
+    if not args.no_fp64:
+        iso["fp64"] = bench_fp64(args, cfg)
     headline_cpu = None
     if not args.no_cpu:
         headline_cpu = cpu_baseline_drum(args.cpu_seconds)

@@ -146,6 +146,8 @@ typedef struct mg_box_config {
   float tol;       /* the CG stops at tol * |b| */
   uint32_t max_cg; /* CG iteration cap */
   uint64_t seed;   /* Rng seed of the initial state (rng.rs) */
+  uint32_t fp64;   /* 0: FP32 state with FP64 CG reductions; 1: FP64 state and arithmetic
+                      (same f32 initial state and geometry, promoted) */
 } mg_box_config;
 
 /* h, L, cell width, particle mass, cells per side. */
@@ -156,11 +158,15 @@ size_t mg_box_count(const mg_box* box);
 /* Runs `steps` steps and waits for them; mg_box_last_step_ms is their wall time on the stream. */
 int mg_box_step(mg_box* box, uint32_t steps);
 float mg_box_last_step_ms(const mg_box* box);
-/* Positions and velocities in the original particle order (2n floats each; either may be null). */
+/* Positions and velocities in the original particle order (2n floats each; either may be null).
+ * An FP64 box rounds them to f32 here; the _f64 forms copy either precision exactly. */
 int mg_box_copy_state(mg_box* box, float* x2n, float* v2n);
 int mg_box_set_state(mg_box* box, const float* x2n, const float* v2n);
+int mg_box_copy_state_f64(mg_box* box, double* x2n, double* v2n);
+int mg_box_set_state_f64(mg_box* box, const double* x2n, const double* v2n);
 /* Density at the final positions of the last step, original order. */
 int mg_box_density(mg_box* box, float* out_n);
+int mg_box_density_f64(mg_box* box, double* out_n);
 /* CG iterations, final residual norm and |b| of the last step. */
 int mg_box_stats(mg_box* box, double* out3);
 /* CG iterations summed over every step this box has run. */

@@ -15,13 +15,13 @@ import numpy as np
 from . import sim as _sim
 
 BOX_DEFAULTS = {"side": 1024, "dx": 1.0e-3, "rest": 1000.0, "mu": 1.0e-3, "dt": 1.0 / 1440.0,
-                "passes": 4, "tol": 1.0e-6, "max_cg": 500, "seed": 1}
+                "passes": 4, "tol": 1.0e-6, "max_cg": 500, "seed": 1, "fp64": 0}
 
 
 class BoxConfig(C.Structure):
     _fields_ = [("side", C.c_uint32), ("dx", C.c_float), ("rest", C.c_float), ("mu", C.c_float),
                 ("dt", C.c_float), ("passes", C.c_uint32), ("tol", C.c_float), ("max_cg", C.c_uint32),
-                ("seed", C.c_uint64)]
+                ("seed", C.c_uint64), ("fp64", C.c_uint32)]
 
 
 def box_config(overrides=None) -> dict:
@@ -33,10 +33,14 @@ def box_config(overrides=None) -> dict:
 def _cfg_struct(cfg) -> BoxConfig:
     c = box_config(cfg)
     return BoxConfig(int(c["side"]), float(c["dx"]), float(c["rest"]), float(c["mu"]), float(c["dt"]),
-                     int(c["passes"]), float(c["tol"]), int(c["max_cg"]), int(c["seed"]))
+                     int(c["passes"]), float(c["tol"]), int(c["max_cg"]), int(c["seed"]), int(c["fp64"]))
 
 
 _ready = set()
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
 def _lib():
@@ -59,6 +63,9 @@ def _lib():
     L.mg_box_copy_state.argtypes = [P, fp, fp]
     L.mg_box_set_state.argtypes = [P, fp, fp]
     L.mg_box_density.argtypes = [P, fp]
+    L.mg_box_copy_state_f64.argtypes = [P, dp, dp]
+    L.mg_box_set_state_f64.argtypes = [P, dp, dp]
+    L.mg_box_density_f64.argtypes = [P, dp]
     L.mg_box_stats.argtypes = [P, dp]
     L.mg_box_neighbors.argtypes = [P, u32p, u32p, C.c_int64]
     L.mg_box_neighbors.restype = C.c_int64
@@ -79,7 +86,8 @@ def box_geometry(cfg=None) -> dict:
 
 
 class Box:
-    """``side**2`` PBF particles in a periodic box, resident on the GPU."""
+    """``side**2`` PBF particles in a periodic box, resident on the GPU (``fp64=1``: FP64 state;
+    its state and density come back as float64, exactly)."""
 
     def __init__(self, cfg=None):
         self.cfg = box_config(cfg)
@@ -89,6 +97,7 @@ class Box:
         s = _cfg_struct(self.cfg)
         _sim._check(self._L.mg_box_create(C.byref(s), C.byref(self._h), err, 512), err, "mg_box_create")
         self.n = int(self._L.mg_box_count(self._h))
+        self.fp64 = bool(self.cfg["fp64"])
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
@@ -126,21 +135,34 @@ class Box:
         return ms, n
 
     def state(self):
-        x = np.zeros(2 * self.n, dtype=np.float32)
-        v = np.zeros(2 * self.n, dtype=np.float32)
-        _sim._check(self._L.mg_box_copy_state(self._h, _sim._fp(x), _sim._fp(v)), what="copy_state")
+        if self.fp64:
+            x = np.zeros(2 * self.n)
+            v = np.zeros(2 * self.n)
+            _sim._check(self._L.mg_box_copy_state_f64(self._h, _dp(x), _dp(v)), what="copy_state_f64")
+        else:
+            x = np.zeros(2 * self.n, dtype=np.float32)
+            v = np.zeros(2 * self.n, dtype=np.float32)
+            _sim._check(self._L.mg_box_copy_state(self._h, _sim._fp(x), _sim._fp(v)), what="copy_state")
         return x.reshape(self.n, 2), v.reshape(self.n, 2)
 
     def set_state(self, x, v):
-        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1)
-        v = np.ascontiguousarray(v, dtype=np.float32).reshape(-1)
+        dt = np.float64 if self.fp64 else np.float32
+        x = np.ascontiguousarray(x, dtype=dt).reshape(-1)
+        v = np.ascontiguousarray(v, dtype=dt).reshape(-1)
         if x.size != 2 * self.n or v.size != 2 * self.n:
-            raise ValueError("state arrays must hold 2*n floats")
-        _sim._check(self._L.mg_box_set_state(self._h, _sim._fp(x), _sim._fp(v)), what="set_state")
+            raise ValueError("state arrays must hold 2*n values")
+        if self.fp64:
+            _sim._check(self._L.mg_box_set_state_f64(self._h, _dp(x), _dp(v)), what="set_state_f64")
+        else:
+            _sim._check(self._L.mg_box_set_state(self._h, _sim._fp(x), _sim._fp(v)), what="set_state")
 
     def density(self) -> np.ndarray:
-        d = np.zeros(self.n, dtype=np.float32)
-        _sim._check(self._L.mg_box_density(self._h, _sim._fp(d)), what="density")
+        if self.fp64:
+            d = np.zeros(self.n)
+            _sim._check(self._L.mg_box_density_f64(self._h, _dp(d)), what="density_f64")
+        else:
+            d = np.zeros(self.n, dtype=np.float32)
+            _sim._check(self._L.mg_box_density(self._h, _sim._fp(d)), what="density")
         return d
 
     def stats(self) -> dict:

@@ -8,8 +8,9 @@
 //                         stable counting sort: cell order, ascending particle id inside a
 //                         cell, and the particle arrays physically reordered by cell so every
 //                         neighbour gather below reads contiguous memory
-//   bx_pairs              the step's pair table: 3x3-cell candidates with |d|^2 <= h^2 at the
-//                         predicted positions (the reference's neighbour lists, pbf.rs:262-288)
+//   bx_pairs_lambda       the step's pair table: 3x3-cell candidates with |d|^2 <= h^2 at the
+//                         predicted positions (the reference's neighbour lists, pbf.rs:262-288),
+//                         built inside the first density pass A (bx_pairs when there is none)
 //   bx_lambda, bx_dp      density constraint passes (rho, C, lambda; dp, x)  pbf.rs:1257-1334
 //   bx_vel_density        v = (x - x0)/dt, density at the final positions    pbf.rs:1656-1674, 2136-2144
 //   bx_cg_init            r = b - A v with the Morris weights kept for the step   pbf.rs:312-407
@@ -19,6 +20,10 @@
 // neighbours therefore rounds exactly as the reference's does. CG dot products and norms are FP64
 // partial sums per workgroup (fixed tree), summed in block order by every consumer workgroup, so
 // the scalars are identical everywhere and results are run-to-run deterministic.
+//
+// Precision: the state kernels are templates on the real type R. R = float is FP32 state with FP64
+// CG reductions (the reference's f32 arithmetic, bit for bit); R = double is FP64 state and
+// arithmetic from the same f32 initial state and geometry, promoted (the oracle's f64 policy).
 //
 // CDNA4 mapping: 256-thread workgroups (4 wave64s; 1024 for the kernels producing FP64 partials),
 // one particle per lane; the blockIdx -> tile map gives each of the 8 XCDs one contiguous run of
@@ -43,12 +48,9 @@
 #define BX_CG_CHUNK 4
 #define BX_WINDOW 16  // steps enqueued between host waits in a multi-step call
 #define BX_SPARE_IT 1  // CG iterations launched beyond the predicted count in a window
-// Loop-shape switches (both forms give identical results; the defaults are the measured faster ones).
+// Loop-shape switch (both forms give identical results; the default is the measured faster one).
 #ifndef BX_BATCH
 #define BX_BATCH 1     // issue the gathers of four table entries before using them
-#endif
-#ifndef BX_INTERIOR
-#define BX_INTERIOR 0  // skip the minimum image for particles whose 3x3 cells do not wrap
 #endif
 #ifndef BX_KMAX
 #define BX_KMAX 32  // pairs kept per particle in the per-step pair table (more: the kernel rescans)
@@ -59,16 +61,27 @@
 // kernel classes of the optional per-kernel timing (mg_box_kernel_times)
 #define BX_NCLASS 8
 enum { BXT_GRID = 0, BXT_LAMBDA, BXT_DP, BXT_VEL, BXT_CGINIT, BXT_SPMV, BXT_UPDATE, BXT_PAIRLAM };
+// precision-independent kernels keep C names; the state kernels are templates on the real type R
+// (float: FP32 state with FP64 reductions; double: FP64 throughout)
 #define BXK extern "C" __global__ void __launch_bounds__(BX_BLOCK)
 #define BXKR extern "C" __global__ void __launch_bounds__(BX_RBLOCK)
+#define BXKT template <class R> __global__ void __launch_bounds__(BX_BLOCK)
+#define BXKRT template <class R> __global__ void __launch_bounds__(BX_RBLOCK)
 #define BDEV __device__ __attribute__((always_inline)) inline
 
-// Geometry and constants of one box, passed by value to every kernel. `halt` is the window stop
-// word: once a step of a multi-step window has run out of CG iterations before converging, the
-// window's remaining kernels return at once (the host restores that step's checkpoint).
-struct BoxGeo {
+template <class R> struct VecT;
+template <> struct VecT<float> { using type = float2; };
+template <> struct VecT<double> { using type = double2; };
+template <class R> using Vec = typename VecT<R>::type;
+
+// Geometry and constants of one box, passed by value to every kernel, in the state's precision
+// (the f64 constants are the f32 geometry promoted, then the derived ones formed in f64, as the
+// oracle's f64 policy forms them). `halt` is the window stop word: once a step of a multi-step
+// window has run out of CG iterations before converging, the window's remaining kernels return at
+// once (the host then makes that step's start state live again).
+template <class R> struct BoxGeo {
   uint32_t n, ncx, nblk, xcd_chunk, nred, xcd_chunk_red;
-  float h, h2, L, halfL, cw, mass, rest, dt, k6, ks, w0, m_over_rest, mu2m, eta2;
+  R h, h2, L, halfL, cw, mass, rest, dt, k6, ks, w0, m_over_rest, mu2m, eta2;
   const uint32_t* halt;
 };
 
@@ -80,64 +93,69 @@ struct BoxScal {
 };
 
 // ---------------------------------------------------------------------------------------------
-// device helpers (f32, no contraction: compiled with -ffp-contract=off)
+// device helpers (no contraction: compiled with -ffp-contract=off; IEEE division and sqrt)
 // ---------------------------------------------------------------------------------------------
-BDEV float2 f2(float x, float y) { return make_float2(x, y); }
-BDEV float2 add2(float2 a, float2 b) { return f2(a.x + b.x, a.y + b.y); }
-BDEV float2 sub2(float2 a, float2 b) { return f2(a.x - b.x, a.y - b.y); }
-BDEV float2 mul2(float2 a, float s) { return f2(a.x * s, a.y * s); }  // Vec2 * f32
-BDEV float2 smul2(float s, float2 a) { return f2(s * a.x, s * a.y); } // f32 * Vec2
-BDEV float2 div2(float2 a, float s) { return f2(a.x / s, a.y / s); }
-BDEV float dot2(float2 a, float2 b) { return a.x * b.x + a.y * b.y; }
-BDEV float len2(float2 a) { return a.x * a.x + a.y * a.y; }
-BDEV float wrapf(float a, float L) {  // f32::rem_euclid
-  float r = fmodf(a, L);
-  return r < 0.0f ? r + fabsf(L) : r;
+BDEV float msqrt(float a) { return sqrtf(a); }
+BDEV double msqrt(double a) { return sqrt(a); }
+BDEV float mfmod(float a, float b) { return fmodf(a, b); }
+BDEV double mfmod(double a, double b) { return fmod(a, b); }
+BDEV float mfloor(float a) { return floorf(a); }
+BDEV double mfloor(double a) { return floor(a); }
+BDEV float mfabs(float a) { return fabsf(a); }
+BDEV double mfabs(double a) { return fabs(a); }
+BDEV float mfmax(float a, float b) { return fmaxf(a, b); }  // f32::max (maxNum)
+BDEV double mfmax(double a, double b) { return fmax(a, b); }
+// glam Vec2 operations, component-wise in glam's order
+#define BX_VEC_OPS(V, S, MK)                                                   \
+  BDEV V mk(S x, S y) { return MK(x, y); }                                     \
+  BDEV V add2(V a, V b) { return MK(a.x + b.x, a.y + b.y); }                   \
+  BDEV V sub2(V a, V b) { return MK(a.x - b.x, a.y - b.y); }                   \
+  BDEV V mul2(V a, S s) { return MK(a.x * s, a.y * s); } /* Vec2 * s */        \
+  BDEV V smul2(S s, V a) { return MK(s * a.x, s * a.y); } /* s * Vec2 */       \
+  BDEV V div2(V a, S s) { return MK(a.x / s, a.y / s); }                       \
+  BDEV S dot2(V a, V b) { return a.x * b.x + a.y * b.y; }                      \
+  BDEV S len2(V a) { return a.x * a.x + a.y * a.y; }
+BX_VEC_OPS(float2, float, make_float2)
+BX_VEC_OPS(double2, double, make_double2)
+#undef BX_VEC_OPS
+template <class R> BDEV R wrapR(R a, R L) {  // rem_euclid
+  R r = mfmod(a, L);
+  return r < (R)0 ? r + mfabs(L) : r;
 }
-BDEV float mi1(float d, const BoxGeo& g) {  // minimum image
+template <class R> BDEV R mi1(R d, const BoxGeo<R>& g) {  // minimum image
   if (d > g.halfL) return d - g.L;
   if (d < -g.halfL) return d + g.L;
   return d;
 }
-BDEV float2 mi2(float2 d, const BoxGeo& g) { return f2(mi1(d.x, g), mi1(d.y, g)); }
-// Pair vector a - b. W = false for a particle whose 3x3 cells do not wrap (interior): every pair
-// vector of such a particle is shorter than L/2 in each component, where the minimum image is the
-// identity, so both forms give the same bits.
-template <bool W>
-BDEV float2 rel(float2 a, float2 b, const BoxGeo& g) {
-  return W ? mi2(sub2(a, b), g) : sub2(a, b);
+// Pair vector a - b with the minimum image.
+template <class R> BDEV Vec<R> rel(Vec<R> a, Vec<R> b, const BoxGeo<R>& g) {
+  const Vec<R> d = sub2(a, b);
+  return mk(mi1(d.x, g), mi1(d.y, g));
 }
-BDEV uint32_t cell1(float a, const BoxGeo& g) {
-  float q = floorf(a / g.cw);
-  if (!(q >= 0.0f)) return 0u;
-  if (q >= (float)(g.ncx - 1u)) return g.ncx - 1u;
+template <class R> BDEV uint32_t cell1(R a, const BoxGeo<R>& g) {
+  R q = mfloor(a / g.cw);
+  if (!(q >= (R)0)) return 0u;
+  if (q >= (R)(g.ncx - 1u)) return g.ncx - 1u;
   return (uint32_t)q;
 }
-// True when the 3x3 cells around the particle's (predicted-position) cell do not wrap; needs at
-// least 4 cells per side so that two cells span at most L/2.
-BDEV bool interior(float2 xp, const BoxGeo& g) {
-  if (!BX_INTERIOR || g.ncx < 4u) return false;
-  const uint32_t cx = cell1(xp.x, g), cy = cell1(xp.y, g);
-  return cx >= 1u && cx + 2u <= g.ncx && cy >= 1u && cy + 2u <= g.ncx;
-}
-BDEV float poly6(float r2, const BoxGeo& g) {  // pbf.rs:119-126
-  if (r2 >= g.h2 || r2 < 0.0f) return 0.0f;
-  float t = g.h2 - r2;
+template <class R> BDEV R poly6(R r2, const BoxGeo<R>& g) {  // pbf.rs:119-126
+  if (r2 >= g.h2 || r2 < (R)0) return (R)0;
+  R t = g.h2 - r2;
   return g.k6 * t * t * t;
 }
-BDEV float2 spiky(float2 d, float r, const BoxGeo& g) {  // pbf.rs:131-137
-  if (r <= 1e-9f || r >= g.h) return f2(0.0f, 0.0f);
-  float coeff = g.ks * (g.h - r) * (g.h - r);
+template <class R> BDEV Vec<R> spiky(Vec<R> d, R r, const BoxGeo<R>& g) {  // pbf.rs:131-137
+  if (r <= (R)1e-9f || r >= g.h) return mk((R)0, (R)0);
+  R coeff = g.ks * (g.h - r) * (g.h - r);
   return mul2(div2(d, r), coeff);
 }
 // XCD-aware tile index: hardware dispatches consecutive workgroups round-robin over the 8 XCDs;
 // remap so XCD x processes tiles [x*chunk, (x+1)*chunk).
-BDEV uint32_t tile_id(const BoxGeo& g) {
+template <class R> BDEV uint32_t tile_id(const BoxGeo<R>& g) {
   const uint32_t b = blockIdx.x;
   if (g.xcd_chunk == 0u) return b;  // tile count not a multiple of 8: identity map
   return (b & 7u) * g.xcd_chunk + (b >> 3);
 }
-BDEV uint32_t tile_id_red(const BoxGeo& g) {
+template <class R> BDEV uint32_t tile_id_red(const BoxGeo<R>& g) {
   const uint32_t b = blockIdx.x;
   if (g.xcd_chunk_red == 0u) return b;
   return (b & 7u) * g.xcd_chunk_red + (b >> 3);
@@ -186,8 +204,8 @@ BDEV double sum_parts(const double* __restrict__ part, uint32_t nb) {
 // side of the periodic seam). cnt[i] is the true count; a particle with more than BX_KMAX pairs, or
 // with a neighbour whose offset does not fit 16 bits, has cnt[i] > BX_KMAX and is handled by
 // rescanning its 3x3 cells (same order, same membership test).
-struct PairView {
-  const float2* __restrict__ XP;
+template <class R> struct PairView {
+  const Vec<R>* __restrict__ XP;
   const uint32_t* __restrict__ start;
   const int16_t* __restrict__ nb;
   const uint32_t* __restrict__ cnt;
@@ -208,9 +226,9 @@ BDEV bool nb_enc(uint32_t j, uint32_t i, uint32_t n, int16_t& e) {  // false: do
   return true;
 }
 // f(j, d, r2) for every pair, with d = xpi - XP[j] (minimum image) and r2 = |d|^2 as the test used.
-template <bool W = true, class F>
-BDEV void scan_pairs_d(const BoxGeo& g, const float2* __restrict__ XP, const uint32_t* __restrict__ start, uint32_t i,
-                       float2 xpi, F&& f) {
+template <class R, class F>
+BDEV void scan_pairs_d(const BoxGeo<R>& g, const Vec<R>* __restrict__ XP, const uint32_t* __restrict__ start,
+                       uint32_t i, Vec<R> xpi, F&& f) {
   const uint32_t cx = cell1(xpi.x, g), cy = cell1(xpi.y, g);
   // the nine cell ranges are loaded up front (independent loads), then walked in order
   uint32_t cs[9], ce[9];
@@ -224,20 +242,20 @@ BDEV void scan_pairs_d(const BoxGeo& g, const float2* __restrict__ XP, const uin
   for (int o = 0; o < 9; o++) {
     for (uint32_t j = cs[o]; j < ce[o]; j++) {
       if (j == i) continue;
-      const float2 d = rel<W>(xpi, XP[j], g);
-      const float r2 = len2(d);
+      const Vec<R> d = rel(xpi, XP[j], g);
+      const R r2 = len2(d);
       if (r2 > g.h2) continue;
       f(j, d, r2);
     }
   }
 }
-template <bool W = true, class F>
-BDEV void scan_pairs(const BoxGeo& g, const float2* __restrict__ XP, const uint32_t* __restrict__ start, uint32_t i,
-                     float2 xpi, F&& f) {
-  scan_pairs_d<W>(g, XP, start, i, xpi, [&](uint32_t j, float2, float) __attribute__((always_inline)) { f(j); });
+template <class R, class F>
+BDEV void scan_pairs(const BoxGeo<R>& g, const Vec<R>* __restrict__ XP, const uint32_t* __restrict__ start,
+                     uint32_t i, Vec<R> xpi, F&& f) {
+  scan_pairs_d(g, XP, start, i, xpi, [&](uint32_t j, Vec<R>, R) __attribute__((always_inline)) { f(j); });
 }
-template <class F>
-BDEV void for_pairs(const BoxGeo& g, const PairView& pv, uint32_t i, F&& f) {
+template <class R, class F>
+BDEV void for_pairs(const BoxGeo<R>& g, const PairView<R>& pv, uint32_t i, F&& f) {
   const uint32_t c = pv.cnt[i];
   if (c <= BX_KMAX) {
     // four table entries at a time: their loads (and the gathers they feed) are issued together
@@ -260,8 +278,8 @@ BDEV void for_pairs(const BoxGeo& g, const PairView& pv, uint32_t i, F&& f) {
 // As for_pairs, with the per-neighbour gathers split out: load(j) returns what use(j, v) needs,
 // and the loads of four table entries are all issued before the first use (memory-level
 // parallelism; without it every neighbour costs two dependent round trips).
-template <class L, class F>
-BDEV void for_pairs_g(const BoxGeo& g, const PairView& pv, uint32_t i, L&& load, F&& use) {
+template <class R, class L, class F>
+BDEV void for_pairs_g(const BoxGeo<R>& g, const PairView<R>& pv, uint32_t i, L&& load, F&& use) {
   const uint32_t c = pv.cnt[i];
   if (c <= BX_KMAX) {
     const int16_t* __restrict__ col = pv.nb + i;
@@ -315,15 +333,16 @@ BDEV uint32_t wave_run(uint32_t key, uint32_t& head) {
   return next - (uint32_t)lane;
 }
 
-BXK bx_predict(BoxGeo g, float2* __restrict__ X, const float2* __restrict__ V, float2* __restrict__ X0,
-               uint32_t* __restrict__ key, uint32_t* __restrict__ count) {
+BXKT bx_predict(BoxGeo<R> g, Vec<R>* __restrict__ X, const Vec<R>* __restrict__ V, Vec<R>* __restrict__ X0,
+                uint32_t* __restrict__ key, uint32_t* __restrict__ count) {
   if (*g.halt) return;
   const uint32_t i = blockIdx.x * BX_BLOCK + threadIdx.x;
   uint32_t k = 0xFFFFFFFFu;
   if (i < g.n) {
-    float2 x = X[i], v = V[i];
+    Vec<R> x = X[i];
+    const Vec<R> v = V[i];
     X0[i] = x;
-    x = f2(wrapf(x.x + v.x * g.dt, g.L), wrapf(x.y + v.y * g.dt, g.L));
+    x = mk(wrapR(x.x + v.x * g.dt, g.L), wrapR(x.y + v.y * g.dt, g.L));
     X[i] = x;
     k = cell1(x.y, g) * g.ncx + cell1(x.x, g);
     key[i] = k;
@@ -393,17 +412,17 @@ BXK bx_scan_apply(const uint32_t* __restrict__ halt, uint32_t* __restrict__ a, u
       off += t;
     }
 }
-BXK bx_scatter(BoxGeo g, const uint32_t* __restrict__ key, const uint32_t* __restrict__ start,
-               uint32_t* __restrict__ cursor, uint32_t* __restrict__ slot) {
-  if (*g.halt) return;
+BXK bx_scatter(const uint32_t* __restrict__ halt, uint32_t n, const uint32_t* __restrict__ key,
+               const uint32_t* __restrict__ start, uint32_t* __restrict__ cursor, uint32_t* __restrict__ slot) {
+  if (*halt) return;
   const uint32_t i = blockIdx.x * BX_BLOCK + threadIdx.x;
-  const uint32_t k = i < g.n ? key[i] : 0xFFFFFFFFu;
+  const uint32_t k = i < n ? key[i] : 0xFFFFFFFFu;
   uint32_t head;
   const uint32_t len = wave_run(k, head);
   uint32_t base = 0;
   if (len && k != 0xFFFFFFFFu) base = start[k] + atomicAdd(&cursor[k], len);
   base = __shfl(base, (int)head, 64);
-  if (i < g.n) slot[base + ((threadIdx.x & 63u) - head)] = i;
+  if (i < n) slot[base + ((threadIdx.x & 63u) - head)] = i;
 }
 // One lane per cell: order the cell's slots by particle id (a cell holds a handful of particles).
 // Lane 0 of the grid also resets this step's CG scalars (nothing reads them before the CG).
@@ -434,14 +453,15 @@ BXK bx_cellsort(const uint32_t* __restrict__ halt, uint32_t ncell, const uint32_
 }
 // Moves every particle to its sorted position (one lane per destination: coalesced writes); x is
 // also written to the predicted copy XP.
-BXK bx_gather(BoxGeo g, const uint32_t* __restrict__ slot, const float2* __restrict__ X, const float2* __restrict__ X0,
-              const float2* __restrict__ V, const uint32_t* __restrict__ id, float2* __restrict__ Xb,
-              float2* __restrict__ XPb, float2* __restrict__ X0b, float2* __restrict__ Vb, uint32_t* __restrict__ idb) {
+BXKT bx_gather(BoxGeo<R> g, const uint32_t* __restrict__ slot, const Vec<R>* __restrict__ X,
+               const Vec<R>* __restrict__ X0, const Vec<R>* __restrict__ V, const uint32_t* __restrict__ id,
+               Vec<R>* __restrict__ Xb, Vec<R>* __restrict__ XPb, Vec<R>* __restrict__ X0b, Vec<R>* __restrict__ Vb,
+               uint32_t* __restrict__ idb) {
   if (*g.halt) return;
   const uint32_t q = blockIdx.x * BX_BLOCK + threadIdx.x;
   if (q >= g.n) return;
   const uint32_t i = slot[q];
-  const float2 x = X[i];
+  const Vec<R> x = X[i];
   Xb[q] = x;
   XPb[q] = x;
   X0b[q] = X0[i];
@@ -450,12 +470,12 @@ BXK bx_gather(BoxGeo g, const uint32_t* __restrict__ slot, const float2* __restr
 }
 
 // Records the pair table of particle i while f(j, d, r2) sees every pair.
-template <bool W, class F>
-BDEV void table_row(const BoxGeo& g, const float2* __restrict__ XP, const uint32_t* __restrict__ start,
-                    int16_t* __restrict__ nb, uint32_t* __restrict__ cnt, uint32_t i, float2 xpi, F&& f) {
+template <class R, class F>
+BDEV void table_row(const BoxGeo<R>& g, const Vec<R>* __restrict__ XP, const uint32_t* __restrict__ start,
+                    int16_t* __restrict__ nb, uint32_t* __restrict__ cnt, uint32_t i, Vec<R> xpi, F&& f) {
   uint32_t c = 0;
   bool fits = true;
-  scan_pairs_d<W>(g, XP, start, i, xpi, [&](uint32_t j, float2 d, float r2) __attribute__((always_inline)) {
+  scan_pairs_d(g, XP, start, i, xpi, [&](uint32_t j, Vec<R> d, R r2) __attribute__((always_inline)) {
     int16_t e = 0;
     fits = nb_enc(j, i, g.n, e) && fits;
     if (c < BX_KMAX) nb[(size_t)c * g.n + i] = e;
@@ -464,121 +484,98 @@ BDEV void table_row(const BoxGeo& g, const float2* __restrict__ XP, const uint32
   });
   cnt[i] = fits ? c : (c > BX_KMAX ? c : BX_KMAX + 1u);
 }
-BXK bx_pairs(BoxGeo g, const float2* __restrict__ XP, const uint32_t* __restrict__ start, int16_t* __restrict__ nb,
-              uint32_t* __restrict__ cnt) {
+BXKT bx_pairs(BoxGeo<R> g, const Vec<R>* __restrict__ XP, const uint32_t* __restrict__ start,
+              int16_t* __restrict__ nb, uint32_t* __restrict__ cnt) {
   if (*g.halt) return;
   const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
   if (i >= g.n) return;
-  auto none = [](uint32_t, float2, float) __attribute__((always_inline)) {};
-  const float2 xpi = XP[i];
-  if (interior(xpi, g)) table_row<false>(g, XP, start, nb, cnt, i, xpi, none);
-  else table_row<true>(g, XP, start, nb, cnt, i, xpi, none);
+  table_row(g, XP, start, nb, cnt, i, XP[i], [](uint32_t, Vec<R>, R) __attribute__((always_inline)) {});
 }
 
 // ---------------------------------------------------------------------------------------------
 // density passes (pbf.rs:1257-1334)
 // ---------------------------------------------------------------------------------------------
 // Density pass A for one particle: rho, C and lambda from the pair vectors d (r2 = |d|^2).
-struct LamAcc {
-  float rho;
-  float2 gs;
-  float sgs;
+template <class R> struct LamAcc {
+  R rho;
+  Vec<R> gs;
+  R sgs;
 };
-BDEV void lambda_term(const BoxGeo& g, float2 d, float r2, LamAcc& a) {
+template <class R> BDEV void lambda_term(const BoxGeo<R>& g, Vec<R> d, R r2, LamAcc<R>& a) {
   a.rho += g.mass * poly6(r2, g);
-  const float r = sqrtf(r2);
-  const float2 gr = mul2(spiky(d, r, g), g.m_over_rest);
+  const R r = msqrt(r2);
+  const Vec<R> gr = mul2(spiky(d, r, g), g.m_over_rest);
   a.gs = add2(a.gs, gr);
   a.sgs += len2(gr);
 }
-BDEV float lambda_of(const BoxGeo& g, LamAcc a) {
-  const float c = fmaxf(a.rho / g.rest - 1.0f, 0.0f);
+template <class R> BDEV R lambda_of(const BoxGeo<R>& g, LamAcc<R> a) {
+  const R c = mfmax(a.rho / g.rest - (R)1, (R)0);
   a.sgs += len2(a.gs);
-  return -c / (a.sgs + 200.0f);
+  return -c / (a.sgs + (R)200);
 }
-template <bool W>
-BDEV void lambda_row(const BoxGeo& g, const PairView& pv, uint32_t i, const float2* __restrict__ X,
-                     float* __restrict__ LAM) {
-  const float2 xi = X[i];
-  LamAcc a{g.w0, f2(0.0f, 0.0f), 0.0f};
+BXKT bx_lambda(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, R* __restrict__ LAM) {
+  if (*g.halt) return;
+  const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
+  if (i >= g.n) return;
+  const Vec<R> xi = X[i];
+  LamAcc<R> a{g.w0, mk((R)0, (R)0), (R)0};
   for_pairs_g(g, pv, i, [&](uint32_t j) __attribute__((always_inline)) { return X[j]; },
-              [&](uint32_t, float2 xj) __attribute__((always_inline)) {
-    const float2 d = rel<W>(xi, xj, g);
+              [&](uint32_t, Vec<R> xj) __attribute__((always_inline)) {
+    const Vec<R> d = rel(xi, xj, g);
     lambda_term(g, d, len2(d), a);
   });
   LAM[i] = lambda_of(g, a);
 }
-BXK bx_lambda(BoxGeo g, PairView pv, const float2* __restrict__ X, float* __restrict__ LAM) {
-  if (*g.halt) return;
-  const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
-  if (i >= g.n) return;
-  if (interior(pv.XP[i], g)) lambda_row<false>(g, pv, i, X, LAM);
-  else lambda_row<true>(g, pv, i, X, LAM);
-}
 // First density pass A of a step, fused with the pair-table build: right after the reorder the
 // positions X equal the predicted ones XP, so the membership test's pair vectors are the ones
 // pass A needs (same values, same order as bx_pairs followed by bx_lambda).
-BXK bx_pairs_lambda(BoxGeo g, const float2* __restrict__ XP, const uint32_t* __restrict__ start,
-                    int16_t* __restrict__ nb, uint32_t* __restrict__ cnt, float* __restrict__ LAM) {
+BXKT bx_pairs_lambda(BoxGeo<R> g, const Vec<R>* __restrict__ XP, const uint32_t* __restrict__ start,
+                     int16_t* __restrict__ nb, uint32_t* __restrict__ cnt, R* __restrict__ LAM) {
   if (*g.halt) return;
   const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
   if (i >= g.n) return;
-  LamAcc a{g.w0, f2(0.0f, 0.0f), 0.0f};
-  auto acc = [&](uint32_t, float2 d, float r2) __attribute__((always_inline)) { lambda_term(g, d, r2, a); };
-  const float2 xpi = XP[i];
-  if (interior(xpi, g)) table_row<false>(g, XP, start, nb, cnt, i, xpi, acc);
-  else table_row<true>(g, XP, start, nb, cnt, i, xpi, acc);
+  LamAcc<R> a{g.w0, mk((R)0, (R)0), (R)0};
+  table_row(g, XP, start, nb, cnt, i, XP[i],
+            [&](uint32_t, Vec<R> d, R r2) __attribute__((always_inline)) { lambda_term(g, d, r2, a); });
   LAM[i] = lambda_of(g, a);
 }
-template <bool W>
-BDEV void dp_row(const BoxGeo& g, const PairView& pv, uint32_t i, const float2* __restrict__ X,
-                 const float* __restrict__ LAM, float2* __restrict__ Xout) {
-  const float2 xi = X[i];
-  const float li = LAM[i];
-  float2 s = f2(0.0f, 0.0f);
-  struct Nb { float2 x; float l; };
-  for_pairs_g(g, pv, i, [&](uint32_t j) __attribute__((always_inline)) { return Nb{X[j], LAM[j]}; },
-              [&](uint32_t, const Nb& a) __attribute__((always_inline)) {
-    const float2 d = rel<W>(xi, a.x, g);
-    const float r = sqrtf(len2(d));
-    s = add2(s, smul2(li + a.l, spiky(d, r, g)));
-  });
-  const float2 dp = mul2(s, g.m_over_rest);
-  Xout[i] = f2(wrapf(xi.x + dp.x, g.L), wrapf(xi.y + dp.y, g.L));
-}
-BXK bx_dp(BoxGeo g, PairView pv, const float2* __restrict__ X, const float* __restrict__ LAM,
-          float2* __restrict__ Xout) {
+BXKT bx_dp(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const R* __restrict__ LAM,
+           Vec<R>* __restrict__ Xout) {
   if (*g.halt) return;
   const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
   if (i >= g.n) return;
-  if (interior(pv.XP[i], g)) dp_row<false>(g, pv, i, X, LAM, Xout);
-  else dp_row<true>(g, pv, i, X, LAM, Xout);
+  const Vec<R> xi = X[i];
+  const R li = LAM[i];
+  Vec<R> s = mk((R)0, (R)0);
+  struct Nb { Vec<R> x; R l; };
+  for_pairs_g(g, pv, i, [&](uint32_t j) __attribute__((always_inline)) { return Nb{X[j], LAM[j]}; },
+              [&](uint32_t, const Nb& a) __attribute__((always_inline)) {
+    const Vec<R> d = rel(xi, a.x, g);
+    const R r = msqrt(len2(d));
+    s = add2(s, smul2(li + a.l, spiky(d, r, g)));
+  });
+  const Vec<R> dp = mul2(s, g.m_over_rest);
+  Xout[i] = mk(wrapR(xi.x + dp.x, g.L), wrapR(xi.y + dp.y, g.L));
 }
 
 // ---------------------------------------------------------------------------------------------
 // velocity, density recompute, implicit viscosity
 // ---------------------------------------------------------------------------------------------
-BXKR bx_vel_density(BoxGeo g, PairView pv, const float2* __restrict__ X, const float2* __restrict__ X0,
-                    float2* __restrict__ V, float* __restrict__ DENS, double* __restrict__ part_b) {
+BXKRT bx_vel_density(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const Vec<R>* __restrict__ X0,
+                     Vec<R>* __restrict__ V, R* __restrict__ DENS, double* __restrict__ part_b) {
   if (*g.halt) return;
   const uint32_t i = tile_id_red(g) * BX_RBLOCK + threadIdx.x;
   double vv = 0.0;
   if (i < g.n) {
-    const float2 xi = X[i];
-    const float2 v = div2(mi2(sub2(xi, X0[i]), g), g.dt);
+    const Vec<R> xi = X[i];
+    const Vec<R> v = div2(rel(xi, X0[i], g), g.dt);
     V[i] = v;
     vv = (double)v.x * (double)v.x + (double)v.y * (double)v.y;
-    float rho = g.w0;
-    auto load = [&](uint32_t j) __attribute__((always_inline)) { return X[j]; };
-    if (interior(pv.XP[i], g)) {
-      for_pairs_g(g, pv, i, load, [&](uint32_t, float2 xj) __attribute__((always_inline)) {
-        rho += g.mass * poly6(len2(rel<false>(xi, xj, g)), g);
-      });
-    } else {
-      for_pairs_g(g, pv, i, load, [&](uint32_t, float2 xj) __attribute__((always_inline)) {
-        rho += g.mass * poly6(len2(rel<true>(xi, xj, g)), g);
-      });
-    }
+    R rho = g.w0;
+    for_pairs_g(g, pv, i, [&](uint32_t j) __attribute__((always_inline)) { return X[j]; },
+                [&](uint32_t, Vec<R> xj) __attribute__((always_inline)) {
+      rho += g.mass * poly6(len2(rel(xi, xj, g)), g);
+    });
     DENS[i] = rho;
   }
   const double t = block_sum<BX_RBLOCK>(vv);
@@ -588,74 +585,56 @@ BXKR bx_vel_density(BoxGeo g, PairView pv, const float2* __restrict__ X, const f
 // Morris weight and unit direction of pair (i, j) (pbf.rs:312-344).
 // The quotient d / r is formed once: the kernel gradient (d / r) * coeff and the direction d / r
 // use the same rounded value, exactly as two separate divisions would give.
-template <bool W = true>
-BDEV void morris(const BoxGeo& g, float2 xi, float rho_i, float2 xj, float dens_j, float& w, float2& e) {
-  const float rho_j = fmaxf(dens_j, 1e-6f);
-  const float2 d = rel<W>(xi, xj, g);
-  const float r = sqrtf(len2(d));
-  const float2 dr = div2(d, r);
-  const float2 gr = (r <= 1e-9f || r >= g.h) ? f2(0.0f, 0.0f) : mul2(dr, g.ks * (g.h - r) * (g.h - r));
-  const float num = (g.mu2m / (rho_i * rho_j)) * fabsf(dot2(d, gr));
+template <class R>
+BDEV void morris(const BoxGeo<R>& g, Vec<R> xi, R rho_i, Vec<R> xj, R dens_j, R& w, Vec<R>& e) {
+  const R rho_j = mfmax(dens_j, (R)1e-6f);
+  const Vec<R> d = rel(xi, xj, g);
+  const R r = msqrt(len2(d));
+  const Vec<R> dr = div2(d, r);
+  const Vec<R> gr = (r <= (R)1e-9f || r >= g.h) ? mk((R)0, (R)0) : mul2(dr, g.ks * (g.h - r) * (g.h - r));
+  const R num = (g.mu2m / (rho_i * rho_j)) * mfabs(dot2(d, gr));
   w = num / (len2(d) + g.eta2);
-  e = r > 1e-9f ? dr : f2(1.0f, 0.0f);
+  e = r > (R)1e-9f ? dr : mk((R)1, (R)0);
 }
 // (L v)_i with the weights computed here; when `store` the weights of a tabled particle are kept
 // (column-major, like the pair table) for the operator applications of this step's CG.
-template <bool W = true>
-BDEV float2 lap_row(const BoxGeo& g, uint32_t i, const PairView& pv, const float2* __restrict__ X,
-                    const float* __restrict__ DENS, const float2* __restrict__ in, bool store,
-                    float* __restrict__ CW, float2* __restrict__ CE) {
-  const float2 xi = X[i], ini = in[i];
-  const float rho_i = fmaxf(DENS[i], 1e-6f);
+template <class R>
+BDEV Vec<R> lap_row(const BoxGeo<R>& g, uint32_t i, const PairView<R>& pv, const Vec<R>* __restrict__ X,
+                    const R* __restrict__ DENS, const Vec<R>* __restrict__ in, bool store, R* __restrict__ CW,
+                    Vec<R>* __restrict__ CE) {
+  const Vec<R> xi = X[i], ini = in[i];
+  const R rho_i = mfmax(DENS[i], (R)1e-6f);
   const bool keep = store && pv.cnt[i] <= BX_KMAX;
-  float2 s = f2(0.0f, 0.0f);
+  Vec<R> s = mk((R)0, (R)0);
   uint32_t k = 0;
-  struct Nb { float2 x; float d; float2 v; };
+  struct Nb { Vec<R> x; R d; Vec<R> v; };
   for_pairs_g(g, pv, i, [&](uint32_t j) __attribute__((always_inline)) { return Nb{X[j], DENS[j], in[j]}; },
               [&](uint32_t, const Nb& a) __attribute__((always_inline)) {
-    float w;
-    float2 e;
-    morris<W>(g, xi, rho_i, a.x, a.d, w, e);
+    R w;
+    Vec<R> e;
+    morris(g, xi, rho_i, a.x, a.d, w, e);
     if (keep) {
       CW[(size_t)k * g.n + i] = w;
       CE[(size_t)k * g.n + i] = e;
     }
     k++;
-    s = add2(s, smul2(4.0f * w * dot2(e, sub2(ini, a.v)), e));
+    s = add2(s, smul2((R)4 * w * dot2(e, sub2(ini, a.v)), e));
   });
   return s;
 }
-// (L v)_i from the kept weights (same values, same order as lap_row).
-BDEV float2 lap_row_kept(const BoxGeo& g, uint32_t i, const PairView& pv, const float2* __restrict__ X,
-                         const float* __restrict__ DENS, const float2* __restrict__ in,
-                         const float* __restrict__ CW, const float2* __restrict__ CE) {
-  const uint32_t c = pv.cnt[i];
-  if (c > BX_KMAX) return lap_row(g, i, pv, X, DENS, in, false, nullptr, nullptr);
-  const float2 ini = in[i];
-  float2 s = f2(0.0f, 0.0f);
-  for (uint32_t k = 0; k < c; k++) {
-    const size_t q = (size_t)k * g.n + i;
-    const uint32_t j = nb_dec(pv.nb[q], i, g.n);
-    const float w = CW[q];
-    const float2 e = CE[q];
-    s = add2(s, smul2(4.0f * w * dot2(e, sub2(ini, in[j])), e));
-  }
-  return s;
-}
 
-BXKR bx_cg_init(BoxGeo g, PairView pv, const float2* __restrict__ X, const float* __restrict__ DENS,
-                const float2* __restrict__ V, float2* __restrict__ R, float2* __restrict__ P,
-                float* __restrict__ CW, float2* __restrict__ CE, double* __restrict__ part_rr0,
-                const BoxScal* __restrict__ sc) {
+BXKRT bx_cg_init(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const R* __restrict__ DENS,
+                 const Vec<R>* __restrict__ V, Vec<R>* __restrict__ Rv, Vec<R>* __restrict__ P,
+                 R* __restrict__ CW, Vec<R>* __restrict__ CE, double* __restrict__ part_rr0,
+                 const BoxScal* __restrict__ sc) {
   if (*g.halt | sc->done) return;  // both words loaded at once
   const uint32_t i = tile_id_red(g) * BX_RBLOCK + threadIdx.x;
   double rr = 0.0;
   if (i < g.n) {
-    const float2 v = V[i];
-    const float2 lv = interior(pv.XP[i], g) ? lap_row<false>(g, i, pv, X, DENS, V, true, CW, CE)
-                                            : lap_row<true>(g, i, pv, X, DENS, V, true, CW, CE);
-    const float2 r = sub2(v, add2(v, mul2(lv, g.dt)));
-    R[i] = r;
+    const Vec<R> v = V[i];
+    const Vec<R> lv = lap_row(g, i, pv, X, DENS, V, true, CW, CE);
+    const Vec<R> r = sub2(v, add2(v, mul2(lv, g.dt)));
+    Rv[i] = r;
     P[i] = r;
     rr = (double)r.x * (double)r.x + (double)r.y * (double)r.y;
   }
@@ -668,16 +647,16 @@ BXKR bx_cg_init(BoxGeo g, PairView pv, const float2* __restrict__ X, const float
 // initial residual; for t > 1 the convergence test on r(t-1).r(t-1), the iteration cap, and
 // beta = rs(t-1)/rs(t-2). The direction p = r + beta p_old is formed on the fly for the particle
 // and its neighbours (no separate direction kernel), then Ap = p + dt L p with the kept weights.
-BXKR bx_cg_dir(BoxGeo g, PairView pv, const float2* __restrict__ X, const float* __restrict__ DENS,
-               const float* __restrict__ CW, const float2* __restrict__ CE, const float2* __restrict__ R,
-               const float2* __restrict__ Pold, float2* __restrict__ Pnew, float2* __restrict__ AP,
-               const double* __restrict__ part_b, const double* __restrict__ part_rr_prev,
-               const double* __restrict__ part_rr_prev2, double* __restrict__ part_pap, BoxScal* __restrict__ sc,
-               uint32_t t) {
+BXKRT bx_cg_dir(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const R* __restrict__ DENS,
+                const R* __restrict__ CW, const Vec<R>* __restrict__ CE, const Vec<R>* __restrict__ Rv,
+                const Vec<R>* __restrict__ Pold, Vec<R>* __restrict__ Pnew, Vec<R>* __restrict__ AP,
+                const double* __restrict__ part_b, const double* __restrict__ part_rr_prev,
+                const double* __restrict__ part_rr_prev2, double* __restrict__ part_pap, BoxScal* __restrict__ sc,
+                uint32_t t) {
   if (*g.halt | sc->done) return;  // both words loaded at once
   double target;
   bool stop = false;
-  float beta = 0.0f;
+  R beta = (R)0;
   if (t == 1u) {
     const double bn = sqrt(sum_parts<BX_RBLOCK>(part_b, g.nred));
     target = (double)sc->tol * bn;
@@ -693,7 +672,7 @@ BXKR bx_cg_dir(BoxGeo g, PairView pv, const float2* __restrict__ X, const float*
     const double rs1 = sum_parts<BX_RBLOCK>(part_rr_prev, g.nred);
     const double rs2 = sum_parts<BX_RBLOCK>(part_rr_prev2, g.nred);
     if (sqrt(rs1) <= target || t - 1u >= sc->max_cg) stop = true;
-    beta = (float)(rs1 / rs2);
+    beta = (R)(rs1 / rs2);
     if (blockIdx.x == 0 && threadIdx.x == 0) sc->res = sqrt(rs1);
   }
   if (stop) {
@@ -707,15 +686,15 @@ BXKR bx_cg_dir(BoxGeo g, PairView pv, const float2* __restrict__ X, const float*
   double pap = 0.0;
   if (i < g.n) {
     const bool first = t == 1u;
-    const float2 p = first ? Pold[i] : add2(R[i], mul2(Pold[i], beta));
-    float2 s = f2(0.0f, 0.0f);
+    const Vec<R> p = first ? Pold[i] : add2(Rv[i], mul2(Pold[i], beta));
+    Vec<R> s = mk((R)0, (R)0);
     const uint32_t c = pv.cnt[i];
     if (c <= BX_KMAX) {
       auto pdir = [&](uint32_t j) __attribute__((always_inline)) {
-        return first ? Pold[j] : add2(R[j], mul2(Pold[j], beta));
+        return first ? Pold[j] : add2(Rv[j], mul2(Pold[j], beta));
       };
-      auto term = [&](size_t q, float2 pj) __attribute__((always_inline)) {
-        s = add2(s, smul2(4.0f * CW[q] * dot2(CE[q], sub2(p, pj)), CE[q]));
+      auto term = [&](size_t q, Vec<R> pj) __attribute__((always_inline)) {
+        s = add2(s, smul2((R)4 * CW[q] * dot2(CE[q], sub2(p, pj)), CE[q]));
       };
       const size_t n = g.n;
       uint32_t k = 0;
@@ -723,7 +702,7 @@ BXKR bx_cg_dir(BoxGeo g, PairView pv, const float2* __restrict__ X, const float*
         const size_t q0 = (size_t)k * n + i, q1 = q0 + n, q2 = q1 + n, q3 = q2 + n;
         const uint32_t j0 = nb_dec(pv.nb[q0], i, g.n), j1 = nb_dec(pv.nb[q1], i, g.n);
         const uint32_t j2 = nb_dec(pv.nb[q2], i, g.n), j3 = nb_dec(pv.nb[q3], i, g.n);
-        const float2 p0 = pdir(j0), p1 = pdir(j1), p2 = pdir(j2), p3 = pdir(j3);
+        const Vec<R> p0 = pdir(j0), p1 = pdir(j1), p2 = pdir(j2), p3 = pdir(j3);
         term(q0, p0);
         term(q1, p1);
         term(q2, p2);
@@ -734,17 +713,17 @@ BXKR bx_cg_dir(BoxGeo g, PairView pv, const float2* __restrict__ X, const float*
         term(q, pdir(nb_dec(pv.nb[q], i, g.n)));
       }
     } else {
-      const float2 xi = X[i];
-      const float rho_i = fmaxf(DENS[i], 1e-6f);
+      const Vec<R> xi = X[i];
+      const R rho_i = mfmax(DENS[i], (R)1e-6f);
       scan_pairs(g, pv.XP, pv.start, i, pv.XP[i], [&](uint32_t j) __attribute__((always_inline)) {
-        const float2 pj = first ? Pold[j] : add2(R[j], mul2(Pold[j], beta));
-        float w;
-        float2 e;
+        const Vec<R> pj = first ? Pold[j] : add2(Rv[j], mul2(Pold[j], beta));
+        R w;
+        Vec<R> e;
         morris(g, xi, rho_i, X[j], DENS[j], w, e);
-        s = add2(s, smul2(4.0f * w * dot2(e, sub2(p, pj)), e));
+        s = add2(s, smul2((R)4 * w * dot2(e, sub2(p, pj)), e));
       });
     }
-    const float2 ap = add2(p, mul2(s, g.dt));
+    const Vec<R> ap = add2(p, mul2(s, g.dt));
     Pnew[i] = p;
     AP[i] = ap;
     pap = (double)p.x * (double)ap.x + (double)p.y * (double)ap.y;
@@ -756,10 +735,10 @@ BXKR bx_cg_dir(BoxGeo g, PairView pv, const float2* __restrict__ X, const float*
 
 // CG iteration t, second half: alpha = rs(t-1)/p.Ap (stop if p.Ap <= 0), v += alpha p,
 // r -= alpha Ap, partial sums of r.r for the next iteration's test.
-BXKR bx_cg_upd(BoxGeo g, float2* __restrict__ V, float2* __restrict__ R, const float2* __restrict__ P,
-               const float2* __restrict__ AP, const double* __restrict__ part_pap,
-               const double* __restrict__ part_rr_prev, double* __restrict__ part_rr_out,
-               BoxScal* __restrict__ sc, uint32_t t) {
+BXKRT bx_cg_upd(BoxGeo<R> g, Vec<R>* __restrict__ V, Vec<R>* __restrict__ Rv, const Vec<R>* __restrict__ P,
+                const Vec<R>* __restrict__ AP, const double* __restrict__ part_pap,
+                const double* __restrict__ part_rr_prev, double* __restrict__ part_rr_out,
+                BoxScal* __restrict__ sc, uint32_t t) {
   if (*g.halt | sc->done) return;  // both words loaded at once
   const double pap = sum_parts<BX_RBLOCK>(part_pap, g.nred);
   if (!(pap > 0.0) || !isfinite(pap)) {
@@ -770,13 +749,13 @@ BXKR bx_cg_upd(BoxGeo g, float2* __restrict__ V, float2* __restrict__ R, const f
     return;
   }
   const double rs_old = sum_parts<BX_RBLOCK>(part_rr_prev, g.nred);
-  const float a = (float)(rs_old / pap);
+  const R a = (R)(rs_old / pap);
   const uint32_t i = blockIdx.x * BX_RBLOCK + threadIdx.x;
   double rr = 0.0;
   if (i < g.n) {
     V[i] = add2(V[i], mul2(P[i], a));
-    const float2 r = sub2(R[i], mul2(AP[i], a));
-    R[i] = r;
+    const Vec<R> r = sub2(Rv[i], mul2(AP[i], a));
+    Rv[i] = r;
     rr = (double)r.x * (double)r.x + (double)r.y * (double)r.y;
   }
   const double tt = block_sum<BX_RBLOCK>(rr);
@@ -788,11 +767,11 @@ BXKR bx_cg_upd(BoxGeo g, float2* __restrict__ V, float2* __restrict__ R, const f
 // In a multi-step window it also records the step's verdict: verdict[1 + s] = iterations, and
 // verdict[0] = the first step s whose CG was still running when its launched iterations ran out;
 // that step also sets the stop word, so the rest of the window does no work.
-BXKR bx_cg_check(BoxGeo g, const double* __restrict__ part_rr_last, BoxScal* __restrict__ sc, uint32_t t,
+BXKR bx_cg_check(uint32_t nred, const double* __restrict__ part_rr_last, BoxScal* __restrict__ sc, uint32_t t,
                  uint32_t* __restrict__ verdict, uint32_t s, uint32_t* __restrict__ halt) {
-  if (*g.halt) return;
+  if (*halt) return;
   if (!sc->done) {
-    const double rs = sum_parts<BX_RBLOCK>(part_rr_last, g.nred);
+    const double rs = sum_parts<BX_RBLOCK>(part_rr_last, nred);
     if (threadIdx.x == 0) {
       sc->res = sqrt(rs);
       if (sqrt(rs) <= sc->target || t >= sc->max_cg) {
@@ -823,15 +802,15 @@ BXK bx_verdict_reset(uint32_t* __restrict__ verdict, uint32_t* __restrict__ halt
 // ---------------------------------------------------------------------------------------------
 // debug export: neighbour lists in the traversal order, rows and entries as original ids
 // ---------------------------------------------------------------------------------------------
-BXK bx_nbr_count(BoxGeo g, PairView pv, const uint32_t* __restrict__ id, uint32_t* __restrict__ cnt) {
+BXKT bx_nbr_count(BoxGeo<R> g, PairView<R> pv, const uint32_t* __restrict__ id, uint32_t* __restrict__ cnt) {
   const uint32_t i = blockIdx.x * BX_BLOCK + threadIdx.x;
   if (i >= g.n) return;
   uint32_t c = 0;
   for_pairs(g, pv, i, [&](uint32_t) __attribute__((always_inline)) { c++; });
   cnt[id[i]] = c;
 }
-BXK bx_nbr_write(BoxGeo g, PairView pv, const uint32_t* __restrict__ id, const uint32_t* __restrict__ off,
-                 uint32_t* __restrict__ nb, uint32_t cap) {
+BXKT bx_nbr_write(BoxGeo<R> g, PairView<R> pv, const uint32_t* __restrict__ id, const uint32_t* __restrict__ off,
+                  uint32_t* __restrict__ nb, uint32_t cap) {
   const uint32_t i = blockIdx.x * BX_BLOCK + threadIdx.x;
   if (i >= g.n) return;
   uint32_t w = off[id[i]];
@@ -846,23 +825,26 @@ BXK bx_nbr_write(BoxGeo g, PairView pv, const uint32_t* __restrict__ id, const u
 // ---------------------------------------------------------------------------------------------
 struct mg_box {
   mg_box_config cfg;
-  BoxGeo g;
-  uint32_t ncell = 0, nscan = 0;
-  float2 *X = nullptr, *XP = nullptr, *X0 = nullptr, *V = nullptr, *X2 = nullptr;
-  float2 *Xb = nullptr, *X0b = nullptr, *Vb = nullptr;
-  float2* VN = nullptr;  // the step's new velocity (velocity update + CG); swapped with V at step end
+  bool f64 = false;       // state precision (cfg.fp64)
+  BoxGeo<float> gf;       // geometry in f32 (the workload's definition)
+  BoxGeo<double> gd;      // and promoted to f64 for an FP64 box
+  uint32_t n = 0, ncell = 0, nscan = 0, nred = 0;
+  // state arrays in the box's precision: positions / velocities are Vec<R>, scalars R
+  void *X = nullptr, *XP = nullptr, *X0 = nullptr, *V = nullptr, *X2 = nullptr;
+  void *Xb = nullptr, *X0b = nullptr, *Vb = nullptr;
+  void* VN = nullptr;  // the step's new velocity (velocity update + CG); swapped with V at step end
   uint32_t *id = nullptr, *idb = nullptr, *key = nullptr, *slot = nullptr, *start = nullptr, *cursor = nullptr,
            *bsum = nullptr;
-  float *LAM = nullptr, *DENS = nullptr;
-  float2 *R = nullptr, *P = nullptr, *AP = nullptr;
+  void *LAM = nullptr, *DENS = nullptr;
+  void *R = nullptr, *P = nullptr, *AP = nullptr;
   int16_t* pnb = nullptr;    // per-step pair table (16-bit relative offsets)
   uint32_t* pcnt = nullptr;  // and the per-particle pair counts
-  float* cw = nullptr;                         // per-step Morris weights of tabled pairs
-  float2* ce = nullptr;                        // and their unit directions
+  void* cw = nullptr;        // per-step Morris weights of tabled pairs
+  void* ce = nullptr;        // and their unit directions
   double* part_b = nullptr;    // |b|^2 partials (one per reduction block)
   double* part_rr = nullptr;   // r.r partials, two generations: [0, nred) and [nred, 2 nred)
   double* part_pap = nullptr;  // p.Ap partials
-  float2* P2 = nullptr;        // second direction buffer (the CG alternates P and P2)
+  void* P2 = nullptr;          // second direction buffer (the CG alternates P and P2)
   uint32_t cg_t = 0;           // next CG iteration number of the current step
   BoxScal* sc = nullptr;
   BoxScal* h_sc = nullptr;  // pinned host mirror
@@ -875,7 +857,7 @@ struct mg_box {
   // multi-step windows: the device verdict [first overrun step, iterations of step 0..W-1] and its
   // pinned host copy; where each step's start state lives (see run_window)
   bool cg_stable = false;  // the last window's steps all took the same CG iteration count
-  struct Start { float2* x; float2* v; uint32_t* id; };
+  struct Start { void* x; void* v; uint32_t* id; };
   Start starts[BX_WINDOW];
   uint32_t* verdict = nullptr;
   uint32_t* h_verdict = nullptr;
@@ -894,6 +876,12 @@ struct mg_box {
 
 namespace {
 
+template <class R> const BoxGeo<R>& geo(const mg_box* B);
+template <> const BoxGeo<float>& geo<float>(const mg_box* B) { return B->gf; }
+template <> const BoxGeo<double>& geo<double>(const mg_box* B) { return B->gd; }
+template <class R> Vec<R>* vp(void* p) { return static_cast<Vec<R>*>(p); }
+template <class R> R* sp(void* p) { return static_cast<R*>(p); }
+
 void box_set_err(char* err, size_t errlen, const std::string& m) {
   if (err && errlen) {
     size_t n = std::min(errlen - 1, m.size());
@@ -903,12 +891,13 @@ void box_set_err(char* err, size_t errlen, const std::string& m) {
 }
 
 // Geometry of the workload (same f32 expressions as the seeding description in the header).
-bool box_geometry(const mg_box_config& c, BoxGeo& g, std::string& err) {
+bool box_geometry(const mg_box_config& c, BoxGeo<float>& g, std::string& err) {
   if (c.side < 8 || c.side > 8192) { err = "side must be in [8, 8192]"; return false; }
   if (!(c.dx > 0.0f) || !(c.rest > 0.0f) || !(c.dt > 0.0f) || !(c.mu >= 0.0f) || !(c.tol > 0.0f)) {
     err = "dx, rest, dt and tol must be positive and mu non-negative";
     return false;
   }
+  if (c.fp64 > 1u) { err = "fp64 must be 0 or 1"; return false; }
   g.n = c.side * c.side;
   g.h = 2.0f * c.dx;
   g.h2 = g.h * g.h;
@@ -948,13 +937,44 @@ bool box_geometry(const mg_box_config& c, BoxGeo& g, std::string& err) {
   return true;
 }
 
+// The FP64 constants: the f32 geometry promoted, the derived ones formed in f64 (pbf.rs:119-137
+// and :312-344 evaluated in f64 over the promoted inputs).
+BoxGeo<double> promote(const BoxGeo<float>& f, const mg_box_config& c) {
+  BoxGeo<double> g;
+  g.n = f.n; g.ncx = f.ncx; g.nblk = f.nblk; g.xcd_chunk = f.xcd_chunk; g.nred = f.nred;
+  g.xcd_chunk_red = f.xcd_chunk_red;
+  const double pi = 3.14159265358979323846;
+  g.h = (double)f.h;
+  g.h2 = g.h * g.h;
+  g.L = (double)f.L;
+  g.halfL = (double)f.halfL;
+  g.cw = (double)f.cw;
+  double a = g.h * g.h;  // powi(h, 8) and powi(h, 5) by binary exponentiation, low bit first
+  a = a * a;
+  g.k6 = 4.0 / (pi * (a * a));
+  g.ks = -30.0 / (pi * (g.h * a));
+  g.mass = (double)f.mass;
+  g.rest = (double)c.rest;
+  g.dt = (double)c.dt;
+  {
+    double t = g.h * g.h - 0.0;
+    g.w0 = g.mass * (g.k6 * t * t * t);
+  }
+  g.m_over_rest = g.mass / g.rest;
+  const double mu = (double)c.mu;
+  g.mu2m = g.mass * (mu + mu);
+  g.eta2 = (double)0.01f * g.h * g.h;
+  g.halt = nullptr;
+  return g;
+}
+
 float wrap_host(float a, float L) {
   float r = std::fmod(a, L);
   return r < 0.0f ? r + std::fabs(L) : r;
 }
 
 // Lattice + jitter + Box-Muller velocities from one Rng(seed) stream in particle order.
-void box_seed(const mg_box_config& c, const BoxGeo& g, std::vector<float>& x, std::vector<float>& v) {
+void box_seed(const mg_box_config& c, const BoxGeo<float>& g, std::vector<float>& x, std::vector<float>& v) {
   const size_t n = (size_t)c.side * c.side;
   x.assign(2 * n, 0.0f);
   v.assign(2 * n, 0.0f);
@@ -979,6 +999,10 @@ void box_seed(const mg_box_config& c, const BoxGeo& g, std::vector<float>& x, st
 template <class T> size_t carve(size_t off, T*& p, size_t count) {
   p = reinterpret_cast<T*>(off);
   return off + ((count * sizeof(T) + 255) / 256) * 256 + 256;
+}
+size_t carve_bytes(size_t off, void*& p, size_t bytes) {
+  p = reinterpret_cast<void*>(off);
+  return off + ((bytes + 255) / 256) * 256 + 256;
 }
 
 inline dim3 blocks_for(uint32_t n) { return dim3((n + BX_BLOCK - 1) / BX_BLOCK); }
@@ -1023,60 +1047,66 @@ void collect_times(mg_box* B) {
   B->pend.clear();
 }
 
+template <class R> PairView<R> pair_view(const mg_box* B) {
+  return PairView<R>{vp<R>(B->XP), B->start, B->pnb, B->pcnt};
+}
+
 // Enqueues one step up to and including the CG set-up; returns false on a launch error. When it
 // returns, X0, V and id hold the step's start state (positions, velocities, ids) in the step's cell
 // order: nothing later in the step writes them (the new velocity goes to VN).
-bool enqueue_front(mg_box* B) {
-  const BoxGeo& g = B->g;
+template <class R> bool enqueue_front(mg_box* B) {
+  const BoxGeo<R>& g = geo<R>(B);
   const uint32_t cg_off = (B->cfg.mu > 0.0f && B->cfg.max_cg > 0) ? 0u : 1u;
   hipStream_t st = B->stream;
   const dim3 blk(BX_BLOCK), gp = blocks_for(g.n);
   {
-  Tic t(B, BXT_GRID);
-  hipLaunchKernelGGL(bx_zero2, blocks_for(B->ncell + 1), blk, 0, st, B->halt, B->start, B->ncell + 1, B->cursor,
-                     B->ncell);
-  hipLaunchKernelGGL(bx_predict, gp, blk, 0, st, g, B->X, B->V, B->X0, B->key, B->start);
-  hipLaunchKernelGGL(bx_scan_totals, dim3(B->nscan), blk, 0, st, B->halt, B->start, B->ncell + 1, B->bsum);
-  hipLaunchKernelGGL(bx_scan_top, dim3(1), blk, 0, st, B->halt, B->bsum, B->nscan);
-  hipLaunchKernelGGL(bx_scan_apply, dim3(B->nscan), blk, 0, st, B->halt, B->start, B->ncell + 1, B->bsum);
-  hipLaunchKernelGGL(bx_scatter, gp, blk, 0, st, g, B->key, B->start, B->cursor, B->slot);
-  hipLaunchKernelGGL(bx_cellsort, blocks_for(B->ncell), blk, 0, st, B->halt, B->ncell, B->start, B->slot, B->id, B->sc, cg_off,
-                     B->cfg.max_cg, B->cfg.tol);
-  hipLaunchKernelGGL(bx_gather, gp, blk, 0, st, g, B->slot, B->X, B->X0, B->V, B->id, B->Xb, B->XP, B->X0b, B->Vb,
-                     B->idb);
+    Tic t(B, BXT_GRID);
+    hipLaunchKernelGGL(bx_zero2, blocks_for(B->ncell + 1), blk, 0, st, B->halt, B->start, B->ncell + 1, B->cursor,
+                       B->ncell);
+    hipLaunchKernelGGL(bx_predict<R>, gp, blk, 0, st, g, vp<R>(B->X), vp<R>(B->V), vp<R>(B->X0), B->key, B->start);
+    hipLaunchKernelGGL(bx_scan_totals, dim3(B->nscan), blk, 0, st, B->halt, B->start, B->ncell + 1, B->bsum);
+    hipLaunchKernelGGL(bx_scan_top, dim3(1), blk, 0, st, B->halt, B->bsum, B->nscan);
+    hipLaunchKernelGGL(bx_scan_apply, dim3(B->nscan), blk, 0, st, B->halt, B->start, B->ncell + 1, B->bsum);
+    hipLaunchKernelGGL(bx_scatter, gp, blk, 0, st, B->halt, g.n, B->key, B->start, B->cursor, B->slot);
+    hipLaunchKernelGGL(bx_cellsort, blocks_for(B->ncell), blk, 0, st, B->halt, B->ncell, B->start, B->slot, B->id,
+                       B->sc, cg_off, B->cfg.max_cg, B->cfg.tol);
+    hipLaunchKernelGGL(bx_gather<R>, gp, blk, 0, st, g, B->slot, vp<R>(B->X), vp<R>(B->X0), vp<R>(B->V), B->id,
+                       vp<R>(B->Xb), vp<R>(B->XP), vp<R>(B->X0b), vp<R>(B->Vb), B->idb);
   }
   std::swap(B->X, B->Xb);
   std::swap(B->X0, B->X0b);
   std::swap(B->V, B->Vb);
   std::swap(B->id, B->idb);
-  const PairView pv{B->XP, B->start, B->pnb, B->pcnt};
+  const PairView<R> pv = pair_view<R>(B);
   if (B->cfg.passes == 0) {
     Tic t(B, BXT_GRID);
-    hipLaunchKernelGGL(bx_pairs, gp, blk, 0, st, g, B->XP, B->start, B->pnb, B->pcnt);
+    hipLaunchKernelGGL(bx_pairs<R>, gp, blk, 0, st, g, vp<R>(B->XP), B->start, B->pnb, B->pcnt);
   }
   for (uint32_t p = 0; p < B->cfg.passes; p++) {
     if (p == 0) {  // the first pass A also builds the pair table
       Tic t(B, BXT_PAIRLAM);
-      hipLaunchKernelGGL(bx_pairs_lambda, gp, blk, 0, st, g, B->XP, B->start, B->pnb, B->pcnt, B->LAM);
+      hipLaunchKernelGGL(bx_pairs_lambda<R>, gp, blk, 0, st, g, vp<R>(B->XP), B->start, B->pnb, B->pcnt,
+                         sp<R>(B->LAM));
     } else {
       Tic t(B, BXT_LAMBDA);
-      hipLaunchKernelGGL(bx_lambda, gp, blk, 0, st, g, pv, B->X, B->LAM);
+      hipLaunchKernelGGL(bx_lambda<R>, gp, blk, 0, st, g, pv, vp<R>(B->X), sp<R>(B->LAM));
     }
     {
       Tic t(B, BXT_DP);
-      hipLaunchKernelGGL(bx_dp, gp, blk, 0, st, g, pv, B->X, B->LAM, B->X2);
+      hipLaunchKernelGGL(bx_dp<R>, gp, blk, 0, st, g, pv, vp<R>(B->X), sp<R>(B->LAM), vp<R>(B->X2));
     }
     std::swap(B->X, B->X2);
   }
   const dim3 rblk(BX_RBLOCK), gr(g.nred);
   {
     Tic t(B, BXT_VEL);
-    hipLaunchKernelGGL(bx_vel_density, gr, rblk, 0, st, g, pv, B->X, B->X0, B->VN, B->DENS, B->part_b);
+    hipLaunchKernelGGL(bx_vel_density<R>, gr, rblk, 0, st, g, pv, vp<R>(B->X), vp<R>(B->X0), vp<R>(B->VN),
+                       sp<R>(B->DENS), B->part_b);
   }
   if (!cg_off) {
     Tic t(B, BXT_CGINIT);
-    hipLaunchKernelGGL(bx_cg_init, gr, rblk, 0, st, g, pv, B->X, B->DENS, B->VN, B->R, B->P, B->cw, B->ce,
-                       B->part_rr, B->sc);
+    hipLaunchKernelGGL(bx_cg_init<R>, gr, rblk, 0, st, g, pv, vp<R>(B->X), sp<R>(B->DENS), vp<R>(B->VN),
+                       vp<R>(B->R), vp<R>(B->P), sp<R>(B->cw), vp<R>(B->ce), B->part_rr, B->sc);
   }
   B->cg_t = 1;
   B->have_neighbors = true;
@@ -1085,44 +1115,46 @@ bool enqueue_front(mg_box* B) {
 
 // Enqueues CG iterations cg_t .. cg_t + iters - 1 and the end-of-chunk convergence check (which
 // also records the window verdict of step `s` when `verdict` is given).
-void enqueue_cg_chunk(mg_box* B, uint32_t iters, uint32_t* verdict = nullptr, uint32_t s = 0) {
-  const BoxGeo& g = B->g;
+template <class R> void enqueue_cg_chunk(mg_box* B, uint32_t iters, uint32_t* verdict = nullptr, uint32_t s = 0) {
+  const BoxGeo<R>& g = geo<R>(B);
   hipStream_t st = B->stream;
-  const PairView pv{B->XP, B->start, B->pnb, B->pcnt};
+  const PairView<R> pv = pair_view<R>(B);
   const dim3 rblk(BX_RBLOCK), gr(g.nred);
   const size_t nr = g.nred;
   for (uint32_t k = 0; k < iters; k++) {
     const uint32_t t = B->cg_t++;
     // iteration t reads the direction of t-1 and writes the other buffer; r.r of iteration u lives
     // in generation u & 1 (the initial residual is generation 0)
-    float2* pold = (t & 1u) ? B->P : B->P2;
-    float2* pnew = (t & 1u) ? B->P2 : B->P;
+    Vec<R>* pold = vp<R>((t & 1u) ? B->P : B->P2);
+    Vec<R>* pnew = vp<R>((t & 1u) ? B->P2 : B->P);
     const double* rr_prev = B->part_rr + ((t - 1u) & 1u) * nr;
     const double* rr_prev2 = B->part_rr + (t & 1u) * nr;
     double* rr_out = B->part_rr + (t & 1u) * nr;
     {
       Tic tt(B, BXT_SPMV);
-      hipLaunchKernelGGL(bx_cg_dir, gr, rblk, 0, st, g, pv, B->X, B->DENS, B->cw, B->ce, B->R, pold, pnew, B->AP,
-                         B->part_b, rr_prev, rr_prev2, B->part_pap, B->sc, t);
+      hipLaunchKernelGGL(bx_cg_dir<R>, gr, rblk, 0, st, g, pv, vp<R>(B->X), sp<R>(B->DENS), sp<R>(B->cw),
+                         vp<R>(B->ce), vp<R>(B->R), pold, pnew, vp<R>(B->AP), B->part_b, rr_prev, rr_prev2,
+                         B->part_pap, B->sc, t);
     }
     {
       Tic tt(B, BXT_UPDATE);
-      hipLaunchKernelGGL(bx_cg_upd, gr, rblk, 0, st, g, B->VN, B->R, pnew, B->AP, B->part_pap, rr_prev, rr_out, B->sc, t);
+      hipLaunchKernelGGL(bx_cg_upd<R>, gr, rblk, 0, st, g, vp<R>(B->VN), vp<R>(B->R), pnew, vp<R>(B->AP),
+                         B->part_pap, rr_prev, rr_out, B->sc, t);
     }
   }
   const uint32_t last = B->cg_t - 1u;
-  hipLaunchKernelGGL(bx_cg_check, dim3(1), rblk, 0, st, g, B->part_rr + (last & 1u) * nr, B->sc, last, verdict, s,
-                     B->halt);
+  hipLaunchKernelGGL(bx_cg_check, dim3(1), rblk, 0, st, g.nred, B->part_rr + (last & 1u) * nr, B->sc, last, verdict,
+                     s, B->halt);
 }
 
 // One full step: the front half and a first CG chunk of `first` iterations are enqueued without a
 // host wait; further chunks follow only if the device has not converged.
-int run_step(mg_box* B, uint32_t first) {
-  if (!enqueue_front(B)) { B->err = "kernel launch failed"; return MG_E_DEVICE; }
+template <class R> int run_step(mg_box* B, uint32_t first) {
+  if (!enqueue_front<R>(B)) { B->err = "kernel launch failed"; return MG_E_DEVICE; }
   uint32_t launched = 0;
   if (B->cfg.mu > 0.0f && B->cfg.max_cg > 0) {
     launched = std::min<uint32_t>(std::max<uint32_t>(first, 1u), B->cfg.max_cg);
-    enqueue_cg_chunk(B, launched);
+    enqueue_cg_chunk<R>(B, launched);
   } else {
     launched = B->cfg.max_cg;  // no CG this step
   }
@@ -1132,7 +1164,7 @@ int run_step(mg_box* B, uint32_t first) {
     collect_times(B);
     if (B->h_sc->done || launched >= B->cfg.max_cg) break;
     uint32_t k = std::min<uint32_t>(BX_CG_CHUNK, B->cfg.max_cg - launched);
-    enqueue_cg_chunk(B, k);
+    enqueue_cg_chunk<R>(B, k);
     launched += k;
     if (hipGetLastError() != hipSuccess) { B->err = "kernel launch failed"; return MG_E_DEVICE; }
   }
@@ -1161,7 +1193,7 @@ void take_buffer(T* keep, T** const (&names)[N]) {
 // so those buffers become the live state again and step b is re-run with host-driven chunks
 // (run_step); the next window is then half as long. The results are those of running every step
 // with run_step: iterations past convergence change nothing.
-int run_window(mg_box* B, uint32_t steps) {
+template <class R> int run_window(mg_box* B, uint32_t steps) {
   const bool cg_on = B->cfg.mu > 0.0f && B->cfg.max_cg > 0;
   uint32_t done = 0;
   while (done < steps) {
@@ -1170,7 +1202,7 @@ int run_window(mg_box* B, uint32_t steps) {
     const uint32_t spare = B->cg_stable ? 0u : BX_SPARE_IT + m / 8u;
     const uint32_t pred = std::min<uint32_t>(m + spare, B->cfg.max_cg);
     if (cg_on && W == 1) {  // a single step: chunks driven from the host
-      const int rc = run_step(B, pred);
+      const int rc = run_step<R>(B, pred);
       if (rc) return rc;
       B->win = B->h_sc->it <= pred ? 2u : 1u;
       B->cg_stable = false;
@@ -1179,9 +1211,9 @@ int run_window(mg_box* B, uint32_t steps) {
     }
     if (cg_on) hipLaunchKernelGGL(bx_verdict_reset, dim3(1), dim3(64), 0, B->stream, B->verdict, B->halt);
     for (uint32_t s = 0; s < W; s++) {
-      if (!enqueue_front(B)) { B->err = "kernel launch failed"; return MG_E_DEVICE; }
+      if (!enqueue_front<R>(B)) { B->err = "kernel launch failed"; return MG_E_DEVICE; }
       B->starts[s] = {B->X0, B->V, B->id};
-      if (cg_on) enqueue_cg_chunk(B, pred, B->verdict, s);
+      if (cg_on) enqueue_cg_chunk<R>(B, pred, B->verdict, s);
       std::swap(B->V, B->VN);
     }
     if (cg_on) hipMemcpyAsync(B->h_verdict, B->verdict, (W + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, B->stream);
@@ -1210,18 +1242,75 @@ int run_window(mg_box* B, uint32_t steps) {
     }
     B->win = std::max<uint32_t>(B->win / 2u, 1u);
     const mg_box::Start& st = B->starts[bad];
-    float2** const pos[] = {&B->X, &B->Xb, &B->X0, &B->X0b, &B->X2};
-    float2** const vel[] = {&B->V, &B->Vb, &B->VN};
+    void** const pos[] = {&B->X, &B->Xb, &B->X0, &B->X0b, &B->X2};
+    void** const vel[] = {&B->V, &B->Vb, &B->VN};
     uint32_t** const ids[] = {&B->id, &B->idb};
     take_buffer(st.x, pos);
     take_buffer(st.v, vel);
     take_buffer(st.id, ids);
     hipLaunchKernelGGL(bx_verdict_reset, dim3(1), dim3(64), 0, B->stream, B->verdict, B->halt);
-    const int rc = run_step(B, pred + BX_CG_CHUNK);
+    const int rc = run_step<R>(B, pred + BX_CG_CHUNK);
     if (rc) return rc;
     done += 1;
   }
   return MG_OK;
+}
+
+// Host copies of the live state in the original particle order, converted to T.
+template <class R, class T> int copy_state_as(mg_box* B, T* x2n, T* v2n) {
+  const uint32_t n = B->n;
+  std::vector<Vec<R>> x(n), v(n);
+  std::vector<uint32_t> id(n);
+  bool ok = hipStreamSynchronize(B->stream) == hipSuccess;
+  ok = ok && hipMemcpy(x.data(), B->X, (size_t)n * sizeof(Vec<R>), hipMemcpyDeviceToHost) == hipSuccess;
+  ok = ok && hipMemcpy(v.data(), B->V, (size_t)n * sizeof(Vec<R>), hipMemcpyDeviceToHost) == hipSuccess;
+  ok = ok && hipMemcpy(id.data(), B->id, (size_t)n * 4, hipMemcpyDeviceToHost) == hipSuccess;
+  if (!ok) return MG_E_DEVICE;
+  for (uint32_t q = 0; q < n; q++) {
+    uint32_t i = id[q];
+    if (i >= n) return MG_E_DEVICE;
+    if (x2n) { x2n[2 * i] = (T)x[q].x; x2n[2 * i + 1] = (T)x[q].y; }
+    if (v2n) { v2n[2 * i] = (T)v[q].x; v2n[2 * i + 1] = (T)v[q].y; }
+  }
+  return MG_OK;
+}
+template <class R, class T> int set_state_from(mg_box* B, const T* x2n, const T* v2n) {
+  const uint32_t n = B->n;
+  std::vector<Vec<R>> x(n), v(n);
+  std::vector<uint32_t> ids(n);
+  for (uint32_t i = 0; i < n; i++) {
+    ids[i] = i;
+    x[i].x = (R)x2n[2 * i]; x[i].y = (R)x2n[2 * i + 1];
+    v[i].x = (R)v2n[2 * i]; v[i].y = (R)v2n[2 * i + 1];
+  }
+  bool ok = hipStreamSynchronize(B->stream) == hipSuccess;
+  ok = ok && hipMemcpy(B->X, x.data(), (size_t)n * sizeof(Vec<R>), hipMemcpyHostToDevice) == hipSuccess;
+  ok = ok && hipMemcpy(B->V, v.data(), (size_t)n * sizeof(Vec<R>), hipMemcpyHostToDevice) == hipSuccess;
+  ok = ok && hipMemcpy(B->id, ids.data(), (size_t)n * 4, hipMemcpyHostToDevice) == hipSuccess;
+  B->have_neighbors = false;
+  return ok ? MG_OK : MG_E_DEVICE;
+}
+template <class R, class T> int density_as(mg_box* B, T* out) {
+  const uint32_t n = B->n;
+  std::vector<R> d(n);
+  std::vector<uint32_t> id(n);
+  bool ok = hipStreamSynchronize(B->stream) == hipSuccess;
+  ok = ok && hipMemcpy(d.data(), B->DENS, (size_t)n * sizeof(R), hipMemcpyDeviceToHost) == hipSuccess;
+  ok = ok && hipMemcpy(id.data(), B->id, (size_t)n * 4, hipMemcpyDeviceToHost) == hipSuccess;
+  if (!ok) return MG_E_DEVICE;
+  for (uint32_t q = 0; q < n; q++) {
+    if (id[q] >= n) return MG_E_DEVICE;
+    out[id[q]] = (T)d[q];
+  }
+  return MG_OK;
+}
+template <class R> void launch_nbr_count(mg_box* B, uint32_t* d_cnt) {
+  hipLaunchKernelGGL(bx_nbr_count<R>, blocks_for(B->n), dim3(BX_BLOCK), 0, B->stream, geo<R>(B), pair_view<R>(B),
+                     B->id, d_cnt);
+}
+template <class R> void launch_nbr_write(mg_box* B, const uint32_t* d_off, uint32_t* d_nb, uint32_t cap) {
+  hipLaunchKernelGGL(bx_nbr_write<R>, blocks_for(B->n), dim3(BX_BLOCK), 0, B->stream, geo<R>(B), pair_view<R>(B),
+                     B->id, d_off, d_nb, cap);
 }
 
 }  // namespace
@@ -1230,7 +1319,7 @@ extern "C" {
 
 int mg_box_geometry(const mg_box_config* cfg, double* out5) {
   if (!cfg || !out5) return MG_E_ARG;
-  BoxGeo g;
+  BoxGeo<float> g;
   std::string e;
   if (!box_geometry(*cfg, g, e)) return MG_E_PARAMS;
   out5[0] = g.h; out5[1] = g.L; out5[2] = g.cw; out5[3] = g.mass; out5[4] = g.ncx;
@@ -1243,26 +1332,32 @@ int mg_box_create(const mg_box_config* cfg, mg_box** out, char* err, size_t errl
   auto* B = new mg_box();
   B->cfg = *cfg;
   std::string e;
-  if (!box_geometry(*cfg, B->g, e)) { box_set_err(err, errlen, e); delete B; return MG_E_PARAMS; }
+  if (!box_geometry(*cfg, B->gf, e)) { box_set_err(err, errlen, e); delete B; return MG_E_PARAMS; }
+  B->f64 = cfg->fp64 != 0u;
+  B->gd = promote(B->gf, *cfg);
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { box_set_err(err, errlen, "no HIP device"); delete B; return MG_E_DEVICE; }
-  const uint32_t n = B->g.n;
-  B->ncell = B->g.ncx * B->g.ncx;
+  const uint32_t n = B->gf.n;
+  B->n = n;
+  B->nred = B->gf.nred;
+  B->ncell = B->gf.ncx * B->gf.ncx;
   B->nscan = (B->ncell + 1 + BX_SCAN_CHUNK - 1) / BX_SCAN_CHUNK;
   if (B->nscan > BX_SCAN_CHUNK) { box_set_err(err, errlen, "too many cells"); delete B; return MG_E_PARAMS; }
+  const size_t rs = B->f64 ? 8 : 4;   // bytes of one real
+  const size_t vb = 2 * rs * n;       // bytes of one Vec<R> array
   size_t off = 0;
-  off = carve(off, B->X, n); off = carve(off, B->XP, n); off = carve(off, B->X0, n); off = carve(off, B->V, n);
-  off = carve(off, B->X2, n); off = carve(off, B->Xb, n); off = carve(off, B->X0b, n); off = carve(off, B->Vb, n);
-  off = carve(off, B->VN, n);
+  off = carve_bytes(off, B->X, vb); off = carve_bytes(off, B->XP, vb); off = carve_bytes(off, B->X0, vb);
+  off = carve_bytes(off, B->V, vb); off = carve_bytes(off, B->X2, vb); off = carve_bytes(off, B->Xb, vb);
+  off = carve_bytes(off, B->X0b, vb); off = carve_bytes(off, B->Vb, vb); off = carve_bytes(off, B->VN, vb);
   off = carve(off, B->id, n); off = carve(off, B->idb, n); off = carve(off, B->key, n); off = carve(off, B->slot, n);
   off = carve(off, B->start, (size_t)B->ncell + 1); off = carve(off, B->cursor, (size_t)B->ncell);
   off = carve(off, B->bsum, (size_t)B->nscan);
-  off = carve(off, B->LAM, n); off = carve(off, B->DENS, n);
-  off = carve(off, B->R, n); off = carve(off, B->P, n); off = carve(off, B->AP, n);
-  off = carve(off, B->part_b, (size_t)B->g.nred); off = carve(off, B->part_rr, 2 * (size_t)B->g.nred);
-  off = carve(off, B->part_pap, (size_t)B->g.nred); off = carve(off, B->P2, n);
+  off = carve_bytes(off, B->LAM, rs * n); off = carve_bytes(off, B->DENS, rs * n);
+  off = carve_bytes(off, B->R, vb); off = carve_bytes(off, B->P, vb); off = carve_bytes(off, B->AP, vb);
+  off = carve(off, B->part_b, (size_t)B->nred); off = carve(off, B->part_rr, 2 * (size_t)B->nred);
+  off = carve(off, B->part_pap, (size_t)B->nred); off = carve_bytes(off, B->P2, vb);
   off = carve(off, B->pnb, (size_t)BX_KMAX * n); off = carve(off, B->pcnt, n);
-  off = carve(off, B->cw, (size_t)BX_KMAX * n); off = carve(off, B->ce, (size_t)BX_KMAX * n);
+  off = carve_bytes(off, B->cw, (size_t)BX_KMAX * rs * n); off = carve_bytes(off, B->ce, (size_t)BX_KMAX * vb);
   off = carve(off, B->verdict, BX_WINDOW + 1);
   off = carve(off, B->halt, 1); off = carve(off, B->sc, 1);
   if (hipMalloc(&B->arena, off) != hipSuccess) { box_set_err(err, errlen, "hipMalloc failed"); delete B; return MG_E_DEVICE; }
@@ -1270,8 +1365,10 @@ int mg_box_create(const mg_box_config* cfg, mg_box** out, char* err, size_t errl
   auto fix = [&](auto*& p) { p = reinterpret_cast<std::remove_reference_t<decltype(p)>>(base + (uintptr_t)p); };
   fix(B->X); fix(B->XP); fix(B->X0); fix(B->V); fix(B->X2); fix(B->Xb); fix(B->X0b); fix(B->Vb); fix(B->VN);
   fix(B->id); fix(B->idb); fix(B->key); fix(B->slot); fix(B->start); fix(B->cursor); fix(B->bsum);
-  fix(B->LAM); fix(B->DENS); fix(B->R); fix(B->P); fix(B->AP); fix(B->part_b); fix(B->part_rr); fix(B->part_pap); fix(B->P2); fix(B->sc); fix(B->pnb); fix(B->pcnt); fix(B->cw); fix(B->ce); fix(B->verdict); fix(B->halt);
-  B->g.halt = B->halt;
+  fix(B->LAM); fix(B->DENS); fix(B->R); fix(B->P); fix(B->AP); fix(B->part_b); fix(B->part_rr); fix(B->part_pap);
+  fix(B->P2); fix(B->sc); fix(B->pnb); fix(B->pcnt); fix(B->cw); fix(B->ce); fix(B->verdict); fix(B->halt);
+  B->gf.halt = B->halt;
+  B->gd.halt = B->halt;
   bool ok = hipMemset(B->arena, 0, off) == hipSuccess;
   ok = ok && hipHostMalloc((void**)&B->h_sc, sizeof(BoxScal), 0) == hipSuccess;
   ok = ok && hipHostMalloc((void**)&B->h_verdict, (BX_WINDOW + 1) * sizeof(uint32_t), 0) == hipSuccess;
@@ -1279,12 +1376,9 @@ int mg_box_create(const mg_box_config* cfg, mg_box** out, char* err, size_t errl
   ok = ok && hipEventCreate(&B->ev0) == hipSuccess && hipEventCreate(&B->ev1) == hipSuccess;
   if (ok) {
     std::vector<float> x, v;
-    box_seed(*cfg, B->g, x, v);
-    std::vector<uint32_t> ids(n);
-    for (uint32_t i = 0; i < n; i++) ids[i] = i;
-    ok = hipMemcpy(B->X, x.data(), x.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
-         hipMemcpy(B->V, v.data(), v.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
-         hipMemcpy(B->id, ids.data(), ids.size() * 4, hipMemcpyHostToDevice) == hipSuccess;
+    box_seed(*cfg, B->gf, x, v);
+    ok = (B->f64 ? set_state_from<double, float>(B, x.data(), v.data())
+                 : set_state_from<float, float>(B, x.data(), v.data())) == MG_OK;
   }
   if (!ok) { box_set_err(err, errlen, "HIP error while creating the box"); mg_box_destroy(B); return MG_E_DEVICE; }
   *out = B;
@@ -1305,12 +1399,12 @@ void mg_box_destroy(mg_box* B) {
   delete B;
 }
 
-size_t mg_box_count(const mg_box* B) { return B ? (size_t)B->g.n : 0; }
+size_t mg_box_count(const mg_box* B) { return B ? (size_t)B->n : 0; }
 
 int mg_box_step(mg_box* B, uint32_t steps) {
   if (!B) return MG_E_ARG;
   hipEventRecord(B->ev0, B->stream);
-  const int rc = run_window(B, steps);
+  const int rc = B->f64 ? run_window<double>(B, steps) : run_window<float>(B, steps);
   if (rc) return rc;
   hipEventRecord(B->ev1, B->stream);
   if (hipEventSynchronize(B->ev1) != hipSuccess) return MG_E_DEVICE;
@@ -1335,50 +1429,29 @@ int mg_box_kernel_times(mg_box* B, double* ms, uint64_t* launches, int nclass) {
 
 int mg_box_copy_state(mg_box* B, float* x2n, float* v2n) {
   if (!B) return MG_E_ARG;
-  const uint32_t n = B->g.n;
-  std::vector<float> x(2 * (size_t)n), v(2 * (size_t)n);
-  std::vector<uint32_t> id(n);
-  bool ok = hipStreamSynchronize(B->stream) == hipSuccess;
-  ok = ok && hipMemcpy(x.data(), B->X, x.size() * 4, hipMemcpyDeviceToHost) == hipSuccess;
-  ok = ok && hipMemcpy(v.data(), B->V, v.size() * 4, hipMemcpyDeviceToHost) == hipSuccess;
-  ok = ok && hipMemcpy(id.data(), B->id, id.size() * 4, hipMemcpyDeviceToHost) == hipSuccess;
-  if (!ok) return MG_E_DEVICE;
-  for (uint32_t q = 0; q < n; q++) {
-    uint32_t i = id[q];
-    if (i >= n) return MG_E_DEVICE;
-    if (x2n) { x2n[2 * i] = x[2 * q]; x2n[2 * i + 1] = x[2 * q + 1]; }
-    if (v2n) { v2n[2 * i] = v[2 * q]; v2n[2 * i + 1] = v[2 * q + 1]; }
-  }
-  return MG_OK;
+  return B->f64 ? copy_state_as<double, float>(B, x2n, v2n) : copy_state_as<float, float>(B, x2n, v2n);
+}
+int mg_box_copy_state_f64(mg_box* B, double* x2n, double* v2n) {
+  if (!B) return MG_E_ARG;
+  return B->f64 ? copy_state_as<double, double>(B, x2n, v2n) : copy_state_as<float, double>(B, x2n, v2n);
 }
 
 int mg_box_set_state(mg_box* B, const float* x2n, const float* v2n) {
   if (!B || !x2n || !v2n) return MG_E_ARG;
-  const uint32_t n = B->g.n;
-  std::vector<uint32_t> ids(n);
-  for (uint32_t i = 0; i < n; i++) ids[i] = i;
-  bool ok = hipStreamSynchronize(B->stream) == hipSuccess;
-  ok = ok && hipMemcpy(B->X, x2n, (size_t)n * 8, hipMemcpyHostToDevice) == hipSuccess;
-  ok = ok && hipMemcpy(B->V, v2n, (size_t)n * 8, hipMemcpyHostToDevice) == hipSuccess;
-  ok = ok && hipMemcpy(B->id, ids.data(), (size_t)n * 4, hipMemcpyHostToDevice) == hipSuccess;
-  B->have_neighbors = false;
-  return ok ? MG_OK : MG_E_DEVICE;
+  return B->f64 ? set_state_from<double, float>(B, x2n, v2n) : set_state_from<float, float>(B, x2n, v2n);
+}
+int mg_box_set_state_f64(mg_box* B, const double* x2n, const double* v2n) {
+  if (!B || !x2n || !v2n) return MG_E_ARG;
+  return B->f64 ? set_state_from<double, double>(B, x2n, v2n) : set_state_from<float, double>(B, x2n, v2n);
 }
 
 int mg_box_density(mg_box* B, float* out) {
   if (!B || !out) return MG_E_ARG;
-  const uint32_t n = B->g.n;
-  std::vector<float> d(n);
-  std::vector<uint32_t> id(n);
-  bool ok = hipStreamSynchronize(B->stream) == hipSuccess;
-  ok = ok && hipMemcpy(d.data(), B->DENS, (size_t)n * 4, hipMemcpyDeviceToHost) == hipSuccess;
-  ok = ok && hipMemcpy(id.data(), B->id, (size_t)n * 4, hipMemcpyDeviceToHost) == hipSuccess;
-  if (!ok) return MG_E_DEVICE;
-  for (uint32_t q = 0; q < n; q++) {
-    if (id[q] >= n) return MG_E_DEVICE;
-    out[id[q]] = d[q];
-  }
-  return MG_OK;
+  return B->f64 ? density_as<double, float>(B, out) : density_as<float, float>(B, out);
+}
+int mg_box_density_f64(mg_box* B, double* out) {
+  if (!B || !out) return MG_E_ARG;
+  return B->f64 ? density_as<double, double>(B, out) : density_as<float, double>(B, out);
 }
 
 int mg_box_stats(mg_box* B, double* out3) {
@@ -1402,15 +1475,14 @@ int mg_box_set_cg_prediction(mg_box* B, uint32_t iterations) {
 int64_t mg_box_neighbors(mg_box* B, uint32_t* off, uint32_t* nb, int64_t cap) {
   if (!B || !off) return MG_E_ARG;
   if (!B->have_neighbors) return MG_E_ARG;
-  const uint32_t n = B->g.n;
-  const BoxGeo& g = B->g;
+  const uint32_t n = B->n;
   uint32_t* d_cnt = nullptr;
   uint32_t* d_nb = nullptr;
   const uint32_t dcap = (uint32_t)std::min<int64_t>(std::max<int64_t>(cap, 1), (int64_t)64 * n + 64);
   if (hipMalloc(&d_cnt, ((size_t)n + 1) * 4) != hipSuccess) return MG_E_DEVICE;
   if (hipMalloc(&d_nb, (size_t)dcap * 4) != hipSuccess) { (void)hipFree(d_cnt); return MG_E_DEVICE; }
-  const PairView pv{B->XP, B->start, B->pnb, B->pcnt};
-  hipLaunchKernelGGL(bx_nbr_count, blocks_for(n), dim3(BX_BLOCK), 0, B->stream, g, pv, B->id, d_cnt);
+  if (B->f64) launch_nbr_count<double>(B, d_cnt);
+  else launch_nbr_count<float>(B, d_cnt);
   std::vector<uint32_t> cnt(n);
   bool ok = hipMemcpyAsync(cnt.data(), d_cnt, (size_t)n * 4, hipMemcpyDeviceToHost, B->stream) == hipSuccess &&
             hipStreamSynchronize(B->stream) == hipSuccess;
@@ -1421,7 +1493,8 @@ int64_t mg_box_neighbors(mg_box* B, uint32_t* off, uint32_t* nb, int64_t cap) {
     ok = hipMemcpy(d_cnt, off, (size_t)n * 4, hipMemcpyHostToDevice) == hipSuccess;
   }
   if (ok) {
-    hipLaunchKernelGGL(bx_nbr_write, blocks_for(n), dim3(BX_BLOCK), 0, B->stream, g, pv, B->id, d_cnt, d_nb, dcap);
+    if (B->f64) launch_nbr_write<double>(B, d_cnt, d_nb, dcap);
+    else launch_nbr_write<float>(B, d_cnt, d_nb, dcap);
     ok = hipStreamSynchronize(B->stream) == hipSuccess;
   }
   if (ok && nb && total <= cap) ok = hipMemcpy(nb, d_nb, (size_t)total * 4, hipMemcpyDeviceToHost) == hipSuccess;

@@ -136,3 +136,69 @@ def test_window_rerun_after_low_prediction_matches_oracle(box_mod, orc):
     assert np.array_equal(x.astype(np.float64), xo)
     assert np.abs(v - vo).max() / max(np.abs(vo).max(), 1e-30) <= V_TOL
     assert b.stats()["cg_iterations"] == o.stats()["cg_iterations"]
+
+
+# ---- FP64 state (fp64 = 1) against the oracle's f64 policy -------------------------------------
+# Bar: the initial state, neighbour lists and, within a step, positions and densities equal the
+# oracle's f64 policy; the CG's FP64 dot products are summed per workgroup in a fixed tree while the
+# oracle sums sequentially, and in f64 the CG scalars are not rounded to f32, so velocities (and,
+# from the next step on, positions) agree to within X64_TOL / V64_TOL relative, not bit for bit.
+X64_TOL = 1e-12
+V64_TOL = 1e-10
+
+
+def _compare64(b, o, label):
+    x, v = b.state()
+    xo, vo = o.state()
+    assert x.dtype == np.float64 and v.dtype == np.float64
+    off, nb = b.neighbors()
+    ooff, onb = o.neighbors()
+    assert np.array_equal(off, ooff) and np.array_equal(nb, onb), f"{label}: neighbour lists differ"
+    L = o.cfg["side"] * o.cfg["dx"]
+    assert np.abs(x - xo).max() <= X64_TOL * L, f"{label}: positions differ (max {np.abs(x - xo).max()})"
+    d, do = b.density(), o.density()
+    assert np.abs(d - do).max() <= X64_TOL * do.max(), f"{label}: densities differ"
+    vmax = max(np.abs(vo).max(), 1e-30)
+    assert np.abs(v - vo).max() / vmax <= V64_TOL, f"{label}: velocities differ"
+    assert b.stats()["cg_iterations"] == o.stats()["cg_iterations"], f"{label}: CG iterations differ"
+
+
+@pytest.mark.parametrize("side,mu", [(32, 1.0e-3), (24, 1.0), (40, 0.0)])
+def test_fp64_steps_match_oracle(box_mod, orc, side, mu):
+    cfg = {"side": side, "mu": mu, "fp64": 1}
+    print(f"box fp64 side={side} mu={mu}", flush=True)
+    b = box_mod.Box(cfg)
+    o = orc.C3Oracle(cfg, "fp64")
+    x, v = b.state()
+    xo, vo = o.state()
+    assert np.array_equal(x, xo) and np.array_equal(v, vo)  # same f32 seed, promoted
+    for k in range(3):
+        b.step()
+        o.step()
+        _compare64(b, o, f"fp64 side={side} mu={mu} step={k}")
+    if mu == 1.0e-3:  # first step: everything before the CG is bit-exact
+        b2 = box_mod.Box(cfg)
+        o2 = orc.C3Oracle(cfg, "fp64")
+        b2.step()
+        o2.step()
+        assert np.array_equal(b2.state()[0], o2.state()[0])
+        assert np.array_equal(b2.density(), o2.density())
+
+
+def test_fp64_multi_step_windows_match_oracle(box_mod, orc):
+    cfg = {"side": 24, "mu": 3.0, "fp64": 1}
+    b = box_mod.Box(cfg)
+    o = orc.C3Oracle(cfg, "fp64")
+    b.step(19)
+    o.step(19)
+    _compare64(b, o, "fp64 windows")
+
+
+def test_fp64_full_size_million_particles(box_mod, orc):
+    cfg = {"side": 1024, "fp64": 1}
+    print("box fp64 full size 1024^2", flush=True)
+    b = box_mod.Box(cfg)
+    b.step()
+    o = orc.C3Oracle(cfg, "fp64")
+    o.step()
+    _compare64(b, o, "fp64 side=1024")
