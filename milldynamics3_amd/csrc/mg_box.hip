@@ -68,6 +68,7 @@ enum { BXT_GRID = 0, BXT_LAMBDA, BXT_DP, BXT_VEL, BXT_CGINIT, BXT_SPMV, BXT_UPDA
 #define BXKT template <class R> __global__ void __launch_bounds__(BX_BLOCK)
 #define BXKRT template <class R> __global__ void __launch_bounds__(BX_RBLOCK)
 #define BDEV __device__ __attribute__((always_inline)) inline
+#define BDEVX __device__ __attribute__((always_inline)) inline  // explicit specializations
 
 template <class R> struct VecT;
 template <> struct VecT<float> { using type = float2; };
@@ -118,6 +119,46 @@ BDEV double mfmax(double a, double b) { return fmax(a, b); }
 BX_VEC_OPS(float2, float, make_float2)
 BX_VEC_OPS(double2, double, make_double2)
 #undef BX_VEC_OPS
+// |d| and d / |d| of a pair vector. In f32 these are the compiler's correctly rounded sequences
+// without their range scaling and special-value handling, which are the identity in the range the
+// pair loops use them in, so the results are bit for bit those of sqrtf and '/', and the two
+// divisions by r share one refined reciprocal:
+//  - pair_len(r2) equals sqrtf(r2) for r2 >= 2^-96. Below that it returns a value < 1e-9, and
+//    every caller only compares such an r with 1e-9 (the kernel gradient and the Morris direction
+//    are zero / fixed there), so results do not depend on it.
+//  - pair_div(d, r) equals (d.x / r, d.y / r) for r in [2^-100, 2^30] (the box side is at most
+//    2^23, so every r > 1e-9 is in range) when both |components| are >= 2^-96; otherwise it takes
+//    the IEEE division. Callers use the quotient only for r > 1e-9.
+// mg_box_check_arith compares both with the IEEE operations on the GPU.
+template <class R> BDEV R pair_len(R r2) { return msqrt(r2); }
+template <class R> BDEV Vec<R> pair_div(Vec<R> d, R r) { return div2(d, r); }
+BDEV float sqrt_core(float x) {  // v_sqrt (1 ulp) and the two neighbouring candidates, as LLVM lowers sqrtf
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const float sm = __uint_as_float(__float_as_uint(s) - 1u);
+  const float sp = __uint_as_float(__float_as_uint(s) + 1u);
+  const float rm = __builtin_fmaf(-sm, s, x);
+  const float rp = __builtin_fmaf(-sp, s, x);
+  const float t = rm <= 0.0f ? sm : s;
+  return rp > 0.0f ? sp : t;
+}
+template <> BDEVX float pair_len<float>(float r2) { return sqrt_core(r2); }
+template <> BDEVX float2 pair_div<float>(float2 d, float r) {
+  const bool tiny = (fabsf(d.x) < 0x1p-96f) | (fabsf(d.y) < 0x1p-96f);
+  if (__builtin_expect(tiny, 0)) return make_float2(d.x / r, d.y / r);
+  // Newton-refined reciprocal, then two residual corrections per quotient (the unscaled path of
+  // the compiler's division: rcp, fma, fma | mul, fma, fma, fma, fma)
+  const float y0 = __builtin_amdgcn_rcpf(r);
+  const float e = __builtin_fmaf(-r, y0, 1.0f);
+  const float y = __builtin_fmaf(e, y0, y0);
+  auto q = [&](float a) __attribute__((always_inline)) {
+    const float q0 = a * y;
+    const float r0 = __builtin_fmaf(-r, q0, a);
+    const float q1 = __builtin_fmaf(r0, y, q0);
+    const float r1 = __builtin_fmaf(-r, q1, a);
+    return __builtin_fmaf(r1, y, q1);
+  };
+  return make_float2(q(d.x), q(d.y));
+}
 template <class R> BDEV R wrapR(R a, R L) {  // rem_euclid
   R r = mfmod(a, L);
   return r < (R)0 ? r + mfabs(L) : r;
@@ -146,7 +187,7 @@ template <class R> BDEV R poly6(R r2, const BoxGeo<R>& g) {  // pbf.rs:119-126
 template <class R> BDEV Vec<R> spiky(Vec<R> d, R r, const BoxGeo<R>& g) {  // pbf.rs:131-137
   if (r <= (R)1e-9f || r >= g.h) return mk((R)0, (R)0);
   R coeff = g.ks * (g.h - r) * (g.h - r);
-  return mul2(div2(d, r), coeff);
+  return mul2(pair_div(d, r), coeff);
 }
 // XCD-aware tile index: hardware dispatches consecutive workgroups round-robin over the 8 XCDs;
 // remap so XCD x processes tiles [x*chunk, (x+1)*chunk).
@@ -503,7 +544,7 @@ template <class R> struct LamAcc {
 };
 template <class R> BDEV void lambda_term(const BoxGeo<R>& g, Vec<R> d, R r2, LamAcc<R>& a) {
   a.rho += g.mass * poly6(r2, g);
-  const R r = msqrt(r2);
+  const R r = pair_len(r2);
   const Vec<R> gr = mul2(spiky(d, r, g), g.m_over_rest);
   a.gs = add2(a.gs, gr);
   a.sgs += len2(gr);
@@ -551,7 +592,7 @@ BXKT bx_dp(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const R* _
   for_pairs_g(g, pv, i, [&](uint32_t j) __attribute__((always_inline)) { return Nb{X[j], LAM[j]}; },
               [&](uint32_t, const Nb& a) __attribute__((always_inline)) {
     const Vec<R> d = rel(xi, a.x, g);
-    const R r = msqrt(len2(d));
+    const R r = pair_len(len2(d));
     s = add2(s, smul2(li + a.l, spiky(d, r, g)));
   });
   const Vec<R> dp = mul2(s, g.m_over_rest);
@@ -589,8 +630,8 @@ template <class R>
 BDEV void morris(const BoxGeo<R>& g, Vec<R> xi, R rho_i, Vec<R> xj, R dens_j, R& w, Vec<R>& e) {
   const R rho_j = mfmax(dens_j, (R)1e-6f);
   const Vec<R> d = rel(xi, xj, g);
-  const R r = msqrt(len2(d));
-  const Vec<R> dr = div2(d, r);
+  const R r = pair_len(len2(d));
+  const Vec<R> dr = pair_div(d, r);
   const Vec<R> gr = (r <= (R)1e-9f || r >= g.h) ? mk((R)0, (R)0) : mul2(dr, g.ks * (g.h - r) * (g.h - r));
   const R num = (g.mu2m / (rho_i * rho_j)) * mfabs(dot2(d, gr));
   w = num / (len2(d) + g.eta2);
@@ -821,6 +862,50 @@ BXKT bx_nbr_write(BoxGeo<R> g, PairView<R> pv, const uint32_t* __restrict__ id, 
 }
 
 // ---------------------------------------------------------------------------------------------
+// self-check: the pair loops' f32 square root and quotient against the IEEE operations
+// ---------------------------------------------------------------------------------------------
+BDEV uint64_t mix64(uint64_t& s) {  // splitmix64
+  uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+// A float with the given exponent range [e0, e1] and a random significand (every 8th draw all
+// ones or all zeros, the significands where a reciprocal's rounding is hardest).
+BDEV float draw_float(uint64_t& s, int e0, int e1) {
+  const uint64_t z = mix64(s);
+  const int e = e0 + (int)((z >> 32) % (uint64_t)(e1 - e0 + 1));
+  uint32_t m = (uint32_t)z & 0x7FFFFFu;
+  const uint32_t sel = (uint32_t)(z >> 23) & 7u;
+  if (sel == 0u) m = 0x7FFFFFu;
+  if (sel == 1u) m = 0u;
+  return __uint_as_float((uint32_t)(e + 127) << 23 | m);
+}
+BXK bx_check_arith(uint64_t seed, uint32_t per_lane, unsigned long long* __restrict__ bad) {
+  uint64_t s = seed ^ ((uint64_t)(blockIdx.x * BX_BLOCK + threadIdx.x) * 0xD1B54A32D192ED03ull);
+  unsigned long long bad_sqrt = 0, bad_div = 0;
+  for (uint32_t k = 0; k < per_lane; k++) {
+    // square root over [2^-96, 2^60], and below 2^-96 the promise that the value stays < 1e-9
+    const float x = draw_float(s, -96, 60);
+    if (__float_as_uint(pair_len<float>(x)) != __float_as_uint(sqrtf(x))) bad_sqrt++;
+    const float xt = draw_float(s, -126, -97);
+    if (!(pair_len<float>(xt) < 1e-9f)) bad_sqrt++;
+    // quotients d / r with r in [2^-30, 2^29] and |d.x|, |d.y| <= r down to 2^-100 r (and zero)
+    const float r = draw_float(s, -30, 29);
+    const uint64_t z = mix64(s);
+    float ux = draw_float(s, -100, -1), uy = draw_float(s, -100, -1);
+    if ((z & 15u) == 0u) ux = 1.0f;
+    if ((z & 15u) == 1u) uy = 0.0f;
+    const float2 d = make_float2((z & 16u) ? -r * ux : r * ux, (z & 32u) ? -r * uy : r * uy);
+    const float2 q = pair_div<float>(d, r);
+    if (__float_as_uint(q.x) != __float_as_uint(d.x / r) || __float_as_uint(q.y) != __float_as_uint(d.y / r))
+      bad_div++;
+  }
+  if (bad_sqrt) atomicAdd(&bad[0], bad_sqrt);
+  if (bad_div) atomicAdd(&bad[1], bad_div);
+}
+
+// ---------------------------------------------------------------------------------------------
 // host runtime
 // ---------------------------------------------------------------------------------------------
 struct mg_box {
@@ -897,6 +982,7 @@ bool box_geometry(const mg_box_config& c, BoxGeo<float>& g, std::string& err) {
     err = "dx, rest, dt and tol must be positive and mu non-negative";
     return false;
   }
+  if (!(c.dx <= 1024.0f)) { err = "dx must be at most 1024 (box side at most 2^23)"; return false; }
   if (c.fp64 > 1u) { err = "fp64 must be 0 or 1"; return false; }
   g.n = c.side * c.side;
   g.h = 2.0f * c.dx;
@@ -1470,6 +1556,27 @@ int mg_box_set_cg_prediction(mg_box* B, uint32_t iterations) {
   B->last_it = iterations;
   B->cg_stable = true;
   return MG_OK;
+}
+
+int mg_box_check_arith(uint64_t samples, uint64_t seed, uint64_t* mismatches2) {
+  if (!mismatches2 || samples == 0) return MG_E_ARG;
+  const uint32_t per_lane = 64;
+  const uint64_t lanes = (samples + per_lane - 1) / per_lane;
+  if (lanes > (1ull << 31)) return MG_E_ARG;
+  unsigned long long* d_bad = nullptr;
+  if (hipMalloc(&d_bad, 2 * sizeof(unsigned long long)) != hipSuccess) return MG_E_DEVICE;
+  bool ok = hipMemset(d_bad, 0, 2 * sizeof(unsigned long long)) == hipSuccess;
+  if (ok) {
+    hipLaunchKernelGGL(bx_check_arith, dim3((uint32_t)((lanes + BX_BLOCK - 1) / BX_BLOCK)), dim3(BX_BLOCK), 0, 0,
+                       seed, per_lane, d_bad);
+    unsigned long long h[2] = {0, 0};
+    ok = hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess &&
+         hipMemcpy(h, d_bad, sizeof h, hipMemcpyDeviceToHost) == hipSuccess;
+    mismatches2[0] = h[0];
+    mismatches2[1] = h[1];
+  }
+  (void)hipFree(d_bad);
+  return ok ? MG_OK : MG_E_DEVICE;
 }
 
 int64_t mg_box_neighbors(mg_box* B, uint32_t* off, uint32_t* nb, int64_t cap) {
