@@ -625,7 +625,8 @@ BXKRT bx_vel_density(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, 
 
 // Morris weight and unit direction of pair (i, j) (pbf.rs:312-344).
 // The quotient d / r is formed once: the kernel gradient (d / r) * coeff and the direction d / r
-// use the same rounded value, exactly as two separate divisions would give.
+// use the same rounded value, exactly as two separate divisions would give. pair_unit forms the
+// same direction alone (the CG recomputes it from the unchanged positions instead of storing it).
 template <class R>
 BDEV void morris(const BoxGeo<R>& g, Vec<R> xi, R rho_i, Vec<R> xj, R dens_j, R& w, Vec<R>& e) {
   const R rho_j = mfmax(dens_j, (R)1e-6f);
@@ -637,12 +638,15 @@ BDEV void morris(const BoxGeo<R>& g, Vec<R> xi, R rho_i, Vec<R> xj, R dens_j, R&
   w = num / (len2(d) + g.eta2);
   e = r > (R)1e-9f ? dr : mk((R)1, (R)0);
 }
+template <class R> BDEV Vec<R> pair_unit(Vec<R> d) {
+  const R r = pair_len(len2(d));
+  return r > (R)1e-9f ? pair_div(d, r) : mk((R)1, (R)0);
+}
 // (L v)_i with the weights computed here; when `store` the weights of a tabled particle are kept
 // (column-major, like the pair table) for the operator applications of this step's CG.
 template <class R>
 BDEV Vec<R> lap_row(const BoxGeo<R>& g, uint32_t i, const PairView<R>& pv, const Vec<R>* __restrict__ X,
-                    const R* __restrict__ DENS, const Vec<R>* __restrict__ in, bool store, R* __restrict__ CW,
-                    Vec<R>* __restrict__ CE) {
+                    const R* __restrict__ DENS, const Vec<R>* __restrict__ in, bool store, R* __restrict__ CW) {
   const Vec<R> xi = X[i], ini = in[i];
   const R rho_i = mfmax(DENS[i], (R)1e-6f);
   const bool keep = store && pv.cnt[i] <= BX_KMAX;
@@ -654,10 +658,7 @@ BDEV Vec<R> lap_row(const BoxGeo<R>& g, uint32_t i, const PairView<R>& pv, const
     R w;
     Vec<R> e;
     morris(g, xi, rho_i, a.x, a.d, w, e);
-    if (keep) {
-      CW[(size_t)k * g.n + i] = w;
-      CE[(size_t)k * g.n + i] = e;
-    }
+    if (keep) CW[(size_t)k * g.n + i] = w;
     k++;
     s = add2(s, smul2((R)4 * w * dot2(e, sub2(ini, a.v)), e));
   });
@@ -666,14 +667,14 @@ BDEV Vec<R> lap_row(const BoxGeo<R>& g, uint32_t i, const PairView<R>& pv, const
 
 BXKRT bx_cg_init(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const R* __restrict__ DENS,
                  const Vec<R>* __restrict__ V, Vec<R>* __restrict__ Rv, Vec<R>* __restrict__ P,
-                 R* __restrict__ CW, Vec<R>* __restrict__ CE, double* __restrict__ part_rr0,
+                 R* __restrict__ CW, double* __restrict__ part_rr0,
                  const BoxScal* __restrict__ sc) {
   if (*g.halt | sc->done) return;  // both words loaded at once
   const uint32_t i = tile_id_red(g) * BX_RBLOCK + threadIdx.x;
   double rr = 0.0;
   if (i < g.n) {
     const Vec<R> v = V[i];
-    const Vec<R> lv = lap_row(g, i, pv, X, DENS, V, true, CW, CE);
+    const Vec<R> lv = lap_row(g, i, pv, X, DENS, V, true, CW);
     const Vec<R> r = sub2(v, add2(v, mul2(lv, g.dt)));
     Rv[i] = r;
     P[i] = r;
@@ -687,9 +688,10 @@ BXKRT bx_cg_init(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, cons
 // sums of the previous kernels (same fixed order everywhere): for t = 1 the target tol·|b| and the
 // initial residual; for t > 1 the convergence test on r(t-1).r(t-1), the iteration cap, and
 // beta = rs(t-1)/rs(t-2). The direction p = r + beta p_old is formed on the fly for the particle
-// and its neighbours (no separate direction kernel), then Ap = p + dt L p with the kept weights.
+// and its neighbours (no separate direction kernel), then Ap = p + dt L p with the kept weights
+// and the unit directions recomputed from the positions (the same values the set-up used).
 BXKRT bx_cg_dir(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const R* __restrict__ DENS,
-                const R* __restrict__ CW, const Vec<R>* __restrict__ CE, const Vec<R>* __restrict__ Rv,
+                const R* __restrict__ CW, const Vec<R>* __restrict__ Rv,
                 const Vec<R>* __restrict__ Pold, Vec<R>* __restrict__ Pnew, Vec<R>* __restrict__ AP,
                 const double* __restrict__ part_b, const double* __restrict__ part_rr_prev,
                 const double* __restrict__ part_rr_prev2, double* __restrict__ part_pap, BoxScal* __restrict__ sc,
@@ -731,11 +733,13 @@ BXKRT bx_cg_dir(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const
     Vec<R> s = mk((R)0, (R)0);
     const uint32_t c = pv.cnt[i];
     if (c <= BX_KMAX) {
+      const Vec<R> xi = X[i];
       auto pdir = [&](uint32_t j) __attribute__((always_inline)) {
         return first ? Pold[j] : add2(Rv[j], mul2(Pold[j], beta));
       };
-      auto term = [&](size_t q, Vec<R> pj) __attribute__((always_inline)) {
-        s = add2(s, smul2((R)4 * CW[q] * dot2(CE[q], sub2(p, pj)), CE[q]));
+      auto term = [&](size_t q, Vec<R> pj, Vec<R> xj) __attribute__((always_inline)) {
+        const Vec<R> e = pair_unit<R>(rel(xi, xj, g));
+        s = add2(s, smul2((R)4 * CW[q] * dot2(e, sub2(p, pj)), e));
       };
       const size_t n = g.n;
       uint32_t k = 0;
@@ -744,14 +748,16 @@ BXKRT bx_cg_dir(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const
         const uint32_t j0 = nb_dec(pv.nb[q0], i, g.n), j1 = nb_dec(pv.nb[q1], i, g.n);
         const uint32_t j2 = nb_dec(pv.nb[q2], i, g.n), j3 = nb_dec(pv.nb[q3], i, g.n);
         const Vec<R> p0 = pdir(j0), p1 = pdir(j1), p2 = pdir(j2), p3 = pdir(j3);
-        term(q0, p0);
-        term(q1, p1);
-        term(q2, p2);
-        term(q3, p3);
+        const Vec<R> x0 = X[j0], x1 = X[j1], x2 = X[j2], x3 = X[j3];
+        term(q0, p0, x0);
+        term(q1, p1, x1);
+        term(q2, p2, x2);
+        term(q3, p3, x3);
       }
       for (; k < c; k++) {
         const size_t q = (size_t)k * n + i;
-        term(q, pdir(nb_dec(pv.nb[q], i, g.n)));
+        const uint32_t j = nb_dec(pv.nb[q], i, g.n);
+        term(q, pdir(j), X[j]);
       }
     } else {
       const Vec<R> xi = X[i];
@@ -925,7 +931,6 @@ struct mg_box {
   int16_t* pnb = nullptr;    // per-step pair table (16-bit relative offsets)
   uint32_t* pcnt = nullptr;  // and the per-particle pair counts
   void* cw = nullptr;        // per-step Morris weights of tabled pairs
-  void* ce = nullptr;        // and their unit directions
   double* part_b = nullptr;    // |b|^2 partials (one per reduction block)
   double* part_rr = nullptr;   // r.r partials, two generations: [0, nred) and [nred, 2 nred)
   double* part_pap = nullptr;  // p.Ap partials
@@ -1192,7 +1197,7 @@ template <class R> bool enqueue_front(mg_box* B) {
   if (!cg_off) {
     Tic t(B, BXT_CGINIT);
     hipLaunchKernelGGL(bx_cg_init<R>, gr, rblk, 0, st, g, pv, vp<R>(B->X), sp<R>(B->DENS), vp<R>(B->VN),
-                       vp<R>(B->R), vp<R>(B->P), sp<R>(B->cw), vp<R>(B->ce), B->part_rr, B->sc);
+                       vp<R>(B->R), vp<R>(B->P), sp<R>(B->cw), B->part_rr, B->sc);
   }
   B->cg_t = 1;
   B->have_neighbors = true;
@@ -1219,7 +1224,7 @@ template <class R> void enqueue_cg_chunk(mg_box* B, uint32_t iters, uint32_t* ve
     {
       Tic tt(B, BXT_SPMV);
       hipLaunchKernelGGL(bx_cg_dir<R>, gr, rblk, 0, st, g, pv, vp<R>(B->X), sp<R>(B->DENS), sp<R>(B->cw),
-                         vp<R>(B->ce), vp<R>(B->R), pold, pnew, vp<R>(B->AP), B->part_b, rr_prev, rr_prev2,
+                         vp<R>(B->R), pold, pnew, vp<R>(B->AP), B->part_b, rr_prev, rr_prev2,
                          B->part_pap, B->sc, t);
     }
     {
@@ -1443,7 +1448,7 @@ int mg_box_create(const mg_box_config* cfg, mg_box** out, char* err, size_t errl
   off = carve(off, B->part_b, (size_t)B->nred); off = carve(off, B->part_rr, 2 * (size_t)B->nred);
   off = carve(off, B->part_pap, (size_t)B->nred); off = carve_bytes(off, B->P2, vb);
   off = carve(off, B->pnb, (size_t)BX_KMAX * n); off = carve(off, B->pcnt, n);
-  off = carve_bytes(off, B->cw, (size_t)BX_KMAX * rs * n); off = carve_bytes(off, B->ce, (size_t)BX_KMAX * vb);
+  off = carve_bytes(off, B->cw, (size_t)BX_KMAX * rs * n);
   off = carve(off, B->verdict, BX_WINDOW + 1);
   off = carve(off, B->halt, 1); off = carve(off, B->sc, 1);
   if (hipMalloc(&B->arena, off) != hipSuccess) { box_set_err(err, errlen, "hipMalloc failed"); delete B; return MG_E_DEVICE; }
@@ -1452,7 +1457,7 @@ int mg_box_create(const mg_box_config* cfg, mg_box** out, char* err, size_t errl
   fix(B->X); fix(B->XP); fix(B->X0); fix(B->V); fix(B->X2); fix(B->Xb); fix(B->X0b); fix(B->Vb); fix(B->VN);
   fix(B->id); fix(B->idb); fix(B->key); fix(B->slot); fix(B->start); fix(B->cursor); fix(B->bsum);
   fix(B->LAM); fix(B->DENS); fix(B->R); fix(B->P); fix(B->AP); fix(B->part_b); fix(B->part_rr); fix(B->part_pap);
-  fix(B->P2); fix(B->sc); fix(B->pnb); fix(B->pcnt); fix(B->cw); fix(B->ce); fix(B->verdict); fix(B->halt);
+  fix(B->P2); fix(B->sc); fix(B->pnb); fix(B->pcnt); fix(B->cw); fix(B->verdict); fix(B->halt);
   B->gf.halt = B->halt;
   B->gd.halt = B->halt;
   bool ok = hipMemset(B->arena, 0, off) == hipSuccess;
