@@ -350,14 +350,6 @@ BDEV void for_pairs_g(const BoxGeo<R>& g, const PairView<R>& pv, uint32_t i, L&&
 // ---------------------------------------------------------------------------------------------
 // grid build + reorder
 // ---------------------------------------------------------------------------------------------
-BXK bx_zero2(const uint32_t* __restrict__ halt, uint32_t* __restrict__ a, uint32_t na, uint32_t* __restrict__ b,
-             uint32_t nb) {
-  if (*halt) return;
-  const uint32_t i = blockIdx.x * BX_BLOCK + threadIdx.x;
-  if (i < na) a[i] = 0u;
-  if (i < nb) b[i] = 0u;
-}
-
 // Lanes of one wave holding the same key form runs (the particles arrive in the previous step's
 // cell order, so most of a wave's keys repeat); one atomic per run instead of one per particle.
 // Returns the run's length on its head lane and 0 elsewhere; head = lane of the run's first entry.
@@ -393,7 +385,8 @@ BXKT bx_predict(BoxGeo<R> g, Vec<R>* __restrict__ X, const Vec<R>* __restrict__ 
   if (len && k != 0xFFFFFFFFu) atomicAdd(&count[k], len);
 }
 
-// Exclusive scan of a[0..m) in three launches: chunk totals, scan of the totals, chunk scan.
+// Exclusive scan of a[0..m) in two launches: chunk totals, then each chunk's scan offset by the sum
+// of the totals before it (at most BX_SCAN_CHUNK chunks).
 BDEV uint32_t block_excl_scan(uint32_t v, uint32_t& total) {
   __shared__ uint32_t s_w[BX_BLOCK / 64];
   const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -424,28 +417,18 @@ BXK bx_scan_totals(const uint32_t* __restrict__ halt, const uint32_t* __restrict
   block_excl_scan(s, tot);
   if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
-BXK bx_scan_top(const uint32_t* __restrict__ halt, uint32_t* __restrict__ bsum, uint32_t nb) {  // one block
-  if (*halt) return;
-  const uint32_t base = threadIdx.x * BX_SCAN_ITEMS;
-  uint32_t s = 0;
-  for (int k = 0; k < BX_SCAN_ITEMS; k++) s += base + k < nb ? bsum[base + k] : 0u;
-  uint32_t tot;
-  uint32_t off = block_excl_scan(s, tot);
-  for (int k = 0; k < BX_SCAN_ITEMS; k++)
-    if (base + k < nb) {
-      uint32_t t = bsum[base + k];
-      bsum[base + k] = off;
-      off += t;
-    }
-}
 BXK bx_scan_apply(const uint32_t* __restrict__ halt, uint32_t* __restrict__ a, uint32_t m,
                   const uint32_t* __restrict__ bsum) {
   if (*halt) return;
+  uint32_t before = 0;  // totals of the chunks before this one
+  for (uint32_t k = threadIdx.x; k < blockIdx.x; k += BX_BLOCK) before += bsum[k];
+  uint32_t chunks_before;
+  block_excl_scan(before, chunks_before);
   const uint32_t base = blockIdx.x * BX_SCAN_CHUNK + threadIdx.x * BX_SCAN_ITEMS;
   uint32_t s = 0;
   for (int k = 0; k < BX_SCAN_ITEMS; k++) s += base + k < m ? a[base + k] : 0u;
   uint32_t tot;
-  uint32_t off = block_excl_scan(s, tot) + bsum[blockIdx.x];
+  uint32_t off = block_excl_scan(s, tot) + chunks_before;
   for (int k = 0; k < BX_SCAN_ITEMS; k++)
     if (base + k < m) {
       uint32_t t = a[base + k];
@@ -493,13 +476,17 @@ BXK bx_cellsort(const uint32_t* __restrict__ halt, uint32_t ncell, const uint32_
   }
 }
 // Moves every particle to its sorted position (one lane per destination: coalesced writes); x is
-// also written to the predicted copy XP.
+// also written to the predicted copy XP. It also clears the next step's cell counts (the other
+// cell-start buffer, whose last user was the previous step) and the scatter cursors.
 BXKT bx_gather(BoxGeo<R> g, const uint32_t* __restrict__ slot, const Vec<R>* __restrict__ X,
                const Vec<R>* __restrict__ X0, const Vec<R>* __restrict__ V, const uint32_t* __restrict__ id,
                Vec<R>* __restrict__ Xb, Vec<R>* __restrict__ XPb, Vec<R>* __restrict__ X0b, Vec<R>* __restrict__ Vb,
-               uint32_t* __restrict__ idb) {
+               uint32_t* __restrict__ idb, uint32_t* __restrict__ count_next, uint32_t* __restrict__ cursor,
+               uint32_t ncell) {
   if (*g.halt) return;
   const uint32_t q = blockIdx.x * BX_BLOCK + threadIdx.x;
+  if (q <= ncell) count_next[q] = 0u;  // ncell + 1 <= n / 4 + 1 <= n
+  if (q < ncell) cursor[q] = 0u;
   if (q >= g.n) return;
   const uint32_t i = slot[q];
   const Vec<R> x = X[i];
@@ -924,8 +911,9 @@ struct mg_box {
   void *X = nullptr, *XP = nullptr, *X0 = nullptr, *V = nullptr, *X2 = nullptr;
   void *Xb = nullptr, *X0b = nullptr, *Vb = nullptr;
   void* VN = nullptr;  // the step's new velocity (velocity update + CG); swapped with V at step end
-  uint32_t *id = nullptr, *idb = nullptr, *key = nullptr, *slot = nullptr, *start = nullptr, *cursor = nullptr,
-           *bsum = nullptr;
+  uint32_t *id = nullptr, *idb = nullptr, *key = nullptr, *slot = nullptr, *cursor = nullptr, *bsum = nullptr;
+  // cell starts of the current step and the buffer the current step clears for the next one
+  uint32_t *start = nullptr, *start_next = nullptr;
   void *LAM = nullptr, *DENS = nullptr;
   void *R = nullptr, *P = nullptr, *AP = nullptr;
   int16_t* pnb = nullptr;    // per-step pair table (16-bit relative offsets)
@@ -947,7 +935,7 @@ struct mg_box {
   // multi-step windows: the device verdict [first overrun step, iterations of step 0..W-1] and its
   // pinned host copy; where each step's start state lives (see run_window)
   bool cg_stable = false;  // the last window's steps all took the same CG iteration count
-  struct Start { void* x; void* v; uint32_t* id; };
+  struct Start { void* x; void* v; uint32_t* id; uint32_t* cells; };
   Start starts[BX_WINDOW];
   uint32_t* verdict = nullptr;
   uint32_t* h_verdict = nullptr;
@@ -1150,19 +1138,18 @@ template <class R> bool enqueue_front(mg_box* B) {
   const uint32_t cg_off = (B->cfg.mu > 0.0f && B->cfg.max_cg > 0) ? 0u : 1u;
   hipStream_t st = B->stream;
   const dim3 blk(BX_BLOCK), gp = blocks_for(g.n);
+  std::swap(B->start, B->start_next);  // counts start from the buffer the previous step cleared
   {
     Tic t(B, BXT_GRID);
-    hipLaunchKernelGGL(bx_zero2, blocks_for(B->ncell + 1), blk, 0, st, B->halt, B->start, B->ncell + 1, B->cursor,
-                       B->ncell);
     hipLaunchKernelGGL(bx_predict<R>, gp, blk, 0, st, g, vp<R>(B->X), vp<R>(B->V), vp<R>(B->X0), B->key, B->start);
     hipLaunchKernelGGL(bx_scan_totals, dim3(B->nscan), blk, 0, st, B->halt, B->start, B->ncell + 1, B->bsum);
-    hipLaunchKernelGGL(bx_scan_top, dim3(1), blk, 0, st, B->halt, B->bsum, B->nscan);
     hipLaunchKernelGGL(bx_scan_apply, dim3(B->nscan), blk, 0, st, B->halt, B->start, B->ncell + 1, B->bsum);
     hipLaunchKernelGGL(bx_scatter, gp, blk, 0, st, B->halt, g.n, B->key, B->start, B->cursor, B->slot);
     hipLaunchKernelGGL(bx_cellsort, blocks_for(B->ncell), blk, 0, st, B->halt, B->ncell, B->start, B->slot, B->id,
                        B->sc, cg_off, B->cfg.max_cg, B->cfg.tol);
     hipLaunchKernelGGL(bx_gather<R>, gp, blk, 0, st, g, B->slot, vp<R>(B->X), vp<R>(B->X0), vp<R>(B->V), B->id,
-                       vp<R>(B->Xb), vp<R>(B->XP), vp<R>(B->X0b), vp<R>(B->Vb), B->idb);
+                       vp<R>(B->Xb), vp<R>(B->XP), vp<R>(B->X0b), vp<R>(B->Vb), B->idb, B->start_next, B->cursor,
+                       B->ncell);
   }
   std::swap(B->X, B->Xb);
   std::swap(B->X0, B->X0b);
@@ -1303,7 +1290,7 @@ template <class R> int run_window(mg_box* B, uint32_t steps) {
     if (cg_on) hipLaunchKernelGGL(bx_verdict_reset, dim3(1), dim3(64), 0, B->stream, B->verdict, B->halt);
     for (uint32_t s = 0; s < W; s++) {
       if (!enqueue_front<R>(B)) { B->err = "kernel launch failed"; return MG_E_DEVICE; }
-      B->starts[s] = {B->X0, B->V, B->id};
+      B->starts[s] = {B->X0, B->V, B->id, B->start};
       if (cg_on) enqueue_cg_chunk<R>(B, pred, B->verdict, s);
       std::swap(B->V, B->VN);
     }
@@ -1339,6 +1326,10 @@ template <class R> int run_window(mg_box* B, uint32_t steps) {
     take_buffer(st.x, pos);
     take_buffer(st.v, vel);
     take_buffer(st.id, ids);
+    // step b's counts were accumulated into st.cells and its gather cleared the other buffer: the
+    // re-run (which swaps first) counts into the cleared one and clears st.cells for its successor
+    uint32_t** const cells[] = {&B->start, &B->start_next};
+    take_buffer(st.cells, cells);
     hipLaunchKernelGGL(bx_verdict_reset, dim3(1), dim3(64), 0, B->stream, B->verdict, B->halt);
     const int rc = run_step<R>(B, pred + BX_CG_CHUNK);
     if (rc) return rc;
@@ -1441,7 +1432,8 @@ int mg_box_create(const mg_box_config* cfg, mg_box** out, char* err, size_t errl
   off = carve_bytes(off, B->V, vb); off = carve_bytes(off, B->X2, vb); off = carve_bytes(off, B->Xb, vb);
   off = carve_bytes(off, B->X0b, vb); off = carve_bytes(off, B->Vb, vb); off = carve_bytes(off, B->VN, vb);
   off = carve(off, B->id, n); off = carve(off, B->idb, n); off = carve(off, B->key, n); off = carve(off, B->slot, n);
-  off = carve(off, B->start, (size_t)B->ncell + 1); off = carve(off, B->cursor, (size_t)B->ncell);
+  off = carve(off, B->start, (size_t)B->ncell + 1); off = carve(off, B->start_next, (size_t)B->ncell + 1);
+  off = carve(off, B->cursor, (size_t)B->ncell);
   off = carve(off, B->bsum, (size_t)B->nscan);
   off = carve_bytes(off, B->LAM, rs * n); off = carve_bytes(off, B->DENS, rs * n);
   off = carve_bytes(off, B->R, vb); off = carve_bytes(off, B->P, vb); off = carve_bytes(off, B->AP, vb);
@@ -1455,7 +1447,7 @@ int mg_box_create(const mg_box_config* cfg, mg_box** out, char* err, size_t errl
   const uintptr_t base = (uintptr_t)B->arena;
   auto fix = [&](auto*& p) { p = reinterpret_cast<std::remove_reference_t<decltype(p)>>(base + (uintptr_t)p); };
   fix(B->X); fix(B->XP); fix(B->X0); fix(B->V); fix(B->X2); fix(B->Xb); fix(B->X0b); fix(B->Vb); fix(B->VN);
-  fix(B->id); fix(B->idb); fix(B->key); fix(B->slot); fix(B->start); fix(B->cursor); fix(B->bsum);
+  fix(B->id); fix(B->idb); fix(B->key); fix(B->slot); fix(B->start); fix(B->start_next); fix(B->cursor); fix(B->bsum);
   fix(B->LAM); fix(B->DENS); fix(B->R); fix(B->P); fix(B->AP); fix(B->part_b); fix(B->part_rr); fix(B->part_pap);
   fix(B->P2); fix(B->sc); fix(B->pnb); fix(B->pcnt); fix(B->cw); fix(B->verdict); fix(B->halt);
   B->gf.halt = B->halt;

@@ -113,6 +113,9 @@ def test_multi_step_windows_match_oracle(box_mod, orc, side, mu, steps):
     o.step(steps)
     x, v = b.state()
     xo, vo = o.state()
+    off, nb = b.neighbors()
+    ooff, onb = o.neighbors()
+    assert np.array_equal(off, ooff) and np.array_equal(nb, onb)
     assert np.array_equal(x.astype(np.float64), xo)
     assert np.array_equal(b.density().astype(np.float64), o.density())
     assert np.abs(v - vo).max() / max(np.abs(vo).max(), 1e-30) <= V_TOL
@@ -133,6 +136,9 @@ def test_window_rerun_after_low_prediction_matches_oracle(box_mod, orc):
     o.step(6)
     x, v = b.state()
     xo, vo = o.state()
+    off, nb = b.neighbors()
+    ooff, onb = o.neighbors()
+    assert np.array_equal(off, ooff) and np.array_equal(nb, onb)
     assert np.array_equal(x.astype(np.float64), xo)
     assert np.abs(v - vo).max() / max(np.abs(vo).max(), 1e-30) <= V_TOL
     assert b.stats()["cg_iterations"] == o.stats()["cg_iterations"]
