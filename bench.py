@@ -13,7 +13,9 @@ particle-steps/s and as algorithmic HBM GB/s from the canonical per-particle byt
 (B = 226 + 32 n_d + 84 n_cg bytes per particle per step).
 
 roofline: the dominant kernel by total time, its algorithmic bytes per launch divided by its
-average launch time from HIP events on the box's stream over the timed steps.
+average launch time from HIP events on the box's stream over the timed steps; `traffic` (measured
+HBM bytes per launch) and `valu_busy` come from the committed rocprofv3 summaries in profiles/
+(the density passes are VALU-bound: their HBM fraction is low by nature).
 cpu_baseline: the C++ restatement of the reference (oracle/, 1 core) on a bounded sample: C2 drum
 substeps (headline unit) and, inside kernel_isolation, box steps.
 """
@@ -42,6 +44,7 @@ CLASS_BYTES = [138.0, 12.0, 20.0, 28.0, 52.0, 44.0, 40.0, 12.0]
 CLASS_KERNELS = [None, "bx_lambda<float>", "bx_dp<float>", "bx_vel_density<float>", "bx_cg_init<float>",
                  "bx_cg_dir<float>", "bx_cg_upd<float>", "bx_pairs_lambda<float>"]
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "box_1024_pmc_traffic.md")
+VALU_PROFILE = os.path.join(ROOT, "profiles", "box_1024_pmc_valu.md")
 C2_PARAMS = {"mill": {"diameter_m": 0.1397, "speed_mode": "percent_critical", "speed_value": 75.0},
              "media": {"density_kg_m3": 7800.0},
              "simulation": {"resolution": 100, "max_balls": 2000}}
@@ -57,6 +60,20 @@ def measured_traffic(kernel, side):
         if len(cells) == 5 and cells[0] == kernel:
             try:
                 return float(cells[4]) * 1024 * 1024
+            except ValueError:
+                return None
+    return None
+
+
+def measured_valu_busy(kernel, side):
+    """VALU busy fraction of `kernel` from the committed rocprofv3 SQ/GRBM summary, or None."""
+    if kernel is None or side != 1024 or not os.path.exists(VALU_PROFILE):
+        return None
+    for line in open(VALU_PROFILE):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 4 and cells[0] == kernel:
+            try:
+                return round(float(cells[3]) / 100.0, 3)
             except ValueError:
                 return None
     return None
@@ -194,6 +211,7 @@ def main() -> int:
     roofline = {"bound": "hbm", "kernel": CLASS_KERNELS[k], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": measured_traffic(CLASS_KERNELS[k], args.side),
+                "valu_busy": measured_valu_busy(CLASS_KERNELS[k], args.side),
                 "bytes_per_launch": CLASS_BYTES[k] * n, "avg_launch_us": round(1000.0 * avg_launch_ms, 2),
                 "launches": int(launches_k[k])}
     iso = {"metric": "particle-steps/s", "value": round(n * args.steps / wall, 1), "ms_per_step": round(ms_per_step, 4),
