@@ -153,7 +153,7 @@ X64_TOL = 1e-12
 V64_TOL = 1e-10
 
 
-def _compare64(b, o, label):
+def _compare64(b, o, label, xtol=X64_TOL):
     x, v = b.state()
     xo, vo = o.state()
     assert x.dtype == np.float64 and v.dtype == np.float64
@@ -161,9 +161,9 @@ def _compare64(b, o, label):
     ooff, onb = o.neighbors()
     assert np.array_equal(off, ooff) and np.array_equal(nb, onb), f"{label}: neighbour lists differ"
     L = o.cfg["side"] * o.cfg["dx"]
-    assert np.abs(x - xo).max() <= X64_TOL * L, f"{label}: positions differ (max {np.abs(x - xo).max()})"
+    assert np.abs(x - xo).max() <= xtol * L, f"{label}: positions differ (max {np.abs(x - xo).max()})"
     d, do = b.density(), o.density()
-    assert np.abs(d - do).max() <= X64_TOL * do.max(), f"{label}: densities differ"
+    assert np.abs(d - do).max() <= xtol * do.max(), f"{label}: densities differ"
     vmax = max(np.abs(vo).max(), 1e-30)
     assert np.abs(v - vo).max() / vmax <= V64_TOL, f"{label}: velocities differ"
     assert b.stats()["cg_iterations"] == o.stats()["cg_iterations"], f"{label}: CG iterations differ"
@@ -220,3 +220,34 @@ def test_pair_arith_matches_ieee():
     out = (C.c_uint64 * 2)()
     assert L.mg_box_check_arith(1 << 28, 20261203, out) == 0
     assert (out[0], out[1]) == (0, 0), f"mismatches: sqrt {out[0]}, quotient {out[1]}"
+
+
+@pytest.mark.parametrize("fp64", [0, 1])
+def test_dense_cluster_rescans_match_oracle(box_mod, orc, fp64):
+    # 300 particles packed at 0.35 dx: up to ~114 pairs each, so the pair table overflows (more than
+    # 32 pairs) and every kernel takes its rescan path for them, the CG's operator included
+    side, dx = 32, 1.0e-3
+    cfg = {"side": side, "mu": 1.0, "fp64": fp64}
+    b = box_mod.Box(cfg)
+    o = orc.C3Oracle(cfg, "fp64" if fp64 else "mixed")
+    x, v = o.state()
+    k = np.arange(300)
+    x[:300, 0] = side * dx / 2 + (k % 17 - 8) * 0.35 * dx
+    x[:300, 1] = side * dx / 2 + (k // 17 - 9) * 0.35 * dx
+    x = x.astype(np.float32).astype(np.float64)
+    v = v.astype(np.float32).astype(np.float64)
+    b.set_state(x, v)
+    o.set_state(x, v)
+    b.step()
+    o.step()
+    assert np.diff(o.neighbors()[0]).max() > 32
+    if fp64:
+        _compare64(b, o, "dense cluster fp64")
+    else:
+        _compare(b, o, "dense cluster")
+    b.step()
+    o.step()
+    if fp64:  # the blob bursts (|v| ~ 10 m/s, 1000x the lattice): reduction-order differences in the CG move x more
+        _compare64(b, o, "dense cluster fp64 step 2", xtol=1e-10)
+    else:
+        _compare(b, o, "dense cluster step 2")
