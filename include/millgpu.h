@@ -177,11 +177,12 @@ uint64_t mg_box_total_cg_iterations(const mg_box* box);
  * and the next window is half as long. This sets the count the next window assumes and marks the
  * counts as steady (a small value exercises the re-run path). */
 int mg_box_set_cg_prediction(mg_box* box, uint32_t iterations);
-/* Self-check of the f32 square root and the quotient by r that the box's pair loops use (the
+/* Self-check of the square root and the quotient by r that the box's pair loops use (the
  * correctly rounded sequences without range scaling, valid in the pair loops' operand range)
  * against the IEEE operations, on `samples` seeded random operands covering that range and its
- * edges. mismatches2: {square root, quotient}; both must be 0. No box needed. */
-int mg_box_check_arith(uint64_t samples, uint64_t seed, uint64_t* mismatches2);
+ * edges. mismatches4: {f32 square root, f32 quotient, f64 square root, f64 quotient}; all must
+ * be 0. No box needed. */
+int mg_box_check_arith(uint64_t samples, uint64_t seed, uint64_t* mismatches4);
 /* Neighbour CSR of the last step in traversal order (original ids); returns the edge count,
  * copying the entries only when they fit `cap`. */
 int64_t mg_box_neighbors(mg_box* box, uint32_t* off_n1, uint32_t* nb, int64_t cap);

@@ -119,17 +119,17 @@ BDEV double mfmax(double a, double b) { return fmax(a, b); }
 BX_VEC_OPS(float2, float, make_float2)
 BX_VEC_OPS(double2, double, make_double2)
 #undef BX_VEC_OPS
-// |d| and d / |d| of a pair vector. In f32 these are the compiler's correctly rounded sequences
-// without their range scaling and special-value handling, which are the identity in the range the
-// pair loops use them in, so the results are bit for bit those of sqrtf and '/', and the two
-// divisions by r share one refined reciprocal:
-//  - pair_len(r2) equals sqrtf(r2) for r2 >= 2^-96. Below that it returns a value < 1e-9, and
-//    every caller only compares such an r with 1e-9 (the kernel gradient and the Morris direction
-//    are zero / fixed there), so results do not depend on it.
+// |d| and d / |d| of a pair vector: the compiler's correctly rounded square root and division
+// sequences without their range scaling and special-value handling, which are the identity in the
+// range the pair loops use them in, so the results are bit for bit those of sqrt and '/', and the
+// two divisions by r share one refined reciprocal:
+//  - pair_len(r2) equals sqrt(r2) for r2 >= 2^-96 (f32) / 2^-767 (f64). Below that it returns a
+//    value < 1e-9, and every caller only compares such an r with 1e-9 (the kernel gradient and the
+//    Morris direction are zero / fixed there), so results do not depend on it.
 //  - pair_div(d, r) equals (d.x / r, d.y / r) for r in [2^-100, 2^30] (the box side is at most
-//    2^23, so every r > 1e-9 is in range) when both |components| are >= 2^-96; otherwise it takes
-//    the IEEE division. Callers use the quotient only for r > 1e-9.
-// mg_box_check_arith compares both with the IEEE operations on the GPU.
+//    2^23, so every r > 1e-9 is in range) when both |components| are >= 2^-96 (f32) / 2^-900
+//    (f64); otherwise it takes the IEEE division. Callers use the quotient only for r > 1e-9.
+// mg_box_check_arith compares them with the IEEE operations on the GPU.
 template <class R> BDEV R pair_len(R r2) { return msqrt(r2); }
 template <class R> BDEV Vec<R> pair_div(Vec<R> d, R r) { return div2(d, r); }
 BDEV float sqrt_core(float x) {  // v_sqrt (1 ulp) and the two neighbouring candidates, as LLVM lowers sqrtf
@@ -142,6 +142,35 @@ BDEV float sqrt_core(float x) {  // v_sqrt (1 ulp) and the two neighbouring cand
   return rp > 0.0f ? sp : t;
 }
 template <> BDEVX float pair_len<float>(float r2) { return sqrt_core(r2); }
+// f64: v_rsq and two coupled Newton steps on (x·y, y/2), then two residual corrections (LLVM's
+// f64 sqrt lowering without its 2^256 scaling below 2^-767)
+BDEV double sqrt_core(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y;
+  double h = y * 0.5;
+  const double r = __builtin_fma(-h, g, 0.5);
+  g = __builtin_fma(g, r, g);
+  double d = __builtin_fma(-g, g, x);
+  h = __builtin_fma(h, r, h);
+  g = __builtin_fma(d, h, g);
+  d = __builtin_fma(-g, g, x);
+  return __builtin_fma(d, h, g);
+}
+template <> BDEVX double pair_len<double>(double r2) { return sqrt_core(r2); }
+template <> BDEVX double2 pair_div<double>(double2 d, double r) {
+  const bool tiny = (fabs(d.x) < 0x1p-900) | (fabs(d.y) < 0x1p-900);
+  if (__builtin_expect(tiny, 0)) return make_double2(d.x / r, d.y / r);
+  // reciprocal refined twice, then one residual correction per quotient (the unscaled path of
+  // the compiler's f64 division)
+  const double y0 = __builtin_amdgcn_rcp(r);
+  const double y1 = __builtin_fma(y0, __builtin_fma(-r, y0, 1.0), y0);
+  const double y = __builtin_fma(y1, __builtin_fma(-r, y1, 1.0), y1);
+  auto q = [&](double a) __attribute__((always_inline)) {
+    const double q0 = a * y;
+    return __builtin_fma(__builtin_fma(-r, q0, a), y, q0);
+  };
+  return make_double2(q(d.x), q(d.y));
+}
 template <> BDEVX float2 pair_div<float>(float2 d, float r) {
   const bool tiny = (fabsf(d.x) < 0x1p-96f) | (fabsf(d.y) < 0x1p-96f);
   if (__builtin_expect(tiny, 0)) return make_float2(d.x / r, d.y / r);
@@ -874,9 +903,18 @@ BDEV float draw_float(uint64_t& s, int e0, int e1) {
   if (sel == 1u) m = 0u;
   return __uint_as_float((uint32_t)(e + 127) << 23 | m);
 }
+BDEV double draw_double(uint64_t& s, int e0, int e1) {
+  const uint64_t z = mix64(s), w = mix64(s);
+  const int e = e0 + (int)((z >> 32) % (uint64_t)(e1 - e0 + 1));
+  uint64_t m = w & 0xFFFFFFFFFFFFFull;
+  const uint32_t sel = (uint32_t)z & 7u;
+  if (sel == 0u) m = 0xFFFFFFFFFFFFFull;
+  if (sel == 1u) m = 0ull;
+  return __longlong_as_double((long long)((uint64_t)(e + 1023) << 52 | m));
+}
 BXK bx_check_arith(uint64_t seed, uint32_t per_lane, unsigned long long* __restrict__ bad) {
   uint64_t s = seed ^ ((uint64_t)(blockIdx.x * BX_BLOCK + threadIdx.x) * 0xD1B54A32D192ED03ull);
-  unsigned long long bad_sqrt = 0, bad_div = 0;
+  unsigned long long bad_sqrt = 0, bad_div = 0, bad_sqrt64 = 0, bad_div64 = 0;
   for (uint32_t k = 0; k < per_lane; k++) {
     // square root over [2^-96, 2^60], and below 2^-96 the promise that the value stays < 1e-9
     const float x = draw_float(s, -96, 60);
@@ -893,9 +931,26 @@ BXK bx_check_arith(uint64_t seed, uint32_t per_lane, unsigned long long* __restr
     const float2 q = pair_div<float>(d, r);
     if (__float_as_uint(q.x) != __float_as_uint(d.x / r) || __float_as_uint(q.y) != __float_as_uint(d.y / r))
       bad_div++;
+    // the same in f64: square root over [2^-767, 2^60], quotients with |d| down to 2^-940 r
+    const double x64 = draw_double(s, -767, 60);
+    if (__double_as_longlong(pair_len<double>(x64)) != __double_as_longlong(sqrt(x64))) bad_sqrt64++;
+    const double xt64 = draw_double(s, -1022, -768);
+    if (!(pair_len<double>(xt64) < 1e-9)) bad_sqrt64++;
+    const double r64 = draw_double(s, -30, 29);
+    const uint64_t z64 = mix64(s);
+    double vx = draw_double(s, -940, -1), vy = draw_double(s, -60, -1);
+    if ((z64 & 15u) == 0u) vx = 1.0;
+    if ((z64 & 15u) == 1u) vy = 0.0;
+    const double2 d64 = make_double2((z64 & 16u) ? -r64 * vx : r64 * vx, (z64 & 32u) ? -r64 * vy : r64 * vy);
+    const double2 q64 = pair_div<double>(d64, r64);
+    if (__double_as_longlong(q64.x) != __double_as_longlong(d64.x / r64) ||
+        __double_as_longlong(q64.y) != __double_as_longlong(d64.y / r64))
+      bad_div64++;
   }
   if (bad_sqrt) atomicAdd(&bad[0], bad_sqrt);
   if (bad_div) atomicAdd(&bad[1], bad_div);
+  if (bad_sqrt64) atomicAdd(&bad[2], bad_sqrt64);
+  if (bad_div64) atomicAdd(&bad[3], bad_div64);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1555,22 +1610,21 @@ int mg_box_set_cg_prediction(mg_box* B, uint32_t iterations) {
   return MG_OK;
 }
 
-int mg_box_check_arith(uint64_t samples, uint64_t seed, uint64_t* mismatches2) {
-  if (!mismatches2 || samples == 0) return MG_E_ARG;
+int mg_box_check_arith(uint64_t samples, uint64_t seed, uint64_t* mismatches4) {
+  if (!mismatches4 || samples == 0) return MG_E_ARG;
   const uint32_t per_lane = 64;
   const uint64_t lanes = (samples + per_lane - 1) / per_lane;
   if (lanes > (1ull << 31)) return MG_E_ARG;
   unsigned long long* d_bad = nullptr;
-  if (hipMalloc(&d_bad, 2 * sizeof(unsigned long long)) != hipSuccess) return MG_E_DEVICE;
-  bool ok = hipMemset(d_bad, 0, 2 * sizeof(unsigned long long)) == hipSuccess;
+  if (hipMalloc(&d_bad, 4 * sizeof(unsigned long long)) != hipSuccess) return MG_E_DEVICE;
+  bool ok = hipMemset(d_bad, 0, 4 * sizeof(unsigned long long)) == hipSuccess;
   if (ok) {
     hipLaunchKernelGGL(bx_check_arith, dim3((uint32_t)((lanes + BX_BLOCK - 1) / BX_BLOCK)), dim3(BX_BLOCK), 0, 0,
                        seed, per_lane, d_bad);
-    unsigned long long h[2] = {0, 0};
+    unsigned long long h[4] = {0, 0, 0, 0};
     ok = hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess &&
          hipMemcpy(h, d_bad, sizeof h, hipMemcpyDeviceToHost) == hipSuccess;
-    mismatches2[0] = h[0];
-    mismatches2[1] = h[1];
+    for (int k = 0; k < 4; k++) mismatches4[k] = h[k];
   }
   (void)hipFree(d_bad);
   return ok ? MG_OK : MG_E_DEVICE;

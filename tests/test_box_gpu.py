@@ -211,15 +211,15 @@ def test_fp64_full_size_million_particles(box_mod, orc):
 
 
 def test_pair_arith_matches_ieee():
-    # the f32 square root and quotient of the pair loops (core sequences without range scaling,
-    # a shared reciprocal) against sqrtf and '/' on 2^28 random operands of the pair loops' range
+    # the f32 and f64 square root and quotient of the pair loops (core sequences without range
+    # scaling, a shared reciprocal) against sqrt and '/' on 2^28 random operands of their range
     import ctypes as C
     from milldynamics3_amd import sim
     L = sim.lib()
     L.mg_box_check_arith.argtypes = [C.c_uint64, C.c_uint64, C.POINTER(C.c_uint64)]
-    out = (C.c_uint64 * 2)()
+    out = (C.c_uint64 * 4)()
     assert L.mg_box_check_arith(1 << 28, 20261203, out) == 0
-    assert (out[0], out[1]) == (0, 0), f"mismatches: sqrt {out[0]}, quotient {out[1]}"
+    assert list(out) == [0, 0, 0, 0], f"mismatches (f32 sqrt, f32 quotient, f64 sqrt, f64 quotient): {list(out)}"
 
 
 @pytest.mark.parametrize("fp64", [0, 1])
