@@ -113,6 +113,31 @@ def cpu_baseline_box(cfg: dict, seconds: float) -> dict:
                       f"1 core, {dt:.1f} s"}
 
 
+def _box_worker(cfg, seconds):
+    from oracle import oracle as orc
+    s = orc.C3Oracle(cfg, "mixed")
+    n = 0
+    t0 = time.perf_counter()
+    while n == 0 or time.perf_counter() - t0 < seconds:
+        s.step()
+        n += 1
+    return n, time.perf_counter() - t0
+
+
+def cpu_baseline_box_cores(cfg: dict, seconds: float, cores: int) -> dict:
+    """The same restatement on `cores` host cores at once, one independent box per core (the
+    path shards that way across cores; the restatement itself is single-threaded)."""
+    import concurrent.futures as cf
+    import multiprocessing as mp
+    with cf.ProcessPoolExecutor(max_workers=cores, mp_context=mp.get_context("spawn")) as ex:
+        res = list(ex.map(_box_worker, [cfg] * cores, [seconds] * cores))
+    npart = cfg["side"] ** 2
+    value = sum(n * npart / dt for n, dt in res)
+    return {"value": value, "unit": "particle-steps/s", "cores": cores, "kind": "port",
+            "sample": f"{cores} independent {cfg['side']}^2 boxes on the C++ restatement, one per core, "
+                      f"{sum(n for n, _ in res)} steps in {max(dt for _, dt in res):.1f} s"}
+
+
 def bench_fp64(args, cfg) -> dict:
     """The same workload with FP64 state and arithmetic: K steps in one call after the warm-up."""
     import torch
@@ -226,6 +251,9 @@ def main() -> int:
     if not args.no_cpu:
         headline_cpu = cpu_baseline_drum(args.cpu_seconds)
         iso["cpu_baseline"] = cpu_baseline_box(cfg, args.cpu_seconds)
+        cores = min(16, os.cpu_count() or 1)  # a one-GPU box's share of the host
+        if cores > 1:
+            iso["cpu_baseline_all_cores"] = cpu_baseline_box_cores(cfg, min(args.cpu_seconds, 5.0), cores)
     line = {
         "metric": METRIC,
         "value": None,
