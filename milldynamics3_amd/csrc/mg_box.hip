@@ -128,7 +128,8 @@ BX_VEC_OPS(double2, double, make_double2)
 //    Morris direction are zero / fixed there), so results do not depend on it.
 //  - pair_div(d, r) equals (d.x / r, d.y / r) for r in [2^-100, 2^30] (the box side is at most
 //    2^23, so every r > 1e-9 is in range) when both |components| are >= 2^-96 (f32) / 2^-900
-//    (f64); otherwise it takes the IEEE division. Callers use the quotient only for r > 1e-9.
+//    (f64); it takes the IEEE division for smaller ones (in f32 for every component whose square
+//    is zero, i.e. below about 2^-75). Callers use the quotient only for r > 1e-9.
 // mg_box_check_arith compares them with the IEEE operations on the GPU.
 template <class R> BDEV R pair_len(R r2) { return msqrt(r2); }
 template <class R> BDEV Vec<R> pair_div(Vec<R> d, R r) { return div2(d, r); }
@@ -172,7 +173,9 @@ template <> BDEVX double2 pair_div<double>(double2 d, double r) {
   return make_double2(q(d.x), q(d.y));
 }
 template <> BDEVX float2 pair_div<float>(float2 d, float r) {
-  const bool tiny = (fabsf(d.x) < 0x1p-96f) | (fabsf(d.y) < 0x1p-96f);
+  // a component below 2^-75 squares to zero (the same products len2 formed): take the IEEE division
+  // for those, which covers every component below the core sequence's 2^-96 bound
+  const bool tiny = (d.x * d.x == 0.0f) | (d.y * d.y == 0.0f);
   if (__builtin_expect(tiny, 0)) return make_float2(d.x / r, d.y / r);
   // Newton-refined reciprocal, then two residual corrections per quotient (the unscaled path of
   // the compiler's division: rcp, fma, fma | mul, fma, fma, fma, fma)
@@ -271,8 +274,8 @@ BDEV double sum_parts(const double* __restrict__ part, uint32_t nb) {
 // reference's per-particle order, column-major (nb[k * n + i]) so lane-adjacent particles read
 // adjacent entries. An entry is the signed 16-bit offset j - i taken modulo n into [-n/2, n/2]
 // (after the cell sort a neighbour sits within about two cell rows of i, or as far on the other
-// side of the periodic seam). cnt[i] is the true count; a particle with more than BX_KMAX pairs, or
-// with a neighbour whose offset does not fit 16 bits, has cnt[i] > BX_KMAX and is handled by
+// side of the periodic y seam). cnt[i] is the true count; a particle with more than BX_KMAX pairs,
+// or with a neighbour whose offset does not fit 16 bits, has cnt[i] > BX_KMAX and is handled by
 // rescanning its 3x3 cells (same order, same membership test).
 template <class R> struct PairView {
   const Vec<R>* __restrict__ XP;
@@ -280,11 +283,13 @@ template <class R> struct PairView {
   const int16_t* __restrict__ nb;
   const uint32_t* __restrict__ cnt;
 };
-BDEV uint32_t nb_dec(int32_t d, uint32_t i, uint32_t n) {  // table entry -> particle index
-  int32_t j = (int32_t)i + d;
-  j += j < 0 ? (int32_t)n : 0;
-  j -= j >= (int32_t)n ? (int32_t)n : 0;
-  return (uint32_t)j;
+// Table entry -> particle index. i + d leaves [0, n) only across the y seam, and then in one
+// direction per particle (below 0 for i < n/2, past n - 1 otherwise, as |d| <= n/2), so one
+// per-particle correction (hoisted out of the pair loops) undoes the modulus.
+BDEV uint32_t nb_dec(int32_t d, uint32_t i, uint32_t n) {
+  const uint32_t t = i + (uint32_t)d;
+  const uint32_t fix = i < (n >> 1) ? n : 0u - n;
+  return t >= n ? t + fix : t;
 }
 BDEV bool nb_enc(uint32_t j, uint32_t i, uint32_t n, int16_t& e) {  // false: does not fit
   int32_t d = (int32_t)j - (int32_t)i;
