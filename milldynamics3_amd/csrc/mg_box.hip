@@ -282,7 +282,15 @@ template <class R> struct PairView {
   const uint32_t* __restrict__ start;
   const int16_t* __restrict__ nb;
   const uint32_t* __restrict__ cnt;
+  uint32_t n;
 };
+// Column k of the table (k is wave-uniform in the pair loops, so the address stays scalar).
+template <class R> BDEV const int16_t* tcol(const PairView<R>& pv, uint32_t k) { return pv.nb + (size_t)k * pv.n; }
+// Neighbour gather p[j] with a 32-bit byte offset (n <= 2^26 and elements of at most 16 bytes keep
+// it below 2^30), so the load takes the array base in scalar registers and one offset register.
+template <class T> BDEV T at(const T* __restrict__ p, uint32_t j) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + j * (uint32_t)sizeof(T));
+}
 // Table entry -> particle index. i + d leaves [0, n) only across the y seam, and then in one
 // direction per particle (below 0 for i < n/2, past n - 1 otherwise, as |d| <= n/2), so one
 // per-particle correction (hoisted out of the pair loops) undoes the modulus.
@@ -334,18 +342,16 @@ BDEV void for_pairs(const BoxGeo<R>& g, const PairView<R>& pv, uint32_t i, F&& f
   const uint32_t c = pv.cnt[i];
   if (c <= BX_KMAX) {
     // four table entries at a time: their loads (and the gathers they feed) are issued together
-    const int16_t* __restrict__ col = pv.nb + i;
-    const size_t n = g.n;
     uint32_t k = 0;
     for (; k + 4u <= c; k += 4u) {
-      const uint32_t j0 = nb_dec(col[(size_t)k * n], i, g.n), j1 = nb_dec(col[(size_t)(k + 1u) * n], i, g.n);
-      const uint32_t j2 = nb_dec(col[(size_t)(k + 2u) * n], i, g.n), j3 = nb_dec(col[(size_t)(k + 3u) * n], i, g.n);
+      const uint32_t j0 = nb_dec(at(tcol(pv, k), i), i, g.n), j1 = nb_dec(at(tcol(pv, k + 1u), i), i, g.n);
+      const uint32_t j2 = nb_dec(at(tcol(pv, k + 2u), i), i, g.n), j3 = nb_dec(at(tcol(pv, k + 3u), i), i, g.n);
       f(j0);
       f(j1);
       f(j2);
       f(j3);
     }
-    for (; k < c; k++) f(nb_dec(col[(size_t)k * n], i, g.n));
+    for (; k < c; k++) f(nb_dec(at(tcol(pv, k), i), i, g.n));
   } else {
     scan_pairs(g, pv.XP, pv.start, i, pv.XP[i], f);
   }
@@ -357,12 +363,10 @@ template <class R, class L, class F>
 BDEV void for_pairs_g(const BoxGeo<R>& g, const PairView<R>& pv, uint32_t i, L&& load, F&& use) {
   const uint32_t c = pv.cnt[i];
   if (c <= BX_KMAX) {
-    const int16_t* __restrict__ col = pv.nb + i;
-    const size_t n = g.n;
     uint32_t k = 0;
     for (; BX_BATCH && k + 4u <= c; k += 4u) {
-      const uint32_t j0 = nb_dec(col[(size_t)k * n], i, g.n), j1 = nb_dec(col[(size_t)(k + 1u) * n], i, g.n);
-      const uint32_t j2 = nb_dec(col[(size_t)(k + 2u) * n], i, g.n), j3 = nb_dec(col[(size_t)(k + 3u) * n], i, g.n);
+      const uint32_t j0 = nb_dec(at(tcol(pv, k), i), i, g.n), j1 = nb_dec(at(tcol(pv, k + 1u), i), i, g.n);
+      const uint32_t j2 = nb_dec(at(tcol(pv, k + 2u), i), i, g.n), j3 = nb_dec(at(tcol(pv, k + 3u), i), i, g.n);
       const auto a0 = load(j0);
       const auto a1 = load(j1);
       const auto a2 = load(j2);
@@ -373,7 +377,7 @@ BDEV void for_pairs_g(const BoxGeo<R>& g, const PairView<R>& pv, uint32_t i, L&&
       use(j3, a3);
     }
     for (; k < c; k++) {
-      const uint32_t j = nb_dec(col[(size_t)k * n], i, g.n);
+      const uint32_t j = nb_dec(at(tcol(pv, k), i), i, g.n);
       use(j, load(j));
     }
   } else {
@@ -581,7 +585,7 @@ BXKT bx_lambda(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, R* __r
   if (i >= g.n) return;
   const Vec<R> xi = X[i];
   LamAcc<R> a{g.w0, mk((R)0, (R)0), (R)0};
-  for_pairs_g(g, pv, i, [&](uint32_t j) __attribute__((always_inline)) { return X[j]; },
+  for_pairs_g(g, pv, i, [&](uint32_t j) __attribute__((always_inline)) { return at(X, j); },
               [&](uint32_t, Vec<R> xj) __attribute__((always_inline)) {
     const Vec<R> d = rel(xi, xj, g);
     lambda_term(g, d, len2(d), a);
@@ -610,7 +614,7 @@ BXKT bx_dp(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const R* _
   const R li = LAM[i];
   Vec<R> s = mk((R)0, (R)0);
   struct Nb { Vec<R> x; R l; };
-  for_pairs_g(g, pv, i, [&](uint32_t j) __attribute__((always_inline)) { return Nb{X[j], LAM[j]}; },
+  for_pairs_g(g, pv, i, [&](uint32_t j) __attribute__((always_inline)) { return Nb{at(X, j), at(LAM, j)}; },
               [&](uint32_t, const Nb& a) __attribute__((always_inline)) {
     const Vec<R> d = rel(xi, a.x, g);
     const R r = pair_len(len2(d));
@@ -634,7 +638,7 @@ BXKRT bx_vel_density(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, 
     V[i] = v;
     vv = (double)v.x * (double)v.x + (double)v.y * (double)v.y;
     R rho = g.w0;
-    for_pairs_g(g, pv, i, [&](uint32_t j) __attribute__((always_inline)) { return X[j]; },
+    for_pairs_g(g, pv, i, [&](uint32_t j) __attribute__((always_inline)) { return at(X, j); },
                 [&](uint32_t, Vec<R> xj) __attribute__((always_inline)) {
       rho += g.mass * poly6(len2(rel(xi, xj, g)), g);
     });
@@ -674,7 +678,9 @@ BDEV Vec<R> lap_row(const BoxGeo<R>& g, uint32_t i, const PairView<R>& pv, const
   Vec<R> s = mk((R)0, (R)0);
   uint32_t k = 0;
   struct Nb { Vec<R> x; R d; Vec<R> v; };
-  for_pairs_g(g, pv, i, [&](uint32_t j) __attribute__((always_inline)) { return Nb{X[j], DENS[j], in[j]}; },
+  for_pairs_g(g, pv, i, [&](uint32_t j) __attribute__((always_inline)) {
+    return Nb{at(X, j), at(DENS, j), at(in, j)};
+  },
               [&](uint32_t, const Nb& a) __attribute__((always_inline)) {
     R w;
     Vec<R> e;
@@ -756,29 +762,26 @@ BXKRT bx_cg_dir(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const
     if (c <= BX_KMAX) {
       const Vec<R> xi = X[i];
       auto pdir = [&](uint32_t j) __attribute__((always_inline)) {
-        return first ? Pold[j] : add2(Rv[j], mul2(Pold[j], beta));
+        return first ? at(Pold, j) : add2(at(Rv, j), mul2(at(Pold, j), beta));
       };
-      auto term = [&](size_t q, Vec<R> pj, Vec<R> xj) __attribute__((always_inline)) {
+      auto term = [&](uint32_t kk, Vec<R> pj, Vec<R> xj) __attribute__((always_inline)) {
         const Vec<R> e = pair_unit<R>(rel(xi, xj, g));
-        s = add2(s, smul2((R)4 * CW[q] * dot2(e, sub2(p, pj)), e));
+        s = add2(s, smul2((R)4 * at(CW + (size_t)kk * g.n, i) * dot2(e, sub2(p, pj)), e));
       };
-      const size_t n = g.n;
       uint32_t k = 0;
       for (; k + 4u <= c; k += 4u) {
-        const size_t q0 = (size_t)k * n + i, q1 = q0 + n, q2 = q1 + n, q3 = q2 + n;
-        const uint32_t j0 = nb_dec(pv.nb[q0], i, g.n), j1 = nb_dec(pv.nb[q1], i, g.n);
-        const uint32_t j2 = nb_dec(pv.nb[q2], i, g.n), j3 = nb_dec(pv.nb[q3], i, g.n);
+        const uint32_t j0 = nb_dec(at(tcol(pv, k), i), i, g.n), j1 = nb_dec(at(tcol(pv, k + 1u), i), i, g.n);
+        const uint32_t j2 = nb_dec(at(tcol(pv, k + 2u), i), i, g.n), j3 = nb_dec(at(tcol(pv, k + 3u), i), i, g.n);
         const Vec<R> p0 = pdir(j0), p1 = pdir(j1), p2 = pdir(j2), p3 = pdir(j3);
-        const Vec<R> x0 = X[j0], x1 = X[j1], x2 = X[j2], x3 = X[j3];
-        term(q0, p0, x0);
-        term(q1, p1, x1);
-        term(q2, p2, x2);
-        term(q3, p3, x3);
+        const Vec<R> x0 = at(X, j0), x1 = at(X, j1), x2 = at(X, j2), x3 = at(X, j3);
+        term(k, p0, x0);
+        term(k + 1u, p1, x1);
+        term(k + 2u, p2, x2);
+        term(k + 3u, p3, x3);
       }
       for (; k < c; k++) {
-        const size_t q = (size_t)k * n + i;
-        const uint32_t j = nb_dec(pv.nb[q], i, g.n);
-        term(q, pdir(j), X[j]);
+        const uint32_t j = nb_dec(at(tcol(pv, k), i), i, g.n);
+        term(k, pdir(j), at(X, j));
       }
     } else {
       const Vec<R> xi = X[i];
@@ -1187,7 +1190,7 @@ void collect_times(mg_box* B) {
 }
 
 template <class R> PairView<R> pair_view(const mg_box* B) {
-  return PairView<R>{vp<R>(B->XP), B->start, B->pnb, B->pcnt};
+  return PairView<R>{vp<R>(B->XP), B->start, B->pnb, B->pcnt, B->n};
 }
 
 // Enqueues one step up to and including the CG set-up; returns false on a launch error. When it
