@@ -195,10 +195,13 @@ template <class R> BDEV R wrapR(R a, R L) {  // rem_euclid
   R r = mfmod(a, L);
   return r < (R)0 ? r + mfabs(L) : r;
 }
-template <class R> BDEV R mi1(R d, const BoxGeo<R>& g) {  // minimum image
-  if (d > g.halfL) return d - g.L;
-  if (d < -g.halfL) return d + g.L;
-  return d;
+// Minimum image: d - L above L/2, d + L below -L/2 (as d - (-L), the same IEEE operation), else d;
+// a select, no branch.
+BDEV float mcopysign(float a, float b) { return copysignf(a, b); }
+BDEV double mcopysign(double a, double b) { return copysign(a, b); }
+template <class R> BDEV R mi1(R d, const BoxGeo<R>& g) {
+  const R t = d - mcopysign(g.L, d);
+  return mfabs(d) > g.halfL ? t : d;
 }
 // Pair vector a - b with the minimum image.
 template <class R> BDEV Vec<R> rel(Vec<R> a, Vec<R> b, const BoxGeo<R>& g) {
