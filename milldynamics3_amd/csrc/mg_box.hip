@@ -79,10 +79,11 @@ template <class R> using Vec = typename VecT<R>::type;
 // (the f64 constants are the f32 geometry promoted, then the derived ones formed in f64, as the
 // oracle's f64 policy forms them). `halt` is the window stop word: once a step of a multi-step
 // window has run out of CG iterations before converging, the window's remaining kernels return at
-// once (the host then makes that step's start state live again).
+// once (the host then makes that step's start state live again). [lo, hi] = [2h, L - 2h] is the
+// interior where pair vectors need no minimum image (see wave_inside).
 template <class R> struct BoxGeo {
   uint32_t n, ncx, nblk, xcd_chunk, nred, xcd_chunk_red;
-  R h, h2, L, halfL, cw, mass, rest, dt, k6, ks, w0, m_over_rest, mu2m, eta2;
+  R h, h2, L, halfL, cw, mass, rest, dt, k6, ks, w0, m_over_rest, mu2m, eta2, lo, hi;
   const uint32_t* halt;
 };
 
@@ -208,6 +209,26 @@ template <class R> BDEV Vec<R> rel(Vec<R> a, Vec<R> b, const BoxGeo<R>& g) {
   const Vec<R> d = sub2(a, b);
   return mk(mi1(d.x, g), mi1(d.y, g));
 }
+// Pair loops come in two forms, chosen per wave: Wrap<true> takes the minimum image, Wrap<false>
+// the plain difference. The plain form is exact for a particle a in [2h, L - 2h]^2 and any partner
+// b in [0, L]^2 (positions are kept wrapped): a component with |a - b| <= L/2 is its own minimum
+// image; one with |a - b| > L/2 has a minimum image of length L - |a - b| >= 2h (less one rounding),
+// so the pair lies beyond h in both forms and adds exactly nothing in either (poly6, the spiky
+// gradient and the Morris weight vanish at r >= h, and a zero weight adds only signed zeros to a
+// sum that started at +0).
+template <bool W> struct Wrap {};
+template <class R> BDEV Vec<R> rel(Vec<R> a, Vec<R> b, const BoxGeo<R>& g, Wrap<true>) { return rel(a, b, g); }
+template <class R> BDEV Vec<R> rel(Vec<R> a, Vec<R> b, const BoxGeo<R>& g, Wrap<false>) { return sub2(a, b); }
+// True when every active lane's particle lies in [lo, hi]^2 (wave-uniform).
+template <class R> BDEV bool wave_inside(Vec<R> x, const BoxGeo<R>& g) {
+  const bool in = x.x >= g.lo && x.x <= g.hi && x.y >= g.lo && x.y <= g.hi;
+  return __ballot(!in) == 0ull;
+}
+// f(Wrap<false>{}) when the wave is inside, else f(Wrap<true>{}) (a uniform branch).
+template <class F> BDEV void by_wrap(bool inside, F&& f) {
+  if (inside) f(Wrap<false>{});
+  else f(Wrap<true>{});
+}
 template <class R> BDEV uint32_t cell1(R a, const BoxGeo<R>& g) {
   R q = mfloor(a / g.cw);
   if (!(q >= (R)0)) return 0u;
@@ -312,9 +333,9 @@ BDEV bool nb_enc(uint32_t j, uint32_t i, uint32_t n, int16_t& e) {  // false: do
   return true;
 }
 // f(j, d, r2) for every pair, with d = xpi - XP[j] (minimum image) and r2 = |d|^2 as the test used.
-template <class R, class F>
+template <class R, bool W, class F>
 BDEV void scan_pairs_d(const BoxGeo<R>& g, const Vec<R>* __restrict__ XP, const uint32_t* __restrict__ start,
-                       uint32_t i, Vec<R> xpi, F&& f) {
+                       uint32_t i, Vec<R> xpi, Wrap<W> w, F&& f) {
   const uint32_t cx = cell1(xpi.x, g), cy = cell1(xpi.y, g);
   // the nine cell ranges are loaded up front (independent loads), then walked in order
   uint32_t cs[9], ce[9];
@@ -328,7 +349,7 @@ BDEV void scan_pairs_d(const BoxGeo<R>& g, const Vec<R>* __restrict__ XP, const 
   for (int o = 0; o < 9; o++) {
     for (uint32_t j = cs[o]; j < ce[o]; j++) {
       if (j == i) continue;
-      const Vec<R> d = rel(xpi, XP[j], g);
+      const Vec<R> d = rel(xpi, XP[j], g, w);
       const R r2 = len2(d);
       if (r2 > g.h2) continue;
       f(j, d, r2);
@@ -338,7 +359,7 @@ BDEV void scan_pairs_d(const BoxGeo<R>& g, const Vec<R>* __restrict__ XP, const 
 template <class R, class F>
 BDEV void scan_pairs(const BoxGeo<R>& g, const Vec<R>* __restrict__ XP, const uint32_t* __restrict__ start,
                      uint32_t i, Vec<R> xpi, F&& f) {
-  scan_pairs_d(g, XP, start, i, xpi, [&](uint32_t j, Vec<R>, R) __attribute__((always_inline)) { f(j); });
+  scan_pairs_d(g, XP, start, i, xpi, Wrap<true>{}, [&](uint32_t j, Vec<R>, R) __attribute__((always_inline)) { f(j); });
 }
 template <class R, class F>
 BDEV void for_pairs(const BoxGeo<R>& g, const PairView<R>& pv, uint32_t i, F&& f) {
@@ -539,12 +560,12 @@ BXKT bx_gather(BoxGeo<R> g, const uint32_t* __restrict__ slot, const Vec<R>* __r
 }
 
 // Records the pair table of particle i while f(j, d, r2) sees every pair.
-template <class R, class F>
+template <class R, bool W, class F>
 BDEV void table_row(const BoxGeo<R>& g, const Vec<R>* __restrict__ XP, const uint32_t* __restrict__ start,
-                    int16_t* __restrict__ nb, uint32_t* __restrict__ cnt, uint32_t i, Vec<R> xpi, F&& f) {
+                    int16_t* __restrict__ nb, uint32_t* __restrict__ cnt, uint32_t i, Vec<R> xpi, Wrap<W> w, F&& f) {
   uint32_t c = 0;
   bool fits = true;
-  scan_pairs_d(g, XP, start, i, xpi, [&](uint32_t j, Vec<R> d, R r2) __attribute__((always_inline)) {
+  scan_pairs_d(g, XP, start, i, xpi, w, [&](uint32_t j, Vec<R> d, R r2) __attribute__((always_inline)) {
     int16_t e = 0;
     fits = nb_enc(j, i, g.n, e) && fits;
     if (c < BX_KMAX) nb[(size_t)c * g.n + i] = e;
@@ -558,7 +579,7 @@ BXKT bx_pairs(BoxGeo<R> g, const Vec<R>* __restrict__ XP, const uint32_t* __rest
   if (*g.halt) return;
   const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
   if (i >= g.n) return;
-  table_row(g, XP, start, nb, cnt, i, XP[i], [](uint32_t, Vec<R>, R) __attribute__((always_inline)) {});
+  table_row(g, XP, start, nb, cnt, i, XP[i], Wrap<true>{}, [](uint32_t, Vec<R>, R) __attribute__((always_inline)) {});
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -588,10 +609,12 @@ BXKT bx_lambda(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, R* __r
   if (i >= g.n) return;
   const Vec<R> xi = X[i];
   LamAcc<R> a{g.w0, mk((R)0, (R)0), (R)0};
-  for_pairs_g(g, pv, i, [&](uint32_t j) __attribute__((always_inline)) { return at(X, j); },
-              [&](uint32_t, Vec<R> xj) __attribute__((always_inline)) {
-    const Vec<R> d = rel(xi, xj, g);
-    lambda_term(g, d, len2(d), a);
+  by_wrap(wave_inside(xi, g), [&](auto w) __attribute__((always_inline)) {
+    for_pairs_g(g, pv, i, [&](uint32_t j) __attribute__((always_inline)) { return at(X, j); },
+                [&](uint32_t, Vec<R> xj) __attribute__((always_inline)) {
+      const Vec<R> d = rel(xi, xj, g, w);
+      lambda_term(g, d, len2(d), a);
+    });
   });
   LAM[i] = lambda_of(g, a);
 }
@@ -604,8 +627,11 @@ BXKT bx_pairs_lambda(BoxGeo<R> g, const Vec<R>* __restrict__ XP, const uint32_t*
   const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
   if (i >= g.n) return;
   LamAcc<R> a{g.w0, mk((R)0, (R)0), (R)0};
-  table_row(g, XP, start, nb, cnt, i, XP[i],
-            [&](uint32_t, Vec<R> d, R r2) __attribute__((always_inline)) { lambda_term(g, d, r2, a); });
+  const Vec<R> xpi = XP[i];
+  by_wrap(wave_inside(xpi, g), [&](auto w) __attribute__((always_inline)) {
+    table_row(g, XP, start, nb, cnt, i, xpi, w,
+              [&](uint32_t, Vec<R> d, R r2) __attribute__((always_inline)) { lambda_term(g, d, r2, a); });
+  });
   LAM[i] = lambda_of(g, a);
 }
 BXKT bx_dp(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const R* __restrict__ LAM,
@@ -617,11 +643,13 @@ BXKT bx_dp(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const R* _
   const R li = LAM[i];
   Vec<R> s = mk((R)0, (R)0);
   struct Nb { Vec<R> x; R l; };
-  for_pairs_g(g, pv, i, [&](uint32_t j) __attribute__((always_inline)) { return Nb{at(X, j), at(LAM, j)}; },
-              [&](uint32_t, const Nb& a) __attribute__((always_inline)) {
-    const Vec<R> d = rel(xi, a.x, g);
-    const R r = pair_len(len2(d));
-    s = add2(s, smul2(li + a.l, spiky(d, r, g)));
+  by_wrap(wave_inside(xi, g), [&](auto w) __attribute__((always_inline)) {
+    for_pairs_g(g, pv, i, [&](uint32_t j) __attribute__((always_inline)) { return Nb{at(X, j), at(LAM, j)}; },
+                [&](uint32_t, const Nb& a) __attribute__((always_inline)) {
+      const Vec<R> d = rel(xi, a.x, g, w);
+      const R r = pair_len(len2(d));
+      s = add2(s, smul2(li + a.l, spiky(d, r, g)));
+    });
   });
   const Vec<R> dp = mul2(s, g.m_over_rest);
   Xout[i] = mk(wrapR(xi.x + dp.x, g.L), wrapR(xi.y + dp.y, g.L));
@@ -641,9 +669,11 @@ BXKRT bx_vel_density(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, 
     V[i] = v;
     vv = (double)v.x * (double)v.x + (double)v.y * (double)v.y;
     R rho = g.w0;
-    for_pairs_g(g, pv, i, [&](uint32_t j) __attribute__((always_inline)) { return at(X, j); },
-                [&](uint32_t, Vec<R> xj) __attribute__((always_inline)) {
-      rho += g.mass * poly6(len2(rel(xi, xj, g)), g);
+    by_wrap(wave_inside(xi, g), [&](auto w) __attribute__((always_inline)) {
+      for_pairs_g(g, pv, i, [&](uint32_t j) __attribute__((always_inline)) { return at(X, j); },
+                  [&](uint32_t, Vec<R> xj) __attribute__((always_inline)) {
+        rho += g.mass * poly6(len2(rel(xi, xj, g, w)), g);
+      });
     });
     DENS[i] = rho;
   }
@@ -655,10 +685,10 @@ BXKRT bx_vel_density(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, 
 // The quotient d / r is formed once: the kernel gradient (d / r) * coeff and the direction d / r
 // use the same rounded value, exactly as two separate divisions would give. pair_unit forms the
 // same direction alone (the CG recomputes it from the unchanged positions instead of storing it).
-template <class R>
-BDEV void morris(const BoxGeo<R>& g, Vec<R> xi, R rho_i, Vec<R> xj, R dens_j, R& w, Vec<R>& e) {
+template <class R, bool W>
+BDEV void morris(const BoxGeo<R>& g, Wrap<W> wr, Vec<R> xi, R rho_i, Vec<R> xj, R dens_j, R& w, Vec<R>& e) {
   const R rho_j = mfmax(dens_j, (R)1e-6f);
-  const Vec<R> d = rel(xi, xj, g);
+  const Vec<R> d = rel(xi, xj, g, wr);
   const R r = pair_len(len2(d));
   const Vec<R> dr = pair_div(d, r);
   const Vec<R> gr = (r <= (R)1e-9f || r >= g.h) ? mk((R)0, (R)0) : mul2(dr, g.ks * (g.h - r) * (g.h - r));
@@ -681,16 +711,18 @@ BDEV Vec<R> lap_row(const BoxGeo<R>& g, uint32_t i, const PairView<R>& pv, const
   Vec<R> s = mk((R)0, (R)0);
   uint32_t k = 0;
   struct Nb { Vec<R> x; R d; Vec<R> v; };
-  for_pairs_g(g, pv, i, [&](uint32_t j) __attribute__((always_inline)) {
-    return Nb{at(X, j), at(DENS, j), at(in, j)};
-  },
-              [&](uint32_t, const Nb& a) __attribute__((always_inline)) {
-    R w;
-    Vec<R> e;
-    morris(g, xi, rho_i, a.x, a.d, w, e);
-    if (keep) CW[(size_t)k * g.n + i] = w;
-    k++;
-    s = add2(s, smul2((R)4 * w * dot2(e, sub2(ini, a.v)), e));
+  by_wrap(wave_inside(xi, g), [&](auto wr) __attribute__((always_inline)) {
+    for_pairs_g(g, pv, i, [&](uint32_t j) __attribute__((always_inline)) {
+      return Nb{at(X, j), at(DENS, j), at(in, j)};
+    },
+                [&](uint32_t, const Nb& a) __attribute__((always_inline)) {
+      R w;
+      Vec<R> e;
+      morris(g, wr, xi, rho_i, a.x, a.d, w, e);
+      if (keep) CW[(size_t)k * g.n + i] = w;
+      k++;
+      s = add2(s, smul2((R)4 * w * dot2(e, sub2(ini, a.v)), e));
+    });
   });
   return s;
 }
@@ -762,41 +794,42 @@ BXKRT bx_cg_dir(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const
     const Vec<R> p = first ? Pold[i] : add2(Rv[i], mul2(Pold[i], beta));
     Vec<R> s = mk((R)0, (R)0);
     const uint32_t c = pv.cnt[i];
-    if (c <= BX_KMAX) {
-      const Vec<R> xi = X[i];
-      auto pdir = [&](uint32_t j) __attribute__((always_inline)) {
-        return first ? at(Pold, j) : add2(at(Rv, j), mul2(at(Pold, j), beta));
-      };
-      auto term = [&](uint32_t kk, Vec<R> pj, Vec<R> xj) __attribute__((always_inline)) {
-        const Vec<R> e = pair_unit<R>(rel(xi, xj, g));
-        s = add2(s, smul2((R)4 * at(CW + (size_t)kk * g.n, i) * dot2(e, sub2(p, pj)), e));
-      };
-      uint32_t k = 0;
-      for (; k + 4u <= c; k += 4u) {
-        const uint32_t j0 = nb_dec(at(tcol(pv, k), i), i, g.n), j1 = nb_dec(at(tcol(pv, k + 1u), i), i, g.n);
-        const uint32_t j2 = nb_dec(at(tcol(pv, k + 2u), i), i, g.n), j3 = nb_dec(at(tcol(pv, k + 3u), i), i, g.n);
-        const Vec<R> p0 = pdir(j0), p1 = pdir(j1), p2 = pdir(j2), p3 = pdir(j3);
-        const Vec<R> x0 = at(X, j0), x1 = at(X, j1), x2 = at(X, j2), x3 = at(X, j3);
-        term(k, p0, x0);
-        term(k + 1u, p1, x1);
-        term(k + 2u, p2, x2);
-        term(k + 3u, p3, x3);
+    const Vec<R> xi = X[i];
+    by_wrap(wave_inside(xi, g), [&](auto wr) __attribute__((always_inline)) {
+      if (c <= BX_KMAX) {
+        auto pdir = [&](uint32_t j) __attribute__((always_inline)) {
+          return first ? at(Pold, j) : add2(at(Rv, j), mul2(at(Pold, j), beta));
+        };
+        auto term = [&](uint32_t kk, Vec<R> pj, Vec<R> xj) __attribute__((always_inline)) {
+          const Vec<R> e = pair_unit<R>(rel(xi, xj, g, wr));
+          s = add2(s, smul2((R)4 * at(CW + (size_t)kk * g.n, i) * dot2(e, sub2(p, pj)), e));
+        };
+        uint32_t k = 0;
+        for (; k + 4u <= c; k += 4u) {
+          const uint32_t j0 = nb_dec(at(tcol(pv, k), i), i, g.n), j1 = nb_dec(at(tcol(pv, k + 1u), i), i, g.n);
+          const uint32_t j2 = nb_dec(at(tcol(pv, k + 2u), i), i, g.n), j3 = nb_dec(at(tcol(pv, k + 3u), i), i, g.n);
+          const Vec<R> p0 = pdir(j0), p1 = pdir(j1), p2 = pdir(j2), p3 = pdir(j3);
+          const Vec<R> x0 = at(X, j0), x1 = at(X, j1), x2 = at(X, j2), x3 = at(X, j3);
+          term(k, p0, x0);
+          term(k + 1u, p1, x1);
+          term(k + 2u, p2, x2);
+          term(k + 3u, p3, x3);
+        }
+        for (; k < c; k++) {
+          const uint32_t j = nb_dec(at(tcol(pv, k), i), i, g.n);
+          term(k, pdir(j), at(X, j));
+        }
+      } else {
+        const R rho_i = mfmax(DENS[i], (R)1e-6f);
+        scan_pairs(g, pv.XP, pv.start, i, pv.XP[i], [&](uint32_t j) __attribute__((always_inline)) {
+          const Vec<R> pj = first ? Pold[j] : add2(Rv[j], mul2(Pold[j], beta));
+          R w;
+          Vec<R> e;
+          morris(g, wr, xi, rho_i, X[j], DENS[j], w, e);
+          s = add2(s, smul2((R)4 * w * dot2(e, sub2(p, pj)), e));
+        });
       }
-      for (; k < c; k++) {
-        const uint32_t j = nb_dec(at(tcol(pv, k), i), i, g.n);
-        term(k, pdir(j), at(X, j));
-      }
-    } else {
-      const Vec<R> xi = X[i];
-      const R rho_i = mfmax(DENS[i], (R)1e-6f);
-      scan_pairs(g, pv.XP, pv.start, i, pv.XP[i], [&](uint32_t j) __attribute__((always_inline)) {
-        const Vec<R> pj = first ? Pold[j] : add2(Rv[j], mul2(Pold[j], beta));
-        R w;
-        Vec<R> e;
-        morris(g, xi, rho_i, X[j], DENS[j], w, e);
-        s = add2(s, smul2((R)4 * w * dot2(e, sub2(p, pj)), e));
-      });
-    }
+    });
     const Vec<R> ap = add2(p, mul2(s, g.dt));
     Pnew[i] = p;
     AP[i] = ap;
@@ -1074,6 +1107,8 @@ bool box_geometry(const mg_box_config& c, BoxGeo<float>& g, std::string& err) {
   g.m_over_rest = g.mass / c.rest;
   g.mu2m = g.mass * (c.mu + c.mu);
   g.eta2 = 0.01f * g.h * g.h;
+  g.lo = 2.0f * g.h;
+  g.hi = g.L - g.lo;
   g.halt = nullptr;
   g.nblk = (g.n + BX_BLOCK - 1) / BX_BLOCK;
   g.xcd_chunk = (g.nblk % 8u == 0u) ? g.nblk / 8u : 0u;
@@ -1109,6 +1144,8 @@ BoxGeo<double> promote(const BoxGeo<float>& f, const mg_box_config& c) {
   const double mu = (double)c.mu;
   g.mu2m = g.mass * (mu + mu);
   g.eta2 = (double)0.01f * g.h * g.h;
+  g.lo = 2.0 * g.h;
+  g.hi = g.L - g.lo;
   g.halt = nullptr;
   return g;
 }
