@@ -130,10 +130,12 @@ BX_VEC_OPS(double2, double, make_double2)
 //  - pair_div(d, r) equals (d.x / r, d.y / r) for r in [2^-100, 2^30] (the box side is at most
 //    2^23, so every r > 1e-9 is in range) when both |components| are >= 2^-96 (f32) / 2^-900
 //    (f64); it takes the IEEE division for smaller ones (in f32 for every component whose square
-//    is zero, i.e. below about 2^-75). Callers use the quotient only for r > 1e-9.
+//    is zero, i.e. below about 2^-75). Callers use the quotient only for r > 1e-9. With
+//    guard = false the small-component test is left out: for callers whose nonzero components
+//    are known to be >= 2^-96 (f32) / 2^-900 (f64).
 // mg_box_check_arith compares them with the IEEE operations on the GPU.
 template <class R> BDEV R pair_len(R r2) { return msqrt(r2); }
-template <class R> BDEV Vec<R> pair_div(Vec<R> d, R r) { return div2(d, r); }
+template <class R> BDEV Vec<R> pair_div(Vec<R> d, R r, bool guard = true) { return div2(d, r); }
 BDEV float sqrt_core(float x) {  // v_sqrt (1 ulp) and the two neighbouring candidates, as LLVM lowers sqrtf
   const float s = __builtin_amdgcn_sqrtf(x);
   const float sm = __uint_as_float(__float_as_uint(s) - 1u);
@@ -159,9 +161,9 @@ BDEV double sqrt_core(double x) {
   return __builtin_fma(d, h, g);
 }
 template <> BDEVX double pair_len<double>(double r2) { return sqrt_core(r2); }
-template <> BDEVX double2 pair_div<double>(double2 d, double r) {
+template <> BDEVX double2 pair_div<double>(double2 d, double r, bool guard) {
   const bool tiny = (fabs(d.x) < 0x1p-900) | (fabs(d.y) < 0x1p-900);
-  if (__builtin_expect(tiny, 0)) return make_double2(d.x / r, d.y / r);
+  if (guard && __builtin_expect(tiny, 0)) return make_double2(d.x / r, d.y / r);
   // reciprocal refined twice, then one residual correction per quotient (the unscaled path of
   // the compiler's f64 division)
   const double y0 = __builtin_amdgcn_rcp(r);
@@ -173,11 +175,11 @@ template <> BDEVX double2 pair_div<double>(double2 d, double r) {
   };
   return make_double2(q(d.x), q(d.y));
 }
-template <> BDEVX float2 pair_div<float>(float2 d, float r) {
+template <> BDEVX float2 pair_div<float>(float2 d, float r, bool guard) {
   // a component below 2^-75 squares to zero (the same products len2 formed): take the IEEE division
   // for those, which covers every component below the core sequence's 2^-96 bound
   const bool tiny = (d.x * d.x == 0.0f) | (d.y * d.y == 0.0f);
-  if (__builtin_expect(tiny, 0)) return make_float2(d.x / r, d.y / r);
+  if (guard && __builtin_expect(tiny, 0)) return make_float2(d.x / r, d.y / r);
   // Newton-refined reciprocal, then two residual corrections per quotient (the unscaled path of
   // the compiler's division: rcp, fma, fma | mul, fma, fma, fma, fma)
   const float y0 = __builtin_amdgcn_rcpf(r);
@@ -215,14 +217,28 @@ template <class R> BDEV Vec<R> rel(Vec<R> a, Vec<R> b, const BoxGeo<R>& g) {
 // image; one with |a - b| > L/2 has a minimum image of length L - |a - b| >= 2h (less one rounding),
 // so the pair lies beyond h in both forms and adds exactly nothing in either (poly6, the spiky
 // gradient and the Morris weight vanish at r >= h, and a zero weight adds only signed zeros to a
-// sum that started at +0).
+// sum that started at +0). For a pair within h of such a particle both coordinates lie in
+// [xi/2, 2 xi] (xi >= 2h), so each component is an exact difference of two floats >= h: zero or
+// at least ulp(h) >= h 2^-24, which for h >= 2^-60 (lo > hi below that) is far above the
+// quotient's small-component bound, and Wrap<false> leaves that test out. Wrap<false> also means
+// that no lane's table row holds an offset taken modulo n (cnt bit 31 clear), so an entry decodes
+// as i + offset.
 template <bool W> struct Wrap {};
+template <bool W> BDEV constexpr bool guarded(Wrap<W>) { return W; }
 template <class R> BDEV Vec<R> rel(Vec<R> a, Vec<R> b, const BoxGeo<R>& g, Wrap<true>) { return rel(a, b, g); }
 template <class R> BDEV Vec<R> rel(Vec<R> a, Vec<R> b, const BoxGeo<R>& g, Wrap<false>) { return sub2(a, b); }
 // True when every active lane's particle lies in [lo, hi]^2 (wave-uniform).
 template <class R> BDEV bool wave_inside(Vec<R> x, const BoxGeo<R>& g) {
   const bool in = x.x >= g.lo && x.x <= g.hi && x.y >= g.lo && x.y <= g.hi;
   return __ballot(!in) == 0ull;
+}
+// Pair-count word: bit 31 marks a table row with an offset taken modulo n.
+#define BX_CNT_WRAPPED 0x80000000u
+BDEV uint32_t pcount(uint32_t w) { return w & ~BX_CNT_WRAPPED; }
+// As wave_inside, and no lane's table row holds a wrapped offset (the Wrap<false> condition).
+template <class R> BDEV bool wave_fast(Vec<R> x, uint32_t cntw, const BoxGeo<R>& g) {
+  const bool ok = x.x >= g.lo && x.x <= g.hi && x.y >= g.lo && x.y <= g.hi && !(cntw & BX_CNT_WRAPPED);
+  return __ballot(!ok) == 0ull;
 }
 // f(Wrap<false>{}) when the wave is inside, else f(Wrap<true>{}) (a uniform branch).
 template <class F> BDEV void by_wrap(bool inside, F&& f) {
@@ -240,10 +256,10 @@ template <class R> BDEV R poly6(R r2, const BoxGeo<R>& g) {  // pbf.rs:119-126
   R t = g.h2 - r2;
   return g.k6 * t * t * t;
 }
-template <class R> BDEV Vec<R> spiky(Vec<R> d, R r, const BoxGeo<R>& g) {  // pbf.rs:131-137
+template <class R> BDEV Vec<R> spiky(Vec<R> d, R r, const BoxGeo<R>& g, bool guard) {  // pbf.rs:131-137
   if (r <= (R)1e-9f || r >= g.h) return mk((R)0, (R)0);
   R coeff = g.ks * (g.h - r) * (g.h - r);
-  return mul2(pair_div(d, r), coeff);
+  return mul2(pair_div(d, r, guard), coeff);
 }
 // XCD-aware tile index: hardware dispatches consecutive workgroups round-robin over the 8 XCDs;
 // remap so XCD x processes tiles [x*chunk, (x+1)*chunk).
@@ -323,11 +339,15 @@ BDEV uint32_t nb_dec(int32_t d, uint32_t i, uint32_t n) {
   const uint32_t fix = i < (n >> 1) ? n : 0u - n;
   return t >= n ? t + fix : t;
 }
-BDEV bool nb_enc(uint32_t j, uint32_t i, uint32_t n, int16_t& e) {  // false: does not fit
+BDEV uint32_t nb_dec(int32_t d, uint32_t i, uint32_t n, Wrap<true>) { return nb_dec(d, i, n); }
+BDEV uint32_t nb_dec(int32_t d, uint32_t i, uint32_t, Wrap<false>) { return i + (uint32_t)d; }
+// false: does not fit; `wrapped` is set when the offset was taken modulo n
+BDEV bool nb_enc(uint32_t j, uint32_t i, uint32_t n, int16_t& e, bool& wrapped) {
   int32_t d = (int32_t)j - (int32_t)i;
   const int32_t half = (int32_t)(n >> 1);
   if (d > half) d -= (int32_t)n;
   else if (d < -half) d += (int32_t)n;
+  wrapped = wrapped || d != (int32_t)j - (int32_t)i;
   if (d < -BX_NB_RANGE || d > BX_NB_RANGE) return false;
   e = (int16_t)d;
   return true;
@@ -363,7 +383,7 @@ BDEV void scan_pairs(const BoxGeo<R>& g, const Vec<R>* __restrict__ XP, const ui
 }
 template <class R, class F>
 BDEV void for_pairs(const BoxGeo<R>& g, const PairView<R>& pv, uint32_t i, F&& f) {
-  const uint32_t c = pv.cnt[i];
+  const uint32_t c = pcount(pv.cnt[i]);
   if (c <= BX_KMAX) {
     // four table entries at a time: their loads (and the gathers they feed) are issued together
     uint32_t k = 0;
@@ -383,14 +403,16 @@ BDEV void for_pairs(const BoxGeo<R>& g, const PairView<R>& pv, uint32_t i, F&& f
 // As for_pairs, with the per-neighbour gathers split out: load(j) returns what use(j, v) needs,
 // and the loads of four table entries are all issued before the first use (memory-level
 // parallelism; without it every neighbour costs two dependent round trips).
-template <class R, class L, class F>
-BDEV void for_pairs_g(const BoxGeo<R>& g, const PairView<R>& pv, uint32_t i, L&& load, F&& use) {
-  const uint32_t c = pv.cnt[i];
+// cntw = cnt[i] (read by the caller for the wave's Wrap choice).
+template <class R, bool W, class L, class F>
+BDEV void for_pairs_g(const BoxGeo<R>& g, const PairView<R>& pv, uint32_t i, uint32_t cntw, Wrap<W> w, L&& load,
+                      F&& use) {
+  const uint32_t c = pcount(cntw);
   if (c <= BX_KMAX) {
     uint32_t k = 0;
     for (; BX_BATCH && k + 4u <= c; k += 4u) {
-      const uint32_t j0 = nb_dec(at(tcol(pv, k), i), i, g.n), j1 = nb_dec(at(tcol(pv, k + 1u), i), i, g.n);
-      const uint32_t j2 = nb_dec(at(tcol(pv, k + 2u), i), i, g.n), j3 = nb_dec(at(tcol(pv, k + 3u), i), i, g.n);
+      const uint32_t j0 = nb_dec(at(tcol(pv, k), i), i, g.n, w), j1 = nb_dec(at(tcol(pv, k + 1u), i), i, g.n, w);
+      const uint32_t j2 = nb_dec(at(tcol(pv, k + 2u), i), i, g.n, w), j3 = nb_dec(at(tcol(pv, k + 3u), i), i, g.n, w);
       const auto a0 = load(j0);
       const auto a1 = load(j1);
       const auto a2 = load(j2);
@@ -401,7 +423,7 @@ BDEV void for_pairs_g(const BoxGeo<R>& g, const PairView<R>& pv, uint32_t i, L&&
       use(j3, a3);
     }
     for (; k < c; k++) {
-      const uint32_t j = nb_dec(at(tcol(pv, k), i), i, g.n);
+      const uint32_t j = nb_dec(at(tcol(pv, k), i), i, g.n, w);
       use(j, load(j));
     }
   } else {
@@ -564,15 +586,15 @@ template <class R, bool W, class F>
 BDEV void table_row(const BoxGeo<R>& g, const Vec<R>* __restrict__ XP, const uint32_t* __restrict__ start,
                     int16_t* __restrict__ nb, uint32_t* __restrict__ cnt, uint32_t i, Vec<R> xpi, Wrap<W> w, F&& f) {
   uint32_t c = 0;
-  bool fits = true;
+  bool fits = true, wrapped = false;
   scan_pairs_d(g, XP, start, i, xpi, w, [&](uint32_t j, Vec<R> d, R r2) __attribute__((always_inline)) {
     int16_t e = 0;
-    fits = nb_enc(j, i, g.n, e) && fits;
+    fits = nb_enc(j, i, g.n, e, wrapped) && fits;
     if (c < BX_KMAX) nb[(size_t)c * g.n + i] = e;
     c++;
     f(j, d, r2);
   });
-  cnt[i] = fits ? c : (c > BX_KMAX ? c : BX_KMAX + 1u);
+  cnt[i] = (fits ? c : (c > BX_KMAX ? c : BX_KMAX + 1u)) | (wrapped ? BX_CNT_WRAPPED : 0u);
 }
 BXKT bx_pairs(BoxGeo<R> g, const Vec<R>* __restrict__ XP, const uint32_t* __restrict__ start,
               int16_t* __restrict__ nb, uint32_t* __restrict__ cnt) {
@@ -591,10 +613,10 @@ template <class R> struct LamAcc {
   Vec<R> gs;
   R sgs;
 };
-template <class R> BDEV void lambda_term(const BoxGeo<R>& g, Vec<R> d, R r2, LamAcc<R>& a) {
+template <class R, bool W> BDEV void lambda_term(const BoxGeo<R>& g, Wrap<W> w, Vec<R> d, R r2, LamAcc<R>& a) {
   a.rho += g.mass * poly6(r2, g);
   const R r = pair_len(r2);
-  const Vec<R> gr = mul2(spiky(d, r, g), g.m_over_rest);
+  const Vec<R> gr = mul2(spiky(d, r, g, guarded(w)), g.m_over_rest);
   a.gs = add2(a.gs, gr);
   a.sgs += len2(gr);
 }
@@ -608,12 +630,13 @@ BXKT bx_lambda(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, R* __r
   const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
   if (i >= g.n) return;
   const Vec<R> xi = X[i];
+  const uint32_t cw = pv.cnt[i];
   LamAcc<R> a{g.w0, mk((R)0, (R)0), (R)0};
-  by_wrap(wave_inside(xi, g), [&](auto w) __attribute__((always_inline)) {
-    for_pairs_g(g, pv, i, [&](uint32_t j) __attribute__((always_inline)) { return at(X, j); },
+  by_wrap(wave_fast(xi, cw, g), [&](auto w) __attribute__((always_inline)) {
+    for_pairs_g(g, pv, i, cw, w, [&](uint32_t j) __attribute__((always_inline)) { return at(X, j); },
                 [&](uint32_t, Vec<R> xj) __attribute__((always_inline)) {
       const Vec<R> d = rel(xi, xj, g, w);
-      lambda_term(g, d, len2(d), a);
+      lambda_term(g, w, d, len2(d), a);
     });
   });
   LAM[i] = lambda_of(g, a);
@@ -630,7 +653,7 @@ BXKT bx_pairs_lambda(BoxGeo<R> g, const Vec<R>* __restrict__ XP, const uint32_t*
   const Vec<R> xpi = XP[i];
   by_wrap(wave_inside(xpi, g), [&](auto w) __attribute__((always_inline)) {
     table_row(g, XP, start, nb, cnt, i, xpi, w,
-              [&](uint32_t, Vec<R> d, R r2) __attribute__((always_inline)) { lambda_term(g, d, r2, a); });
+              [&](uint32_t, Vec<R> d, R r2) __attribute__((always_inline)) { lambda_term(g, w, d, r2, a); });
   });
   LAM[i] = lambda_of(g, a);
 }
@@ -643,12 +666,13 @@ BXKT bx_dp(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const R* _
   const R li = LAM[i];
   Vec<R> s = mk((R)0, (R)0);
   struct Nb { Vec<R> x; R l; };
-  by_wrap(wave_inside(xi, g), [&](auto w) __attribute__((always_inline)) {
-    for_pairs_g(g, pv, i, [&](uint32_t j) __attribute__((always_inline)) { return Nb{at(X, j), at(LAM, j)}; },
+  const uint32_t cw = pv.cnt[i];
+  by_wrap(wave_fast(xi, cw, g), [&](auto w) __attribute__((always_inline)) {
+    for_pairs_g(g, pv, i, cw, w, [&](uint32_t j) __attribute__((always_inline)) { return Nb{at(X, j), at(LAM, j)}; },
                 [&](uint32_t, const Nb& a) __attribute__((always_inline)) {
       const Vec<R> d = rel(xi, a.x, g, w);
       const R r = pair_len(len2(d));
-      s = add2(s, smul2(li + a.l, spiky(d, r, g)));
+      s = add2(s, smul2(li + a.l, spiky(d, r, g, guarded(w))));
     });
   });
   const Vec<R> dp = mul2(s, g.m_over_rest);
@@ -669,8 +693,9 @@ BXKRT bx_vel_density(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, 
     V[i] = v;
     vv = (double)v.x * (double)v.x + (double)v.y * (double)v.y;
     R rho = g.w0;
-    by_wrap(wave_inside(xi, g), [&](auto w) __attribute__((always_inline)) {
-      for_pairs_g(g, pv, i, [&](uint32_t j) __attribute__((always_inline)) { return at(X, j); },
+    const uint32_t cw = pv.cnt[i];
+    by_wrap(wave_fast(xi, cw, g), [&](auto w) __attribute__((always_inline)) {
+      for_pairs_g(g, pv, i, cw, w, [&](uint32_t j) __attribute__((always_inline)) { return at(X, j); },
                   [&](uint32_t, Vec<R> xj) __attribute__((always_inline)) {
         rho += g.mass * poly6(len2(rel(xi, xj, g, w)), g);
       });
@@ -690,15 +715,15 @@ BDEV void morris(const BoxGeo<R>& g, Wrap<W> wr, Vec<R> xi, R rho_i, Vec<R> xj, 
   const R rho_j = mfmax(dens_j, (R)1e-6f);
   const Vec<R> d = rel(xi, xj, g, wr);
   const R r = pair_len(len2(d));
-  const Vec<R> dr = pair_div(d, r);
+  const Vec<R> dr = pair_div(d, r, guarded(wr));
   const Vec<R> gr = (r <= (R)1e-9f || r >= g.h) ? mk((R)0, (R)0) : mul2(dr, g.ks * (g.h - r) * (g.h - r));
   const R num = (g.mu2m / (rho_i * rho_j)) * mfabs(dot2(d, gr));
   w = num / (len2(d) + g.eta2);
   e = r > (R)1e-9f ? dr : mk((R)1, (R)0);
 }
-template <class R> BDEV Vec<R> pair_unit(Vec<R> d) {
+template <class R> BDEV Vec<R> pair_unit(Vec<R> d, bool guard) {
   const R r = pair_len(len2(d));
-  return r > (R)1e-9f ? pair_div(d, r) : mk((R)1, (R)0);
+  return r > (R)1e-9f ? pair_div(d, r, guard) : mk((R)1, (R)0);
 }
 // (L v)_i with the weights computed here; when `store` the weights of a tabled particle are kept
 // (column-major, like the pair table) for the operator applications of this step's CG.
@@ -707,12 +732,13 @@ BDEV Vec<R> lap_row(const BoxGeo<R>& g, uint32_t i, const PairView<R>& pv, const
                     const R* __restrict__ DENS, const Vec<R>* __restrict__ in, bool store, R* __restrict__ CW) {
   const Vec<R> xi = X[i], ini = in[i];
   const R rho_i = mfmax(DENS[i], (R)1e-6f);
-  const bool keep = store && pv.cnt[i] <= BX_KMAX;
+  const uint32_t cw = pv.cnt[i];
+  const bool keep = store && pcount(cw) <= BX_KMAX;
   Vec<R> s = mk((R)0, (R)0);
   uint32_t k = 0;
   struct Nb { Vec<R> x; R d; Vec<R> v; };
-  by_wrap(wave_inside(xi, g), [&](auto wr) __attribute__((always_inline)) {
-    for_pairs_g(g, pv, i, [&](uint32_t j) __attribute__((always_inline)) {
+  by_wrap(wave_fast(xi, cw, g), [&](auto wr) __attribute__((always_inline)) {
+    for_pairs_g(g, pv, i, cw, wr, [&](uint32_t j) __attribute__((always_inline)) {
       return Nb{at(X, j), at(DENS, j), at(in, j)};
     },
                 [&](uint32_t, const Nb& a) __attribute__((always_inline)) {
@@ -793,21 +819,21 @@ BXKRT bx_cg_dir(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const
     const bool first = t == 1u;
     const Vec<R> p = first ? Pold[i] : add2(Rv[i], mul2(Pold[i], beta));
     Vec<R> s = mk((R)0, (R)0);
-    const uint32_t c = pv.cnt[i];
+    const uint32_t cw = pv.cnt[i], c = pcount(cw);
     const Vec<R> xi = X[i];
-    by_wrap(wave_inside(xi, g), [&](auto wr) __attribute__((always_inline)) {
+    by_wrap(wave_fast(xi, cw, g), [&](auto wr) __attribute__((always_inline)) {
       if (c <= BX_KMAX) {
         auto pdir = [&](uint32_t j) __attribute__((always_inline)) {
           return first ? at(Pold, j) : add2(at(Rv, j), mul2(at(Pold, j), beta));
         };
         auto term = [&](uint32_t kk, Vec<R> pj, Vec<R> xj) __attribute__((always_inline)) {
-          const Vec<R> e = pair_unit<R>(rel(xi, xj, g, wr));
+          const Vec<R> e = pair_unit<R>(rel(xi, xj, g, wr), guarded(wr));
           s = add2(s, smul2((R)4 * at(CW + (size_t)kk * g.n, i) * dot2(e, sub2(p, pj)), e));
         };
         uint32_t k = 0;
         for (; k + 4u <= c; k += 4u) {
-          const uint32_t j0 = nb_dec(at(tcol(pv, k), i), i, g.n), j1 = nb_dec(at(tcol(pv, k + 1u), i), i, g.n);
-          const uint32_t j2 = nb_dec(at(tcol(pv, k + 2u), i), i, g.n), j3 = nb_dec(at(tcol(pv, k + 3u), i), i, g.n);
+          const uint32_t j0 = nb_dec(at(tcol(pv, k), i), i, g.n, wr), j1 = nb_dec(at(tcol(pv, k + 1u), i), i, g.n, wr);
+          const uint32_t j2 = nb_dec(at(tcol(pv, k + 2u), i), i, g.n, wr), j3 = nb_dec(at(tcol(pv, k + 3u), i), i, g.n, wr);
           const Vec<R> p0 = pdir(j0), p1 = pdir(j1), p2 = pdir(j2), p3 = pdir(j3);
           const Vec<R> x0 = at(X, j0), x1 = at(X, j1), x2 = at(X, j2), x3 = at(X, j3);
           term(k, p0, x0);
@@ -816,7 +842,7 @@ BXKRT bx_cg_dir(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const
           term(k + 3u, p3, x3);
         }
         for (; k < c; k++) {
-          const uint32_t j = nb_dec(at(tcol(pv, k), i), i, g.n);
+          const uint32_t j = nb_dec(at(tcol(pv, k), i), i, g.n, wr);
           term(k, pdir(j), at(X, j));
         }
       } else {
@@ -1109,6 +1135,7 @@ bool box_geometry(const mg_box_config& c, BoxGeo<float>& g, std::string& err) {
   g.eta2 = 0.01f * g.h * g.h;
   g.lo = 2.0f * g.h;
   g.hi = g.L - g.lo;
+  if (!(g.h >= 0x1p-60f)) g.hi = -1.0f;  // no interior: every wave takes the minimum image
   g.halt = nullptr;
   g.nblk = (g.n + BX_BLOCK - 1) / BX_BLOCK;
   g.xcd_chunk = (g.nblk % 8u == 0u) ? g.nblk / 8u : 0u;
@@ -1146,6 +1173,7 @@ BoxGeo<double> promote(const BoxGeo<float>& f, const mg_box_config& c) {
   g.eta2 = (double)0.01f * g.h * g.h;
   g.lo = 2.0 * g.h;
   g.hi = g.L - g.lo;
+  if (!(g.h >= 0x1p-60)) g.hi = -1.0;
   g.halt = nullptr;
   return g;
 }
