@@ -66,7 +66,13 @@ enum { BXT_GRID = 0, BXT_LAMBDA, BXT_DP, BXT_VEL, BXT_CGINIT, BXT_SPMV, BXT_UPDA
 #define BXK extern "C" __global__ void __launch_bounds__(BX_BLOCK)
 #define BXKR extern "C" __global__ void __launch_bounds__(BX_RBLOCK)
 #define BXKT template <class R> __global__ void __launch_bounds__(BX_BLOCK)
-#define BXKRT template <class R> __global__ void __launch_bounds__(BX_RBLOCK)
+// Waves per SIMD the 1024-lane kernels are compiled for: two workgroups per CU need 8, i.e. at most
+// 64 VGPRs and 80 SGPRs per wave (SGPRs are allocated in blocks of 16, so 81 already allows only
+// one workgroup and halves the resident waves). The FP32 kernels fit; the FP64 ones do not (they
+// would spill), so they keep one workgroup per CU.
+template <class R> struct BxRWaves { static constexpr int value = 1; };
+template <> struct BxRWaves<float> { static constexpr int value = 8; };
+#define BXKRT template <class R> __global__ void __launch_bounds__(BX_RBLOCK, BxRWaves<R>::value)
 #define BDEV __device__ __attribute__((always_inline)) inline
 #define BDEVX __device__ __attribute__((always_inline)) inline  // explicit specializations
 
@@ -88,8 +94,10 @@ template <class R> struct BoxGeo {
 };
 
 // Device scalars of the CG loop (pbf.rs:423-480).
+// rr[u & 1] = r(u).r(u) once a CG kernel has summed it (block 0 stores it for later kernels).
 struct BoxScal {
   double bnorm, target, res;
+  double rr[2];
   uint32_t it, done, max_cg;
   float tol;
 };
@@ -308,6 +316,36 @@ BDEV double sum_parts(const double* __restrict__ part, uint32_t nb) {
   double s = 0.0;
   for (uint32_t k = threadIdx.x; k < nb; k += NT) s += part[k];
   return block_sum<NT>(s);
+}
+// Two such sums in one workgroup reduction (each component in exactly sum_parts' order).
+template <int NT>
+BDEV double2 sum_parts2(const double* __restrict__ pa, const double* __restrict__ pb, uint32_t nb) {
+  __shared__ double2 s_w2[NT / 64];
+  __shared__ double2 s_tot2;
+  double2 v = make_double2(0.0, 0.0);
+  for (uint32_t k = threadIdx.x; k < nb; k += NT) {
+    v.x += pa[k];
+    v.y += pb[k];
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    v.x += __shfl_down(v.x, o, 64);
+    v.y += __shfl_down(v.y, o, 64);
+  }
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) s_w2[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double2 r = make_double2(0.0, 0.0);
+    for (int k = 0; k < NT / 64; k++) {
+      r.x += s_w2[k].x;
+      r.y += s_w2[k].y;
+    }
+    s_tot2 = r;
+  }
+  __syncthreads();
+  const double2 r = s_tot2;
+  __syncthreads();
+  return r;
 }
 
 // Per-step pair table: the neighbours of sorted particle i at the predicted positions, in the
@@ -775,36 +813,52 @@ BXKRT bx_cg_init(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, cons
 // CG iteration t (pbf.rs:423-480), first half. Every block evaluates the scalars from the partial
 // sums of the previous kernels (same fixed order everywhere): for t = 1 the target tol·|b| and the
 // initial residual; for t > 1 the convergence test on r(t-1).r(t-1), the iteration cap, and
-// beta = rs(t-1)/rs(t-2). The direction p = r + beta p_old is formed on the fly for the particle
+// beta = rs(t-1)/rs(t-2), with rs(t-2) read from where iteration t-1 stored it. The direction p = r + beta p_old is formed on the fly for the particle
 // and its neighbours (no separate direction kernel), then Ap = p + dt L p with the kept weights
 // and the unit directions recomputed from the positions (the same values the set-up used).
 BXKRT bx_cg_dir(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const R* __restrict__ DENS,
                 const R* __restrict__ CW, const Vec<R>* __restrict__ Rv,
                 const Vec<R>* __restrict__ Pold, Vec<R>* __restrict__ Pnew, Vec<R>* __restrict__ AP,
                 const double* __restrict__ part_b, const double* __restrict__ part_rr_prev,
-                const double* __restrict__ part_rr_prev2, double* __restrict__ part_pap, BoxScal* __restrict__ sc,
-                uint32_t t) {
+                double* __restrict__ part_pap, BoxScal* __restrict__ sc, uint32_t t) {
   if (*g.halt | sc->done) return;  // both words loaded at once
+  const uint32_t i = tile_id_red(g) * BX_RBLOCK + threadIdx.x;
+  const bool first = t == 1u;
+  // the particle's own inputs are requested before the scalar reduction (they do not depend on it)
+  Vec<R> xi = mk((R)0, (R)0), ri = xi, pi = xi;
+  uint32_t cw = 0;
+  if (i < g.n) {
+    xi = X[i];
+    pi = Pold[i];
+    if (!first) ri = Rv[i];
+    cw = pv.cnt[i];
+  }
   double target;
   bool stop = false;
   R beta = (R)0;
-  if (t == 1u) {
-    const double bn = sqrt(sum_parts<BX_RBLOCK>(part_b, g.nred));
+  if (first) {
+    const double2 s2 = sum_parts2<BX_RBLOCK>(part_b, part_rr_prev, g.nred);
+    const double bn = sqrt(s2.x), rs0 = s2.y;
     target = (double)sc->tol * bn;
-    const double rs0 = sum_parts<BX_RBLOCK>(part_rr_prev, g.nred);
     if (!(bn > 0.0) || !isfinite(bn) || sqrt(rs0) <= target) stop = true;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       sc->bnorm = bn;
       sc->target = target;
       sc->res = (bn > 0.0 && isfinite(bn)) ? sqrt(rs0) : 0.0;
+      sc->rr[0] = rs0;
     }
   } else {
     target = sc->target;
+    // r(t-1).r(t-1) from the partials of the last update; r(t-2).r(t-2) as the previous
+    // iteration's kernel summed it (the same value, stored in the other slot)
+    const double rs2 = sc->rr[t & 1u];
     const double rs1 = sum_parts<BX_RBLOCK>(part_rr_prev, g.nred);
-    const double rs2 = sum_parts<BX_RBLOCK>(part_rr_prev2, g.nred);
     if (sqrt(rs1) <= target || t - 1u >= sc->max_cg) stop = true;
     beta = (R)(rs1 / rs2);
-    if (blockIdx.x == 0 && threadIdx.x == 0) sc->res = sqrt(rs1);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc->res = sqrt(rs1);
+      sc->rr[(t - 1u) & 1u] = rs1;
+    }
   }
   if (stop) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -813,14 +867,11 @@ BXKRT bx_cg_dir(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const
     }
     return;
   }
-  const uint32_t i = tile_id_red(g) * BX_RBLOCK + threadIdx.x;
   double pap = 0.0;
   if (i < g.n) {
-    const bool first = t == 1u;
-    const Vec<R> p = first ? Pold[i] : add2(Rv[i], mul2(Pold[i], beta));
+    const Vec<R> p = first ? pi : add2(ri, mul2(pi, beta));
     Vec<R> s = mk((R)0, (R)0);
-    const uint32_t cw = pv.cnt[i], c = pcount(cw);
-    const Vec<R> xi = X[i];
+    const uint32_t c = pcount(cw);
     by_wrap(wave_fast(xi, cw, g), [&](auto wr) __attribute__((always_inline)) {
       if (c <= BX_KMAX) {
         auto pdir = [&](uint32_t j) __attribute__((always_inline)) {
@@ -870,9 +921,18 @@ BXKRT bx_cg_dir(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const
 // r -= alpha Ap, partial sums of r.r for the next iteration's test.
 BXKRT bx_cg_upd(BoxGeo<R> g, Vec<R>* __restrict__ V, Vec<R>* __restrict__ Rv, const Vec<R>* __restrict__ P,
                 const Vec<R>* __restrict__ AP, const double* __restrict__ part_pap,
-                const double* __restrict__ part_rr_prev, double* __restrict__ part_rr_out,
-                BoxScal* __restrict__ sc, uint32_t t) {
+                double* __restrict__ part_rr_out, BoxScal* __restrict__ sc, uint32_t t) {
   if (*g.halt | sc->done) return;  // both words loaded at once
+  const uint32_t i = blockIdx.x * BX_RBLOCK + threadIdx.x;
+  // the particle's vectors are requested before the scalar reduction (they do not depend on it)
+  Vec<R> vi = mk((R)0, (R)0), pi = vi, ri = vi, api = vi;
+  if (i < g.n) {
+    vi = V[i];
+    pi = P[i];
+    ri = Rv[i];
+    api = AP[i];
+  }
+  const double rs_old = sc->rr[(t - 1u) & 1u];  // r(t-1).r(t-1), stored by this iteration's first half
   const double pap = sum_parts<BX_RBLOCK>(part_pap, g.nred);
   if (!(pap > 0.0) || !isfinite(pap)) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -881,13 +941,11 @@ BXKRT bx_cg_upd(BoxGeo<R> g, Vec<R>* __restrict__ V, Vec<R>* __restrict__ Rv, co
     }
     return;
   }
-  const double rs_old = sum_parts<BX_RBLOCK>(part_rr_prev, g.nred);
   const R a = (R)(rs_old / pap);
-  const uint32_t i = blockIdx.x * BX_RBLOCK + threadIdx.x;
   double rr = 0.0;
   if (i < g.n) {
-    V[i] = add2(V[i], mul2(P[i], a));
-    const Vec<R> r = sub2(Rv[i], mul2(AP[i], a));
+    V[i] = add2(vi, mul2(pi, a));
+    const Vec<R> r = sub2(ri, mul2(api, a));
     Rv[i] = r;
     rr = (double)r.x * (double)r.x + (double)r.y * (double)r.y;
   }
@@ -1337,18 +1395,16 @@ template <class R> void enqueue_cg_chunk(mg_box* B, uint32_t iters, uint32_t* ve
     Vec<R>* pold = vp<R>((t & 1u) ? B->P : B->P2);
     Vec<R>* pnew = vp<R>((t & 1u) ? B->P2 : B->P);
     const double* rr_prev = B->part_rr + ((t - 1u) & 1u) * nr;
-    const double* rr_prev2 = B->part_rr + (t & 1u) * nr;
     double* rr_out = B->part_rr + (t & 1u) * nr;
     {
       Tic tt(B, BXT_SPMV);
       hipLaunchKernelGGL(bx_cg_dir<R>, gr, rblk, 0, st, g, pv, vp<R>(B->X), sp<R>(B->DENS), sp<R>(B->cw),
-                         vp<R>(B->R), pold, pnew, vp<R>(B->AP), B->part_b, rr_prev, rr_prev2,
-                         B->part_pap, B->sc, t);
+                         vp<R>(B->R), pold, pnew, vp<R>(B->AP), B->part_b, rr_prev, B->part_pap, B->sc, t);
     }
     {
       Tic tt(B, BXT_UPDATE);
       hipLaunchKernelGGL(bx_cg_upd<R>, gr, rblk, 0, st, g, vp<R>(B->VN), vp<R>(B->R), pnew, vp<R>(B->AP),
-                         B->part_pap, rr_prev, rr_out, B->sc, t);
+                         B->part_pap, rr_out, B->sc, t);
     }
   }
   const uint32_t last = B->cg_t - 1u;
