@@ -65,7 +65,8 @@ enum { BXT_GRID = 0, BXT_LAMBDA, BXT_DP, BXT_VEL, BXT_CGINIT, BXT_SPMV, BXT_UPDA
 // (float: FP32 state with FP64 reductions; double: FP64 throughout)
 #define BXK extern "C" __global__ void __launch_bounds__(BX_BLOCK)
 #define BXKR extern "C" __global__ void __launch_bounds__(BX_RBLOCK)
-#define BXKT template <class R> __global__ void __launch_bounds__(BX_BLOCK)
+// the 256-lane state kernels fit 8 waves per SIMD in both precisions; the bound keeps them there
+#define BXKT template <class R> __global__ void __launch_bounds__(BX_BLOCK, 8)
 // Waves per SIMD the 1024-lane kernels are compiled for: two workgroups per CU need 8, i.e. at most
 // 64 VGPRs and 80 SGPRs per wave (SGPRs are allocated in blocks of 16, so 81 already allows only
 // one workgroup and halves the resident waves). The FP32 kernels fit; the FP64 ones do not (they
