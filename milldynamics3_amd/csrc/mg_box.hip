@@ -49,8 +49,11 @@
 #define BX_WINDOW 16  // steps enqueued between host waits in a multi-step call
 #define BX_SPARE_IT 1  // CG iterations launched beyond the predicted count in a window
 // Loop-shape switch (both forms give identical results; the default is the measured faster one).
+#ifndef BX_GATHER
+#define BX_GATHER 8    // table entries whose gathers are issued together in the FP32 density passes
+#endif
 #ifndef BX_BATCH
-#define BX_BATCH 1     // issue the gathers of four table entries before using them
+#define BX_BATCH 1     // issue the gathers of BX_GATHER table entries before using them
 #endif
 #ifndef BX_KMAX
 #define BX_KMAX 32  // pairs kept per particle in the per-step pair table (more: the kernel rescans)
@@ -442,24 +445,27 @@ BDEV void for_pairs(const BoxGeo<R>& g, const PairView<R>& pv, uint32_t i, F&& f
 // As for_pairs, with the per-neighbour gathers split out: load(j) returns what use(j, v) needs,
 // and the loads of four table entries are all issued before the first use (memory-level
 // parallelism; without it every neighbour costs two dependent round trips).
-// cntw = cnt[i] (read by the caller for the wave's Wrap choice).
-template <class R, bool W, class L, class F>
+// cntw = cnt[i] (read by the caller for the wave's Wrap choice). G table entries are gathered
+// together: 8 in the FP32 density passes, 4 where more would not fit the 64-VGPR budget of 8
+// waves per SIMD (the FP64 kernels and the Morris-weight set-up).
+template <class R> struct Gather { static constexpr int value = 4; };
+template <> struct Gather<float> { static constexpr int value = BX_GATHER; };
+template <int G = 0, class R, bool W, class L, class F>
 BDEV void for_pairs_g(const BoxGeo<R>& g, const PairView<R>& pv, uint32_t i, uint32_t cntw, Wrap<W> w, L&& load,
                       F&& use) {
+  constexpr int NG = G ? G : Gather<R>::value;
   const uint32_t c = pcount(cntw);
   if (c <= BX_KMAX) {
     uint32_t k = 0;
-    for (; BX_BATCH && k + 4u <= c; k += 4u) {
-      const uint32_t j0 = nb_dec(at(tcol(pv, k), i), i, g.n, w), j1 = nb_dec(at(tcol(pv, k + 1u), i), i, g.n, w);
-      const uint32_t j2 = nb_dec(at(tcol(pv, k + 2u), i), i, g.n, w), j3 = nb_dec(at(tcol(pv, k + 3u), i), i, g.n, w);
-      const auto a0 = load(j0);
-      const auto a1 = load(j1);
-      const auto a2 = load(j2);
-      const auto a3 = load(j3);
-      use(j0, a0);
-      use(j1, a1);
-      use(j2, a2);
-      use(j3, a3);
+    for (; BX_BATCH && k + (uint32_t)NG <= c; k += NG) {
+      uint32_t jj[NG];
+      decltype(load(0u)) aa[NG];
+#pragma unroll
+      for (int q = 0; q < NG; q++) jj[q] = nb_dec(at(tcol(pv, k + q), i), i, g.n, w);
+#pragma unroll
+      for (int q = 0; q < NG; q++) aa[q] = load(jj[q]);
+#pragma unroll
+      for (int q = 0; q < NG; q++) use(jj[q], aa[q]);
     }
     for (; k < c; k++) {
       const uint32_t j = nb_dec(at(tcol(pv, k), i), i, g.n, w);
@@ -777,7 +783,7 @@ BDEV Vec<R> lap_row(const BoxGeo<R>& g, uint32_t i, const PairView<R>& pv, const
   uint32_t k = 0;
   struct Nb { Vec<R> x; R d; Vec<R> v; };
   by_wrap(wave_fast(xi, cw, g), [&](auto wr) __attribute__((always_inline)) {
-    for_pairs_g(g, pv, i, cw, wr, [&](uint32_t j) __attribute__((always_inline)) {
+    for_pairs_g<4>(g, pv, i, cw, wr, [&](uint32_t j) __attribute__((always_inline)) {
       return Nb{at(X, j), at(DENS, j), at(in, j)};
     },
                 [&](uint32_t, const Nb& a) __attribute__((always_inline)) {
@@ -883,15 +889,18 @@ BXKRT bx_cg_dir(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const
           s = add2(s, smul2((R)4 * at(CW + (size_t)kk * g.n, i) * dot2(e, sub2(p, pj)), e));
         };
         uint32_t k = 0;
-        for (; k + 4u <= c; k += 4u) {
-          const uint32_t j0 = nb_dec(at(tcol(pv, k), i), i, g.n, wr), j1 = nb_dec(at(tcol(pv, k + 1u), i), i, g.n, wr);
-          const uint32_t j2 = nb_dec(at(tcol(pv, k + 2u), i), i, g.n, wr), j3 = nb_dec(at(tcol(pv, k + 3u), i), i, g.n, wr);
-          const Vec<R> p0 = pdir(j0), p1 = pdir(j1), p2 = pdir(j2), p3 = pdir(j3);
-          const Vec<R> x0 = at(X, j0), x1 = at(X, j1), x2 = at(X, j2), x3 = at(X, j3);
-          term(k, p0, x0);
-          term(k + 1u, p1, x1);
-          term(k + 2u, p2, x2);
-          term(k + 3u, p3, x3);
+        constexpr int NG = 4;
+        for (; k + (uint32_t)NG <= c; k += NG) {
+          uint32_t jj[NG];
+          Vec<R> pp[NG], xx[NG];
+#pragma unroll
+          for (int q = 0; q < NG; q++) jj[q] = nb_dec(at(tcol(pv, k + q), i), i, g.n, wr);
+#pragma unroll
+          for (int q = 0; q < NG; q++) pp[q] = pdir(jj[q]);
+#pragma unroll
+          for (int q = 0; q < NG; q++) xx[q] = at(X, jj[q]);
+#pragma unroll
+          for (int q = 0; q < NG; q++) term(k + q, pp[q], xx[q]);
         }
         for (; k < c; k++) {
           const uint32_t j = nb_dec(at(tcol(pv, k), i), i, g.n, wr);
