@@ -43,7 +43,7 @@
 
 #define BX_BLOCK 256
 #define BX_RBLOCK 1024  // workgroup size of the kernels that produce FP64 partial sums
-#define BX_SCAN_ITEMS 16
+#define BX_SCAN_ITEMS 4   // one 16-byte vector per lane (coalesced)
 #define BX_SCAN_CHUNK (BX_BLOCK * BX_SCAN_ITEMS)
 #define BX_CG_CHUNK 4
 #define BX_WINDOW 16  // steps enqueued between host waits in a multi-step call
@@ -515,7 +515,7 @@ BXKT bx_predict(BoxGeo<R> g, Vec<R>* __restrict__ X, const Vec<R>* __restrict__ 
 }
 
 // Exclusive scan of a[0..m) in two launches: chunk totals, then each chunk's scan offset by the sum
-// of the totals before it (at most BX_SCAN_CHUNK chunks).
+// of the totals before it.
 BDEV uint32_t block_excl_scan(uint32_t v, uint32_t& total) {
   __shared__ uint32_t s_w[BX_BLOCK / 64];
   const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -536,34 +536,45 @@ BDEV uint32_t block_excl_scan(uint32_t v, uint32_t& total) {
   total = tot;
   return before + x - v;
 }
+// Lane t of a chunk holds items [4t, 4t + 4) as one 16-byte vector (the arrays are 256-byte
+// aligned and chunks start at multiples of 4 items); a partial vector at the end is read and
+// written item by item.
+BDEV uint4 scan_load4(const uint32_t* __restrict__ a, uint32_t base, uint32_t m) {
+  if (base + 4u <= m) return *reinterpret_cast<const uint4*>(a + base);
+  uint4 v = make_uint4(0u, 0u, 0u, 0u);
+  if (base < m) v.x = a[base];
+  if (base + 1u < m) v.y = a[base + 1u];
+  if (base + 2u < m) v.z = a[base + 2u];
+  return v;
+}
 BXK bx_scan_totals(const uint32_t* __restrict__ halt, const uint32_t* __restrict__ a, uint32_t m,
                    uint32_t* __restrict__ bsum) {
   if (*halt) return;
   const uint32_t base = blockIdx.x * BX_SCAN_CHUNK + threadIdx.x * BX_SCAN_ITEMS;
-  uint32_t s = 0;
-  for (int k = 0; k < BX_SCAN_ITEMS; k++) s += base + k < m ? a[base + k] : 0u;
+  const uint4 v = scan_load4(a, base, m);
   uint32_t tot;
-  block_excl_scan(s, tot);
+  block_excl_scan(v.x + v.y + v.z + v.w, tot);
   if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 BXK bx_scan_apply(const uint32_t* __restrict__ halt, uint32_t* __restrict__ a, uint32_t m,
                   const uint32_t* __restrict__ bsum) {
   if (*halt) return;
+  const uint32_t base = blockIdx.x * BX_SCAN_CHUNK + threadIdx.x * BX_SCAN_ITEMS;
+  const uint4 v = scan_load4(a, base, m);  // requested before the totals are summed
   uint32_t before = 0;  // totals of the chunks before this one
   for (uint32_t k = threadIdx.x; k < blockIdx.x; k += BX_BLOCK) before += bsum[k];
   uint32_t chunks_before;
   block_excl_scan(before, chunks_before);
-  const uint32_t base = blockIdx.x * BX_SCAN_CHUNK + threadIdx.x * BX_SCAN_ITEMS;
-  uint32_t s = 0;
-  for (int k = 0; k < BX_SCAN_ITEMS; k++) s += base + k < m ? a[base + k] : 0u;
   uint32_t tot;
-  uint32_t off = block_excl_scan(s, tot) + chunks_before;
-  for (int k = 0; k < BX_SCAN_ITEMS; k++)
-    if (base + k < m) {
-      uint32_t t = a[base + k];
-      a[base + k] = off;
-      off += t;
-    }
+  const uint32_t off = block_excl_scan(v.x + v.y + v.z + v.w, tot) + chunks_before;
+  const uint4 o = make_uint4(off, off + v.x, off + v.x + v.y, off + v.x + v.y + v.z);
+  if (base + 4u <= m) {
+    *reinterpret_cast<uint4*>(a + base) = o;
+  } else {
+    if (base < m) a[base] = o.x;
+    if (base + 1u < m) a[base + 1u] = o.y;
+    if (base + 2u < m) a[base + 2u] = o.z;
+  }
 }
 BXK bx_scatter(const uint32_t* __restrict__ halt, uint32_t n, const uint32_t* __restrict__ key,
                const uint32_t* __restrict__ start, uint32_t* __restrict__ cursor, uint32_t* __restrict__ slot) {
@@ -1621,7 +1632,7 @@ int mg_box_create(const mg_box_config* cfg, mg_box** out, char* err, size_t errl
   B->nred = B->gf.nred;
   B->ncell = B->gf.ncx * B->gf.ncx;
   B->nscan = (B->ncell + 1 + BX_SCAN_CHUNK - 1) / BX_SCAN_CHUNK;
-  if (B->nscan > BX_SCAN_CHUNK) { box_set_err(err, errlen, "too many cells"); delete B; return MG_E_PARAMS; }
+  if (B->nscan > (1u << 16)) { box_set_err(err, errlen, "too many cells"); delete B; return MG_E_PARAMS; }
   const size_t rs = B->f64 ? 8 : 4;   // bytes of one real
   const size_t vb = 2 * rs * n;       // bytes of one Vec<R> array
   size_t off = 0;
