@@ -637,27 +637,35 @@ BXKT bx_gather(BoxGeo<R> g, const uint32_t* __restrict__ slot, const Vec<R>* __r
   idb[q] = id[i];
 }
 
-// Records the pair table of particle i while f(j, d, r2) sees every pair.
+// Records the pair table of particle i while f(j, d, r2) sees every pair; returns the count word.
+// With lrow the row's entries are also kept in LDS (entry k at lrow[k * BX_BLOCK]).
 template <class R, bool W, class F>
-BDEV void table_row(const BoxGeo<R>& g, const Vec<R>* __restrict__ XP, const uint32_t* __restrict__ start,
-                    int16_t* __restrict__ nb, uint32_t* __restrict__ cnt, uint32_t i, Vec<R> xpi, Wrap<W> w, F&& f) {
+BDEV uint32_t table_row(const BoxGeo<R>& g, const Vec<R>* __restrict__ XP, const uint32_t* __restrict__ start,
+                        int16_t* __restrict__ nb, uint32_t* __restrict__ cnt, uint32_t i, Vec<R> xpi, Wrap<W> w,
+                        int16_t* lrow, F&& f) {
   uint32_t c = 0;
   bool fits = true, wrapped = false;
   scan_pairs_d(g, XP, start, i, xpi, w, [&](uint32_t j, Vec<R> d, R r2) __attribute__((always_inline)) {
     int16_t e = 0;
     fits = nb_enc(j, i, g.n, e, wrapped) && fits;
-    if (c < BX_KMAX) nb[(size_t)c * g.n + i] = e;
+    if (c < BX_KMAX) {
+      nb[(size_t)c * g.n + i] = e;
+      if (lrow) lrow[c * BX_BLOCK] = e;
+    }
     c++;
     f(j, d, r2);
   });
-  cnt[i] = (fits ? c : (c > BX_KMAX ? c : BX_KMAX + 1u)) | (wrapped ? BX_CNT_WRAPPED : 0u);
+  const uint32_t cw = (fits ? c : (c > BX_KMAX ? c : BX_KMAX + 1u)) | (wrapped ? BX_CNT_WRAPPED : 0u);
+  cnt[i] = cw;
+  return cw;
 }
 BXKT bx_pairs(BoxGeo<R> g, const Vec<R>* __restrict__ XP, const uint32_t* __restrict__ start,
               int16_t* __restrict__ nb, uint32_t* __restrict__ cnt) {
   if (*g.halt) return;
   const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
   if (i >= g.n) return;
-  table_row(g, XP, start, nb, cnt, i, XP[i], Wrap<true>{}, [](uint32_t, Vec<R>, R) __attribute__((always_inline)) {});
+  table_row(g, XP, start, nb, cnt, i, XP[i], Wrap<true>{}, nullptr,
+            [](uint32_t, Vec<R>, R) __attribute__((always_inline)) {});
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -698,18 +706,48 @@ BXKT bx_lambda(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, R* __r
   LAM[i] = lambda_of(g, a);
 }
 // First density pass A of a step, fused with the pair-table build: right after the reorder the
-// positions X equal the predicted ones XP, so the membership test's pair vectors are the ones
-// pass A needs (same values, same order as bx_pairs followed by bx_lambda).
+// positions X equal the predicted ones XP, so pass A runs on XP (same values, same order as
+// bx_pairs followed by bx_lambda). Two phases: the 3x3-cell scan only tests membership and records
+// the row (in the table and in LDS), then pass A walks the row from LDS. Summing inside the scan
+// would issue the lambda arithmetic for every candidate step in which any lane of the wave has a
+// member (about three times the member count).
 BXKT bx_pairs_lambda(BoxGeo<R> g, const Vec<R>* __restrict__ XP, const uint32_t* __restrict__ start,
                      int16_t* __restrict__ nb, uint32_t* __restrict__ cnt, R* __restrict__ LAM) {
+  __shared__ int16_t s_row[BX_KMAX * BX_BLOCK];  // each lane's table row (16 KB per workgroup)
   if (*g.halt) return;
   const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
   if (i >= g.n) return;
-  LamAcc<R> a{g.w0, mk((R)0, (R)0), (R)0};
+  int16_t* lrow = s_row + threadIdx.x;
   const Vec<R> xpi = XP[i];
+  uint32_t cw = 0;
   by_wrap(wave_inside(xpi, g), [&](auto w) __attribute__((always_inline)) {
-    table_row(g, XP, start, nb, cnt, i, xpi, w,
-              [&](uint32_t, Vec<R> d, R r2) __attribute__((always_inline)) { lambda_term(g, w, d, r2, a); });
+    cw = table_row(g, XP, start, nb, cnt, i, xpi, w, lrow, [](uint32_t, Vec<R>, R) __attribute__((always_inline)) {});
+  });
+  // pass A over the row just recorded, in its order (lanes read only their own LDS column)
+  LamAcc<R> a{g.w0, mk((R)0, (R)0), (R)0};
+  const uint32_t c = pcount(cw);
+  by_wrap(wave_fast(xpi, cw, g), [&](auto w) __attribute__((always_inline)) {
+    auto term = [&](Vec<R> xj) __attribute__((always_inline)) {
+      const Vec<R> d = rel(xpi, xj, g, w);
+      lambda_term(g, w, d, len2(d), a);
+    };
+    if (c <= BX_KMAX) {
+      constexpr int NG = Gather<R>::value;
+      uint32_t k = 0;
+      for (; k + (uint32_t)NG <= c; k += NG) {
+        uint32_t jj[NG];
+        Vec<R> xx[NG];
+#pragma unroll
+        for (int q = 0; q < NG; q++) jj[q] = nb_dec(lrow[(k + q) * BX_BLOCK], i, g.n, w);
+#pragma unroll
+        for (int q = 0; q < NG; q++) xx[q] = at(XP, jj[q]);
+#pragma unroll
+        for (int q = 0; q < NG; q++) term(xx[q]);
+      }
+      for (; k < c; k++) term(at(XP, nb_dec(lrow[k * BX_BLOCK], i, g.n, w)));
+    } else {
+      scan_pairs(g, XP, start, i, xpi, [&](uint32_t j) __attribute__((always_inline)) { term(XP[j]); });
+    }
   });
   LAM[i] = lambda_of(g, a);
 }
