@@ -353,8 +353,8 @@ BDEV double2 sum_parts2(const double* __restrict__ pa, const double* __restrict_
 }
 
 // Per-step pair table: the neighbours of sorted particle i at the predicted positions, in the
-// reference's per-particle order, column-major (nb[k * n + i]) so lane-adjacent particles read
-// adjacent entries. An entry is the signed 16-bit offset j - i taken modulo n into [-n/2, n/2]
+// reference's per-particle order, in quads of four entries per particle (see tquad) so
+// lane-adjacent particles read adjacent 8-byte words. An entry is the signed 16-bit offset j - i taken modulo n into [-n/2, n/2]
 // (after the cell sort a neighbour sits within about two cell rows of i, or as far on the other
 // side of the periodic y seam). cnt[i] is the true count; a particle with more than BX_KMAX pairs,
 // or with a neighbour whose offset does not fit 16 bits, has cnt[i] > BX_KMAX and is handled by
@@ -366,12 +366,20 @@ template <class R> struct PairView {
   const uint32_t* __restrict__ cnt;
   uint32_t n;
 };
-// Column k of the table (k is wave-uniform in the pair loops, so the address stays scalar).
-template <class R> BDEV const int16_t* tcol(const PairView<R>& pv, uint32_t k) { return pv.nb + (size_t)k * pv.n; }
+// The table is stored in quads: entries 4q .. 4q + 3 of particle i are the 8 bytes of quad (q, i),
+// at uint2 index q * n + i, so one coalesced 8-byte load per lane fetches four entries (entry k in
+// half-word k & 3, low half first).
 // Neighbour gather p[j] with a 32-bit byte offset (n <= 2^26 and elements of at most 16 bytes keep
 // it below 2^30), so the load takes the array base in scalar registers and one offset register.
 template <class T> BDEV T at(const T* __restrict__ p, uint32_t j) {
   return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + j * (uint32_t)sizeof(T));
+}
+template <class R> BDEV uint2 tquad(const PairView<R>& pv, uint32_t q, uint32_t i) {
+  return at(reinterpret_cast<const uint2*>(pv.nb) + (size_t)q * pv.n, i);
+}
+BDEV int32_t qent(uint2 v, int e) {  // entry e (0..3) of a quad, sign-extended
+  const uint32_t w = (e & 2) ? v.y : v.x;
+  return (e & 1) ? ((int32_t)w >> 16) : (int32_t)(int16_t)(w & 0xFFFFu);
 }
 // Table entry -> particle index. i + d leaves [0, n) only across the y seam, and then in one
 // direction per particle (below 0 for i < n/2, past n - 1 otherwise, as |d| <= n/2), so one
@@ -427,17 +435,12 @@ template <class R, class F>
 BDEV void for_pairs(const BoxGeo<R>& g, const PairView<R>& pv, uint32_t i, F&& f) {
   const uint32_t c = pcount(pv.cnt[i]);
   if (c <= BX_KMAX) {
-    // four table entries at a time: their loads (and the gathers they feed) are issued together
-    uint32_t k = 0;
-    for (; k + 4u <= c; k += 4u) {
-      const uint32_t j0 = nb_dec(at(tcol(pv, k), i), i, g.n), j1 = nb_dec(at(tcol(pv, k + 1u), i), i, g.n);
-      const uint32_t j2 = nb_dec(at(tcol(pv, k + 2u), i), i, g.n), j3 = nb_dec(at(tcol(pv, k + 3u), i), i, g.n);
-      f(j0);
-      f(j1);
-      f(j2);
-      f(j3);
+    for (uint32_t k = 0; k < c; k += 4u) {
+      const uint2 v = tquad(pv, k >> 2, i);
+#pragma unroll
+      for (int e = 0; e < 4; e++)
+        if (k + e < c) f(nb_dec(qent(v, e), i, g.n));
     }
-    for (; k < c; k++) f(nb_dec(at(tcol(pv, k), i), i, g.n));
   } else {
     scan_pairs(g, pv.XP, pv.start, i, pv.XP[i], f);
   }
@@ -456,20 +459,29 @@ BDEV void for_pairs_g(const BoxGeo<R>& g, const PairView<R>& pv, uint32_t i, uin
   constexpr int NG = G ? G : Gather<R>::value;
   const uint32_t c = pcount(cntw);
   if (c <= BX_KMAX) {
+    static_assert(NG % 4 == 0, "batches are whole quads");
     uint32_t k = 0;
     for (; BX_BATCH && k + (uint32_t)NG <= c; k += NG) {
+      uint2 qq[NG / 4];
+#pragma unroll
+      for (int t = 0; t < NG / 4; t++) qq[t] = tquad(pv, (k >> 2) + t, i);
       uint32_t jj[NG];
       decltype(load(0u)) aa[NG];
 #pragma unroll
-      for (int q = 0; q < NG; q++) jj[q] = nb_dec(at(tcol(pv, k + q), i), i, g.n, w);
+      for (int q = 0; q < NG; q++) jj[q] = nb_dec(qent(qq[q >> 2], q & 3), i, g.n, w);
 #pragma unroll
       for (int q = 0; q < NG; q++) aa[q] = load(jj[q]);
 #pragma unroll
       for (int q = 0; q < NG; q++) use(jj[q], aa[q]);
     }
-    for (; k < c; k++) {
-      const uint32_t j = nb_dec(at(tcol(pv, k), i), i, g.n, w);
-      use(j, load(j));
+    for (; k < c; k += 4u) {  // the last whole or partial quads
+      const uint2 v = tquad(pv, k >> 2, i);
+#pragma unroll
+      for (int e = 0; e < 4; e++)
+        if (k + e < c) {
+          const uint32_t j = nb_dec(qent(v, e), i, g.n, w);
+          use(j, load(j));
+        }
     }
   } else {
     scan_pairs(g, pv.XP, pv.start, i, pv.XP[i], [&](uint32_t j) __attribute__((always_inline)) { use(j, load(j)); });
@@ -638,19 +650,22 @@ BXKT bx_gather(BoxGeo<R> g, const uint32_t* __restrict__ slot, const Vec<R>* __r
 }
 
 // Records the pair table of particle i while f(j, d, r2) sees every pair; returns the count word.
-// With lrow the row's entries are also kept in LDS (entry k at lrow[k * BX_BLOCK]).
+// Each entry is stored at its slot of the quad layout as it is found (collecting whole quads in
+// registers first measured slower: the selects cost more in this VALU-bound scan than the narrower
+// stores); with lrow the row is also kept in LDS, in the same quad layout (quad q at lrow[q *
+// BX_BLOCK]).
 template <class R, bool W, class F>
 BDEV uint32_t table_row(const BoxGeo<R>& g, const Vec<R>* __restrict__ XP, const uint32_t* __restrict__ start,
                         int16_t* __restrict__ nb, uint32_t* __restrict__ cnt, uint32_t i, Vec<R> xpi, Wrap<W> w,
-                        int16_t* lrow, F&& f) {
+                        uint2* lrow, F&& f) {
   uint32_t c = 0;
   bool fits = true, wrapped = false;
   scan_pairs_d(g, XP, start, i, xpi, w, [&](uint32_t j, Vec<R> d, R r2) __attribute__((always_inline)) {
     int16_t e = 0;
     fits = nb_enc(j, i, g.n, e, wrapped) && fits;
     if (c < BX_KMAX) {
-      nb[(size_t)c * g.n + i] = e;
-      if (lrow) lrow[c * BX_BLOCK] = e;
+      nb[(((size_t)(c >> 2) * g.n + i) << 2) + (c & 3u)] = e;
+      if (lrow) reinterpret_cast<int16_t*>(lrow + (c >> 2) * BX_BLOCK)[c & 3u] = e;
     }
     c++;
     f(j, d, r2);
@@ -713,11 +728,11 @@ BXKT bx_lambda(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, R* __r
 // member (about three times the member count).
 BXKT bx_pairs_lambda(BoxGeo<R> g, const Vec<R>* __restrict__ XP, const uint32_t* __restrict__ start,
                      int16_t* __restrict__ nb, uint32_t* __restrict__ cnt, R* __restrict__ LAM) {
-  __shared__ int16_t s_row[BX_KMAX * BX_BLOCK];  // each lane's table row (16 KB per workgroup)
+  __shared__ uint2 s_row[(BX_KMAX / 4) * BX_BLOCK];  // each lane's table row (16 KB per workgroup)
   if (*g.halt) return;
   const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
   if (i >= g.n) return;
-  int16_t* lrow = s_row + threadIdx.x;
+  uint2* lrow = s_row + threadIdx.x;
   const Vec<R> xpi = XP[i];
   uint32_t cw = 0;
   by_wrap(wave_inside(xpi, g), [&](auto w) __attribute__((always_inline)) {
@@ -738,13 +753,18 @@ BXKT bx_pairs_lambda(BoxGeo<R> g, const Vec<R>* __restrict__ XP, const uint32_t*
         uint32_t jj[NG];
         Vec<R> xx[NG];
 #pragma unroll
-        for (int q = 0; q < NG; q++) jj[q] = nb_dec(lrow[(k + q) * BX_BLOCK], i, g.n, w);
+        for (int q = 0; q < NG; q++) jj[q] = nb_dec(qent(lrow[((k >> 2) + (q >> 2)) * BX_BLOCK], q & 3), i, g.n, w);
 #pragma unroll
         for (int q = 0; q < NG; q++) xx[q] = at(XP, jj[q]);
 #pragma unroll
         for (int q = 0; q < NG; q++) term(xx[q]);
       }
-      for (; k < c; k++) term(at(XP, nb_dec(lrow[k * BX_BLOCK], i, g.n, w)));
+      for (; k < c; k += 4u) {
+        const uint2 v = lrow[(k >> 2) * BX_BLOCK];
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+          if (k + e < c) term(at(XP, nb_dec(qent(v, e), i, g.n, w)));
+      }
     } else {
       scan_pairs(g, XP, start, i, xpi, [&](uint32_t j) __attribute__((always_inline)) { term(XP[j]); });
     }
@@ -938,22 +958,27 @@ BXKRT bx_cg_dir(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const
           s = add2(s, smul2((R)4 * at(CW + (size_t)kk * g.n, i) * dot2(e, sub2(p, pj)), e));
         };
         uint32_t k = 0;
-        constexpr int NG = 4;
-        for (; k + (uint32_t)NG <= c; k += NG) {
-          uint32_t jj[NG];
-          Vec<R> pp[NG], xx[NG];
+        for (; k + 4u <= c; k += 4u) {  // one quad of the table per batch
+          const uint2 v = tquad(pv, k >> 2, i);
+          uint32_t jj[4];
+          Vec<R> pp[4], xx[4];
 #pragma unroll
-          for (int q = 0; q < NG; q++) jj[q] = nb_dec(at(tcol(pv, k + q), i), i, g.n, wr);
+          for (int q = 0; q < 4; q++) jj[q] = nb_dec(qent(v, q), i, g.n, wr);
 #pragma unroll
-          for (int q = 0; q < NG; q++) pp[q] = pdir(jj[q]);
+          for (int q = 0; q < 4; q++) pp[q] = pdir(jj[q]);
 #pragma unroll
-          for (int q = 0; q < NG; q++) xx[q] = at(X, jj[q]);
+          for (int q = 0; q < 4; q++) xx[q] = at(X, jj[q]);
 #pragma unroll
-          for (int q = 0; q < NG; q++) term(k + q, pp[q], xx[q]);
+          for (int q = 0; q < 4; q++) term(k + q, pp[q], xx[q]);
         }
-        for (; k < c; k++) {
-          const uint32_t j = nb_dec(at(tcol(pv, k), i), i, g.n, wr);
-          term(k, pdir(j), at(X, j));
+        if (k < c) {  // the partial last quad
+          const uint2 v = tquad(pv, k >> 2, i);
+#pragma unroll
+          for (int e = 0; e < 3; e++)
+            if (k + e < c) {
+              const uint32_t j = nb_dec(qent(v, e), i, g.n, wr);
+              term(k + e, pdir(j), at(X, j));
+            }
         }
       } else {
         const R rho_i = mfmax(DENS[i], (R)1e-6f);
