@@ -84,6 +84,17 @@ template <class R> struct VecT;
 template <> struct VecT<float> { using type = float2; };
 template <> struct VecT<double> { using type = double2; };
 template <class R> using Vec = typename VecT<R>::type;
+template <class R> struct Vec4T;
+template <> struct Vec4T<float> { using type = float4; };
+template <> struct Vec4T<double> { using type = double4; };
+template <class R> using Vec4 = typename Vec4T<R>::type;
+template <class V> BDEV auto v4c(const V& v, int q) { return q == 0 ? v.x : (q == 1 ? v.y : (q == 2 ? v.z : v.w)); }
+template <class V, class T> BDEV void set4(V& v, int q, T x) {
+  if (q == 0) v.x = x;
+  else if (q == 1) v.y = x;
+  else if (q == 2) v.z = x;
+  else v.w = x;
+}
 
 // Geometry and constants of one box, passed by value to every kernel, in the state's precision
 // (the f64 constants are the f32 geometry promoted, then the derived ones formed in f64, as the
@@ -839,30 +850,54 @@ template <class R> BDEV Vec<R> pair_unit(Vec<R> d, bool guard) {
   const R r = pair_len(len2(d));
   return r > (R)1e-9f ? pair_div(d, r, guard) : mk((R)1, (R)0);
 }
-// (L v)_i with the weights computed here; when `store` the weights of a tabled particle are kept
-// (column-major, like the pair table) for the operator applications of this step's CG.
+// (L v)_i with the weights computed here; the weights of a tabled particle are kept for the
+// operator applications of this step's CG, in the table's quad layout: the four weights of one
+// quad of table entries leave as one vector store.
 template <class R>
 BDEV Vec<R> lap_row(const BoxGeo<R>& g, uint32_t i, const PairView<R>& pv, const Vec<R>* __restrict__ X,
-                    const R* __restrict__ DENS, const Vec<R>* __restrict__ in, bool store, R* __restrict__ CW) {
+                    const R* __restrict__ DENS, const Vec<R>* __restrict__ in, R* __restrict__ CW) {
   const Vec<R> xi = X[i], ini = in[i];
   const R rho_i = mfmax(DENS[i], (R)1e-6f);
-  const uint32_t cw = pv.cnt[i];
-  const bool keep = store && pcount(cw) <= BX_KMAX;
+  const uint32_t cw = pv.cnt[i], c = pcount(cw);
   Vec<R> s = mk((R)0, (R)0);
-  uint32_t k = 0;
+  Vec4<R>* const cwq = reinterpret_cast<Vec4<R>*>(CW);
   struct Nb { Vec<R> x; R d; Vec<R> v; };
   by_wrap(wave_fast(xi, cw, g), [&](auto wr) __attribute__((always_inline)) {
-    for_pairs_g<4>(g, pv, i, cw, wr, [&](uint32_t j) __attribute__((always_inline)) {
-      return Nb{at(X, j), at(DENS, j), at(in, j)};
-    },
-                [&](uint32_t, const Nb& a) __attribute__((always_inline)) {
+    auto load = [&](uint32_t j) __attribute__((always_inline)) { return Nb{at(X, j), at(DENS, j), at(in, j)}; };
+    auto term = [&](const Nb& a) __attribute__((always_inline)) {
       R w;
       Vec<R> e;
       morris(g, wr, xi, rho_i, a.x, a.d, w, e);
-      if (keep) CW[(size_t)k * g.n + i] = w;
-      k++;
       s = add2(s, smul2((R)4 * w * dot2(e, sub2(ini, a.v)), e));
-    });
+      return w;
+    };
+    if (c <= BX_KMAX) {
+      uint32_t k = 0;
+      for (; k + 4u <= c; k += 4u) {  // one quad of table entries per batch
+        const uint2 v = tquad(pv, k >> 2, i);
+        Nb aa[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) aa[q] = load(nb_dec(qent(v, q), i, g.n, wr));
+        Vec<R>* const wq = reinterpret_cast<Vec<R>*>(cwq + (size_t)(k >> 2) * g.n + i);
+        const R w0 = term(aa[0]);
+        const R w1 = term(aa[1]);
+        wq[0] = mk(w0, w1);
+        const R w2 = term(aa[2]);
+        const R w3 = term(aa[3]);
+        wq[1] = mk(w2, w3);
+      }
+      if (k < c) {  // the partial last quad (its unused weights are zero)
+        const uint2 v = tquad(pv, k >> 2, i);
+        Vec4<R> w4;
+        w4.x = w4.y = w4.z = w4.w = (R)0;
+#pragma unroll
+        for (int q = 0; q < 3; q++)
+          if (k + q < c) set4(w4, q, term(load(nb_dec(qent(v, q), i, g.n, wr))));
+        cwq[(size_t)(k >> 2) * g.n + i] = w4;
+      }
+    } else {
+      scan_pairs(g, pv.XP, pv.start, i, pv.XP[i], [&](uint32_t j) __attribute__((always_inline)) { term(load(j)); });
+    }
   });
   return s;
 }
@@ -876,7 +911,7 @@ BXKRT bx_cg_init(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, cons
   double rr = 0.0;
   if (i < g.n) {
     const Vec<R> v = V[i];
-    const Vec<R> lv = lap_row(g, i, pv, X, DENS, V, true, CW);
+    const Vec<R> lv = lap_row(g, i, pv, X, DENS, V, CW);
     const Vec<R> r = sub2(v, add2(v, mul2(lv, g.dt)));
     Rv[i] = r;
     P[i] = r;
@@ -953,13 +988,18 @@ BXKRT bx_cg_dir(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const
         auto pdir = [&](uint32_t j) __attribute__((always_inline)) {
           return first ? at(Pold, j) : add2(at(Rv, j), mul2(at(Pold, j), beta));
         };
-        auto term = [&](uint32_t kk, Vec<R> pj, Vec<R> xj) __attribute__((always_inline)) {
+        auto term = [&](R wk, Vec<R> pj, Vec<R> xj) __attribute__((always_inline)) {
           const Vec<R> e = pair_unit<R>(rel(xi, xj, g, wr), guarded(wr));
-          s = add2(s, smul2((R)4 * at(CW + (size_t)kk * g.n, i) * dot2(e, sub2(p, pj)), e));
+          s = add2(s, smul2((R)4 * wk * dot2(e, sub2(p, pj)), e));
+        };
+        // the kept weights are in the table's quad layout: one vector load per four pairs
+        auto wquad = [&](uint32_t q) __attribute__((always_inline)) {
+          return at(reinterpret_cast<const Vec4<R>*>(CW) + (size_t)q * g.n, i);
         };
         uint32_t k = 0;
         for (; k + 4u <= c; k += 4u) {  // one quad of the table per batch
           const uint2 v = tquad(pv, k >> 2, i);
+          const Vec4<R> w4 = wquad(k >> 2);
           uint32_t jj[4];
           Vec<R> pp[4], xx[4];
 #pragma unroll
@@ -969,15 +1009,16 @@ BXKRT bx_cg_dir(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const
 #pragma unroll
           for (int q = 0; q < 4; q++) xx[q] = at(X, jj[q]);
 #pragma unroll
-          for (int q = 0; q < 4; q++) term(k + q, pp[q], xx[q]);
+          for (int q = 0; q < 4; q++) term(v4c(w4, q), pp[q], xx[q]);
         }
         if (k < c) {  // the partial last quad
           const uint2 v = tquad(pv, k >> 2, i);
+          const Vec4<R> w4 = wquad(k >> 2);
 #pragma unroll
           for (int e = 0; e < 3; e++)
             if (k + e < c) {
               const uint32_t j = nb_dec(qent(v, e), i, g.n, wr);
-              term(k + e, pdir(j), at(X, j));
+              term(v4c(w4, e), pdir(j), at(X, j));
             }
         }
       } else {
