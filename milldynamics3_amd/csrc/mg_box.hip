@@ -715,7 +715,22 @@ template <class R> BDEV R lambda_of(const BoxGeo<R>& g, LamAcc<R> a) {
   a.sgs += len2(a.gs);
   return -c / (a.sgs + (R)200);
 }
-BXKT bx_lambda(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, R* __restrict__ LAM) {
+// Pass A's output for pass B. FP32: the particle's position (the one pass A read) and lambda in
+// one 16-byte record (x, y, lambda, 0), so pass B gathers a neighbour's position and lambda in one
+// load. FP64: lambda alone (a 32-byte record measured slower than the 16 + 8 bytes it replaces).
+// Both use the same buffer.
+template <class R> struct PackXL { static constexpr bool value = false; };
+template <> struct PackXL<float> { static constexpr bool value = true; };
+template <class R> BDEV void put_lambda(void* __restrict__ LB, uint32_t i, Vec<R> x, R lam) {
+  if constexpr (PackXL<R>::value) {
+    Vec4<R> o;
+    o.x = x.x; o.y = x.y; o.z = lam; o.w = (R)0;
+    static_cast<Vec4<R>*>(LB)[i] = o;
+  } else {
+    static_cast<R*>(LB)[i] = lam;
+  }
+}
+BXKT bx_lambda(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, void* __restrict__ LB) {
   if (*g.halt) return;
   const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
   if (i >= g.n) return;
@@ -729,7 +744,7 @@ BXKT bx_lambda(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, R* __r
       lambda_term(g, w, d, len2(d), a);
     });
   });
-  LAM[i] = lambda_of(g, a);
+  put_lambda<R>(LB, i, xi, lambda_of(g, a));
 }
 // First density pass A of a step, fused with the pair-table build: right after the reorder the
 // positions X equal the predicted ones XP, so pass A runs on XP (same values, same order as
@@ -738,7 +753,7 @@ BXKT bx_lambda(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, R* __r
 // would issue the lambda arithmetic for every candidate step in which any lane of the wave has a
 // member (about three times the member count).
 BXKT bx_pairs_lambda(BoxGeo<R> g, const Vec<R>* __restrict__ XP, const uint32_t* __restrict__ start,
-                     int16_t* __restrict__ nb, uint32_t* __restrict__ cnt, R* __restrict__ LAM) {
+                     int16_t* __restrict__ nb, uint32_t* __restrict__ cnt, void* __restrict__ LB) {
   __shared__ uint2 s_row[(BX_KMAX / 4) * BX_BLOCK];  // each lane's table row (16 KB per workgroup)
   if (*g.halt) return;
   const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
@@ -780,25 +795,42 @@ BXKT bx_pairs_lambda(BoxGeo<R> g, const Vec<R>* __restrict__ XP, const uint32_t*
       scan_pairs(g, XP, start, i, xpi, [&](uint32_t j) __attribute__((always_inline)) { term(XP[j]); });
     }
   });
-  LAM[i] = lambda_of(g, a);
+  put_lambda<R>(LB, i, xpi, lambda_of(g, a));
 }
-BXKT bx_dp(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const R* __restrict__ LAM,
+// Pass B (FP32: positions and lambdas from pass A's records, which hold the positions X).
+BXKT bx_dp(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const void* __restrict__ LB,
            Vec<R>* __restrict__ Xout) {
   if (*g.halt) return;
   const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
   if (i >= g.n) return;
-  const Vec<R> xi = X[i];
-  const R li = LAM[i];
+  Vec<R> xi;
+  R li;
+  const Vec4<R>* const XL = static_cast<const Vec4<R>*>(LB);
+  const R* const LAM = static_cast<const R*>(LB);
+  if constexpr (PackXL<R>::value) {
+    const Vec4<R> oi = XL[i];
+    xi = mk(oi.x, oi.y);
+    li = oi.z;
+  } else {
+    xi = X[i];
+    li = LAM[i];
+  }
   Vec<R> s = mk((R)0, (R)0);
-  struct Nb { Vec<R> x; R l; };
   const uint32_t cw = pv.cnt[i];
+  auto term = [&](auto w, Vec<R> xj, R lj) __attribute__((always_inline)) {
+    const Vec<R> d = rel(xi, xj, g, w);
+    const R r = pair_len(len2(d));
+    s = add2(s, smul2(li + lj, spiky(d, r, g, guarded(w))));
+  };
   by_wrap(wave_fast(xi, cw, g), [&](auto w) __attribute__((always_inline)) {
-    for_pairs_g(g, pv, i, cw, w, [&](uint32_t j) __attribute__((always_inline)) { return Nb{at(X, j), at(LAM, j)}; },
-                [&](uint32_t, const Nb& a) __attribute__((always_inline)) {
-      const Vec<R> d = rel(xi, a.x, g, w);
-      const R r = pair_len(len2(d));
-      s = add2(s, smul2(li + a.l, spiky(d, r, g, guarded(w))));
-    });
+    if constexpr (PackXL<R>::value) {
+      for_pairs_g(g, pv, i, cw, w, [&](uint32_t j) __attribute__((always_inline)) { return at(XL, j); },
+                  [&](uint32_t, const Vec4<R>& a) __attribute__((always_inline)) { term(w, mk(a.x, a.y), a.z); });
+    } else {
+      struct Nb { Vec<R> x; R l; };
+      for_pairs_g(g, pv, i, cw, w, [&](uint32_t j) __attribute__((always_inline)) { return Nb{at(X, j), at(LAM, j)}; },
+                  [&](uint32_t, const Nb& a) __attribute__((always_inline)) { term(w, a.x, a.l); });
+    }
   });
   const Vec<R> dp = mul2(s, g.m_over_rest);
   Xout[i] = mk(wrapR(xi.x + dp.x, g.L), wrapR(xi.y + dp.y, g.L));
@@ -1222,7 +1254,7 @@ struct mg_box {
   uint32_t *id = nullptr, *idb = nullptr, *key = nullptr, *slot = nullptr, *cursor = nullptr, *bsum = nullptr;
   // cell starts of the current step and the buffer the current step clears for the next one
   uint32_t *start = nullptr, *start_next = nullptr;
-  void *LAM = nullptr, *DENS = nullptr;
+  void *LAM = nullptr, *DENS = nullptr;  // LAM: pass A's output for pass B (see put_lambda)
   void *R = nullptr, *P = nullptr, *AP = nullptr;
   int16_t* pnb = nullptr;    // per-step pair table (16-bit relative offsets)
   uint32_t* pcnt = nullptr;  // and the per-particle pair counts
@@ -1478,14 +1510,15 @@ template <class R> bool enqueue_front(mg_box* B) {
     if (p == 0) {  // the first pass A also builds the pair table
       Tic t(B, BXT_PAIRLAM);
       hipLaunchKernelGGL(bx_pairs_lambda<R>, gp, blk, 0, st, g, vp<R>(B->XP), B->start, B->pnb, B->pcnt,
-                         sp<R>(B->LAM));
+                         B->LAM);
     } else {
       Tic t(B, BXT_LAMBDA);
-      hipLaunchKernelGGL(bx_lambda<R>, gp, blk, 0, st, g, pv, vp<R>(B->X), sp<R>(B->LAM));
+      hipLaunchKernelGGL(bx_lambda<R>, gp, blk, 0, st, g, pv, vp<R>(B->X), B->LAM);
     }
     {
       Tic t(B, BXT_DP);
-      hipLaunchKernelGGL(bx_dp<R>, gp, blk, 0, st, g, pv, vp<R>(B->X), sp<R>(B->LAM), vp<R>(B->X2));
+      hipLaunchKernelGGL(bx_dp<R>, gp, blk, 0, st, g, pv, vp<R>(B->X), static_cast<const void*>(B->LAM),
+                         vp<R>(B->X2));
     }
     std::swap(B->X, B->X2);
   }
@@ -1747,7 +1780,7 @@ int mg_box_create(const mg_box_config* cfg, mg_box** out, char* err, size_t errl
   off = carve(off, B->start, (size_t)B->ncell + 1); off = carve(off, B->start_next, (size_t)B->ncell + 1);
   off = carve(off, B->cursor, (size_t)B->ncell);
   off = carve(off, B->bsum, (size_t)B->nscan);
-  off = carve_bytes(off, B->LAM, rs * n); off = carve_bytes(off, B->DENS, rs * n);
+  off = carve_bytes(off, B->LAM, 4 * rs * n); off = carve_bytes(off, B->DENS, rs * n);
   off = carve_bytes(off, B->R, vb); off = carve_bytes(off, B->P, vb); off = carve_bytes(off, B->AP, vb);
   off = carve(off, B->part_b, (size_t)B->nred); off = carve(off, B->part_rr, 2 * (size_t)B->nred);
   off = carve(off, B->part_pap, (size_t)B->nred); off = carve_bytes(off, B->P2, vb);
