@@ -90,6 +90,18 @@ def test_full_size_million_particles(box_mod, orc):
     assert np.abs(v1.sum(0) - v0.sum(0)).max() < 8 * ulp * np.sqrt(b.n)
 
 
+def test_four_million_particles(box_mod, orc):
+    # the 2048^2 box of the throughput runs: 1,048,577 cell starts (1025 scan chunks) and the
+    # interior fast path on ~96 % of waves; one step bit-exact against the oracle
+    cfg = {"side": 2048}
+    print("box full size 2048^2", flush=True)
+    b = box_mod.Box(cfg)
+    b.step()
+    o = orc.C3Oracle(cfg, "mixed")
+    o.step()
+    _compare(b, o, "side=2048")
+
+
 def test_capped_cg_matches_oracle(box_mod, orc):
     # the CG stops at max_cg without converging; both sides must stop at the same iterate
     cfg = {"side": 48, "mu": 1.0, "max_cg": 3}
