@@ -46,6 +46,7 @@
 #define BX_SCAN_ITEMS 4   // one 16-byte vector per lane (coalesced)
 #define BX_SCAN_CHUNK (BX_BLOCK * BX_SCAN_ITEMS)
 #define BX_CG_CHUNK 4
+#define BX_NRED 512    // most workgroups of the FP64 reduction kernels (two rounds of one per CU)
 #define BX_WINDOW 16  // steps enqueued between host waits in a multi-step call
 #define BX_SPARE_IT 1  // CG iterations launched beyond the predicted count in a window
 // Loop-shape switch (both forms give identical results; the default is the measured faster one).
@@ -103,10 +104,29 @@ template <class V, class T> BDEV void set4(V& v, int q, T x) {
 // once (the host then makes that step's start state live again). [lo, hi] = [2h, L - 2h] is the
 // interior where pair vectors need no minimum image (see wave_inside).
 template <class R> struct BoxGeo {
-  uint32_t n, ncx, nblk, xcd_chunk, nred, xcd_chunk_red;
+  uint32_t n, ncx, nblk, xcd_chunk, nred, xcd_chunk_red, ppt;
   R h, h2, L, halfL, cw, mass, rest, dt, k6, ks, w0, m_over_rest, mu2m, eta2, lo, hi;
   const uint32_t* halt;
 };
+// Particles per lane in the reduction kernels. FP64 (one workgroup per CU) caps the workgroups at
+// BX_NRED and loops over g.ppt particles per lane; FP32 keeps one particle per lane (a compile-time
+// 1: the loop's live values would push its CG kernels over the 64-VGPR budget).
+template <class R> BDEV uint32_t red_ppt(const BoxGeo<R>& g) {
+  if constexpr (BxRWaves<R>::value == 8) return 1u;
+  else return g.ppt;
+}
+// body(i, k) for the lane's k-th particle i (FP32: the one particle, as a plain branch).
+template <class R, class F> BDEV void for_lane_particles(const BoxGeo<R>& g, uint32_t base, F&& body) {
+  if constexpr (BxRWaves<R>::value == 8) {
+    if (base < g.n) body(base, 0u);
+  } else {
+    for (uint32_t k = 0; k < g.ppt; k++) {
+      const uint32_t i = base + k * BX_RBLOCK;
+      if (i >= g.n) break;
+      body(i, k);
+    }
+  }
+}
 
 // Device scalars of the CG loop (pbf.rs:423-480).
 // rr[u & 1] = r(u).r(u) once a CG kernel has summed it (block 0 stores it for later kernels).
@@ -842,13 +862,14 @@ BXKT bx_dp(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const void
 BXKRT bx_vel_density(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const Vec<R>* __restrict__ X0,
                      Vec<R>* __restrict__ V, R* __restrict__ DENS, double* __restrict__ part_b) {
   if (*g.halt) return;
-  const uint32_t i = tile_id_red(g) * BX_RBLOCK + threadIdx.x;
+  const uint32_t ppt = red_ppt(g);
+  const uint32_t base = tile_id_red(g) * BX_RBLOCK * ppt + threadIdx.x;
   double vv = 0.0;
-  if (i < g.n) {
+  for_lane_particles(g, base, [&](uint32_t i, uint32_t k) __attribute__((always_inline)) {
     const Vec<R> xi = X[i];
     const Vec<R> v = div2(rel(xi, X0[i], g), g.dt);
     V[i] = v;
-    vv = (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+    vv += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
     R rho = g.w0;
     const uint32_t cw = pv.cnt[i];
     by_wrap(wave_fast(xi, cw, g), [&](auto w) __attribute__((always_inline)) {
@@ -858,7 +879,7 @@ BXKRT bx_vel_density(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, 
       });
     });
     DENS[i] = rho;
-  }
+  });
   const double t = block_sum<BX_RBLOCK>(vv);
   if (threadIdx.x == 0) part_b[blockIdx.x] = t;
 }
@@ -939,16 +960,17 @@ BXKRT bx_cg_init(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, cons
                  R* __restrict__ CW, double* __restrict__ part_rr0,
                  const BoxScal* __restrict__ sc) {
   if (*g.halt | sc->done) return;  // both words loaded at once
-  const uint32_t i = tile_id_red(g) * BX_RBLOCK + threadIdx.x;
+  const uint32_t ppt = red_ppt(g);
+  const uint32_t base = tile_id_red(g) * BX_RBLOCK * ppt + threadIdx.x;
   double rr = 0.0;
-  if (i < g.n) {
+  for_lane_particles(g, base, [&](uint32_t i, uint32_t k) __attribute__((always_inline)) {
     const Vec<R> v = V[i];
     const Vec<R> lv = lap_row(g, i, pv, X, DENS, V, CW);
     const Vec<R> r = sub2(v, add2(v, mul2(lv, g.dt)));
     Rv[i] = r;
     P[i] = r;
-    rr = (double)r.x * (double)r.x + (double)r.y * (double)r.y;
-  }
+    rr += (double)r.x * (double)r.x + (double)r.y * (double)r.y;
+  });
   const double t = block_sum<BX_RBLOCK>(rr);
   if (threadIdx.x == 0) part_rr0[blockIdx.x] = t;
 }
@@ -965,16 +987,18 @@ BXKRT bx_cg_dir(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const
                 const double* __restrict__ part_b, const double* __restrict__ part_rr_prev,
                 double* __restrict__ part_pap, BoxScal* __restrict__ sc, uint32_t t) {
   if (*g.halt | sc->done) return;  // both words loaded at once
-  const uint32_t i = tile_id_red(g) * BX_RBLOCK + threadIdx.x;
+  const uint32_t ppt = red_ppt(g);
+  const uint32_t base = tile_id_red(g) * BX_RBLOCK * ppt + threadIdx.x;
   const bool first = t == 1u;
-  // the particle's own inputs are requested before the scalar reduction (they do not depend on it)
-  Vec<R> xi = mk((R)0, (R)0), ri = xi, pi = xi;
-  uint32_t cw = 0;
-  if (i < g.n) {
-    xi = X[i];
-    pi = Pold[i];
-    if (!first) ri = Rv[i];
-    cw = pv.cnt[i];
+  // the first particle's own inputs are requested before the scalar reduction (they do not depend
+  // on it)
+  Vec<R> xi0 = mk((R)0, (R)0), ri0 = xi0, pi0 = xi0;
+  uint32_t cw0 = 0;
+  if (base < g.n) {
+    xi0 = X[base];
+    pi0 = Pold[base];
+    if (!first) ri0 = Rv[base];
+    cw0 = pv.cnt[base];
   }
   double target;
   bool stop = false;
@@ -1011,7 +1035,11 @@ BXKRT bx_cg_dir(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const
     return;
   }
   double pap = 0.0;
-  if (i < g.n) {
+  for_lane_particles(g, base, [&](uint32_t i, uint32_t kp) __attribute__((always_inline)) {
+    const Vec<R> xi = kp ? X[i] : xi0;
+    const Vec<R> pi = kp ? Pold[i] : pi0;
+    const Vec<R> ri = (kp && !first) ? Rv[i] : ri0;
+    const uint32_t cw = kp ? pv.cnt[i] : cw0;
     const Vec<R> p = first ? pi : add2(ri, mul2(pi, beta));
     Vec<R> s = mk((R)0, (R)0);
     const uint32_t c = pcount(cw);
@@ -1067,8 +1095,8 @@ BXKRT bx_cg_dir(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const
     const Vec<R> ap = add2(p, mul2(s, g.dt));
     Pnew[i] = p;
     AP[i] = ap;
-    pap = (double)p.x * (double)ap.x + (double)p.y * (double)ap.y;
-  }
+    pap += (double)p.x * (double)ap.x + (double)p.y * (double)ap.y;
+  });
   const double tt = block_sum<BX_RBLOCK>(pap);
   if (threadIdx.x == 0) part_pap[blockIdx.x] = tt;
   if (blockIdx.x == 0 && threadIdx.x == 0) sc->it = t;
@@ -1080,14 +1108,15 @@ BXKRT bx_cg_upd(BoxGeo<R> g, Vec<R>* __restrict__ V, Vec<R>* __restrict__ Rv, co
                 const Vec<R>* __restrict__ AP, const double* __restrict__ part_pap,
                 double* __restrict__ part_rr_out, BoxScal* __restrict__ sc, uint32_t t) {
   if (*g.halt | sc->done) return;  // both words loaded at once
-  const uint32_t i = blockIdx.x * BX_RBLOCK + threadIdx.x;
-  // the particle's vectors are requested before the scalar reduction (they do not depend on it)
-  Vec<R> vi = mk((R)0, (R)0), pi = vi, ri = vi, api = vi;
-  if (i < g.n) {
-    vi = V[i];
-    pi = P[i];
-    ri = Rv[i];
-    api = AP[i];
+  const uint32_t ppt = red_ppt(g);
+  const uint32_t base = blockIdx.x * BX_RBLOCK * ppt + threadIdx.x;
+  // the first particle's vectors are requested before the scalar reduction (they do not depend on it)
+  Vec<R> vi0 = mk((R)0, (R)0), pi0 = vi0, ri0 = vi0, api0 = vi0;
+  if (base < g.n) {
+    vi0 = V[base];
+    pi0 = P[base];
+    ri0 = Rv[base];
+    api0 = AP[base];
   }
   const double rs_old = sc->rr[(t - 1u) & 1u];  // r(t-1).r(t-1), stored by this iteration's first half
   const double pap = sum_parts<BX_RBLOCK>(part_pap, g.nred);
@@ -1100,12 +1129,13 @@ BXKRT bx_cg_upd(BoxGeo<R> g, Vec<R>* __restrict__ V, Vec<R>* __restrict__ Rv, co
   }
   const R a = (R)(rs_old / pap);
   double rr = 0.0;
-  if (i < g.n) {
+  for_lane_particles(g, base, [&](uint32_t i, uint32_t k) __attribute__((always_inline)) {
+    const Vec<R> vi = k ? V[i] : vi0, pi = k ? P[i] : pi0, ri = k ? Rv[i] : ri0, api = k ? AP[i] : api0;
     V[i] = add2(vi, mul2(pi, a));
     const Vec<R> r = sub2(ri, mul2(api, a));
     Rv[i] = r;
-    rr = (double)r.x * (double)r.x + (double)r.y * (double)r.y;
-  }
+    rr += (double)r.x * (double)r.x + (double)r.y * (double)r.y;
+  });
   const double tt = block_sum<BX_RBLOCK>(rr);
   if (threadIdx.x == 0) part_rr_out[blockIdx.x] = tt;
 }
@@ -1354,7 +1384,8 @@ bool box_geometry(const mg_box_config& c, BoxGeo<float>& g, std::string& err) {
   g.halt = nullptr;
   g.nblk = (g.n + BX_BLOCK - 1) / BX_BLOCK;
   g.xcd_chunk = (g.nblk % 8u == 0u) ? g.nblk / 8u : 0u;
-  g.nred = (g.n + BX_RBLOCK - 1) / BX_RBLOCK;
+  g.nred = (g.n + BX_RBLOCK - 1) / BX_RBLOCK;  // FP32 reduction kernels: one particle per lane
+  g.ppt = 1;
   g.xcd_chunk_red = (g.nred % 8u == 0u) ? g.nred / 8u : 0u;
   return true;
 }
@@ -1364,7 +1395,11 @@ bool box_geometry(const mg_box_config& c, BoxGeo<float>& g, std::string& err) {
 BoxGeo<double> promote(const BoxGeo<float>& f, const mg_box_config& c) {
   BoxGeo<double> g;
   g.n = f.n; g.ncx = f.ncx; g.nblk = f.nblk; g.xcd_chunk = f.xcd_chunk; g.nred = f.nred;
-  g.xcd_chunk_red = f.xcd_chunk_red;
+  // FP64 reduction kernels: at most BX_NRED workgroups, each lane taking ppt particles, so the
+  // partials every CG consumer block sums stay at most BX_NRED and the workgroups fit one round
+  g.nred = std::min<uint32_t>(f.nred, BX_NRED);
+  g.ppt = (g.n + g.nred * BX_RBLOCK - 1) / (g.nred * BX_RBLOCK);
+  g.xcd_chunk_red = (g.nred % 8u == 0u) ? g.nred / 8u : 0u;
   const double pi = 3.14159265358979323846;
   g.h = (double)f.h;
   g.h2 = g.h * g.h;
