@@ -102,6 +102,22 @@ def test_four_million_particles(box_mod, orc):
     _compare(b, o, "side=2048")
 
 
+def test_full_size_windows_match_oracle(box_mod, orc):
+    # five steps in one call at 1M particles (one window, as the throughput runs time them)
+    cfg = {"side": 1024}
+    print("box full size 1024^2, 5 steps in one call", flush=True)
+    b = box_mod.Box(cfg)
+    b.step(5)
+    o = orc.C3Oracle(cfg, "mixed")
+    o.step(5)
+    x, v = b.state()
+    xo, vo = o.state()
+    assert np.array_equal(x.astype(np.float64), xo)
+    assert np.array_equal(b.density().astype(np.float64), o.density())
+    assert np.abs(v - vo).max() / max(np.abs(vo).max(), 1e-30) <= V_TOL
+    assert b.stats()["cg_iterations"] == o.stats()["cg_iterations"]
+
+
 def test_capped_cg_matches_oracle(box_mod, orc):
     # the CG stops at max_cg without converging; both sides must stop at the same iterate
     cfg = {"side": 48, "mu": 1.0, "max_cg": 3}
@@ -203,6 +219,17 @@ def test_fp64_steps_match_oracle(box_mod, orc, side, mu):
         o2.step()
         assert np.array_equal(b2.state()[0], o2.state()[0])
         assert np.array_equal(b2.density(), o2.density())
+
+
+def test_fp64_interior_path_matches_oracle(box_mod, orc):
+    # side 128: most waves take the plain-difference path; mu = 1 runs ~20 CG iterations per step
+    cfg = {"side": 128, "mu": 1.0, "fp64": 1}
+    b = box_mod.Box(cfg)
+    o = orc.C3Oracle(cfg, "fp64")
+    for k in range(2):
+        b.step()
+        o.step()
+        _compare64(b, o, f"fp64 side=128 mu=1 step={k}")
 
 
 def test_fp64_multi_step_windows_match_oracle(box_mod, orc):
