@@ -239,13 +239,16 @@ def main() -> int:
     if k in CG_CLASSES and int(launches_k[k]) > 0:
         work_frac = min(1.0, working_k / int(launches_k[k]))
     bytes_per_launch = CLASS_BYTES[k] * n * work_frac
+
+    def scaled(v, f):  # measured traffic is a working launch's: weighted like the bytes
+        return None if v is None else round(v * f, 1)
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
     kernels = {CLASS_NAMES[i]: {"total_ms": round(float(ms_all[i]), 4), "launches": int(launches_all[i]),
                                 "avg_us": round(1000.0 * float(ms_all[i]) / max(int(launches_all[i]), 1), 2)}
                for i in range(8)}
     roofline = {"bound": "hbm", "kernel": CLASS_KERNELS[k], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic(CLASS_KERNELS[k], args.side),
+                "traffic": scaled(measured_traffic(CLASS_KERNELS[k], args.side), work_frac),
                 "valu_busy": measured_valu_busy(CLASS_KERNELS[k], args.side),
                 "bytes_per_launch": round(bytes_per_launch, 1), "avg_launch_us": round(1000.0 * avg_launch_ms, 2),
                 "launches": int(launches_k[k])}
