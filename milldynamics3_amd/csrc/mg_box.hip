@@ -317,12 +317,17 @@ template <class R> BDEV uint32_t tile_id_red(const BoxGeo<R>& g) {
   return (b & 7u) * g.xcd_chunk_red + (b >> 3);
 }
 // Candidate cell o (0..8) of cell (cx, cy) in the reference's per-particle order, wrapped.
+// A cell coordinate c + o (c in [0, ncx), o in {-1, 0, 1}) wrapped into [0, ncx): (c + o) mod ncx
+// by two selects (a modulus by the run-time ncx costs a multiply-high sequence per use).
+BDEV uint32_t wrap_cell(uint32_t c, int o, uint32_t ncx) {
+  if (o < 0) return c == 0u ? ncx - 1u : c - 1u;
+  if (o > 0) return c + 1u >= ncx ? 0u : c + 1u;
+  return c;
+}
 BDEV uint32_t nbr_cell(uint32_t cx, uint32_t cy, int o, uint32_t ncx) {
   const int ox = o == 0 ? -1 : (o == 1 ? -1 : (o == 2 ? 0 : (o == 3 ? 0 : (o == 4 ? 1 : (o == 5 ? 1 : (o == 6 ? 0 : (o == 7 ? -1 : 1)))))));
   const int oy = o == 0 ? -1 : (o == 1 ? 0 : (o == 2 ? -1 : (o == 3 ? 0 : (o == 4 ? 0 : (o == 5 ? 1 : (o == 6 ? 1 : (o == 7 ? 1 : -1)))))));
-  uint32_t ax = (uint32_t)((int)cx + ox + (int)ncx) % ncx;
-  uint32_t ay = (uint32_t)((int)cy + oy + (int)ncx) % ncx;
-  return ay * ncx + ax;
+  return wrap_cell(cy, oy, ncx) * ncx + wrap_cell(cx, ox, ncx);
 }
 // Fixed-order workgroup sum of one double per lane (wave shuffles, then the waves in order);
 // every lane receives the total.
@@ -422,16 +427,20 @@ BDEV uint32_t nb_dec(int32_t d, uint32_t i, uint32_t n) {
 }
 BDEV uint32_t nb_dec(int32_t d, uint32_t i, uint32_t n, Wrap<true>) { return nb_dec(d, i, n); }
 BDEV uint32_t nb_dec(int32_t d, uint32_t i, uint32_t, Wrap<false>) { return i + (uint32_t)d; }
-// false: does not fit; `wrapped` is set when the offset was taken modulo n
-BDEV bool nb_enc(uint32_t j, uint32_t i, uint32_t n, int16_t& e, bool& wrapped) {
-  int32_t d = (int32_t)j - (int32_t)i;
-  const int32_t half = (int32_t)(n >> 1);
-  if (d > half) d -= (int32_t)n;
-  else if (d < -half) d += (int32_t)n;
-  wrapped = wrapped || d != (int32_t)j - (int32_t)i;
-  if (d < -BX_NB_RANGE || d > BX_NB_RANGE) return false;
-  e = (int16_t)d;
-  return true;
+// A copy of a wave-uniform value in a vector register, for the selects of the table build (a select
+// reads at most one scalar register, so a scalar operand is otherwise moved to a vector register on
+// every use inside the loop).
+BDEV int32_t lane_copy(int32_t s) {
+  int32_t v;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(v) : "s"(s));
+  return v;
+}
+// The table entry of neighbour j of particle i: j - i taken modulo n into [-n/2, n/2]. An entry
+// that does not fit 16 bits (|entry| > BX_NB_RANGE) marks its row for the rescan, and an entry
+// other than j - i marks the row as wrapped (cnt bit 31).
+// n and -n are passed in vector registers (lane_copy).
+BDEV int32_t nb_off(int32_t raw, int32_t half, int32_t vn, int32_t vneg_n) {
+  return raw + (raw > half ? vneg_n : (raw < -half ? vn : 0));
 }
 // f(j, d, r2) for every pair, with d = xpi - XP[j] (minimum image) and r2 = |d|^2 as the test used.
 template <class R, bool W, class F>
@@ -449,11 +458,10 @@ BDEV void scan_pairs_d(const BoxGeo<R>& g, const Vec<R>* __restrict__ XP, const 
 #pragma unroll
   for (int o = 0; o < 9; o++) {
     for (uint32_t j = cs[o]; j < ce[o]; j++) {
-      if (j == i) continue;
-      const Vec<R> d = rel(xpi, XP[j], g, w);
+      // particle i itself is loaded like any candidate and left out by the one membership test
+      const Vec<R> d = rel(xpi, at(XP, j), g, w);
       const R r2 = len2(d);
-      if (r2 > g.h2) continue;
-      f(j, d, r2);
+      if (j != i && !(r2 > g.h2)) f(j, d, r2);
     }
   }
 }
@@ -681,36 +689,51 @@ BXKT bx_gather(BoxGeo<R> g, const uint32_t* __restrict__ slot, const Vec<R>* __r
 }
 
 // Records the pair table of particle i while f(j, d, r2) sees every pair; returns the count word.
-// Each entry is stored at its slot of the quad layout as it is found (collecting whole quads in
-// registers first measured slower: the selects cost more in this VALU-bound scan than the narrower
-// stores); with lrow the row is also kept in LDS, in the same quad layout (quad q at lrow[q *
-// BX_BLOCK]).
+// The scan writes each entry to the lane's LDS row (quad q of the row at lrow[q * BX_BLOCK], entry k
+// in half-word k & 3) as it is found, then the row's quads go to the table with one 8-byte store
+// each. The per-member work runs whenever any lane of the wave has a member (about 50 times per
+// wave at 10 members per particle), so it is kept to the encoding, one LDS half-word store and a
+// running byte offset; global half-word stores in the scan measured slower. Entries past the count
+// in the last quad are left as they are (no reader goes past the count).
 template <class R, bool W, class F>
 BDEV uint32_t table_row(const BoxGeo<R>& g, const Vec<R>* __restrict__ XP, const uint32_t* __restrict__ start,
                         int16_t* __restrict__ nb, uint32_t* __restrict__ cnt, uint32_t i, Vec<R> xpi, Wrap<W> w,
                         uint2* lrow, F&& f) {
   uint32_t c = 0;
-  bool fits = true, wrapped = false;
+  // extremes of j - i and of the entries over the row: wrapped and fits follow from them after the
+  // scan (an entry that does not fit is stored truncated; its row is rescanned, never read)
+  int32_t rmin = 0, rmax = 0, emin = 0, emax = 0;
+  char* lp = reinterpret_cast<char*>(lrow);  // entry c at byte (c / 4) * 8 * BX_BLOCK + (c % 4) * 2 of the row
+  const int32_t n = (int32_t)g.n, vn = lane_copy(n), vneg_n = lane_copy(-n);
   scan_pairs_d(g, XP, start, i, xpi, w, [&](uint32_t j, Vec<R> d, R r2) __attribute__((always_inline)) {
-    int16_t e = 0;
-    fits = nb_enc(j, i, g.n, e, wrapped) && fits;
+    const int32_t raw = (int32_t)j - (int32_t)i, e = nb_off(raw, n >> 1, vn, vneg_n);
+    rmin = raw < rmin ? raw : rmin;
+    rmax = raw > rmax ? raw : rmax;
+    emin = e < emin ? e : emin;
+    emax = e > emax ? e : emax;
     if (c < BX_KMAX) {
-      nb[(((size_t)(c >> 2) * g.n + i) << 2) + (c & 3u)] = e;
-      if (lrow) reinterpret_cast<int16_t*>(lrow + (c >> 2) * BX_BLOCK)[c & 3u] = e;
+      *reinterpret_cast<int16_t*>(lp) = (int16_t)e;
+      lp += (c & 3u) == 3u ? BX_BLOCK * 8u - 6u : 2u;
     }
     c++;
     f(j, d, r2);
   });
+  const uint32_t nq = ((c < BX_KMAX ? c : BX_KMAX) + 3u) >> 2;
+  for (uint32_t q = 0; q < nq; q++)  // quad q of particle i: uint2 index q * n + i (byte offset < 64 n <= 2^32)
+    *reinterpret_cast<uint2*>(reinterpret_cast<char*>(nb) + (q * g.n + i) * 8u) = lrow[q * BX_BLOCK];
+  const bool wrapped = rmax > (n >> 1) || rmin < -(n >> 1);
+  const bool fits = emin >= -BX_NB_RANGE && emax <= BX_NB_RANGE;
   const uint32_t cw = (fits ? c : (c > BX_KMAX ? c : BX_KMAX + 1u)) | (wrapped ? BX_CNT_WRAPPED : 0u);
   cnt[i] = cw;
   return cw;
 }
 BXKT bx_pairs(BoxGeo<R> g, const Vec<R>* __restrict__ XP, const uint32_t* __restrict__ start,
               int16_t* __restrict__ nb, uint32_t* __restrict__ cnt) {
+  __shared__ uint2 s_row[(BX_KMAX / 4) * BX_BLOCK];  // each lane's table row (16 KB per workgroup)
   if (*g.halt) return;
   const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
   if (i >= g.n) return;
-  table_row(g, XP, start, nb, cnt, i, XP[i], Wrap<true>{}, nullptr,
+  table_row(g, XP, start, nb, cnt, i, XP[i], Wrap<true>{}, s_row + threadIdx.x,
             [](uint32_t, Vec<R>, R) __attribute__((always_inline)) {});
 }
 

@@ -62,6 +62,20 @@ def test_steps_match_oracle(box_mod, orc, side, mu, steps):
         assert b.stats()["cg_iterations"] == 0
 
 
+# Density pass counts other than 4: with no pass the pair table is built by its own kernel (no pass A
+# to fuse with), with one pass the table-building pass A is the only one.
+@pytest.mark.parametrize("passes,side", [(0, 64), (1, 64), (0, 136), (2, 136)])
+def test_density_pass_counts_match_oracle(box_mod, orc, passes, side):
+    cfg = {"side": side, "mu": 1.0e-3, "passes": passes}
+    print(f"box parity passes={passes} side={side}", flush=True)
+    b = box_mod.Box(cfg)
+    o = orc.C3Oracle(cfg, "mixed")
+    for k in range(2):
+        b.step()
+        o.step()
+        _compare(b, o, f"passes={passes} side={side} step={k}")
+
+
 def test_deterministic_across_runs(box_mod):
     cfg = {"side": 96, "mu": 1.0}
     a = box_mod.Box(cfg)
