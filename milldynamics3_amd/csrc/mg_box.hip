@@ -513,14 +513,17 @@ BDEV void for_pairs_g(const BoxGeo<R>& g, const PairView<R>& pv, uint32_t i, uin
 #pragma unroll
       for (int q = 0; q < NG; q++) use(jj[q], aa[q]);
     }
-    for (; k < c; k += 4u) {  // the last whole or partial quads
+    for (; k < c; k += 4u) {  // the last whole or partial quads: all four loads before the first use
       const uint2 v = tquad(pv, k >> 2, i);
+      uint32_t jj[4];
+      decltype(load(0u)) aa[4];
+#pragma unroll
+      for (int e = 0; e < 4; e++) jj[e] = k + e < c ? nb_dec(qent(v, e), i, g.n, w) : i;  // past the count: i's own
+#pragma unroll
+      for (int e = 0; e < 4; e++) aa[e] = load(jj[e]);
 #pragma unroll
       for (int e = 0; e < 4; e++)
-        if (k + e < c) {
-          const uint32_t j = nb_dec(qent(v, e), i, g.n, w);
-          use(j, load(j));
-        }
+        if (k + e < c) use(jj[e], aa[e]);
     }
   } else {
     scan_pairs(g, pv.XP, pv.start, i, pv.XP[i], [&](uint32_t j) __attribute__((always_inline)) { use(j, load(j)); });
@@ -828,11 +831,14 @@ BXKT bx_pairs_lambda(BoxGeo<R> g, const Vec<R>* __restrict__ XP, const uint32_t*
 #pragma unroll
         for (int q = 0; q < NG; q++) term(xx[q]);
       }
-      for (; k < c; k += 4u) {
+      for (; k < c; k += 4u) {  // as in for_pairs_g: the quad's loads first, past the count i's own
         const uint2 v = lrow[(k >> 2) * BX_BLOCK];
+        Vec<R> xx[4];
+#pragma unroll
+        for (int e = 0; e < 4; e++) xx[e] = at(XP, k + e < c ? nb_dec(qent(v, e), i, g.n, w) : i);
 #pragma unroll
         for (int e = 0; e < 4; e++)
-          if (k + e < c) term(at(XP, nb_dec(qent(v, e), i, g.n, w)));
+          if (k + e < c) term(xx[e]);
       }
     } else {
       scan_pairs(g, XP, start, i, xpi, [&](uint32_t j) __attribute__((always_inline)) { term(XP[j]); });
@@ -962,13 +968,16 @@ BDEV Vec<R> lap_row(const BoxGeo<R>& g, uint32_t i, const PairView<R>& pv, const
         const R w3 = term(aa[3]);
         wq[1] = mk(w2, w3);
       }
-      if (k < c) {  // the partial last quad (its unused weights are zero)
+      if (k < c) {  // the partial last quad (its unused weights are zero); loads first, past the count i's own
         const uint2 v = tquad(pv, k >> 2, i);
+        Nb aa[3];
+#pragma unroll
+        for (int q = 0; q < 3; q++) aa[q] = load(k + q < c ? nb_dec(qent(v, q), i, g.n, wr) : i);
         Vec4<R> w4;
         w4.x = w4.y = w4.z = w4.w = (R)0;
 #pragma unroll
         for (int q = 0; q < 3; q++)
-          if (k + q < c) set4(w4, q, term(load(nb_dec(qent(v, q), i, g.n, wr))));
+          if (k + q < c) set4(w4, q, term(aa[q]));
         cwq[(size_t)(k >> 2) * g.n + i] = w4;
       }
     } else {
@@ -1094,15 +1103,20 @@ BXKRT bx_cg_dir(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const
 #pragma unroll
           for (int q = 0; q < 4; q++) term(v4c(w4, q), pp[q], xx[q]);
         }
-        if (k < c) {  // the partial last quad
+        if (k < c) {  // the partial last quad: loads first, past the count i's own
           const uint2 v = tquad(pv, k >> 2, i);
           const Vec4<R> w4 = wquad(k >> 2);
+          uint32_t jj[3];
+          Vec<R> pp[3], xx[3];
+#pragma unroll
+          for (int e = 0; e < 3; e++) jj[e] = k + e < c ? nb_dec(qent(v, e), i, g.n, wr) : i;
+#pragma unroll
+          for (int e = 0; e < 3; e++) pp[e] = pdir(jj[e]);
+#pragma unroll
+          for (int e = 0; e < 3; e++) xx[e] = at(X, jj[e]);
 #pragma unroll
           for (int e = 0; e < 3; e++)
-            if (k + e < c) {
-              const uint32_t j = nb_dec(qent(v, e), i, g.n, wr);
-              term(v4c(w4, e), pdir(j), at(X, j));
-            }
+            if (k + e < c) term(v4c(w4, e), pp[e], xx[e]);
         }
       } else {
         const R rho_i = mfmax(DENS[i], (R)1e-6f);
