@@ -500,7 +500,11 @@ BDEV void for_pairs_g(const BoxGeo<R>& g, const PairView<R>& pv, uint32_t i, uin
   if (c <= BX_KMAX) {
     static_assert(NG % 4 == 0, "batches are whole quads");
     uint32_t k = 0;
-    for (; BX_BATCH && k + (uint32_t)NG <= c; k += NG) {
+    // whole batches of NG entries, then quads with the entries past the count switched off. With
+    // NG = 4 the quad loop takes every row whole: a wave then runs max(ceil(c / 4)) quads, where
+    // whole quads first and a partial one after would run max(floor(c / 4)) + 1 (rows of 9 and 12
+    // pairs in one wave: 4 instead of 3).
+    for (; BX_BATCH && NG > 4 && k + (uint32_t)NG <= c; k += NG) {
       uint2 qq[NG / 4];
 #pragma unroll
       for (int t = 0; t < NG / 4; t++) qq[t] = tquad(pv, (k >> 2) + t, i);
@@ -954,31 +958,24 @@ BDEV Vec<R> lap_row(const BoxGeo<R>& g, uint32_t i, const PairView<R>& pv, const
       return w;
     };
     if (c <= BX_KMAX) {
-      uint32_t k = 0;
-      for (; k + 4u <= c; k += 4u) {  // one quad of table entries per batch
-        const uint2 v = tquad(pv, k >> 2, i);
+      // one quad of table entries per batch, entries past the count switched off (loads of
+      // particle i's own data, weight 0), so a wave runs max(ceil(c / 4)) batches; the next batch's
+      // quad is requested one batch ahead
+      uint2 vnext = tquad(pv, 0u, i);
+      for (uint32_t k = 0; k < c; k += 4u) {
+        const uint2 v = vnext;
+        if (k + 4u < c) vnext = tquad(pv, (k >> 2) + 1u, i);
         Nb aa[4];
 #pragma unroll
-        for (int q = 0; q < 4; q++) aa[q] = load(nb_dec(qent(v, q), i, g.n, wr));
+        for (int q = 0; q < 4; q++) aa[q] = load(k + q < c ? nb_dec(qent(v, q), i, g.n, wr) : i);
         Vec<R>* const wq = reinterpret_cast<Vec<R>*>(cwq + (size_t)(k >> 2) * g.n + i);
-        const R w0 = term(aa[0]);
-        const R w1 = term(aa[1]);
+        R w0 = (R)0, w1 = (R)0, w2 = (R)0, w3 = (R)0;
+        if (k < c) w0 = term(aa[0]);
+        if (k + 1u < c) w1 = term(aa[1]);
         wq[0] = mk(w0, w1);
-        const R w2 = term(aa[2]);
-        const R w3 = term(aa[3]);
+        if (k + 2u < c) w2 = term(aa[2]);
+        if (k + 3u < c) w3 = term(aa[3]);
         wq[1] = mk(w2, w3);
-      }
-      if (k < c) {  // the partial last quad (its unused weights are zero); loads first, past the count i's own
-        const uint2 v = tquad(pv, k >> 2, i);
-        Nb aa[3];
-#pragma unroll
-        for (int q = 0; q < 3; q++) aa[q] = load(k + q < c ? nb_dec(qent(v, q), i, g.n, wr) : i);
-        Vec4<R> w4;
-        w4.x = w4.y = w4.z = w4.w = (R)0;
-#pragma unroll
-        for (int q = 0; q < 3; q++)
-          if (k + q < c) set4(w4, q, term(aa[q]));
-        cwq[(size_t)(k >> 2) * g.n + i] = w4;
       }
     } else {
       scan_pairs(g, pv.XP, pv.start, i, pv.XP[i], [&](uint32_t j) __attribute__((always_inline)) { term(load(j)); });
@@ -1088,34 +1085,21 @@ BXKRT bx_cg_dir(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const
         auto wquad = [&](uint32_t q) __attribute__((always_inline)) {
           return at(reinterpret_cast<const Vec4<R>*>(CW) + (size_t)q * g.n, i);
         };
-        uint32_t k = 0;
-        for (; k + 4u <= c; k += 4u) {  // one quad of the table per batch
+        // one quad of the table per batch, entries past the count switched off (loads of particle
+        // i's own data), so a wave runs max(ceil(c / 4)) batches
+        for (uint32_t k = 0; k < c; k += 4u) {
           const uint2 v = tquad(pv, k >> 2, i);
           const Vec4<R> w4 = wquad(k >> 2);
           uint32_t jj[4];
           Vec<R> pp[4], xx[4];
 #pragma unroll
-          for (int q = 0; q < 4; q++) jj[q] = nb_dec(qent(v, q), i, g.n, wr);
+          for (int e = 0; e < 4; e++) jj[e] = k + e < c ? nb_dec(qent(v, e), i, g.n, wr) : i;
 #pragma unroll
-          for (int q = 0; q < 4; q++) pp[q] = pdir(jj[q]);
+          for (int e = 0; e < 4; e++) pp[e] = pdir(jj[e]);
 #pragma unroll
-          for (int q = 0; q < 4; q++) xx[q] = at(X, jj[q]);
+          for (int e = 0; e < 4; e++) xx[e] = at(X, jj[e]);
 #pragma unroll
-          for (int q = 0; q < 4; q++) term(v4c(w4, q), pp[q], xx[q]);
-        }
-        if (k < c) {  // the partial last quad: loads first, past the count i's own
-          const uint2 v = tquad(pv, k >> 2, i);
-          const Vec4<R> w4 = wquad(k >> 2);
-          uint32_t jj[3];
-          Vec<R> pp[3], xx[3];
-#pragma unroll
-          for (int e = 0; e < 3; e++) jj[e] = k + e < c ? nb_dec(qent(v, e), i, g.n, wr) : i;
-#pragma unroll
-          for (int e = 0; e < 3; e++) pp[e] = pdir(jj[e]);
-#pragma unroll
-          for (int e = 0; e < 3; e++) xx[e] = at(X, jj[e]);
-#pragma unroll
-          for (int e = 0; e < 3; e++)
+          for (int e = 0; e < 4; e++)
             if (k + e < c) term(v4c(w4, e), pp[e], xx[e]);
         }
       } else {
