@@ -193,6 +193,12 @@ int64_t mg_box_neighbors(mg_box* box, uint32_t* off_n1, uint32_t* nb, int64_t ca
  * apply, 6 CG vector update, 7 pair table + first density pass A. */
 int mg_box_set_timing(mg_box* box, uint32_t class_mask);
 int mg_box_kernel_times(mg_box* box, double* ms, uint64_t* launches, int nclass);
+/* Average duration of one launch of kernel class cls (1: density pass A, 2: density pass B; other
+ * classes return MG_E_ARG): the last step's last launch of that class is launched again `reps`
+ * times back to back on the box's stream between one HIP event pair (the launches rewrite the
+ * values they wrote, so the box's state is unchanged). No counterpart in the reference: a
+ * measurement hook. */
+int mg_box_replay_ms(mg_box* box, uint32_t cls, uint32_t reps, double* ms_per_launch);
 
 #ifdef __cplusplus
 }

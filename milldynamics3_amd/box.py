@@ -74,6 +74,7 @@ def _lib():
     L.mg_box_total_cg_iterations.restype = C.c_uint64
     L.mg_box_set_cg_prediction.argtypes = [P, C.c_uint32]
     L.mg_box_kernel_times.argtypes = [P, dp, C.POINTER(C.c_uint64), C.c_int]
+    L.mg_box_replay_ms.argtypes = [P, C.c_uint32, C.c_uint32, dp]
     _ready.add(id(L))
     return L
 
@@ -133,6 +134,13 @@ class Box:
         _sim._check(self._L.mg_box_kernel_times(self._h, ms.ctypes.data_as(C.POINTER(C.c_double)),
                                                 n.ctypes.data_as(C.POINTER(C.c_uint64)), 8), what="kernel_times")
         return ms, n
+
+    def replay_ms(self, kernel_class: int, reps: int) -> float:
+        """Average ms of one launch of kernel class 1 (density pass A) or 2 (density pass B): the
+        last step's last launch of the class run again `reps` times back to back (state unchanged)."""
+        ms = C.c_double()
+        _sim._check(self._L.mg_box_replay_ms(self._h, kernel_class, reps, C.byref(ms)), what="replay_ms")
+        return ms.value
 
     def state(self):
         if self.fp64:

@@ -34,6 +34,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -1341,6 +1342,10 @@ struct mg_box {
   std::vector<hipEvent_t> pool;
   double k_ms[BX_NCLASS] = {};
   uint64_t k_n[BX_NCLASS] = {};
+  // the last launch of the density passes A (after the first) and B with its arguments, for
+  // mg_box_replay_ms: each writes only outputs that no later kernel of the step overwrites, from
+  // inputs it does not write, so launching it again rewrites the same values
+  std::function<void()> replay[BX_NCLASS];
 };
 
 namespace {
@@ -1569,12 +1574,20 @@ template <class R> bool enqueue_front(mg_box* B) {
                          B->LAM);
     } else {
       Tic t(B, BXT_LAMBDA);
-      hipLaunchKernelGGL(bx_lambda<R>, gp, blk, 0, st, g, pv, vp<R>(B->X), B->LAM);
+      const BoxGeo<R> gc = g;
+      Vec<R>* const x = vp<R>(B->X);
+      void* const lam = B->LAM;
+      B->replay[BXT_LAMBDA] = [=]() { hipLaunchKernelGGL(bx_lambda<R>, gp, blk, 0, st, gc, pv, x, lam); };
+      B->replay[BXT_LAMBDA]();
     }
     {
       Tic t(B, BXT_DP);
-      hipLaunchKernelGGL(bx_dp<R>, gp, blk, 0, st, g, pv, vp<R>(B->X), static_cast<const void*>(B->LAM),
-                         vp<R>(B->X2));
+      const BoxGeo<R> gc = g;
+      const Vec<R>* const x = vp<R>(B->X);
+      const void* const lam = B->LAM;
+      Vec<R>* const xo = vp<R>(B->X2);
+      B->replay[BXT_DP] = [=]() { hipLaunchKernelGGL(bx_dp<R>, gp, blk, 0, st, gc, pv, x, lam, xo); };
+      B->replay[BXT_DP]();
     }
     std::swap(B->X, B->X2);
   }
@@ -1759,6 +1772,7 @@ template <class R, class T> int copy_state_as(mg_box* B, T* x2n, T* v2n) {
 }
 template <class R, class T> int set_state_from(mg_box* B, const T* x2n, const T* v2n) {
   const uint32_t n = B->n;
+  for (auto& f : B->replay) f = nullptr;  // their inputs no longer match the state
   std::vector<Vec<R>> x(n), v(n);
   std::vector<uint32_t> ids(n);
   for (uint32_t i = 0; i < n; i++) {
@@ -1902,6 +1916,24 @@ int mg_box_set_timing(mg_box* B, uint32_t class_mask) {
   if (!B) return MG_E_ARG;
   B->timing = class_mask;
   for (int k = 0; k < BX_NCLASS; k++) { B->k_ms[k] = 0.0; B->k_n[k] = 0; }
+  return MG_OK;
+}
+
+int mg_box_replay_ms(mg_box* B, uint32_t cls, uint32_t reps, double* ms_per_launch) {
+  if (!B || !ms_per_launch || reps == 0 || cls >= BX_NCLASS) return MG_E_ARG;
+  if ((cls != BXT_LAMBDA && cls != BXT_DP) || !B->replay[cls]) return MG_E_ARG;
+  hipEvent_t a = take_event(B), b = take_event(B);
+  if (!a || !b) return MG_E_DEVICE;
+  bool ok = hipEventRecord(a, B->stream) == hipSuccess;
+  for (uint32_t r = 0; ok && r < reps; r++) B->replay[cls]();
+  ok = ok && hipGetLastError() == hipSuccess && hipEventRecord(b, B->stream) == hipSuccess &&
+       hipEventSynchronize(b) == hipSuccess;
+  float ms = 0.0f;
+  ok = ok && hipEventElapsedTime(&ms, a, b) == hipSuccess;
+  B->pool.push_back(a);
+  B->pool.push_back(b);
+  if (!ok) return MG_E_DEVICE;
+  *ms_per_launch = (double)ms / reps;
   return MG_OK;
 }
 

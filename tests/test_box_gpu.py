@@ -306,3 +306,25 @@ def test_dense_cluster_rescans_match_oracle(box_mod, orc, fp64):
         _compare64(b, o, "dense cluster fp64 step 2", xtol=1e-10)
     else:
         _compare(b, o, "dense cluster step 2")
+
+
+def test_replay_timing_leaves_state_unchanged(box_mod, orc):
+    """mg_box_replay_ms relaunches the last density passes A and B: the state, the densities and
+    the next step are those of a box that never replayed (and both equal the oracle's)."""
+    cfg = {"side": 128, "mu": 1.0e-3}
+    print("box replay timing side=128", flush=True)
+    b = box_mod.Box(cfg)
+    o = orc.C3Oracle(cfg, "mixed")
+    b.step(2)
+    o.step()
+    o.step()
+    x0, v0 = b.state()
+    assert b.replay_ms(1, 5) > 0.0
+    assert b.replay_ms(2, 5) > 0.0
+    x1, v1 = b.state()
+    assert np.array_equal(x0, x1) and np.array_equal(v0, v1)
+    with pytest.raises(RuntimeError):
+        b.replay_ms(5, 1)  # CG classes are not relaunchable
+    b.step()
+    o.step()
+    _compare(b, o, "after replay")
