@@ -35,6 +35,7 @@ struct C3Config {
   float tol = 1.0e-6f;
   uint32_t max_cg = 500;
   uint64_t seed = 1;
+  float yield_stress = 0.0f;  // Bingham yield stress (Pa); 0: Newtonian
 };
 
 // Box geometry shared by the seeding and the step (all f32, as the GPU path computes it).
@@ -142,6 +143,41 @@ static Nbrs c3_neighbors(const std::vector<V2<R>>& x, const C3Box& b) {
   return L;
 }
 
+// Per-particle shear rate sqrt(2 D:D) from the SPH velocity gradient over the neighbour lists
+// (pbf.rs:821-849), with the minimum-image pair vector of the periodic box.
+template <class R>
+static std::vector<R> c3_shear_rates(const std::vector<V2<R>>& x, const std::vector<V2<R>>& v, const Nbrs& nbr,
+                                     const std::vector<R>& density, R mass, R h, const C3Box& b) {
+  const size_t n = x.size();
+  std::vector<R> out(n, (R)0);
+  for (size_t i = 0; i < n; i++) {
+    R gxx = 0, gxy = 0, gyx = 0, gyy = 0;
+    for (uint32_t k = nbr.off[i]; k < nbr.off[i + 1]; k++) {
+      const uint32_t j = nbr.nb[k];
+      const V2<R> d(b.mi(x[i].x - x[j].x), b.mi(x[i].y - x[j].y));
+      const R r = d.len();
+      const V2<R> g = spiky_grad(d, r, h);
+      const R coeff = mass / rmax(density[j], F(1e-6));
+      const V2<R> dv = v[j] - v[i];
+      gxx += coeff * dv.x * g.x;
+      gxy += coeff * dv.x * g.y;
+      gyx += coeff * dv.y * g.x;
+      gyy += coeff * dv.y * g.y;
+    }
+    const R dxy = F(0.5) * (gxy + gyx);
+    out[i] = std::sqrt(F(2.0) * (gxx * gxx + gyy * gyy + F(2.0) * dxy * dxy));
+  }
+  return out;
+}
+// Papanastasiou-regularised Bingham viscosity mu + tau_y (1 - exp(-g/g0)) / g, g0 = tau_y / (100 mu),
+// with its g -> 0 limit below 1e-4 g0 and non-finite rates taken as 0 (pbf.rs:854-868, :490).
+template <class R> static R c3_bingham(R g, R mu, R tau_y) {
+  const R g0 = tau_y / (F(100.0) * mu);
+  if (!std::isfinite(g)) g = (R)0;
+  const R extra = g < F(1e-4) * g0 ? tau_y / g0 : tau_y * (F(1.0) - std::exp(-g / g0)) / g;
+  return mu + extra;
+}
+
 template <class R, class A> struct C3Sim {
   C3Config cfg;
   C3Box box;
@@ -224,7 +260,14 @@ template <class R, class A> struct C3Sim {
     dens = density;
     last_cg = 0;
     if (!(mu > (R)0)) return;
-    // Morris weights (pbf.rs:312-344), Newtonian
+    // effective viscosity per particle: Bingham from the shear rates at the recomputed densities
+    // and the updated velocities (pbf.rs:2147-2151), else mu everywhere
+    std::vector<R> mu_eff(n, mu);
+    if (cfg.yield_stress > 0.0f) {
+      const std::vector<R> rates = c3_shear_rates(x, v, nbr, density, mass, h, box);
+      for (size_t i = 0; i < n; i++) mu_eff[i] = c3_bingham(rates[i], mu, (R)cfg.yield_stress);
+    }
+    // Morris weights (pbf.rs:312-344)
     const R eta2 = F(0.01) * h * h;
     std::vector<R> W(nbr.nb.size());
     std::vector<V2<R>> D(nbr.nb.size());
@@ -236,7 +279,7 @@ template <class R, class A> struct C3Sim {
         V2<R> d = mi(x[i] - x[j]);
         R r = d.len();
         V2<R> g = spiky_grad(d, r, h);
-        R num = (mass * (mu + mu) / (rho_i * rho_j)) * std::fabs(d.dot(g));
+        R num = (mass * (mu_eff[i] + mu_eff[j]) / (rho_i * rho_j)) * std::fabs(d.dot(g));
         W[k] = num / (d.len2() + eta2);
         D[k] = r > F(1e-9) ? d / r : V2<R>(1, 0);
       }
@@ -293,6 +336,7 @@ static orc::C3Config c3_cfg_from(const double* c) {
   orc::C3Config k;
   k.side = (uint32_t)c[0]; k.dx = (float)c[1]; k.rest = (float)c[2]; k.mu = (float)c[3]; k.dt = (float)c[4];
   k.passes = (uint32_t)c[5]; k.tol = (float)c[6]; k.max_cg = (uint32_t)c[7]; k.seed = (uint64_t)c[8];
+  k.yield_stress = (float)c[9];
   return k;
 }
 #define C3DISPATCH(h, ...)                                                                   \
@@ -303,7 +347,8 @@ static orc::C3Config c3_cfg_from(const double* c) {
   } while (0)
 
 extern "C" {
-// cfg: side, dx, rest, mu, dt, passes, tol, max_cg, seed. mode: 0 = f32/f32, 1 = f32/f64, 2 = f64.
+// cfg: side, dx, rest, mu, dt, passes, tol, max_cg, seed, yield_stress. mode: 0 = f32/f32,
+// 1 = f32/f64, 2 = f64.
 void* c3_create(const double* cfg, int mode) {
   auto* h = new C3Handle{mode, nullptr};
   orc::C3Config c = c3_cfg_from(cfg);
@@ -381,6 +426,25 @@ int64_t c3_neighbors_of(const double* cfg, const float* pts, size_t n, uint32_t*
   std::memcpy(off, L.off.data(), L.off.size() * 4);
   if ((int64_t)L.nb.size() <= nb_cap) std::memcpy(nb, L.nb.data(), L.nb.size() * 4);
   return (int64_t)L.nb.size();
+}
+// Shear rates (f32, pbf.rs:821-849) of caller points with velocities and densities over the box's
+// neighbour lists of those points (cfg gives side/dx; h and L follow from them).
+void c3_shear_rates_of(const double* cfg, const float* pts, const float* vel, const float* dens, size_t n,
+                       float mass, float* out) {
+  orc::C3Box b = orc::C3Box::make(c3_cfg_from(cfg));
+  std::vector<orc::V2<float>> p(n), v(n);
+  for (size_t i = 0; i < n; i++) {
+    p[i] = orc::V2<float>(pts[2 * i], pts[2 * i + 1]);
+    v[i] = orc::V2<float>(vel[2 * i], vel[2 * i + 1]);
+  }
+  orc::Nbrs L = orc::c3_neighbors(p, b);
+  std::vector<float> d(dens, dens + n);
+  std::vector<float> r = orc::c3_shear_rates(p, v, L, d, mass, b.h, b);
+  std::memcpy(out, r.data(), n * 4);
+}
+// Bingham effective viscosity (f32, pbf.rs:854-868) of each rate.
+void c3_bingham_of(const float* rates, size_t n, float mu, float tau_y, float* out) {
+  for (size_t i = 0; i < n; i++) out[i] = orc::c3_bingham(rates[i], mu, tau_y);
 }
 // Initial state of the workload (f32), for checking the product's seeding.
 void c3_seed_state(const double* cfg, float* x2n, float* v2n) {

@@ -371,8 +371,8 @@ def resolved_ratio(hc):
 
 # ---- kernel-isolation workload (periodic box), c3_oracle.cpp --------------------------------
 C3_DEFAULTS = {"side": 1024, "dx": 1.0e-3, "rest": 1000.0, "mu": 1.0e-3, "dt": 1.0 / 1440.0,
-               "passes": 4, "tol": 1.0e-6, "max_cg": 500, "seed": 1}
-_C3_KEYS = ["side", "dx", "rest", "mu", "dt", "passes", "tol", "max_cg", "seed"]
+               "passes": 4, "tol": 1.0e-6, "max_cg": 500, "seed": 1, "yield_stress": 0.0}
+_C3_KEYS = ["side", "dx", "rest", "mu", "dt", "passes", "tol", "max_cg", "seed", "yield_stress"]
 C3_MODES = {"ref": 0, "mixed": 1, "fp64": 2}
 
 
@@ -411,6 +411,8 @@ def _c3_lib():
     L.c3_neighbors_of.restype = C.c_int64
     L.c3_neighbors_of.argtypes = [dp, fp, C.c_size_t, u32p, u32p, C.c_int64]
     L.c3_seed_state.argtypes = [dp, fp, fp]
+    L.c3_shear_rates_of.argtypes = [dp, fp, fp, fp, C.c_size_t, C.c_float, fp]
+    L.c3_bingham_of.argtypes = [fp, C.c_size_t, C.c_float, C.c_float, fp]
     L._c3_ready = True
     return L
 
@@ -441,6 +443,27 @@ def c3_neighbors_of(cfg, points):
     e = _c3_lib().c3_neighbors_of(_dp(_c3_arr(cfg)), _fp(pts), n, _u32(off), _u32(nb), cap)
     assert e <= cap
     return off, nb[:e].copy()
+
+
+def c3_shear_rates_of(cfg, points, velocities, densities, mass):
+    """Per-particle shear rates (f32, pbf.rs:821-849) of caller points, velocities and densities
+    over their neighbour lists in the workload's periodic box."""
+    pts = np.ascontiguousarray(points, dtype=np.float32).reshape(-1)
+    vel = np.ascontiguousarray(velocities, dtype=np.float32).reshape(-1)
+    n = pts.size // 2
+    dens = np.ascontiguousarray(densities, dtype=np.float32).reshape(-1)
+    assert vel.size == 2 * n and dens.size == n
+    out = np.zeros(n, dtype=np.float32)
+    _c3_lib().c3_shear_rates_of(_dp(_c3_arr(cfg)), _fp(pts), _fp(vel), _fp(dens), n, float(mass), _fp(out))
+    return out
+
+
+def c3_bingham_of(rates, mu, tau_y):
+    """Bingham effective viscosity (f32, pbf.rs:854-868) of each rate."""
+    r = np.ascontiguousarray(rates, dtype=np.float32).reshape(-1)
+    out = np.zeros(r.size, dtype=np.float32)
+    _c3_lib().c3_bingham_of(_fp(r), r.size, float(mu), float(tau_y), _fp(out))
+    return out
 
 
 class C3Oracle:

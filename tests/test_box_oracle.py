@@ -132,3 +132,86 @@ def test_mixed_policy_tracks_fp64(orc):
     L = orc.c3_box(cfg)["L"]
     assert np.abs(xm - xd).max() / L < 1e-6
     assert np.abs(vm - vd).max() / np.abs(vd).max() < 1e-3
+
+
+# ---- Bingham rheology (pbf.rs:821-868, :2147-2151) ----------------------------------------------
+def _lattice(side, dx=1.0e-3):
+    ix, iy = np.meshgrid(np.arange(side), np.arange(side))
+    return np.stack([(ix.ravel() + 0.5) * dx, (iy.ravel() + 0.5) * dx], 1).astype(np.float32)
+
+
+def _interior(pts, c, dx, margin=3):
+    return (np.abs(pts[:, 0] - c) < c - margin * dx) & (np.abs(pts[:, 1] - c) < c - margin * dx)
+
+
+def test_shear_rate_of_linear_fields_on_a_lattice(orc):
+    """Constant velocity: exactly 0. Rigid rotation: 0 up to rounding (D is the symmetric part).
+    Simple shear v = (g y, 0): rate = g S with the lattice sum S = sum_j V (dy_j)^2 c(r_j) / r_j,
+    c = 30 / (pi h^5) (h - r)^2 (the spiky gradient of a linear field), evaluated here by brute
+    force in f64; the same along x; doubling the velocities doubles the rates exactly; doubling the
+    densities halves them."""
+    side, dx = 32, 1.0e-3
+    cfg = {"side": side, "dx": dx}
+    box = orc.c3_box(cfg)
+    h, m = box["h"], box["mass"]
+    pts = _lattice(side, dx)
+    n = len(pts)
+    c = side * dx / 2
+    inner = _interior(pts, c, dx)
+    rho = np.full(n, 1020.0, np.float32)
+    const = np.tile(np.array([[0.3, -0.2]], np.float32), (n, 1))
+    assert not orc.c3_shear_rates_of(cfg, pts, const, rho, m).any()
+    w = 5.0
+    rot = np.stack([-w * (pts[:, 1] - c), w * (pts[:, 0] - c)], 1)
+    assert orc.c3_shear_rates_of(cfg, pts, rot, rho, m)[inner].max() / w < 1e-5
+    # brute-force lattice sum over the offsets inside the kernel radius
+    S = 0.0
+    for a in range(-3, 4):
+        for b in range(-3, 4):
+            r = dx * np.hypot(a, b)
+            if 0 < r < h:
+                S += (m / 1020.0) * (b * dx) ** 2 * 30.0 / (np.pi * h ** 5) * (h - r) ** 2 / r
+    g = 10.0
+    shear = np.stack([g * (pts[:, 1] - c), np.zeros(n)], 1)
+    r1 = orc.c3_shear_rates_of(cfg, pts, shear, rho, m)
+    assert np.allclose(r1[inner] / g, S, rtol=1e-5, atol=0)
+    shear_x = np.stack([np.zeros(n), g * (pts[:, 0] - c)], 1)
+    assert np.allclose(orc.c3_shear_rates_of(cfg, pts, shear_x, rho, m)[inner] / g, S, rtol=1e-5, atol=0)
+    r2 = orc.c3_shear_rates_of(cfg, pts, 2.0 * shear.astype(np.float32), rho, m)
+    assert np.array_equal(r2, 2.0 * r1)
+    r3 = orc.c3_shear_rates_of(cfg, pts, shear, 2.0 * rho, m)
+    assert np.allclose(r3, 0.5 * r1, rtol=1e-6, atol=0)
+
+
+def test_bingham_viscosity_limits(orc):
+    """g -> 0 (and non-finite g): the cap (1 + 100) mu; g >> g0: mu + tau_y / g; continuous across
+    the 1e-4 g0 switch to its limit (to f32 cancellation); decreasing in g."""
+    mu, tau = 1.0e-3, 2.0
+    g0 = tau / (100.0 * mu)
+    out = orc.c3_bingham_of(np.array([0.0, np.nan, np.inf], np.float32), mu, tau)
+    assert np.allclose(out, 101.0 * mu, rtol=1e-6, atol=0)
+    big = np.array([1e3 * g0, 1e5 * g0], np.float32)
+    assert np.allclose(orc.c3_bingham_of(big, mu, tau), mu + tau / big.astype(np.float64), rtol=1e-6, atol=0)
+    lo, hi = orc.c3_bingham_of(np.array([0.999e-4 * g0, 1.001e-4 * g0], np.float32), mu, tau)
+    # 1 - exp(-x) at x = 1e-4 cancels in f32 (ulp(1) / x = 6e-4 relative), as in the reference
+    assert abs(hi - lo) / lo < 1e-3
+    g = np.logspace(-1, 4, 200).astype(np.float32) * np.float32(g0)  # above the f32 cancellation range
+    vals = orc.c3_bingham_of(g, mu, tau).astype(np.float64)
+    assert np.all(np.diff(vals) <= 1e-9 * vals[:-1])
+
+
+def test_bingham_step_is_more_viscous_and_tends_to_newtonian(orc):
+    """A yield stress only raises the effective viscosity: more CG iterations and less kinetic
+    energy than the Newtonian step; a vanishing yield stress gives the Newtonian step back."""
+    base = {"side": 24, "mu": 1.0e-3}
+    newt = orc.C3Oracle(base, "mixed")
+    bing = orc.C3Oracle(dict(base, yield_stress=1.0), "mixed")
+    tiny = orc.C3Oracle(dict(base, yield_stress=1.0e-9), "mixed")
+    for s in (newt, bing, tiny):
+        s.step()
+    _, vn = newt.state()
+    _, vb = bing.state()
+    _, vt = tiny.state()
+    assert bing.stats()["cg_iterations"] > newt.stats()["cg_iterations"]
+    assert (vb * vb).sum() < (vn * vn).sum()
+    assert np.abs(vt - vn).max() <= 1e-5 * np.abs(vn).max()
