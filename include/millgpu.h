@@ -148,6 +148,8 @@ typedef struct mg_box_config {
   uint64_t seed;   /* Rng seed of the initial state (rng.rs) */
   uint32_t fp64;   /* 0: FP32 state with FP64 CG reductions; 1: FP64 state and arithmetic
                       (same f32 initial state and geometry, promoted) */
+  float yield_stress; /* Bingham yield stress (Pa, the reference's yield_stress_pa with rheology
+                         "bingham"); 0: Newtonian */
 } mg_box_config;
 
 /* h, L, cell width, particle mass, cells per side. */

@@ -15,13 +15,13 @@ import numpy as np
 from . import sim as _sim
 
 BOX_DEFAULTS = {"side": 1024, "dx": 1.0e-3, "rest": 1000.0, "mu": 1.0e-3, "dt": 1.0 / 1440.0,
-                "passes": 4, "tol": 1.0e-6, "max_cg": 500, "seed": 1, "fp64": 0}
+                "passes": 4, "tol": 1.0e-6, "max_cg": 500, "seed": 1, "fp64": 0, "yield_stress": 0.0}
 
 
 class BoxConfig(C.Structure):
     _fields_ = [("side", C.c_uint32), ("dx", C.c_float), ("rest", C.c_float), ("mu", C.c_float),
                 ("dt", C.c_float), ("passes", C.c_uint32), ("tol", C.c_float), ("max_cg", C.c_uint32),
-                ("seed", C.c_uint64), ("fp64", C.c_uint32)]
+                ("seed", C.c_uint64), ("fp64", C.c_uint32), ("yield_stress", C.c_float)]
 
 
 def box_config(overrides=None) -> dict:
@@ -33,7 +33,8 @@ def box_config(overrides=None) -> dict:
 def _cfg_struct(cfg) -> BoxConfig:
     c = box_config(cfg)
     return BoxConfig(int(c["side"]), float(c["dx"]), float(c["rest"]), float(c["mu"]), float(c["dt"]),
-                     int(c["passes"]), float(c["tol"]), int(c["max_cg"]), int(c["seed"]), int(c["fp64"]))
+                     int(c["passes"]), float(c["tol"]), int(c["max_cg"]), int(c["seed"]), int(c["fp64"]),
+                     float(c["yield_stress"]))
 
 
 _ready = set()

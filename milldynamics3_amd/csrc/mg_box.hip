@@ -33,6 +33,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <functional>
 #include <string>
@@ -136,6 +137,7 @@ struct BoxScal {
   double rr[2];
   uint32_t it, done, max_cg;
   float tol;
+  const void* mu_eff;  // per-particle effective viscosity (Bingham boxes), else null; set at creation
 };
 
 // ---------------------------------------------------------------------------------------------
@@ -146,6 +148,8 @@ BDEV double msqrt(double a) { return sqrt(a); }
 BDEV float mfmod(float a, float b) { return fmodf(a, b); }
 BDEV double mfmod(double a, double b) { return fmod(a, b); }
 BDEV float mfloor(float a) { return floorf(a); }
+BDEV float mexp(float a) { return expf(a); }
+BDEV double mexp(double a) { return exp(a); }
 BDEV double mfloor(double a) { return floor(a); }
 BDEV float mfabs(float a) { return fabsf(a); }
 BDEV double mfabs(double a) { return fabs(a); }
@@ -918,18 +922,56 @@ BXKRT bx_vel_density(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, 
   if (threadIdx.x == 0) part_b[blockIdx.x] = t;
 }
 
+// Effective viscosity of a Bingham box (pbf.rs:2147-2151): the shear rate sqrt(2 D:D) of each
+// particle from the SPH velocity gradient over the step's pairs, at the final positions, the updated
+// velocities and the recomputed densities (pbf.rs:821-849), then the Papanastasiou-regularised
+// mu + tau_y (1 - exp(-g / g0)) / g with g0 = tau_y / (100 mu), its g -> 0 limit below 1e-4 g0 and a
+// non-finite rate taken as 0 (pbf.rs:854-868, :490). Sums in the table's (reference) pair order.
+BXKT bx_shear_mu(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const Vec<R>* __restrict__ V,
+                 const R* __restrict__ DENS, R* __restrict__ MU, R mu, R tau_y) {
+  if (*g.halt) return;
+  const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
+  if (i >= g.n) return;
+  const Vec<R> xi = X[i], vi = V[i];
+  R gxx = (R)0, gxy = (R)0, gyx = (R)0, gyy = (R)0;
+  const uint32_t cw = pv.cnt[i];
+  struct Nb { Vec<R> x; Vec<R> v; R d; };
+  by_wrap(wave_fast(xi, cw, g), [&](auto w) __attribute__((always_inline)) {
+    for_pairs_g<4>(g, pv, i, cw, w,
+                   [&](uint32_t j) __attribute__((always_inline)) { return Nb{at(X, j), at(V, j), at(DENS, j)}; },
+                   [&](uint32_t, const Nb& a) __attribute__((always_inline)) {
+      const Vec<R> d = rel(xi, a.x, g, w);
+      const R r = pair_len(len2(d));
+      const Vec<R> gr = spiky(d, r, g, guarded(w));
+      const R coeff = g.mass / mfmax(a.d, (R)1e-6f);
+      const Vec<R> dv = sub2(a.v, vi);
+      gxx += coeff * dv.x * gr.x;
+      gxy += coeff * dv.x * gr.y;
+      gyx += coeff * dv.y * gr.x;
+      gyy += coeff * dv.y * gr.y;
+    });
+  });
+  const R dxy = (R)0.5f * (gxy + gyx);
+  R rate = msqrt((R)2 * (gxx * gxx + gyy * gyy + (R)2 * dxy * dxy));
+  const R g0 = tau_y / ((R)100 * mu);
+  if (!isfinite(rate)) rate = (R)0;
+  const R extra = rate < (R)1e-4f * g0 ? tau_y / g0 : tau_y * ((R)1 - mexp(-rate / g0)) / rate;
+  MU[i] = mu + extra;
+}
+
 // Morris weight and unit direction of pair (i, j) (pbf.rs:312-344).
 // The quotient d / r is formed once: the kernel gradient (d / r) * coeff and the direction d / r
 // use the same rounded value, exactly as two separate divisions would give. pair_unit forms the
 // same direction alone (the CG recomputes it from the unchanged positions instead of storing it).
+// mu2m = mass * (mu_i + mu_j): g.mu2m for a Newtonian box, else from the effective viscosities.
 template <class R, bool W>
-BDEV void morris(const BoxGeo<R>& g, Wrap<W> wr, Vec<R> xi, R rho_i, Vec<R> xj, R dens_j, R& w, Vec<R>& e) {
+BDEV void morris(const BoxGeo<R>& g, Wrap<W> wr, Vec<R> xi, R rho_i, Vec<R> xj, R dens_j, R mu2m, R& w, Vec<R>& e) {
   const R rho_j = mfmax(dens_j, (R)1e-6f);
   const Vec<R> d = rel(xi, xj, g, wr);
   const R r = pair_len(len2(d));
   const Vec<R> dr = pair_div(d, r, guarded(wr));
   const Vec<R> gr = (r <= (R)1e-9f || r >= g.h) ? mk((R)0, (R)0) : mul2(dr, g.ks * (g.h - r) * (g.h - r));
-  const R num = (g.mu2m / (rho_i * rho_j)) * mfabs(dot2(d, gr));
+  const R num = (mu2m / (rho_i * rho_j)) * mfabs(dot2(d, gr));
   w = num / (len2(d) + g.eta2);
   e = r > (R)1e-9f ? dr : mk((R)1, (R)0);
 }
@@ -940,21 +982,28 @@ template <class R> BDEV Vec<R> pair_unit(Vec<R> d, bool guard) {
 // (L v)_i with the weights computed here; the weights of a tabled particle are kept for the
 // operator applications of this step's CG, in the table's quad layout: the four weights of one
 // quad of table entries leave as one vector store.
-template <class R>
+// BG: Bingham box, the pair's viscosity sum from the effective viscosities MU (pbf.rs:2147-2154).
+template <class R, bool BG>
 BDEV Vec<R> lap_row(const BoxGeo<R>& g, uint32_t i, const PairView<R>& pv, const Vec<R>* __restrict__ X,
-                    const R* __restrict__ DENS, const Vec<R>* __restrict__ in, R* __restrict__ CW) {
+                    const R* __restrict__ DENS, const Vec<R>* __restrict__ in, R* __restrict__ CW,
+                    const R* __restrict__ MU) {
   const Vec<R> xi = X[i], ini = in[i];
   const R rho_i = mfmax(DENS[i], (R)1e-6f);
+  R mu_i = (R)0;
+  if constexpr (BG) mu_i = MU[i];
   const uint32_t cw = pv.cnt[i], c = pcount(cw);
   Vec<R> s = mk((R)0, (R)0);
   Vec4<R>* const cwq = reinterpret_cast<Vec4<R>*>(CW);
-  struct Nb { Vec<R> x; R d; Vec<R> v; };
+  struct Nb { Vec<R> x; R d; Vec<R> v; R mu; };
   by_wrap(wave_fast(xi, cw, g), [&](auto wr) __attribute__((always_inline)) {
-    auto load = [&](uint32_t j) __attribute__((always_inline)) { return Nb{at(X, j), at(DENS, j), at(in, j)}; };
+    auto load = [&](uint32_t j) __attribute__((always_inline)) {
+      if constexpr (BG) return Nb{at(X, j), at(DENS, j), at(in, j), at(MU, j)};
+      else return Nb{at(X, j), at(DENS, j), at(in, j), (R)0};
+    };
     auto term = [&](const Nb& a) __attribute__((always_inline)) {
       R w;
       Vec<R> e;
-      morris(g, wr, xi, rho_i, a.x, a.d, w, e);
+      morris(g, wr, xi, rho_i, a.x, a.d, BG ? g.mass * (mu_i + a.mu) : g.mu2m, w, e);
       s = add2(s, smul2((R)4 * w * dot2(e, sub2(ini, a.v)), e));
       return w;
     };
@@ -985,17 +1034,18 @@ BDEV Vec<R> lap_row(const BoxGeo<R>& g, uint32_t i, const PairView<R>& pv, const
   return s;
 }
 
-BXKRT bx_cg_init(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const R* __restrict__ DENS,
-                 const Vec<R>* __restrict__ V, Vec<R>* __restrict__ Rv, Vec<R>* __restrict__ P,
-                 R* __restrict__ CW, double* __restrict__ part_rr0,
-                 const BoxScal* __restrict__ sc) {
+template <class R, bool BG>
+BDEV void cg_init_body(const BoxGeo<R>& g, const PairView<R>& pv, const Vec<R>* __restrict__ X,
+                       const R* __restrict__ DENS, const Vec<R>* __restrict__ V, Vec<R>* __restrict__ Rv,
+                       Vec<R>* __restrict__ P, R* __restrict__ CW, double* __restrict__ part_rr0,
+                       const BoxScal* __restrict__ sc, const R* __restrict__ MU) {
   if (*g.halt | sc->done) return;  // both words loaded at once
   const uint32_t ppt = red_ppt(g);
   const uint32_t base = tile_id_red(g) * BX_RBLOCK * ppt + threadIdx.x;
   double rr = 0.0;
   for_lane_particles(g, base, [&](uint32_t i, uint32_t k) __attribute__((always_inline)) {
     const Vec<R> v = V[i];
-    const Vec<R> lv = lap_row(g, i, pv, X, DENS, V, CW);
+    const Vec<R> lv = lap_row<R, BG>(g, i, pv, X, DENS, V, CW, MU);
     const Vec<R> r = sub2(v, add2(v, mul2(lv, g.dt)));
     Rv[i] = r;
     P[i] = r;
@@ -1003,6 +1053,18 @@ BXKRT bx_cg_init(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, cons
   });
   const double t = block_sum<BX_RBLOCK>(rr);
   if (threadIdx.x == 0) part_rr0[blockIdx.x] = t;
+}
+BXKRT bx_cg_init(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const R* __restrict__ DENS,
+                 const Vec<R>* __restrict__ V, Vec<R>* __restrict__ Rv, Vec<R>* __restrict__ P,
+                 R* __restrict__ CW, double* __restrict__ part_rr0, const BoxScal* __restrict__ sc) {
+  cg_init_body<R, false>(g, pv, X, DENS, V, Rv, P, CW, part_rr0, sc, nullptr);
+}
+// The Bingham box's set-up: the Morris weights from the effective viscosities MU.
+BXKRT bx_cg_init_bingham(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const R* __restrict__ DENS,
+                         const Vec<R>* __restrict__ V, Vec<R>* __restrict__ Rv, Vec<R>* __restrict__ P,
+                         R* __restrict__ CW, double* __restrict__ part_rr0, const BoxScal* __restrict__ sc,
+                         const R* __restrict__ MU) {
+  cg_init_body<R, true>(g, pv, X, DENS, V, Rv, P, CW, part_rr0, sc, MU);
 }
 
 // CG iteration t (pbf.rs:423-480), first half. Every block evaluates the scalars from the partial
@@ -1105,11 +1167,12 @@ BXKRT bx_cg_dir(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const
         }
       } else {
         const R rho_i = mfmax(DENS[i], (R)1e-6f);
+        const R* const MU = static_cast<const R*>(sc->mu_eff);  // a Bingham box's effective viscosities
         scan_pairs(g, pv.XP, pv.start, i, pv.XP[i], [&](uint32_t j) __attribute__((always_inline)) {
           const Vec<R> pj = first ? Pold[j] : add2(Rv[j], mul2(Pold[j], beta));
           R w;
           Vec<R> e;
-          morris(g, wr, xi, rho_i, X[j], DENS[j], w, e);
+          morris(g, wr, xi, rho_i, X[j], DENS[j], MU ? g.mass * (MU[i] + MU[j]) : g.mu2m, w, e);
           s = add2(s, smul2((R)4 * w * dot2(e, sub2(p, pj)), e));
         });
       }
@@ -1307,6 +1370,7 @@ struct mg_box {
   // cell starts of the current step and the buffer the current step clears for the next one
   uint32_t *start = nullptr, *start_next = nullptr;
   void *LAM = nullptr, *DENS = nullptr;  // LAM: pass A's output for pass B (see put_lambda)
+  void* MU = nullptr;  // effective viscosity per particle (Bingham boxes: cfg.yield_stress > 0)
   void *R = nullptr, *P = nullptr, *AP = nullptr;
   int16_t* pnb = nullptr;    // per-step pair table (16-bit relative offsets)
   uint32_t* pcnt = nullptr;  // and the per-particle pair counts
@@ -1373,6 +1437,10 @@ bool box_geometry(const mg_box_config& c, BoxGeo<float>& g, std::string& err) {
   }
   if (!(c.dx <= 1024.0f)) { err = "dx must be at most 1024 (box side at most 2^23)"; return false; }
   if (c.fp64 > 1u) { err = "fp64 must be 0 or 1"; return false; }
+  if (!(c.yield_stress >= 0.0f) || !std::isfinite(c.yield_stress)) {
+    err = "yield_stress must be finite and non-negative";
+    return false;
+  }
   g.n = c.side * c.side;
   g.h = 2.0f * c.dx;
   g.h2 = g.h * g.h;
@@ -1597,10 +1665,20 @@ template <class R> bool enqueue_front(mg_box* B) {
     hipLaunchKernelGGL(bx_vel_density<R>, gr, rblk, 0, st, g, pv, vp<R>(B->X), vp<R>(B->X0), vp<R>(B->VN),
                        sp<R>(B->DENS), B->part_b);
   }
+  const bool bingham = B->cfg.yield_stress > 0.0f;
+  if (!cg_off && bingham) {
+    Tic t(B, BXT_VEL);
+    hipLaunchKernelGGL(bx_shear_mu<R>, gp, blk, 0, st, g, pv, vp<R>(B->X), vp<R>(B->VN), sp<R>(B->DENS),
+                       sp<R>(B->MU), (R)B->cfg.mu, (R)B->cfg.yield_stress);
+  }
   if (!cg_off) {
     Tic t(B, BXT_CGINIT);
-    hipLaunchKernelGGL(bx_cg_init<R>, gr, rblk, 0, st, g, pv, vp<R>(B->X), sp<R>(B->DENS), vp<R>(B->VN),
-                       vp<R>(B->R), vp<R>(B->P), sp<R>(B->cw), B->part_rr, B->sc);
+    if (bingham)
+      hipLaunchKernelGGL(bx_cg_init_bingham<R>, gr, rblk, 0, st, g, pv, vp<R>(B->X), sp<R>(B->DENS), vp<R>(B->VN),
+                         vp<R>(B->R), vp<R>(B->P), sp<R>(B->cw), B->part_rr, B->sc, sp<R>(B->MU));
+    else
+      hipLaunchKernelGGL(bx_cg_init<R>, gr, rblk, 0, st, g, pv, vp<R>(B->X), sp<R>(B->DENS), vp<R>(B->VN),
+                         vp<R>(B->R), vp<R>(B->P), sp<R>(B->cw), B->part_rr, B->sc);
   }
   B->cg_t = 1;
   B->have_neighbors = true;
@@ -1851,6 +1929,7 @@ int mg_box_create(const mg_box_config* cfg, mg_box** out, char* err, size_t errl
   off = carve(off, B->cursor, (size_t)B->ncell);
   off = carve(off, B->bsum, (size_t)B->nscan);
   off = carve_bytes(off, B->LAM, 4 * rs * n); off = carve_bytes(off, B->DENS, rs * n);
+  off = carve_bytes(off, B->MU, cfg->yield_stress > 0.0f ? rs * n : 0);
   off = carve_bytes(off, B->R, vb); off = carve_bytes(off, B->P, vb); off = carve_bytes(off, B->AP, vb);
   off = carve(off, B->part_b, (size_t)B->nred); off = carve(off, B->part_rr, 2 * (size_t)B->nred);
   off = carve(off, B->part_pap, (size_t)B->nred); off = carve_bytes(off, B->P2, vb);
@@ -1863,11 +1942,16 @@ int mg_box_create(const mg_box_config* cfg, mg_box** out, char* err, size_t errl
   auto fix = [&](auto*& p) { p = reinterpret_cast<std::remove_reference_t<decltype(p)>>(base + (uintptr_t)p); };
   fix(B->X); fix(B->XP); fix(B->X0); fix(B->V); fix(B->X2); fix(B->Xb); fix(B->X0b); fix(B->Vb); fix(B->VN);
   fix(B->id); fix(B->idb); fix(B->key); fix(B->slot); fix(B->start); fix(B->start_next); fix(B->cursor); fix(B->bsum);
-  fix(B->LAM); fix(B->DENS); fix(B->R); fix(B->P); fix(B->AP); fix(B->part_b); fix(B->part_rr); fix(B->part_pap);
+  fix(B->LAM); fix(B->DENS); fix(B->MU); fix(B->R); fix(B->P); fix(B->AP); fix(B->part_b); fix(B->part_rr); fix(B->part_pap);
   fix(B->P2); fix(B->sc); fix(B->pnb); fix(B->pcnt); fix(B->cw); fix(B->verdict); fix(B->halt);
   B->gf.halt = B->halt;
   B->gd.halt = B->halt;
   bool ok = hipMemset(B->arena, 0, off) == hipSuccess;
+  if (cfg->yield_stress > 0.0f) {  // the CG's rescan path finds the effective viscosities here
+    const void* mu_eff = B->MU;
+    ok = ok && hipMemcpy(reinterpret_cast<char*>(B->sc) + offsetof(BoxScal, mu_eff), &mu_eff, sizeof mu_eff,
+                         hipMemcpyHostToDevice) == hipSuccess;
+  }
   ok = ok && hipHostMalloc((void**)&B->h_sc, sizeof(BoxScal), 0) == hipSuccess;
   ok = ok && hipHostMalloc((void**)&B->h_verdict, (BX_WINDOW + 1) * sizeof(uint32_t), 0) == hipSuccess;
   ok = ok && hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) == hipSuccess;

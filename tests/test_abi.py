@@ -48,6 +48,9 @@ def test_box_geometry_rejects_bad_config():
         box_geometry({"side": 4})
     with pytest.raises(ValueError):
         box_geometry({"dx": -1.0})
+    for bad in (-1.0, float("inf"), float("nan")):
+        with pytest.raises(ValueError):
+            box_geometry({"yield_stress": bad})
 
 
 def test_params_validation_messages_without_device(L):

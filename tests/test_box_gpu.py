@@ -277,12 +277,14 @@ def test_pair_arith_matches_ieee():
     assert list(out) == [0, 0, 0, 0], f"mismatches (f32 sqrt, f32 quotient, f64 sqrt, f64 quotient): {list(out)}"
 
 
-@pytest.mark.parametrize("fp64", [0, 1])
-def test_dense_cluster_rescans_match_oracle(box_mod, orc, fp64):
+@pytest.mark.parametrize("fp64,yield_stress", [(0, 0.0), (1, 0.0), (0, 1.0), (1, 1.0)])
+def test_dense_cluster_rescans_match_oracle(box_mod, orc, fp64, yield_stress):
     # 300 particles packed at 0.35 dx: up to ~114 pairs each, so the pair table overflows (more than
-    # 32 pairs) and every kernel takes its rescan path for them, the CG's operator included
+    # 32 pairs) and every kernel takes its rescan path for them, the CG's operator included (with a
+    # yield stress: its effective viscosities too)
     side, dx = 32, 1.0e-3
-    cfg = {"side": side, "mu": 1.0, "fp64": fp64}
+    cfg = {"side": side, "mu": 1.0, "fp64": fp64, "yield_stress": yield_stress}
+    print(f"box dense cluster fp64={fp64} yield_stress={yield_stress}", flush=True)
     b = box_mod.Box(cfg)
     o = orc.C3Oracle(cfg, "fp64" if fp64 else "mixed")
     x, v = o.state()
@@ -328,3 +330,34 @@ def test_replay_timing_leaves_state_unchanged(box_mod, orc):
     b.step()
     o.step()
     _compare(b, o, "after replay")
+
+
+# Bingham boxes (pbf.rs:821-868, :2147-2151): shear rates and effective viscosities on the GPU, in
+# the table's pair order. The f32 exp of the regularisation is the device's expf (the oracle's is
+# the host's), which may differ by an ulp; the effective viscosities only enter the CG, whose
+# velocities are held to the same 1e-6 bar.
+@pytest.mark.parametrize("side,mu,yield_stress,steps", [(64, 1.0e-3, 1.0, 3), (128, 1.0e-3, 1.0, 2),
+                                                        (40, 1.0e-2, 0.5, 2)])
+def test_bingham_steps_match_oracle(box_mod, orc, side, mu, yield_stress, steps):
+    cfg = {"side": side, "mu": mu, "yield_stress": yield_stress}
+    print(f"box bingham side={side} mu={mu} yield_stress={yield_stress}", flush=True)
+    b = box_mod.Box(cfg)
+    o = orc.C3Oracle(cfg, "mixed")
+    newt = orc.C3Oracle({"side": side, "mu": mu}, "mixed")
+    for k in range(steps):
+        b.step()
+        o.step()
+        newt.step()
+        _compare(b, o, f"bingham side={side} step={k}")
+    assert o.stats()["cg_iterations"] > newt.stats()["cg_iterations"]  # the yield stress is active
+
+
+def test_fp64_bingham_matches_oracle(box_mod, orc):
+    cfg = {"side": 32, "mu": 1.0e-3, "yield_stress": 1.0, "fp64": 1}
+    print("box fp64 bingham side=32", flush=True)
+    b = box_mod.Box(cfg)
+    o = orc.C3Oracle(cfg, "fp64")
+    for k in range(3):
+        b.step()
+        o.step()
+        _compare64(b, o, f"fp64 bingham step={k}")
