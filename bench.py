@@ -166,6 +166,29 @@ def bench_fp64(args, cfg) -> dict:
             "fp32_vs_fp64": precision_vs_fp64(cfg, args.warmup + args.steps)}
 
 
+def bench_bingham(args, cfg, yield_stress=1.0) -> dict:
+    """The same workload as a Bingham fluid (yield stress in Pa): the shear-rate / effective-viscosity
+    kernel before the CG set-up; K steps in one call after the warm-up; the last step's mean shear
+    rate."""
+    import torch
+    from milldynamics3_amd.box import Box
+    box = Box(dict(cfg, yield_stress=yield_stress))
+    n = box.n
+    box.step(args.warmup)
+    torch.cuda.synchronize()
+    cg0 = box.total_cg_iterations
+    t0 = time.perf_counter()
+    box.step(args.steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    out = {"metric": "particle-steps/s", "value": round(n * args.steps / wall, 1),
+           "ms_per_step": round(1000.0 * wall / args.steps, 4), "yield_stress_pa": yield_stress,
+           "cg_iterations_per_step": (box.total_cg_iterations - cg0) / args.steps,
+           "mean_shear_rate_per_s": box.mean_shear_rate(), "dtype": "f32 state, f64 CG reductions"}
+    box.close()
+    return out
+
+
 def precision_vs_fp64(cfg, steps) -> dict:
     """Error of the FP32-state box's outputs against the FP64-state box after the same `steps` steps
     from the same initial state (the FP64 box is the oracle's f64 policy on the GPU): positions in
@@ -206,6 +229,7 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-state box measurement")
+    ap.add_argument("--no-bingham", action="store_true", help="skip the Bingham-fluid box measurement")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one plain device copy of this size (calibrates PMC byte counters)")
     args = ap.parse_args()
@@ -300,8 +324,11 @@ def main() -> int:
            "config": {"workload": f"periodic PBF box {args.side}x{args.side}", "particles": n, "mu": args.mu,
                       "dtype": "f32 state, f64 CG reductions"},
            "kernels_probe": kernels}
+    iso["mean_shear_rate_per_s"] = box.mean_shear_rate()
     if not args.no_fp64:
         iso["fp64"] = bench_fp64(args, cfg)
+    if not args.no_bingham:
+        iso["bingham"] = bench_bingham(args, cfg)
     headline_cpu = None
     if not args.no_cpu:
         headline_cpu = cpu_baseline_drum(args.cpu_seconds)

@@ -171,6 +171,12 @@ int mg_box_density(mg_box* box, float* out_n);
 int mg_box_density_f64(mg_box* box, double* out_n);
 /* CG iterations, final residual norm and |b| of the last step. */
 int mg_box_stats(mg_box* box, double* out3);
+/* Mean shear rate (1/s) of the last step: the mean over particles of sqrt(2 D:D) from the SPH
+ * velocity gradient at the step's final positions, velocities and densities over its neighbour
+ * lists, as the reference reports each substep (FluidStepStats::mean_shear_rate_per_s,
+ * pbf.rs:800-818, :2303-2304); the sum is FP64, rounded to f32 for an FP32 box. 0 before the first
+ * step. Computed on request (one kernel), not inside the step. */
+int mg_box_mean_shear_rate(mg_box* box, double* out);
 /* CG iterations summed over every step this box has run. */
 uint64_t mg_box_total_cg_iterations(const mg_box* box);
 /* Multi-step calls run windows of up to 16 steps between host waits, each step's CG given the

@@ -76,6 +76,7 @@ def _lib():
     L.mg_box_set_cg_prediction.argtypes = [P, C.c_uint32]
     L.mg_box_kernel_times.argtypes = [P, dp, C.POINTER(C.c_uint64), C.c_int]
     L.mg_box_replay_ms.argtypes = [P, C.c_uint32, C.c_uint32, dp]
+    L.mg_box_mean_shear_rate.argtypes = [P, dp]
     _ready.add(id(L))
     return L
 
@@ -173,6 +174,12 @@ class Box:
             d = np.zeros(self.n, dtype=np.float32)
             _sim._check(self._L.mg_box_density(self._h, _sim._fp(d)), what="density")
         return d
+
+    def mean_shear_rate(self) -> float:
+        """Mean shear rate (1/s) of the last step (see include/millgpu.h), 0 before the first step."""
+        out = C.c_double()
+        _sim._check(self._L.mg_box_mean_shear_rate(self._h, C.byref(out)), what="mean_shear_rate")
+        return out.value
 
     def stats(self) -> dict:
         o = np.zeros(3)

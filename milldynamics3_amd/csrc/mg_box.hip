@@ -922,16 +922,12 @@ BXKRT bx_vel_density(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, 
   if (threadIdx.x == 0) part_b[blockIdx.x] = t;
 }
 
-// Effective viscosity of a Bingham box (pbf.rs:2147-2151): the shear rate sqrt(2 D:D) of each
-// particle from the SPH velocity gradient over the step's pairs, at the final positions, the updated
-// velocities and the recomputed densities (pbf.rs:821-849), then the Papanastasiou-regularised
-// mu + tau_y (1 - exp(-g / g0)) / g with g0 = tau_y / (100 mu), its g -> 0 limit below 1e-4 g0 and a
-// non-finite rate taken as 0 (pbf.rs:854-868, :490). Sums in the table's (reference) pair order.
-BXKT bx_shear_mu(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const Vec<R>* __restrict__ V,
-                 const R* __restrict__ DENS, R* __restrict__ MU, R mu, R tau_y) {
-  if (*g.halt) return;
-  const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
-  if (i >= g.n) return;
+// Shear rate sqrt(2 D:D) of particle i from the SPH velocity gradient over the step's pairs, at the
+// final positions, the given velocities and the recomputed densities (pbf.rs:821-849), summed in the
+// table's (reference) pair order.
+template <class R>
+BDEV R shear_rate(const BoxGeo<R>& g, const PairView<R>& pv, uint32_t i, const Vec<R>* __restrict__ X,
+                  const Vec<R>* __restrict__ V, const R* __restrict__ DENS) {
   const Vec<R> xi = X[i], vi = V[i];
   R gxx = (R)0, gxy = (R)0, gyx = (R)0, gyy = (R)0;
   const uint32_t cw = pv.cnt[i];
@@ -952,11 +948,42 @@ BXKT bx_shear_mu(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, cons
     });
   });
   const R dxy = (R)0.5f * (gxy + gyx);
-  R rate = msqrt((R)2 * (gxx * gxx + gyy * gyy + (R)2 * dxy * dxy));
+  return msqrt((R)2 * (gxx * gxx + gyy * gyy + (R)2 * dxy * dxy));
+}
+// Effective viscosity of a Bingham box (pbf.rs:2147-2151): the Papanastasiou-regularised
+// mu + tau_y (1 - exp(-g / g0)) / g of each particle's shear rate g, with g0 = tau_y / (100 mu), its
+// g -> 0 limit below 1e-4 g0 and a non-finite rate taken as 0 (pbf.rs:854-868, :490).
+BXKT bx_shear_mu(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const Vec<R>* __restrict__ V,
+                 const R* __restrict__ DENS, R* __restrict__ MU, R mu, R tau_y) {
+  if (*g.halt) return;
+  const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
+  if (i >= g.n) return;
+  R rate = shear_rate(g, pv, i, X, V, DENS);
   const R g0 = tau_y / ((R)100 * mu);
   if (!isfinite(rate)) rate = (R)0;
   const R extra = rate < (R)1e-4f * g0 ? tau_y / g0 : tau_y * ((R)1 - mexp(-rate / g0)) / rate;
   MU[i] = mu + extra;
+}
+// Mean shear rate of the last step (pbf.rs:800-818, reported at :2303-2304): per reduction block the
+// FP64 sum of the finite rates (part_sum) and their count (part_cnt).
+BXKRT bx_shear_mean(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const Vec<R>* __restrict__ V,
+                    const R* __restrict__ DENS, double* __restrict__ part_sum, double* __restrict__ part_cnt) {
+  const uint32_t ppt = red_ppt(g);
+  const uint32_t base = tile_id_red(g) * BX_RBLOCK * ppt + threadIdx.x;
+  double sum = 0.0, cnt = 0.0;
+  for_lane_particles(g, base, [&](uint32_t i, uint32_t) __attribute__((always_inline)) {
+    const R rate = shear_rate(g, pv, i, X, V, DENS);
+    if (isfinite(rate)) {
+      sum += (double)rate;
+      cnt += 1.0;
+    }
+  });
+  const double ts = block_sum<BX_RBLOCK>(sum);
+  const double tc = block_sum<BX_RBLOCK>(cnt);
+  if (threadIdx.x == 0) {
+    part_sum[blockIdx.x] = ts;
+    part_cnt[blockIdx.x] = tc;
+  }
 }
 
 // Morris weight and unit direction of pair (i, j) (pbf.rs:312-344).
@@ -2052,6 +2079,31 @@ int mg_box_density(mg_box* B, float* out) {
 int mg_box_density_f64(mg_box* B, double* out) {
   if (!B || !out) return MG_E_ARG;
   return B->f64 ? density_as<double, double>(B, out) : density_as<float, double>(B, out);
+}
+
+extern "C++" template <class R> int mean_shear_as(mg_box* B, double* out) {
+  const BoxGeo<R>& g = geo<R>(B);
+  const PairView<R> pv = pair_view<R>(B);
+  // part_b and part_pap are free between steps (each step rewrites them before reading)
+  hipLaunchKernelGGL(bx_shear_mean<R>, dim3(g.nred), dim3(BX_RBLOCK), 0, B->stream, g, pv, vp<R>(B->X), vp<R>(B->V),
+                     sp<R>(B->DENS), B->part_b, B->part_pap);
+  std::vector<double> ps(g.nred), pc(g.nred);
+  bool ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(B->stream) == hipSuccess;
+  ok = ok && hipMemcpy(ps.data(), B->part_b, g.nred * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess;
+  ok = ok && hipMemcpy(pc.data(), B->part_pap, g.nred * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess;
+  if (!ok) return MG_E_DEVICE;
+  double sum = 0.0, cnt = 0.0;
+  for (uint32_t k = 0; k < g.nred; k++) { sum += ps[k]; cnt += pc[k]; }
+  const double mean = cnt > 0.0 ? sum / cnt : 0.0;
+  *out = B->f64 ? mean : (double)(float)mean;  // FP32 boxes report the f32 value (pbf.rs:880)
+  return MG_OK;
+}
+
+int mg_box_mean_shear_rate(mg_box* B, double* out) {
+  if (!B || !out) return MG_E_ARG;
+  if (!B->have_neighbors) { *out = 0.0; return MG_OK; }
+  if (hipStreamSynchronize(B->stream) != hipSuccess) return MG_E_DEVICE;
+  return B->f64 ? mean_shear_as<double>(B, out) : mean_shear_as<float>(B, out);
 }
 
 int mg_box_stats(mg_box* B, double* out3) {

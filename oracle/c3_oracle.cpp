@@ -209,6 +209,18 @@ template <class R, class A> struct C3Sim {
     return s;
   }
 
+  // Mean shear rate of the last step (pbf.rs:800-818 at :2303-2304: final positions, the step's
+  // velocities, lists and recomputed densities); the sum in the accumulation type A.
+  double mean_shear() const {
+    if (nbr.off.empty()) return 0.0;
+    const std::vector<R> rates = c3_shear_rates(x, v, nbr, dens, (R)box.mass, (R)box.h, box);
+    A sum = 0;
+    uint32_t c = 0;
+    for (R g : rates)
+      if (std::isfinite(g)) { sum += (A)g; c++; }
+    return c == 0 ? 0.0 : (double)(R)(sum / (A)c);
+  }
+
   void step() {
     const size_t n = x.size();
     const R dt = (R)cfg.dt, h = (R)box.h, rest = (R)cfg.rest, mass = (R)box.mass, mu = (R)cfg.mu;
@@ -402,6 +414,12 @@ void c3_stats(void* hv, double* out) {
   auto* h = (C3Handle*)hv;
   C3DISPATCH(h, { out[0] = s->last_cg; out[1] = (double)s->last_res; out[2] = (double)s->last_bnorm; });
 }
+double c3_mean_shear(void* hv) {
+  auto* h = (C3Handle*)hv;
+  double m = 0.0;
+  C3DISPATCH(h, m = s->mean_shear());
+  return m;
+}
 void c3_density(void* hv, double* out) {
   auto* h = (C3Handle*)hv;
   C3DISPATCH(h, { for (size_t i = 0; i < s->dens.size(); i++) out[i] = (double)s->dens[i]; });
@@ -427,17 +445,24 @@ int64_t c3_neighbors_of(const double* cfg, const float* pts, size_t n, uint32_t*
   if ((int64_t)L.nb.size() <= nb_cap) std::memcpy(nb, L.nb.data(), L.nb.size() * 4);
   return (int64_t)L.nb.size();
 }
-// Shear rates (f32, pbf.rs:821-849) of caller points with velocities and densities over the box's
-// neighbour lists of those points (cfg gives side/dx; h and L follow from them).
+// Shear rates (f32, pbf.rs:821-849) of caller points with velocities and densities over the given
+// neighbour CSR (off: n + 1 entries), or when off is null over the box's lists of those points (cfg
+// gives side/dx; h and L follow from them).
 void c3_shear_rates_of(const double* cfg, const float* pts, const float* vel, const float* dens, size_t n,
-                       float mass, float* out) {
+                       float mass, const uint32_t* off, const uint32_t* nb, float* out) {
   orc::C3Box b = orc::C3Box::make(c3_cfg_from(cfg));
   std::vector<orc::V2<float>> p(n), v(n);
   for (size_t i = 0; i < n; i++) {
     p[i] = orc::V2<float>(pts[2 * i], pts[2 * i + 1]);
     v[i] = orc::V2<float>(vel[2 * i], vel[2 * i + 1]);
   }
-  orc::Nbrs L = orc::c3_neighbors(p, b);
+  orc::Nbrs L;
+  if (off) {
+    L.off.assign(off, off + n + 1);
+    L.nb.assign(nb, nb + off[n]);
+  } else {
+    L = orc::c3_neighbors(p, b);
+  }
   std::vector<float> d(dens, dens + n);
   std::vector<float> r = orc::c3_shear_rates(p, v, L, d, mass, b.h, b);
   std::memcpy(out, r.data(), n * 4);

@@ -411,7 +411,9 @@ def _c3_lib():
     L.c3_neighbors_of.restype = C.c_int64
     L.c3_neighbors_of.argtypes = [dp, fp, C.c_size_t, u32p, u32p, C.c_int64]
     L.c3_seed_state.argtypes = [dp, fp, fp]
-    L.c3_shear_rates_of.argtypes = [dp, fp, fp, fp, C.c_size_t, C.c_float, fp]
+    L.c3_mean_shear.restype = C.c_double
+    L.c3_mean_shear.argtypes = [P]
+    L.c3_shear_rates_of.argtypes = [dp, fp, fp, fp, C.c_size_t, C.c_float, u32p, u32p, fp]
     L.c3_bingham_of.argtypes = [fp, C.c_size_t, C.c_float, C.c_float, fp]
     L._c3_ready = True
     return L
@@ -445,16 +447,23 @@ def c3_neighbors_of(cfg, points):
     return off, nb[:e].copy()
 
 
-def c3_shear_rates_of(cfg, points, velocities, densities, mass):
+def c3_shear_rates_of(cfg, points, velocities, densities, mass, lists=None):
     """Per-particle shear rates (f32, pbf.rs:821-849) of caller points, velocities and densities
-    over their neighbour lists in the workload's periodic box."""
+    over `lists` (neighbour CSR offsets, entries) or their own neighbour lists in the workload's
+    periodic box."""
     pts = np.ascontiguousarray(points, dtype=np.float32).reshape(-1)
     vel = np.ascontiguousarray(velocities, dtype=np.float32).reshape(-1)
     n = pts.size // 2
     dens = np.ascontiguousarray(densities, dtype=np.float32).reshape(-1)
     assert vel.size == 2 * n and dens.size == n
     out = np.zeros(n, dtype=np.float32)
-    _c3_lib().c3_shear_rates_of(_dp(_c3_arr(cfg)), _fp(pts), _fp(vel), _fp(dens), n, float(mass), _fp(out))
+    off = nb = None
+    if lists is not None:
+        off = np.ascontiguousarray(lists[0], dtype=np.uint32)
+        nb = np.ascontiguousarray(lists[1], dtype=np.uint32)
+        assert off.size == n + 1 and nb.size >= off[-1]
+    _c3_lib().c3_shear_rates_of(_dp(_c3_arr(cfg)), _fp(pts), _fp(vel), _fp(dens), n, float(mass),
+                                None if off is None else _u32(off), None if nb is None else _u32(nb), _fp(out))
     return out
 
 
@@ -493,6 +502,10 @@ class C3Oracle:
         x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
         v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
         self._L.c3_set_state(self._h, _dp(x), _dp(v))
+
+    def mean_shear_rate(self) -> float:
+        """Mean shear rate of the last step (pbf.rs:800-818), 0 before the first step."""
+        return float(self._L.c3_mean_shear(self._h))
 
     def stats(self) -> dict:
         o = np.zeros(3)

@@ -361,3 +361,29 @@ def test_fp64_bingham_matches_oracle(box_mod, orc):
         b.step()
         o.step()
         _compare64(b, o, f"fp64 bingham step={k}")
+
+
+# Mean shear rate of the last step (pbf.rs:800-818): per-particle rates in the reference's pair order,
+# FP64 sums. Without viscosity the velocities are bit-exact, so the f32 means agree to one ulp (the
+# sums differ only in order); after the CG the velocities carry the CG's 1e-6 bar.
+@pytest.mark.parametrize("cfg", [{"side": 48, "mu": 0.0}, {"side": 64, "mu": 1.0e-3},
+                                 {"side": 128, "mu": 1.0e-3, "yield_stress": 1.0},
+                                 {"side": 32, "mu": 1.0e-3, "fp64": 1}])
+def test_mean_shear_rate_matches_oracle(box_mod, orc, cfg):
+    print(f"box mean shear rate {cfg}", flush=True)
+    b = box_mod.Box(cfg)
+    o = orc.C3Oracle(cfg, "fp64" if cfg.get("fp64") else "mixed")
+    assert b.mean_shear_rate() == 0.0
+    for _ in range(2):
+        b.step()
+        o.step()
+    m, mo = b.mean_shear_rate(), o.mean_shear_rate()
+    assert mo > 0.0
+    if cfg.get("fp64"):
+        assert abs(m - mo) <= 1e-9 * mo
+    elif cfg["mu"] == 0.0:
+        assert abs(m - mo) <= float(np.spacing(np.float32(mo)))
+    else:
+        assert abs(m - mo) <= 1e-5 * mo
+    x0, v0 = b.state()
+    assert b.mean_shear_rate() == m and np.array_equal(b.state()[0], x0)  # a read: no state change

@@ -215,3 +215,21 @@ def test_bingham_step_is_more_viscous_and_tends_to_newtonian(orc):
     assert bing.stats()["cg_iterations"] > newt.stats()["cg_iterations"]
     assert (vb * vb).sum() < (vn * vn).sum()
     assert np.abs(vt - vn).max() <= 1e-5 * np.abs(vn).max()
+
+
+def test_mean_shear_rate_is_the_mean_of_the_rates(orc):
+    """The oracle's per-step mean shear rate is the f32 rounding of the f64 mean of the shear rates
+    that the rate function (pinned above) gives for the step's final positions, velocities and
+    densities over the step's own lists (pbf.rs:2303-2304)."""
+    cfg = {"side": 24, "mu": 1.0e-3}
+    o = orc.C3Oracle(cfg, "mixed")
+    assert o.mean_shear_rate() == 0.0
+    for _ in range(2):
+        o.step()
+    x, v = o.state()
+    box = orc.c3_box(cfg)
+    rates = orc.c3_shear_rates_of(cfg, x, v, o.density(), box["mass"], lists=o.neighbors())
+    assert np.all(np.isfinite(rates))
+    m = o.mean_shear_rate()
+    assert m > 0.0
+    assert m == float(np.float32(np.sum(rates.astype(np.float64)) / rates.size))
