@@ -953,7 +953,9 @@ BDEV R shear_rate(const BoxGeo<R>& g, const PairView<R>& pv, uint32_t i, const V
 // Effective viscosity of a Bingham box (pbf.rs:2147-2151): the Papanastasiou-regularised
 // mu + tau_y (1 - exp(-g / g0)) / g of each particle's shear rate g, with g0 = tau_y / (100 mu), its
 // g -> 0 limit below 1e-4 g0 and a non-finite rate taken as 0 (pbf.rs:854-868, :490).
-BXKT bx_shear_mu(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const Vec<R>* __restrict__ V,
+// (FP64: no 8-waves pin, under which its 40 bytes of gathers in flight per neighbour quad spill)
+template <class R> __global__ void __launch_bounds__(BX_BLOCK, BxRWaves<R>::value)
+bx_shear_mu(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const Vec<R>* __restrict__ V,
                  const R* __restrict__ DENS, R* __restrict__ MU, R mu, R tau_y) {
   if (*g.halt) return;
   const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
