@@ -522,12 +522,24 @@ int run_substeps(Batch& B, uint32_t k, const float* dt_override, bool use_graph)
       if (!B.graph_exec || B.graph_k != chunk) {
         if (B.graph_exec) { hipGraphExecDestroy(B.graph_exec); B.graph_exec = nullptr; }
         if (B.graph) { hipGraphDestroy(B.graph); B.graph = nullptr; }
-        hipStream_t cs;
-        hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
-        if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) return MG_E_DEVICE;
-        for (uint32_t s = 0; s < chunk; s++) mg_launch_substep(B.d_records, (int)B.drums.size(), (int)s, cs);
-        if (hipStreamEndCapture(cs, &B.graph) != hipSuccess) return MG_E_DEVICE;
-        hipStreamDestroy(cs);
+        hipStream_t cs = nullptr;
+        if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) return MG_E_DEVICE;
+        if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+          (void)hipStreamDestroy(cs);
+          return MG_E_DEVICE;
+        }
+        bool launched = true;
+        for (uint32_t s = 0; s < chunk; s++)
+          launched &= mg_launch_substep(B.d_records, (int)B.drums.size(), (int)s, cs) == hipSuccess;
+        // End the capture on every path, so the stream never stays in capture mode.
+        hipGraph_t g = nullptr;
+        const bool ended = hipStreamEndCapture(cs, &g) == hipSuccess;
+        (void)hipStreamDestroy(cs);
+        if (!ended || !launched) {
+          if (g) (void)hipGraphDestroy(g);
+          return MG_E_DEVICE;
+        }
+        B.graph = g;
         if (hipGraphInstantiate(&B.graph_exec, B.graph, nullptr, nullptr, 0) != hipSuccess) return MG_E_DEVICE;
         B.graph_k = chunk;
       }

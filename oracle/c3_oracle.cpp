@@ -178,6 +178,150 @@ template <class R> static R c3_bingham(R g, R mu, R tau_y) {
   return mu + extra;
 }
 
+// One density pass over the lists (pbf.rs:1257-1334): density, one-sided C, lambda and the
+// Jacobi displacement dp at positions x (not moved here). gcache holds each edge's spiky gradient,
+// as the reference's per-edge cache does (pbf.rs:1268-1285).
+template <class R>
+static void c3_density_pass(const std::vector<V2<R>>& x, const Nbrs& nbr, const C3Box& b, R mass, R rest,
+                            std::vector<R>& density, std::vector<R>& lambda, std::vector<V2<R>>& dp,
+                            std::vector<V2<R>>& gcache) {
+  const size_t n = x.size();
+  const R h = (R)b.h;
+  density.assign(n, (R)0);
+  lambda.assign(n, (R)0);
+  dp.assign(n, V2<R>());
+  gcache.resize(nbr.nb.size());
+  for (size_t i = 0; i < n; i++) {
+    R rho = mass * poly6((R)0, h);
+    V2<R> gs;
+    R sgs = 0;
+    for (uint32_t k = nbr.off[i]; k < nbr.off[i + 1]; k++) {
+      uint32_t j = nbr.nb[k];
+      V2<R> d(b.mi(x[i].x - x[j].x), b.mi(x[i].y - x[j].y));
+      R r2 = d.len2();
+      rho += mass * poly6(r2, h);
+      R r = d.len();
+      V2<R> rg = spiky_grad(d, r, h);
+      gcache[k] = rg;
+      V2<R> g = rg * (mass / rest);
+      gs += g;
+      sgs += g.len2();
+    }
+    density[i] = rho;
+    R c = rmax(density[i] / rest - F(1.0), (R)0);
+    sgs += gs.len2();
+    lambda[i] = -c / (sgs + F(200.0));
+  }
+  for (size_t i = 0; i < n; i++) {
+    V2<R> s;
+    for (uint32_t k = nbr.off[i]; k < nbr.off[i + 1]; k++) s += (lambda[i] + lambda[nbr.nb[k]]) * gcache[k];
+    dp[i] = s * (mass / rest);
+  }
+}
+
+// Density at positions x over the lists (pbf.rs:2136-2144).
+template <class R>
+static std::vector<R> c3_density_at(const std::vector<V2<R>>& x, const Nbrs& nbr, const C3Box& b, R mass) {
+  const size_t n = x.size();
+  const R h = (R)b.h;
+  std::vector<R> density(n);
+  for (size_t i = 0; i < n; i++) {
+    R rho = mass * poly6((R)0, h);
+    for (uint32_t k = nbr.off[i]; k < nbr.off[i + 1]; k++) {
+      const uint32_t j = nbr.nb[k];
+      rho += mass * poly6(V2<R>(b.mi(x[i].x - x[j].x), b.mi(x[i].y - x[j].y)).len2(), h);
+    }
+    density[i] = rho;
+  }
+  return density;
+}
+
+// Morris weights c_ij and unit directions of every edge (pbf.rs:312-344).
+template <class R>
+static void c3_morris(const std::vector<V2<R>>& x, const Nbrs& nbr, const std::vector<R>& density,
+                      const std::vector<R>& mu_eff, const C3Box& b, R mass, std::vector<R>& W,
+                      std::vector<V2<R>>& D) {
+  const size_t n = x.size();
+  const R h = (R)b.h;
+  const R eta2 = F(0.01) * h * h;
+  W.assign(nbr.nb.size(), (R)0);
+  D.assign(nbr.nb.size(), V2<R>());
+  for (size_t i = 0; i < n; i++) {
+    R rho_i = rmax(density[i], F(1e-6));
+    for (uint32_t k = nbr.off[i]; k < nbr.off[i + 1]; k++) {
+      uint32_t j = nbr.nb[k];
+      R rho_j = rmax(density[j], F(1e-6));
+      V2<R> d(b.mi(x[i].x - x[j].x), b.mi(x[i].y - x[j].y));
+      R r = d.len();
+      V2<R> g = spiky_grad(d, r, h);
+      R num = (mass * (mu_eff[i] + mu_eff[j]) / (rho_i * rho_j)) * std::fabs(d.dot(g));
+      W[k] = num / (d.len2() + eta2);
+      D[k] = r > F(1e-9) ? d / r : V2<R>(1, 0);
+    }
+  }
+}
+
+// Central-force viscous operator (L v)_i = sum_j 4 c_ij e_ij (e_ij . (v_i - v_j)) (pbf.rs:359-377,
+// CENTRAL_VISCOSITY_FACTOR = 4 at :384).
+template <class R>
+static void c3_laplacian(const Nbrs& nbr, const std::vector<R>& W, const std::vector<V2<R>>& D,
+                         const std::vector<V2<R>>& vv, std::vector<V2<R>>& out) {
+  const size_t n = vv.size();
+  out.resize(n);
+  for (size_t i = 0; i < n; i++) {
+    V2<R> s;
+    for (uint32_t k = nbr.off[i]; k < nbr.off[i + 1]; k++) {
+      V2<R> e = D[k];
+      s += (F(4.0) * W[k] * e.dot(vv[i] - vv[nbr.nb[k]])) * e;
+    }
+    out[i] = s;
+  }
+}
+
+template <class R, class A> static A c3_dotv(const std::vector<V2<R>>& a, const std::vector<V2<R>>& b) {  // pbf.rs:292-294
+  A s = 0;
+  for (size_t i = 0; i < a.size(); i++) s += (A)a[i].x * (A)b[i].x + (A)a[i].y * (A)b[i].y;
+  return s;
+}
+
+// solve_implicit_viscosity (pbf.rs:423-480): CG on (I + dt L) v = v with warm start v, stopping
+// at tol * |b| after at most max_cg iterations; dots and norms in A. Returns the iterations run;
+// res and bnorm receive the last residual norm and |b|.
+template <class R, class A>
+static uint32_t c3_cg(std::vector<V2<R>>& v, const Nbrs& nbr, const std::vector<R>& W, const std::vector<V2<R>>& D,
+                      R dt, float tol, uint32_t max_cg, A& res, A& bnorm) {
+  const size_t n = v.size();
+  A bn = std::sqrt(c3_dotv<R, A>(v, v));
+  bnorm = bn;
+  res = 0;
+  if (bn <= (A)0 || !std::isfinite(bn)) return 0;
+  A target = (A)tol * bn;
+  std::vector<V2<R>> lv(n), r(n), p(n), ap(n);
+  c3_laplacian(nbr, W, D, v, lv);
+  for (size_t i = 0; i < n; i++) r[i] = v[i] - (v[i] + lv[i] * dt);
+  p = r;
+  A rs_old = c3_dotv<R, A>(r, r);
+  res = std::sqrt(rs_old);
+  if (std::sqrt(rs_old) <= target) return 0;
+  uint32_t iters = 0;
+  for (uint32_t it = 1; it <= max_cg; it++) {
+    iters = it;
+    c3_laplacian(nbr, W, D, p, lv);
+    for (size_t i = 0; i < n; i++) ap[i] = p[i] + lv[i] * dt;
+    A pap = c3_dotv<R, A>(p, ap);
+    if (pap <= (A)0 || !std::isfinite(pap)) return iters;
+    R alpha = (R)(rs_old / pap);
+    for (size_t i = 0; i < n; i++) { v[i] += p[i] * alpha; r[i] -= ap[i] * alpha; }
+    A rs_new = c3_dotv<R, A>(r, r);
+    res = std::sqrt(rs_new);
+    if (std::sqrt(rs_new) <= target) return iters;
+    R bta = (R)(rs_new / rs_old);
+    for (size_t i = 0; i < n; i++) p[i] = r[i] + p[i] * bta;
+    rs_old = rs_new;
+  }
+  return iters;
+}
+
 template <class R, class A> struct C3Sim {
   C3Config cfg;
   C3Box box;
@@ -202,12 +346,6 @@ template <class R, class A> struct C3Sim {
   }
   V2<R> mi(V2<R> d) const { return V2<R>(box.mi(d.x), box.mi(d.y)); }
   V2<R> wrap(V2<R> p) const { return V2<R>(box.wrap(p.x), box.wrap(p.y)); }
-  static A adot(V2<R> a, V2<R> b) { return (A)a.x * (A)b.x + (A)a.y * (A)b.y; }
-  static A dotv(const std::vector<V2<R>>& a, const std::vector<V2<R>>& b) {  // pbf.rs:292-294
-    A s = 0;
-    for (size_t i = 0; i < a.size(); i++) s += adot(a[i], b[i]);
-    return s;
-  }
 
   // Mean shear rate of the last step (pbf.rs:800-818 at :2303-2304: final positions, the step's
   // velocities, lists and recomputed densities); the sum in the accumulation type A.
@@ -230,45 +368,16 @@ template <class R, class A> struct C3Sim {
     // neighbour lists at the predicted positions (pbf.rs:1238-1239)
     nbr = c3_neighbors(x, box);
     // density passes (pbf.rs:1257-1334), exactly cfg.passes
-    std::vector<R> density(n, 0), lambda(n, 0);
-    std::vector<V2<R>> dp(n), gcache(nbr.nb.size());
+    std::vector<R> density, lambda;
+    std::vector<V2<R>> dp, gcache;
     for (uint32_t pass = 0; pass < cfg.passes; pass++) {
-      for (size_t i = 0; i < n; i++) {
-        R rho = mass * poly6((R)0, h);
-        V2<R> gs;
-        R sgs = 0;
-        for (uint32_t k = nbr.off[i]; k < nbr.off[i + 1]; k++) {
-          uint32_t j = nbr.nb[k];
-          V2<R> d = mi(x[i] - x[j]);
-          R r2 = d.len2();
-          rho += mass * poly6(r2, h);
-          R r = d.len();
-          V2<R> rg = spiky_grad(d, r, h);
-          gcache[k] = rg;
-          V2<R> g = rg * (mass / rest);
-          gs += g;
-          sgs += g.len2();
-        }
-        density[i] = rho;
-        R c = rmax(density[i] / rest - F(1.0), (R)0);
-        sgs += gs.len2();
-        lambda[i] = -c / (sgs + F(200.0));
-      }
-      for (size_t i = 0; i < n; i++) {
-        V2<R> s;
-        for (uint32_t k = nbr.off[i]; k < nbr.off[i + 1]; k++) s += (lambda[i] + lambda[nbr.nb[k]]) * gcache[k];
-        dp[i] = s * (mass / rest);
-      }
+      c3_density_pass(x, nbr, box, mass, rest, density, lambda, dp, gcache);
       for (size_t i = 0; i < n; i++) x[i] = wrap(x[i] + dp[i]);
     }
     // velocity (pbf.rs:1656-1674) with the minimum-image displacement
     for (size_t i = 0; i < n; i++) v[i] = mi(x[i] - x0[i]) / dt;
     // density recompute (pbf.rs:2136-2144)
-    for (size_t i = 0; i < n; i++) {
-      R rho = mass * poly6((R)0, h);
-      for (uint32_t k = nbr.off[i]; k < nbr.off[i + 1]; k++) rho += mass * poly6(mi(x[i] - x[nbr.nb[k]]).len2(), h);
-      density[i] = rho;
-    }
+    density = c3_density_at(x, nbr, box, mass);
     dens = density;
     last_cg = 0;
     if (!(mu > (R)0)) return;
@@ -279,61 +388,10 @@ template <class R, class A> struct C3Sim {
       const std::vector<R> rates = c3_shear_rates(x, v, nbr, density, mass, h, box);
       for (size_t i = 0; i < n; i++) mu_eff[i] = c3_bingham(rates[i], mu, (R)cfg.yield_stress);
     }
-    // Morris weights (pbf.rs:312-344)
-    const R eta2 = F(0.01) * h * h;
-    std::vector<R> W(nbr.nb.size());
-    std::vector<V2<R>> D(nbr.nb.size());
-    for (size_t i = 0; i < n; i++) {
-      R rho_i = rmax(density[i], F(1e-6));
-      for (uint32_t k = nbr.off[i]; k < nbr.off[i + 1]; k++) {
-        uint32_t j = nbr.nb[k];
-        R rho_j = rmax(density[j], F(1e-6));
-        V2<R> d = mi(x[i] - x[j]);
-        R r = d.len();
-        V2<R> g = spiky_grad(d, r, h);
-        R num = (mass * (mu_eff[i] + mu_eff[j]) / (rho_i * rho_j)) * std::fabs(d.dot(g));
-        W[k] = num / (d.len2() + eta2);
-        D[k] = r > F(1e-9) ? d / r : V2<R>(1, 0);
-      }
-    }
-    auto lap = [&](const std::vector<V2<R>>& vv, std::vector<V2<R>>& out) {  // pbf.rs:359-377
-      for (size_t i = 0; i < n; i++) {
-        V2<R> s;
-        for (uint32_t k = nbr.off[i]; k < nbr.off[i + 1]; k++) {
-          V2<R> e = D[k];
-          s += (F(4.0) * W[k] * e.dot(vv[i] - vv[nbr.nb[k]])) * e;
-        }
-        out[i] = s;
-      }
-    };
-    // solve_implicit_viscosity (pbf.rs:423-480) with tol and max_cg from the workload
-    A bn = std::sqrt(dotv(v, v));
-    last_bnorm = bn;
-    last_res = 0;
-    if (bn <= (A)0 || !std::isfinite(bn)) return;
-    A target = (A)cfg.tol * bn;
-    std::vector<V2<R>> lv(n), r(n), p(n), ap(n);
-    lap(v, lv);
-    for (size_t i = 0; i < n; i++) r[i] = v[i] - (v[i] + lv[i] * dt);
-    p = r;
-    A rs_old = dotv(r, r);
-    last_res = std::sqrt(rs_old);
-    if (std::sqrt(rs_old) <= target) return;
-    for (uint32_t it = 1; it <= cfg.max_cg; it++) {
-      last_cg = it;
-      lap(p, lv);
-      for (size_t i = 0; i < n; i++) ap[i] = p[i] + lv[i] * dt;
-      A pap = dotv(p, ap);
-      if (pap <= (A)0 || !std::isfinite(pap)) return;
-      R alpha = (R)(rs_old / pap);
-      for (size_t i = 0; i < n; i++) { v[i] += p[i] * alpha; r[i] -= ap[i] * alpha; }
-      A rs_new = dotv(r, r);
-      last_res = std::sqrt(rs_new);
-      if (std::sqrt(rs_new) <= target) return;
-      R bta = (R)(rs_new / rs_old);
-      for (size_t i = 0; i < n; i++) p[i] = r[i] + p[i] * bta;
-      rs_old = rs_new;
-    }
+    std::vector<R> W;
+    std::vector<V2<R>> D;
+    c3_morris(x, nbr, density, mu_eff, box, mass, W, D);
+    last_cg = c3_cg<R, A>(v, nbr, W, D, dt, cfg.tol, cfg.max_cg, last_res, last_bnorm);
   }
 };
 
@@ -479,5 +537,82 @@ void c3_seed_state(const double* cfg, float* x2n, float* v2n) {
   orc::c3_seed(c, b, x, v);
   std::memcpy(x2n, x.data(), x.size() * 4);
   std::memcpy(v2n, v.data(), v.size() * 4);
+}
+}
+
+// ---- stage hooks over caller points (any count) in the box of cfg, for the oracle's own pins.
+// mode 0: f32 arithmetic (f32 dots), 2: f64. Points and velocities are given as doubles.
+template <class R> static std::vector<orc::V2<R>> c3_pts(const double* p, size_t n) {
+  std::vector<orc::V2<R>> out(n);
+  for (size_t i = 0; i < n; i++) out[i] = orc::V2<R>((R)p[2 * i], (R)p[2 * i + 1]);
+  return out;
+}
+template <class R>
+static void c3_pass_of_t(const double* cfg, const double* pts, size_t n, double* dens, double* lam, double* dp) {
+  orc::C3Config c = c3_cfg_from(cfg);
+  orc::C3Box b = orc::C3Box::make(c);
+  std::vector<orc::V2<R>> x = c3_pts<R>(pts, n);
+  orc::Nbrs L = orc::c3_neighbors(x, b);
+  std::vector<R> d, l;
+  std::vector<orc::V2<R>> p, g;
+  orc::c3_density_pass(x, L, b, (R)b.mass, (R)c.rest, d, l, p, g);
+  for (size_t i = 0; i < n; i++) {
+    dens[i] = (double)d[i];
+    lam[i] = (double)l[i];
+    dp[2 * i] = (double)p[i].x;
+    dp[2 * i + 1] = (double)p[i].y;
+  }
+}
+// Density at the points, Morris weights with mu everywhere, then either the operator applied to
+// vel (solve == 0: out = L vel) or the implicit viscosity CG from vel (solve != 0: out = v after
+// the CG with cfg's dt, tol and max_cg; stats3 = iterations, residual norm, |b|).
+template <class R>
+static void c3_visc_of_t(const double* cfg, const double* pts, const double* vel, size_t n, int solve, double* out,
+                         double* stats3) {
+  orc::C3Config c = c3_cfg_from(cfg);
+  orc::C3Box b = orc::C3Box::make(c);
+  std::vector<orc::V2<R>> x = c3_pts<R>(pts, n), v = c3_pts<R>(vel, n), o;
+  orc::Nbrs L = orc::c3_neighbors(x, b);
+  std::vector<R> dens = orc::c3_density_at(x, L, b, (R)b.mass);
+  std::vector<R> mu(n, (R)c.mu), W;
+  std::vector<orc::V2<R>> D;
+  orc::c3_morris(x, L, dens, mu, b, (R)b.mass, W, D);
+  if (solve) {
+    R res = 0, bn = 0;
+    uint32_t it = orc::c3_cg<R, R>(v, L, W, D, (R)c.dt, c.tol, c.max_cg, res, bn);
+    o = v;
+    if (stats3) { stats3[0] = it; stats3[1] = (double)res; stats3[2] = (double)bn; }
+  } else {
+    orc::c3_laplacian(L, W, D, v, o);
+  }
+  for (size_t i = 0; i < n; i++) { out[2 * i] = (double)o[i].x; out[2 * i + 1] = (double)o[i].y; }
+}
+extern "C" {
+// One density pass (pbf.rs:1257-1334) over caller points: density, lambda, dp (2n).
+void c3_pass_of(const double* cfg, int mode, const double* pts, size_t n, double* dens, double* lam, double* dp) {
+  if (mode == 2) c3_pass_of_t<double>(cfg, pts, n, dens, lam, dp);
+  else c3_pass_of_t<float>(cfg, pts, n, dens, lam, dp);
+}
+void c3_visc_of(const double* cfg, int mode, const double* pts, const double* vel, size_t n, int solve, double* out,
+                double* stats3) {
+  if (mode == 2) c3_visc_of_t<double>(cfg, pts, vel, n, solve, out, stats3);
+  else c3_visc_of_t<float>(cfg, pts, vel, n, solve, out, stats3);
+}
+// Density at caller points (pbf.rs:2136-2144) and the Morris weights of their edges in list order
+// (pbf.rs:312-344, mu everywhere); returns the edge count, copying weights only when they fit.
+int64_t c3_weights_of(const double* cfg, const double* pts, size_t n, double* dens, uint32_t* off, uint32_t* nb,
+                      double* w, int64_t cap) {
+  orc::C3Config c = c3_cfg_from(cfg);
+  orc::C3Box b = orc::C3Box::make(c);
+  std::vector<orc::V2<double>> x = c3_pts<double>(pts, n);
+  orc::Nbrs L = orc::c3_neighbors(x, b);
+  std::vector<double> d = orc::c3_density_at(x, L, b, (double)b.mass), mu(n, (double)c.mu), W;
+  std::vector<orc::V2<double>> D;
+  orc::c3_morris(x, L, d, mu, b, (double)b.mass, W, D);
+  std::memcpy(dens, d.data(), n * 8);
+  std::memcpy(off, L.off.data(), (n + 1) * 4);
+  int64_t e = (int64_t)L.nb.size();
+  if (e <= cap) { std::memcpy(nb, L.nb.data(), (size_t)e * 4); std::memcpy(w, W.data(), (size_t)e * 8); }
+  return e;
 }
 }

@@ -415,6 +415,10 @@ def _c3_lib():
     L.c3_mean_shear.argtypes = [P]
     L.c3_shear_rates_of.argtypes = [dp, fp, fp, fp, C.c_size_t, C.c_float, u32p, u32p, fp]
     L.c3_bingham_of.argtypes = [fp, C.c_size_t, C.c_float, C.c_float, fp]
+    L.c3_pass_of.argtypes = [dp, C.c_int, dp, C.c_size_t, dp, dp, dp]
+    L.c3_visc_of.argtypes = [dp, C.c_int, dp, dp, C.c_size_t, C.c_int, dp, dp]
+    L.c3_weights_of.restype = C.c_int64
+    L.c3_weights_of.argtypes = [dp, dp, C.c_size_t, dp, u32p, u32p, dp, C.c_int64]
     L._c3_ready = True
     return L
 
@@ -473,6 +477,52 @@ def c3_bingham_of(rates, mu, tau_y):
     out = np.zeros(r.size, dtype=np.float32)
     _c3_lib().c3_bingham_of(_fp(r), r.size, float(mu), float(tau_y), _fp(out))
     return out
+
+
+def _pts64(points):
+    return np.ascontiguousarray(points, dtype=np.float64).reshape(-1)
+
+
+def c3_pass_of(cfg, points, mode: str = "fp64"):
+    """One density pass (pbf.rs:1257-1334) over caller points in the workload's box: densities,
+    lambdas and the Jacobi displacements (positions are not moved). mode "ref" (f32) or "fp64"."""
+    pts = _pts64(points)
+    n = pts.size // 2
+    dens, lam, dp = np.zeros(n), np.zeros(n), np.zeros(2 * n)
+    _c3_lib().c3_pass_of(_dp(_c3_arr(cfg)), 2 if mode == "fp64" else 0, _dp(pts), n, _dp(dens), _dp(lam), _dp(dp))
+    return dens, lam, dp.reshape(n, 2)
+
+
+def c3_viscous_of(cfg, points, velocities, solve: bool, mode: str = "fp64"):
+    """Over caller points with cfg's mu everywhere: the central-force viscous operator applied to
+    the velocities (solve=False, pbf.rs:359-377) or the velocities after the implicit viscosity CG
+    with cfg's dt, tol and max_cg (solve=True, pbf.rs:423-480; also returns iterations, residual
+    norm and |b|). Densities are those at the points (pbf.rs:2136-2144)."""
+    pts = _pts64(points)
+    vel = _pts64(velocities)
+    n = pts.size // 2
+    out = np.zeros(2 * n)
+    st = np.zeros(3)
+    _c3_lib().c3_visc_of(_dp(_c3_arr(cfg)), 2 if mode == "fp64" else 0, _dp(pts), _dp(vel), n, 1 if solve else 0,
+                         _dp(out), _dp(st))
+    if solve:
+        return out.reshape(n, 2), {"cg_iterations": int(st[0]), "residual": st[1], "b_norm": st[2]}
+    return out.reshape(n, 2)
+
+
+def c3_weights_of(cfg, points):
+    """FP64 densities at caller points and the Morris weight of every edge of their neighbour
+    lists (pbf.rs:312-344): (density, offsets, neighbours, weights)."""
+    pts = _pts64(points)
+    n = pts.size // 2
+    dens = np.zeros(n)
+    off = np.zeros(n + 1, dtype=np.uint32)
+    cap = 64 * n + 64
+    nb = np.zeros(cap, dtype=np.uint32)
+    w = np.zeros(cap)
+    e = _c3_lib().c3_weights_of(_dp(_c3_arr(cfg)), _dp(pts), n, _dp(dens), _u32(off), _u32(nb), _dp(w), cap)
+    assert e <= cap
+    return dens, off, nb[:e].copy(), w[:e].copy()
 
 
 class C3Oracle:
