@@ -208,6 +208,56 @@ int mg_box_kernel_times(mg_box* box, double* ms, uint64_t* launches, int nclass)
  * measurement hook. */
 int mg_box_replay_ms(mg_box* box, uint32_t cls, uint32_t reps, double* ms_per_launch);
 
+/* ---- Uniform grid in the reference's layout (grid.rs) --------------------------------------
+ * A grid handle owns the device workspace and outputs of builds over up to max_points points.
+ * Points are 2n floats [x0, y0, x1, y1, ...] in device memory; `stream` is a hipStream_t (NULL:
+ * the null stream). A build waits once for the layout decision; the queries wait for their
+ * totals. Outputs stay valid until the next build on the same handle. */
+typedef struct mg_grid mg_grid;
+int mg_grid_create(uint32_t max_points, mg_grid** out, char* err, size_t errlen);
+void mg_grid_destroy(mg_grid* grid);
+/* Replaces UniformGrid::build (grid.rs:101-153): cell keys (floor(x / cell) as i32, :280-285) of
+ * the finite points, the Dense layout (x-major k = (cx - min_cx) * height + (cy - min_cy),
+ * :232-236) when width * height <= 2 m + 512, else the Sparse layout (:204-226); points of a cell
+ * in ascending index. MG_E_ARG for a non-positive or non-finite cell_size (the reference asserts,
+ * grid.rs:102-105) or more points than the handle's capacity. */
+int mg_grid_build(mg_grid* grid, const float* d_xy, uint32_t n, float cell_size, void* stream);
+/* dense, min_cx, min_cy, width, height, cell_start entries, index entries (finite points), n. */
+int mg_grid_meta(const mg_grid* grid, int64_t* meta8);
+/* Host copies of cell_start, indices and (Sparse) the (cx, cy) keys of the occupied cells; any
+ * pointer may be NULL. */
+int mg_grid_copy(mg_grid* grid, uint32_t* cell_start, size_t cs_cap, uint32_t* indices, size_t idx_cap,
+                 int32_t* keys2, size_t keys_cap);
+/* Device views of the same arrays (valid until the next build). */
+const uint32_t* mg_grid_cell_start_device(const mg_grid* grid);
+const uint32_t* mg_grid_indices_device(const mg_grid* grid);
+const int32_t* mg_grid_keys_device(const mg_grid* grid);
+/* Replaces UniformGrid::for_each_candidate_pair (grid.rs:291-362) as a list: pairs (i, j), i < j,
+ * in the reference's emission order into d_pairs2 (2 x cap u32, device); *n_pairs = all pairs
+ * (those at or beyond cap are not written). */
+int mg_grid_candidate_pairs(mg_grid* grid, uint32_t* d_pairs2, uint64_t cap, uint64_t* n_pairs, void* stream);
+/* Replaces build_neighbor_lists (pbf.rs:262-288) over the points the grid was built from (same
+ * d_xy): per point, the partners with |x_i - x_j|^2 <= h^2 in the order the reference appends
+ * them. d_offsets: n + 1 u32 (device); d_nbrs: cap u32 (device); *n_edges = all entries. */
+int mg_grid_neighbor_lists(mg_grid* grid, const float* d_xy, float h, uint32_t* d_offsets, uint32_t* d_nbrs,
+                           uint64_t cap, uint64_t* n_edges, void* stream);
+/* Replaces UniformGrid::for_each_near (grid.rs:371-399) for nq query points: CSR of the candidate
+ * indices in visiting order. d_offsets: nq + 1 u32 (device). */
+int mg_grid_near(mg_grid* grid, const float* d_queries, uint32_t nq, uint32_t* d_offsets, uint32_t* d_out,
+                 uint64_t cap, uint64_t* n_out, void* stream);
+/* Message of the handle's last failure. */
+const char* mg_grid_last_error(const mg_grid* grid);
+
+/* ---- device memory of this library's HIP runtime ----------------------------------------------
+ * No reference counterpart: plumbing for callers without a HIP runtime of their own (bindings,
+ * tests). Device pointers of another HIP runtime loaded into the same process (for example one
+ * bundled with a framework) are not valid arguments of this library. */
+int mg_device_malloc(size_t bytes, void** out);
+int mg_device_free(void* ptr);
+int mg_memcpy_htod(void* dst, const void* src, size_t bytes);
+int mg_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+int mg_device_synchronize(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -2057,6 +2057,37 @@ int orc_grid_build(const float* pts, size_t n, float cell, int64_t* meta, uint32
   std::memcpy(indices, g.indices.data(), g.indices.size() * 4);
   return 1;
 }
+// Sparse-layout keys (cx, cy) of the occupied cells, in cell order (grid.rs:204-226); returns
+// their count (0 for the Dense layout), copying them when they fit.
+int64_t orc_grid_keys(const float* pts, size_t n, float cell, int32_t* keys2, int64_t cap) {
+  std::vector<V2<float>> p(n);
+  for (size_t i = 0; i < n; i++) p[i] = V2<float>(pts[2 * i], pts[2 * i + 1]);
+  Grid<float> g;
+  g.build(p, cell);
+  const int64_t k = (int64_t)g.keys.size();
+  if (k <= cap)
+    for (int64_t i = 0; i < k; i++) { keys2[2 * i] = g.keys[i].first; keys2[2 * i + 1] = g.keys[i].second; }
+  return k;
+}
+// for_each_near (grid.rs:371-399) of nq query points over the grid of pts: CSR of the candidates
+// in visiting order; returns the total, copying the entries when they fit.
+int64_t orc_grid_near(const float* pts, size_t n, float cell, const float* q, size_t nq, uint32_t* off, uint32_t* out,
+                      int64_t cap) {
+  std::vector<V2<float>> p(n);
+  for (size_t i = 0; i < n; i++) p[i] = V2<float>(pts[2 * i], pts[2 * i + 1]);
+  Grid<float> g;
+  g.build(p, cell);
+  int64_t k = 0;
+  off[0] = 0;
+  for (size_t a = 0; a < nq; a++) {
+    g.near(V2<float>(q[2 * a], q[2 * a + 1]), [&](uint32_t j) {
+      if (k < cap) out[k] = j;
+      k++;
+    });
+    off[a + 1] = (uint32_t)k;
+  }
+  return k;
+}
 // Candidate pairs in emission order (grid.rs:291-362).
 int64_t orc_grid_pairs(const float* pts, size_t n, float cell, uint32_t* out, int64_t cap) {
   std::vector<V2<float>> p(n);

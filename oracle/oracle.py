@@ -60,6 +60,10 @@ def oracle_lib():
     lib.orc_grid_build.argtypes = [fp, C.c_size_t, C.c_float, i64p, u32p, C.c_int, u32p, C.c_int]
     lib.orc_grid_pairs.restype = C.c_int64
     lib.orc_grid_pairs.argtypes = [fp, C.c_size_t, C.c_float, u32p, C.c_int64]
+    lib.orc_grid_keys.restype = C.c_int64
+    lib.orc_grid_keys.argtypes = [fp, C.c_size_t, C.c_float, C.POINTER(C.c_int32), C.c_int64]
+    lib.orc_grid_near.restype = C.c_int64
+    lib.orc_grid_near.argtypes = [fp, C.c_size_t, C.c_float, fp, C.c_size_t, u32p, u32p, C.c_int64]
     lib.orc_neighbor_lists.restype = C.c_int
     lib.orc_neighbor_lists.argtypes = [fp, C.c_size_t, C.c_float, u32p, u32p, C.c_int64, i64p]
     lib.orc_rng.argtypes = [C.c_uint64, C.POINTER(C.c_uint64), fp, C.c_int]
@@ -303,6 +307,29 @@ def grid_build(points, cell: float):
     return {"dense": bool(meta[0]), "min_cx": int(meta[1]), "min_cy": int(meta[2]),
             "width": int(meta[3]), "height": int(meta[4]),
             "cell_start": cs[: meta[5]].copy(), "indices": idx[: meta[6]].copy()}
+
+
+def grid_keys(points, cell: float) -> np.ndarray:
+    """Sparse-layout cell keys (cx, cy) in cell order (grid.rs:204-226); empty for a Dense grid."""
+    lib = oracle_lib()
+    pts = np.ascontiguousarray(np.asarray(points, dtype=np.float32).reshape(-1))
+    n = pts.size // 2
+    out = np.zeros(2 * n + 2, dtype=np.int32)
+    k = lib.orc_grid_keys(_fp(pts), n, cell, out.ctypes.data_as(C.POINTER(C.c_int32)), n + 1)
+    return out[: 2 * k].reshape(-1, 2)
+
+
+def grid_near(points, cell: float, queries):
+    """for_each_near (grid.rs:371-399) of each query point: (offsets, candidate indices)."""
+    lib = oracle_lib()
+    pts = np.ascontiguousarray(np.asarray(points, dtype=np.float32).reshape(-1))
+    q = np.ascontiguousarray(np.asarray(queries, dtype=np.float32).reshape(-1))
+    n, nq = pts.size // 2, q.size // 2
+    off = np.zeros(nq + 1, dtype=np.uint32)
+    k = lib.orc_grid_near(_fp(pts), n, cell, _fp(q), nq, _u32(off), None, 0)
+    out = np.zeros(max(k, 1), dtype=np.uint32)
+    lib.orc_grid_near(_fp(pts), n, cell, _fp(q), nq, _u32(off), _u32(out), k)
+    return off, out[:k]
 
 
 def grid_pairs(points, cell: float) -> np.ndarray:
