@@ -247,6 +247,19 @@ int mg_grid_near(mg_grid* grid, const float* d_queries, uint32_t nq, uint32_t* d
                  uint64_t cap, uint64_t* n_out, void* stream);
 /* Message of the handle's last failure. */
 const char* mg_grid_last_error(const mg_grid* grid);
+/* No counterpart in the reference: mg_grid_build for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). It
+ * allocates nothing: its scratch is one fixed device array (about 33 MiB on each GPU it runs on,
+ * held until the process ends) sized for at most 1,048,576 points (more: MG_E_ARG). The caller
+ * passes the points and the outputs in device memory: cell_start (cs_cap entries; a Dense grid
+ * needs width * height + 1 <= 2 m + 513, a Sparse one at most m + 1), indices (idx_cap >= m, the
+ * finite points) and keys2 (keys_cap int32, 2 per occupied cell; a Sparse grid may need 2 m).
+ * Too small an output: MG_E_CAPACITY before any output is written. meta8 (host) receives what
+ * mg_grid_meta reports. Every call on one GPU shares that GPU's scratch, so calls that overlap
+ * on one GPU race. Waits for `stream` before returning. */
+int mg_grid_build_noalloc(const float* d_xy, uint32_t n, float cell_size, int64_t* meta8, uint32_t* d_cell_start,
+                          uint64_t cs_cap, uint32_t* d_indices, uint64_t idx_cap, int32_t* d_keys2, uint64_t keys_cap,
+                          void* stream, char* err, size_t errlen);
 
 /* ---- device memory of this library's HIP runtime ----------------------------------------------
  * No reference counterpart: plumbing for callers without a HIP runtime of their own (bindings,

@@ -30,6 +30,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "../../include/millgpu.h"
@@ -396,6 +397,10 @@ GRK gr_set_u32(uint32_t* __restrict__ p, uint32_t v) {
 GRK gr_copy_u32(uint32_t* __restrict__ dst, const uint32_t* __restrict__ src) {
   if (threadIdx.x == 0) *dst = *src;
 }
+GRK gr_copy_n(uint32_t* __restrict__ dst, const uint32_t* __restrict__ src, uint32_t n) {
+  uint32_t i = blockIdx.x * GR_BLOCK + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
 // The layout the host chose from the bounding box (kernel arguments, so nothing is read from
 // host memory after the launch returns).
 GRK gr_set_layout(GrCtrl* __restrict__ c, uint32_t dense, int64_t H, uint32_t lo_bits, uint32_t ncells) {
@@ -680,14 +685,22 @@ bool scan_u32(const GrBufs& b, const uint32_t* v, uint32_t n, uint32_t* out, hip
   return last() == hipSuccess;
 }
 
+// Stable LSD radix sort of (keyA, idxA)[0..m) on the low `bits` bits of the key; the sorted
+// indices end in idx_final (the last pass scatters straight into it), the sorted keys in key_out.
 template <class K>
-bool radix_sort(const GrBufs& b, uint32_t m, uint32_t bits, K*& key_out, uint32_t*& idx_out, hipStream_t s) {
+bool radix_sort(const GrBufs& b, uint32_t m, uint32_t bits, K*& key_out, uint32_t* idx_final, hipStream_t s) {
   K* kin = static_cast<K*>(b.keyA);
   K* kout = static_cast<K*>(b.keyB);
   uint32_t* iin = b.idxA;
   uint32_t* iout = b.idxB;
   const uint32_t nt = std::max(tiles_of(m), 1u);
+  if (bits == 0) {  // one cell: index order is the cell order
+    hipLaunchKernelGGL(gr_copy_n, dim3(std::max(blocks_of(m), 1u)), dim3(GR_BLOCK), 0, s, idx_final, iin, m);
+    key_out = kin;
+    return last() == hipSuccess;
+  }
   for (uint32_t shift = 0; shift < bits; shift += GR_RADIX_BITS) {
+    if (shift + GR_RADIX_BITS >= bits) iout = idx_final;
     if (sizeof(K) == 4) {
       hipLaunchKernelGGL(gr_radix_hist32, dim3(nt), dim3(GR_BLOCK), 0, s, (const uint32_t*)kin, m, shift, b.hist, nt);
     } else {
@@ -705,7 +718,6 @@ bool radix_sort(const GrBufs& b, uint32_t m, uint32_t bits, K*& key_out, uint32_
     std::swap(iin, iout);
   }
   key_out = kin;
-  idx_out = iin;
   return last() == hipSuccess;
 }
 
@@ -716,9 +728,16 @@ void set_err(char* err, size_t len, const std::string& m) {
   }
 }
 
+// Entries the output buffers of a build hold (cell_start, indices, Sparse keys).
+struct OutCaps {
+  uint64_t cs = UINT64_MAX, idx = UINT64_MAX, keys = UINT64_MAX;
+};
+
 // UniformGrid::build over n device points with buffers b (capacity >= n). Fills the host-side
-// summary of g.
-int build_into(mg_grid& g, const GrBufs& b, const float2* xy, uint32_t n, float cell, hipStream_t s) {
+// summary of g. Returns MG_E_CAPACITY, before any output is written, when the layout needs more
+// entries than `caps` allows.
+int build_into(mg_grid& g, const GrBufs& b, const float2* xy, uint32_t n, float cell, hipStream_t s,
+               OutCaps caps = OutCaps()) {
   if (!(cell > 0.0f) || !std::isfinite(cell)) {
     g.err = "cell_size must be positive";
     return MG_E_ARG;
@@ -739,6 +758,7 @@ int build_into(mg_grid& g, const GrBufs& b, const float2* xy, uint32_t n, float 
   g.m = hc.m;
   const uint32_t m = hc.m;
   if (m == 0) {  // grid.rs:131-139: the empty Sparse layout, cell_start = [0]
+    if (caps.cs < 1) { g.err = "cell_start buffer too small (1 entry needed)"; return MG_E_CAPACITY; }
     g.dense = false; g.W = g.H = 0; g.mnx = g.mny = 0; g.ncells = 0;
     hipLaunchKernelGGL(gr_set_u32, dim3(1), dim3(64), 0, s, b.cell_start, 0u);
   } else {
@@ -747,6 +767,12 @@ int build_into(mg_grid& g, const GrBufs& b, const float2* xy, uint32_t n, float 
     const uint64_t total = tc > (unsigned __int128)UINT64_MAX ? UINT64_MAX : (uint64_t)tc;
     const uint64_t budget = 2ull * m + 512ull;
     g.dense = total <= budget;
+    const uint64_t need_cs = g.dense ? total + 1 : (uint64_t)m + 1;
+    if (caps.cs < need_cs || caps.idx < m || (!g.dense && caps.keys < m)) {
+      g.err = "output buffers too small for this grid (cell_start " + std::to_string(need_cs) + ", indices " +
+              std::to_string(m) + (g.dense ? std::string() : ", keys " + std::to_string(m)) + " entries needed)";
+      return MG_E_CAPACITY;
+    }
     g.W = W; g.H = H; g.mnx = hc.mnx; g.mny = hc.mny;
     const uint32_t lo_bits = g.dense ? 0u : bitlen((uint64_t)(H - 1));
     hipLaunchKernelGGL(gr_set_layout, dim3(1), dim3(64), 0, s, b.ctrl, g.dense ? 1u : 0u, H, lo_bits,
@@ -758,9 +784,10 @@ int build_into(mg_grid& g, const GrBufs& b, const float2* xy, uint32_t n, float 
       hipLaunchKernelGGL(gr_sort_keys32, dim3(nt), dim3(GR_BLOCK), 0, s, xy, n, cell, b.ctrl, b.tile_off, compact,
                          (uint32_t*)b.keyA, b.idxA);
       uint32_t* sk;
-      uint32_t* si;
-      if (!radix_sort<uint32_t>(b, m, bitlen(total - 1), sk, si, s)) { g.err = "HIP error in the grid sort"; return MG_E_DEVICE; }
-      hipMemcpyAsync(b.indices, si, (size_t)m * 4, hipMemcpyDeviceToDevice, s);
+      if (!radix_sort<uint32_t>(b, m, bitlen(total - 1), sk, b.indices, s)) {
+        g.err = "HIP error in the grid sort";
+        return MG_E_DEVICE;
+      }
       hipLaunchKernelGGL(gr_dense_starts, dim3(blocks_of((uint64_t)m + 1)), dim3(GR_BLOCK), 0, s, sk, m, (uint32_t)total,
                          b.cell_start);
     } else {
@@ -768,9 +795,10 @@ int build_into(mg_grid& g, const GrBufs& b, const float2* xy, uint32_t n, float 
       hipLaunchKernelGGL(gr_sort_keys64, dim3(nt), dim3(GR_BLOCK), 0, s, xy, n, cell, b.ctrl, b.tile_off, compact,
                          (uint64_t*)b.keyA, b.idxA);
       uint64_t* sk;
-      uint32_t* si;
-      if (!radix_sort<uint64_t>(b, m, lo + hi, sk, si, s)) { g.err = "HIP error in the grid sort"; return MG_E_DEVICE; }
-      hipMemcpyAsync(b.indices, si, (size_t)m * 4, hipMemcpyDeviceToDevice, s);
+      if (!radix_sort<uint64_t>(b, m, lo + hi, sk, b.indices, s)) {
+        g.err = "HIP error in the grid sort";
+        return MG_E_DEVICE;
+      }
       hipLaunchKernelGGL(gr_sparse_flags, dim3(blocks_of(m)), dim3(GR_BLOCK), 0, s, sk, m, b.flag);
       if (!scan_u32(b, b.flag, m, b.slot, s)) { g.err = "grid too large for the scan"; return MG_E_ARG; }
       hipLaunchKernelGGL(gr_sparse_cells, dim3(blocks_of((uint64_t)m + 1)), dim3(GR_BLOCK), 0, s, sk, b.flag, b.slot, m,
@@ -955,5 +983,117 @@ int mg_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
   return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? MG_OK : MG_E_DEVICE;
 }
 int mg_device_synchronize(void) { return hipDeviceSynchronize() == hipSuccess ? MG_OK : MG_E_DEVICE; }
+
+}  // extern "C"
+
+// ---- the build that allocates nothing ------------------------------------------------------------
+// mg_grid_build_noalloc keeps the scratch of a build (control block, tile counts and offsets, the
+// radix sort's key/index ping-pong buffers and digit histograms, the sparse boundary flags and
+// slots, the scan's tile sums) in one file-scope device array of fixed size, sized for
+// GR_NOALLOC_MAX points; the caller passes the points and the output buffers. Every call on a GPU
+// uses that GPU's copy of the array: calls that overlap on one GPU race.
+#define GR_NOALLOC_MAX (1u << 20)
+
+static __device__ __align__(256) unsigned char scratch[
+    // ctrl, tile_cnt, tile_off, idxA, idxB, keyA, keyB, hist, dtot, flag, slot, sums, sums_off
+    256 + 2 * (((GR_NOALLOC_MAX / GR_TILE + 3) * 4 + 255) / 256 * 256) +
+    2 * (((GR_NOALLOC_MAX + 1) * 4 + 255) / 256 * 256) + 2 * (((GR_NOALLOC_MAX + 1) * 8 + 255) / 256 * 256) +
+    (((size_t)GR_BINS * (GR_NOALLOC_MAX / GR_TILE + 2) * 4 + 255) / 256 * 256) + GR_BINS * 4 +
+    2 * (((GR_NOALLOC_MAX + 2) * 4 + 255) / 256 * 256) + 2 * (((GR_SCAN_TILES + 2) * 4 + 255) / 256 * 256)];
+
+namespace {
+
+// Carves the scratch buffers of a build over up to cap points from base; returns the bytes used.
+size_t noalloc_layout(GrBufs& b, uint32_t cap, char* base) {
+  const uint64_t nt = tiles_of(cap) + 1;
+  size_t off = 0;
+  off = carve(off, b.ctrl, 1, base);
+  off = carve(off, b.tile_cnt, nt + 1, base);
+  off = carve(off, b.tile_off, nt + 1, base);
+  off = carve(off, b.idxA, cap + 1, base);
+  off = carve(off, b.idxB, cap + 1, base);
+  uint64_t* ka;
+  uint64_t* kb;
+  off = carve(off, ka, cap + 1, base);
+  off = carve(off, kb, cap + 1, base);
+  b.keyA = ka;
+  b.keyB = kb;
+  off = carve(off, b.hist, (size_t)GR_BINS * nt, base);
+  off = carve(off, b.dtot, GR_BINS, base);
+  off = carve(off, b.flag, (size_t)cap + 2, base);
+  off = carve(off, b.slot, (size_t)cap + 2, base);
+  off = carve(off, b.sums, GR_SCAN_TILES + 2, base);
+  off = carve(off, b.sums_off, GR_SCAN_TILES + 2, base);
+  return off;
+}
+
+// Device address of `scratch` on each device, looked up once per device with that device current.
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[64] = {};
+
+int scratch_of_current_device(void** out, std::string& err) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+    err = "no current HIP device";
+    return MG_E_DEVICE;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+      err = "hipGetSymbolAddress failed for the grid scratch array";
+      return MG_E_DEVICE;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return MG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mg_grid_build_noalloc(const float* d_xy, uint32_t n, float cell_size, int64_t* meta8, uint32_t* d_cell_start,
+                          uint64_t cs_cap, uint32_t* d_indices, uint64_t idx_cap, int32_t* d_keys2, uint64_t keys_cap,
+                          void* stream, char* err, size_t errlen) {
+  if (!meta8 || (n && !d_xy) || !d_cell_start || (idx_cap && !d_indices) || (keys_cap && !d_keys2)) {
+    set_err(err, errlen, "null argument");
+    return MG_E_ARG;
+  }
+  if (n > GR_NOALLOC_MAX) {
+    set_err(err, errlen, "mg_grid_build_noalloc takes at most 1048576 points (its scratch array is fixed)");
+    return MG_E_ARG;
+  }
+  std::string e;
+  void* base = nullptr;
+  int rc = scratch_of_current_device(&base, e);
+  if (rc != MG_OK) {
+    set_err(err, errlen, e);
+    return rc;
+  }
+  mg_grid g;  // host-side summary only: no device memory of its own
+  if (noalloc_layout(g.b, GR_NOALLOC_MAX, static_cast<char*>(base)) > sizeof(scratch)) {
+    set_err(err, errlen, "grid scratch layout exceeds the scratch array");
+    return MG_E_ARG;
+  }
+  g.b.cell_start = d_cell_start;
+  g.b.indices = d_indices;
+  g.b.keys = reinterpret_cast<int2*>(d_keys2);
+  OutCaps caps;
+  caps.cs = cs_cap;
+  caps.idx = idx_cap;
+  caps.keys = keys_cap / 2;
+  rc = build_into(g, g.b, reinterpret_cast<const float2*>(d_xy), n, cell_size, (hipStream_t)stream, caps);
+  if (rc != MG_OK) {
+    set_err(err, errlen, g.err);
+    return rc;
+  }
+  if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess) {
+    set_err(err, errlen, "HIP error in the grid build");
+    return MG_E_DEVICE;
+  }
+  return mg_grid_meta(&g, meta8);
+}
 
 }  // extern "C"
