@@ -150,6 +150,16 @@ typedef struct mg_box_config {
                       (same f32 initial state and geometry, promoted) */
   float yield_stress; /* Bingham yield stress (Pa, the reference's yield_stress_pa with rheology
                          "bingham"); 0: Newtonian */
+  uint32_t adaptive;  /* 0: exactly `passes` density passes; 1: the reference's adaptive count
+                         (pbf.rs:1261-1333): at least max(passes, 1), stopping after a pass whose
+                         largest constraint is <= 0.02, at most 6 more (passes <= 10) */
+  float gravity;      /* added to v.y * dt in the predict (pbf.rs:1225-1235; the reference's
+                         GRAVITY is -9.81); 0 in the kernel-isolation workload */
+  float speed_clamp;  /* v_max of the fluid speed clamp after the CG (clamp_fluid_speed,
+                         pbf.rs:987-996); 0: off */
+  uint32_t energy_stats; /* 1: f64 kinetic energies of every step (mg_box_energy) */
+  float density_ratio; /* density of the unjittered lattice over rest (the particle mass follows);
+                          0: 1.02, the kernel-isolation workload */
 } mg_box_config;
 
 /* h, L, cell width, particle mass, cells per side. */
@@ -177,6 +187,12 @@ int mg_box_stats(mg_box* box, double* out3);
  * pbf.rs:800-818, :2303-2304); the sum is FP64, rounded to f32 for an FP32 box. 0 before the first
  * step. Computed on request (one kernel), not inside the step. */
 int mg_box_mean_shear_rate(mg_box* box, double* out);
+/* Density passes the last step ran (the adaptive count when cfg.adaptive). */
+uint32_t mg_box_density_passes(mg_box* box);
+/* Kinetic energy 0.5 m sum |v|^2 (f64 sum of the per-particle |v|^2, kinetic_energy_f64,
+ * pbf.rs:924-926) of the last step before the CG, after it and after the speed clamp; needs
+ * cfg.energy_stats = 1 (else MG_E_ARG). */
+int mg_box_energy(mg_box* box, double* out3);
 /* CG iterations summed over every step this box has run. */
 uint64_t mg_box_total_cg_iterations(const mg_box* box);
 /* Multi-step calls run windows of up to 16 steps between host waits, each step's CG given the

@@ -15,13 +15,17 @@ import numpy as np
 from . import sim as _sim
 
 BOX_DEFAULTS = {"side": 1024, "dx": 1.0e-3, "rest": 1000.0, "mu": 1.0e-3, "dt": 1.0 / 1440.0,
-                "passes": 4, "tol": 1.0e-6, "max_cg": 500, "seed": 1, "fp64": 0, "yield_stress": 0.0}
+                "passes": 4, "tol": 1.0e-6, "max_cg": 500, "seed": 1, "fp64": 0, "yield_stress": 0.0,
+                "adaptive": 0, "gravity": 0.0, "speed_clamp": 0.0, "energy_stats": 0,
+                "density_ratio": 0.0}
 
 
 class BoxConfig(C.Structure):
     _fields_ = [("side", C.c_uint32), ("dx", C.c_float), ("rest", C.c_float), ("mu", C.c_float),
                 ("dt", C.c_float), ("passes", C.c_uint32), ("tol", C.c_float), ("max_cg", C.c_uint32),
-                ("seed", C.c_uint64), ("fp64", C.c_uint32), ("yield_stress", C.c_float)]
+                ("seed", C.c_uint64), ("fp64", C.c_uint32), ("yield_stress", C.c_float),
+                ("adaptive", C.c_uint32), ("gravity", C.c_float), ("speed_clamp", C.c_float),
+                ("energy_stats", C.c_uint32), ("density_ratio", C.c_float)]
 
 
 def box_config(overrides=None) -> dict:
@@ -34,7 +38,8 @@ def _cfg_struct(cfg) -> BoxConfig:
     c = box_config(cfg)
     return BoxConfig(int(c["side"]), float(c["dx"]), float(c["rest"]), float(c["mu"]), float(c["dt"]),
                      int(c["passes"]), float(c["tol"]), int(c["max_cg"]), int(c["seed"]), int(c["fp64"]),
-                     float(c["yield_stress"]))
+                     float(c["yield_stress"]), int(c["adaptive"]), float(c["gravity"]), float(c["speed_clamp"]),
+                     int(c["energy_stats"]), float(c["density_ratio"]))
 
 
 _ready = set()
@@ -77,6 +82,9 @@ def _lib():
     L.mg_box_kernel_times.argtypes = [P, dp, C.POINTER(C.c_uint64), C.c_int]
     L.mg_box_replay_ms.argtypes = [P, C.c_uint32, C.c_uint32, dp]
     L.mg_box_mean_shear_rate.argtypes = [P, dp]
+    L.mg_box_density_passes.argtypes = [P]
+    L.mg_box_density_passes.restype = C.c_uint32
+    L.mg_box_energy.argtypes = [P, dp]
     _ready.add(id(L))
     return L
 
@@ -180,6 +188,16 @@ class Box:
         out = C.c_double()
         _sim._check(self._L.mg_box_mean_shear_rate(self._h, C.byref(out)), what="mean_shear_rate")
         return out.value
+
+    def density_passes(self) -> int:
+        """Density passes the last step ran."""
+        return int(self._L.mg_box_density_passes(self._h))
+
+    def energy(self) -> dict:
+        """Kinetic energies of the last step (needs energy_stats=1): before the CG, after it, after the clamp."""
+        o = np.zeros(3)
+        _sim._check(self._L.mg_box_energy(self._h, _dp(o)), what="energy")
+        return {"ke_before_cg": o[0], "ke_after_cg": o[1], "ke_after_clamp": o[2]}
 
     def stats(self) -> dict:
         o = np.zeros(3)

@@ -66,6 +66,7 @@
 #endif
 // kernel classes of the optional per-kernel timing (mg_box_kernel_times)
 #define BX_NCLASS 8
+#define BX_MAXPASS 16  // most density passes of one step (adaptive: base + 6)
 enum { BXT_GRID = 0, BXT_LAMBDA, BXT_DP, BXT_VEL, BXT_CGINIT, BXT_SPMV, BXT_UPDATE, BXT_PAIRLAM };
 // precision-independent kernels keep C names; the state kernels are templates on the real type R
 // (float: FP32 state with FP64 reductions; double: FP64 throughout)
@@ -108,7 +109,13 @@ template <class V, class T> BDEV void set4(V& v, int q, T x) {
 template <class R> struct BoxGeo {
   uint32_t n, ncx, nblk, xcd_chunk, nred, xcd_chunk_red, ppt;
   R h, h2, L, halfL, cw, mass, rest, dt, k6, ks, w0, m_over_rest, mu2m, eta2, lo, hi;
+  R gdt;  // gravity * dt, added to v.y in the predict
   const uint32_t* halt;
+  // Adaptive density passes (pbf.rs:1261-1333): pass p (0-based) runs when p < base or the largest
+  // constraint of pass p - 1 exceeded 0.02. dens[p] holds that largest constraint's bits (R's bits
+  // widened to 64; non-negative values order like their bits), dens[BX_MAXPASS] the passes run.
+  uint32_t adaptive, base;
+  unsigned long long* dens;
 };
 // Particles per lane in the reduction kernels. FP64 (one workgroup per CU) caps the workgroups at
 // BX_NRED and loops over g.ppt particles per lane; FP32 keeps one particle per lane (a compile-time
@@ -138,6 +145,7 @@ struct BoxScal {
   uint32_t it, done, max_cg;
   float tol;
   const void* mu_eff;  // per-particle effective viscosity (Bingham boxes), else null; set at creation
+  unsigned long long dens[BX_MAXPASS + 1];  // BoxGeo::dens
 };
 
 // ---------------------------------------------------------------------------------------------
@@ -565,9 +573,15 @@ BXKT bx_predict(BoxGeo<R> g, Vec<R>* __restrict__ X, const Vec<R>* __restrict__ 
   uint32_t k = 0xFFFFFFFFu;
   if (i < g.n) {
     Vec<R> x = X[i];
-    const Vec<R> v = V[i];
+    Vec<R> v = V[i];
+    // step 0: a non-finite particle restarts at rest at the origin (pbf.rs:1200-1205)
+    if (!(isfinite(x.x) && isfinite(x.y) && isfinite(v.x) && isfinite(v.y))) {
+      x = mk((R)0, (R)0);
+      v = mk((R)0, (R)0);
+    }
     X0[i] = x;
-    x = mk(wrapR(x.x + v.x * g.dt, g.L), wrapR(x.y + v.y * g.dt, g.L));
+    const R vy = v.y + g.gdt;  // v.y += g dt; x += v dt (pbf.rs:1225-1235)
+    x = mk(wrapR(x.x + v.x * g.dt, g.L), wrapR(x.y + vy * g.dt, g.L));
     X[i] = x;
     k = cell1(x.y, g) * g.ncx + cell1(x.x, g);
     key[i] = k;
@@ -659,6 +673,7 @@ BXK bx_cellsort(const uint32_t* __restrict__ halt, uint32_t ncell, const uint32_
   if (*halt) return;
   const uint32_t c = blockIdx.x * BX_BLOCK + threadIdx.x;
   if (c == 0) {
+    for (int p = 0; p <= BX_MAXPASS; p++) sc->dens[p] = 0ull;
     sc->done = cg_off;
     sc->it = 0u;
     sc->max_cg = max_cg;
@@ -765,10 +780,31 @@ template <class R, bool W> BDEV void lambda_term(const BoxGeo<R>& g, Wrap<W> w, 
   a.gs = add2(a.gs, gr);
   a.sgs += len2(gr);
 }
-template <class R> BDEV R lambda_of(const BoxGeo<R>& g, LamAcc<R> a) {
-  const R c = mfmax(a.rho / g.rest - (R)1, (R)0);
+template <class R> BDEV R lambda_of(const BoxGeo<R>& g, LamAcc<R> a, R& c) {
+  c = mfmax(a.rho / g.rest - (R)1, (R)0);
   a.sgs += len2(a.gs);
   return -c / (a.sgs + (R)200);
+}
+// Bits of a non-negative real widened to 64 (they order like the values) and back.
+BDEV unsigned long long rbits(float v) { return (unsigned long long)__float_as_uint(v); }
+BDEV unsigned long long rbits(double v) { return (unsigned long long)__double_as_longlong(v); }
+template <class R> BDEV R from_rbits(unsigned long long b);
+template <> BDEV float from_rbits<float>(unsigned long long b) { return __uint_as_float((uint32_t)b); }
+template <> BDEV double from_rbits<double>(unsigned long long b) { return __longlong_as_double((long long)b); }
+// Folds the wave's largest constraint (c >= 0: mfmax against 0 never yields NaN) into the pass's
+// slot: every lane of the wave calls it; lane 0 issues one atomic.
+template <class R> BDEV void fold_max_c(const BoxGeo<R>& g, uint32_t p, R c) {
+  unsigned long long b = rbits(c);
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long t = (unsigned long long)__shfl_xor((long long)b, o, 64);
+    b = t > b ? t : b;
+  }
+  if ((threadIdx.x & 63u) == 0u && b) atomicMax(&g.dens[p], b);
+}
+// Whether density pass p runs this step (the reference stops after pass p - 1 once it ran at least
+// `base` passes and its largest constraint was at most 0.02; a pass that did not run leaves 0).
+template <class R> BDEV bool pass_live(const BoxGeo<R>& g, uint32_t p) {
+  return !g.adaptive || p < g.base || from_rbits<R>(g.dens[p - 1]) > (R)0.02f;
 }
 // Pass A's output for pass B. FP32: the particle's position (the one pass A read) and lambda in
 // one 16-byte record (x, y, lambda, 0), so pass B gathers a neighbour's position and lambda in one
@@ -785,21 +821,24 @@ template <class R> BDEV void put_lambda(void* __restrict__ LB, uint32_t i, Vec<R
     static_cast<R*>(LB)[i] = lam;
   }
 }
-BXKT bx_lambda(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, void* __restrict__ LB) {
-  if (*g.halt) return;
+BXKT bx_lambda(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, void* __restrict__ LB, uint32_t p) {
+  if (*g.halt || !pass_live(g, p)) return;
   const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
-  if (i >= g.n) return;
-  const Vec<R> xi = X[i];
-  const uint32_t cw = pv.cnt[i];
-  LamAcc<R> a{g.w0, mk((R)0, (R)0), (R)0};
-  by_wrap(wave_fast(xi, cw, g), [&](auto w) __attribute__((always_inline)) {
-    for_pairs_g(g, pv, i, cw, w, [&](uint32_t j) __attribute__((always_inline)) { return at(X, j); },
-                [&](uint32_t, Vec<R> xj) __attribute__((always_inline)) {
-      const Vec<R> d = rel(xi, xj, g, w);
-      lambda_term(g, w, d, len2(d), a);
+  R c = (R)0;
+  if (i < g.n) {
+    const Vec<R> xi = X[i];
+    const uint32_t cw = pv.cnt[i];
+    LamAcc<R> a{g.w0, mk((R)0, (R)0), (R)0};
+    by_wrap(wave_fast(xi, cw, g), [&](auto w) __attribute__((always_inline)) {
+      for_pairs_g(g, pv, i, cw, w, [&](uint32_t j) __attribute__((always_inline)) { return at(X, j); },
+                  [&](uint32_t, Vec<R> xj) __attribute__((always_inline)) {
+        const Vec<R> d = rel(xi, xj, g, w);
+        lambda_term(g, w, d, len2(d), a);
+      });
     });
-  });
-  put_lambda<R>(LB, i, xi, lambda_of(g, a));
+    put_lambda<R>(LB, i, xi, lambda_of(g, a, c));
+  }
+  if (g.adaptive) fold_max_c(g, p, c);
 }
 // First density pass A of a step, fused with the pair-table build: right after the reorder the
 // positions X equal the predicted ones XP, so pass A runs on XP (same values, same order as
@@ -812,7 +851,8 @@ BXKT bx_pairs_lambda(BoxGeo<R> g, const Vec<R>* __restrict__ XP, const uint32_t*
   __shared__ uint2 s_row[(BX_KMAX / 4) * BX_BLOCK];  // each lane's table row (16 KB per workgroup)
   if (*g.halt) return;
   const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
-  if (i >= g.n) return;
+  R cmax = (R)0;
+  if (i < g.n) {  // every lane of the wave reaches the constraint reduction below
   uint2* lrow = s_row + threadIdx.x;
   const Vec<R> xpi = XP[i];
   uint32_t cw = 0;
@@ -853,13 +893,20 @@ BXKT bx_pairs_lambda(BoxGeo<R> g, const Vec<R>* __restrict__ XP, const uint32_t*
       scan_pairs(g, XP, start, i, xpi, [&](uint32_t j) __attribute__((always_inline)) { term(XP[j]); });
     }
   });
-  put_lambda<R>(LB, i, xpi, lambda_of(g, a));
+  put_lambda<R>(LB, i, xpi, lambda_of(g, a, cmax));
+  }
+  if (g.adaptive) fold_max_c(g, 0u, cmax);
 }
 // Pass B (FP32: positions and lambdas from pass A's records, which hold the positions X).
 BXKT bx_dp(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const void* __restrict__ LB,
-           Vec<R>* __restrict__ Xout) {
+           Vec<R>* __restrict__ Xout, uint32_t p) {
   if (*g.halt) return;
   const uint32_t i = tile_id(g) * BX_BLOCK + threadIdx.x;
+  if (!pass_live(g, p)) {  // a pass that does not run leaves the positions as they are
+    if (i < g.n) Xout[i] = X[i];
+    return;
+  }
+  if (g.adaptive && blockIdx.x == 0 && threadIdx.x == 0) g.dens[BX_MAXPASS] = p + 1u;
   if (i >= g.n) return;
   Vec<R> xi;
   R li;
@@ -897,17 +944,20 @@ BXKT bx_dp(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const void
 // ---------------------------------------------------------------------------------------------
 // velocity, density recompute, implicit viscosity
 // ---------------------------------------------------------------------------------------------
+// part_ke (optional): per-block sums of the f64-widened |v|^2 (kinetic_energy_f64, pbf.rs:924-926).
 BXKRT bx_vel_density(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, const Vec<R>* __restrict__ X0,
-                     Vec<R>* __restrict__ V, R* __restrict__ DENS, double* __restrict__ part_b) {
+                     Vec<R>* __restrict__ V, R* __restrict__ DENS, double* __restrict__ part_b,
+                     double* __restrict__ part_ke) {
   if (*g.halt) return;
   const uint32_t ppt = red_ppt(g);
   const uint32_t base = tile_id_red(g) * BX_RBLOCK * ppt + threadIdx.x;
-  double vv = 0.0;
+  double vv = 0.0, ke = 0.0;
   for_lane_particles(g, base, [&](uint32_t i, uint32_t k) __attribute__((always_inline)) {
     const Vec<R> xi = X[i];
     const Vec<R> v = div2(rel(xi, X0[i], g), g.dt);
     V[i] = v;
     vv += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+    ke += (double)len2(v);
     R rho = g.w0;
     const uint32_t cw = pv.cnt[i];
     by_wrap(wave_fast(xi, cw, g), [&](auto w) __attribute__((always_inline)) {
@@ -920,6 +970,39 @@ BXKRT bx_vel_density(BoxGeo<R> g, PairView<R> pv, const Vec<R>* __restrict__ X, 
   });
   const double t = block_sum<BX_RBLOCK>(vv);
   if (threadIdx.x == 0) part_b[blockIdx.x] = t;
+  if (part_ke) {
+    const double e = block_sum<BX_RBLOCK>(ke);
+    if (threadIdx.x == 0) part_ke[blockIdx.x] = e;
+  }
+}
+
+// After the CG: the per-block f64 sums of |v|^2 before (part_a) and after (part_c) the fluid speed
+// clamp (clamp_fluid_speed, pbf.rs:987-996: a non-finite speed zeroes v, a speed above v_max scales
+// v by v_max / speed), which runs when v_max > 0.
+BXKRT bx_clamp_ke(BoxGeo<R> g, Vec<R>* __restrict__ V, R v_max, double* __restrict__ part_a,
+                  double* __restrict__ part_c) {
+  if (*g.halt) return;
+  const uint32_t ppt = red_ppt(g);
+  const uint32_t base = tile_id_red(g) * BX_RBLOCK * ppt + threadIdx.x;
+  double a = 0.0, c = 0.0;
+  for_lane_particles(g, base, [&](uint32_t i, uint32_t) __attribute__((always_inline)) {
+    Vec<R> v = V[i];
+    a += (double)len2(v);
+    if (v_max > (R)0) {
+      const R speed = msqrt(len2(v));
+      if (!isfinite(speed)) v = mk((R)0, (R)0);
+      else if (speed > v_max) v = mul2(v, v_max / speed);
+      V[i] = v;
+    }
+    c += (double)len2(v);
+  });
+  if (!part_a) return;  // no energy sums asked for (uniform: the whole workgroup returns)
+  const double ta = block_sum<BX_RBLOCK>(a);
+  const double tc = block_sum<BX_RBLOCK>(c);
+  if (threadIdx.x == 0) {
+    part_a[blockIdx.x] = ta;
+    part_c[blockIdx.x] = tc;
+  }
 }
 
 // Shear rate sqrt(2 D:D) of particle i from the SPH velocity gradient over the step's pairs, at the
@@ -1407,6 +1490,7 @@ struct mg_box {
   double* part_b = nullptr;    // |b|^2 partials (one per reduction block)
   double* part_rr = nullptr;   // r.r partials, two generations: [0, nred) and [nred, 2 nred)
   double* part_pap = nullptr;  // p.Ap partials
+  double* part_ke = nullptr;   // energy_stats: |v|^2 partials before the CG, after it, after the clamp
   void* P2 = nullptr;          // second direction buffer (the CG alternates P and P2)
   uint32_t cg_t = 0;           // next CG iteration number of the current step
   BoxScal* sc = nullptr;
@@ -1470,6 +1554,16 @@ bool box_geometry(const mg_box_config& c, BoxGeo<float>& g, std::string& err) {
     err = "yield_stress must be finite and non-negative";
     return false;
   }
+  if (c.adaptive > 1u || c.energy_stats > 1u) { err = "adaptive and energy_stats must be 0 or 1"; return false; }
+  if (c.adaptive && std::max(c.passes, 1u) + 6u > BX_MAXPASS) {
+    err = "adaptive density passes: passes must be at most 10";
+    return false;
+  }
+  if (!c.adaptive && c.passes > 64u) { err = "passes must be at most 64"; return false; }
+  if (!std::isfinite(c.gravity) || !(c.speed_clamp >= 0.0f) || !std::isfinite(c.speed_clamp)) {
+    err = "gravity must be finite and speed_clamp finite and non-negative";
+    return false;
+  }
   g.n = c.side * c.side;
   g.h = 2.0f * c.dx;
   g.h2 = g.h * g.h;
@@ -1491,7 +1585,11 @@ bool box_geometry(const mg_box_config& c, BoxGeo<float>& g, std::string& err) {
       float t = g.h * g.h - r2;
       lw += g.k6 * t * t * t;
     }
-  g.mass = 1.02f * c.rest / lw;
+  if (!(c.density_ratio >= 0.0f) || !std::isfinite(c.density_ratio)) {
+    err = "density_ratio must be finite and non-negative";
+    return false;
+  }
+  g.mass = (c.density_ratio > 0.0f ? c.density_ratio : 1.02f) * c.rest / lw;
   g.rest = c.rest;
   g.dt = c.dt;
   {
@@ -1505,6 +1603,10 @@ bool box_geometry(const mg_box_config& c, BoxGeo<float>& g, std::string& err) {
   g.hi = g.L - g.lo;
   if (!(g.h >= 0x1p-60f)) g.hi = -1.0f;  // no interior: every wave takes the minimum image
   g.halt = nullptr;
+  g.gdt = c.gravity * c.dt;
+  g.adaptive = c.adaptive;
+  g.base = std::max(c.passes, 1u);
+  g.dens = nullptr;
   g.nblk = (g.n + BX_BLOCK - 1) / BX_BLOCK;
   g.xcd_chunk = (g.nblk % 8u == 0u) ? g.nblk / 8u : 0u;
   g.nred = (g.n + BX_RBLOCK - 1) / BX_RBLOCK;  // FP32 reduction kernels: one particle per lane
@@ -1548,6 +1650,10 @@ BoxGeo<double> promote(const BoxGeo<float>& f, const mg_box_config& c) {
   g.hi = g.L - g.lo;
   if (!(g.h >= 0x1p-60)) g.hi = -1.0;
   g.halt = nullptr;
+  g.gdt = (double)c.gravity * (double)c.dt;
+  g.adaptive = f.adaptive;
+  g.base = f.base;
+  g.dens = nullptr;
   return g;
 }
 
@@ -1664,7 +1770,8 @@ template <class R> bool enqueue_front(mg_box* B) {
     Tic t(B, BXT_GRID);
     hipLaunchKernelGGL(bx_pairs<R>, gp, blk, 0, st, g, vp<R>(B->XP), B->start, B->pnb, B->pcnt);
   }
-  for (uint32_t p = 0; p < B->cfg.passes; p++) {
+  const uint32_t npass = B->cfg.adaptive ? std::max(B->cfg.passes, 1u) + 6u : B->cfg.passes;
+  for (uint32_t p = 0; p < npass; p++) {
     if (p == 0) {  // the first pass A also builds the pair table
       Tic t(B, BXT_PAIRLAM);
       hipLaunchKernelGGL(bx_pairs_lambda<R>, gp, blk, 0, st, g, vp<R>(B->XP), B->start, B->pnb, B->pcnt,
@@ -1674,7 +1781,7 @@ template <class R> bool enqueue_front(mg_box* B) {
       const BoxGeo<R> gc = g;
       Vec<R>* const x = vp<R>(B->X);
       void* const lam = B->LAM;
-      B->replay[BXT_LAMBDA] = [=]() { hipLaunchKernelGGL(bx_lambda<R>, gp, blk, 0, st, gc, pv, x, lam); };
+      B->replay[BXT_LAMBDA] = [=]() { hipLaunchKernelGGL(bx_lambda<R>, gp, blk, 0, st, gc, pv, x, lam, p); };
       B->replay[BXT_LAMBDA]();
     }
     {
@@ -1683,7 +1790,7 @@ template <class R> bool enqueue_front(mg_box* B) {
       const Vec<R>* const x = vp<R>(B->X);
       const void* const lam = B->LAM;
       Vec<R>* const xo = vp<R>(B->X2);
-      B->replay[BXT_DP] = [=]() { hipLaunchKernelGGL(bx_dp<R>, gp, blk, 0, st, gc, pv, x, lam, xo); };
+      B->replay[BXT_DP] = [=]() { hipLaunchKernelGGL(bx_dp<R>, gp, blk, 0, st, gc, pv, x, lam, xo, p); };
       B->replay[BXT_DP]();
     }
     std::swap(B->X, B->X2);
@@ -1692,7 +1799,7 @@ template <class R> bool enqueue_front(mg_box* B) {
   {
     Tic t(B, BXT_VEL);
     hipLaunchKernelGGL(bx_vel_density<R>, gr, rblk, 0, st, g, pv, vp<R>(B->X), vp<R>(B->X0), vp<R>(B->VN),
-                       sp<R>(B->DENS), B->part_b);
+                       sp<R>(B->DENS), B->part_b, B->part_ke);
   }
   const bool bingham = B->cfg.yield_stress > 0.0f;
   if (!cg_off && bingham) {
@@ -1746,6 +1853,17 @@ template <class R> void enqueue_cg_chunk(mg_box* B, uint32_t iters, uint32_t* ve
                      s, B->halt);
 }
 
+// After a step's CG: the fluid speed clamp and the kinetic-energy sums, when the box has them.
+template <class R> void enqueue_epilogue(mg_box* B) {
+  if (!(B->cfg.speed_clamp > 0.0f) && !B->cfg.energy_stats) return;
+  const BoxGeo<R>& g = geo<R>(B);
+  const size_t nr = g.nred;
+  double* pa = B->part_ke ? B->part_ke + nr : nullptr;
+  double* pc = B->part_ke ? B->part_ke + 2 * nr : nullptr;
+  hipLaunchKernelGGL(bx_clamp_ke<R>, dim3(g.nred), dim3(BX_RBLOCK), 0, B->stream, g, vp<R>(B->VN),
+                     (R)B->cfg.speed_clamp, pa, pc);
+}
+
 // One full step: the front half and a first CG chunk of `first` iterations are enqueued without a
 // host wait; further chunks follow only if the device has not converged.
 template <class R> int run_step(mg_box* B, uint32_t first) {
@@ -1767,6 +1885,7 @@ template <class R> int run_step(mg_box* B, uint32_t first) {
     launched += k;
     if (hipGetLastError() != hipSuccess) { B->err = "kernel launch failed"; return MG_E_DEVICE; }
   }
+  enqueue_epilogue<R>(B);
   std::swap(B->V, B->VN);
   B->last_it = B->h_sc->it;
   B->total_it += B->h_sc->it;
@@ -1813,6 +1932,7 @@ template <class R> int run_window(mg_box* B, uint32_t steps) {
       if (!enqueue_front<R>(B)) { B->err = "kernel launch failed"; return MG_E_DEVICE; }
       B->starts[s] = {B->X0, B->V, B->id, B->start};
       if (cg_on) enqueue_cg_chunk<R>(B, pred, B->verdict, s);
+      enqueue_epilogue<R>(B);
       std::swap(B->V, B->VN);
     }
     if (cg_on) hipMemcpyAsync(B->h_verdict, B->verdict, (W + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, B->stream);
@@ -1962,6 +2082,7 @@ int mg_box_create(const mg_box_config* cfg, mg_box** out, char* err, size_t errl
   off = carve_bytes(off, B->R, vb); off = carve_bytes(off, B->P, vb); off = carve_bytes(off, B->AP, vb);
   off = carve(off, B->part_b, (size_t)B->nred); off = carve(off, B->part_rr, 2 * (size_t)B->nred);
   off = carve(off, B->part_pap, (size_t)B->nred); off = carve_bytes(off, B->P2, vb);
+  off = carve(off, B->part_ke, cfg->energy_stats ? 3 * (size_t)B->nred : 0);
   off = carve(off, B->pnb, (size_t)BX_KMAX * n); off = carve(off, B->pcnt, n);
   off = carve_bytes(off, B->cw, (size_t)BX_KMAX * rs * n);
   off = carve(off, B->verdict, BX_WINDOW + 1);
@@ -1973,8 +2094,10 @@ int mg_box_create(const mg_box_config* cfg, mg_box** out, char* err, size_t errl
   fix(B->id); fix(B->idb); fix(B->key); fix(B->slot); fix(B->start); fix(B->start_next); fix(B->cursor); fix(B->bsum);
   fix(B->LAM); fix(B->DENS); fix(B->MU); fix(B->R); fix(B->P); fix(B->AP); fix(B->part_b); fix(B->part_rr); fix(B->part_pap);
   fix(B->P2); fix(B->sc); fix(B->pnb); fix(B->pcnt); fix(B->cw); fix(B->verdict); fix(B->halt);
+  if (cfg->energy_stats) fix(B->part_ke); else B->part_ke = nullptr;
   B->gf.halt = B->halt;
   B->gd.halt = B->halt;
+  B->gf.dens = B->gd.dens = B->sc->dens;
   bool ok = hipMemset(B->arena, 0, off) == hipSuccess;
   if (cfg->yield_stress > 0.0f) {  // the CG's rescan path finds the effective viscosities here
     const void* mu_eff = B->MU;
@@ -2114,6 +2237,33 @@ int mg_box_stats(mg_box* B, double* out3) {
   BoxScal s{};
   if (hipMemcpy(&s, B->sc, sizeof s, hipMemcpyDeviceToHost) != hipSuccess) return MG_E_DEVICE;
   out3[0] = s.it; out3[1] = s.res; out3[2] = s.bnorm;
+  return MG_OK;
+}
+
+uint32_t mg_box_density_passes(mg_box* B) {
+  if (!B || !B->have_neighbors) return 0;
+  if (!B->cfg.adaptive) return B->cfg.passes;
+  unsigned long long p = 0;
+  if (hipStreamSynchronize(B->stream) != hipSuccess ||
+      hipMemcpy(&p, &B->sc->dens[BX_MAXPASS], sizeof p, hipMemcpyDeviceToHost) != hipSuccess)
+    return 0;
+  return (uint32_t)p;
+}
+
+int mg_box_energy(mg_box* B, double* out3) {
+  if (!B || !out3) return MG_E_ARG;
+  if (!B->part_ke || !B->have_neighbors) return MG_E_ARG;
+  const uint32_t nr = B->f64 ? B->gd.nred : B->gf.nred;
+  std::vector<double> part(3 * (size_t)nr);
+  if (hipStreamSynchronize(B->stream) != hipSuccess ||
+      hipMemcpy(part.data(), B->part_ke, part.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+    return MG_E_DEVICE;
+  const double half_m = 0.5 * (B->f64 ? B->gd.mass : (double)B->gf.mass);
+  for (int k = 0; k < 3; k++) {
+    double s = 0.0;
+    for (uint32_t b = 0; b < nr; b++) s += part[(size_t)k * nr + b];
+    out3[k] = half_m * s;
+  }
   return MG_OK;
 }
 

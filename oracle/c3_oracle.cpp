@@ -13,15 +13,21 @@
 //   density recompute       at the final positions                     pbf.rs:2136-2144
 //   Morris weights          c_ij, unit directions                      pbf.rs:312-344
 //   implicit viscosity CG   (I + dt L) v = v, warm start v             pbf.rs:359-480
-// with exactly `passes` density passes (the reference's adaptive count is not used here) and the
-// CG stopping at `tol`·|b| after at most `max_cg` iterations (the reference uses 1e-3 and 50).
+// with exactly `passes` density passes, or (adaptive) the reference's count: at least
+// max(passes, 1), stopping once max C <= 0.02, at most 6 more (pbf.rs:1261-1333); the CG stopping
+// at `tol`·|b| after at most `max_cg` iterations (the reference uses 1e-3 and 50). Options the
+// drum step has and the box leaves off by default: gravity in the predict (pbf.rs:1225-1235), the
+// step-0 reset of non-finite particles (pbf.rs:1200-1205, always on), the fluid speed clamp after
+// the CG (pbf.rs:987-996, :2321) and f64 kinetic-energy snapshots around the CG and the clamp
+// (pbf.rs:924-926).
 //
 // Workload definition (synthetic; documented beside the GPU path):
 //   side x side square lattice, spacing dx, kernel radius h = 2 dx, L = side·dx;
 //   particle i = iy·side + ix at ((ix + 0.5) dx + jx, (iy + 0.5) dx + jy), jitter U(-0.1dx, 0.1dx);
 //   velocity per component 0.01·N(0, 1) by Box-Muller (cosine branch), computed in f64;
 //   one Rng(seed) stream in particle order: jx, jy, u1, u2, u3, u4;
-//   mass = 1.02·rest / Σ_lattice W (the unjittered lattice is 2 % over-dense).
+//   mass = 1.02·rest / Σ_lattice W (the unjittered lattice is 2 % over-dense; density_ratio
+//   replaces the 1.02).
 
 namespace orc {
 
@@ -36,6 +42,10 @@ struct C3Config {
   uint32_t max_cg = 500;
   uint64_t seed = 1;
   float yield_stress = 0.0f;  // Bingham yield stress (Pa); 0: Newtonian
+  uint32_t adaptive = 0;      // 1: the reference's adaptive pass count with base max(passes, 1)
+  float gravity = 0.0f;       // added to v.y * dt in the predict (the reference's GRAVITY is -9.81)
+  float speed_clamp = 0.0f;   // fluid speed clamp v_max after the CG (pbf.rs:987-996); 0: off
+  float density_ratio = 0.0f; // density of the unjittered lattice over rest (0: 1.02)
 };
 
 // Box geometry shared by the seeding and the step (all f32, as the GPU path computes it).
@@ -59,7 +69,8 @@ struct C3Box {
         float rx = (float)p * c.dx, ry = (float)q * c.dx;
         lw += poly6<float>(rx * rx + ry * ry, b.h);
       }
-    b.mass = 1.02f * c.rest / lw;
+    const float ratio = c.density_ratio > 0.0f ? c.density_ratio : 1.02f;
+    b.mass = ratio * c.rest / lw;
     return b;
   }
   // rem_euclid (f32), as the reference wraps angles
@@ -182,9 +193,10 @@ template <class R> static R c3_bingham(R g, R mu, R tau_y) {
 // Jacobi displacement dp at positions x (not moved here). gcache holds each edge's spiky gradient,
 // as the reference's per-edge cache does (pbf.rs:1268-1285).
 template <class R>
-static void c3_density_pass(const std::vector<V2<R>>& x, const Nbrs& nbr, const C3Box& b, R mass, R rest,
-                            std::vector<R>& density, std::vector<R>& lambda, std::vector<V2<R>>& dp,
-                            std::vector<V2<R>>& gcache) {
+static R c3_density_pass(const std::vector<V2<R>>& x, const Nbrs& nbr, const C3Box& b, R mass, R rest,
+                         std::vector<R>& density, std::vector<R>& lambda, std::vector<V2<R>>& dp,
+                         std::vector<V2<R>>& gcache) {
+  R max_c = 0;  // the largest constraint of the pass (f32::max from 0, NaN-ignoring)
   const size_t n = x.size();
   const R h = (R)b.h;
   density.assign(n, (R)0);
@@ -209,6 +221,7 @@ static void c3_density_pass(const std::vector<V2<R>>& x, const Nbrs& nbr, const 
     }
     density[i] = rho;
     R c = rmax(density[i] / rest - F(1.0), (R)0);
+    max_c = rmax(max_c, c);
     sgs += gs.len2();
     lambda[i] = -c / (sgs + F(200.0));
   }
@@ -216,6 +229,22 @@ static void c3_density_pass(const std::vector<V2<R>>& x, const Nbrs& nbr, const 
     V2<R> s;
     for (uint32_t k = nbr.off[i]; k < nbr.off[i + 1]; k++) s += (lambda[i] + lambda[nbr.nb[k]]) * gcache[k];
     dp[i] = s * (mass / rest);
+  }
+  return max_c;
+}
+
+// kinetic_energy_f64 (pbf.rs:924-926): 0.5 m times the f64 sum of the f32 |v|^2.
+template <class R> static double c3_kinetic_energy(const std::vector<V2<R>>& v, R mass) {
+  double s = 0.0;
+  for (const V2<R>& a : v) s += (double)a.len2();
+  return 0.5 * (double)mass * s;
+}
+// clamp_fluid_speed (pbf.rs:987-996).
+template <class R> static void c3_clamp_speed(std::vector<V2<R>>& v, R v_max) {
+  for (V2<R>& a : v) {
+    const R speed = a.len();
+    if (!std::isfinite(speed)) a = V2<R>();
+    else if (speed > v_max) a = a * (v_max / speed);
   }
 }
 
@@ -330,6 +359,8 @@ template <class R, class A> struct C3Sim {
   std::vector<R> dens;      // density at the final positions of the last step
   uint32_t last_cg = 0;
   A last_res = 0, last_bnorm = 0;
+  uint32_t last_passes = 0;                           // density passes of the last step
+  double ke_before_cg = 0, ke_after_cg = 0, ke_after_clamp = 0;  // f64 snapshots of the last step
 
   void init(const C3Config& c) {
     cfg = c;
@@ -361,25 +392,49 @@ template <class R, class A> struct C3Sim {
 
   void step() {
     const size_t n = x.size();
-    const R dt = (R)cfg.dt, h = (R)box.h, rest = (R)cfg.rest, mass = (R)box.mass, mu = (R)cfg.mu;
-    // predict (pbf.rs:1225-1235), gravity 0, positions wrapped into the box
+    const R dt = (R)cfg.dt, rest = (R)cfg.rest, mass = (R)box.mass, mu = (R)cfg.mu;
+    // step 0: a non-finite particle restarts at rest at the origin (pbf.rs:1200-1205)
+    for (size_t i = 0; i < n; i++)
+      if (!x[i].finite() || !v[i].finite()) { x[i] = V2<R>(); v[i] = V2<R>(); }
+    // predict (pbf.rs:1225-1235), positions wrapped into the box
     std::vector<V2<R>> x0 = x;
-    for (size_t i = 0; i < n; i++) x[i] = wrap(x[i] + v[i] * dt);
+    const R gdt = (R)cfg.gravity * dt;
+    for (size_t i = 0; i < n; i++) {
+      v[i].y += gdt;
+      x[i] = wrap(x[i] + v[i] * dt);
+    }
     // neighbour lists at the predicted positions (pbf.rs:1238-1239)
     nbr = c3_neighbors(x, box);
-    // density passes (pbf.rs:1257-1334), exactly cfg.passes
+    // density passes (pbf.rs:1257-1334): exactly cfg.passes, or the reference's adaptive count
     std::vector<R> density, lambda;
     std::vector<V2<R>> dp, gcache;
-    for (uint32_t pass = 0; pass < cfg.passes; pass++) {
-      c3_density_pass(x, nbr, box, mass, rest, density, lambda, dp, gcache);
+    const uint32_t base = cfg.adaptive ? std::max(cfg.passes, 1u) : cfg.passes;
+    uint32_t count = 0;
+    while (count < (cfg.adaptive ? base + 6u : base)) {
+      const R max_c = c3_density_pass(x, nbr, box, mass, rest, density, lambda, dp, gcache);
       for (size_t i = 0; i < n; i++) x[i] = wrap(x[i] + dp[i]);
+      count++;
+      if (cfg.adaptive && count >= base && max_c <= F(0.02)) break;
     }
+    last_passes = count;
     // velocity (pbf.rs:1656-1674) with the minimum-image displacement
     for (size_t i = 0; i < n; i++) v[i] = mi(x[i] - x0[i]) / dt;
     // density recompute (pbf.rs:2136-2144)
     density = c3_density_at(x, nbr, box, mass);
     dens = density;
     last_cg = 0;
+    ke_before_cg = c3_kinetic_energy(v, mass);
+    viscosity(mu);
+    ke_after_cg = c3_kinetic_energy(v, mass);
+    if (cfg.speed_clamp > 0.0f) c3_clamp_speed(v, (R)cfg.speed_clamp);
+    ke_after_clamp = c3_kinetic_energy(v, mass);
+  }
+
+  // Implicit viscosity of the step's final state (pbf.rs:2136-2299, fluid-only form).
+  void viscosity(R mu) {
+    const size_t n = x.size();
+    const R dt = (R)cfg.dt, h = (R)box.h, mass = (R)box.mass;
+    const std::vector<R>& density = dens;
     if (!(mu > (R)0)) return;
     // effective viscosity per particle: Bingham from the shear rates at the recomputed densities
     // and the updated velocities (pbf.rs:2147-2151), else mu everywhere
@@ -407,6 +462,8 @@ static orc::C3Config c3_cfg_from(const double* c) {
   k.side = (uint32_t)c[0]; k.dx = (float)c[1]; k.rest = (float)c[2]; k.mu = (float)c[3]; k.dt = (float)c[4];
   k.passes = (uint32_t)c[5]; k.tol = (float)c[6]; k.max_cg = (uint32_t)c[7]; k.seed = (uint64_t)c[8];
   k.yield_stress = (float)c[9];
+  k.adaptive = (uint32_t)c[10]; k.gravity = (float)c[11]; k.speed_clamp = (float)c[12];
+  k.density_ratio = (float)c[13];
   return k;
 }
 #define C3DISPATCH(h, ...)                                                                   \
@@ -417,7 +474,9 @@ static orc::C3Config c3_cfg_from(const double* c) {
   } while (0)
 
 extern "C" {
-// cfg: side, dx, rest, mu, dt, passes, tol, max_cg, seed, yield_stress. mode: 0 = f32/f32,
+// cfg: side, dx, rest, mu, dt, passes, tol, max_cg, seed, yield_stress, adaptive, gravity, speed_clamp,
+// density_ratio.
+// mode: 0 = f32/f32,
 // 1 = f32/f64, 2 = f64.
 void* c3_create(const double* cfg, int mode) {
   auto* h = new C3Handle{mode, nullptr};
@@ -466,6 +525,11 @@ void c3_set_state(void* hv, const double* x2n, const double* v2n) {
       s->v[i].x = (RR)v2n[2 * i]; s->v[i].y = (RR)v2n[2 * i + 1];
     }
   });
+}
+// out: density passes, f64 kinetic energy before the CG, after it and after the speed clamp
+void c3_step_stats(void* hv, double* out) {
+  auto* h = (C3Handle*)hv;
+  C3DISPATCH(h, { out[0] = s->last_passes; out[1] = s->ke_before_cg; out[2] = s->ke_after_cg; out[3] = s->ke_after_clamp; });
 }
 // out: cg iterations, final residual norm, |b|
 void c3_stats(void* hv, double* out) {

@@ -398,8 +398,10 @@ def resolved_ratio(hc):
 
 # ---- kernel-isolation workload (periodic box), c3_oracle.cpp --------------------------------
 C3_DEFAULTS = {"side": 1024, "dx": 1.0e-3, "rest": 1000.0, "mu": 1.0e-3, "dt": 1.0 / 1440.0,
-               "passes": 4, "tol": 1.0e-6, "max_cg": 500, "seed": 1, "yield_stress": 0.0}
-_C3_KEYS = ["side", "dx", "rest", "mu", "dt", "passes", "tol", "max_cg", "seed", "yield_stress"]
+               "passes": 4, "tol": 1.0e-6, "max_cg": 500, "seed": 1, "yield_stress": 0.0,
+               "adaptive": 0, "gravity": 0.0, "speed_clamp": 0.0, "density_ratio": 0.0}
+_C3_KEYS = ["side", "dx", "rest", "mu", "dt", "passes", "tol", "max_cg", "seed", "yield_stress", "adaptive",
+            "gravity", "speed_clamp", "density_ratio"]
 C3_MODES = {"ref": 0, "mixed": 1, "fp64": 2}
 
 
@@ -432,6 +434,7 @@ def _c3_lib():
     L.c3_state.argtypes = [P, dp, dp]
     L.c3_set_state.argtypes = [P, dp, dp]
     L.c3_stats.argtypes = [P, dp]
+    L.c3_step_stats.argtypes = [P, dp]
     L.c3_density.argtypes = [P, dp]
     L.c3_neighbors.restype = C.c_int64
     L.c3_neighbors.argtypes = [P, u32p, u32p, C.c_int64]
@@ -588,6 +591,12 @@ class C3Oracle:
         o = np.zeros(3)
         self._L.c3_stats(self._h, _dp(o))
         return {"cg_iterations": int(o[0]), "residual": o[1], "b_norm": o[2]}
+
+    def step_stats(self) -> dict:
+        """Density passes of the last step and its f64 kinetic energies around the CG and the clamp."""
+        o = np.zeros(4)
+        self._L.c3_step_stats(self._h, _dp(o))
+        return {"density_passes": int(o[0]), "ke_before_cg": o[1], "ke_after_cg": o[2], "ke_after_clamp": o[3]}
 
     def density(self) -> np.ndarray:
         o = np.zeros(self.n)

@@ -424,3 +424,89 @@ def test_bingham_windows_match_single_steps(box_mod, orc, fp64):
     assert np.abs(v - vo).max() / max(np.abs(vo).max(), 1e-30) <= 1e-3
     m, mo = b.mean_shear_rate(), o.mean_shear_rate()
     assert mo > 0.0 and abs(m - mo) <= 1e-3 * mo
+
+
+# ---- options the drum step has: adaptive density passes with the reference's CG constants,
+# gravity in the predict, the step-0 reset of non-finite particles, the speed clamp and the f64
+# kinetic energies (pbf.rs:924-926, :987-996, :1200-1235, :1261-1333) ---------------------------
+OPTION_CASES = [
+    {"side": 64, "passes": 3, "adaptive": 1, "density_ratio": 1.005, "tol": 1.0e-3, "max_cg": 50},
+    {"side": 128, "passes": 1, "adaptive": 1, "density_ratio": 1.005, "mu": 1.0, "tol": 1.0e-3, "max_cg": 50},
+    {"side": 64, "passes": 3, "adaptive": 1},
+    {"side": 48, "passes": 3, "adaptive": 1, "density_ratio": 0.9},
+    {"side": 10, "passes": 1, "adaptive": 1, "density_ratio": 1.005},
+    {"side": 64, "gravity": -9.81},
+    {"side": 64, "mu": 1.0, "speed_clamp": 0.015, "energy_stats": 1},
+    {"side": 128, "tol": 1.0e-3, "max_cg": 50, "energy_stats": 1},
+]
+
+
+@pytest.mark.parametrize("cfg", OPTION_CASES)
+def test_step_options_match_oracle(box_mod, orc, cfg):
+    print(f"box options {cfg}", flush=True)
+    b = box_mod.Box(cfg)
+    o = orc.C3Oracle(cfg, "mixed")
+    for k in range(3):
+        b.step()
+        o.step()
+        _compare(b, o, f"{cfg} step={k}")
+        so = o.step_stats()
+        assert b.density_passes() == so["density_passes"]
+        if cfg.get("energy_stats"):
+            e = b.energy()
+            assert e["ke_before_cg"] == pytest.approx(so["ke_before_cg"], rel=1e-12)
+            assert e["ke_after_cg"] == pytest.approx(so["ke_after_cg"], rel=1e-5)
+            assert e["ke_after_clamp"] == pytest.approx(so["ke_after_clamp"], rel=1e-5)
+
+
+@pytest.mark.parametrize("cfg", [{"side": 64, "passes": 3, "adaptive": 1, "density_ratio": 1.005, "mu": 1.0},
+                                 {"side": 48, "mu": 1.0e-3, "gravity": -9.81, "speed_clamp": 0.02}])
+def test_step_options_through_windows(box_mod, orc, cfg):
+    b = box_mod.Box(cfg)
+    o = orc.C3Oracle(cfg, "mixed")
+    b.step(12)
+    o.step(12)
+    x, v = b.state()
+    xo, vo = o.state()
+    assert np.array_equal(x.astype(np.float64), xo)
+    assert np.abs(v - vo).max() / max(np.abs(vo).max(), 1e-30) <= V_TOL
+    assert b.density_passes() == o.step_stats()["density_passes"]
+
+
+def test_non_finite_particles_restart_at_the_origin(box_mod, orc):
+    cfg = {"side": 64, "mu": 1.0e-3}
+    b = box_mod.Box(cfg)
+    o = orc.C3Oracle(cfg, "mixed")
+    x, v = o.state()
+    x[5] = (np.nan, 0.01)
+    v[77] = (np.inf, 0.0)
+    x[1000] = (-np.inf, np.nan)
+    b.set_state(x.astype(np.float32), v.astype(np.float32))
+    o.set_state(x, v)
+    for k in range(2):
+        b.step()
+        o.step()
+        _compare(b, o, f"non-finite step={k}")
+    assert np.isfinite(b.state()[0]).all()
+
+
+def test_fp64_adaptive_passes_match_oracle(box_mod, orc):
+    cfg = {"side": 32, "fp64": 1, "adaptive": 1, "passes": 3, "density_ratio": 1.005, "mu": 1.0}
+    b = box_mod.Box(cfg)
+    o = orc.C3Oracle(cfg, "fp64")
+    for k in range(3):
+        b.step()
+        o.step()
+        _compare64(b, o, f"fp64 adaptive step={k}")
+        assert b.density_passes() == o.step_stats()["density_passes"]
+
+
+def test_million_particles_adaptive_reference_constants(box_mod, orc):
+    cfg = {"side": 1024, "passes": 3, "adaptive": 1, "density_ratio": 1.005, "tol": 1.0e-3, "max_cg": 50}
+    print("box 1M adaptive", flush=True)
+    b = box_mod.Box(cfg)
+    o = orc.C3Oracle(cfg, "mixed")
+    b.step()
+    o.step()
+    _compare(b, o, "1M adaptive")
+    assert b.density_passes() == o.step_stats()["density_passes"]

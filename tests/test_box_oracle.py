@@ -233,3 +233,81 @@ def test_mean_shear_rate_is_the_mean_of_the_rates(orc):
     m = o.mean_shear_rate()
     assert m > 0.0
     assert m == float(np.float32(np.sum(rates.astype(np.float64)) / rates.size))
+
+
+# ---- options the drum step has: adaptive passes, gravity, non-finite reset, speed clamp, energies --
+def test_adaptive_passes_follow_the_reference_rule(orc):
+    # The count lies in [base, base + 6] (pbf.rs:1261-1333), and the state equals a run with exactly
+    # that many passes (the adaptive loop only decides where to stop).
+    for side, base in ((48, 3), (64, 1)):
+        a = orc.C3Oracle({"side": side, "passes": base, "adaptive": 1, "mu": 1.0e-3}, "fp64")
+        a.step()
+        k = a.step_stats()["density_passes"]
+        assert base <= k <= base + 6
+        e = orc.C3Oracle({"side": side, "passes": k, "mu": 1.0e-3}, "fp64")
+        e.step()
+        assert np.array_equal(a.state()[0], e.state()[0]) and np.array_equal(a.state()[1], e.state()[1])
+
+
+def test_adaptive_passes_stop_at_the_tolerance_or_the_cap(orc):
+    # The workload's lattice is 2 % over-dense on average, so some constraint always exceeds 0.02
+    # and the count reaches the cap, base + 6; an under-dense lattice (ratio 0.9) stops right at the
+    # base; a ratio of 1.005 stops in between, and in fewer passes as the flow relaxes.
+    def counts(ratio, base, steps):
+        s = orc.C3Oracle({"side": 48, "passes": base, "adaptive": 1, "mu": 1.0e-3, "density_ratio": ratio}, "fp64")
+        out = []
+        for _ in range(steps):
+            s.step()
+            out.append(s.step_stats()["density_passes"])
+        return out
+    assert counts(1.02, 3, 2) == [9, 9]
+    assert counts(0.9, 3, 2) == [3, 3]
+    mid = counts(1.005, 1, 3)
+    assert all(1 < k < 7 for k in mid) and mid[-1] < mid[0]
+
+
+def test_gravity_adds_g_dt_to_every_velocity(orc):
+    g, dt = -9.81, 1.0 / 1440.0
+    s = orc.C3Oracle({"side": 32, "passes": 0, "mu": 0.0, "gravity": g}, "fp64")
+    _, v0 = s.state()
+    s.step()
+    _, v1 = s.state()
+    dv = v1 - v0
+    assert np.allclose(dv[:, 1], np.float32(g) * np.float32(dt), rtol=1e-6, atol=1e-12)
+    assert np.abs(dv[:, 0]).max() < 1e-12
+
+
+def test_non_finite_particles_restart_at_the_origin(orc):
+    s = orc.C3Oracle({"side": 32, "passes": 0, "mu": 0.0}, "fp64")
+    x, v = s.state()
+    x[3] = (np.nan, 0.01)
+    v[7] = (np.inf, 0.0)
+    x[11] = (-np.inf, np.nan)
+    s.set_state(x, v)
+    s.step()
+    x1, v1 = s.state()
+    for i in (3, 7, 11):
+        assert np.all(x1[i] == 0.0) and np.all(v1[i] == 0.0)
+    assert np.isfinite(x1).all() and np.isfinite(v1).all()
+
+
+def test_speed_clamp_and_kinetic_energies(orc):
+    vmax = 0.015
+    cfg = {"side": 32, "mu": 1.0e-3, "speed_clamp": vmax}
+    s = orc.C3Oracle(cfg, "fp64")
+    u = orc.C3Oracle(dict(cfg, speed_clamp=0.0), "fp64")
+    s.step()
+    u.step()
+    _, vs = s.state()
+    _, vu = u.state()
+    speed_u = np.linalg.norm(vu, axis=1)
+    assert np.linalg.norm(vs, axis=1).max() <= vmax * (1 + 1e-12)
+    below = speed_u <= vmax
+    assert below.any() and (~below).any()
+    assert np.array_equal(vs[below], vu[below])
+    assert np.allclose(vs[~below], vu[~below] * (vmax / speed_u[~below])[:, None], rtol=1e-12)
+    st = s.step_stats()
+    m = orc.c3_box(cfg)["mass"]
+    assert st["ke_after_clamp"] == pytest.approx(0.5 * m * (vs ** 2).sum(), rel=1e-12)
+    assert st["ke_after_cg"] == pytest.approx(0.5 * m * (vu ** 2).sum(), rel=1e-12)
+    assert st["ke_after_clamp"] < st["ke_after_cg"] < st["ke_before_cg"]
