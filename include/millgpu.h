@@ -277,6 +277,17 @@ int mg_grid_build_noalloc(const float* d_xy, uint32_t n, float cell_size, int64_
                           uint64_t cs_cap, uint32_t* d_indices, uint64_t idx_cap, int32_t* d_keys2, uint64_t keys_cap,
                           void* stream, char* err, size_t errlen);
 
+/* ---- on-demand metrics of drum states (metrics.rs) --------------------------------------------
+ * Replaces metrics::mixing_index (metrics.rs:506-576): the Lacey index of the dye field of n fluid
+ * particles (device xy, 2n floats; device dye, n floats) in a drum of the given radius, bit for bit
+ * the reference's f32 sums; *defined = 0 where the reference returns None. */
+int mg_mixing_index(const float* d_xy, const float* d_dye, uint32_t n, float radius, float* out, int* defined);
+/* The same for n_drums drums in one launch (one workgroup per drum): drum d holds particles
+ * [d_offsets[d], d_offsets[d + 1]) of d_xy / d_dye and radius d_radius[d]; results in device
+ * d_out[d] / d_ok[d]. */
+int mg_mixing_index_batch(const float* d_xy, const float* d_dye, const uint32_t* d_offsets, const float* d_radius,
+                          uint32_t n_drums, float* d_out, int32_t* d_ok, void* stream);
+
 /* ---- device memory of this library's HIP runtime ----------------------------------------------
  * No reference counterpart: plumbing for callers without a HIP runtime of their own (bindings,
  * tests). Device pointers of another HIP runtime loaded into the same process (for example one

@@ -436,6 +436,49 @@ template <class R> static Nbrs build_neighbor_lists(const std::vector<V2<R>>& x,
   return L;
 }
 
+// metrics::mixing_index (metrics.rs:506-576): Lacey index of the dye field on a 16 x 16 grid over
+// [-radius, radius]^2, count-weighted; f32 sums in particle order as the reference forms them.
+template <class R, class D>
+static bool mixing_index_of(const V2<R>* x, const D* dye, size_t n, R rad, float& out) {
+  if (n == 0 || rad <= (R)0) return false;
+  const int G = 16;
+  R cell = F(2.0) * rad / (R)G;
+  std::vector<R> ds_(G * G, 0);
+  std::vector<uint32_t> cnt(G * G, 0);
+  R tot = 0;
+  for (size_t i = 0; i < n; i++) {
+    int gi = std::min(std::max(sat_i32(std::floor((x[i].x + rad) / cell)), 0), G - 1);
+    int gj = std::min(std::max(sat_i32(std::floor((x[i].y + rad) / cell)), 0), G - 1);
+    size_t idx = (size_t)gj * G + gi;
+    ds_[idx] += (R)dye[i];
+    cnt[idx]++;
+    tot += (R)dye[i];
+  }
+  R pm = tot / (R)n;
+  std::vector<R> means;
+  std::vector<uint32_t> counts;
+  for (size_t q = 0; q < ds_.size(); q++)
+    if (cnt[q] > 0) { means.push_back(ds_[q] / (R)cnt[q]); counts.push_back(cnt[q]); }
+  if (means.size() < 2) return false;
+  uint32_t wsum_u = 0;
+  for (auto c : counts) wsum_u += c;
+  R ws = (R)wsum_u;
+  R ov = 0;
+  for (size_t q = 0; q < means.size(); q++) ov += (R)counts[q] * powi(means[q] - pm, 2);
+  ov = ov / ws;
+  R s0 = pm * (F(1.0) - pm);
+  R wa = 0;
+  for (auto c : counts) wa += (R)c * (R)c;
+  wa = wa / ws;
+  R sr = s0 / rmax(wa, (R)1);
+  if (s0 < F(1e-9)) return false;
+  R den = s0 - sr;
+  if (std::fabs(den) < F(1e-9)) { out = 1.0f; return true; }
+  R m = (s0 - ov) / den;
+  out = (float)rclamp(m, (R)0, (R)1);
+  return true;
+}
+
 // ------------------------------------------------------------------------------------------
 // pbf.rs:119-219 — kernels and boundary sampling.
 // ------------------------------------------------------------------------------------------
@@ -1783,46 +1826,8 @@ template <class R, class A> struct Sim {
   }
 
   // metrics.rs:506-576
-  bool mixing_index(float& out) const {
-    size_t n = fl.n();
-    R rad = (R)P.radius_m();
-    if (n == 0 || rad <= (R)0) return false;
-    const int G = 16;
-    R cell = F(2.0) * rad / (R)G;
-    std::vector<R> ds_(G * G, 0);
-    std::vector<uint32_t> cnt(G * G, 0);
-    R tot = 0;
-    for (size_t i = 0; i < n; i++) {
-      int gi = std::min(std::max(sat_i32(std::floor((fl.x[i].x + rad) / cell)), 0), G - 1);
-      int gj = std::min(std::max(sat_i32(std::floor((fl.x[i].y + rad) / cell)), 0), G - 1);
-      size_t idx = (size_t)gj * G + gi;
-      ds_[idx] += (R)fl.dye[i];
-      cnt[idx]++;
-      tot += (R)fl.dye[i];
-    }
-    R pm = tot / (R)n;
-    std::vector<R> means;
-    std::vector<uint32_t> counts;
-    for (size_t q = 0; q < ds_.size(); q++)
-      if (cnt[q] > 0) { means.push_back(ds_[q] / (R)cnt[q]); counts.push_back(cnt[q]); }
-    if (means.size() < 2) return false;
-    uint32_t wsum_u = 0;
-    for (auto c : counts) wsum_u += c;
-    R ws = (R)wsum_u;
-    R ov = 0;
-    for (size_t q = 0; q < means.size(); q++) ov += (R)counts[q] * powi(means[q] - pm, 2);
-    ov = ov / ws;
-    R s0 = pm * (F(1.0) - pm);
-    R wa = 0;
-    for (auto c : counts) wa += (R)c * (R)c;
-    wa = wa / ws;
-    R sr = s0 / rmax(wa, (R)1);
-    if (s0 < F(1e-9)) return false;
-    R den = s0 - sr;
-    if (std::fabs(den) < F(1e-9)) { out = 1.0f; return true; }
-    R m = (s0 - ov) / den;
-    out = (float)rclamp(m, (R)0, (R)1);
-    return true;
+  bool mixing_index(float& out) const {  // metrics.rs:506-576
+    return orc::mixing_index_of<R>(fl.x.data(), fl.dye.data(), fl.n(), (R)P.radius_m(), out);
   }
 };
 
@@ -2087,6 +2092,15 @@ int64_t orc_grid_near(const float* pts, size_t n, float cell, const float* q, si
     off[a + 1] = (uint32_t)k;
   }
   return k;
+}
+// metrics::mixing_index over caller points and dye (f32); returns 0 for None.
+int orc_mixing_index_of(const float* pts, const float* dye, size_t n, float radius, float* out) {
+  std::vector<V2<float>> p(n);
+  for (size_t i = 0; i < n; i++) p[i] = V2<float>(pts[2 * i], pts[2 * i + 1]);
+  float m = 0.0f;
+  const bool ok = mixing_index_of<float>(p.data(), dye, n, radius, m);
+  *out = m;
+  return ok ? 1 : 0;
 }
 // Candidate pairs in emission order (grid.rs:291-362).
 int64_t orc_grid_pairs(const float* pts, size_t n, float cell, uint32_t* out, int64_t cap) {

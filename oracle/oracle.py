@@ -62,6 +62,8 @@ def oracle_lib():
     lib.orc_grid_pairs.argtypes = [fp, C.c_size_t, C.c_float, u32p, C.c_int64]
     lib.orc_grid_keys.restype = C.c_int64
     lib.orc_grid_keys.argtypes = [fp, C.c_size_t, C.c_float, C.POINTER(C.c_int32), C.c_int64]
+    lib.orc_mixing_index_of.restype = C.c_int
+    lib.orc_mixing_index_of.argtypes = [fp, fp, C.c_size_t, C.c_float, fp]
     lib.orc_grid_near.restype = C.c_int64
     lib.orc_grid_near.argtypes = [fp, C.c_size_t, C.c_float, fp, C.c_size_t, u32p, u32p, C.c_int64]
     lib.orc_neighbor_lists.restype = C.c_int
@@ -307,6 +309,16 @@ def grid_build(points, cell: float):
     return {"dense": bool(meta[0]), "min_cx": int(meta[1]), "min_cy": int(meta[2]),
             "width": int(meta[3]), "height": int(meta[4]),
             "cell_start": cs[: meta[5]].copy(), "indices": idx[: meta[6]].copy()}
+
+
+def mixing_index_of(points, dye, radius: float):
+    """metrics::mixing_index (metrics.rs:506-576) of caller points and dye; None when undefined."""
+    lib = oracle_lib()
+    pts = np.ascontiguousarray(np.asarray(points, dtype=np.float32).reshape(-1))
+    d = np.ascontiguousarray(np.asarray(dye, dtype=np.float32).reshape(-1))
+    out = np.zeros(1, dtype=np.float32)
+    ok = lib.orc_mixing_index_of(_fp(pts), _fp(d), d.size, float(radius), _fp(out))
+    return float(out[0]) if ok else None
 
 
 def grid_keys(points, cell: float) -> np.ndarray:
