@@ -5,7 +5,8 @@ Headline metric (drum-substeps/s, batched FP32 vs plain): the drum substep engin
 in this build (an earlier form faulted on the GPU and the cause has not been shown), so the
 headline value is reported as null with the reason, never estimated.
 
-Measured on the GPU: the kernel-isolation workload of the same fluid hot path, a periodic box of
+Timed on the GPU (config.workload, ms_per_step, roofline and cpu_baseline all describe it): the
+kernel-isolation workload of the same fluid hot path, a periodic box of
 side x side PBF particles (default 1024^2 = 1,048,576): per step the grid build + cell reorder,
 four density passes, velocity update, density recompute and the implicit viscosity CG to 1e-6
 relative residual, FP32 state with FP64 CG reductions, state resident in HBM. Reported as
@@ -15,11 +16,14 @@ particle-steps/s and as algorithmic HBM GB/s from the canonical per-particle byt
 roofline: the dominant kernel by total time, its algorithmic bytes per launch divided by its
 average launch time from HIP events on the box's stream (density passes: the timed region's last
 launch relaunched back to back inside one event pair; CG kernels: one event pair per launch over
-K more steps); `traffic` (measured
-HBM bytes per launch) and `valu_busy` come from the committed rocprofv3 summaries in profiles/
-(the density passes are VALU-bound: their HBM fraction is low by nature).
-cpu_baseline: the C++ restatement of the reference (oracle/, 1 core) on a bounded sample: C2 drum
-substeps (headline unit) and, inside kernel_isolation, box steps.
+K more steps); `traffic` (HBM bytes per launch) and `valu_busy` are not measured in this process
+(hardware counters need a rocprofv3 --pmc run of their own): they are read from the committed
+summaries named in `traffic_source` / `valu_source` (the density passes are VALU-bound: their
+HBM fraction is low by nature).
+cpu_baseline: the C++ restatement of the reference (oracle/, 1 core) on a bounded sample of the
+same box workload; cpu_baseline_drum: the same restatement on C2 drum substeps (the headline unit,
+context only while drums do not step on the GPU). kernel_isolation.grid: the reference-layout
+uniform grid build (grid.rs:101-153) over the box's 1,048,576 initial points.
 """
 from __future__ import annotations
 
@@ -189,6 +193,39 @@ def bench_bingham(args, cfg, yield_stress=1.0) -> dict:
     return out
 
 
+def bench_grid(args) -> dict:
+    """The reference-layout uniform grid (grid.rs:101-153) over the box's initial points at cell h:
+    the build (one host wait for the layout, then the radix-sorted cell order and cell starts) and
+    the neighbour lists (pbf.rs:262-288), each timed over K calls after one warm-up call.
+    Algorithmic bytes of a build: the points read twice (8 + 8 B) and the indices written (4 B) per
+    point, plus 4 B per cell start."""
+    import numpy as np
+    from milldynamics3_amd.box import Box
+    from milldynamics3_amd.grid import UniformGrid
+    b = Box({"side": args.side})
+    x, _ = b.state()
+    b.close()
+    h = float(np.float32(2.0e-3))
+    g = UniformGrid(len(x))
+    g.build(x, h)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        g._check(g._L.mg_grid_build(g._h, g.points.ptr, g.n, h, None), "mg_grid_build")
+    build_ms = 1000.0 * (time.perf_counter() - t0) / args.steps
+    md = g.meta()
+    g.neighbor_lists(h)
+    t0 = time.perf_counter()
+    for _ in range(max(args.steps // 4, 1)):
+        off, nb = g.neighbor_lists(h)
+    nbr_ms = 1000.0 * (time.perf_counter() - t0) / max(args.steps // 4, 1)
+    bytes_build = len(x) * 20.0 + 4.0 * md["n_cell_start"]
+    return {"points": len(x), "cell": h, "dense": md["dense"], "cells": md["n_cell_start"] - 1,
+            "build_ms": round(build_ms, 4), "points_per_s": round(len(x) / (build_ms * 1e-3), 1),
+            "build_gbs_algorithmic": round(bytes_build / (build_ms * 1e-3) / 1e9, 1),
+            "neighbor_lists_ms_incl_copy": round(nbr_ms, 3), "edges": int(len(nb)),
+            "timing": "host wall clock around each synchronous call (a build waits once for its layout)"}
+
+
 def precision_vs_fp64(cfg, steps) -> dict:
     """Error of the FP32-state box's outputs against the FP64-state box after the same `steps` steps
     from the same initial state (the FP64 box is the oracle's f64 policy on the GPU): positions in
@@ -311,7 +348,11 @@ def main() -> int:
     roofline = {"bound": "hbm", "kernel": CLASS_KERNELS[k], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": scaled(measured_traffic(CLASS_KERNELS[k], args.side), work_frac),
+                "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT) + " (rocprofv3 FETCH_SIZE/WRITE_SIZE "
+                                  "per dispatch, copy-calibrated; not measured in this run)",
                 "valu_busy": measured_valu_busy(CLASS_KERNELS[k], args.side),
+                "valu_source": os.path.relpath(VALU_PROFILE, ROOT) + " (not measured in this run)",
+                "workload": f"periodic PBF box {args.side}x{args.side}",
                 "bytes_per_launch": round(bytes_per_launch, 1), "avg_launch_us": round(1000.0 * avg_launch_ms, 2),
                 "launches": replay_reps or int(launches_k[k]),
                 "timing": "back-to-back relaunches, one event pair" if replay_reps else "event pair per launch",
@@ -329,10 +370,11 @@ def main() -> int:
         iso["fp64"] = bench_fp64(args, cfg)
     if not args.no_bingham:
         iso["bingham"] = bench_bingham(args, cfg)
-    headline_cpu = None
+    iso["grid"] = bench_grid(args)
+    box_cpu, drum_cpu = None, None
     if not args.no_cpu:
-        headline_cpu = cpu_baseline_drum(args.cpu_seconds)
-        iso["cpu_baseline"] = cpu_baseline_box(cfg, args.cpu_seconds)
+        box_cpu = cpu_baseline_box(cfg, args.cpu_seconds)
+        drum_cpu = cpu_baseline_drum(args.cpu_seconds)
         cores = min(16, os.cpu_count() or 1)  # a one-GPU box's share of the host
         if cores > 1:
             iso["cpu_baseline_all_cores"] = cpu_baseline_box_cores(cfg, min(args.cpu_seconds, 5.0), cores)
@@ -343,18 +385,23 @@ def main() -> int:
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": None,
+        "ms_per_step": round(ms_per_step, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "C2 single large drum (2000 discs, resolution 100): not measured",
-                   "status": "drum substep engine switched off in this build (earlier form faulted on the "
-                             "GPU; cause not shown); measured workload: kernel_isolation",
+        "config": {"workload": f"kernel isolation: periodic PBF box {args.side}x{args.side} ({n} particles), "
+                               f"4 density passes + viscosity CG to 1e-6 |b|, mu {args.mu} Pa s, f32 state with "
+                               "f64 CG reductions, inputs resident in HBM",
+                   "timed": f"{args.steps} box steps in one call; ms_per_step is one box step",
+                   "headline_status": "value is null: the headline unit is drum substeps and the drum substep "
+                                      "engine is switched off in this build (an earlier form faulted on the GPU; "
+                                      "cause not shown)",
                    "drum_engine_enabled": bool(drum_engine_enabled())},
         "roofline": roofline,
-        "cpu_baseline": headline_cpu,
+        "cpu_baseline": box_cpu,
+        "cpu_baseline_drum": drum_cpu,
         "kernel_isolation": iso,
     }
     print(json.dumps(line), flush=True)
