@@ -23,7 +23,10 @@ HBM fraction is low by nature).
 cpu_baseline: the C++ restatement of the reference (oracle/, 1 core) on a bounded sample of the
 same box workload; cpu_baseline_drum: the same restatement on C2 drum substeps (the headline unit,
 context only while drums do not step on the GPU). kernel_isolation.grid: the reference-layout
-uniform grid build (grid.rs:101-153) over the box's 1,048,576 initial points.
+uniform grid build (grid.rs:101-153) over the box's 1,048,576 initial points;
+kernel_isolation.mixing_index: the Lacey mixing index (metrics.rs:506-576) of 1024 synthetic
+C2-sized drum states in one launch; each with the restatement's rate beside it.
+--part grid / --part mixing time only that operator (for a profiler run).
 """
 from __future__ import annotations
 
@@ -91,6 +94,15 @@ def cpu_count_used() -> int:
     return 1
 
 
+def sync():
+    """Both HIP runtimes of the process: torch's and the library's own (the box, grid and metrics
+    kernels run on the library's streams)."""
+    import torch
+    from milldynamics3_amd.grid import synchronize
+    synchronize()
+    torch.cuda.synchronize()
+
+
 def cpu_baseline_drum(seconds: float) -> dict:
     from oracle import oracle as orc
     s = orc.OracleSim(C2_PARAMS, "ref")
@@ -153,11 +165,11 @@ def bench_fp64(args, cfg) -> dict:
     box = Box(dict(cfg, fp64=1))
     n = box.n
     box.step(args.warmup)
-    torch.cuda.synchronize()
+    sync()
     cg0 = box.total_cg_iterations
     t0 = time.perf_counter()
     box.step(args.steps)
-    torch.cuda.synchronize()
+    sync()
     wall = time.perf_counter() - t0
     avg_cg = (box.total_cg_iterations - cg0) / args.steps
     box.close()
@@ -179,11 +191,11 @@ def bench_bingham(args, cfg, yield_stress=1.0) -> dict:
     box = Box(dict(cfg, yield_stress=yield_stress))
     n = box.n
     box.step(args.warmup)
-    torch.cuda.synchronize()
+    sync()
     cg0 = box.total_cg_iterations
     t0 = time.perf_counter()
     box.step(args.steps)
-    torch.cuda.synchronize()
+    sync()
     wall = time.perf_counter() - t0
     out = {"metric": "particle-steps/s", "value": round(n * args.steps / wall, 1),
            "ms_per_step": round(1000.0 * wall / args.steps, 4), "yield_stress_pa": yield_stress,
@@ -193,12 +205,13 @@ def bench_bingham(args, cfg, yield_stress=1.0) -> dict:
     return out
 
 
-def bench_grid(args) -> dict:
+def bench_grid(args, cpu=True) -> dict:
     """The reference-layout uniform grid (grid.rs:101-153) over the box's initial points at cell h:
     the build (one host wait for the layout, then the radix-sorted cell order and cell starts) and
     the neighbour lists (pbf.rs:262-288), each timed over K calls after one warm-up call.
     Algorithmic bytes of a build: the points read twice (8 + 8 B) and the indices written (4 B) per
-    point, plus 4 B per cell start."""
+    point, plus 4 B per cell start. cpu_baseline: the oracle's restatement of the same build
+    (1 core) on the same points."""
     import numpy as np
     from milldynamics3_amd.box import Box
     from milldynamics3_amd.grid import UniformGrid
@@ -208,9 +221,11 @@ def bench_grid(args) -> dict:
     h = float(np.float32(2.0e-3))
     g = UniformGrid(len(x))
     g.build(x, h)
+    sync()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         g._check(g._L.mg_grid_build(g._h, g.points.ptr, g.n, h, None), "mg_grid_build")
+    sync()
     build_ms = 1000.0 * (time.perf_counter() - t0) / args.steps
     md = g.meta()
     g.neighbor_lists(h)
@@ -218,12 +233,78 @@ def bench_grid(args) -> dict:
     for _ in range(max(args.steps // 4, 1)):
         off, nb = g.neighbor_lists(h)
     nbr_ms = 1000.0 * (time.perf_counter() - t0) / max(args.steps // 4, 1)
+    g.close()
     bytes_build = len(x) * 20.0 + 4.0 * md["n_cell_start"]
-    return {"points": len(x), "cell": h, "dense": md["dense"], "cells": md["n_cell_start"] - 1,
-            "build_ms": round(build_ms, 4), "points_per_s": round(len(x) / (build_ms * 1e-3), 1),
-            "build_gbs_algorithmic": round(bytes_build / (build_ms * 1e-3) / 1e9, 1),
-            "neighbor_lists_ms_incl_copy": round(nbr_ms, 3), "edges": int(len(nb)),
-            "timing": "host wall clock around each synchronous call (a build waits once for its layout)"}
+    out = {"points": len(x), "cell": h, "dense": md["dense"], "cells": md["n_cell_start"] - 1,
+           "build_ms": round(build_ms, 4), "points_per_s": round(len(x) / (build_ms * 1e-3), 1),
+           "build_gbs_algorithmic": round(bytes_build / (build_ms * 1e-3) / 1e9, 1),
+           "neighbor_lists_ms_incl_copy": round(nbr_ms, 3), "edges": int(len(nb)),
+           "timing": "host wall clock around K synchronous builds (a build waits once for its layout)"}
+    if cpu:
+        from oracle import oracle as orc
+        orc.grid_build(x, h)
+        reps, t0 = 0, time.perf_counter()
+        while reps == 0 or time.perf_counter() - t0 < min(args.cpu_seconds, 3.0):
+            orc.grid_build(x, h)
+            reps += 1
+        cpu_ms = 1000.0 * (time.perf_counter() - t0) / reps
+        out["cpu_baseline"] = {"value": round(len(x) / (cpu_ms * 1e-3), 1), "unit": "points/s", "cores": 1,
+                               "kind": "port", "build_ms": round(cpu_ms, 3),
+                               "sample": f"{reps} builds of the same {len(x)} points on the C++ restatement "
+                                         "of UniformGrid::build, 1 core"}
+    return out
+
+
+def bench_mixing(args, drums=1024, per_drum=12697, cpu=True) -> dict:
+    """metrics::mixing_index (metrics.rs:506-576) over a batch of drums in one launch: synthetic
+    drum states (uniform points in a disc of the C2 drum's radius, dye 0/1 by half-plane with 10 %
+    flipped), `per_drum` fluid particles each (C2's fluid count). Algorithmic bytes: position and
+    dye read once (12 B per particle)."""
+    import numpy as np
+    from milldynamics3_amd import metrics
+    from milldynamics3_amd.grid import DeviceArray
+    rng = np.random.default_rng(5)
+    R = np.float32(0.1397 / 2)
+    n = drums * per_drum
+    r = R * np.sqrt(rng.random(n, dtype=np.float32))
+    a = np.float32(2 * np.pi) * rng.random(n, dtype=np.float32)
+    pts = np.stack([r * np.cos(a), r * np.sin(a)], 1).astype(np.float32)
+    dye = (pts[:, 0] < 0).astype(np.float32)
+    flip = rng.random(n) < 0.1
+    dye[flip] = 1.0 - dye[flip]
+    off = (np.arange(drums + 1, dtype=np.uint64) * per_drum).astype(np.uint32)
+    rad = np.full(drums, R, dtype=np.float32)
+    L = metrics._lib()
+    dx, dd, doff, drad = (DeviceArray.from_numpy(v) for v in (pts, dye, off, rad))
+    dout, dok = DeviceArray(4 * drums), DeviceArray(4 * drums)
+
+    def launch():
+        rc = L.mg_mixing_index_batch(dx.ptr, dd.ptr, doff.ptr, drad.ptr, drums, dout.ptr, dok.ptr, None)
+        if rc != 0:
+            raise RuntimeError(f"mg_mixing_index_batch failed ({rc})")
+    launch()
+    sync()
+    reps = max(args.steps, 1)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        launch()
+    sync()
+    ms = 1000.0 * (time.perf_counter() - t0) / reps
+    out = {"drums": drums, "particles_per_drum": per_drum, "batch_ms": round(ms, 4),
+           "drums_per_s": round(drums / (ms * 1e-3), 1),
+           "gbs_algorithmic": round(12.0 * n / (ms * 1e-3) / 1e9, 1),
+           "timing": f"host wall clock around {reps} back-to-back batch launches"}
+    if cpu:
+        from oracle import oracle as orc
+        k, t0 = 0, time.perf_counter()
+        while k == 0 or (time.perf_counter() - t0 < min(args.cpu_seconds, 2.0) and k < drums):
+            s0 = k * per_drum
+            orc.mixing_index_of(pts[s0:s0 + per_drum], dye[s0:s0 + per_drum], float(R))
+            k += 1
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(k / dt, 1), "unit": "drums/s", "cores": 1, "kind": "port",
+                               "sample": f"{k} drums of the same batch on the C++ restatement, 1 core"}
+    return out
 
 
 def precision_vs_fp64(cfg, steps) -> dict:
@@ -269,6 +350,8 @@ def main() -> int:
     ap.add_argument("--no-bingham", action="store_true", help="skip the Bingham-fluid box measurement")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one plain device copy of this size (calibrates PMC byte counters)")
+    ap.add_argument("--part", choices=("all", "grid", "mixing"), default="all",
+                    help="grid / mixing: time only that operator (for a profiler run) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -292,6 +375,10 @@ def main() -> int:
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    if args.part != "all":
+        part = bench_grid(args, cpu=not args.no_cpu) if args.part == "grid" else bench_mixing(args, cpu=not args.no_cpu)
+        print(json.dumps({"part": args.part, **part}), flush=True)
+        return 0
     cfg = {"side": args.side, "mu": args.mu}
     box = Box(cfg)
     n = box.n
@@ -306,11 +393,11 @@ def main() -> int:
     k = int(np.argmax(ms_all[1:8])) + 1  # grid build is several launches together: not a single kernel
     # 2) the timed region: K steps, no per-kernel events
     box.set_timing(0)
-    torch.cuda.synchronize()
+    sync()
     cg0 = box.total_cg_iterations
     t0 = time.perf_counter()
     box.step(args.steps)  # K steps in one call (the library waits for each step's CG to converge)
-    torch.cuda.synchronize()
+    sync()
     wall = time.perf_counter() - t0
     n_cg = box.total_cg_iterations - cg0
     # 3) the same K steps again with HIP events around the dominant kernel only (its average launch)
@@ -370,7 +457,8 @@ def main() -> int:
         iso["fp64"] = bench_fp64(args, cfg)
     if not args.no_bingham:
         iso["bingham"] = bench_bingham(args, cfg)
-    iso["grid"] = bench_grid(args)
+    iso["grid"] = bench_grid(args, cpu=not args.no_cpu)
+    iso["mixing_index"] = bench_mixing(args, cpu=not args.no_cpu)
     box_cpu, drum_cpu = None, None
     if not args.no_cpu:
         box_cpu = cpu_baseline_box(cfg, args.cpu_seconds)

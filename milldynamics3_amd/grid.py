@@ -39,8 +39,15 @@ def _lib():
     L.mg_device_free.argtypes = [P]
     L.mg_memcpy_htod.argtypes = [P, P, C.c_size_t]
     L.mg_memcpy_dtoh.argtypes = [P, P, C.c_size_t]
+    L.mg_device_synchronize.argtypes = []
     _ready.add(id(L))
     return L
+
+
+def synchronize():
+    """Waits for all work of the library's own HIP runtime on its current device (the library's
+    streams are not torch's: ``torch.cuda.synchronize()`` does not wait for them)."""
+    _sim._check(_lib().mg_device_synchronize(), what="mg_device_synchronize")
 
 
 class DeviceArray:
