@@ -288,6 +288,60 @@ int mg_mixing_index(const float* d_xy, const float* d_dye, uint32_t n, float rad
 int mg_mixing_index_batch(const float* d_xy, const float* d_dye, const uint32_t* d_offsets, const float* d_radius,
                           uint32_t n_drums, float* d_out, int32_t* d_ok, void* stream);
 
+/* One drum's scalars for mg_drum_metrics_batch: the Drum (geometry.rs:18-22, LiftersParams), the
+ * drum angle, the ball population's shared radius / mass / inertia (dem.rs:277-309), the fluid's
+ * smoothing length and rest density (pbf.rs:1018-1028), and the drum's counts. */
+typedef struct mg_metrics_drum {
+  float radius_m, omega, drum_angle;
+  uint32_t lifter_count;
+  float lifter_height_m, lifter_base_width_m, lifter_top_width_m, lifter_phase_deg;
+  float ball_radius, ball_mass, ball_inertia;
+  float h, rest_density;
+  uint32_t n_balls, n_fluid;
+} mg_metrics_drum;
+
+/* metrics::compute's state-derived fields (metrics.rs:78-189 Metrics, in its field order); each
+ * has_* = 0 where the reference returns None. */
+typedef struct mg_drum_metrics {
+  float toe_angle_rad, shoulder_angle_rad;
+  int32_t has_toe_shoulder;
+  float centroid_x, centroid_y;
+  int32_t has_centroid;
+  float pool_angle_min_rad, pool_angle_max_rad;
+  int32_t has_pool_extent;
+  float free_surface_angle_rad, free_surface_offset_m;
+  int32_t has_free_surface;
+  float pool_depth_m;
+  int32_t has_pool_depth;
+  float mixing_index;
+  int32_t has_mixing_index;
+  float total_kinetic_energy_j, max_ball_overlap_fraction, max_ball_wall_overlap_fraction;
+  float max_fluid_density_error_fraction, mean_fluid_density_error_fraction;
+  float max_fluid_compression_error_fraction, mean_fluid_compression_error_fraction;
+  int32_t has_density_errors;
+  float drum_omega_rad_s;
+  uint32_t fluid_particle_count;
+} mg_drum_metrics;
+
+/* Replaces metrics::compute (metrics.rs:195-254) for n_drums drum states at once: toe/shoulder
+ * (:327-357), charge centroid (:362-369), pool angular extent (:377-401), free-surface line
+ * (:414-453), pool depth (:463-476), mixing index (:506-576), ball kinetic energy (:581-590), ball
+ * and ball-wall overlaps (:599-634), density and compression errors (:649-710). Drum d's balls and
+ * fluid follow drum d-1's in the device arrays (d_ball_x / d_ball_v: 2 floats per ball,
+ * d_ball_omega: 1; d_fluid_x: 2 per particle, d_fluid_dye / d_fluid_density: 1); drums and out are
+ * host arrays of n_drums records. Results equal the reference's f32 arithmetic bit for bit (sums
+ * in the reference's order, wall angles by the host libm's atan2f algorithm); the free-surface
+ * angle and offset use the host libm's atan2f / sinf / cosf. MG_E_ARG where the reference panics:
+ * a non-finite position among at least 6 wall-adjacent balls or 2 wall-adjacent fluid particles
+ * (its angle sort unwraps a NaN comparison); err names the drum. */
+int mg_drum_metrics_batch(const mg_metrics_drum* drums, uint32_t n_drums, const float* d_ball_x, const float* d_ball_v,
+                          const float* d_ball_omega, const float* d_fluid_x, const float* d_fluid_dye,
+                          const float* d_fluid_density, mg_drum_metrics* out, char* err, size_t errlen);
+/* The library's atan2f (the host libm's algorithm, used on the device for wall angles), on the
+ * host over n pairs. No reference counterpart: exported so callers can check it against their
+ * libm. */
+void mg_atan2f_ref(const float* y, const float* x, float* out, uint64_t n);
+
 /* ---- device memory of this library's HIP runtime ----------------------------------------------
  * No reference counterpart: plumbing for callers without a HIP runtime of their own (bindings,
  * tests). Device pointers of another HIP runtime loaded into the same process (for example one

@@ -480,6 +480,201 @@ static bool mixing_index_of(const V2<R>* x, const D* dye, size_t n, R rad, float
 }
 
 // ------------------------------------------------------------------------------------------
+// metrics.rs:195-710 — metrics::compute over a drum state (f32 throughout, as the reference).
+// Angles are p.y.atan2(p.x) from the host libm, as the reference's f32::atan2.
+// ------------------------------------------------------------------------------------------
+struct DrumMetrics {
+  float toe = 0, shoulder = 0;
+  bool has_toe_shoulder = false;
+  float cx = 0, cy = 0;
+  bool has_centroid = false;
+  float pool_lo = 0, pool_hi = 0;
+  bool has_pool = false;
+  float fs_angle = 0, fs_offset = 0;
+  bool has_fs = false;
+  float depth = 0;
+  bool has_depth = false;
+  float mixing = 0;
+  bool has_mixing = false;
+  float ke = 0, max_overlap = 0, max_wall_overlap = 0;
+  float max_derr = 0, mean_derr = 0, max_cerr = 0, mean_cerr = 0;
+  bool has_errors = false;
+  float omega = 0;
+  uint32_t n_fluid = 0;
+  bool non_finite_angle = false;  // the reference's sort_by(partial_cmp().unwrap()) panics here
+};
+
+// The world angle of p in [0, 2 pi): p.y.atan2(p.x).rem_euclid(TAU) (metrics.rs:341).
+static inline float wall_angle(V2<float> p) { return rem_euclid(std::atan2(p.y, p.x), C<float>::TAU); }
+
+// metrics.rs:293-308: bounds of the cluster across the largest angular gap of sorted angles
+// (n >= 2): (cluster_start, cluster_end, max_gap).
+static void largest_gap_cluster_bounds(const std::vector<float>& a, float& start, float& end, float& gap) {
+  const size_t n = a.size();
+  float max_gap = a[0] + C<float>::TAU - a[n - 1];
+  size_t k = n - 1;
+  for (size_t i = 0; i + 1 < n; i++) {
+    const float g = a[i + 1] - a[i];
+    if (g > max_gap) {
+      max_gap = g;
+      k = i;
+    }
+  }
+  end = a[k];
+  start = a[(k + 1) % n];
+  gap = max_gap;
+}
+
+// Sorts finite angles ascending; false (the reference would panic) when one is NaN.
+static bool sort_angles(std::vector<float>& a) {
+  for (float v : a)
+    if (std::isnan(v)) return false;
+  std::sort(a.begin(), a.end());
+  return true;
+}
+
+static DrumMetrics drum_metrics(const Drum<float>& drum, float angle, const V2<float>* bx, const V2<float>* bv,
+                                const float* bw, size_t nb, float br, float bmass, float binertia, const V2<float>* fx,
+                                const float* dye, const float* dens, size_t nf, float h, float rest) {
+  DrumMetrics m;
+  const float TAU = C<float>::TAU, PI = C<float>::PI;
+  m.omega = drum.omega;
+  m.n_fluid = (uint32_t)nf;
+  // charge_toe_shoulder (metrics.rs:327-357)
+  if (nb > 0) {
+    const float margin = 2.5f * br;
+    std::vector<float> ang;
+    for (size_t i = 0; i < nb; i++)
+      if (drum.sdf_world(bx[i], angle).first < margin) ang.push_back(wall_angle(bx[i]));
+    if (ang.size() >= 6) {
+      if (!sort_angles(ang)) {
+        m.non_finite_angle = true;
+      } else {
+        float st, en, gap;
+        largest_gap_cluster_bounds(ang, st, en, gap);
+        if (gap > PI * 0.3f) {
+          m.has_toe_shoulder = true;
+          m.toe = drum.omega >= 0.0f ? st : en;
+          m.shoulder = drum.omega >= 0.0f ? en : st;
+        }
+      }
+    }
+  }
+  // charge_centroid (metrics.rs:362-369)
+  if (nb > 0) {
+    V2<float> sum;
+    for (size_t i = 0; i < nb; i++) sum = sum + bx[i];
+    m.has_centroid = true;
+    m.cx = sum.x / (float)nb;
+    m.cy = sum.y / (float)nb;
+  }
+  // slurry_pool_angular_extent (metrics.rs:377-401)
+  if (nf > 0) {
+    const float margin = 1.5f * h;
+    std::vector<float> ang;
+    for (size_t i = 0; i < nf; i++)
+      if (drum.sdf_world(fx[i], angle).first < margin) ang.push_back(wall_angle(fx[i]));
+    if (ang.size() == 1) {  // returned as is, without the sort
+      m.has_pool = true;
+      m.pool_lo = m.pool_hi = ang[0];
+    } else if (ang.size() > 1) {
+      if (!sort_angles(ang)) {
+        m.non_finite_angle = true;
+      } else {
+        float st, en, gap;
+        largest_gap_cluster_bounds(ang, st, en, gap);
+        m.has_pool = true;
+        m.pool_lo = st;
+        m.pool_hi = en;
+      }
+    }
+  }
+  // fluid_free_surface_line (metrics.rs:414-453)
+  if (nf > 0) {
+    const float margin = 1.5f * h;
+    std::vector<V2<float>> pts;
+    for (size_t i = 0; i < nf; i++)
+      if (drum.sdf_world(fx[i], angle).first >= margin) pts.push_back(fx[i]);
+    if (pts.size() >= 3) {
+      const float n = (float)pts.size();
+      V2<float> sum;
+      for (auto& p : pts) sum = sum + p;
+      const V2<float> c(sum.x / n, sum.y / n);
+      float cxx = 0, cyy = 0, cxy = 0;
+      for (auto& p : pts) {
+        const V2<float> d = p - c;
+        cxx += d.x * d.x;
+        cyy += d.y * d.y;
+        cxy += d.x * d.y;
+      }
+      cxx /= n;
+      cyy /= n;
+      cxy /= n;
+      const float a = 0.5f * std::atan2(2.0f * cxy, cxx - cyy);
+      const float sa = std::sin(a), ca = std::cos(a);
+      m.has_fs = true;
+      m.fs_angle = a;
+      m.fs_offset = c.x * -sa + c.y * ca;
+    }
+  }
+  // slurry_pool_depth_at_bottom (metrics.rs:463-476); angular_separation (metrics.rs:282-285)
+  if (nf > 0) {
+    const float down = 3.0f * (PI * 0.5f);
+    const float band = 5.0f * PI / 180.0f;
+    bool any = false;
+    float mn = 0;
+    for (size_t i = 0; i < nf; i++) {
+      const float a = wall_angle(fx[i]);
+      const float d = rem_euclid(a - down, TAU);
+      if (rmin(d, TAU - d) <= band) {
+        const float dist = fx[i].len();
+        mn = any ? rmin(mn, dist) : dist;
+        any = true;
+      }
+    }
+    if (any) {
+      m.has_depth = true;
+      m.depth = drum.radius - mn;
+    }
+  }
+  // mixing_index (metrics.rs:506-576)
+  m.has_mixing = mixing_index_of<float>(fx, dye, nf, drum.radius, m.mixing);
+  // total_kinetic_energy_j (metrics.rs:581-590)
+  for (size_t i = 0; i < nb; i++) m.ke += 0.5f * bmass * bv[i].len2() + 0.5f * binertia * bw[i] * bw[i];
+  // max_ball_overlap_fraction (metrics.rs:599-613), over the grid's candidate pairs
+  if (nb >= 2 && br > 0.0f) {
+    std::vector<V2<float>> pts(bx, bx + nb);
+    Grid<float> g;
+    g.build(pts, rmax(2.0f * br * 1.05f, 1e-6f));
+    g.candidate_pairs([&](uint32_t i, uint32_t j) {
+      const float dist = (bx[i] - bx[j]).len();
+      const float ov = rmax(2.0f * br - dist, 0.0f);
+      m.max_overlap = rmax(m.max_overlap, ov / br);
+    });
+  }
+  // max_ball_wall_overlap_fraction (metrics.rs:621-634)
+  if (nb > 0 && br > 0.0f)
+    for (size_t i = 0; i < nb; i++)
+      m.max_wall_overlap = rmax(m.max_wall_overlap, rmax(br - drum.sdf_world(bx[i], angle).first, 0.0f) / br);
+  // density and compression errors (metrics.rs:649-710)
+  if (nf > 0 && rest > 0.0f) {
+    m.has_errors = true;
+    float se = 0, sc = 0;
+    for (size_t i = 0; i < nf; i++) {
+      const float e = std::fabs(dens[i] - rest) / rest;
+      const float c = rmax(dens[i] / rest - 1.0f, 0.0f);
+      m.max_derr = rmax(m.max_derr, e);
+      m.max_cerr = rmax(m.max_cerr, c);
+      se += e;
+      sc += c;
+    }
+    m.mean_derr = se / (float)nf;
+    m.mean_cerr = sc / (float)nf;
+  }
+  return m;
+}
+
+// ------------------------------------------------------------------------------------------
 // pbf.rs:119-219 — kernels and boundary sampling.
 // ------------------------------------------------------------------------------------------
 template <class R> static R poly6(R r2, R h) {
@@ -2101,6 +2296,28 @@ int orc_mixing_index_of(const float* pts, const float* dye, size_t n, float radi
   const bool ok = mixing_index_of<float>(p.data(), dye, n, radius, m);
   *out = m;
   return ok ? 1 : 0;
+}
+// metrics::compute (metrics.rs:195-254) of caller arrays. drum7 = radius, omega, lifter count,
+// height, base width, top width, phase (deg). out[26]: see oracle.py DRUM_METRICS_KEYS.
+// Returns 0 where the reference would panic (a NaN angle in a sorted wall list).
+int orc_drum_metrics(const double* drum7, float angle, const float* bx, const float* bv, const float* bw, size_t nb,
+                     float br, float bmass, float binertia, const float* fx, const float* dye, const float* dens,
+                     size_t nf, float h, float rest, double* out) {
+  Drum<float> d;
+  d.radius = (float)drum7[0]; d.omega = (float)drum7[1]; d.lifter_count = (uint32_t)drum7[2];
+  d.l_height = (float)drum7[3]; d.l_base = (float)drum7[4]; d.l_top = (float)drum7[5]; d.l_phase_deg = (float)drum7[6];
+  std::vector<V2<float>> b(nb), v(nb), f(nf);
+  for (size_t i = 0; i < nb; i++) { b[i] = V2<float>(bx[2 * i], bx[2 * i + 1]); v[i] = V2<float>(bv[2 * i], bv[2 * i + 1]); }
+  for (size_t i = 0; i < nf; i++) f[i] = V2<float>(fx[2 * i], fx[2 * i + 1]);
+  const DrumMetrics m = drum_metrics(d, angle, b.data(), v.data(), bw, nb, br, bmass, binertia, f.data(), dye, dens, nf,
+                                     h, rest);
+  const double o[26] = {m.toe, m.shoulder, (double)m.has_toe_shoulder, m.cx, m.cy, (double)m.has_centroid,
+                        m.pool_lo, m.pool_hi, (double)m.has_pool, m.fs_angle, m.fs_offset, (double)m.has_fs,
+                        m.depth, (double)m.has_depth, m.mixing, (double)m.has_mixing, m.ke, m.max_overlap,
+                        m.max_wall_overlap, m.max_derr, m.mean_derr, m.max_cerr, m.mean_cerr, (double)m.has_errors,
+                        m.omega, (double)m.n_fluid};
+  std::memcpy(out, o, sizeof o);
+  return m.non_finite_angle ? 0 : 1;
 }
 // Candidate pairs in emission order (grid.rs:291-362).
 int64_t orc_grid_pairs(const float* pts, size_t n, float cell, uint32_t* out, int64_t cap) {

@@ -64,6 +64,9 @@ def oracle_lib():
     lib.orc_grid_keys.argtypes = [fp, C.c_size_t, C.c_float, C.POINTER(C.c_int32), C.c_int64]
     lib.orc_mixing_index_of.restype = C.c_int
     lib.orc_mixing_index_of.argtypes = [fp, fp, C.c_size_t, C.c_float, fp]
+    lib.orc_drum_metrics.restype = C.c_int
+    lib.orc_drum_metrics.argtypes = [dp, C.c_float, fp, fp, fp, C.c_size_t, C.c_float, C.c_float, C.c_float,
+                                     fp, fp, fp, C.c_size_t, C.c_float, C.c_float, dp]
     lib.orc_grid_near.restype = C.c_int64
     lib.orc_grid_near.argtypes = [fp, C.c_size_t, C.c_float, fp, C.c_size_t, u32p, u32p, C.c_int64]
     lib.orc_neighbor_lists.restype = C.c_int
@@ -390,6 +393,38 @@ def sdf_world(drum, p, angle):
     out = np.zeros(3)
     lib.orc_sdf_world(_dp(drum), float(p[0]), float(p[1]), float(angle), _dp(out))
     return float(out[0]), out[1:].copy()
+
+
+# orc_drum_metrics output slots (metrics.rs:78-189 field order, None as a has_* flag)
+DRUM_METRICS_KEYS = ("toe_angle_rad", "shoulder_angle_rad", "has_toe_shoulder", "centroid_x", "centroid_y",
+                     "has_centroid", "pool_angle_min_rad", "pool_angle_max_rad", "has_pool_extent",
+                     "free_surface_angle_rad", "free_surface_offset_m", "has_free_surface", "pool_depth_m",
+                     "has_pool_depth", "mixing_index", "has_mixing_index", "total_kinetic_energy_j",
+                     "max_ball_overlap_fraction", "max_ball_wall_overlap_fraction",
+                     "max_fluid_density_error_fraction", "mean_fluid_density_error_fraction",
+                     "max_fluid_compression_error_fraction", "mean_fluid_compression_error_fraction",
+                     "has_density_errors", "drum_omega_rad_s", "fluid_particle_count")
+
+
+def _f32(a, cols=None):
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+    return a.reshape(-1, cols) if cols else a.reshape(-1)
+
+
+def drum_metrics(drum, angle, ball_x, ball_v, ball_omega, ball_radius, ball_mass, ball_inertia,
+                 fluid_x, dye, density, h, rest_density):
+    """metrics::compute (metrics.rs:195-254) of caller arrays: a dict keyed by DRUM_METRICS_KEYS
+    (f32 values), or ValueError where the reference panics (NaN angle in a sorted wall list)."""
+    lib = oracle_lib()
+    bx, bv, bw = _f32(ball_x, 2), _f32(ball_v, 2), _f32(ball_omega)
+    fx, dy, de = _f32(fluid_x, 2), _f32(dye), _f32(density)
+    out = np.zeros(len(DRUM_METRICS_KEYS))
+    ok = lib.orc_drum_metrics(_dp(np.asarray(drum, np.float64)), float(angle), _fp(bx), _fp(bv), _fp(bw), len(bx),
+                              float(ball_radius), float(ball_mass), float(ball_inertia), _fp(fx), _fp(dy), _fp(de),
+                              len(fx), float(h), float(rest_density), _dp(out))
+    if not ok:
+        raise ValueError("non-finite angle in a sorted wall list (the reference panics)")
+    return dict(zip(DRUM_METRICS_KEYS, out.tolist()))
 
 
 def toi_swept(drum, x0, x1, a0, a1, radius):
