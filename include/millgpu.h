@@ -342,6 +342,47 @@ int mg_drum_metrics_batch(const mg_metrics_drum* drums, uint32_t n_drums, const 
  * libm. */
 void mg_atan2f_ref(const float* y, const float* x, float* out, uint64_t n);
 
+/* ---- grinding-proxy stress accumulators (stress.rs) -------------------------------------------
+ * ImpactRecord (dem.rs:152-169): kind 0 ball-ball, 1 ball-wall; partner 0xFFFFFFFF
+ * (ImpactRecord::WALL) for a wall impact. */
+typedef struct mg_impact_record {
+  uint32_t kind, ball, partner;
+  float normal_speed, slip_speed, restitution;
+} mg_impact_record;
+/* The arguments of StressAccum::update (stress.rs:110-120) for one drum and one sub-step. */
+typedef struct mg_stress_step {
+  double m3, scale_j, d, now_s, sub_dt, alpha, wall_work_j;
+} mg_stress_step;
+/* StressMetrics (stress.rs:243-273). */
+typedef struct mg_stress_metrics {
+  float ball_mass_3d_kg, scale_energy_j, impacts_per_ball_per_s[4], impacts_total_per_ball_per_s;
+  float mean_se_scaled[4], mean_se_scaled_drop, mean_se_scaled_shear, stress_intensity_scaled;
+  float wall_power_per_ball_w, specific_power_w_per_kg, thresholds_scaled[6], effective_power_per_ball_w[6];
+  double cumulative_wall_energy_per_ball_j, cumulative_specific_energy_j_per_kg;
+  double cumulative_effective_energy_per_ball_j[6];
+} mg_stress_metrics;
+typedef struct mg_stress mg_stress;
+/* StressAccum::default (stress.rs:83-99) for n_drums drums, each with a debounce table of at
+ * least table_capacity keys (rounded up to a power of two, at most 2^24) in device memory. */
+int mg_stress_create(uint32_t n_drums, uint32_t table_capacity, mg_stress** out, char* err, size_t errlen);
+void mg_stress_destroy(mg_stress* stress);
+/* StressAccum::set_debounce_factor (stress.rs:103-106) of one drum (0 disables the debounce). */
+int mg_stress_set_debounce_factor(mg_stress* stress, uint32_t drum, double factor);
+/* StressAccum::update (stress.rs:110-171) of every drum with one sub-step's records: drum d's
+ * records are d_records[d_offsets[d] .. d_offsets[d + 1]) (device arrays), steps[d] (host) its
+ * arguments. Results equal the reference's f64 arithmetic bit for bit (records in order, se_bin by
+ * the host libm's log10). Waits for the device. MG_E_ARG if a drum's now_s decreases;
+ * MG_E_CAPACITY if a drum's debounce table had no room for a key (its counts may then differ from
+ * the reference's from that record on). */
+int mg_stress_update(mg_stress* stress, const mg_impact_record* d_records, const uint32_t* d_offsets,
+                     const mg_stress_step* steps, char* err, size_t errlen);
+/* One drum's 49 accumulator values: count_rate 4, se_hat_rate 4, hist_rate 28, above_rate_j 6,
+ * cum_wall_j, cum_above_j 6 (stress.rs:83-99 field order). */
+int mg_stress_state(mg_stress* stress, uint32_t drum, double* out49);
+/* StressAccum::metrics (stress.rs:186-238) of one drum. */
+int mg_stress_get_metrics(mg_stress* stress, uint32_t drum, uint32_t n_balls, double m3, double scale_j, double d,
+                      double power_draw_w, mg_stress_metrics* out);
+
 /* ---- device memory of this library's HIP runtime ----------------------------------------------
  * No reference counterpart: plumbing for callers without a HIP runtime of their own (bindings,
  * tests). Device pointers of another HIP runtime loaded into the same process (for example one

@@ -1876,7 +1876,7 @@ template <class R> int run_step(mg_box* B, uint32_t first) {
     launched = B->cfg.max_cg;  // no CG this step
   }
   for (;;) {
-    hipMemcpyAsync(B->h_sc, B->sc, sizeof(BoxScal), hipMemcpyDeviceToHost, B->stream);
+    (void)hipMemcpyAsync(B->h_sc, B->sc, sizeof(BoxScal), hipMemcpyDeviceToHost, B->stream);
     if (hipStreamSynchronize(B->stream) != hipSuccess) { B->err = "HIP stream error"; return MG_E_DEVICE; }
     collect_times(B);
     if (B->h_sc->done || launched >= B->cfg.max_cg) break;
@@ -1935,7 +1935,7 @@ template <class R> int run_window(mg_box* B, uint32_t steps) {
       enqueue_epilogue<R>(B);
       std::swap(B->V, B->VN);
     }
-    if (cg_on) hipMemcpyAsync(B->h_verdict, B->verdict, (W + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, B->stream);
+    if (cg_on) (void)hipMemcpyAsync(B->h_verdict, B->verdict, (W + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, B->stream);
     if (hipStreamSynchronize(B->stream) != hipSuccess) { B->err = "HIP stream error"; return MG_E_DEVICE; }
     if (hipGetLastError() != hipSuccess) { B->err = "kernel launch failed"; return MG_E_DEVICE; }
     collect_times(B);
@@ -2137,12 +2137,12 @@ size_t mg_box_count(const mg_box* B) { return B ? (size_t)B->n : 0; }
 
 int mg_box_step(mg_box* B, uint32_t steps) {
   if (!B) return MG_E_ARG;
-  hipEventRecord(B->ev0, B->stream);
+  (void)hipEventRecord(B->ev0, B->stream);
   const int rc = B->f64 ? run_window<double>(B, steps) : run_window<float>(B, steps);
   if (rc) return rc;
-  hipEventRecord(B->ev1, B->stream);
+  (void)hipEventRecord(B->ev1, B->stream);
   if (hipEventSynchronize(B->ev1) != hipSuccess) return MG_E_DEVICE;
-  hipEventElapsedTime(&B->last_ms, B->ev0, B->ev1);
+  (void)hipEventElapsedTime(&B->last_ms, B->ev0, B->ev1);
   return MG_OK;
 }
 

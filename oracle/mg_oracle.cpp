@@ -823,6 +823,37 @@ struct Stress {
     for (int t = 0; t < 6; t++) ema(above_rate[t], above[t]);
   }
   double total_rate() const { return count_rate[0] + count_rate[1] + count_rate[2] + count_rate[3]; }
+  // stress.rs:186-254: StressMetrics as 36 doubles: ball_mass_3d_kg, scale_energy_j,
+  // impacts_per_ball_per_s[4], impacts_total_per_ball_per_s, mean_se_scaled[4], mean_se_scaled_drop,
+  // mean_se_scaled_shear, stress_intensity_scaled, wall_power_per_ball_w, specific_power_w_per_kg,
+  // thresholds_scaled[6], effective_power_per_ball_w[6] (the f32 fields rounded to f32), then the f64
+  // cumulative_wall_energy_per_ball_j, cumulative_specific_energy_j_per_kg,
+  // cumulative_effective_energy_per_ball_j[6].
+  void metrics(size_t n_balls, double m3, double scale_j, double d, double power, double* o) const {
+    const double n = (double)std::max<size_t>(n_balls, 1);
+    int k = 0;
+    o[k++] = (float)m3;
+    o[k++] = (float)scale_j;
+    for (int c = 0; c < 4; c++) o[k++] = (float)(count_rate[c] / n);
+    const double total = total_rate();
+    o[k++] = (float)(total / n);
+    for (int c = 0; c < 4; c++) o[k++] = count_rate[c] > 0.0 ? (float)(se_rate[c] / count_rate[c]) : 0.0f;
+    auto pooled = [&](int a, int b) {
+      const double w = count_rate[a] + count_rate[b];
+      return w > 0.0 ? (float)((se_rate[a] + se_rate[b]) / w) : 0.0f;
+    };
+    o[k++] = pooled(0, 2);
+    o[k++] = pooled(1, 3);
+    o[k++] = total > 0.0 ? (float)((se_rate[0] + se_rate[1] + se_rate[2] + se_rate[3]) / total) : 0.0f;
+    const double ppb = power * d / n;
+    o[k++] = (float)ppb;
+    o[k++] = (float)(ppb / m3);
+    for (int t = 0; t < 6; t++) o[k++] = (float)THR[t];
+    for (int t = 0; t < 6; t++) o[k++] = (float)(above_rate[t] / n);
+    o[k++] = cum_wall * d / n;
+    o[k++] = cum_wall * d / n / m3;
+    for (int t = 0; t < 6; t++) o[k++] = cum_above[t] / n;
+  }
 };
 constexpr double Stress::THR[6];
 
@@ -2189,6 +2220,33 @@ void orc_stress(void* hv, double* o) {
     for (int c = 0; c < 6; c++) o[k++] = s->stress.cum_above[c];
   });
 }
+// A standalone StressAccum (stress.rs:83-254) for the accumulator's own checks.
+void* orc_stress_new() { return new Stress(); }
+void orc_stress_free(void* h) { delete (Stress*)h; }
+void orc_stress_set_factor(void* h, double f) { ((Stress*)h)->factor = f; }
+// One update with n records: kind (0 ball-ball, 1 ball-wall), ball, partner (UINT32_MAX: wall),
+// normal and slip speeds.
+void orc_stress_update(void* h, const int32_t* kind, const uint32_t* ball, const uint32_t* partner, const float* vn,
+                       const float* slip, size_t n, double m3, double scale_j, double d, double now, double sub_dt,
+                       double alpha, double wall_work) {
+  std::vector<Impact> r(n);
+  for (size_t i = 0; i < n; i++) r[i] = Impact{kind[i], ball[i], partner[i], vn[i], slip[i], 0.0f};
+  ((Stress*)h)->update(r, m3, scale_j, d, now, sub_dt, alpha, wall_work);
+}
+void orc_stress_state(void* h, double* o) {
+  const Stress* st = (const Stress*)h;
+  int k = 0;
+  for (int c = 0; c < 4; c++) o[k++] = st->count_rate[c];
+  for (int c = 0; c < 4; c++) o[k++] = st->se_rate[c];
+  for (int c = 0; c < 28; c++) o[k++] = st->hist_rate[c];
+  for (int c = 0; c < 6; c++) o[k++] = st->above_rate[c];
+  o[k++] = st->cum_wall;
+  for (int c = 0; c < 6; c++) o[k++] = st->cum_above[c];
+}
+void orc_stress_metrics(void* h, size_t n_balls, double m3, double scale_j, double d, double power, double* o) {
+  ((const Stress*)h)->metrics(n_balls, m3, scale_j, d, power, o);
+}
+size_t orc_se_bin(double e) { return Stress::se_bin(e); }
 int orc_impacts(void* hv, double* o, int cap) {
   OrcHandle* h = (OrcHandle*)hv;
   int n = 0;

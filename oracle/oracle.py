@@ -64,6 +64,15 @@ def oracle_lib():
     lib.orc_grid_keys.argtypes = [fp, C.c_size_t, C.c_float, C.POINTER(C.c_int32), C.c_int64]
     lib.orc_mixing_index_of.restype = C.c_int
     lib.orc_mixing_index_of.argtypes = [fp, fp, C.c_size_t, C.c_float, fp]
+    lib.orc_stress_new.restype = P
+    lib.orc_stress_new.argtypes = []
+    lib.orc_stress_free.argtypes = [P]
+    lib.orc_stress_set_factor.argtypes = [P, C.c_double]
+    lib.orc_stress_update.argtypes = [P, C.POINTER(C.c_int32), u32p, u32p, fp, fp, C.c_size_t] + [C.c_double] * 7
+    lib.orc_stress_state.argtypes = [P, dp]
+    lib.orc_stress_metrics.argtypes = [P, C.c_size_t, C.c_double, C.c_double, C.c_double, C.c_double, dp]
+    lib.orc_se_bin.restype = C.c_size_t
+    lib.orc_se_bin.argtypes = [C.c_double]
     lib.orc_drum_metrics.restype = C.c_int
     lib.orc_drum_metrics.argtypes = [dp, C.c_float, fp, fp, fp, C.c_size_t, C.c_float, C.c_float, C.c_float,
                                      fp, fp, fp, C.c_size_t, C.c_float, C.c_float, dp]
@@ -425,6 +434,56 @@ def drum_metrics(drum, angle, ball_x, ball_v, ball_omega, ball_radius, ball_mass
     if not ok:
         raise ValueError("non-finite angle in a sorted wall list (the reference panics)")
     return dict(zip(DRUM_METRICS_KEYS, out.tolist()))
+
+
+STRESS_METRICS_KEYS = (["ball_mass_3d_kg", "scale_energy_j"] + [f"impacts_per_ball_per_s_{c}" for c in range(4)] +
+                       ["impacts_total_per_ball_per_s"] + [f"mean_se_scaled_{c}" for c in range(4)] +
+                       ["mean_se_scaled_drop", "mean_se_scaled_shear", "stress_intensity_scaled",
+                        "wall_power_per_ball_w", "specific_power_w_per_kg"] +
+                       [f"thresholds_scaled_{t}" for t in range(6)] + [f"effective_power_per_ball_w_{t}" for t in range(6)] +
+                       ["cumulative_wall_energy_per_ball_j", "cumulative_specific_energy_j_per_kg"] +
+                       [f"cumulative_effective_energy_per_ball_j_{t}" for t in range(6)])
+
+
+class StressAccum:
+    """stress.rs:83-254 restated: update() with impact records, state() as the 49 accumulator
+    doubles [count 4, se 4, hist 28, above 6, cum_wall, cum_above 6], metrics() as a dict."""
+
+    def __init__(self):
+        self._lib = oracle_lib()
+        self._h = self._lib.orc_stress_new()
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._lib.orc_stress_free(self._h)
+            self._h = None
+
+    def set_debounce_factor(self, factor):
+        self._lib.orc_stress_set_factor(self._h, float(factor))
+
+    def update(self, kind, ball, partner, vn, slip, m3, scale_j, d, now, sub_dt, alpha, wall_work):
+        k = np.ascontiguousarray(np.asarray(kind, np.int32).reshape(-1))
+        b = np.ascontiguousarray(np.asarray(ball, np.uint32).reshape(-1))
+        p = np.ascontiguousarray(np.asarray(partner, np.uint32).reshape(-1))
+        v = np.ascontiguousarray(np.asarray(vn, np.float32).reshape(-1))
+        s = np.ascontiguousarray(np.asarray(slip, np.float32).reshape(-1))
+        self._lib.orc_stress_update(self._h, k.ctypes.data_as(C.POINTER(C.c_int32)), _u32(b), _u32(p), _fp(v), _fp(s),
+                                    len(k), m3, scale_j, d, now, sub_dt, alpha, wall_work)
+
+    def state(self):
+        o = np.zeros(49)
+        self._lib.orc_stress_state(self._h, _dp(o))
+        return o
+
+    def metrics(self, n_balls, m3, scale_j, d, power):
+        o = np.zeros(len(STRESS_METRICS_KEYS))
+        self._lib.orc_stress_metrics(self._h, int(n_balls), m3, scale_j, d, power, _dp(o))
+        return dict(zip(STRESS_METRICS_KEYS, o.tolist()))
+
+
+def se_bin(e):
+    """stress.rs:58-64 as the oracle evaluates it."""
+    return int(oracle_lib().orc_se_bin(float(e)))
 
 
 def toi_swept(drum, x0, x1, a0, a1, radius):
