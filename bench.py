@@ -25,7 +25,9 @@ same box workload; cpu_baseline_drum: the same restatement on C2 drum substeps (
 context only while drums do not step on the GPU). kernel_isolation.grid: the reference-layout
 uniform grid build (grid.rs:101-153) over the box's 1,048,576 initial points;
 kernel_isolation.mixing_index: the Lacey mixing index (metrics.rs:506-576) of 1024 synthetic
-C2-sized drum states in one launch; each with the restatement's rate beside it.
+C2-sized drum states in one launch; kernel_isolation.drum_metrics: metrics::compute of 256 C2 drum
+states in one call; kernel_isolation.stress: StressAccum::update of 256 drums per call; each with
+the restatement's rate beside it where one exists.
 --part grid / --part mixing time only that operator (for a profiler run).
 """
 from __future__ import annotations
@@ -307,6 +309,100 @@ def bench_mixing(args, drums=1024, per_drum=12697, cpu=True) -> dict:
     return out
 
 
+def bench_drum_metrics(args, drums=256, cpu=True) -> dict:
+    """metrics::compute (metrics.rs:195-254) for a batch of drum states in one call: `drums` copies
+    of the 2000-disc drum (12,697 fluid particles) stepped 12 substeps on the restatement, each
+    turned by its own drum angle. Timed: the whole synchronous call (upload of the per-drum
+    records, five kernels, read-back); the drum states are resident in device memory."""
+    import numpy as np
+    from milldynamics3_amd import metrics
+    from milldynamics3_amd.grid import DeviceArray
+    from oracle import oracle as orc
+    o = orc.OracleSim(C2_PARAMS, "ref")
+    o.step_fixed(12)
+    st = o.state()
+    nb, nf = len(st["ball_x"]), len(st["fluid_x"])
+    rng = np.random.default_rng(3)
+    dens = (1000.0 * (1.0 + 0.03 * rng.standard_normal(nf))).astype(np.float32)
+    L = metrics._metrics_lib()
+    recs = (metrics._MetricsDrum * drums)()
+    for k in range(drums):
+        r = recs[k]
+        r.radius_m, r.omega, r.drum_angle = 0.1397 / 2, o.omega, float(k) * 0.01
+        r.ball_radius, r.ball_mass, r.ball_inertia = o.ball_radius, o.ball_mass, o.ball_inertia
+        r.h, r.rest_density, r.n_balls, r.n_fluid = o.h, 1000.0, nb, nf
+    tile = lambda a, c: DeviceArray.from_numpy(np.tile(np.asarray(a, np.float32).reshape(-1, c), (drums, 1)))
+    arrays = [tile(st["ball_x"], 2), tile(st["ball_v"], 2), tile(st["ball_omega"], 1), tile(st["fluid_x"], 2),
+              tile(st["dye"], 1), tile(dens, 1)]
+    out = (metrics._DrumMetrics * drums)()
+    err = __import__("ctypes").create_string_buffer(256)
+
+    def call():
+        rc = L.mg_drum_metrics_batch(recs, drums, *[a.ptr for a in arrays], out, err, 256)
+        if rc != 0:
+            raise RuntimeError(err.value.decode())
+    call()
+    reps = max(args.steps // 4, 1)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        call()
+    ms = 1000.0 * (time.perf_counter() - t0) / reps
+    res = {"drums": drums, "balls_per_drum": nb, "fluid_per_drum": nf, "batch_ms": round(ms, 3),
+           "drums_per_s": round(drums / (ms * 1e-3), 1),
+           "timing": f"host wall clock around {reps} synchronous batch calls"}
+    if cpu:
+        drum = orc.drum_array(0.1397 / 2, o.omega)
+        k, t0 = 0, time.perf_counter()
+        while k == 0 or (time.perf_counter() - t0 < min(args.cpu_seconds, 2.0) and k < drums):
+            orc.drum_metrics(drum, float(k) * 0.01, st["ball_x"], st["ball_v"], st["ball_omega"], o.ball_radius,
+                             o.ball_mass, o.ball_inertia, st["fluid_x"], st["dye"], dens, o.h, 1000.0)
+            k += 1
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": round(k / dt, 1), "unit": "drums/s", "cores": 1, "kind": "port",
+                               "sample": f"{k} drums of the same batch on the C++ restatement, 1 core"}
+    return res
+
+
+def bench_stress(args, drums=256, records=40) -> dict:
+    """StressAccum::update (stress.rs:110-171) of `drums` accumulators per call, `records` impact
+    records per drum per sub-step (pairs among 64 balls and wall contacts), K sub-steps. Timed: the
+    synchronous update calls with the records already in device memory."""
+    import numpy as np
+    from milldynamics3_amd import stress
+    from milldynamics3_amd.grid import DeviceArray
+    import ctypes as C
+    rng = np.random.default_rng(4)
+    n = drums * records
+    kind = (rng.random(n) < 0.3).astype(np.uint32)
+    ball = rng.integers(0, 64, n).astype(np.uint32)
+    partner = np.where(kind == 1, stress.WALL, (ball + 1 + rng.integers(0, 63, n)) % 64).astype(np.uint32)
+    rec = stress.records(kind, ball, partner, rng.uniform(0.05, 2.0, n), rng.uniform(0.0, 1.0, n))
+    acc = stress.StressAccumBatch(drums, table_capacity=4096)
+    L = stress._lib()
+    d_rec = DeviceArray.from_numpy(rec.view(np.uint8))
+    d_off = DeviceArray.from_numpy((np.arange(drums + 1) * records).astype(np.uint32))
+    steps = (stress._Step * drums)()
+    err = C.create_string_buffer(256)
+    m3, d = 2.0e-5, 0.002
+    reps = max(args.steps, 1)
+    t0 = None
+    for s in range(reps + 1):
+        for k in range(drums):
+            st = steps[k]
+            st.m3, st.scale_j, st.d, st.now_s = m3, m3 * 9.81 * d, d, s * 1e-4
+            st.sub_dt, st.alpha, st.wall_work_j = 1e-4, 0.01, 0.0
+        if s == 1:
+            t0 = time.perf_counter()
+        rc = L.mg_stress_update(acc._h, d_rec.ptr, d_off.ptr, steps, err, 256)
+        if rc != 0:
+            raise RuntimeError(err.value.decode())
+    ms = 1000.0 * (time.perf_counter() - t0) / reps
+    acc.close()
+    return {"drums": drums, "records_per_drum": records, "update_ms": round(ms, 4),
+            "drum_updates_per_s": round(drums / (ms * 1e-3), 1),
+            "timing": f"host wall clock around {reps} synchronous updates (host step records included)"}
+
+
 def precision_vs_fp64(cfg, steps) -> dict:
     """Error of the FP32-state box's outputs against the FP64-state box after the same `steps` steps
     from the same initial state (the FP64 box is the oracle's f64 policy on the GPU): positions in
@@ -350,7 +446,7 @@ def main() -> int:
     ap.add_argument("--no-bingham", action="store_true", help="skip the Bingham-fluid box measurement")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one plain device copy of this size (calibrates PMC byte counters)")
-    ap.add_argument("--part", choices=("all", "grid", "mixing"), default="all",
+    ap.add_argument("--part", choices=("all", "grid", "mixing", "drum_metrics", "stress"), default="all",
                     help="grid / mixing: time only that operator (for a profiler run) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -376,7 +472,10 @@ def main() -> int:
         torch.cuda.synchronize()
         del src, dst
     if args.part != "all":
-        part = bench_grid(args, cpu=not args.no_cpu) if args.part == "grid" else bench_mixing(args, cpu=not args.no_cpu)
+        part = {"grid": lambda: bench_grid(args, cpu=not args.no_cpu),
+                "mixing": lambda: bench_mixing(args, cpu=not args.no_cpu),
+                "drum_metrics": lambda: bench_drum_metrics(args, cpu=not args.no_cpu),
+                "stress": lambda: bench_stress(args)}[args.part]()
         print(json.dumps({"part": args.part, **part}), flush=True)
         return 0
     cfg = {"side": args.side, "mu": args.mu}
@@ -459,6 +558,8 @@ def main() -> int:
         iso["bingham"] = bench_bingham(args, cfg)
     iso["grid"] = bench_grid(args, cpu=not args.no_cpu)
     iso["mixing_index"] = bench_mixing(args, cpu=not args.no_cpu)
+    iso["drum_metrics"] = bench_drum_metrics(args, cpu=not args.no_cpu)
+    iso["stress"] = bench_stress(args)
     box_cpu, drum_cpu = None, None
     if not args.no_cpu:
         box_cpu = cpu_baseline_box(cfg, args.cpu_seconds)
