@@ -302,6 +302,22 @@ __device__ __forceinline__ uint32_t dm_wave_or(uint32_t v) {
   for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
   return v;
 }
+// acc + row[0] + row[1] + ... + row[m-1], added one at a time in that order (the LDS row is read
+// four values per load so the loads run ahead of the dependent adds).
+__device__ __forceinline__ float dm_seq_sum(float acc, const float* row, uint32_t m) {
+  uint32_t q = 0;
+#pragma unroll 4
+  for (; q + 4 <= m; q += 4) {
+    const float4 v = *reinterpret_cast<const float4*>(row + q);
+    acc += v.x;
+    acc += v.y;
+    acc += v.z;
+    acc += v.w;
+  }
+  for (; q < m; q++) acc += row[q];
+  return acc;
+}
+
 // Maximum over the workgroup (every thread must call it); the result is valid in thread 0.
 __device__ float dm_block_max(float v, float* s_red) {
   v = dm_wave_max(v);
@@ -318,7 +334,7 @@ __device__ float dm_block_max(float v, float* s_red) {
 extern "C" __global__ void __launch_bounds__(DM_BLOCK)
 dm_balls(const DmDrum* __restrict__ drums, const float2* __restrict__ lift, const float2* __restrict__ bx,
          const float2* __restrict__ bv, const float* __restrict__ bw, DmItem* __restrict__ toe, DmOut* __restrict__ out) {
-  __shared__ float s_v[3][DM_BLOCK];
+  __shared__ __align__(16) float s_v[3][DM_BLOCK];
   __shared__ float s_red[DM_BLOCK / 64];
   __shared__ uint32_t s_n;
   const DmDrum D = drums[blockIdx.x];
@@ -344,10 +360,7 @@ dm_balls(const DmDrum* __restrict__ drums, const float2* __restrict__ lift, cons
       s_v[2][t] = 0.5f * D.bmass * (v.x * v.x + v.y * v.y) + 0.5f * D.binertia * w * w;
     }
     __syncthreads();
-    if (t < 3) {
-      const uint32_t m = min(D.nb - base, (uint32_t)DM_BLOCK);
-      for (uint32_t q = 0; q < m; q++) acc += s_v[t][q];
-    }
+    if (t < 3) acc = dm_seq_sum(acc, s_v[t], min(D.nb - base, (uint32_t)DM_BLOCK));
     __syncthreads();
   }
   wov = dm_block_max(wov, s_red);
@@ -357,43 +370,55 @@ dm_balls(const DmDrum* __restrict__ drums, const float2* __restrict__ lift, cons
   if (t == 2) o->ke = acc;
 }
 
-extern "C" __global__ void __launch_bounds__(DM_BLOCK)
+// 1024 lanes per drum (16 waves, so LDS reads of one wave overlap the arithmetic of others).
+#define DM_OV_BLOCK 1024
+extern "C" __global__ void __launch_bounds__(DM_OV_BLOCK)
 dm_ball_overlap(const DmDrum* __restrict__ drums, const float2* __restrict__ bx, DmOut* __restrict__ out) {
-  __shared__ float2 s_p[DM_BLOCK];
-  __shared__ float s_red[DM_BLOCK / 64];
+  __shared__ __align__(16) float2 s_p[DM_OV_BLOCK];
+  __shared__ float s_red[DM_OV_BLOCK / 64];
   const DmDrum D = drums[blockIdx.x];
   const uint32_t t = threadIdx.x;
   float best = 0.0f;
   if (D.nb >= 2 && D.br > 0.0f) {
     const float two_r = 2.0f * D.br, lim = 4.5f * D.br * D.br;
-    for (uint32_t ib = 0; ib < D.nb; ib += DM_BLOCK) {
+    for (uint32_t ib = 0; ib < D.nb; ib += DM_OV_BLOCK) {
       const uint32_t i = ib + t;
       const float2 pi = i < D.nb ? bx[D.b0 + i] : make_float2(0.0f, 0.0f);
-      for (uint32_t jb = ib; jb < D.nb; jb += DM_BLOCK) {  // tiles at or after the i tile
+      for (uint32_t jb = ib; jb < D.nb; jb += DM_OV_BLOCK) {  // tiles at or after the i tile
         __syncthreads();
         if (jb + t < D.nb) s_p[t] = bx[D.b0 + jb + t];
         __syncthreads();
         if (i < D.nb) {
-          const uint32_t m = min(D.nb - jb, (uint32_t)DM_BLOCK);
-          for (uint32_t q = 0; q < m; q++) {
-            if (jb + q <= i) continue;
-            const float dx = pi.x - s_p[q].x, dy = pi.y - s_p[q].y;
-            const float d2 = dx * dx + dy * dy;
-            if (d2 < lim) best = fmaxf(best, fmaxf(two_r - sqrtf(d2), 0.0f) / D.br);
+          const uint32_t m = min(D.nb - jb, (uint32_t)DM_OV_BLOCK);
+          for (uint32_t q = 0; q < m; q += 2) {  // two positions per LDS read
+            const float4 two = *reinterpret_cast<const float4*>(&s_p[q]);
+            {
+              const float dx = pi.x - two.x, dy = pi.y - two.y, d2 = dx * dx + dy * dy;
+              if (jb + q > i && d2 < lim) best = fmaxf(best, fmaxf(two_r - sqrtf(d2), 0.0f) / D.br);
+            }
+            {
+              const float dx = pi.x - two.z, dy = pi.y - two.w, d2 = dx * dx + dy * dy;
+              if (q + 1 < m && jb + q + 1 > i && d2 < lim) best = fmaxf(best, fmaxf(two_r - sqrtf(d2), 0.0f) / D.br);
+            }
           }
         }
       }
     }
   }
-  best = dm_block_max(best, s_red);
-  if (t == 0) out[blockIdx.x].max_overlap = best;
+  best = dm_wave_max(best);
+  if ((t & 63) == 0) s_red[t >> 6] = best;
+  __syncthreads();
+  if (t == 0) {
+    float r = s_red[0];
+    for (int w = 1; w < DM_OV_BLOCK / 64; w++) r = fmaxf(r, s_red[w]);
+    out[blockIdx.x].max_overlap = r;
+  }
 }
 
 extern "C" __global__ void __launch_bounds__(DM_BLOCK)
 dm_fluid(const DmDrum* __restrict__ drums, const float2* __restrict__ lift, const float2* __restrict__ fx,
          const float* __restrict__ dens, DmItem* __restrict__ pool, DmOut* __restrict__ out) {
-  __shared__ float s_v[4][DM_BLOCK];
-  __shared__ uint32_t s_inc[DM_BLOCK];
+  __shared__ __align__(16) float s_v[4][DM_BLOCK];
   __shared__ float s_red[DM_BLOCK / 64];
   __shared__ uint32_t s_n, s_fs;
   __shared__ float s_c[2];
@@ -431,24 +456,16 @@ dm_fluid(const DmDrum* __restrict__ drums, const float2* __restrict__ lift, cons
         max_e = fmaxf(max_e, e);
         max_c = fmaxf(max_c, c);
       }
-      s_v[0][t] = p.x;
-      s_v[1][t] = p.y;
+      // an excluded point adds +0.0, which leaves the running sum unchanged (it is never -0.0)
+      s_v[0][t] = inc ? p.x : 0.0f;
+      s_v[1][t] = inc ? p.y : 0.0f;
       s_v[2][t] = e;
       s_v[3][t] = c;
-      s_inc[t] = inc ? 1u : 0u;
+      const uint64_t b = __ballot(inc);
+      if ((t & 63) == 0) atomicAdd(&s_fs, (uint32_t)__popcll(b));
     }
     __syncthreads();
-    if (t < 5) {
-      const uint32_t m = min(D.nf - base, (uint32_t)DM_BLOCK);
-      if (t == 4) {
-        uint32_t k = 0;
-        for (uint32_t q = 0; q < m; q++) k += s_inc[q];
-        s_fs += k;
-      } else {
-        for (uint32_t q = 0; q < m; q++)
-          if (t >= 2 || s_inc[q]) acc += s_v[t][q];
-      }
-    }
+    if (t < 4) acc = dm_seq_sum(acc, s_v[t], min(D.nf - base, (uint32_t)DM_BLOCK));
     __syncthreads();
   }
   DmOut* o = out + blockIdx.x;
@@ -470,17 +487,12 @@ dm_fluid(const DmDrum* __restrict__ drums, const float2* __restrict__ lift, cons
         const float2 p = fx[D.f0 + i];
         const bool inc = dm_sdf_world(p, D, lift) >= margin;
         const float dx = p.x - cx, dy = p.y - cy;
-        s_v[0][t] = dx * dx;
-        s_v[1][t] = dy * dy;
-        s_v[2][t] = dx * dy;
-        s_inc[t] = inc ? 1u : 0u;
+        s_v[0][t] = inc ? dx * dx : 0.0f;
+        s_v[1][t] = inc ? dy * dy : 0.0f;
+        s_v[2][t] = inc ? dx * dy : 0.0f;
       }
       __syncthreads();
-      if (t < 3) {
-        const uint32_t m = min(D.nf - base, (uint32_t)DM_BLOCK);
-        for (uint32_t q = 0; q < m; q++)
-          if (s_inc[q]) cov += s_v[t][q];
-      }
+      if (t < 3) cov = dm_seq_sum(cov, s_v[t], min(D.nf - base, (uint32_t)DM_BLOCK));
       __syncthreads();
     }
   }
@@ -688,7 +700,7 @@ int mg_drum_metrics_batch(const mg_metrics_drum* drums, uint32_t n_drums, const 
     hipLaunchKernelGGL(dm_balls, dim3(n_drums), dim3(DM_BLOCK), 0, nullptr, dd, dl,
                        reinterpret_cast<const float2*>(d_ball_x), reinterpret_cast<const float2*>(d_ball_v), d_ball_omega,
                        dtoe, dout);
-    hipLaunchKernelGGL(dm_ball_overlap, dim3(n_drums), dim3(DM_BLOCK), 0, nullptr, dd,
+    hipLaunchKernelGGL(dm_ball_overlap, dim3(n_drums), dim3(DM_OV_BLOCK), 0, nullptr, dd,
                        reinterpret_cast<const float2*>(d_ball_x), dout);
     hipLaunchKernelGGL(dm_fluid, dim3(n_drums), dim3(DM_BLOCK), 0, nullptr, dd, dl,
                        reinterpret_cast<const float2*>(d_fluid_x), d_fluid_density, dpool, dout);
