@@ -21,6 +21,7 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -2259,6 +2260,48 @@ int orc_impacts(void* hv, double* o, int cap) {
     }
   });
   return n;
+}
+// DemState::step_recorded (dem.rs:530-1066) on the handle's balls alone, at drum angle `angle`;
+// ext_imp (2 floats per ball) / ext_ang may be null (no coupling impulses). stats[18]: wall work,
+// dissipated energy, coupling work, collisions, impact histogram 12, max displacement over
+// diameter, candidate pairs. The impacts are read with orc_impacts.
+void orc_dem_step(void* hv, double angle, uint32_t iterations, double dt, const float* ext_imp, const float* ext_ang,
+                  double* stats) {
+  OrcHandle* h = (OrcHandle*)hv;
+  DISPATCH(h, {
+    using RR = std::remove_reference<decltype(s->drum_angle)>::type;
+    const size_t nb = s->bl.n();
+    Coupling<RR> cp;
+    const Coupling<RR>* ext = nullptr;
+    if (ext_imp && ext_ang) {
+      cp.imp.resize(nb);
+      cp.ang.resize(nb);
+      for (size_t i = 0; i < nb; i++) {
+        cp.imp[i] = V2<RR>((RR)ext_imp[2 * i], (RR)ext_imp[2 * i + 1]);
+        cp.ang[i] = (RR)ext_ang[i];
+      }
+      ext = &cp;
+    }
+    s->dem_step(s->drum(), (RR)angle, iterations, (RR)dt, ext);
+    int k = 0;
+    stats[k++] = s->ds.wall_work; stats[k++] = s->ds.dissipated; stats[k++] = s->ds.coupling_work;
+    stats[k++] = s->ds.collisions;
+    for (int b = 0; b < 12; b++) stats[k++] = s->ds.hist[b];
+    stats[k++] = s->ds.max_disp; stats[k++] = s->ds.n_pairs;
+  });
+}
+// The f32 values the DEM step of this handle uses: drum radius, omega, lifter count, height, base,
+// top, phase (deg), ball radius, mass, inertia, restitution bb / bw, friction bb / bw kinetic,
+// bb / bw static, friction velocity scale, rolling friction (18 values).
+void orc_dem_params(void* hv, double* o) {
+  OrcHandle* h = (OrcHandle*)hv;
+  DISPATCH(h, {
+    const Params& P = s->P;
+    double v[18] = {P.radius_m(), P.omega(), (double)P.lifter_count, P.lifter_height, P.lifter_base, P.lifter_top,
+                    P.lifter_phase_deg, (double)s->bl.radius, (double)s->bl.mass, (double)s->bl.inertia, P.rest_bb,
+                    P.rest_bw, P.fric_bb, P.fric_bw, P.fric_bb_static, P.fric_bw_static, P.fric_vscale, P.rolling};
+    std::memcpy(o, v, sizeof v);
+  });
 }
 void orc_fluid_bins(void* hv, double* o) {
   OrcHandle* h = (OrcHandle*)hv;

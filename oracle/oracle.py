@@ -64,6 +64,8 @@ def oracle_lib():
     lib.orc_grid_keys.argtypes = [fp, C.c_size_t, C.c_float, C.POINTER(C.c_int32), C.c_int64]
     lib.orc_mixing_index_of.restype = C.c_int
     lib.orc_mixing_index_of.argtypes = [fp, fp, C.c_size_t, C.c_float, fp]
+    lib.orc_dem_step.argtypes = [P, C.c_double, C.c_uint32, C.c_double, fp, fp, dp]
+    lib.orc_dem_params.argtypes = [P, dp]
     lib.orc_stress_new.restype = P
     lib.orc_stress_new.argtypes = []
     lib.orc_stress_free.argtypes = [P]
@@ -259,6 +261,32 @@ class OracleSim:
         o = np.zeros(49)
         self._lib.orc_stress(self._h, _dp(o))
         return o
+
+    DEM_STATS_KEYS = (["wall_work_j", "dissipated_energy_j", "coupling_work_j", "collision_count"] +
+                      [f"impact_energy_histogram_{b}" for b in range(12)] +
+                      ["max_substep_displacement_over_diameter", "n_pairs"])
+    DEM_PARAM_KEYS = ("radius_m", "omega", "lifter_count", "lifter_height_m", "lifter_base_width_m",
+                      "lifter_top_width_m", "lifter_phase_deg", "ball_radius", "ball_mass", "ball_inertia",
+                      "restitution_ball_ball", "restitution_ball_wall", "friction_ball_ball", "friction_ball_wall",
+                      "friction_ball_ball_static", "friction_ball_wall_static", "friction_velocity_scale_m_s",
+                      "rolling_friction")
+
+    def dem_step(self, drum_angle, iterations, dt, ext_imp=None, ext_ang=None) -> dict:
+        """DemState::step_recorded (dem.rs:530-1066) on this simulation's balls alone (the fluid is
+        untouched); returns DemStepStats, the records are in impacts()."""
+        st = np.zeros(len(self.DEM_STATS_KEYS))
+        ei = ea = None
+        if ext_imp is not None:
+            self._ext = (np.ascontiguousarray(np.asarray(ext_imp, np.float32).reshape(-1)),
+                         np.ascontiguousarray(np.asarray(ext_ang, np.float32).reshape(-1)))
+            ei, ea = _fp(self._ext[0]), _fp(self._ext[1])
+        self._lib.orc_dem_step(self._h, float(drum_angle), int(iterations), float(dt), ei, ea, _dp(st))
+        return dict(zip(self.DEM_STATS_KEYS, st.tolist()))
+
+    def dem_params(self) -> dict:
+        o = np.zeros(len(self.DEM_PARAM_KEYS))
+        self._lib.orc_dem_params(self._h, _dp(o))
+        return dict(zip(self.DEM_PARAM_KEYS, o.tolist()))
 
     def impacts(self) -> np.ndarray:
         cap = 4 * (self.n_balls + 8) + 64
