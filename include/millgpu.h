@@ -379,6 +379,53 @@ int mg_stress_update(mg_stress* stress, const mg_impact_record* d_records, const
 /* One drum's 49 accumulator values: count_rate 4, se_hat_rate 4, hist_rate 28, above_rate_j 6,
  * cum_wall_j, cum_above_j 6 (stress.rs:83-99 field order). */
 int mg_stress_state(mg_stress* stress, uint32_t drum, double* out49);
+/* ---- ball populations: DemState::step_recorded (dem.rs:530-1066) --------------------------------
+ * One population's scalars for one step: the Drum (geometry.rs:18-22, LiftersParams), the drum
+ * angle, the shared ball radius / mass / inertia (dem.rs:277-309), the MediaParams the DEM reads
+ * (params.rs), the solver iterations and the sub-step dt, the ball count, and whether coupling
+ * impulses are applied (step_with_external_forces, dem.rs:513-528). */
+typedef struct mg_dem_drum {
+  float radius_m, omega, drum_angle;
+  uint32_t lifter_count;
+  float lifter_height_m, lifter_base_width_m, lifter_top_width_m, lifter_phase_deg;
+  float ball_radius, ball_mass, ball_inertia;
+  float restitution_ball_ball, restitution_ball_wall, friction_ball_ball, friction_ball_wall;
+  float friction_ball_ball_static, friction_ball_wall_static, friction_velocity_scale_m_s, rolling_friction;
+  uint32_t iterations;
+  float dt;
+  uint32_t n_balls, has_external;
+} mg_dem_drum;
+/* DemStepStats (dem.rs:175-222) plus the step's candidate-pair and impact-record counts and the
+ * number of balls the swept time-of-impact test clamped. */
+typedef struct mg_dem_stats {
+  float wall_work_j;
+  uint32_t collision_count, impact_energy_histogram[12];
+  float dissipated_energy_j, max_substep_displacement_over_diameter, coupling_work_j;
+  uint32_t n_pairs, n_impacts, n_swept;
+} mg_dem_stats;
+typedef struct mg_dem mg_dem;
+/* Workspace for n_drums ball populations of at most max_balls balls each (at most 2048: a
+ * population's ball state is held in LDS during the pair sweeps), pairs_per_ball candidate pairs
+ * per ball (0: 24) and max_impacts impact records per population and step. */
+int mg_dem_create(uint32_t n_drums, uint32_t max_balls, uint32_t pairs_per_ball, uint32_t max_impacts, mg_dem** out,
+                  char* err, size_t errlen);
+void mg_dem_destroy(mg_dem* dem);
+/* DemState::step_recorded (dem.rs:530-1066) for every population at once: population d's balls
+ * follow population d-1's in the device arrays (x, v: 2 floats per ball; theta, omega: 1), updated
+ * in place; d_ext_impulse (2 floats per ball) / d_ext_angular are the coupling impulses of the
+ * populations with has_external (CouplingImpulses, coupling.rs:36-52), else may be NULL. stats
+ * (host, n_drums) receives each population's DemStepStats. Results equal the reference's f32
+ * arithmetic bit for bit: the pair sweeps keep its order, the friction blend uses the host libm's
+ * expf algorithm, impact bins its log10f, the swept time of impact runs on the host with its sinf /
+ * cosf. MG_E_CAPACITY if a population has more candidate pairs than the workspace holds (nothing
+ * after the grid ran) or more impact records (the count is exact, the surplus records dropped). */
+int mg_dem_step(mg_dem* dem, const mg_dem_drum* drums, float* d_x, float* d_v, float* d_theta, float* d_omega,
+                const float* d_ext_impulse, const float* d_ext_angular, mg_dem_stats* stats, char* err,
+                size_t errlen);
+/* The last step's impact records of one population, in the reference's order (ImpactRecord,
+ * dem.rs:152-169); *n receives the count. */
+int mg_dem_impacts(mg_dem* dem, uint32_t drum, mg_impact_record* out, uint32_t cap, uint32_t* n);
+
 /* StressAccum::metrics (stress.rs:186-238) of one drum. */
 int mg_stress_get_metrics(mg_stress* stress, uint32_t drum, uint32_t n_balls, double m3, double scale_j, double d,
                       double power_draw_w, mg_stress_metrics* out);

@@ -1,0 +1,1234 @@
+// DemState::step_recorded on the GPU for a batch of ball populations (crates/mill-core/src/dem.rs:530-1066).
+//
+// The reference advances the balls in this order: sanitize, predict (gravity, optional coupling
+// impulses), a uniform grid over the predicted positions and its candidate pairs, a swept
+// time-of-impact clamp for balls that end deep in the wall, Gauss-Seidel XPBD projection (pairs in
+// the grid's emission order, then walls), velocities from positions, Coulomb friction (pairs
+// sorted by (i, j), then walls), velocities again, restitution with impact records (pairs, then
+// walls), rolling resistance, and the step's energies. Results depend on the order of the pair
+// sweeps, so each population keeps the reference's order: one 256-lane workgroup per population,
+// the per-ball stages spread over the lanes and the pair sweeps walked by lane 0 over LDS copies of
+// the ball state, with the pair list streamed through LDS in chunks by all lanes. Many populations
+// run side by side (one workgroup each): a batch is what fills the GPU.
+//
+// Kernels per step: dem_predict, dem_grid (grid.rs:101-362 restated in one workgroup: keys,
+// bounding box, Dense or Sparse layout, stable counting sort, candidate pairs in emission order),
+// dem_ccd_flag (balls whose end position lies deeper than 0.4 r in the wall), dem_project,
+// dem_sort (the contact book sorted by (i, j)), dem_friction, dem_restitution, dem_finish (rolling
+// resistance and energies). The swept time of impact of the flagged balls (geometry.rs:146-194,
+// rare: a ball crossing the wall within one step) is evaluated on the host, because it rotates
+// the drum to intermediate angles with the libm's sinf / cosf; the host writes the clamped
+// positions back before dem_project.
+//
+// f32 sums the reference forms in ball order (energies, wall work) are formed in ball order by one
+// lane over LDS-staged terms; the friction blend's exp is the host libm's expf algorithm restated
+// (mg_expf), and the impact-energy bin (log10-based, dem.rs:99-109) is a count of 11 thresholds the
+// host finds once with its own log10f, so device results equal the reference's f32 arithmetic.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/millgpu.h"
+#include "mg_drumgeo.h"
+
+#define DE_BLOCK 256
+#define DE_MAX_BALLS 2048
+#define DE_CHUNK 1024  // pairs staged in LDS per chunk of a sequential sweep
+#define DE_NO 0xFFFFFFFFu
+
+namespace {
+
+constexpr float DE_PI = 3.14159265358979323846f;
+constexpr float DE_G = -9.81f;
+
+// One population's scalars for one step (host-prepared).
+struct DeDrum {
+  float radius, omega;
+  uint32_t nl, lift0;
+  float rt, bh, thw;
+  float rc0, rs0;    // from_angle(-angle)
+  float rc1, rs1;    // from_angle(-angle_next)
+  float wc1, ws1;    // from_angle(angle_next)
+  float r, mass, inertia, w, w_rot, thr;
+  float rest_bb, rest_bw, fric_bb, fric_bw, fric_bb_s, fric_bw_s, fric_vs, rolling;
+  uint32_t iters;
+  float dt;
+  uint32_t b0, n;    // first ball in the caller's arrays, ball count
+  uint32_t has_ext;
+};
+
+// Per population results of a step.
+struct DeOut {
+  float e_pred, maxd, maxs, ww, e_final;
+  double cw;
+  uint32_t n_pairs, pair_overflow, n_ccd, coll, n_imp;
+  uint32_t hist[12];
+};
+
+struct DeItem {  // a sorted contact: (i, j) and its accumulated normal impulse
+  uint32_t i, j;
+  float lam;
+};
+
+__device__ __forceinline__ mgg::Geo de_geo(const DeDrum& D, const float2* lift) {
+  return mgg::Geo{D.radius, D.omega, D.nl, D.rt, D.bh, D.thw, lift + D.lift0};
+}
+
+// angle_diff (dem.rs:1069-1078): a - b wrapped into [-pi, pi] by whole turns (bounded).
+__device__ __forceinline__ float de_angle_diff(float a, float b) {
+  float d = a - b;
+  for (int k = 0; k < 64 && d > DE_PI; k++) d -= 2.0f * DE_PI;
+  for (int k = 0; k < 64 && d < -DE_PI; k++) d += 2.0f * DE_PI;
+  return d;
+}
+
+// The host libm's expf (the exp2-table method with a degree-3 polynomial, evaluated in double and
+// rounded once), restated so the friction blend equals the reference's f32::exp.
+__device__ __constant__ uint64_t kExp2Tab[32] = {
+    0x3ff0000000000000ull, 0x3fefd9b0d3158574ull, 0x3fefb5586cf9890full, 0x3fef9301d0125b51ull,
+    0x3fef72b83c7d517bull, 0x3fef54873168b9aaull, 0x3fef387a6e756238ull, 0x3fef1e9df51fdee1ull,
+    0x3fef06fe0a31b715ull, 0x3feef1a7373aa9cbull, 0x3feedea64c123422ull, 0x3feece086061892dull,
+    0x3feebfdad5362a27ull, 0x3feeb42b569d4f82ull, 0x3feeab07dd485429ull, 0x3feea47eb03a5585ull,
+    0x3feea09e667f3bcdull, 0x3fee9f75e8ec5f74ull, 0x3feea11473eb0187ull, 0x3feea589994cce13ull,
+    0x3feeace5422aa0dbull, 0x3feeb737b0cdc5e5ull, 0x3feec49182a3f090ull, 0x3feed503b23e255dull,
+    0x3feee89f995ad3adull, 0x3feeff76f2fb5e47ull, 0x3fef199bdd85529cull, 0x3fef3720dcef9069ull,
+    0x3fef5818dcfba487ull, 0x3fef7c97337b9b5full, 0x3fefa4afa2a490daull, 0x3fefd0765b6e4540ull};
+__device__ inline float de_expf(float x) {
+  const uint32_t ux = __float_as_uint(x);
+  const uint32_t abstop = (ux >> 20) & 0x7ffu;
+  if (abstop >= 0x42bu) {  // |x| >= 88, infinities, NaN
+    if (ux == 0xff800000u) return 0.0f;
+    if (abstop >= 0x7f8u) return x + x;
+    if (x > 0x1.62e42ep6f) return INFINITY;
+    if (x < -0x1.9fe368p6f) return 0.0f;
+  }
+  const double inv_ln2_n = 0x1.71547652b82fep+0 * 32, shift = 0x1.8p+52;
+  const double c0 = 0x1.c6af84b912394p-5 / 32 / 32 / 32, c1 = 0x1.ebfce50fac4f3p-3 / 32 / 32,
+               c2 = 0x1.62e42ff0c52d6p-1 / 32;
+  double z = inv_ln2_n * (double)x;
+  double kd = z + shift;
+  const uint64_t ki = (uint64_t)__double_as_longlong(kd);
+  kd -= shift;
+  const double r = z - kd;
+  const uint64_t t = kExp2Tab[ki % 32] + (ki << 47);
+  const double s = __longlong_as_double((long long)t);
+  z = c0 * r + c1;
+  const double r2 = r * r;
+  double y = c2 * r + 1.0;
+  y = z * r2 + y;
+  y = y * s;
+  return (float)y;
+}
+
+// impact_energy_bin (dem.rs:99-109) as a count of the host's 11 thresholds; -1 for no bin.
+__device__ __forceinline__ int de_ebin(float e, const float* __restrict__ thr) {
+  if (!isfinite(e) || e <= 0.0f) return -1;
+  int b = 0;
+  for (int k = 0; k < 11; k++) b += e >= thr[k] ? 1 : 0;
+  return b;
+}
+
+// effective_friction (dem.rs:128-138).
+__device__ inline float de_friction(float mk, float ms, float vt, float vs) {
+  if (ms <= mk) return mk;
+  const float ratio = vt / vs;
+  return mk + (ms - mk) * de_expf(-(ratio * ratio));
+}
+
+__device__ __forceinline__ int32_t de_sat_i32(float v) {
+  if (isnan(v)) return 0;
+  if (v <= -2147483648.0f) return INT32_MIN;
+  if (v >= 2147483647.0f) return INT32_MAX;
+  return (int32_t)v;
+}
+__device__ __forceinline__ int32_t de_sadd(int32_t a, int32_t b) {
+  const int64_t r = (int64_t)a + b;
+  return r > INT32_MAX ? INT32_MAX : (r < INT32_MIN ? INT32_MIN : (int32_t)r);
+}
+__device__ __forceinline__ float de_wave_max(float v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ int32_t de_wave_min_i(int32_t v) {
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ int32_t de_wave_max_i(int32_t v) {
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Exclusive scan of one value per lane over the workgroup; *total gets the sum.
+__device__ uint32_t de_block_scan(uint32_t v, uint32_t* s_w, uint32_t* total) {
+  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t inc = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t u = __shfl_up(inc, o, 64);
+    if (lane >= (uint32_t)o) inc += u;
+  }
+  if (lane == 63) s_w[w] = inc;
+  __syncthreads();
+  uint32_t before = 0, tot = 0;
+  for (uint32_t q = 0; q < DE_BLOCK / 64; q++) {
+    before += q < w ? s_w[q] : 0u;
+    tot += s_w[q];
+  }
+  __syncthreads();
+  *total = tot;
+  return before + inc - v;
+}
+
+// Mechanical energy of one ball's three terms, in the order mechanical_energy_j adds them
+// (dem.rs:253-261).
+__device__ __forceinline__ void de_mech_terms(const DeDrum& D, float2 x, float2 v, float om, float* t) {
+  t[0] = 0.5f * D.mass * (v.x * v.x + v.y * v.y);
+  t[1] = 0.5f * D.inertia * om * om;
+  t[2] = D.mass * 9.81f * x.y;
+}
+
+}  // namespace
+
+// ---- 1. sanitize + predict (dem.rs:555-598) -------------------------------------------------------
+extern "C" __global__ void __launch_bounds__(DE_BLOCK)
+dem_predict(const DeDrum* __restrict__ drums, float2* __restrict__ x, float2* __restrict__ v, float* __restrict__ th,
+            float* __restrict__ om, const float2* __restrict__ ext_imp, const float* __restrict__ ext_ang,
+            float2* __restrict__ x0, float2* __restrict__ vpre, float* __restrict__ th0, float* __restrict__ ompre,
+            uint32_t stride, DeOut* __restrict__ out) {
+  __shared__ __align__(16) float s_e[3 * DE_BLOCK];
+  __shared__ double s_cw[DE_BLOCK];
+  __shared__ float s_red[DE_BLOCK / 64];
+  const DeDrum D = drums[blockIdx.x];
+  const uint32_t t = threadIdx.x;
+  const size_t sb = (size_t)blockIdx.x * stride;
+  float e = 0.0f;   // lane 0: mechanical energy, three terms per ball in ball order
+  double cw = 0.0;  // lane 0: coupling work in ball order
+  float maxd = 0.0f, maxs = 0.0f;
+  for (uint32_t base = 0; base < D.n; base += DE_BLOCK) {
+    const uint32_t i = base + t;
+    if (i < D.n) {
+      const uint32_t g = D.b0 + i;
+      float2 xi = x[g], vi = v[g];
+      float thi = th[g], omi = om[g];
+      if (!(isfinite(xi.x) && isfinite(xi.y)) || !(isfinite(vi.x) && isfinite(vi.y))) {
+        xi = make_float2(0.0f, 0.0f);
+        vi = make_float2(0.0f, 0.0f);
+      }
+      x0[sb + i] = xi;
+      vpre[sb + i] = vi;
+      th0[sb + i] = thi;
+      ompre[sb + i] = omi;
+      vi.y += DE_G * D.dt;
+      double c = 0.0;
+      if (D.has_ext) {
+        const double kb = 0.5 * (double)D.mass * (double)(vi.x * vi.x + vi.y * vi.y) +
+                          0.5 * (double)D.inertia * (double)(omi * omi);
+        const float2 im = ext_imp[g];
+        vi.x += im.x * D.w;
+        vi.y += im.y * D.w;
+        omi += ext_ang[g] * D.w_rot;
+        const double ka = 0.5 * (double)D.mass * (double)(vi.x * vi.x + vi.y * vi.y) +
+                          0.5 * (double)D.inertia * (double)(omi * omi);
+        c = ka - kb;
+      }
+      xi.x += vi.x * D.dt;
+      xi.y += vi.y * D.dt;
+      thi += omi * D.dt;
+      x[g] = xi;
+      v[g] = vi;
+      th[g] = thi;
+      om[g] = omi;
+      const float sp = sqrtf(vi.x * vi.x + vi.y * vi.y);
+      maxd = fmaxf(maxd, sp * D.dt / fmaxf(2.0f * D.r, 1e-12f));
+      maxs = fmaxf(maxs, sp);
+      de_mech_terms(D, xi, vi, omi, &s_e[3 * t]);
+      s_cw[t] = c;
+    }
+    __syncthreads();
+    if (t == 0) {
+      const uint32_t m = min(D.n - base, (uint32_t)DE_BLOCK);
+      for (uint32_t q = 0; q < 3 * m; q++) e += s_e[q];
+      if (D.has_ext)
+        for (uint32_t q = 0; q < m; q++) cw += s_cw[q];
+    }
+    __syncthreads();
+  }
+  maxd = de_wave_max(maxd);
+  maxs = de_wave_max(maxs);
+  if ((t & 63) == 0) s_red[t >> 6] = maxd;
+  __syncthreads();
+  float md = s_red[0];
+  for (int q = 1; q < DE_BLOCK / 64; q++) md = fmaxf(md, s_red[q]);
+  __syncthreads();
+  if ((t & 63) == 0) s_red[t >> 6] = maxs;
+  __syncthreads();
+  if (t == 0) {
+    float ms = s_red[0];
+    for (int q = 1; q < DE_BLOCK / 64; q++) ms = fmaxf(ms, s_red[q]);
+    DeOut& o = out[blockIdx.x];
+    o.e_pred = e;
+    o.cw = cw;
+    o.maxd = md;
+    o.maxs = ms;
+  }
+}
+
+// ---- 2. grid + candidate pairs (grid.rs:101-362 over the predicted positions, dem.rs:600-608) ------
+namespace {
+
+struct DeGridLds {
+  int2 key[DE_MAX_BALLS];              // cell key per ball (finite balls)
+  uint8_t fin[DE_MAX_BALLS];           // 1: finite position
+  uint32_t idx[DE_MAX_BALLS];          // ball indices in cell order
+  uint32_t start[2 * DE_MAX_BALLS + 514];  // cell starts (Dense: W*H + 1; Sparse: cells + 1)
+  int2 ckey[DE_MAX_BALLS];             // Sparse: occupied cells' keys in (cx, cy) order
+  uint32_t w[DE_BLOCK / 64];
+  int32_t box[4];
+  uint32_t m;
+};
+
+__device__ __forceinline__ bool de_key_less(int2 a, int2 b) { return a.x < b.x || (a.x == b.x && a.y < b.y); }
+
+// Appends the candidate pair (min, max) with a zero normal impulse; counts pairs beyond the capacity.
+__device__ __forceinline__ void de_emit(uint2* P, float* Lm, uint32_t cap, uint32_t& np, uint32_t a, uint32_t b) {
+  if (np < cap) {
+    P[np] = make_uint2(min(a, b), max(a, b));
+    Lm[np] = 0.0f;
+  }
+  np++;
+}
+
+// Range of cell (cx, cy) in idx; false if it is empty or absent (grid.rs:242-271).
+__device__ bool de_range(const DeGridLds& L, bool dense, int32_t mnx, int32_t mny, int64_t W, int64_t H, uint32_t nc,
+                         int32_t cx, int32_t cy, uint32_t& s, uint32_t& e) {
+  if (dense) {
+    const int64_t dx = (int64_t)cx - mnx, dy = (int64_t)cy - mny;
+    if (dx < 0 || dy < 0 || dx >= W || dy >= H) return false;
+    const uint32_t k = (uint32_t)(dx * H + dy);
+    s = L.start[k];
+    e = L.start[k + 1];
+    return s != e;
+  }
+  uint32_t lo = 0, hi = nc;  // lower bound of (cx, cy) in the sorted keys
+  const int2 q = make_int2(cx, cy);
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (de_key_less(L.ckey[mid], q)) lo = mid + 1; else hi = mid;
+  }
+  if (lo == nc || L.ckey[lo].x != cx || L.ckey[lo].y != cy) return false;
+  s = L.start[lo];
+  e = L.start[lo + 1];
+  return true;
+}
+
+}  // namespace
+
+extern "C" __global__ void __launch_bounds__(DE_BLOCK)
+dem_grid(const DeDrum* __restrict__ drums, const float2* __restrict__ x, uint2* __restrict__ pairs,
+         float* __restrict__ lam, uint32_t pcap, DeOut* __restrict__ out) {
+  __shared__ DeGridLds L;
+  const DeDrum D = drums[blockIdx.x];
+  DeOut& o = out[blockIdx.x];
+  const uint32_t t = threadIdx.x, n = D.n;
+  const float margin = (float)max(D.iters, 1u) * 0.2f * 2.0f * D.r;
+  const float cell = fmaxf(fmaxf(2.0f * D.r * 1.05f, 2.0f * D.r + o.maxs * D.dt + margin), 1e-6f);
+  // keys and the bounding box of the finite points
+  int32_t mnx = INT32_MAX, mny = INT32_MAX, mxx = INT32_MIN, mxy = INT32_MIN;
+  uint32_t cnt = 0;
+  for (uint32_t i = t; i < n; i += DE_BLOCK) {
+    const float2 p = x[D.b0 + i];
+    const bool f = isfinite(p.x) && isfinite(p.y);
+    L.fin[i] = f ? 1 : 0;
+    if (f) {
+      const int2 k = make_int2(de_sat_i32(floorf(p.x / cell)), de_sat_i32(floorf(p.y / cell)));
+      L.key[i] = k;
+      mnx = min(mnx, k.x); mny = min(mny, k.y); mxx = max(mxx, k.x); mxy = max(mxy, k.y);
+      cnt++;
+    }
+  }
+  mnx = de_wave_min_i(mnx); mny = de_wave_min_i(mny); mxx = de_wave_max_i(mxx); mxy = de_wave_max_i(mxy);
+  for (int q = 32; q > 0; q >>= 1) cnt += __shfl_xor(cnt, q, 64);
+  if (t == 0) { L.box[0] = INT32_MAX; L.box[1] = INT32_MAX; L.box[2] = INT32_MIN; L.box[3] = INT32_MIN; L.m = 0; }
+  __syncthreads();
+  if ((t & 63) == 0) {
+    atomicMin(&L.box[0], mnx); atomicMin(&L.box[1], mny); atomicMax(&L.box[2], mxx); atomicMax(&L.box[3], mxy);
+    atomicAdd(&L.m, cnt);
+  }
+  __syncthreads();
+  const uint32_t m = L.m;
+  mnx = L.box[0]; mny = L.box[1];
+  const int64_t W = m ? (int64_t)L.box[2] - mnx + 1 : 0, H = m ? (int64_t)L.box[3] - mny + 1 : 0;
+  const uint64_t budget = 2ull * m + 512ull;  // grid.rs:113-120: Dense iff W * H <= 2 m + 512
+  const bool dense = m > 0 && (uint64_t)W <= budget && (uint64_t)H <= budget && (uint64_t)W * (uint64_t)H <= budget;
+  const uint64_t total = dense ? (uint64_t)W * (uint64_t)H : 0u;
+  uint32_t nc = 0;  // cells of the layout (Dense: W*H; Sparse: occupied)
+  if (dense) {
+    nc = (uint32_t)total;
+    for (uint32_t k = t; k <= nc; k += DE_BLOCK) L.start[k] = 0;
+    __syncthreads();
+    for (uint32_t i = t; i < n; i += DE_BLOCK)
+      if (L.fin[i]) {
+        const uint32_t k = (uint32_t)(((int64_t)L.key[i].x - mnx) * H + ((int64_t)L.key[i].y - mny));
+        atomicAdd(&L.start[k + 1], 1u);
+      }
+    __syncthreads();
+    if (t == 0) {  // inclusive scan of the counts, then the stable scatter in ball order
+      for (uint32_t k = 1; k <= nc; k++) L.start[k] += L.start[k - 1];
+      for (uint32_t i = 0; i < n; i++)
+        if (L.fin[i]) {
+          const uint32_t k = (uint32_t)(((int64_t)L.key[i].x - mnx) * H + ((int64_t)L.key[i].y - mny));
+          // start[k] is advanced while filling and restored below from start[k + 1]'s prefix
+          L.idx[L.start[k]++] = i;
+        }
+      for (uint32_t k = nc; k > 0; k--) L.start[k] = L.start[k - 1];
+      L.start[0] = 0;
+    }
+    __syncthreads();
+  } else if (m > 0) {
+    if (t == 0) {  // Sparse: finite balls ordered by (key, index) (insertion into runs of equal keys)
+      uint32_t c = 0;
+      for (uint32_t i = 0; i < n; i++) {
+        if (!L.fin[i]) continue;
+        const int2 k = L.key[i];
+        uint32_t p = c;  // keep idx sorted by key, ascending index inside equal keys
+        while (p > 0 && de_key_less(k, L.key[L.idx[p - 1]])) {
+          L.idx[p] = L.idx[p - 1];
+          p--;
+        }
+        L.idx[p] = i;
+        c++;
+      }
+      L.start[0] = 0;
+      for (uint32_t a = 0; a < c; a++) {
+        const int2 k = L.key[L.idx[a]];
+        if (a == 0 || k.x != L.ckey[nc - 1].x || k.y != L.ckey[nc - 1].y) {
+          if (a > 0) L.start[nc] = a;
+          L.ckey[nc++] = k;
+        }
+      }
+      L.start[nc] = c;
+      L.w[0] = nc;
+    }
+    __syncthreads();
+    nc = L.w[0];
+    __syncthreads();
+  }
+  // candidate pairs in emission order (grid.rs:291-362), lane 0
+  if (t == 0) {
+    uint32_t np = 0;
+    uint2* P = pairs + (size_t)blockIdx.x * pcap;
+    float* Lm = lam + (size_t)blockIdx.x * pcap;
+    for (uint32_t k = 0; k < nc && m > 0; k++) {
+      const uint32_t s = L.start[k], e = L.start[k + 1];
+      if (s == e) continue;
+      int32_t cx, cy;
+      if (dense) {
+        cx = (int32_t)((int64_t)mnx + (int64_t)k / H);
+        cy = (int32_t)((int64_t)mny + (int64_t)k % H);
+      } else {
+        cx = L.ckey[k].x;
+        cy = L.ckey[k].y;
+      }
+      for (uint32_t a = s; a < e; a++)
+        for (uint32_t b = a + 1; b < e; b++) de_emit(P, Lm, pcap, np, L.idx[a], L.idx[b]);
+      for (int q = 0; q < 4; q++) {  // forward cells (1, 0), (1, 1), (0, 1), (-1, 1)
+        const int32_t ox = q < 2 ? 1 : (q == 2 ? 0 : -1), oy = q == 0 ? 0 : 1;
+        const int32_t tx = de_sadd(cx, ox), ty = de_sadd(cy, oy);
+        if (tx == cx && ty == cy) continue;
+        uint32_t os, oe;
+        if (!de_range(L, dense, mnx, mny, W, H, nc, tx, ty, os, oe)) continue;
+        for (uint32_t a = s; a < e; a++)
+          for (uint32_t b = os; b < oe; b++) de_emit(P, Lm, pcap, np, L.idx[a], L.idx[b]);
+      }
+    }
+    o.n_pairs = np;
+    o.pair_overflow = np > pcap ? 1u : 0u;
+  }
+}
+
+// ---- 3. balls ending deep in the wall (dem.rs:611-621): listed for the host's swept test ----------
+extern "C" __global__ void __launch_bounds__(DE_BLOCK)
+dem_ccd_flag(const DeDrum* __restrict__ drums, const float2* __restrict__ lift, const float2* __restrict__ x,
+             uint32_t* __restrict__ flagged, uint32_t stride, DeOut* __restrict__ out) {
+  __shared__ uint32_t s_n;
+  const DeDrum D = drums[blockIdx.x];
+  const mgg::Geo g = de_geo(D, lift);
+  if (threadIdx.x == 0) s_n = 0;
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < D.n; i += DE_BLOCK) {
+    const float2 p = x[D.b0 + i];
+    const float c = mgg::sdf_world_d(p.x, p.y, D.rc1, D.rs1, g) - D.r;
+    if (c < -0.2f * 2.0f * D.r) flagged[(size_t)blockIdx.x * stride + atomicAdd(&s_n, 1u)] = i;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) out[blockIdx.x].n_ccd = s_n;
+}
+
+// Writes the host's clamped positions: list of (population, ball) and positions.
+extern "C" __global__ void dem_ccd_apply(const DeDrum* __restrict__ drums, const uint2* __restrict__ who,
+                                        const float2* __restrict__ pos, uint32_t count, float2* __restrict__ x) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  x[drums[who[k].x].b0 + who[k].y] = pos[k];
+}
+
+// ---- 4. XPBD projection (dem.rs:627-668) and the velocity rebuild (:670-673) ---------------------
+extern "C" __global__ void __launch_bounds__(DE_BLOCK)
+dem_project(const DeDrum* __restrict__ drums, const float2* __restrict__ lift, float2* __restrict__ x,
+            float2* __restrict__ v, const float* __restrict__ th, float* __restrict__ om,
+            const float2* __restrict__ x0, const float* __restrict__ th0, const uint2* __restrict__ pairs,
+            float* __restrict__ lam, float* __restrict__ lam_bw, uint32_t stride, uint32_t pcap,
+            const DeOut* __restrict__ out) {
+  __shared__ float2 s_x[DE_MAX_BALLS];
+  __shared__ float s_lbw[DE_MAX_BALLS];
+  __shared__ uint2 s_p[DE_CHUNK];
+  __shared__ float s_l[DE_CHUNK];
+  const DeDrum D = drums[blockIdx.x];
+  const mgg::Geo g = de_geo(D, lift);
+  const uint32_t t = threadIdx.x, n = D.n;
+  const size_t sb = (size_t)blockIdx.x * stride, pb = (size_t)blockIdx.x * pcap;
+  const uint32_t np = min(out[blockIdx.x].n_pairs, pcap);
+  for (uint32_t i = t; i < n; i += DE_BLOCK) {
+    s_x[i] = x[D.b0 + i];
+    s_lbw[i] = 0.0f;
+  }
+  __syncthreads();
+  const float w = D.w, ws = D.w + D.w, cap_c = -0.2f * 2.0f * D.r, two_r = 2.0f * D.r;
+  for (uint32_t it = 0; it < max(D.iters, 1u); it++) {
+    for (uint32_t c0 = 0; c0 < np; c0 += DE_CHUNK) {
+      const uint32_t m = min(np - c0, (uint32_t)DE_CHUNK);
+      for (uint32_t q = t; q < m; q += DE_BLOCK) {
+        s_p[q] = pairs[pb + c0 + q];
+        s_l[q] = lam[pb + c0 + q];
+      }
+      __syncthreads();
+      if (t == 0) {
+        for (uint32_t q = 0; q < m; q++) {
+          const uint32_t i = s_p[q].x, j = s_p[q].y;
+          const float dx = s_x[i].x - s_x[j].x, dy = s_x[i].y - s_x[j].y;
+          const float dist = sqrtf(dx * dx + dy * dy);
+          const float c = dist - two_r;
+          if (c >= 0.0f) continue;
+          float nx = 1.0f, ny = 0.0f;
+          if (dist > 1e-9f) {
+            nx = dx / dist;
+            ny = dy / dist;
+          }
+          if (ws <= 0.0f) continue;
+          const float ce = fmaxf(c, cap_c);
+          const float dl = -ce / ws;
+          const float s = w * dl;
+          s_x[i].x += s * nx;
+          s_x[i].y += s * ny;
+          s_x[j].x -= s * nx;
+          s_x[j].y -= s * ny;
+          s_l[q] += dl;
+        }
+      }
+      __syncthreads();
+      for (uint32_t q = t; q < m; q += DE_BLOCK) lam[pb + c0 + q] = s_l[q];
+      __syncthreads();
+    }
+    for (uint32_t i = t; i < n; i += DE_BLOCK) {  // walls: each ball on its own
+      float nx, ny;
+      const float d = mgg::sdf_world(s_x[i].x, s_x[i].y, D.rc1, D.rs1, D.wc1, D.ws1, g, nx, ny);
+      const float c = d - D.r;
+      if (c >= 0.0f) continue;
+      if (w <= 0.0f) continue;
+      const float ce = fmaxf(c, cap_c);
+      const float dl = -ce / w;
+      const float s = w * dl;
+      s_x[i].x += s * nx;
+      s_x[i].y += s * ny;
+      s_lbw[i] += dl;
+    }
+    __syncthreads();
+  }
+  for (uint32_t i = t; i < n; i += DE_BLOCK) {
+    const float2 p = s_x[i], p0 = x0[sb + i];
+    x[D.b0 + i] = p;
+    v[D.b0 + i] = make_float2((p.x - p0.x) / D.dt, (p.y - p0.y) / D.dt);
+    om[D.b0 + i] = de_angle_diff(th[D.b0 + i], th0[sb + i]) / D.dt;
+    lam_bw[sb + i] = s_lbw[i];
+  }
+}
+
+// ---- 5. ContactBook::sorted_ball_ball (dem.rs:459-467): by i (counting sort), then j ----------------
+extern "C" __global__ void __launch_bounds__(DE_BLOCK)
+dem_sort(const DeDrum* __restrict__ drums, const uint2* __restrict__ pairs, const float* __restrict__ lam,
+         DeItem* __restrict__ sorted, uint32_t pcap, const DeOut* __restrict__ out) {
+  __shared__ uint32_t s_c[DE_MAX_BALLS + 1];
+  __shared__ uint32_t s_w[DE_BLOCK / 64];
+  __shared__ uint32_t s_carry;
+  const DeDrum D = drums[blockIdx.x];
+  const uint32_t t = threadIdx.x, n = D.n;
+  const size_t pb = (size_t)blockIdx.x * pcap;
+  const uint32_t np = min(out[blockIdx.x].n_pairs, pcap);
+  for (uint32_t i = t; i <= n; i += DE_BLOCK) s_c[i] = 0;
+  if (t == 0) s_carry = 0;
+  __syncthreads();
+  for (uint32_t q = t; q < np; q += DE_BLOCK) atomicAdd(&s_c[pairs[pb + q].x], 1u);
+  __syncthreads();
+  for (uint32_t b = 0; b < n; b += DE_BLOCK) {  // exclusive scan of the counts in place
+    const uint32_t i = b + t;
+    const uint32_t c = i < n ? s_c[i] : 0u;
+    uint32_t tot;
+    const uint32_t ex = de_block_scan(c, s_w, &tot);
+    const uint32_t carry = s_carry;
+    if (i < n) s_c[i] = carry + ex;
+    __syncthreads();
+    if (t == 0) s_carry = carry + tot;
+    __syncthreads();
+  }
+  if (t == 0) s_c[n] = s_carry;
+  __syncthreads();
+  // each pair takes a slot of its bucket (any order; s_c[i] advances from the bucket's start to
+  // its end) and each bucket is then sorted by j
+  for (uint32_t q = t; q < np; q += DE_BLOCK) {
+    const uint2 p = pairs[pb + q];
+    const uint32_t slot = atomicAdd(&s_c[p.x], 1u);
+    sorted[pb + slot] = DeItem{p.x, p.y, lam[pb + q]};
+  }
+  __syncthreads();
+  // s_c[i] now holds the end of bucket i (= start of bucket i + 1 before the fill): bucket i spans
+  // [start, s_c[i]) with start = (i == 0 ? 0 : s_c[i - 1])
+  for (uint32_t i = t; i < n; i += DE_BLOCK) {
+    const uint32_t s = i == 0 ? 0u : s_c[i - 1], e = s_c[i];
+    for (uint32_t a = s + 1; a < e; a++) {  // insertion sort by j (buckets hold a few pairs)
+      const DeItem it = sorted[pb + a];
+      uint32_t b = a;
+      while (b > s && sorted[pb + b - 1].j > it.j) {
+        sorted[pb + b] = sorted[pb + b - 1];
+        b--;
+      }
+      sorted[pb + b] = it;
+    }
+  }
+}
+
+// ---- 6. friction (dem.rs:676-890) and the velocity rebuild (:892-895) ------------------------------
+extern "C" __global__ void __launch_bounds__(DE_BLOCK)
+dem_friction(const DeDrum* __restrict__ drums, const float2* __restrict__ lift, float2* __restrict__ x,
+             float2* __restrict__ v, float* __restrict__ th, float* __restrict__ om, const float2* __restrict__ x0,
+             const float* __restrict__ th0, const DeItem* __restrict__ sorted, const float* __restrict__ lam_bw,
+             uint32_t stride, uint32_t pcap, DeOut* __restrict__ out) {
+  __shared__ float2 s_x[DE_MAX_BALLS], s_v[DE_MAX_BALLS];
+  __shared__ float s_th[DE_MAX_BALLS], s_om[DE_MAX_BALLS];
+  __shared__ DeItem s_it[DE_CHUNK];
+  __shared__ float s_ww[2 * DE_BLOCK];
+  const DeDrum D = drums[blockIdx.x];
+  const mgg::Geo g = de_geo(D, lift);
+  const uint32_t t = threadIdx.x, n = D.n;
+  const size_t sb = (size_t)blockIdx.x * stride, pb = (size_t)blockIdx.x * pcap;
+  const uint32_t np = min(out[blockIdx.x].n_pairs, pcap);
+  for (uint32_t i = t; i < n; i += DE_BLOCK) {
+    s_x[i] = x[D.b0 + i];
+    s_v[i] = v[D.b0 + i];
+    s_th[i] = th[D.b0 + i];
+    s_om[i] = om[D.b0 + i];
+  }
+  __syncthreads();
+  const float r = D.r, w = D.w, wr = D.w_rot, dt = D.dt;
+  const float wst_bb = w + w + r * r * wr + r * r * wr;
+  for (uint32_t c0 = 0; c0 < np; c0 += DE_CHUNK) {
+    const uint32_t m = min(np - c0, (uint32_t)DE_CHUNK);
+    for (uint32_t q = t; q < m; q += DE_BLOCK) s_it[q] = sorted[pb + c0 + q];
+    __syncthreads();
+    if (t == 0) {
+      for (uint32_t q = 0; q < m; q++) {
+        const DeItem c = s_it[q];
+        if (c.lam <= 0.0f) continue;
+        const uint32_t i = c.i, j = c.j;
+        const float dx = s_x[i].x - s_x[j].x, dy = s_x[i].y - s_x[j].y;
+        const float dist = sqrtf(dx * dx + dy * dy);
+        float nx = 1.0f, ny = 0.0f;
+        if (dist > 1e-9f) {
+          nx = dx / dist;
+          ny = dy / dist;
+        }
+        const float tx = -ny, ty = nx;
+        const float vt = ((s_v[i].x - s_v[j].x) * tx + (s_v[i].y - s_v[j].y) * ty) - r * s_om[i] - r * s_om[j];
+        if (wst_bb <= 0.0f) continue;
+        const float raw = -vt * dt / wst_bb;
+        const float mm = de_friction(D.fric_bb, D.fric_bb_s, vt, D.fric_vs) * c.lam;
+        float dlt = raw;
+        if (dlt < -mm) dlt = -mm;
+        if (dlt > mm) dlt = mm;
+        const float s = w * dlt;
+        s_x[i].x += s * tx;
+        s_x[i].y += s * ty;
+        s_x[j].x -= s * tx;
+        s_x[j].y -= s * ty;
+        const float dth = r * wr * dlt;
+        s_th[i] -= dth;
+        s_th[j] -= dth;
+        const float dvx = s * tx / dt, dvy = s * ty / dt;
+        const float dom = -r * wr * dlt / dt;
+        s_v[i].x += dvx;
+        s_v[i].y += dvy;
+        s_v[j].x -= dvx;
+        s_v[j].y -= dvy;
+        s_om[i] += dom;
+        s_om[j] += dom;
+      }
+    }
+    __syncthreads();
+  }
+  // walls: each ball on its own; the wall-work terms are added in ball order by lane 0
+  float ww = 0.0f;
+  const float wst_bw = w + r * r * wr;
+  for (uint32_t base = 0; base < n; base += DE_BLOCK) {
+    const uint32_t i = base + t;
+    float a = 0.0f, b = 0.0f;  // an inactive ball adds +0.0 (the running sum is never -0.0)
+    if (i < n) {
+      const float lamw = lam_bw[sb + i];
+      if (lamw > 0.0f) {
+        float nx, ny;
+        mgg::sdf_world(s_x[i].x, s_x[i].y, D.rc1, D.rs1, D.wc1, D.ws1, g, nx, ny);
+        const float tx = -ny, ty = nx;
+        const float cx = s_x[i].x - r * nx, cy = s_x[i].y - r * ny;
+        const float vwx = D.omega * -cy, vwy = D.omega * cx;
+        const float vt = ((s_v[i].x - vwx) * tx + (s_v[i].y - vwy) * ty) - r * s_om[i];
+        if (wst_bw > 0.0f) {
+          const float raw = -vt * dt / wst_bw;
+          const float mm = de_friction(D.fric_bw, D.fric_bw_s, vt, D.fric_vs) * lamw;
+          float dlt = raw;
+          if (dlt < -mm) dlt = -mm;
+          if (dlt > mm) dlt = mm;
+          const float s = w * dlt;
+          s_x[i].x += s * tx;
+          s_x[i].y += s * ty;
+          s_th[i] -= r * wr * dlt;
+          s_v[i].x += s * tx / dt;
+          s_v[i].y += s * ty / dt;
+          s_om[i] += -r * wr * dlt / dt;
+          a = (dlt / dt) * (tx * vwx + ty * vwy);
+          b = (lamw / dt) * (nx * vwx + ny * vwy);
+        }
+      }
+    }
+    s_ww[2 * t] = a;
+    s_ww[2 * t + 1] = b;
+    __syncthreads();
+    if (t == 0) {
+      const uint32_t mm = min(n - base, (uint32_t)DE_BLOCK);
+      for (uint32_t q = 0; q < 2 * mm; q++) ww += s_ww[q];
+    }
+    __syncthreads();
+  }
+  for (uint32_t i = t; i < n; i += DE_BLOCK) {
+    const float2 p = s_x[i], p0 = x0[sb + i];
+    x[D.b0 + i] = p;
+    v[D.b0 + i] = make_float2((p.x - p0.x) / dt, (p.y - p0.y) / dt);
+    th[D.b0 + i] = s_th[i];
+    om[D.b0 + i] = de_angle_diff(s_th[i], th0[sb + i]) / dt;
+  }
+  if (t == 0) out[blockIdx.x].ww = ww;
+}
+
+// ---- 7. restitution and impact records (dem.rs:897-1034) ------------------------------------------
+extern "C" __global__ void __launch_bounds__(DE_BLOCK)
+dem_restitution(const DeDrum* __restrict__ drums, const float2* __restrict__ lift, const float2* __restrict__ x,
+                float2* __restrict__ v, const float2* __restrict__ vpre, const float* __restrict__ ompre,
+                const DeItem* __restrict__ sorted, const float* __restrict__ lam_bw, const float* __restrict__ ebin,
+                mg_impact_record* __restrict__ rec, uint32_t rcap, uint32_t stride, uint32_t pcap,
+                DeOut* __restrict__ out) {
+  __shared__ float2 s_x[DE_MAX_BALLS], s_v[DE_MAX_BALLS], s_vp[DE_MAX_BALLS];
+  __shared__ float s_op[DE_MAX_BALLS];
+  __shared__ DeItem s_it[DE_CHUNK / 2];
+  __shared__ uint32_t s_hist[12], s_coll, s_nrec;
+  __shared__ uint32_t s_w[DE_BLOCK / 64];
+  const DeDrum D = drums[blockIdx.x];
+  const mgg::Geo g = de_geo(D, lift);
+  const uint32_t t = threadIdx.x, n = D.n;
+  const size_t sb = (size_t)blockIdx.x * stride, pb = (size_t)blockIdx.x * pcap, rb = (size_t)blockIdx.x * rcap;
+  const uint32_t np = min(out[blockIdx.x].n_pairs, pcap);
+  for (uint32_t i = t; i < n; i += DE_BLOCK) {
+    s_x[i] = x[D.b0 + i];
+    s_v[i] = v[D.b0 + i];
+    s_vp[i] = vpre[sb + i];
+    s_op[i] = ompre[sb + i];
+  }
+  if (t < 12) s_hist[t] = 0;
+  if (t == 0) { s_coll = 0; s_nrec = 0; }
+  __syncthreads();
+  const float r = D.r, w = D.w;
+  for (uint32_t c0 = 0; c0 < np; c0 += DE_CHUNK / 2) {
+    const uint32_t m = min(np - c0, (uint32_t)(DE_CHUNK / 2));
+    for (uint32_t q = t; q < m; q += DE_BLOCK) s_it[q] = sorted[pb + c0 + q];
+    __syncthreads();
+    if (t == 0) {
+      for (uint32_t q = 0; q < m; q++) {
+        const DeItem c = s_it[q];
+        if (c.lam <= 0.0f) continue;
+        const uint32_t i = c.i, j = c.j;
+        const float dx = s_x[i].x - s_x[j].x, dy = s_x[i].y - s_x[j].y;
+        const float dist = sqrtf(dx * dx + dy * dy);
+        float nx = 1.0f, ny = 0.0f;
+        if (dist > 1e-9f) {
+          nx = dx / dist;
+          ny = dy / dist;
+        }
+        const float vnp = (s_vp[i].x - s_vp[j].x) * nx + (s_vp[i].y - s_vp[j].y) * ny;
+        const bool fresh = vnp < -0.02f;
+        if (fresh) {
+          s_coll++;
+          const int b = de_ebin(0.5f * (D.mass * 0.5f) * vnp * vnp, ebin);
+          if (b >= 0) s_hist[b]++;
+        }
+        const float e = fresh ? D.rest_bb : 0.0f;
+        if (-vnp > D.thr) {
+          const float tx = -ny, ty = nx;
+          const float slip = ((s_vp[i].x - s_vp[j].x) * tx + (s_vp[i].y - s_vp[j].y) * ty) - r * s_op[i] - r * s_op[j];
+          if (s_nrec < rcap) rec[rb + s_nrec] = mg_impact_record{0u, i, j, -vnp, fabsf(slip), e};
+          s_nrec++;
+        }
+        const float target = fmaxf(fmaxf(-e * vnp, 0.0f), fmaxf(vnp, 0.0f));
+        const float vnn = (s_v[i].x - s_v[j].x) * nx + (s_v[i].y - s_v[j].y) * ny;
+        const float dvn = target - vnn;
+        const float ws = w + w;
+        if (ws <= 0.0f) continue;
+        const float im = dvn / ws;
+        const float s = w * im;
+        s_v[i].x += s * nx;
+        s_v[i].y += s * ny;
+        s_v[j].x -= s * nx;
+        s_v[j].y -= s * ny;
+      }
+    }
+    __syncthreads();
+  }
+  // walls: each ball on its own; records in ball order after the pair records
+  for (uint32_t base = 0; base < n; base += DE_BLOCK) {
+    const uint32_t i = base + t;
+    uint32_t want = 0;
+    mg_impact_record rr{};
+    if (i < n) {
+      const float lamw = lam_bw[sb + i];
+      if (lamw > 0.0f) {
+        float nx, ny;
+        mgg::sdf_world(s_x[i].x, s_x[i].y, D.rc1, D.rs1, D.wc1, D.ws1, g, nx, ny);
+        const float cx = s_x[i].x - r * nx, cy = s_x[i].y - r * ny;
+        const float vwx = D.omega * -cy, vwy = D.omega * cx;
+        const float vnp = (s_vp[i].x - vwx) * nx + (s_vp[i].y - vwy) * ny;
+        const bool fresh = vnp < -0.02f;
+        if (fresh) {
+          atomicAdd(&s_coll, 1u);
+          const int b = de_ebin(0.5f * D.mass * vnp * vnp, ebin);
+          if (b >= 0) atomicAdd(&s_hist[b], 1u);
+        }
+        const float e = fresh ? D.rest_bw : 0.0f;
+        if (-vnp > D.thr) {
+          const float tx = -ny, ty = nx;
+          const float slip = ((s_vp[i].x - vwx) * tx + (s_vp[i].y - vwy) * ty) - r * s_op[i];
+          rr = mg_impact_record{1u, i, 0xFFFFFFFFu, -vnp, fabsf(slip), e};
+          want = 1;
+        }
+        const float target = fmaxf(fmaxf(-e * vnp, 0.0f), fmaxf(vnp, 0.0f));
+        const float vnn = (s_v[i].x - vwx) * nx + (s_v[i].y - vwy) * ny;
+        const float dvn = target - vnn;
+        if (w > 0.0f) {
+          s_v[i].x += dvn * nx;
+          s_v[i].y += dvn * ny;
+        }
+      }
+    }
+    uint32_t tot;
+    const uint32_t rank = de_block_scan(want, s_w, &tot);
+    const uint32_t at = s_nrec + rank;
+    if (want && at < rcap) rec[rb + at] = rr;
+    __syncthreads();
+    if (t == 0) s_nrec += tot;
+    __syncthreads();
+  }
+  for (uint32_t i = t; i < n; i += DE_BLOCK) v[D.b0 + i] = s_v[i];
+  if (t == 0) {
+    DeOut& o = out[blockIdx.x];
+    o.coll = s_coll;
+    o.n_imp = s_nrec;
+    for (int b = 0; b < 12; b++) o.hist[b] = s_hist[b];
+  }
+}
+
+// ---- 8. rolling resistance (dem.rs:1036-1055) and the final energy (:1057-1065) -------------------
+extern "C" __global__ void __launch_bounds__(DE_BLOCK)
+dem_finish(const DeDrum* __restrict__ drums, const float2* __restrict__ x, const float2* __restrict__ v,
+           float* __restrict__ om, const DeItem* __restrict__ sorted, const float* __restrict__ lam_bw,
+           uint32_t stride, uint32_t pcap, DeOut* __restrict__ out) {
+  __shared__ float s_tl[DE_MAX_BALLS];
+  __shared__ DeItem s_it[DE_CHUNK];
+  __shared__ __align__(16) float s_e[3 * DE_BLOCK];
+  const DeDrum D = drums[blockIdx.x];
+  const uint32_t t = threadIdx.x, n = D.n;
+  const size_t sb = (size_t)blockIdx.x * stride, pb = (size_t)blockIdx.x * pcap;
+  const uint32_t np = min(out[blockIdx.x].n_pairs, pcap);
+  if (D.rolling > 0.0f && D.inertia > 0.0f) {
+    for (uint32_t i = t; i < n; i += DE_BLOCK) s_tl[i] = 0.0f;
+    __syncthreads();
+    for (uint32_t c0 = 0; c0 < np; c0 += DE_CHUNK) {  // total normal impulse per ball, contact order
+      const uint32_t m = min(np - c0, (uint32_t)DE_CHUNK);
+      for (uint32_t q = t; q < m; q += DE_BLOCK) s_it[q] = sorted[pb + c0 + q];
+      __syncthreads();
+      if (t == 0)
+        for (uint32_t q = 0; q < m; q++) {
+          const float l = fmaxf(s_it[q].lam, 0.0f);
+          s_tl[s_it[q].i] += l;
+          s_tl[s_it[q].j] += l;
+        }
+      __syncthreads();
+    }
+    for (uint32_t i = t; i < n; i += DE_BLOCK) {
+      const float tl = s_tl[i] + fmaxf(lam_bw[sb + i], 0.0f);
+      const float o = om[D.b0 + i];
+      if (tl <= 0.0f || o == 0.0f) continue;
+      const float nf = tl / D.dt;
+      const float mdo = D.rolling * nf * D.r / D.inertia * D.dt;
+      const float dom = fminf(mdo, fabsf(o));
+      const float sg = isnan(o) ? o : (signbit(o) ? -1.0f : 1.0f);
+      om[D.b0 + i] = o - sg * dom;
+    }
+    __syncthreads();
+  }
+  float e = 0.0f;
+  for (uint32_t base = 0; base < n; base += DE_BLOCK) {
+    const uint32_t i = base + t;
+    if (i < n) de_mech_terms(D, x[D.b0 + i], v[D.b0 + i], om[D.b0 + i], &s_e[3 * t]);
+    __syncthreads();
+    if (t == 0) {
+      const uint32_t m = min(n - base, (uint32_t)DE_BLOCK);
+      for (uint32_t q = 0; q < 3 * m; q++) e += s_e[q];
+    }
+    __syncthreads();
+  }
+  if (t == 0) out[blockIdx.x].e_final = e;
+}
+
+// ---- host -----------------------------------------------------------------------------------------
+struct mg_dem {
+  uint32_t nd = 0, max_balls = 0, pcap = 0, rcap = 0;
+  char* dev = nullptr;
+  DeDrum* drums = nullptr;
+  float2* lift = nullptr;
+  uint32_t lift_cap = 0;
+  DeOut* out = nullptr;
+  float2 *x0 = nullptr, *vpre = nullptr;
+  float *th0 = nullptr, *ompre = nullptr, *lam_bw = nullptr, *lam = nullptr, *ebin = nullptr;
+  uint32_t* flagged = nullptr;
+  uint2* pairs = nullptr;
+  DeItem* sorted = nullptr;
+  mg_impact_record* rec = nullptr;
+  std::vector<DeOut> h_out;
+  std::string msg;
+};
+
+namespace {
+
+int de_fail(mg_dem* h, char* err, size_t errlen, int rc, const std::string& m) {
+  if (h) h->msg = m;
+  if (err && errlen) {
+    std::strncpy(err, m.c_str(), errlen - 1);
+    err[errlen - 1] = 0;
+  }
+  return rc;
+}
+
+// impact_energy_bin (dem.rs:99-109) with the host libm's log10f.
+int ebin_host(float e) {
+  if (!std::isfinite(e) || e <= 0.0f) return -1;
+  const float ec = std::min(std::max(e, 1e-6f), 1.0f);
+  const float lmin = std::log10(1e-6f), lmax = std::log10(1.0f);
+  const float t = (std::log10(ec) - lmin) / (lmax - lmin);
+  const float u = t * 12.0f;
+  const uint64_t idx = !(u > 0.0f) ? 0u : (u >= 1.8446744073709552e19f ? UINT64_MAX : (uint64_t)u);
+  return (int)std::min<uint64_t>(idx, 11);
+}
+
+// thr[k - 1] = the smallest positive finite float whose bin is at least k (k = 1..11).
+void ebin_thresholds(float* thr) {
+  for (int k = 1; k < 12; k++) {
+    uint32_t lo = 1, hi = 0x7f7fffffu;  // bin(lo) < k <= bin(FLT_MAX) = 11
+    while (hi - lo > 1) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      float f;
+      std::memcpy(&f, &mid, 4);
+      if (ebin_host(f) >= k) hi = mid; else lo = mid;
+    }
+    std::memcpy(&thr[k - 1], &hi, 4);
+  }
+}
+
+// Drum::toi_swept (geometry.rs:146-194) on the host, rotating with the host libm's sinf / cosf.
+bool toi_swept_host(const mgg::Geo& g, float x0x, float x0y, float x1x, float x1y, float a0, float a1, float rb,
+                    float& out) {
+  const float c0 = std::cos(-a0), s0 = std::sin(-a0);
+  if (mgg::sdf_local(x0x * c0 - x0y * s0, x0x * s0 + x0y * c0, g) - rb <= 0.0f) return false;
+  const float px = x1x - x0x, py = x1y - x0y;
+  const float pl = std::sqrt(px * px + py * py);
+  const float da = a1 - a0;
+  const float pos_bound = std::sqrt(x0x * x0x + x0y * x0y) + pl;
+  const float lsb = pl + std::fabs(da) * pos_bound;
+  if (lsb < 1e-9f) return false;
+  float t = 0.0f;
+  for (int it = 0; it < 32; it++) {
+    const float at = a0 + da * t;
+    const float xtx = x0x + px * t, xty = x0y + py * t;
+    const float c = std::cos(-at), s = std::sin(-at);
+    const float d = mgg::sdf_local(xtx * c - xty * s, xtx * s + xty * c, g) - rb;
+    if (d <= 0.0f) {
+      out = t;
+      return true;
+    }
+    t += std::fmax(d / lsb, 1e-6f);
+    if (t >= 1.0f) return false;
+  }
+  out = std::fmin(t, 1.0f);
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mg_dem_create(uint32_t n_drums, uint32_t max_balls, uint32_t pairs_per_ball, uint32_t max_impacts, mg_dem** out,
+                  char* err, size_t errlen) {
+  if (!out || n_drums == 0) return de_fail(nullptr, err, errlen, MG_E_ARG, "n_drums must be positive");
+  if (max_balls == 0 || max_balls > DE_MAX_BALLS)
+    return de_fail(nullptr, err, errlen, MG_E_ARG, "max_balls must be in 1..2048 (the ball state of a population "
+                                                   "is held in LDS during the pair sweeps)");
+  if (pairs_per_ball == 0) pairs_per_ball = 24;
+  auto* h = new mg_dem();
+  h->nd = n_drums;
+  h->max_balls = max_balls;
+  h->pcap = std::max<uint32_t>(pairs_per_ball * max_balls, 64);
+  h->rcap = std::max<uint32_t>(max_impacts, 1);
+  h->lift_cap = 64 * n_drums;
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t nb = (size_t)n_drums * max_balls, np = (size_t)n_drums * h->pcap;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o = off;
+    off += up(bytes);
+    return o;
+  };
+  const size_t o_drums = take(n_drums * sizeof(DeDrum)), o_lift = take(h->lift_cap * sizeof(float2));
+  const size_t o_out = take(n_drums * sizeof(DeOut)), o_x0 = take(nb * 8), o_vp = take(nb * 8);
+  const size_t o_th0 = take(nb * 4), o_op = take(nb * 4), o_lbw = take(nb * 4), o_flag = take(nb * 4);
+  const size_t o_ebin = take(16 * 4), o_pairs = take(np * 8), o_lam = take(np * 4), o_sorted = take(np * sizeof(DeItem));
+  const size_t o_rec = take((size_t)n_drums * h->rcap * sizeof(mg_impact_record));
+  if (hipMalloc(&h->dev, off) != hipSuccess) {
+    delete h;
+    return de_fail(nullptr, err, errlen, MG_E_DEVICE, "hipMalloc failed for the DEM workspace");
+  }
+  h->drums = reinterpret_cast<DeDrum*>(h->dev + o_drums);
+  h->lift = reinterpret_cast<float2*>(h->dev + o_lift);
+  h->out = reinterpret_cast<DeOut*>(h->dev + o_out);
+  h->x0 = reinterpret_cast<float2*>(h->dev + o_x0);
+  h->vpre = reinterpret_cast<float2*>(h->dev + o_vp);
+  h->th0 = reinterpret_cast<float*>(h->dev + o_th0);
+  h->ompre = reinterpret_cast<float*>(h->dev + o_op);
+  h->lam_bw = reinterpret_cast<float*>(h->dev + o_lbw);
+  h->flagged = reinterpret_cast<uint32_t*>(h->dev + o_flag);
+  h->ebin = reinterpret_cast<float*>(h->dev + o_ebin);
+  h->pairs = reinterpret_cast<uint2*>(h->dev + o_pairs);
+  h->lam = reinterpret_cast<float*>(h->dev + o_lam);
+  h->sorted = reinterpret_cast<DeItem*>(h->dev + o_sorted);
+  h->rec = reinterpret_cast<mg_impact_record*>(h->dev + o_rec);
+  float thr[11];
+  ebin_thresholds(thr);
+  if (hipMemcpy(h->ebin, thr, sizeof thr, hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(h->dev);
+    delete h;
+    return de_fail(nullptr, err, errlen, MG_E_DEVICE, "HIP copy error");
+  }
+  h->h_out.resize(n_drums);
+  *out = h;
+  return MG_OK;
+}
+
+void mg_dem_destroy(mg_dem* h) {
+  if (!h) return;
+  if (h->dev) (void)hipFree(h->dev);
+  delete h;
+}
+
+int mg_dem_step(mg_dem* h, const mg_dem_drum* drums, float* d_x, float* d_v, float* d_theta, float* d_omega,
+                const float* d_ext_impulse, const float* d_ext_angular, mg_dem_stats* stats, char* err,
+                size_t errlen) {
+  if (!h || !drums || !stats) return de_fail(h, err, errlen, MG_E_ARG, "null argument");
+  const uint32_t nd = h->nd;
+  std::vector<DeDrum> hd(nd);
+  std::vector<float2> lift;
+  uint64_t tb = 0;
+  bool any_ext = false;
+  for (uint32_t d = 0; d < nd; d++) {
+    const mg_dem_drum& m = drums[d];
+    if (m.n_balls > h->max_balls)
+      return de_fail(h, err, errlen, MG_E_ARG, "drum " + std::to_string(d) + ": more balls than the handle's max_balls");
+    DeDrum& D = hd[d];
+    D.radius = m.radius_m;
+    D.omega = m.omega;
+    D.nl = m.lifter_count;
+    D.lift0 = (uint32_t)lift.size();
+    D.rt = m.radius_m - m.lifter_height_m;
+    D.bh = m.lifter_base_width_m * 0.5f;
+    D.thw = m.lifter_top_width_m * 0.5f;
+    const float phase = m.lifter_phase_deg * (float)0.017453292519943295;  // f32::to_radians
+    const float tau = 6.28318530717958647692f;
+    for (uint32_t k = 0; k < m.lifter_count; k++) {
+      const float a = phase + (float)k * tau / (float)m.lifter_count;
+      lift.push_back(make_float2(std::sin(a), std::cos(a)));
+    }
+    const float an = m.drum_angle + m.omega * m.dt;  // dem.rs:610
+    D.rc0 = std::cos(-m.drum_angle);
+    D.rs0 = std::sin(-m.drum_angle);
+    D.rc1 = std::cos(-an);
+    D.rs1 = std::sin(-an);
+    D.wc1 = std::cos(an);
+    D.ws1 = std::sin(an);
+    D.r = m.ball_radius;
+    D.mass = m.ball_mass;
+    D.inertia = m.ball_inertia;
+    D.w = m.ball_mass > 0.0f ? 1.0f / m.ball_mass : 0.0f;        // Balls::inv_mass (dem.rs:335-341)
+    D.w_rot = m.ball_inertia > 0.0f ? 1.0f / m.ball_inertia : 0.0f;  // inv_inertia (:343-349)
+    D.thr = 0.1428f * std::sqrt(9.81f * 2.0f * m.ball_radius);    // restitution_velocity_threshold (:44-46)
+    D.rest_bb = m.restitution_ball_ball;
+    D.rest_bw = m.restitution_ball_wall;
+    D.fric_bb = m.friction_ball_ball;
+    D.fric_bw = m.friction_ball_wall;
+    D.fric_bb_s = m.friction_ball_ball_static;
+    D.fric_bw_s = m.friction_ball_wall_static;
+    D.fric_vs = m.friction_velocity_scale_m_s;
+    D.rolling = m.rolling_friction;
+    D.iters = m.iterations;
+    D.dt = m.dt;
+    D.b0 = (uint32_t)tb;
+    // an empty population or a non-positive step leaves the state as it is (dem.rs:542-544)
+    D.n = (m.n_balls == 0 || !(m.dt > 0.0f)) ? 0u : m.n_balls;
+    D.has_ext = m.has_external ? 1u : 0u;
+    any_ext |= D.has_ext != 0;
+    tb += m.n_balls;
+  }
+  if (tb && (!d_x || !d_v || !d_theta || !d_omega)) return de_fail(h, err, errlen, MG_E_ARG, "null state array");
+  if (any_ext && (!d_ext_impulse || !d_ext_angular)) return de_fail(h, err, errlen, MG_E_ARG, "null coupling impulses");
+  if (lift.size() > h->lift_cap) return de_fail(h, err, errlen, MG_E_ARG, "more than 64 lifters per drum on average");
+  if (lift.empty()) lift.push_back(make_float2(0.0f, 1.0f));
+  bool ok = hipMemcpy(h->drums, hd.data(), nd * sizeof(DeDrum), hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(h->lift, lift.data(), lift.size() * sizeof(float2), hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemset(h->out, 0, nd * sizeof(DeOut)) == hipSuccess;
+  if (!ok) return de_fail(h, err, errlen, MG_E_DEVICE, "HIP copy error");
+  auto* x = reinterpret_cast<float2*>(d_x);
+  auto* v = reinterpret_cast<float2*>(d_v);
+  const uint32_t S = h->max_balls, P = h->pcap;
+  const dim3 G(nd), B(DE_BLOCK);
+  hipLaunchKernelGGL(dem_predict, G, B, 0, nullptr, h->drums, x, v, d_theta, d_omega,
+                     reinterpret_cast<const float2*>(d_ext_impulse), d_ext_angular, h->x0, h->vpre, h->th0, h->ompre, S,
+                     h->out);
+  hipLaunchKernelGGL(dem_grid, G, B, 0, nullptr, h->drums, x, h->pairs, h->lam, P, h->out);
+  hipLaunchKernelGGL(dem_ccd_flag, G, B, 0, nullptr, h->drums, h->lift, x, h->flagged, S, h->out);
+  ok = hipGetLastError() == hipSuccess &&
+       hipMemcpy(h->h_out.data(), h->out, nd * sizeof(DeOut), hipMemcpyDeviceToHost) == hipSuccess;
+  if (!ok) return de_fail(h, err, errlen, MG_E_DEVICE, "HIP error in the DEM predict / grid kernels");
+  for (uint32_t d = 0; d < nd; d++)
+    if (h->h_out[d].pair_overflow)
+      return de_fail(h, err, errlen, MG_E_CAPACITY,
+                     "drum " + std::to_string(d) + ": " + std::to_string(h->h_out[d].n_pairs) +
+                         " candidate pairs exceed the handle's capacity (create with a larger pairs_per_ball)");
+  // swept time of impact of the balls ending deep in the wall, on the host (dem.rs:611-637)
+  std::vector<uint2> who;
+  std::vector<float2> pos;
+  std::vector<uint32_t> swept(nd, 0);
+  for (uint32_t d = 0; d < nd; d++) {
+    const uint32_t k = h->h_out[d].n_ccd;
+    if (!k) continue;
+    std::vector<uint32_t> idx(k);
+    ok = hipMemcpy(idx.data(), h->flagged + (size_t)d * S, k * 4, hipMemcpyDeviceToHost) == hipSuccess;
+    if (!ok) return de_fail(h, err, errlen, MG_E_DEVICE, "HIP copy error");
+    std::sort(idx.begin(), idx.end());
+    const DeDrum& D = hd[d];
+    mgg::Geo g{D.radius, D.omega, D.nl, D.rt, D.bh, D.thw, lift.data() + D.lift0};
+    const float an = drums[d].drum_angle + drums[d].omega * drums[d].dt;
+    for (uint32_t i : idx) {
+      float2 p0, p1;
+      ok = hipMemcpy(&p0, h->x0 + (size_t)d * S + i, 8, hipMemcpyDeviceToHost) == hipSuccess &&
+           hipMemcpy(&p1, x + D.b0 + i, 8, hipMemcpyDeviceToHost) == hipSuccess;
+      if (!ok) return de_fail(h, err, errlen, MG_E_DEVICE, "HIP copy error");
+      float tt;
+      if (!toi_swept_host(g, p0.x, p0.y, p1.x, p1.y, drums[d].drum_angle, an, D.r, tt)) continue;
+      const float toi = std::fmin(1.0f, tt);
+      if (!(toi < 1.0f)) continue;
+      const float px = p1.x - p0.x, py = p1.y - p0.y;
+      const float pl = std::sqrt(px * px + py * py);
+      const float skin = 1e-2f * D.r;
+      const float ts = pl > 1e-9f ? skin / pl : 1.0f;
+      const float tc = std::fmin(toi + ts, 1.0f);
+      who.push_back(make_uint2(d, i));
+      pos.push_back(make_float2(p0.x + px * tc, p0.y + py * tc));
+      swept[d]++;
+    }
+  }
+  if (!who.empty()) {
+    void* dw = nullptr;
+    ok = hipMalloc(&dw, who.size() * 16) == hipSuccess &&
+         hipMemcpy(dw, who.data(), who.size() * 8, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(static_cast<char*>(dw) + who.size() * 8, pos.data(), pos.size() * 8, hipMemcpyHostToDevice) ==
+             hipSuccess;
+    if (ok)
+      hipLaunchKernelGGL(dem_ccd_apply, dim3((uint32_t)((who.size() + 255) / 256)), dim3(256), 0, nullptr, h->drums,
+                         static_cast<const uint2*>(dw),
+                         reinterpret_cast<const float2*>(static_cast<char*>(dw) + who.size() * 8), (uint32_t)who.size(),
+                         x);
+    ok = ok && hipDeviceSynchronize() == hipSuccess;
+    if (dw) (void)hipFree(dw);
+    if (!ok) return de_fail(h, err, errlen, MG_E_DEVICE, "HIP error applying the swept clamp");
+  }
+  hipLaunchKernelGGL(dem_project, G, B, 0, nullptr, h->drums, h->lift, x, v, d_theta, d_omega, h->x0, h->th0, h->pairs,
+                     h->lam, h->lam_bw, S, P, h->out);
+  hipLaunchKernelGGL(dem_sort, G, B, 0, nullptr, h->drums, h->pairs, h->lam, h->sorted, P, h->out);
+  hipLaunchKernelGGL(dem_friction, G, B, 0, nullptr, h->drums, h->lift, x, v, d_theta, d_omega, h->x0, h->th0,
+                     h->sorted, h->lam_bw, S, P, h->out);
+  hipLaunchKernelGGL(dem_restitution, G, B, 0, nullptr, h->drums, h->lift, x, v, h->vpre, h->ompre, h->sorted,
+                     h->lam_bw, h->ebin, h->rec, h->rcap, S, P, h->out);
+  hipLaunchKernelGGL(dem_finish, G, B, 0, nullptr, h->drums, x, v, d_omega, h->sorted, h->lam_bw, S, P, h->out);
+  ok = hipGetLastError() == hipSuccess &&
+       hipMemcpy(h->h_out.data(), h->out, nd * sizeof(DeOut), hipMemcpyDeviceToHost) == hipSuccess;
+  if (!ok) return de_fail(h, err, errlen, MG_E_DEVICE, "HIP error in the DEM kernels");
+  int rc = MG_OK;
+  for (uint32_t d = 0; d < nd; d++) {
+    const DeOut& o = h->h_out[d];
+    mg_dem_stats& s = stats[d];
+    std::memset(&s, 0, sizeof s);
+    if (hd[d].n == 0) continue;  // DemStepStats::default()
+    s.wall_work_j = o.ww;
+    s.collision_count = o.coll;
+    for (int b = 0; b < 12; b++) s.impact_energy_histogram[b] = o.hist[b];
+    s.dissipated_energy_j = o.e_pred + o.ww - o.e_final;
+    s.max_substep_displacement_over_diameter = o.maxd;
+    s.coupling_work_j = (float)o.cw;
+    s.n_pairs = o.n_pairs;
+    s.n_impacts = o.n_imp;
+    s.n_swept = swept[d];
+    if (o.n_imp > h->rcap && rc == MG_OK)
+      rc = de_fail(h, err, errlen, MG_E_CAPACITY,
+                   "drum " + std::to_string(d) + ": more impact records than the handle holds (the count is exact, "
+                   "the records beyond max_impacts are dropped)");
+  }
+  return rc;
+}
+
+int mg_dem_impacts(mg_dem* h, uint32_t drum, mg_impact_record* out, uint32_t cap, uint32_t* n) {
+  if (!h || drum >= h->nd || !n) return MG_E_ARG;
+  const uint32_t have = std::min(h->h_out[drum].n_imp, h->rcap);
+  *n = h->h_out[drum].n_imp;
+  if (!out || cap == 0 || have == 0) return MG_OK;
+  const uint32_t k = std::min(have, cap);
+  return hipMemcpy(out, h->rec + (size_t)drum * h->rcap, k * sizeof(mg_impact_record), hipMemcpyDeviceToHost) ==
+                 hipSuccess
+             ? MG_OK
+             : MG_E_DEVICE;
+}
+
+}  // extern "C"
