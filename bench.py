@@ -26,7 +26,8 @@ context only while drums do not step on the GPU). kernel_isolation.grid: the ref
 uniform grid build (grid.rs:101-153) over the box's 1,048,576 initial points;
 kernel_isolation.mixing_index: the Lacey mixing index (metrics.rs:506-576) of 1024 synthetic
 C2-sized drum states in one launch; kernel_isolation.drum_metrics: metrics::compute of 256 C2 drum
-states in one call; kernel_isolation.stress: StressAccum::update of 256 drums per call; each with
+states in one call; kernel_isolation.stress: StressAccum::update of 256 drums per call;
+kernel_isolation.dem: DemState::step_recorded of 256 dry 2000-ball populations per call; each with
 the restatement's rate beside it where one exists.
 --part grid / --part mixing time only that operator (for a profiler run).
 """
@@ -403,6 +404,61 @@ def bench_stress(args, drums=256, records=40) -> dict:
             "timing": f"host wall clock around {reps} synchronous updates (host step records included)"}
 
 
+def bench_dem(args, pops=256, cpu=True) -> dict:
+    """DemState::step_recorded (dem.rs:530-1066) for `pops` ball populations of the 2000-disc drum
+    (dry: 2000 balls at 75 % critical speed) per call, K steps after one warm-up step. Timed: the
+    synchronous mg_dem_step calls with the state resident in device memory (the host evaluates
+    the rare swept clamp in between). cpu_baseline: the oracle's DEM step of one population."""
+    import ctypes as C
+    import numpy as np
+    from milldynamics3_amd import dem
+    from milldynamics3_amd.grid import DeviceArray
+    from oracle import oracle as orc
+    params = dict(C2_PARAMS, slurry={"enabled": False})
+    o = orc.OracleSim(params, "ref")
+    st, pr = o.state(), o.dem_params()
+    n = len(st["ball_x"])
+    iters = orc._merge(orc._DEFAULTS, params)["simulation"]["dem_iterations"]
+    L = dem._lib()
+    b = dem.DemBatch(pops, n)
+    recs = (dem._Drum * pops)()
+    for k in range(pops):
+        r = recs[k]
+        r.radius_m, r.omega, r.drum_angle = pr["radius_m"], pr["omega"], 0.001 * k
+        r.ball_radius, r.ball_mass, r.ball_inertia = pr["ball_radius"], pr["ball_mass"], pr["ball_inertia"]
+        for key in dem.MEDIA_KEYS:
+            setattr(r, key, pr[key])
+        r.iterations, r.dt, r.n_balls, r.has_external = iters, o.fixed_sub_dt, n, 0
+    tile = lambda a, c: DeviceArray.from_numpy(np.tile(np.asarray(a, np.float32).reshape(-1, c), (pops, 1)))
+    dx, dv, dth, dom = tile(st["ball_x"], 2), tile(st["ball_v"], 2), tile(st["ball_theta"], 1), tile(st["ball_omega"], 1)
+    stats = (dem._Stats * pops)()
+    err = C.create_string_buffer(512)
+
+    def call():
+        rc = L.mg_dem_step(b._h, recs, dx.ptr, dv.ptr, dth.ptr, dom.ptr, None, None, stats, err, 512)
+        if rc != 0:
+            raise RuntimeError(err.value.decode())
+    call()
+    reps = max(args.steps // 2, 1)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        call()
+    ms = 1000.0 * (time.perf_counter() - t0) / reps
+    b.close()
+    res = {"populations": pops, "balls_per_population": n, "iterations": iters, "step_ms": round(ms, 3),
+           "population_steps_per_s": round(pops / (ms * 1e-3), 1), "pairs_per_population": int(stats[0].n_pairs),
+           "timing": f"host wall clock around {reps} synchronous batch steps"}
+    if cpu:
+        k, t0 = 0, time.perf_counter()
+        while k == 0 or time.perf_counter() - t0 < min(args.cpu_seconds, 2.0):
+            o.dem_step(0.0, iters, o.fixed_sub_dt)
+            k += 1
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": round(k / dt, 1), "unit": "population-steps/s", "cores": 1, "kind": "port",
+                               "sample": f"{k} DEM steps of one 2000-ball population on the C++ restatement, 1 core"}
+    return res
+
+
 def precision_vs_fp64(cfg, steps) -> dict:
     """Error of the FP32-state box's outputs against the FP64-state box after the same `steps` steps
     from the same initial state (the FP64 box is the oracle's f64 policy on the GPU): positions in
@@ -446,7 +502,7 @@ def main() -> int:
     ap.add_argument("--no-bingham", action="store_true", help="skip the Bingham-fluid box measurement")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one plain device copy of this size (calibrates PMC byte counters)")
-    ap.add_argument("--part", choices=("all", "grid", "mixing", "drum_metrics", "stress"), default="all",
+    ap.add_argument("--part", choices=("all", "grid", "mixing", "drum_metrics", "stress", "dem"), default="all",
                     help="grid / mixing: time only that operator (for a profiler run) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -475,7 +531,8 @@ def main() -> int:
         part = {"grid": lambda: bench_grid(args, cpu=not args.no_cpu),
                 "mixing": lambda: bench_mixing(args, cpu=not args.no_cpu),
                 "drum_metrics": lambda: bench_drum_metrics(args, cpu=not args.no_cpu),
-                "stress": lambda: bench_stress(args)}[args.part]()
+                "stress": lambda: bench_stress(args),
+                "dem": lambda: bench_dem(args, cpu=not args.no_cpu)}[args.part]()
         print(json.dumps({"part": args.part, **part}), flush=True)
         return 0
     cfg = {"side": args.side, "mu": args.mu}
@@ -560,6 +617,7 @@ def main() -> int:
     iso["mixing_index"] = bench_mixing(args, cpu=not args.no_cpu)
     iso["drum_metrics"] = bench_drum_metrics(args, cpu=not args.no_cpu)
     iso["stress"] = bench_stress(args)
+    iso["dem"] = bench_dem(args, cpu=not args.no_cpu)
     box_cpu, drum_cpu = None, None
     if not args.no_cpu:
         box_cpu = cpu_baseline_box(cfg, args.cpu_seconds)
