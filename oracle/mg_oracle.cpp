@@ -481,6 +481,160 @@ static bool mixing_index_of(const V2<R>* x, const D* dye, size_t n, R rad, float
 }
 
 // ------------------------------------------------------------------------------------------
+// surface.rs:39-420 — free-surface extraction (f32): splat field, wall mask, marching squares,
+// chains, one Chaikin pass, projection inside the wall. glam's Vec2::lerp is restated as
+// a + (b - a) * t (glam 0.29 is not vendored here: the last bit of contour points is unpinned).
+// ------------------------------------------------------------------------------------------
+namespace surf {
+constexpr int G = 128;
+inline float splat_kernel(float r2, float radius) {  // surface.rs:39-46
+  const float radius2 = radius * radius;
+  if (r2 >= radius2) return 0.0f;
+  const float t = 1.0f - r2 / radius2;
+  return t * t * t;
+}
+inline size_t grid_index(float world, float R, float cell) {  // surface.rs:140-143
+  const float raw = rclamp(std::round((world + R) / cell), 0.0f, (float)(G - 1));
+  return std::isnan(raw) ? 0 : (size_t)raw;
+}
+inline V2<float> grid_point(size_t gi, size_t gj, float R, float cell) {  // surface.rs:145-150
+  return V2<float>(-R + (float)gi * cell, -R + (float)gj * cell);
+}
+inline V2<float> lerp(V2<float> a, V2<float> b, float t) { return a + (b - a) * t; }
+// surface.rs:84-133
+static std::vector<float> build_field(const std::vector<V2<float>>& x, const Drum<float>& drum, float angle, float rs) {
+  const float R = drum.radius;
+  std::vector<float> field((size_t)G * G, 0.0f);
+  if (x.empty() || R <= 0.0f) return field;
+  const float cell = 2.0f * R / ((float)G - 1.0f);
+  for (const auto& p : x) {
+    const size_t i0 = grid_index(p.x - rs, R, cell), i1 = grid_index(p.x + rs, R, cell);
+    const size_t j0 = grid_index(p.y - rs, R, cell), j1 = grid_index(p.y + rs, R, cell);
+    for (size_t gj = j0; gj <= j1; gj++)
+      for (size_t gi = i0; gi <= i1; gi++) field[gj * G + gi] += splat_kernel((grid_point(gi, gj, R, cell) - p).len2(), rs);
+  }
+  const V2<float> to_local = from_angle(-angle);
+  for (size_t gj = 0; gj < (size_t)G; gj++)
+    for (size_t gi = 0; gi < (size_t)G; gi++) {
+      float& f = field[gj * G + gi];
+      if (f == 0.0f) continue;
+      if (drum.sdf(rotate(grid_point(gi, gj, R, cell), to_local)) < 0.0f) f = 0.0f;
+    }
+  return field;
+}
+struct Edge {
+  int vert, i, j;  // 0: horizontal (i, j)-(i+1, j); 1: vertical (i, j)-(i, j+1)
+  bool operator==(const Edge& o) const { return vert == o.vert && i == o.i && j == o.j; }
+  bool operator<(const Edge& o) const { return vert != o.vert ? vert < o.vert : (i != o.i ? i < o.i : j < o.j); }
+};
+inline Edge cell_edge(int ci, int cj, int side) {  // surface.rs:209-217
+  switch (side) {
+    case 0: return {0, ci, cj};
+    case 1: return {1, ci + 1, cj};
+    case 2: return {0, ci, cj + 1};
+    default: return {1, ci, cj};
+  }
+}
+inline int edge_pairs(int c, int (*out)[2]) {  // surface.rs:194-207
+  switch (c) {
+    case 1: case 14: out[0][0] = 3; out[0][1] = 0; return 1;
+    case 2: case 13: out[0][0] = 0; out[0][1] = 1; return 1;
+    case 3: case 12: out[0][0] = 3; out[0][1] = 1; return 1;
+    case 4: case 11: out[0][0] = 1; out[0][1] = 2; return 1;
+    case 6: case 9: out[0][0] = 0; out[0][1] = 2; return 1;
+    case 7: case 8: out[0][0] = 3; out[0][1] = 2; return 1;
+    case 5: out[0][0] = 3; out[0][1] = 0; out[1][0] = 1; out[1][1] = 2; return 2;
+    case 10: out[0][0] = 0; out[0][1] = 1; out[1][0] = 3; out[1][1] = 2; return 2;
+    default: return 0;
+  }
+}
+// surface.rs:279-357
+static std::vector<std::vector<V2<float>>> polygons_at(const std::vector<float>& field, float thr, float R, float cell,
+                                                      const Drum<float>& drum, float angle) {
+  std::vector<std::pair<Edge, Edge>> seg;
+  for (int cj = 0; cj < G - 1; cj++)
+    for (int ci = 0; ci < G - 1; ci++) {
+      auto corner = [&](int di, int dj) { return field[(size_t)(cj + dj) * G + (ci + di)] > thr; };
+      const int c = (int)corner(0, 0) | (int)corner(1, 0) << 1 | (int)corner(1, 1) << 2 | (int)corner(0, 1) << 3;
+      int pr[2][2];
+      const int k = edge_pairs(c, pr);
+      for (int q = 0; q < k; q++) seg.push_back({cell_edge(ci, cj, pr[q][0]), cell_edge(ci, cj, pr[q][1])});
+    }
+  std::vector<std::vector<V2<float>>> polys;
+  if (seg.empty()) return polys;
+  std::map<Edge, std::vector<size_t>> by_edge;
+  for (size_t k = 0; k < seg.size(); k++) {
+    by_edge[seg[k].first].push_back(k);
+    by_edge[seg[k].second].push_back(k);
+  }
+  std::vector<bool> visited(seg.size(), false);
+  auto extend = [&](std::vector<Edge>& chain, bool fwd) {  // surface.rs:371-401
+    for (;;) {
+      const Edge tip = fwd ? chain.back() : chain.front();
+      auto it = by_edge.find(tip);
+      if (it == by_edge.end()) return;
+      size_t next = SIZE_MAX;
+      for (size_t k : it->second)
+        if (!visited[k]) { next = k; break; }
+      if (next == SIZE_MAX) return;
+      visited[next] = true;
+      const Edge other = seg[next].first == tip ? seg[next].second : seg[next].first;
+      if (fwd) chain.push_back(other); else chain.insert(chain.begin(), other);
+    }
+  };
+  for (size_t st = 0; st < seg.size(); st++) {
+    if (visited[st]) continue;
+    visited[st] = true;
+    std::vector<Edge> chain = {seg[st].first, seg[st].second};
+    extend(chain, true);
+    extend(chain, false);
+    if (chain.size() > 1 && chain.front() == chain.back()) chain.pop_back();
+    std::vector<V2<float>> pts;
+    for (const Edge& e : chain) {  // interp_edge (surface.rs:166-192)
+      const size_t i0 = e.i, j0 = e.j, i1 = e.vert ? e.i : e.i + 1, j1 = e.vert ? e.j + 1 : e.j;
+      const float v0 = field[j0 * G + i0], v1 = field[j1 * G + i1];
+      const float t = std::fabs(v1 - v0) > 1e-9f ? rclamp((thr - v0) / (v1 - v0), 0.0f, 1.0f) : 0.5f;
+      pts.push_back(lerp(grid_point(i0, j0, R, cell), grid_point(i1, j1, R, cell), t));
+    }
+    std::vector<V2<float>> sm;  // chaikin_smooth_closed (surface.rs:403-418)
+    if (pts.size() < 3) {
+      sm = pts;
+    } else {
+      for (size_t i = 0; i < pts.size(); i++) {
+        const V2<float> a = pts[i], b = pts[(i + 1) % pts.size()];
+        sm.push_back(lerp(a, b, 0.25f));
+        sm.push_back(lerp(a, b, 0.75f));
+      }
+    }
+    for (auto& p : sm) {  // project_inside_wall (surface.rs:359-366)
+      const auto dn = drum.sdf_world(p, angle);
+      if (dn.first < 0.0f) p = p - dn.second * dn.first;
+    }
+    polys.push_back(sm);
+  }
+  return polys;
+}
+// surface.rs:233-277
+static std::vector<std::vector<V2<float>>> extract(const std::vector<V2<float>>& x, const Drum<float>& drum, float angle,
+                                                  float h, float rest, float pmass, std::vector<float>* field_out) {
+  const float R = drum.radius;
+  if (x.empty() || R <= 0.0f) return {};
+  const float rs = 1.5f * h;
+  const std::vector<float> field = build_field(x, drum, angle, rs);
+  if (field_out) *field_out = field;
+  const float cell = 2.0f * R / ((float)G - 1.0f);
+  const float nd = rest / rmax(pmass, 1e-12f);
+  const float phi = nd * (C<float>::PI * rs * rs / 4.0f);
+  auto polys = polygons_at(field, 0.5f * phi, R, cell, drum, angle);
+  if (!polys.empty()) return polys;
+  float peak = 0.0f;
+  for (float f : field) peak = rmax(peak, f);
+  if (peak <= 0.0f) return {};
+  return polygons_at(field, 0.5f * peak, R, cell, drum, angle);
+}
+}  // namespace surf
+
+// ------------------------------------------------------------------------------------------
 // metrics.rs:195-710 — metrics::compute over a drum state (f32 throughout, as the reference).
 // Angles are p.y.atan2(p.x) from the host libm, as the reference's f32::atan2.
 // ------------------------------------------------------------------------------------------
@@ -2420,6 +2574,31 @@ int orc_drum_metrics(const double* drum7, float angle, const float* bx, const fl
   std::memcpy(out, o, sizeof o);
   return m.non_finite_angle ? 0 : 1;
 }
+// extract_surface (surface.rs:233-277) of caller points: the flatten_polygons layout
+// (surface.rs:420-431) into out (returns its length, or the length needed when cap is short);
+// field (128 * 128 floats) receives the masked splat field when non-null.
+int64_t orc_surface(const double* drum7, float angle, const float* fx, size_t n, float h, float rest, float pmass,
+                    float* out, int64_t cap, float* field) {
+  Drum<float> d;
+  d.radius = (float)drum7[0]; d.omega = (float)drum7[1]; d.lifter_count = (uint32_t)drum7[2];
+  d.l_height = (float)drum7[3]; d.l_base = (float)drum7[4]; d.l_top = (float)drum7[5]; d.l_phase_deg = (float)drum7[6];
+  std::vector<V2<float>> x(n);
+  for (size_t i = 0; i < n; i++) x[i] = V2<float>(fx[2 * i], fx[2 * i + 1]);
+  std::vector<float> f;
+  const auto polys = surf::extract(x, d, angle, h, rest, pmass, field ? &f : nullptr);
+  if (field) {
+    if (f.empty()) f.assign((size_t)surf::G * surf::G, 0.0f);
+    std::memcpy(field, f.data(), f.size() * sizeof(float));
+  }
+  std::vector<float> flat = {(float)polys.size()};
+  for (const auto& p : polys) {
+    flat.push_back((float)p.size());
+    for (const auto& q : p) { flat.push_back(q.x); flat.push_back(q.y); }
+  }
+  if ((int64_t)flat.size() <= cap) std::memcpy(out, flat.data(), flat.size() * sizeof(float));
+  return (int64_t)flat.size();
+}
+float orc_splat_kernel(float r2, float radius) { return surf::splat_kernel(r2, radius); }
 // Candidate pairs in emission order (grid.rs:291-362).
 int64_t orc_grid_pairs(const float* pts, size_t n, float cell, uint32_t* out, int64_t cap) {
   std::vector<V2<float>> p(n);

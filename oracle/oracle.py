@@ -65,6 +65,10 @@ def oracle_lib():
     lib.orc_mixing_index_of.restype = C.c_int
     lib.orc_mixing_index_of.argtypes = [fp, fp, C.c_size_t, C.c_float, fp]
     lib.orc_dem_step.argtypes = [P, C.c_double, C.c_uint32, C.c_double, fp, fp, dp]
+    lib.orc_surface.restype = C.c_int64
+    lib.orc_surface.argtypes = [dp, C.c_float, fp, C.c_size_t, C.c_float, C.c_float, C.c_float, fp, C.c_int64, fp]
+    lib.orc_splat_kernel.restype = C.c_float
+    lib.orc_splat_kernel.argtypes = [C.c_float, C.c_float]
     lib.orc_dem_params.argtypes = [P, dp]
     lib.orc_stress_new.restype = P
     lib.orc_stress_new.argtypes = []
@@ -512,6 +516,39 @@ class StressAccum:
 def se_bin(e):
     """stress.rs:58-64 as the oracle evaluates it."""
     return int(oracle_lib().orc_se_bin(float(e)))
+
+
+def unflatten_polygons(flat):
+    """The flatten_polygons layout (surface.rs:420-431) back into a list of (k, 2) arrays."""
+    flat = np.asarray(flat, np.float32)
+    n, k, out = int(flat[0]), 1, []
+    for _ in range(n):
+        m = int(flat[k])
+        out.append(flat[k + 1: k + 1 + 2 * m].reshape(-1, 2).copy())
+        k += 1 + 2 * m
+    return out
+
+
+def extract_surface(drum, angle, fluid_x, h, rest_density, particle_mass, with_field=False):
+    """extract_surface (surface.rs:233-277): the contour polygons, and the masked splat field."""
+    lib = oracle_lib()
+    fx = _f32(fluid_x, 2)
+    field = np.zeros(128 * 128, np.float32) if with_field else None
+    cap = 1 << 16
+    while True:
+        out = np.zeros(cap, np.float32)
+        need = lib.orc_surface(_dp(np.asarray(drum, np.float64)), float(angle), _fp(fx), len(fx), float(h),
+                               float(rest_density), float(particle_mass), _fp(out), cap,
+                               _fp(field) if with_field else None)
+        if need <= cap:
+            break
+        cap = int(need)
+    polys = unflatten_polygons(out[:need])
+    return (polys, field.reshape(128, 128)) if with_field else polys
+
+
+def splat_kernel(r2, radius):
+    return float(oracle_lib().orc_splat_kernel(float(r2), float(radius)))
 
 
 def toi_swept(drum, x0, x1, a0, a1, radius):
