@@ -27,8 +27,9 @@ uniform grid build (grid.rs:101-153) over the box's 1,048,576 initial points;
 kernel_isolation.mixing_index: the Lacey mixing index (metrics.rs:506-576) of 1024 synthetic
 C2-sized drum states in one launch; kernel_isolation.drum_metrics: metrics::compute of 256 C2 drum
 states in one call; kernel_isolation.stress: StressAccum::update of 256 drums per call;
-kernel_isolation.dem: DemState::step_recorded of 256 dry 2000-ball populations per call; each with
-the restatement's rate beside it where one exists.
+kernel_isolation.dem: DemState::step_recorded of 256 dry 2000-ball populations per call;
+kernel_isolation.surface: extract_surface of 256 C2 drum states per call; each with the
+restatement's rate beside it where one exists.
 --part grid / --part mixing time only that operator (for a profiler run).
 """
 from __future__ import annotations
@@ -459,6 +460,46 @@ def bench_dem(args, pops=256, cpu=True) -> dict:
     return res
 
 
+def bench_surface(args, drums=256, cpu=True) -> dict:
+    """surface::extract_surface (surface.rs:233-277) for `drums` drum states per call: copies of the
+    2000-disc drum (12,697 fluid particles) stepped 12 substeps on the restatement, each turned by
+    its own drum angle. Timed: the whole synchronous mg_surface_extract (per-drum records, four
+    kernels, the segment read-back and the host chain walk) with the positions resident in device
+    memory. cpu_baseline: the restatement's extract_surface on one drum."""
+    import numpy as np
+    from milldynamics3_amd import surface
+    from oracle import oracle as orc
+    o = orc.OracleSim(C2_PARAMS, "ref")
+    o.step_fixed(12)
+    fx = o.state()["fluid_x"].astype(np.float32)
+    R = 0.1397 / 2
+    batch = surface.SurfaceBatch(drums, drums * len(fx))
+    recs = batch.upload([{"radius_m": R, "drum_angle": 0.01 * k, "h": o.h, "rest_density": 1000.0,
+                          "particle_mass": o.fluid_mass, "fluid_x": fx} for k in range(drums)])
+    batch.run(recs)
+    reps = max(args.steps // 4, 1)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        batch.run(recs)
+    ms = 1000.0 * (time.perf_counter() - t0) / reps
+    polys = batch.polygons(0)
+    batch.close()
+    res = {"drums": drums, "fluid_per_drum": len(fx), "polygons_drum0": len(polys),
+           "points_drum0": int(sum(len(q) for q in polys)), "batch_ms": round(ms, 3),
+           "drums_per_s": round(drums / (ms * 1e-3), 1),
+           "timing": f"host wall clock around {reps} synchronous batch extractions"}
+    if cpu:
+        drum = orc.drum_array(R)
+        k, t0 = 0, time.perf_counter()
+        while k == 0 or (time.perf_counter() - t0 < min(args.cpu_seconds, 2.0) and k < drums):
+            orc.extract_surface(drum, 0.01 * k, fx, o.h, 1000.0, o.fluid_mass)
+            k += 1
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": round(k / dt, 1), "unit": "drums/s", "cores": 1, "kind": "port",
+                               "sample": f"{k} drums of the same batch on the C++ restatement, 1 core"}
+    return res
+
+
 def precision_vs_fp64(cfg, steps) -> dict:
     """Error of the FP32-state box's outputs against the FP64-state box after the same `steps` steps
     from the same initial state (the FP64 box is the oracle's f64 policy on the GPU): positions in
@@ -502,7 +543,7 @@ def main() -> int:
     ap.add_argument("--no-bingham", action="store_true", help="skip the Bingham-fluid box measurement")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one plain device copy of this size (calibrates PMC byte counters)")
-    ap.add_argument("--part", choices=("all", "grid", "mixing", "drum_metrics", "stress", "dem"), default="all",
+    ap.add_argument("--part", choices=("all", "grid", "mixing", "drum_metrics", "stress", "dem", "surface"), default="all",
                     help="grid / mixing: time only that operator (for a profiler run) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -532,7 +573,8 @@ def main() -> int:
                 "mixing": lambda: bench_mixing(args, cpu=not args.no_cpu),
                 "drum_metrics": lambda: bench_drum_metrics(args, cpu=not args.no_cpu),
                 "stress": lambda: bench_stress(args),
-                "dem": lambda: bench_dem(args, cpu=not args.no_cpu)}[args.part]()
+                "dem": lambda: bench_dem(args, cpu=not args.no_cpu),
+                "surface": lambda: bench_surface(args, cpu=not args.no_cpu)}[args.part]()
         print(json.dumps({"part": args.part, **part}), flush=True)
         return 0
     cfg = {"side": args.side, "mu": args.mu}
@@ -618,6 +660,7 @@ def main() -> int:
     iso["drum_metrics"] = bench_drum_metrics(args, cpu=not args.no_cpu)
     iso["stress"] = bench_stress(args)
     iso["dem"] = bench_dem(args, cpu=not args.no_cpu)
+    iso["surface"] = bench_surface(args, cpu=not args.no_cpu)
     box_cpu, drum_cpu = None, None
     if not args.no_cpu:
         box_cpu = cpu_baseline_box(cfg, args.cpu_seconds)

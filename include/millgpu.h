@@ -426,6 +426,37 @@ int mg_dem_step(mg_dem* dem, const mg_dem_drum* drums, float* d_x, float* d_v, f
  * dem.rs:152-169); *n receives the count. */
 int mg_dem_impacts(mg_dem* dem, uint32_t drum, mg_impact_record* out, uint32_t cap, uint32_t* n);
 
+/* ---- fluid free surface: surface::extract_surface (surface.rs:233-277) ---------------------------
+ * One drum's inputs: the Drum (geometry.rs:18-22, LiftersParams), its angle, the FluidParticles
+ * scalars the extraction reads (h, rest_density, particle_mass, pbf.rs) and the particle count. */
+typedef struct mg_surface_drum {
+  float radius_m, drum_angle;
+  uint32_t lifter_count;
+  float lifter_height_m, lifter_base_width_m, lifter_top_width_m, lifter_phase_deg;
+  float h, rest_density, particle_mass;
+  uint32_t n_fluid;
+} mg_surface_drum;
+typedef struct mg_surface mg_surface;
+/* Workspace for n_drums drums (at most 65535) holding at most max_fluid fluid particles in total. */
+int mg_surface_create(uint32_t n_drums, uint32_t max_fluid, mg_surface** out, char* err, size_t errlen);
+void mg_surface_destroy(mg_surface* surface);
+/* extract_surface (surface.rs:233-277) for every drum at once; Simulation::fluid_surface
+ * (lib.rs:353-360) calls it with the simulation's drum and angle. Drum d's particles follow drum
+ * d-1's in d_fluid_x (device, 2 floats each). The device splats the 128 x 128 occupancy field,
+ * masks it with the wall SDF, classifies the marching-squares cells and interpolates the crossing
+ * points; the host chains the segments, smooths and projects them (the reference's segment order
+ * and HashMap walk). The field, segments and points equal the reference's f32 arithmetic bit for
+ * bit; rotations use the host libm's sinf / cosf. Waits for the device. */
+int mg_surface_extract(mg_surface* surface, const mg_surface_drum* drums, const float* d_fluid_x, char* err,
+                       size_t errlen);
+/* The last extraction's polygons of one drum flattened as flatten_polygons (surface.rs:420-430)
+ * does, [n_polys, len_0, x, y, ..., len_1, ...] (mill-wasm lib.rs:145-152 fluidSurface): copies them
+ * to out when they fit in cap floats; returns their length, or a negative MG_E_* code. */
+int64_t mg_surface_polygons(mg_surface* surface, uint32_t drum, float* out, int64_t cap);
+/* The last extraction's masked occupancy field of one drum (build_field, surface.rs:84-133):
+ * 128 x 128 floats, row j = grid y index; zeros for a drum without fluid. */
+int mg_surface_field(mg_surface* surface, uint32_t drum, float* out16384);
+
 /* StressAccum::metrics (stress.rs:186-238) of one drum. */
 int mg_stress_get_metrics(mg_stress* stress, uint32_t drum, uint32_t n_balls, double m3, double scale_j, double d,
                       double power_draw_w, mg_stress_metrics* out);

@@ -59,7 +59,7 @@ __host__ __device__ inline float sdf_local(float lx, float ly, const Geo& g) {
 // (rc, rs) = from_angle(-angle), (wc, ws) = from_angle(angle). A non-finite point reads (0, 0).
 __host__ __device__ inline float sdf_world(float px, float py, float rc, float rs, float wc, float ws, const Geo& g,
                                            float& nx, float& ny) {
-  if (!(isfinite(px) && isfinite(py))) {
+  if (!(__builtin_isfinite(px) && __builtin_isfinite(py))) {
     nx = 0.0f;
     ny = 0.0f;
     return 0.0f;
@@ -77,7 +77,7 @@ __host__ __device__ inline float sdf_world(float px, float py, float rc, float r
     nly = gy * rcp;
   } else {  // -p_local.normalize_or_zero()
     const float rcp = 1.0f / sqrtf(lx * lx + ly * ly);
-    if (isfinite(rcp) && rcp > 0.0f) {
+    if (__builtin_isfinite(rcp) && rcp > 0.0f) {
       nlx = -(lx * rcp);
       nly = -(ly * rcp);
     } else {
@@ -92,7 +92,7 @@ __host__ __device__ inline float sdf_world(float px, float py, float rc, float r
 
 // The distance alone.
 __host__ __device__ inline float sdf_world_d(float px, float py, float rc, float rs, const Geo& g) {
-  if (!(isfinite(px) && isfinite(py))) return 0.0f;
+  if (!(__builtin_isfinite(px) && __builtin_isfinite(py))) return 0.0f;
   return sdf_local(px * rc - py * rs, px * rs + py * rc, g);
 }
 
