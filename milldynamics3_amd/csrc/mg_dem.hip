@@ -68,6 +68,9 @@ struct DeOut {
   double cw;
   uint32_t n_pairs, pair_overflow, n_ccd, coll, n_imp;
   uint32_t hist[12];
+  uint32_t depth;   // levels of the projection sweep's schedule (DE_LCAP + 1: sequential)
+  uint32_t nact;    // active contacts (normal impulse > 0) of the contact book
+  uint32_t cdepth;  // levels of the contact sweeps' schedule (DE_LCAP + 1: sequential)
 };
 
 struct DeItem {  // a sorted contact: (i, j) and its accumulated normal impulse
@@ -376,16 +379,45 @@ dem_grid(const DeDrum* __restrict__ drums, const float2* __restrict__ x, uint2* 
         atomicAdd(&L.start[k + 1], 1u);
       }
     __syncthreads();
-    if (t == 0) {  // inclusive scan of the counts, then the stable scatter in ball order
-      for (uint32_t k = 1; k <= nc; k++) L.start[k] += L.start[k - 1];
-      for (uint32_t i = 0; i < n; i++)
-        if (L.fin[i]) {
-          const uint32_t k = (uint32_t)(((int64_t)L.key[i].x - mnx) * H + ((int64_t)L.key[i].y - mny));
-          // start[k] is advanced while filling and restored below from start[k + 1]'s prefix
-          L.idx[L.start[k]++] = i;
+    uint32_t carry = 0;  // inclusive scan of the counts: start[k] = balls in cells before k
+    for (uint32_t b = 1; b <= nc; b += DE_BLOCK) {
+      const uint32_t k = b + t;
+      const uint32_t c = k <= nc ? L.start[k] : 0u;
+      uint32_t tot;
+      const uint32_t ex = de_block_scan(c, L.w, &tot);
+      if (k <= nc) L.start[k] = carry + ex + c;
+      carry += tot;
+      __syncthreads();
+    }
+    // each ball takes a slot of its cell (any order; start[k] advances from the cell's start to its
+    // end and is shifted back after), then each cell's balls are put in ascending index order (a
+    // cell holds a few balls)
+    for (uint32_t i = t; i < n; i += DE_BLOCK)
+      if (L.fin[i]) {
+        const uint32_t k = (uint32_t)(((int64_t)L.key[i].x - mnx) * H + ((int64_t)L.key[i].y - mny));
+        L.idx[atomicAdd(&L.start[k], 1u)] = i;
+      }
+    __syncthreads();
+    for (int32_t b = (int32_t)(nc / DE_BLOCK) * DE_BLOCK; b >= 0; b -= DE_BLOCK) {  // highest block first
+      const uint32_t k = (uint32_t)b + t;
+      const uint32_t v = (k > 0 && k <= nc) ? L.start[k - 1] : 0u;
+      __syncthreads();
+      if (k > 0 && k <= nc) L.start[k] = v;
+      __syncthreads();
+    }
+    if (t == 0) L.start[0] = 0;
+    __syncthreads();
+    for (uint32_t k = t; k < nc; k += DE_BLOCK) {
+      const uint32_t s0 = L.start[k], e0 = L.start[k + 1];
+      for (uint32_t a = s0 + 1; a < e0; a++) {
+        const uint32_t v = L.idx[a];
+        uint32_t b = a;
+        while (b > s0 && L.idx[b - 1] > v) {
+          L.idx[b] = L.idx[b - 1];
+          b--;
         }
-      for (uint32_t k = nc; k > 0; k--) L.start[k] = L.start[k - 1];
-      L.start[0] = 0;
+        L.idx[b] = v;
+      }
     }
     __syncthreads();
   } else if (m > 0) {
@@ -417,36 +449,58 @@ dem_grid(const DeDrum* __restrict__ drums, const float2* __restrict__ x, uint2* 
     nc = L.w[0];
     __syncthreads();
   }
-  // candidate pairs in emission order (grid.rs:291-362), lane 0
-  if (t == 0) {
-    uint32_t np = 0;
-    uint2* P = pairs + (size_t)blockIdx.x * pcap;
-    float* Lm = lam + (size_t)blockIdx.x * pcap;
-    for (uint32_t k = 0; k < nc && m > 0; k++) {
-      const uint32_t s = L.start[k], e = L.start[k + 1];
-      if (s == e) continue;
-      int32_t cx, cy;
-      if (dense) {
-        cx = (int32_t)((int64_t)mnx + (int64_t)k / H);
-        cy = (int32_t)((int64_t)mny + (int64_t)k % H);
-      } else {
-        cx = L.ckey[k].x;
-        cy = L.ckey[k].y;
-      }
-      for (uint32_t a = s; a < e; a++)
-        for (uint32_t b = a + 1; b < e; b++) de_emit(P, Lm, pcap, np, L.idx[a], L.idx[b]);
-      for (int q = 0; q < 4; q++) {  // forward cells (1, 0), (1, 1), (0, 1), (-1, 1)
-        const int32_t ox = q < 2 ? 1 : (q == 2 ? 0 : -1), oy = q == 0 ? 0 : 1;
-        const int32_t tx = de_sadd(cx, ox), ty = de_sadd(cy, oy);
-        if (tx == cx && ty == cy) continue;
-        uint32_t os, oe;
-        if (!de_range(L, dense, mnx, mny, W, H, nc, tx, ty, os, oe)) continue;
-        for (uint32_t a = s; a < e; a++)
-          for (uint32_t b = os; b < oe; b++) de_emit(P, Lm, pcap, np, L.idx[a], L.idx[b]);
+  // candidate pairs in emission order (grid.rs:291-362): a cell's pairs (inside it, then with the
+  // forward cells (1, 0), (1, 1), (0, 1), (-1, 1)) follow the cells before it, so each cell counts
+  // its pairs, a scan gives its offset and it writes them in the reference's order
+  uint2* P = pairs + (size_t)blockIdx.x * pcap;
+  float* Lm = lam + (size_t)blockIdx.x * pcap;
+  uint32_t base = 0;
+  for (uint32_t b = 0; b < nc && m > 0; b += DE_BLOCK) {  // nc and m are uniform
+    const uint32_t k = b + t;
+    uint32_t cnt = 0, s0 = 0, e0 = 0, os[4] = {0, 0, 0, 0}, oe[4] = {0, 0, 0, 0};
+    if (k < nc) {
+      s0 = L.start[k];
+      e0 = L.start[k + 1];
+      if (s0 != e0) {
+        int32_t cx, cy;
+        if (dense) {
+          cx = (int32_t)((int64_t)mnx + (int64_t)k / H);
+          cy = (int32_t)((int64_t)mny + (int64_t)k % H);
+        } else {
+          cx = L.ckey[k].x;
+          cy = L.ckey[k].y;
+        }
+        const uint32_t c = e0 - s0;
+        cnt = c * (c - 1u) / 2u;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const int32_t ox = q < 2 ? 1 : (q == 2 ? 0 : -1), oy = q == 0 ? 0 : 1;
+          const int32_t tx = de_sadd(cx, ox), ty = de_sadd(cy, oy);
+          if (tx == cx && ty == cy) continue;
+          uint32_t a, e;
+          if (!de_range(L, dense, mnx, mny, W, H, nc, tx, ty, a, e)) continue;
+          os[q] = a;
+          oe[q] = e;
+          cnt += c * (e - a);
+        }
       }
     }
-    o.n_pairs = np;
-    o.pair_overflow = np > pcap ? 1u : 0u;
+    uint32_t tot;
+    uint32_t np = base + de_block_scan(cnt, L.w, &tot);
+    if (cnt) {
+      for (uint32_t a = s0; a < e0; a++)
+        for (uint32_t b2 = a + 1; b2 < e0; b2++) de_emit(P, Lm, pcap, np, L.idx[a], L.idx[b2]);
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        for (uint32_t a = s0; a < e0; a++)
+          for (uint32_t b2 = os[q]; b2 < oe[q]; b2++) de_emit(P, Lm, pcap, np, L.idx[a], L.idx[b2]);
+    }
+    base += tot;
+    __syncthreads();
+  }
+  if (t == 0) {
+    o.n_pairs = base;
+    o.pair_overflow = base > pcap ? 1u : 0u;
   }
 }
 
@@ -476,61 +530,194 @@ extern "C" __global__ void dem_ccd_apply(const DeDrum* __restrict__ drums, const
   x[drums[who[k].x].b0 + who[k].y] = pos[k];
 }
 
-// ---- 4. XPBD projection (dem.rs:627-668) and the velocity rebuild (:670-673) ---------------------
+// Level schedule of a sweep over m items (ball pairs i < j) in sweep order, inactive items skipped:
+// level(k) = 1 + the largest level of an earlier active item on ball i or on ball j (0 when there is
+// none). Items of one level touch distinct balls, and each ball's items have rising levels, so
+// running the levels in order (the items of a level side by side) updates every ball exactly as the
+// sequential sweep does. Wave 0 assigns the levels, 64 items at a time: lane k starts from the
+// balls' last levels (s_last, n entries of LDS) and takes in, in lane order, the levels of the
+// earlier lanes of its batch that share a ball. Then all lanes sort the items by level: lev_g[k]
+// (global) holds the levels, s_off[0 .. depth] the level starts, and position p of the level order
+// gets pk_g[p] = i | j << 16 and k_g[p] = k. Returns the depth, or DE_LCAP + 1 (nothing sorted)
+// for a deeper schedule. Every thread of the workgroup calls it.
+#define DE_LCAP 4096
+template <class ItemFn>
+__device__ uint32_t de_levels(ItemFn item, uint32_t m, uint32_t n, int32_t* s_last, uint32_t* s_off, uint32_t* s_w,
+                              uint32_t* s_depth, uint32_t* __restrict__ lev_g, uint32_t* __restrict__ pk_g,
+                              uint32_t* __restrict__ k_g) {
+  const uint32_t t = threadIdx.x, lane = t & 63u;
+  for (uint32_t b = t; b < n; b += DE_BLOCK) s_last[b] = -1;
+  __syncthreads();
+  if (t < 64) {
+    int32_t depth = 0;
+    for (uint32_t base = 0; base < m; base += 64) {
+      const uint32_t k = base + lane;
+      uint32_t i = 0, j = 0;
+      const bool act = k < m && item(k, i, j);
+      int32_t l = act ? 1 + max(s_last[i], s_last[j]) : -1;
+      uint64_t am = __ballot(act);
+      while (am) {
+        const int src = __ffsll((unsigned long long)am) - 1;
+        am &= am - 1ull;
+        const int32_t ls = __builtin_amdgcn_readlane(l, src);
+        const uint32_t is = (uint32_t)__builtin_amdgcn_readlane((int)i, src);
+        const uint32_t js = (uint32_t)__builtin_amdgcn_readlane((int)j, src);
+        if (act && (int)lane > src && (i == is || i == js || j == is || j == js)) l = max(l, ls + 1);
+      }
+      if (act) {
+        atomicMax(&s_last[i], l);
+        atomicMax(&s_last[j], l);
+      }
+      if (k < m) lev_g[k] = act ? (uint32_t)l : DE_NO;
+      depth = max(depth, l + 1);
+    }
+    for (int o = 32; o > 0; o >>= 1) depth = max(depth, __shfl_xor(depth, o, 64));
+    if (lane == 0) *s_depth = (uint32_t)depth;
+  }
+  __syncthreads();
+  const uint32_t depth = *s_depth;
+  if (depth > DE_LCAP) return DE_LCAP + 1;
+  for (uint32_t L = t; L <= depth; L += DE_BLOCK) s_off[L] = 0;
+  __syncthreads();
+  for (uint32_t k = t; k < m; k += DE_BLOCK) {
+    const uint32_t l = lev_g[k];
+    if (l != DE_NO) atomicAdd(&s_off[l], 1u);
+  }
+  __syncthreads();
+  uint32_t carry = 0;
+  for (uint32_t b = 0; b < depth; b += DE_BLOCK) {  // exclusive scan of the level counts
+    const uint32_t L = b + t;
+    const uint32_t c = L < depth ? s_off[L] : 0u;
+    uint32_t tot;
+    const uint32_t ex = de_block_scan(c, s_w, &tot);
+    if (L < depth) s_off[L] = carry + ex;
+    carry += tot;
+    __syncthreads();
+  }
+  if (t == 0) s_off[depth] = carry;
+  __syncthreads();
+  for (uint32_t k = t; k < m; k += DE_BLOCK) {  // any order inside a level
+    const uint32_t l = lev_g[k];
+    if (l == DE_NO) continue;
+    uint32_t i = 0, j = 0;
+    item(k, i, j);
+    const uint32_t p = atomicAdd(&s_off[l], 1u);
+    pk_g[p] = i | (j << 16);
+    k_g[p] = k;
+  }
+  __syncthreads();
+  // s_off[L] now ends level L: shift back to the starts, the highest block of 256 first (a block
+  // reads the entry just below it, which a lower block would overwrite)
+  for (int32_t b = (int32_t)(depth / DE_BLOCK) * DE_BLOCK; b >= 0; b -= DE_BLOCK) {
+    const uint32_t L = (uint32_t)b + t;
+    const uint32_t v = (L > 0 && L <= depth) ? s_off[L - 1] : 0u;
+    __syncthreads();
+    if (L <= depth && L > 0) s_off[L] = v;
+    __syncthreads();
+  }
+  if (t == 0) s_off[0] = 0;
+  __syncthreads();
+  return depth;
+}
+
+// ---- 4. XPBD projection (dem.rs:710-750) and the velocity rebuild (:752-756) ---------------------
+// Pairs in emission order, then walls, per iteration.
+// One workgroup per population, the positions in LDS. The pair sweep runs by levels (de_levels over
+// the emission order): per level the lanes update distinct pairs side by side, with the same f32
+// operations as the sequential sweep, and each pair's accumulated impulse is kept in level order
+// (lam_s) and written back to emission order at the end. A schedule deeper than DE_LCAP levels
+// falls back to lane 0 walking the pairs in order.
+__device__ __forceinline__ void de_project_pair(float2* s_x, uint32_t pk, float& lam, float w, float ws, float cap_c,
+                                                float two_r) {
+  const uint32_t i = pk & 0xFFFFu, j = pk >> 16;
+  const float dx = s_x[i].x - s_x[j].x, dy = s_x[i].y - s_x[j].y;
+  const float dist = sqrtf(dx * dx + dy * dy);
+  const float c = dist - two_r;
+  if (c >= 0.0f) return;
+  float nx = 1.0f, ny = 0.0f;
+  if (dist > 1e-9f) {
+    nx = dx / dist;
+    ny = dy / dist;
+  }
+  if (ws <= 0.0f) return;
+  const float ce = fmaxf(c, cap_c);
+  const float dl = -ce / ws;
+  const float s = w * dl;
+  s_x[i].x += s * nx;
+  s_x[i].y += s * ny;
+  s_x[j].x -= s * nx;
+  s_x[j].y -= s * ny;
+  lam += dl;
+}
+
 extern "C" __global__ void __launch_bounds__(DE_BLOCK)
 dem_project(const DeDrum* __restrict__ drums, const float2* __restrict__ lift, float2* __restrict__ x,
             float2* __restrict__ v, const float* __restrict__ th, float* __restrict__ om,
             const float2* __restrict__ x0, const float* __restrict__ th0, const uint2* __restrict__ pairs,
-            float* __restrict__ lam, float* __restrict__ lam_bw, uint32_t stride, uint32_t pcap,
-            const DeOut* __restrict__ out) {
+            float* __restrict__ lam, float* __restrict__ lam_bw, uint32_t* __restrict__ lev_g,
+            uint32_t* __restrict__ pk_g, uint32_t* __restrict__ k_g, float* __restrict__ lam_s, uint32_t stride,
+            uint32_t pcap, DeOut* __restrict__ out) {
   __shared__ float2 s_x[DE_MAX_BALLS];
   __shared__ float s_lbw[DE_MAX_BALLS];
-  __shared__ uint2 s_p[DE_CHUNK];
+  __shared__ int32_t s_last[DE_MAX_BALLS];
+  __shared__ uint32_t s_off[DE_LCAP + 1];
+  __shared__ uint32_t s_pk[DE_CHUNK];
   __shared__ float s_l[DE_CHUNK];
+  __shared__ uint32_t s_w[DE_BLOCK / 64], s_depth;
   const DeDrum D = drums[blockIdx.x];
   const mgg::Geo g = de_geo(D, lift);
   const uint32_t t = threadIdx.x, n = D.n;
   const size_t sb = (size_t)blockIdx.x * stride, pb = (size_t)blockIdx.x * pcap;
   const uint32_t np = min(out[blockIdx.x].n_pairs, pcap);
+  const uint2* P = pairs + pb;
+  auto item = [P](uint32_t k, uint32_t& i, uint32_t& j) {
+    const uint2 p = P[k];
+    i = p.x;
+    j = p.y;
+    return true;
+  };
+  const uint32_t depth = de_levels(item, np, n, s_last, s_off, s_w, &s_depth, lev_g + pb, pk_g + pb, k_g + pb);
+  const bool leveled = depth <= DE_LCAP;
+  if (t == 0) out[blockIdx.x].depth = depth;
   for (uint32_t i = t; i < n; i += DE_BLOCK) {
     s_x[i] = x[D.b0 + i];
     s_lbw[i] = 0.0f;
   }
+  if (leveled)
+    for (uint32_t p = t; p < np; p += DE_BLOCK) lam_s[pb + p] = 0.0f;
   __syncthreads();
   const float w = D.w, ws = D.w + D.w, cap_c = -0.2f * 2.0f * D.r, two_r = 2.0f * D.r;
   for (uint32_t it = 0; it < max(D.iters, 1u); it++) {
+    uint32_t L = 0;
     for (uint32_t c0 = 0; c0 < np; c0 += DE_CHUNK) {
-      const uint32_t m = min(np - c0, (uint32_t)DE_CHUNK);
+      const uint32_t m = min(np - c0, (uint32_t)DE_CHUNK), c1 = c0 + m;
       for (uint32_t q = t; q < m; q += DE_BLOCK) {
-        s_p[q] = pairs[pb + c0 + q];
-        s_l[q] = lam[pb + c0 + q];
-      }
-      __syncthreads();
-      if (t == 0) {
-        for (uint32_t q = 0; q < m; q++) {
-          const uint32_t i = s_p[q].x, j = s_p[q].y;
-          const float dx = s_x[i].x - s_x[j].x, dy = s_x[i].y - s_x[j].y;
-          const float dist = sqrtf(dx * dx + dy * dy);
-          const float c = dist - two_r;
-          if (c >= 0.0f) continue;
-          float nx = 1.0f, ny = 0.0f;
-          if (dist > 1e-9f) {
-            nx = dx / dist;
-            ny = dy / dist;
-          }
-          if (ws <= 0.0f) continue;
-          const float ce = fmaxf(c, cap_c);
-          const float dl = -ce / ws;
-          const float s = w * dl;
-          s_x[i].x += s * nx;
-          s_x[i].y += s * ny;
-          s_x[j].x -= s * nx;
-          s_x[j].y -= s * ny;
-          s_l[q] += dl;
+        if (leveled) {
+          s_pk[q] = pk_g[pb + c0 + q];
+          s_l[q] = lam_s[pb + c0 + q];
+        } else {
+          const uint2 p = P[c0 + q];
+          s_pk[q] = p.x | (p.y << 16);
+          s_l[q] = lam[pb + c0 + q];
         }
       }
       __syncthreads();
-      for (uint32_t q = t; q < m; q += DE_BLOCK) lam[pb + c0 + q] = s_l[q];
+      if (leveled) {
+        while (L < depth && s_off[L] < c1) {  // the levels (or their parts) inside this chunk
+          const uint32_t a = max(s_off[L], c0), e = min(s_off[L + 1], c1);
+          for (uint32_t p = a + t; p < e; p += DE_BLOCK) de_project_pair(s_x, s_pk[p - c0], s_l[p - c0], w, ws, cap_c, two_r);
+          __syncthreads();
+          if (s_off[L + 1] > c1) break;  // the level goes on in the next chunk
+          L++;
+        }
+      } else if (t == 0) {
+        for (uint32_t q = 0; q < m; q++) de_project_pair(s_x, s_pk[q], s_l[q], w, ws, cap_c, two_r);
+      }
+      __syncthreads();
+      for (uint32_t q = t; q < m; q += DE_BLOCK) {
+        if (leveled) lam_s[pb + c0 + q] = s_l[q];
+        else lam[pb + c0 + q] = s_l[q];
+      }
       __syncthreads();
     }
     for (uint32_t i = t; i < n; i += DE_BLOCK) {  // walls: each ball on its own
@@ -548,6 +735,8 @@ dem_project(const DeDrum* __restrict__ drums, const float2* __restrict__ lift, f
     }
     __syncthreads();
   }
+  if (leveled)
+    for (uint32_t p = t; p < np; p += DE_BLOCK) lam[pb + k_g[pb + p]] = lam_s[pb + p];
   for (uint32_t i = t; i < n; i += DE_BLOCK) {
     const float2 p = s_x[i], p0 = x0[sb + i];
     x[D.b0 + i] = p;
@@ -558,15 +747,23 @@ dem_project(const DeDrum* __restrict__ drums, const float2* __restrict__ lift, f
 }
 
 // ---- 5. ContactBook::sorted_ball_ball (dem.rs:459-467): by i (counting sort), then j ----------------
+// The sweeps over the sorted book (friction, restitution, the rolling-resistance totals) act only on
+// contacts with a positive normal impulse: those are compacted in book order (spk: i | j << 16,
+// slam: the impulse) and given a level schedule (de_levels; level order in cpk / clam, level starts
+// in coff), which the three sweeps share.
 extern "C" __global__ void __launch_bounds__(DE_BLOCK)
 dem_sort(const DeDrum* __restrict__ drums, const uint2* __restrict__ pairs, const float* __restrict__ lam,
-         DeItem* __restrict__ sorted, uint32_t pcap, const DeOut* __restrict__ out) {
+         DeItem* __restrict__ sorted, uint32_t* __restrict__ spk, float* __restrict__ slam,
+         uint32_t* __restrict__ lev_g, uint32_t* __restrict__ cpk, uint32_t* __restrict__ ck,
+         float* __restrict__ clam, uint32_t* __restrict__ coff, uint32_t pcap, DeOut* __restrict__ out) {
   __shared__ uint32_t s_c[DE_MAX_BALLS + 1];
+  __shared__ int32_t s_last[DE_MAX_BALLS];
+  __shared__ uint32_t s_off[DE_LCAP + 1];
   __shared__ uint32_t s_w[DE_BLOCK / 64];
-  __shared__ uint32_t s_carry;
+  __shared__ uint32_t s_carry, s_depth;
   const DeDrum D = drums[blockIdx.x];
   const uint32_t t = threadIdx.x, n = D.n;
-  const size_t pb = (size_t)blockIdx.x * pcap;
+  const size_t pb = (size_t)blockIdx.x * pcap, ob = (size_t)blockIdx.x * (DE_LCAP + 1);
   const uint32_t np = min(out[blockIdx.x].n_pairs, pcap);
   for (uint32_t i = t; i <= n; i += DE_BLOCK) s_c[i] = 0;
   if (t == 0) s_carry = 0;
@@ -608,23 +805,78 @@ dem_sort(const DeDrum* __restrict__ drums, const uint2* __restrict__ pairs, cons
       sorted[pb + b] = it;
     }
   }
+  __syncthreads();
+  uint32_t na = 0;  // the active contacts in book order
+  for (uint32_t b = 0; b < np; b += DE_BLOCK) {
+    const uint32_t k = b + t;
+    DeItem it{0u, 0u, 0.0f};
+    if (k < np) it = sorted[pb + k];
+    const bool act = k < np && it.lam > 0.0f;
+    uint32_t tot;
+    const uint32_t rank = de_block_scan(act ? 1u : 0u, s_w, &tot);
+    if (act) {
+      spk[pb + na + rank] = it.i | (it.j << 16);
+      slam[pb + na + rank] = it.lam;
+    }
+    na += tot;
+    __syncthreads();
+  }
+  const uint32_t* SP = spk + pb;
+  auto item = [SP](uint32_t k, uint32_t& i, uint32_t& j) {
+    const uint32_t v = SP[k];
+    i = v & 0xFFFFu;
+    j = v >> 16;
+    return true;
+  };
+  const uint32_t depth = de_levels(item, na, n, s_last, s_off, s_w, &s_depth, lev_g + pb, cpk + pb, ck + pb);
+  if (depth <= DE_LCAP) {
+    for (uint32_t L = t; L <= depth; L += DE_BLOCK) coff[ob + L] = s_off[L];
+    for (uint32_t p = t; p < na; p += DE_BLOCK) clam[pb + p] = slam[pb + ck[pb + p]];
+  }
+  if (t == 0) {
+    out[blockIdx.x].nact = na;
+    out[blockIdx.x].cdepth = depth;
+  }
+}
+
+// Runs f(i, j, lam) over the active contacts in book order on the level schedule (all lanes, a
+// barrier per level) or, for a schedule deeper than DE_LCAP, on lane 0 in book order. Every thread
+// calls it; it ends with a barrier.
+template <class F>
+__device__ void de_contact_sweep(F f, const DeOut& o, const uint32_t* __restrict__ spk, const float* __restrict__ slam,
+                                 const uint32_t* __restrict__ cpk, const float* __restrict__ clam,
+                                 const uint32_t* __restrict__ coff) {
+  const uint32_t t = threadIdx.x, na = o.nact, depth = o.cdepth;
+  if (depth <= DE_LCAP) {
+    for (uint32_t L = 0; L < depth; L++) {
+      const uint32_t a = coff[L], e = coff[L + 1];
+      for (uint32_t p = a + t; p < e; p += DE_BLOCK) {
+        const uint32_t v = cpk[p];
+        f(v & 0xFFFFu, v >> 16, clam[p]);
+      }
+      __syncthreads();
+    }
+  } else {
+    if (t == 0)
+      for (uint32_t k = 0; k < na; k++) f(spk[k] & 0xFFFFu, spk[k] >> 16, slam[k]);
+    __syncthreads();
+  }
 }
 
 // ---- 6. friction (dem.rs:676-890) and the velocity rebuild (:892-895) ------------------------------
 extern "C" __global__ void __launch_bounds__(DE_BLOCK)
 dem_friction(const DeDrum* __restrict__ drums, const float2* __restrict__ lift, float2* __restrict__ x,
              float2* __restrict__ v, float* __restrict__ th, float* __restrict__ om, const float2* __restrict__ x0,
-             const float* __restrict__ th0, const DeItem* __restrict__ sorted, const float* __restrict__ lam_bw,
-             uint32_t stride, uint32_t pcap, DeOut* __restrict__ out) {
+             const float* __restrict__ th0, const uint32_t* __restrict__ spk, const float* __restrict__ slam,
+             const uint32_t* __restrict__ cpk, const float* __restrict__ clam, const uint32_t* __restrict__ coff,
+             const float* __restrict__ lam_bw, uint32_t stride, uint32_t pcap, DeOut* __restrict__ out) {
   __shared__ float2 s_x[DE_MAX_BALLS], s_v[DE_MAX_BALLS];
   __shared__ float s_th[DE_MAX_BALLS], s_om[DE_MAX_BALLS];
-  __shared__ DeItem s_it[DE_CHUNK];
   __shared__ float s_ww[2 * DE_BLOCK];
   const DeDrum D = drums[blockIdx.x];
   const mgg::Geo g = de_geo(D, lift);
   const uint32_t t = threadIdx.x, n = D.n;
-  const size_t sb = (size_t)blockIdx.x * stride, pb = (size_t)blockIdx.x * pcap;
-  const uint32_t np = min(out[blockIdx.x].n_pairs, pcap);
+  const size_t sb = (size_t)blockIdx.x * stride, pb = (size_t)blockIdx.x * pcap, ob = (size_t)blockIdx.x * (DE_LCAP + 1);
   for (uint32_t i = t; i < n; i += DE_BLOCK) {
     s_x[i] = x[D.b0 + i];
     s_v[i] = v[D.b0 + i];
@@ -634,50 +886,40 @@ dem_friction(const DeDrum* __restrict__ drums, const float2* __restrict__ lift, 
   __syncthreads();
   const float r = D.r, w = D.w, wr = D.w_rot, dt = D.dt;
   const float wst_bb = w + w + r * r * wr + r * r * wr;
-  for (uint32_t c0 = 0; c0 < np; c0 += DE_CHUNK) {
-    const uint32_t m = min(np - c0, (uint32_t)DE_CHUNK);
-    for (uint32_t q = t; q < m; q += DE_BLOCK) s_it[q] = sorted[pb + c0 + q];
-    __syncthreads();
-    if (t == 0) {
-      for (uint32_t q = 0; q < m; q++) {
-        const DeItem c = s_it[q];
-        if (c.lam <= 0.0f) continue;
-        const uint32_t i = c.i, j = c.j;
-        const float dx = s_x[i].x - s_x[j].x, dy = s_x[i].y - s_x[j].y;
-        const float dist = sqrtf(dx * dx + dy * dy);
-        float nx = 1.0f, ny = 0.0f;
-        if (dist > 1e-9f) {
-          nx = dx / dist;
-          ny = dy / dist;
-        }
-        const float tx = -ny, ty = nx;
-        const float vt = ((s_v[i].x - s_v[j].x) * tx + (s_v[i].y - s_v[j].y) * ty) - r * s_om[i] - r * s_om[j];
-        if (wst_bb <= 0.0f) continue;
-        const float raw = -vt * dt / wst_bb;
-        const float mm = de_friction(D.fric_bb, D.fric_bb_s, vt, D.fric_vs) * c.lam;
-        float dlt = raw;
-        if (dlt < -mm) dlt = -mm;
-        if (dlt > mm) dlt = mm;
-        const float s = w * dlt;
-        s_x[i].x += s * tx;
-        s_x[i].y += s * ty;
-        s_x[j].x -= s * tx;
-        s_x[j].y -= s * ty;
-        const float dth = r * wr * dlt;
-        s_th[i] -= dth;
-        s_th[j] -= dth;
-        const float dvx = s * tx / dt, dvy = s * ty / dt;
-        const float dom = -r * wr * dlt / dt;
-        s_v[i].x += dvx;
-        s_v[i].y += dvy;
-        s_v[j].x -= dvx;
-        s_v[j].y -= dvy;
-        s_om[i] += dom;
-        s_om[j] += dom;
-      }
+  auto pair = [&](uint32_t i, uint32_t j, float lamc) {
+    const float dx = s_x[i].x - s_x[j].x, dy = s_x[i].y - s_x[j].y;
+    const float dist = sqrtf(dx * dx + dy * dy);
+    float nx = 1.0f, ny = 0.0f;
+    if (dist > 1e-9f) {
+      nx = dx / dist;
+      ny = dy / dist;
     }
-    __syncthreads();
-  }
+    const float tx = -ny, ty = nx;
+    const float vt = ((s_v[i].x - s_v[j].x) * tx + (s_v[i].y - s_v[j].y) * ty) - r * s_om[i] - r * s_om[j];
+    if (wst_bb <= 0.0f) return;
+    const float raw = -vt * dt / wst_bb;
+    const float mm = de_friction(D.fric_bb, D.fric_bb_s, vt, D.fric_vs) * lamc;
+    float dlt = raw;
+    if (dlt < -mm) dlt = -mm;
+    if (dlt > mm) dlt = mm;
+    const float s = w * dlt;
+    s_x[i].x += s * tx;
+    s_x[i].y += s * ty;
+    s_x[j].x -= s * tx;
+    s_x[j].y -= s * ty;
+    const float dth = r * wr * dlt;
+    s_th[i] -= dth;
+    s_th[j] -= dth;
+    const float dvx = s * tx / dt, dvy = s * ty / dt;
+    const float dom = -r * wr * dlt / dt;
+    s_v[i].x += dvx;
+    s_v[i].y += dvy;
+    s_v[j].x -= dvx;
+    s_v[j].y -= dvy;
+    s_om[i] += dom;
+    s_om[j] += dom;
+  };
+  de_contact_sweep(pair, out[blockIdx.x], spk + pb, slam + pb, cpk + pb, clam + pb, coff + ob);
   // walls: each ball on its own; the wall-work terms are added in ball order by lane 0
   float ww = 0.0f;
   const float wst_bw = w + r * r * wr;
@@ -734,19 +976,21 @@ dem_friction(const DeDrum* __restrict__ drums, const float2* __restrict__ lift, 
 extern "C" __global__ void __launch_bounds__(DE_BLOCK)
 dem_restitution(const DeDrum* __restrict__ drums, const float2* __restrict__ lift, const float2* __restrict__ x,
                 float2* __restrict__ v, const float2* __restrict__ vpre, const float* __restrict__ ompre,
-                const DeItem* __restrict__ sorted, const float* __restrict__ lam_bw, const float* __restrict__ ebin,
-                mg_impact_record* __restrict__ rec, uint32_t rcap, uint32_t stride, uint32_t pcap,
-                DeOut* __restrict__ out) {
+                const uint32_t* __restrict__ spk, const float* __restrict__ slam, const uint32_t* __restrict__ cpk,
+                const float* __restrict__ clam, const uint32_t* __restrict__ coff, const float* __restrict__ lam_bw,
+                const float* __restrict__ ebin, mg_impact_record* __restrict__ rec, uint32_t rcap, uint32_t stride,
+                uint32_t pcap, DeOut* __restrict__ out) {
   __shared__ float2 s_x[DE_MAX_BALLS], s_v[DE_MAX_BALLS], s_vp[DE_MAX_BALLS];
   __shared__ float s_op[DE_MAX_BALLS];
-  __shared__ DeItem s_it[DE_CHUNK / 2];
   __shared__ uint32_t s_hist[12], s_coll, s_nrec;
   __shared__ uint32_t s_w[DE_BLOCK / 64];
   const DeDrum D = drums[blockIdx.x];
   const mgg::Geo g = de_geo(D, lift);
   const uint32_t t = threadIdx.x, n = D.n;
   const size_t sb = (size_t)blockIdx.x * stride, pb = (size_t)blockIdx.x * pcap, rb = (size_t)blockIdx.x * rcap;
-  const uint32_t np = min(out[blockIdx.x].n_pairs, pcap);
+  const size_t ob = (size_t)blockIdx.x * (DE_LCAP + 1);
+  const DeOut& o = out[blockIdx.x];
+  const uint32_t na = o.nact;
   for (uint32_t i = t; i < n; i += DE_BLOCK) {
     s_x[i] = x[D.b0 + i];
     s_v[i] = v[D.b0 + i];
@@ -757,51 +1001,69 @@ dem_restitution(const DeDrum* __restrict__ drums, const float2* __restrict__ lif
   if (t == 0) { s_coll = 0; s_nrec = 0; }
   __syncthreads();
   const float r = D.r, w = D.w;
-  for (uint32_t c0 = 0; c0 < np; c0 += DE_CHUNK / 2) {
-    const uint32_t m = min(np - c0, (uint32_t)(DE_CHUNK / 2));
-    for (uint32_t q = t; q < m; q += DE_BLOCK) s_it[q] = sorted[pb + c0 + q];
-    __syncthreads();
-    if (t == 0) {
-      for (uint32_t q = 0; q < m; q++) {
-        const DeItem c = s_it[q];
-        if (c.lam <= 0.0f) continue;
-        const uint32_t i = c.i, j = c.j;
-        const float dx = s_x[i].x - s_x[j].x, dy = s_x[i].y - s_x[j].y;
-        const float dist = sqrtf(dx * dx + dy * dy);
-        float nx = 1.0f, ny = 0.0f;
-        if (dist > 1e-9f) {
-          nx = dx / dist;
-          ny = dy / dist;
-        }
-        const float vnp = (s_vp[i].x - s_vp[j].x) * nx + (s_vp[i].y - s_vp[j].y) * ny;
-        const bool fresh = vnp < -0.02f;
-        if (fresh) {
-          s_coll++;
-          const int b = de_ebin(0.5f * (D.mass * 0.5f) * vnp * vnp, ebin);
-          if (b >= 0) s_hist[b]++;
-        }
-        const float e = fresh ? D.rest_bb : 0.0f;
-        if (-vnp > D.thr) {
-          const float tx = -ny, ty = nx;
-          const float slip = ((s_vp[i].x - s_vp[j].x) * tx + (s_vp[i].y - s_vp[j].y) * ty) - r * s_op[i] - r * s_op[j];
-          if (s_nrec < rcap) rec[rb + s_nrec] = mg_impact_record{0u, i, j, -vnp, fabsf(slip), e};
-          s_nrec++;
-        }
-        const float target = fmaxf(fmaxf(-e * vnp, 0.0f), fmaxf(vnp, 0.0f));
-        const float vnn = (s_v[i].x - s_v[j].x) * nx + (s_v[i].y - s_v[j].y) * ny;
-        const float dvn = target - vnn;
-        const float ws = w + w;
-        if (ws <= 0.0f) continue;
-        const float im = dvn / ws;
-        const float s = w * im;
-        s_v[i].x += s * nx;
-        s_v[i].y += s * ny;
-        s_v[j].x -= s * nx;
-        s_v[j].y -= s * ny;
+  // the contact's normal, its pre-step normal speed and whether it is a fresh impact: these read
+  // positions and pre-step velocities, which the sweep below does not change
+  auto normal = [&](uint32_t i, uint32_t j, float& nx, float& ny) {
+    const float dx = s_x[i].x - s_x[j].x, dy = s_x[i].y - s_x[j].y;
+    const float dist = sqrtf(dx * dx + dy * dy);
+    nx = 1.0f;
+    ny = 0.0f;
+    if (dist > 1e-9f) {
+      nx = dx / dist;
+      ny = dy / dist;
+    }
+    return (s_vp[i].x - s_vp[j].x) * nx + (s_vp[i].y - s_vp[j].y) * ny;
+  };
+  // collisions, the energy histogram and the impact records, in book order (record k goes after
+  // the records of the contacts before it)
+  for (uint32_t b = 0; b < na; b += DE_BLOCK) {  // na is uniform
+    const uint32_t k = b + t;
+    uint32_t want = 0;
+    mg_impact_record rr{};
+    if (k < na) {
+      const uint32_t pk = spk[pb + k], i = pk & 0xFFFFu, j = pk >> 16;
+      float nx, ny;
+      const float vnp = normal(i, j, nx, ny);
+      const bool fresh = vnp < -0.02f;
+      if (fresh) {
+        atomicAdd(&s_coll, 1u);
+        const int bin = de_ebin(0.5f * (D.mass * 0.5f) * vnp * vnp, ebin);
+        if (bin >= 0) atomicAdd(&s_hist[bin], 1u);
+      }
+      const float e = fresh ? D.rest_bb : 0.0f;
+      if (-vnp > D.thr) {
+        const float tx = -ny, ty = nx;
+        const float slip = ((s_vp[i].x - s_vp[j].x) * tx + (s_vp[i].y - s_vp[j].y) * ty) - r * s_op[i] - r * s_op[j];
+        rr = mg_impact_record{0u, i, j, -vnp, fabsf(slip), e};
+        want = 1;
       }
     }
+    uint32_t tot;
+    const uint32_t rank = de_block_scan(want, s_w, &tot);
+    const uint32_t at = s_nrec + rank;
+    if (want && at < rcap) rec[rb + at] = rr;
+    __syncthreads();
+    if (t == 0) s_nrec += tot;
     __syncthreads();
   }
+  // the velocity targets, contact after contact on each ball
+  auto pair = [&](uint32_t i, uint32_t j, float) {
+    float nx, ny;
+    const float vnp = normal(i, j, nx, ny);
+    const float e = vnp < -0.02f ? D.rest_bb : 0.0f;
+    const float target = fmaxf(fmaxf(-e * vnp, 0.0f), fmaxf(vnp, 0.0f));
+    const float vnn = (s_v[i].x - s_v[j].x) * nx + (s_v[i].y - s_v[j].y) * ny;
+    const float dvn = target - vnn;
+    const float ws = w + w;
+    if (ws <= 0.0f) return;
+    const float im = dvn / ws;
+    const float s = w * im;
+    s_v[i].x += s * nx;
+    s_v[i].y += s * ny;
+    s_v[j].x -= s * nx;
+    s_v[j].y -= s * ny;
+  };
+  de_contact_sweep(pair, o, spk + pb, slam + pb, cpk + pb, clam + pb, coff + ob);
   // walls: each ball on its own; records in ball order after the pair records
   for (uint32_t base = 0; base < n; base += DE_BLOCK) {
     const uint32_t i = base + t;
@@ -857,30 +1119,23 @@ dem_restitution(const DeDrum* __restrict__ drums, const float2* __restrict__ lif
 // ---- 8. rolling resistance (dem.rs:1036-1055) and the final energy (:1057-1065) -------------------
 extern "C" __global__ void __launch_bounds__(DE_BLOCK)
 dem_finish(const DeDrum* __restrict__ drums, const float2* __restrict__ x, const float2* __restrict__ v,
-           float* __restrict__ om, const DeItem* __restrict__ sorted, const float* __restrict__ lam_bw,
-           uint32_t stride, uint32_t pcap, DeOut* __restrict__ out) {
+           float* __restrict__ om, const uint32_t* __restrict__ spk, const float* __restrict__ slam,
+           const uint32_t* __restrict__ cpk, const float* __restrict__ clam, const uint32_t* __restrict__ coff,
+           const float* __restrict__ lam_bw, uint32_t stride, uint32_t pcap, DeOut* __restrict__ out) {
   __shared__ float s_tl[DE_MAX_BALLS];
-  __shared__ DeItem s_it[DE_CHUNK];
   __shared__ __align__(16) float s_e[3 * DE_BLOCK];
   const DeDrum D = drums[blockIdx.x];
   const uint32_t t = threadIdx.x, n = D.n;
-  const size_t sb = (size_t)blockIdx.x * stride, pb = (size_t)blockIdx.x * pcap;
-  const uint32_t np = min(out[blockIdx.x].n_pairs, pcap);
+  const size_t sb = (size_t)blockIdx.x * stride, pb = (size_t)blockIdx.x * pcap, ob = (size_t)blockIdx.x * (DE_LCAP + 1);
   if (D.rolling > 0.0f && D.inertia > 0.0f) {
     for (uint32_t i = t; i < n; i += DE_BLOCK) s_tl[i] = 0.0f;
     __syncthreads();
-    for (uint32_t c0 = 0; c0 < np; c0 += DE_CHUNK) {  // total normal impulse per ball, contact order
-      const uint32_t m = min(np - c0, (uint32_t)DE_CHUNK);
-      for (uint32_t q = t; q < m; q += DE_BLOCK) s_it[q] = sorted[pb + c0 + q];
-      __syncthreads();
-      if (t == 0)
-        for (uint32_t q = 0; q < m; q++) {
-          const float l = fmaxf(s_it[q].lam, 0.0f);
-          s_tl[s_it[q].i] += l;
-          s_tl[s_it[q].j] += l;
-        }
-      __syncthreads();
-    }
+    // total normal impulse per ball in contact order (contacts without impulse add max(lam, 0) = +0.0)
+    auto pair = [&](uint32_t i, uint32_t j, float l) {
+      s_tl[i] += l;
+      s_tl[j] += l;
+    };
+    de_contact_sweep(pair, out[blockIdx.x], spk + pb, slam + pb, cpk + pb, clam + pb, coff + ob);
     for (uint32_t i = t; i < n; i += DE_BLOCK) {
       const float tl = s_tl[i] + fmaxf(lam_bw[sb + i], 0.0f);
       const float o = om[D.b0 + i];
@@ -916,7 +1171,10 @@ struct mg_dem {
   uint32_t lift_cap = 0;
   DeOut* out = nullptr;
   float2 *x0 = nullptr, *vpre = nullptr;
-  float *th0 = nullptr, *ompre = nullptr, *lam_bw = nullptr, *lam = nullptr, *ebin = nullptr;
+  float *th0 = nullptr, *ompre = nullptr, *lam_bw = nullptr, *lam = nullptr, *ebin = nullptr, *lam_s = nullptr;
+  uint32_t *lev = nullptr, *lpk = nullptr, *lk = nullptr;  // a level schedule (projection, then contacts)
+  uint32_t *spk = nullptr, *coff = nullptr;  // active contacts in book order; contact level starts
+  float *slam = nullptr, *clam = nullptr;
   uint32_t* flagged = nullptr;
   uint2* pairs = nullptr;
   DeItem* sorted = nullptr;
@@ -1018,6 +1276,9 @@ int mg_dem_create(uint32_t n_drums, uint32_t max_balls, uint32_t pairs_per_ball,
   const size_t o_out = take(n_drums * sizeof(DeOut)), o_x0 = take(nb * 8), o_vp = take(nb * 8);
   const size_t o_th0 = take(nb * 4), o_op = take(nb * 4), o_lbw = take(nb * 4), o_flag = take(nb * 4);
   const size_t o_ebin = take(16 * 4), o_pairs = take(np * 8), o_lam = take(np * 4), o_sorted = take(np * sizeof(DeItem));
+  const size_t o_lev = take(np * 4), o_lpk = take(np * 4), o_lk = take(np * 4), o_lam_s = take(np * 4);
+  const size_t o_spk = take(np * 4), o_slam = take(np * 4), o_clam = take(np * 4);
+  const size_t o_coff = take((size_t)n_drums * (DE_LCAP + 1) * 4);
   const size_t o_rec = take((size_t)n_drums * h->rcap * sizeof(mg_impact_record));
   if (hipMalloc(&h->dev, off) != hipSuccess) {
     delete h;
@@ -1035,6 +1296,14 @@ int mg_dem_create(uint32_t n_drums, uint32_t max_balls, uint32_t pairs_per_ball,
   h->ebin = reinterpret_cast<float*>(h->dev + o_ebin);
   h->pairs = reinterpret_cast<uint2*>(h->dev + o_pairs);
   h->lam = reinterpret_cast<float*>(h->dev + o_lam);
+  h->lev = reinterpret_cast<uint32_t*>(h->dev + o_lev);
+  h->lpk = reinterpret_cast<uint32_t*>(h->dev + o_lpk);
+  h->lk = reinterpret_cast<uint32_t*>(h->dev + o_lk);
+  h->lam_s = reinterpret_cast<float*>(h->dev + o_lam_s);
+  h->spk = reinterpret_cast<uint32_t*>(h->dev + o_spk);
+  h->slam = reinterpret_cast<float*>(h->dev + o_slam);
+  h->clam = reinterpret_cast<float*>(h->dev + o_clam);
+  h->coff = reinterpret_cast<uint32_t*>(h->dev + o_coff);
   h->sorted = reinterpret_cast<DeItem*>(h->dev + o_sorted);
   h->rec = reinterpret_cast<mg_impact_record*>(h->dev + o_rec);
   float thr[11];
@@ -1186,13 +1455,15 @@ int mg_dem_step(mg_dem* h, const mg_dem_drum* drums, float* d_x, float* d_v, flo
     if (!ok) return de_fail(h, err, errlen, MG_E_DEVICE, "HIP error applying the swept clamp");
   }
   hipLaunchKernelGGL(dem_project, G, B, 0, nullptr, h->drums, h->lift, x, v, d_theta, d_omega, h->x0, h->th0, h->pairs,
-                     h->lam, h->lam_bw, S, P, h->out);
-  hipLaunchKernelGGL(dem_sort, G, B, 0, nullptr, h->drums, h->pairs, h->lam, h->sorted, P, h->out);
-  hipLaunchKernelGGL(dem_friction, G, B, 0, nullptr, h->drums, h->lift, x, v, d_theta, d_omega, h->x0, h->th0,
-                     h->sorted, h->lam_bw, S, P, h->out);
-  hipLaunchKernelGGL(dem_restitution, G, B, 0, nullptr, h->drums, h->lift, x, v, h->vpre, h->ompre, h->sorted,
-                     h->lam_bw, h->ebin, h->rec, h->rcap, S, P, h->out);
-  hipLaunchKernelGGL(dem_finish, G, B, 0, nullptr, h->drums, x, v, d_omega, h->sorted, h->lam_bw, S, P, h->out);
+                     h->lam, h->lam_bw, h->lev, h->lpk, h->lk, h->lam_s, S, P, h->out);
+  hipLaunchKernelGGL(dem_sort, G, B, 0, nullptr, h->drums, h->pairs, h->lam, h->sorted, h->spk, h->slam, h->lev, h->lpk,
+                     h->lk, h->clam, h->coff, P, h->out);
+  hipLaunchKernelGGL(dem_friction, G, B, 0, nullptr, h->drums, h->lift, x, v, d_theta, d_omega, h->x0, h->th0, h->spk,
+                     h->slam, h->lpk, h->clam, h->coff, h->lam_bw, S, P, h->out);
+  hipLaunchKernelGGL(dem_restitution, G, B, 0, nullptr, h->drums, h->lift, x, v, h->vpre, h->ompre, h->spk, h->slam,
+                     h->lpk, h->clam, h->coff, h->lam_bw, h->ebin, h->rec, h->rcap, S, P, h->out);
+  hipLaunchKernelGGL(dem_finish, G, B, 0, nullptr, h->drums, x, v, d_omega, h->spk, h->slam, h->lpk, h->clam, h->coff,
+                     h->lam_bw, S, P, h->out);
   ok = hipGetLastError() == hipSuccess &&
        hipMemcpy(h->h_out.data(), h->out, nd * sizeof(DeOut), hipMemcpyDeviceToHost) == hipSuccess;
   if (!ok) return de_fail(h, err, errlen, MG_E_DEVICE, "HIP error in the DEM kernels");
