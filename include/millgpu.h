@@ -426,6 +426,65 @@ int mg_dem_step(mg_dem* dem, const mg_dem_drum* drums, float* d_x, float* d_v, f
  * dem.rs:152-169); *n receives the count. */
 int mg_dem_impacts(mg_dem* dem, uint32_t drum, mg_impact_record* out, uint32_t cap, uint32_t* n);
 
+/* ---- slurry: FluidParticles::step_coupled (pbf.rs:1174-2365) --------------------------------------
+ * One drum's scalars for one step: the Drum (geometry.rs:18-22, LiftersParams) and its angle, the
+ * FluidParticles scalars (h, rest_density, particle_mass; pbf.rs:1018-1028), the ball scalars the
+ * coupling reads (dem.rs:277-309: radius, mass, inertia, true radius), the SlurryParams the step reads
+ * (params.rs: viscosity, rheology 0 newtonian / 1 bingham, yield stress, wall and ball no-slip,
+ * wettability, surface tension), the density iterations (pbf_iterations) and the sub-step dt, the
+ * counts, and whether the caller's guess array holds a warm start for this drum (the previous joint
+ * viscosity solve's velocity change, FluidParticles::viscosity_guess). */
+typedef struct mg_fluid_drum {
+  float radius_m, omega, drum_angle;
+  uint32_t lifter_count;
+  float lifter_height_m, lifter_base_width_m, lifter_top_width_m, lifter_phase_deg;
+  float h, rest_density, particle_mass;
+  float ball_radius, ball_mass, ball_inertia, ball_true_radius;
+  float viscosity_pa_s;
+  uint32_t rheology;
+  float yield_stress_pa, wall_no_slip, ball_no_slip, wettability, surface_tension_n_m;
+  uint32_t iterations;
+  float dt;
+  uint32_t n_fluid, n_balls;
+  int32_t has_guess;
+} mg_fluid_drum;
+/* FluidStepStats (pbf.rs:1018-1028 and the step's tail) plus the step's structure counts: the wall
+ * impulse and angular impulse are f64 sums, the 32 wall-angle bins f32 sums; KE deltas are the f64
+ * kinetic-energy changes of the no-slip blend, form drag, buoyancy, viscosity solve and speed clamp.
+ * joint_solve = 1 when the viscosity solve included the balls (the guess array then holds the
+ * drum's next warm start); error_bits != 0 when a workspace capacity was exceeded. */
+typedef struct mg_fluid_stats {
+  uint32_t viscosity_iterations, density_iterations;
+  float mean_shear_rate_per_s, wall_work_j;
+  double wall_impulse_x, wall_impulse_y, wall_angular_impulse;
+  float wall_angle_bins[32];
+  double ke_delta_noslip_j, ke_delta_drag_j, ke_delta_buoyancy_j, ke_delta_viscosity_j, ke_delta_clamp_j;
+  uint32_t coupling_clamp_hits, n_groups, n_links, links_in_range;
+  int32_t joint_solve;
+  uint32_t error_bits;
+} mg_fluid_stats;
+typedef struct mg_fluid mg_fluid;
+/* Workspace for n_drums drums (at most 65535); its device memory grows with the batch it steps. */
+int mg_fluid_create(uint32_t n_drums, mg_fluid** out, char* err, size_t errlen);
+void mg_fluid_destroy(mg_fluid* fluid);
+/* Fixed density-pass and CG-iteration counts for every drum (on = 1), as the reference's tests
+ * force them; on = 0 restores the adaptive passes and the CG tolerance. No reference counterpart. */
+int mg_fluid_force_iterations(mg_fluid* fluid, int on, uint32_t density_passes, uint32_t cg_iterations);
+/* FluidParticles::step_coupled (pbf.rs:1174-2365) for every drum at once: drum d's fluid follows
+ * drum d-1's in d_x / d_v / d_guess (2 floats per particle; x and v updated in place, guess written
+ * for drums whose solve was joint), its balls follow drum d-1's in d_ball_x / d_ball_v (2 floats)
+ * and d_ball_omega (1), and the coupling impulses it returns (CouplingImpulses, coupling.rs:36-52)
+ * are written to d_impulse (2 floats per ball) and d_angular (1). stats (host, n_drums) receives
+ * each drum's FluidStepStats. Positions, velocities and impulses up to the viscosity solve equal the
+ * reference's f32 arithmetic bit for bit (adhesion, form drag and buoyancy in the reference's ball
+ * order; the form-drag and Bingham exponentials by the host libm's expf algorithm, the wall-angle bins
+ * by its atan2f); the CG's f64 dot products are summed in another order than the reference's, so the
+ * solve agrees to rounding. Waits for the device. MG_E_CAPACITY when a drum exceeded a workspace
+ * capacity (stats[d].error_bits says which; that drum's results are not valid). */
+int mg_fluid_step(mg_fluid* fluid, const mg_fluid_drum* drums, float* d_x, float* d_v, float* d_guess,
+                  const float* d_ball_x, const float* d_ball_v, const float* d_ball_omega, float* d_impulse,
+                  float* d_angular, mg_fluid_stats* stats, char* err, size_t errlen);
+
 /* ---- fluid free surface: surface::extract_surface (surface.rs:233-277) ---------------------------
  * One drum's inputs: the Drum (geometry.rs:18-22, LiftersParams), its angle, the FluidParticles
  * scalars the extraction reads (h, rest_density, particle_mass, pbf.rs) and the particle count. */

@@ -2457,6 +2457,45 @@ void orc_dem_params(void* hv, double* o) {
     std::memcpy(o, v, sizeof v);
   });
 }
+// FluidParticles::step_coupled (pbf.rs:1174-2365) on the handle's fluid alone, at drum angle
+// `angle`, with the handle's balls as they are (unchanged); iterations = pbf_iterations. Writes the
+// coupling impulses (imp: 2 per ball, ang: 1 per ball) and the FluidStepStats as 50 doubles:
+// visc_iters, density_iters, mean_shear, wall_work, wall_imp x, y, wall_ang, ke_noslip, ke_drag,
+// ke_buoy, ke_visc, ke_clamp, clamp_hits, n_groups, n_links, links_in_range, has_guess, 0, bins 32.
+void orc_fluid_step(void* hv, double angle, uint32_t iterations, double dt, double* imp, double* ang,
+                    double* stats) {
+  OrcHandle* h = (OrcHandle*)hv;
+  DISPATCH(h, {
+    using RR = std::remove_reference<decltype(s->drum_angle)>::type;
+    Coupling<RR> cp = s->fluid_step(s->drum(), (RR)angle, iterations, (RR)dt);
+    for (size_t i = 0; i < cp.imp.size(); i++) {
+      imp[2 * i] = (double)cp.imp[i].x;
+      imp[2 * i + 1] = (double)cp.imp[i].y;
+      ang[i] = (double)cp.ang[i];
+    }
+    const FluidStats& f = s->fs;
+    double v[18] = {(double)f.visc_iters, (double)f.density_iters, f.mean_shear, f.wall_work, f.wall_imp[0],
+                    f.wall_imp[1], f.wall_ang, f.ke_noslip, f.ke_drag, f.ke_buoy, f.ke_visc, f.ke_clamp,
+                    (double)f.clamp_hits, (double)f.n_groups, (double)f.n_links, (double)f.links_in_range,
+                    s->fl.guess.size() == s->fl.n() && s->fl.n() > 0 ? 1.0 : 0.0, 0.0};
+    std::memcpy(stats, v, sizeof v);
+    for (int b = 0; b < 32; b++) stats[18 + b] = f.bins[b];
+  });
+}
+// The f32 values the fluid step of this handle reads: h, rest density, particle mass, ball radius,
+// mass, inertia, true radius, viscosity, rheology, yield stress, wall no-slip, ball no-slip,
+// wettability, surface tension, pbf iterations (15 values).
+void orc_fluid_params(void* hv, double* o) {
+  OrcHandle* h = (OrcHandle*)hv;
+  DISPATCH(h, {
+    const Params& P = s->P;
+    double v[15] = {(double)s->fl.h, (double)s->fl.rest, (double)s->fl.mass, (double)s->bl.radius,
+                    (double)s->bl.mass, (double)s->bl.inertia, (double)s->bl.true_radius, P.viscosity,
+                    (double)P.rheology, P.yield_stress, P.wall_no_slip, P.ball_no_slip, P.wettability,
+                    P.surface_tension, (double)P.pbf_iterations};
+    std::memcpy(o, v, sizeof v);
+  });
+}
 void orc_fluid_bins(void* hv, double* o) {
   OrcHandle* h = (OrcHandle*)hv;
   DISPATCH(h, for (int b = 0; b < 32; b++) o[b] = s->fs.bins[b]);

@@ -70,6 +70,8 @@ def oracle_lib():
     lib.orc_splat_kernel.restype = C.c_float
     lib.orc_splat_kernel.argtypes = [C.c_float, C.c_float]
     lib.orc_dem_params.argtypes = [P, dp]
+    lib.orc_fluid_step.argtypes = [P, C.c_double, C.c_uint32, C.c_double, dp, dp, dp]
+    lib.orc_fluid_params.argtypes = [P, dp]
     lib.orc_stress_new.restype = P
     lib.orc_stress_new.argtypes = []
     lib.orc_stress_free.argtypes = [P]
@@ -286,6 +288,30 @@ class OracleSim:
             ei, ea = _fp(self._ext[0]), _fp(self._ext[1])
         self._lib.orc_dem_step(self._h, float(drum_angle), int(iterations), float(dt), ei, ea, _dp(st))
         return dict(zip(self.DEM_STATS_KEYS, st.tolist()))
+
+    FLUID_STATS_KEYS = ["viscosity_iterations", "density_iterations", "mean_shear_rate_per_s", "wall_work_j",
+                        "wall_impulse_x", "wall_impulse_y", "wall_angular_impulse", "ke_delta_noslip_j",
+                        "ke_delta_drag_j", "ke_delta_buoyancy_j", "ke_delta_viscosity_j", "ke_delta_clamp_j",
+                        "coupling_clamp_hits", "n_groups", "n_links", "links_in_range", "has_guess", "_pad"]
+    FLUID_PARAM_KEYS = ["h", "rest_density", "particle_mass", "ball_radius", "ball_mass", "ball_inertia",
+                        "ball_true_radius", "viscosity_pa_s", "rheology", "yield_stress_pa", "wall_no_slip",
+                        "ball_no_slip", "wettability", "surface_tension_n_m", "pbf_iterations"]
+
+    def fluid_step(self, drum_angle, iterations, dt):
+        """FluidParticles::step_coupled (pbf.rs:1174-2365) on this simulation's fluid with its balls
+        as they are (the balls are not moved); returns (impulse (n, 2), angular (n,), stats dict with
+        the 32 wall-angle bins under "wall_angle_bins")."""
+        imp, ang = np.zeros(2 * self.n_balls + 2), np.zeros(self.n_balls + 1)
+        st = np.zeros(len(self.FLUID_STATS_KEYS) + 32)
+        self._lib.orc_fluid_step(self._h, float(drum_angle), int(iterations), float(dt), _dp(imp), _dp(ang), _dp(st))
+        d = dict(zip(self.FLUID_STATS_KEYS, st[: len(self.FLUID_STATS_KEYS)].tolist()))
+        d["wall_angle_bins"] = st[len(self.FLUID_STATS_KEYS):].copy()
+        return imp[: 2 * self.n_balls].reshape(-1, 2), ang[: self.n_balls], d
+
+    def fluid_params(self) -> dict:
+        o = np.zeros(len(self.FLUID_PARAM_KEYS))
+        self._lib.orc_fluid_params(self._h, _dp(o))
+        return dict(zip(self.FLUID_PARAM_KEYS, o.tolist()))
 
     def dem_params(self) -> dict:
         o = np.zeros(len(self.DEM_PARAM_KEYS))
