@@ -24,13 +24,13 @@ extern "C" {
 #define MG_E_DEVICE -2   /* HIP runtime error or no GPU */
 #define MG_E_CAPACITY -3 /* a per-drum scratch capacity was exceeded (see mg_sim_error_bits) */
 #define MG_E_ARG -4      /* invalid argument (null handle, short buffer, bad index) */
-#define MG_E_DISABLED -5 /* the drum substep engine is switched off in this build (see mg_drum_engine_enabled) */
+#define MG_E_DISABLED -5 /* reserved: a switched-off engine (mg_drum_engine_enabled() == 0); not returned by this build */
 
 typedef struct mg_sim mg_sim;
 typedef struct mg_batch mg_batch;
 
-/* 1 if this build runs drum substeps on the GPU, 0 if the step entry points return
- * MG_E_DISABLED. No reference counterpart: a build-level switch of this library. */
+/* 1: this build runs drum substeps on the GPU (0 would mean the step entry points return
+ * MG_E_DISABLED). No reference counterpart: a build-level property of this library. */
 int mg_drum_engine_enabled(void);
 
 /* EnergyBudget (lib.rs:61-78), f64 throughout. */
@@ -102,28 +102,26 @@ int mg_sim_set_stress_debounce_factor(mg_sim* sim, double factor); /* lib.rs:244
 /* Whole-state snapshot (state + hidden CG warm start + EMAs + stress accumulator + energy budget). */
 int mg_sim_save_state(mg_sim* sim, void* buf, size_t cap, size_t* needed);
 int mg_sim_load_state(mg_sim* sim, const void* buf, size_t len);
-/* Debug exports of the last substep's step-2 fluid grid (grid.rs layout) and neighbour CSR
- * (pbf.rs:235-288). meta: dense, min_cx, min_cy, width, height, n_cell_start, n_indices. */
-int mg_sim_debug_grid(mg_sim* sim, int64_t* meta7, uint32_t* cell_start, size_t cs_cap, uint32_t* indices,
-                      size_t idx_cap);
-int mg_sim_debug_neighbors(mg_sim* sim, uint32_t* offsets, size_t off_cap, uint32_t* nbrs, size_t nb_cap,
-                           size_t* n_edges);
-/* Forces the density pass count / CG iteration count of following substeps (0 = adaptive). */
+/* Forces the density pass count and CG iteration count of following substeps, as the reference's
+ * tests do; in a batch the first forced drum's counts apply to every drum. */
 int mg_sim_force_iterations(mg_sim* sim, uint32_t density_passes, uint32_t cg_iterations);
 uint32_t mg_sim_error_bits(mg_sim* sim);
 
-/* ---- batched drums: one persistent launch per substep over all drums ---- */
+/* ---- batched drums: every drum of the batch advances in the same launches ----------------------
+ * A substep of a batch is one fluid step, one DEM step and one stress update for all drums at once;
+ * drum d's state follows drum d-1's in the batch's device arrays. */
 int mg_batch_create(const char* const* params_json, uint32_t n_drums, mg_batch** out, char* err, size_t errlen);
 void mg_batch_destroy(mg_batch* batch);
 uint32_t mg_batch_size(const mg_batch* batch);
-/* Advances every drum by `substeps` of its own fixed substep (step_fixed each). use_graph != 0
- * replays a captured hipGraph of the launches. */
+/* Advances every drum by `substeps` of its own fixed substep (step_fixed each); waits for the
+ * device. use_graph is accepted for compatibility and has no effect in this build. */
 int mg_batch_step(mg_batch* batch, uint32_t substeps, int use_graph);
 /* Borrowed per-drum view (valid until the batch is destroyed; do not destroy it). */
 mg_sim* mg_batch_drum(mg_batch* batch, uint32_t index);
-/* Waits for the batch's stream; returns MG_E_CAPACITY if any drum reported an overflow. */
+/* Waits for the device; returns the last mg_batch_step's status (MG_E_CAPACITY if a drum exceeded a
+ * workspace capacity) and its message. */
 int mg_batch_sync(mg_batch* batch, char* err, size_t errlen);
-/* Device time (ms) of the last mg_batch_step, measured with HIP events on the batch stream. */
+/* Host wall-clock time (ms) of the last mg_batch_step (it waits for the device). */
 float mg_batch_last_step_ms(const mg_batch* batch);
 
 const char* mg_version(void);
@@ -394,6 +392,9 @@ typedef struct mg_dem_drum {
   uint32_t iterations;
   float dt;
   uint32_t n_balls, has_external;
+  /* 0: the energies and the wall work are f32 sums in ball order, as the reference forms them;
+   * 1: the same terms summed in f64 (the precision form's f64 energy accumulators). */
+  uint32_t energy_f64;
 } mg_dem_drum;
 /* DemStepStats (dem.rs:175-222) plus the step's candidate-pair and impact-record counts and the
  * number of balls the swept time-of-impact test clamped. */

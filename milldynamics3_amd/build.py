@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmillgpu.so")
 ARCH = os.environ.get("MG_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["mg_stages.hip", "mg_box.hip", "mg_grid.hip", "mg_metrics.hip", "mg_stress.hip", "mg_dem.hip", "mg_surface.hip", "mg_fluid.hip", "mg_host.cpp", "mg_params.cpp"]
+SOURCES = ["mg_box.hip", "mg_grid.hip", "mg_metrics.hip", "mg_stress.hip", "mg_dem.hip", "mg_surface.hip", "mg_fluid.hip", "mg_host.cpp", "mg_params.cpp"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math"]
 HIP_ONLY = [f"--offload-arch={ARCH}", "-fhip-fp32-correctly-rounded-divide-sqrt",
             "-fno-gpu-flush-denormals-to-zero",

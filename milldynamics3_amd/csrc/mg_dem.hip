@@ -36,6 +36,7 @@
 #include "../../include/millgpu.h"
 #include "mg_drumgeo.h"
 #include "mg_expf.h"
+#include "mg_internal.h"
 
 #define DE_BLOCK 256
 #define DE_MAX_BALLS 2048
@@ -61,11 +62,13 @@ struct DeDrum {
   float dt;
   uint32_t b0, n;    // first ball in the caller's arrays, ball count
   uint32_t has_ext;
+  uint32_t f64;  // energies and wall work summed in f64 (the precision form) instead of f32
 };
 
 // Per population results of a step.
 struct DeOut {
-  float e_pred, maxd, maxs, ww, e_final;
+  double e_pred, ww, e_final;  // f32 sums (held exactly) or f64 sums, per DeDrum::f64
+  float maxd, maxs;
   double cw;
   uint32_t n_pairs, pair_overflow, n_ccd, coll, n_imp;
   uint32_t hist[12];
@@ -172,6 +175,7 @@ dem_predict(const DeDrum* __restrict__ drums, float2* __restrict__ x, float2* __
   const uint32_t t = threadIdx.x;
   const size_t sb = (size_t)blockIdx.x * stride;
   float e = 0.0f;   // lane 0: mechanical energy, three terms per ball in ball order
+  double ed = 0.0;  // the same in f64 (DeDrum::f64)
   double cw = 0.0;  // lane 0: coupling work in ball order
   float maxd = 0.0f, maxs = 0.0f;
   for (uint32_t base = 0; base < D.n; base += DE_BLOCK) {
@@ -217,7 +221,10 @@ dem_predict(const DeDrum* __restrict__ drums, float2* __restrict__ x, float2* __
     __syncthreads();
     if (t == 0) {
       const uint32_t m = min(D.n - base, (uint32_t)DE_BLOCK);
-      for (uint32_t q = 0; q < 3 * m; q++) e += s_e[q];
+      if (D.f64)
+        for (uint32_t q = 0; q < 3 * m; q++) ed += (double)s_e[q];
+      else
+        for (uint32_t q = 0; q < 3 * m; q++) e += s_e[q];
       if (D.has_ext)
         for (uint32_t q = 0; q < m; q++) cw += s_cw[q];
     }
@@ -236,7 +243,7 @@ dem_predict(const DeDrum* __restrict__ drums, float2* __restrict__ x, float2* __
     float ms = s_red[0];
     for (int q = 1; q < DE_BLOCK / 64; q++) ms = fmaxf(ms, s_red[q]);
     DeOut& o = out[blockIdx.x];
-    o.e_pred = e;
+    o.e_pred = D.f64 ? ed : (double)e;
     o.cw = cw;
     o.maxd = md;
     o.maxs = ms;
@@ -885,6 +892,7 @@ dem_friction(const DeDrum* __restrict__ drums, const float2* __restrict__ lift, 
   de_contact_sweep(pair, out[blockIdx.x], spk + pb, slam + pb, cpk + pb, clam + pb, coff + ob);
   // walls: each ball on its own; the wall-work terms are added in ball order by lane 0
   float ww = 0.0f;
+  double wwd = 0.0;
   const float wst_bw = w + r * r * wr;
   for (uint32_t base = 0; base < n; base += DE_BLOCK) {
     const uint32_t i = base + t;
@@ -921,7 +929,10 @@ dem_friction(const DeDrum* __restrict__ drums, const float2* __restrict__ lift, 
     __syncthreads();
     if (t == 0) {
       const uint32_t mm = min(n - base, (uint32_t)DE_BLOCK);
-      for (uint32_t q = 0; q < 2 * mm; q++) ww += s_ww[q];
+      if (D.f64)
+        for (uint32_t q = 0; q < 2 * mm; q++) wwd += (double)s_ww[q];
+      else
+        for (uint32_t q = 0; q < 2 * mm; q++) ww += s_ww[q];
     }
     __syncthreads();
   }
@@ -932,7 +943,7 @@ dem_friction(const DeDrum* __restrict__ drums, const float2* __restrict__ lift, 
     th[D.b0 + i] = s_th[i];
     om[D.b0 + i] = de_angle_diff(s_th[i], th0[sb + i]) / dt;
   }
-  if (t == 0) out[blockIdx.x].ww = ww;
+  if (t == 0) out[blockIdx.x].ww = D.f64 ? wwd : (double)ww;
 }
 
 // ---- 7. restitution and impact records (dem.rs:897-1034) ------------------------------------------
@@ -1112,17 +1123,21 @@ dem_finish(const DeDrum* __restrict__ drums, const float2* __restrict__ x, const
     __syncthreads();
   }
   float e = 0.0f;
+  double ed = 0.0;
   for (uint32_t base = 0; base < n; base += DE_BLOCK) {
     const uint32_t i = base + t;
     if (i < n) de_mech_terms(D, x[D.b0 + i], v[D.b0 + i], om[D.b0 + i], &s_e[3 * t]);
     __syncthreads();
     if (t == 0) {
       const uint32_t m = min(n - base, (uint32_t)DE_BLOCK);
-      for (uint32_t q = 0; q < 3 * m; q++) e += s_e[q];
+      if (D.f64)
+        for (uint32_t q = 0; q < 3 * m; q++) ed += (double)s_e[q];
+      else
+        for (uint32_t q = 0; q < 3 * m; q++) e += s_e[q];
     }
     __syncthreads();
   }
-  if (t == 0) out[blockIdx.x].e_final = e;
+  if (t == 0) out[blockIdx.x].e_final = D.f64 ? ed : (double)e;
 }
 
 // ---- host -----------------------------------------------------------------------------------------
@@ -1341,6 +1356,7 @@ int mg_dem_step(mg_dem* h, const mg_dem_drum* drums, float* d_x, float* d_v, flo
     // an empty population or a non-positive step leaves the state as it is (dem.rs:542-544)
     D.n = (m.n_balls == 0 || !(m.dt > 0.0f)) ? 0u : m.n_balls;
     D.has_ext = m.has_external ? 1u : 0u;
+    D.f64 = m.energy_f64 ? 1u : 0u;
     any_ext |= D.has_ext != 0;
     tb += m.n_balls;
   }
@@ -1436,10 +1452,11 @@ int mg_dem_step(mg_dem* h, const mg_dem_drum* drums, float* d_x, float* d_v, flo
     mg_dem_stats& s = stats[d];
     std::memset(&s, 0, sizeof s);
     if (hd[d].n == 0) continue;  // DemStepStats::default()
-    s.wall_work_j = o.ww;
+    s.wall_work_j = (float)o.ww;
     s.collision_count = o.coll;
     for (int b = 0; b < 12; b++) s.impact_energy_histogram[b] = o.hist[b];
-    s.dissipated_energy_j = o.e_pred + o.ww - o.e_final;
+    s.dissipated_energy_j = hd[d].f64 ? (float)(o.e_pred + o.ww - o.e_final)
+                                      : (float)o.e_pred + (float)o.ww - (float)o.e_final;
     s.max_substep_displacement_over_diameter = o.maxd;
     s.coupling_work_j = (float)o.cw;
     s.n_pairs = o.n_pairs;
@@ -1451,6 +1468,12 @@ int mg_dem_step(mg_dem* h, const mg_dem_drum* drums, float* d_x, float* d_v, flo
                    "the records beyond max_impacts are dropped)");
   }
   return rc;
+}
+
+const mg_impact_record* mg_dem_records_device(const mg_dem* h, uint32_t* stride) {
+  if (!h) return nullptr;
+  if (stride) *stride = h->rcap;
+  return h->rec;
 }
 
 int mg_dem_impacts(mg_dem* h, uint32_t drum, mg_impact_record* out, uint32_t cap, uint32_t* n) {

@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "../../include/millgpu.h"
+#include "mg_internal.h"
 
 #define ST_VALUES 49
 #define ST_LANES 64
@@ -47,7 +48,8 @@ __device__ __forceinline__ uint32_t st_slot(uint64_t key, uint32_t bits) {
 
 // One lane per drum. acc[d * 49 ..]: count 4, se 4, hist 28, above 6, cum_wall 1, cum_above 6.
 extern "C" __global__ void __launch_bounds__(ST_LANES)
-st_update(const mg_impact_record* __restrict__ rec, const uint32_t* __restrict__ off, const StDrum* __restrict__ par,
+st_update(const mg_impact_record* __restrict__ rec, const uint32_t* __restrict__ beg,
+          const uint32_t* __restrict__ end, const StDrum* __restrict__ par,
           const double* __restrict__ thr_bin, uint32_t n_drums, double* __restrict__ acc, uint64_t* __restrict__ keys,
           double* __restrict__ times, uint32_t cap_bits, uint32_t* __restrict__ err) {
   __shared__ double s_hist[ST_LANES][ST_BINS];
@@ -63,7 +65,7 @@ st_update(const mg_impact_record* __restrict__ rec, const uint32_t* __restrict__
   double* T = times + (size_t)d * cap;
   double above[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   uint32_t full = 0;
-  for (uint32_t r = off[d]; r < off[d + 1]; r++) {
+  for (uint32_t r = beg[d]; r < end[d]; r++) {
     const mg_impact_record q = rec[r];
     if (p.window > 0.0) {
       const uint32_t a = q.partner == 0xFFFFFFFFu ? q.ball : min(q.ball, q.partner);
@@ -231,6 +233,12 @@ int mg_stress_set_debounce_factor(mg_stress* h, uint32_t drum, double factor) {
 int mg_stress_update(mg_stress* h, const mg_impact_record* d_records, const uint32_t* d_offsets,
                      const mg_stress_step* steps, char* err, size_t errlen) {
   if (!h || !d_offsets || !steps) return set_msg(h, err, errlen, MG_E_ARG, "null argument");
+  return mg_stress_update_ranges(h, d_records, d_offsets, d_offsets + 1, steps, err, errlen);
+}
+
+int mg_stress_update_ranges(mg_stress* h, const mg_impact_record* d_records, const uint32_t* d_beg,
+                            const uint32_t* d_end, const mg_stress_step* steps, char* err, size_t errlen) {
+  if (!h || !d_beg || !d_end || !steps) return set_msg(h, err, errlen, MG_E_ARG, "null argument");
   std::vector<StDrum> par(h->n);
   for (uint32_t d = 0; d < h->n; d++) {
     const mg_stress_step& s = steps[d];
@@ -241,8 +249,8 @@ int mg_stress_update(mg_stress* h, const mg_impact_record* d_records, const uint
   }
   if (hipMemcpy(h->par, par.data(), par.size() * sizeof(StDrum), hipMemcpyHostToDevice) != hipSuccess)
     return set_msg(h, err, errlen, MG_E_DEVICE, "HIP copy error");
-  hipLaunchKernelGGL(st_update, dim3((h->n + ST_LANES - 1) / ST_LANES), dim3(ST_LANES), 0, nullptr, d_records, d_offsets,
-                     h->par, h->thr, h->n, h->acc, h->keys, h->times, h->cap_bits, h->err);
+  hipLaunchKernelGGL(st_update, dim3((h->n + ST_LANES - 1) / ST_LANES), dim3(ST_LANES), 0, nullptr, d_records, d_beg,
+                     d_end, h->par, h->thr, h->n, h->acc, h->keys, h->times, h->cap_bits, h->err);
   std::vector<uint32_t> e(h->n);
   if (hipGetLastError() != hipSuccess ||
       hipMemcpy(e.data(), h->err, e.size() * 4, hipMemcpyDeviceToHost) != hipSuccess)
@@ -256,6 +264,50 @@ int mg_stress_update(mg_stress* h, const mg_impact_record* d_records, const uint
       return set_msg(h, err, errlen, MG_E_CAPACITY,
                      "drum " + std::to_string(d) + ": debounce table full (create with a larger table_capacity)");
   return MG_OK;
+}
+
+size_t mg_stress_blob_bytes(const mg_stress* h) {
+  return h ? 8 * ST_VALUES + 16 * ((size_t)1 << h->cap_bits) + 24 : 0;
+}
+
+int mg_stress_save(mg_stress* h, uint32_t drum, void* out) {
+  if (!h || drum >= h->n || !out) return MG_E_ARG;
+  const size_t cap = (size_t)1 << h->cap_bits;
+  char* p = static_cast<char*>(out);
+  bool ok = hipMemcpy(p, h->acc + (size_t)drum * ST_VALUES, 8 * ST_VALUES, hipMemcpyDeviceToHost) == hipSuccess &&
+            hipMemcpy(p + 8 * ST_VALUES, h->keys + drum * cap, 8 * cap, hipMemcpyDeviceToHost) == hipSuccess &&
+            hipMemcpy(p + 8 * ST_VALUES + 8 * cap, h->times + drum * cap, 8 * cap, hipMemcpyDeviceToHost) == hipSuccess;
+  p += 8 * ST_VALUES + 16 * cap;
+  const double meta[3] = {h->factor[drum], h->last_now[drum], (double)h->started[drum]};
+  std::memcpy(p, meta, 24);
+  return ok ? MG_OK : MG_E_DEVICE;
+}
+
+int mg_stress_load(mg_stress* h, uint32_t drum, const void* in) {
+  if (!h || drum >= h->n || !in) return MG_E_ARG;
+  const size_t cap = (size_t)1 << h->cap_bits;
+  const char* p = static_cast<const char*>(in);
+  bool ok = hipMemcpy(h->acc + (size_t)drum * ST_VALUES, p, 8 * ST_VALUES, hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(h->keys + drum * cap, p + 8 * ST_VALUES, 8 * cap, hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(h->times + drum * cap, p + 8 * ST_VALUES + 8 * cap, 8 * cap, hipMemcpyHostToDevice) == hipSuccess;
+  p += 8 * ST_VALUES + 16 * cap;
+  double meta[3];
+  std::memcpy(meta, p, 24);
+  h->factor[drum] = meta[0];
+  h->last_now[drum] = meta[1];
+  h->started[drum] = meta[2] != 0.0 ? 1 : 0;
+  return ok ? MG_OK : MG_E_DEVICE;
+}
+
+int mg_stress_reset(mg_stress* h, uint32_t drum) {
+  if (!h || drum >= h->n) return MG_E_ARG;
+  const size_t cap = (size_t)1 << h->cap_bits;
+  h->last_now[drum] = 0.0;
+  h->started[drum] = 0;
+  return hipMemset(h->acc + (size_t)drum * ST_VALUES, 0, 8 * ST_VALUES) == hipSuccess &&
+                 hipMemset(h->keys + drum * cap, 0xFF, 8 * cap) == hipSuccess
+             ? MG_OK
+             : MG_E_DEVICE;
 }
 
 int mg_stress_state(mg_stress* h, uint32_t drum, double* out49) {

@@ -28,7 +28,8 @@ class _Drum(C.Structure):
                  ("lifter_top_width_m", C.c_float), ("lifter_phase_deg", C.c_float), ("ball_radius", C.c_float),
                  ("ball_mass", C.c_float), ("ball_inertia", C.c_float)] +
                 [(k, C.c_float) for k in MEDIA_KEYS] +
-                [("iterations", C.c_uint32), ("dt", C.c_float), ("n_balls", C.c_uint32), ("has_external", C.c_uint32)])
+                [("iterations", C.c_uint32), ("dt", C.c_float), ("n_balls", C.c_uint32), ("has_external", C.c_uint32),
+                 ("energy_f64", C.c_uint32)])
 
 
 class _Stats(C.Structure):
@@ -93,6 +94,7 @@ class DemBatch:
             for key in MEDIA_KEYS:
                 setattr(r, key, float(p[key]))
             r.iterations, r.dt = int(p["iterations"]), float(p["dt"])
+            r.energy_f64 = 1 if p.get("energy_f64") else 0
             x = np.asarray(p["x"], np.float32).reshape(-1, 2)
             r.n_balls = len(x)
             xs.append(x)

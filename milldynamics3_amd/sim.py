@@ -112,8 +112,6 @@ def lib():
     L.mg_sim_set_stress_debounce_factor.argtypes = [P, C.c_double]
     L.mg_sim_save_state.argtypes = [P, C.c_void_p, sz, C.POINTER(sz)]
     L.mg_sim_load_state.argtypes = [P, C.c_void_p, sz]
-    L.mg_sim_debug_grid.argtypes = [P, C.POINTER(C.c_int64), u32p, sz, u32p, sz]
-    L.mg_sim_debug_neighbors.argtypes = [P, u32p, sz, u32p, sz, C.POINTER(sz)]
     L.mg_sim_force_iterations.argtypes = [P, C.c_uint32, C.c_uint32]
     L.mg_batch_create.argtypes = [C.POINTER(ch), C.c_uint32, C.POINTER(P), ch, sz]
     L.mg_batch_destroy.argtypes = [P]
@@ -357,26 +355,6 @@ class Simulation:
     # -- parity/debug ---------------------------------------------------------------------------
     def force_iterations(self, density_passes: int = 0, cg_iterations: int = 0):
         self._L.mg_sim_force_iterations(self._h, density_passes, cg_iterations)
-
-    def debug_grid(self) -> dict:
-        meta = np.zeros(7, np.int64)
-        cs = np.zeros(2 * self.n_fluid + 1024, np.uint32)
-        idx = np.zeros(max(self.n_fluid, 1), np.uint32)
-        u32 = lambda a: a.ctypes.data_as(C.POINTER(C.c_uint32))
-        self._L.mg_sim_debug_grid(self._h, meta.ctypes.data_as(C.POINTER(C.c_int64)), u32(cs), cs.size,
-                                  u32(idx), idx.size)
-        return {"dense": bool(meta[0]), "min_cx": int(meta[1]), "min_cy": int(meta[2]),
-                "width": int(meta[3]), "height": int(meta[4]), "cell_start": cs[: meta[5]].copy(),
-                "indices": idx[: self.n_fluid].copy()}
-
-    def debug_neighbors(self):
-        off = np.zeros(self.n_fluid + 1, np.uint32)
-        ne = C.c_size_t(0)
-        u32 = lambda a: a.ctypes.data_as(C.POINTER(C.c_uint32))
-        self._L.mg_sim_debug_neighbors(self._h, u32(off), off.size, None, 0, C.byref(ne))
-        nb = np.zeros(max(ne.value, 1), np.uint32)
-        self._L.mg_sim_debug_neighbors(self._h, u32(off), off.size, u32(nb), nb.size, C.byref(ne))
-        return off, nb[: ne.value].copy()
 
 
 class Batch:

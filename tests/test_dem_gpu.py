@@ -26,8 +26,8 @@ def dm():
     return dem
 
 
-def _pop(orc, params, initial=None, ext_seed=None, state_edit=None):
-    o = orc.OracleSim(dict(params, slurry={"enabled": False}), "ref", initial_balls=initial)
+def _pop(orc, params, initial=None, ext_seed=None, state_edit=None, mode="ref"):
+    o = orc.OracleSim(dict(params, slurry={"enabled": False}), mode, initial_balls=initial)
     if state_edit is not None:
         st = o.state()
         state_edit(st)
@@ -41,7 +41,8 @@ def _pop(orc, params, initial=None, ext_seed=None, state_edit=None):
            "ball_radius": pr["ball_radius"], "ball_mass": pr["ball_mass"], "ball_inertia": pr["ball_inertia"],
            "iterations": p["simulation"]["dem_iterations"], "dt": o.fixed_sub_dt,
            "x": st["ball_x"].astype(np.float32), "v": st["ball_v"].astype(np.float32),
-           "theta": st["ball_theta"].astype(np.float32), "omega_b": st["ball_omega"].astype(np.float32)}
+           "theta": st["ball_theta"].astype(np.float32), "omega_b": st["ball_omega"].astype(np.float32),
+           "energy_f64": mode != "ref"}
     for k in ("restitution_ball_ball", "restitution_ball_wall", "friction_ball_ball", "friction_ball_wall",
               "friction_ball_ball_static", "friction_ball_wall_static", "friction_velocity_scale_m_s",
               "rolling_friction"):
@@ -156,3 +157,16 @@ def test_sparse_layout_of_a_few_spread_balls(dm, orc):
     o, p, _ = _pop(orc, {"mill": {"diameter_m": 2.0, "speed_value": 5.0}, "media": {"ball_diameter_m": 0.002},
                          "simulation": {"coarse_graining_mode": "off"}}, initial=xy)
     _run(dm, [("sparse", o, p, None)], 8)
+
+
+def test_f64_energy_sums_match_the_mixed_policy(dm, orc):
+    """energy_f64 = 1: the energies and wall work are the same f32 terms summed in f64 in ball order
+    (the oracle's "mixed" policy, which the drum engine runs); the state stays bit-exact and so do the
+    f32 roundings of the f64 sums (wall work, dissipated energy)."""
+    cases = []
+    o, p, rng = _pop(orc, {"simulation": {"max_balls": 300, "resolution": 25}}, mode="mixed")
+    cases.append(("realtime", o, p, rng))
+    o, p, rng = _pop(orc, {"mill": {"direction": "clockwise", "speed_value": 180.0}, "lifters": LIFTERS,
+                           "simulation": {"max_balls": 400}}, ext_seed=5, mode="mixed")
+    cases.append(("lifters, impulses", o, p, rng))
+    _run(dm, cases, 8)
