@@ -439,6 +439,10 @@ struct FlPtrs {
   float4* gc;          // c, lever_c.x, lever_c.y, lever2_c
   uint32_t *fgoff, *fgids;
   // squeeze links
+  // form drag and buoyancy items per ball (GS_ITEMS slots each) and their velocity-free sums
+  uint32_t *gdj, *gdn, *gbj, *gbn;
+  float4 *gdr, *gds, *gbr;
+  float2 *gdb, *gbs;
   uint32_t *lcnt;
   float4* lrec;        // n.x, n.y, c, other (bits)
   uint32_t *locnt, *looff, *locur, *loids;
@@ -1103,253 +1107,410 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_grid_b(FlPtrs P) {
   if (D.grp_on)
     wg_grid_build(P.x + D.f0, D.n, D.h, D.ccap[3], P.gstart[3] + D.c0[3], P.gcur[3] + D.c0[3], P.gidx[3] + D.f0,
                   G + 3, &S.err, si, su);
-  if (D.links_on)
+  if (D.links_on || D.drag_on || D.buoy_on)
     wg_grid_build(P.bx + D.b0, D.nb, D.g5c, D.ccap[4], P.gstart[4] + D.c0[4], P.gcur[4] + D.c0[4], P.gidx[4] + D.b0,
                   G + 4, &S.err, si, su);
 }
 
 // 6.5 form drag (pbf.rs:1777-2013) then 6.6 buoyancy (pbf.rs:2035-2122): Gauss-Seidel over the
-// balls in index order (each ball's kicks change the fluid velocities the next ball reads), one
-// wave per drum; candidates evaluated 64 at a time, the sums walked in the reference's order.
-extern "C" __global__ void __launch_bounds__(64) fl_gs(FlPtrs P) {
-  __shared__ uint32_t s_j[FL_ITEMS];
-  __shared__ float s_rx[FL_ITEMS], s_ry[FL_ITEMS], s_ph[FL_ITEMS];
-  const uint32_t d = blockIdx.x, lane = threadIdx.x;
-  const FlDrum& D = P.drums[d];
-  FlState& S = P.st[d];
-  if (D.n == 0) return;
+// balls in index order — each ball reads the fluid velocities the balls before it wrote (seeded
+// balls are numbered row by row, so neighbouring balls in index order nearly always share fluid
+// particles and the chain of dependent balls is as long as the population). Everything a ball
+// needs that does not depend on the fluid velocities is computed for all balls at once first
+// (fl_gs_pre, one wave per ball): its items in candidate order (drag: particles within h + r, with
+// the kernel weight; buoyancy: particles within h, with their weight and position), and the
+// order-sensitive f32 sums over them that do not read velocities (entrained mass, first moment,
+// inertia and the rotation sums s, s_scale for drag; local density and the weight of the
+// off-axis items for buoyancy). The sweep (fl_gs, one wave per drum, the drum's fluid velocities
+// held in LDS when they fit) then walks the balls in order: gather the items' velocities, the
+// three velocity sums in item order (one per lane), the ball's scalar update, the kicks.
+#define GS_ITEMS 224
+
+// Wave-local memory ordering for LDS written by some lanes and read by others of the same wave.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Per-ball items and velocity-independent sums, one wave per ball.
+extern "C" __global__ void __launch_bounds__(FL_T) fl_gs_pre(FlPtrs P, const uint2* __restrict__ btiles) {
+  __shared__ float s_a[4][GS_ITEMS], s_b[4][GS_ITEMS], s_c[4][GS_ITEMS], s_d[4][GS_ITEMS];
+  const uint2 tl = btiles[blockIdx.x];
+  const FlDrum& D = P.drums[tl.x];
+  const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6, b = tl.y + wv;
+  if (b >= D.nb) return;
+  const size_t gb = (size_t)D.b0 + b;
+  const float2 c = P.bx[gb];
   const float2* x = P.x + D.f0;
-  float2* v = P.v + D.f0;
-  const float2* bx = P.bx + D.b0;
-  float2* imp = P.imp + D.b0;
-  float* ang = P.ang + D.b0;
+  float *sa = s_a[wv], *sb = s_b[wv], *sc = s_c[wv], *sd = s_d[wv];
   bool over = false;
-  if (D.drag_on) {
-    const GView g = gview(P.grids + 5 * (size_t)d + 2, P.gstart[2] + D.c0[2], P.gidx[2] + D.f0);
-    for (uint32_t b = 0; b < D.nb; b++) {
-      const float2 c = bx[b];
-      if (!fin2(c)) continue;
-      Near q;
-      near_cells(g, c, q);
-      if (q.total == 0) continue;
-      float m_ent = 0.0f, i_ent = 0.0f, l_sum = 0.0f, rsx = 0.0f, rsy = 0.0f, vsx = 0.0f, vsy = 0.0f;
-      uint32_t nit = 0;
-      for (uint32_t c0 = 0; c0 < q.total; c0 += 64) {
-        const uint32_t cc = c0 + lane;
-        bool valid = false;
-        uint32_t j = 0;
-        float rix = 0.0f, riy = 0.0f, r2 = 0.0f, phi = 0.0f, w = 0.0f, vjx = 0.0f, vjy = 0.0f;
-        if (cc < q.total) {
-          j = near_at(g, q, cc);
-          const float2 xj = x[j];
-          rix = xj.x - c.x;
-          riy = xj.y - c.y;
-          r2 = rix * rix + riy * riy;
-          if (__builtin_isfinite(r2) && !(r2 >= D.hc2)) {
-            phi = poly6(r2, D.hc2, D.p6hc) / D.w0;
-            if (phi > 0.0f) {
-              valid = true;
-              w = D.mass * phi;
-              const float2 vj = v[j];
-              vjx = vj.x;
-              vjy = vj.y;
-            }
-          }
-        }
-        uint64_t m = __ballot(valid);
-        while (m) {
-          const int L = first_lane(m);
-          m &= m - 1;
-          const float ww = rdl(w, L), ax = rdl(rix, L), ay = rdl(riy, L), ux = rdl(vjx, L), uy = rdl(vjy, L);
-          m_ent += ww;
-          rsx += ww * ax;
-          rsy += ww * ay;
-          vsx += ww * ux;
-          vsy += ww * uy;
-          i_ent += ww * rdl(r2, L);
-          l_sum += ww * (ax * uy - ay * ux);
-          if (nit < FL_ITEMS) {
-            if (lane == 0) {
-              s_j[nit] = rdlu(j, L);
-              s_rx[nit] = ax;
-              s_ry[nit] = ay;
-              s_ph[nit] = rdl(phi, L);
-            }
-          } else {
-            over = true;
-          }
-          nit++;
+  uint32_t nit = 0;
+  if (D.drag_on && fin2(c)) {
+    const GView g = gview(P.grids + 5 * (size_t)tl.x + 2, P.gstart[2] + D.c0[2], P.gidx[2] + D.f0);
+    Near q;
+    near_cells(g, c, q);
+    for (uint32_t c0 = 0; c0 < q.total; c0 += 64) {
+      const uint32_t cc = c0 + lane;
+      bool valid = false;
+      uint32_t j = 0;
+      float rix = 0.0f, riy = 0.0f, phi = 0.0f;
+      if (cc < q.total) {
+        j = near_at(g, q, cc);
+        const float2 xj = x[j];
+        rix = xj.x - c.x;
+        riy = xj.y - c.y;
+        const float r2 = rix * rix + riy * riy;
+        if (__builtin_isfinite(r2) && !(r2 >= D.hc2)) {
+          phi = poly6(r2, D.hc2, D.p6hc) / D.w0;
+          valid = phi > 0.0f;
         }
       }
-      __syncthreads();
-      nit = min(nit, (uint32_t)FL_ITEMS);
-      if (m_ent <= 1e-12f || nit == 0) continue;
-      const float rbx = rsx / m_ent, rby = rsy / m_ent, vbx = vsx / m_ent, vby = vsy / m_ent;
-      const float2 bvb = P.bv[D.b0 + b];
-      const float vrx = vbx - bvb.x, vry = vby - bvb.y;
+      const uint64_t m = __ballot(valid);
+      const uint32_t pos = nit + (uint32_t)__builtin_popcountll(m & ((1ull << lane) - 1ull));
+      if (valid && pos < GS_ITEMS) {
+        P.gdj[gb * GS_ITEMS + pos] = j;
+        P.gdr[gb * GS_ITEMS + pos] = make_float4(rix, riy, phi, D.mass * phi);
+        sa[pos] = rix;
+        sb[pos] = riy;
+        sc[pos] = phi;
+      }
+      nit += (uint32_t)__builtin_popcountll(m);
+    }
+    if (nit > GS_ITEMS) {
+      over = true;
+      nit = GS_ITEMS;
+    }
+    wave_sync();
+    float acc = 0.0f;  // entrained mass, first moment x / y, inertia: one sum per lane, item order
+    if (lane < 4) {
+      for (uint32_t i = 0; i < nit; i++) {
+        const float w = D.mass * sc[i], ax = sa[i], ay = sb[i];
+        acc += lane == 0 ? w : lane == 1 ? w * ax : lane == 2 ? w * ay : w * (ax * ax + ay * ay);
+      }
+    }
+    const float m_ent = rdl(acc, 0), rsx = rdl(acc, 1), rsy = rdl(acc, 2), i_ent = rdl(acc, 3);
+    float s = 0.0f, ss = 0.0f, rbx = 0.0f, rby = 0.0f;
+    if (m_ent > 1e-12f && nit > 0) {
+      rbx = rsx / m_ent;
+      rby = rsy / m_ent;
+      const float pbx = -rby, pby = rbx;
+      float s2 = 0.0f;
+      if (lane < 2) {
+        for (uint32_t i = 0; i < nit; i++) {
+          const float ax = sa[i], ay = sb[i], p = sc[i];
+          const float phi2 = p * p;
+          s2 += lane == 0 ? D.mass * phi2 * (ax * (ax - pby) - ay * (-ay - pbx)) : D.mass * phi2 * (ax * ax + ay * ay);
+        }
+      }
+      s = rdl(s2, 0);
+      ss = rdl(s2, 1);
+    }
+    if (lane == 0) {
+      P.gds[gb] = make_float4(m_ent, i_ent, s, ss);
+      P.gdb[gb] = make_float2(rbx, rby);
+    }
+    wave_sync();
+  }
+  if (lane == 0) P.gdn[gb] = nit;
+  uint32_t nw = 0;
+  if (D.buoy_on && fin2(c)) {
+    const GView g = gview(P.grids + 5 * (size_t)tl.x, P.gstart[0] + D.c0[0], P.gidx[0] + D.f0);
+    Near q;
+    near_cells(g, c, q);
+    for (uint32_t c0 = 0; c0 < q.total; c0 += 64) {
+      const uint32_t cc = c0 + lane;
+      uint32_t j = 0;
+      float w = 0.0f;
+      float2 xj = make_float2(0.0f, 0.0f);
+      if (cc < q.total) {
+        j = near_at(g, q, cc);
+        xj = x[j];
+        const float dx = c.x - xj.x, dy = c.y - xj.y;
+        w = D.mass * poly6(dx * dx + dy * dy, D.h2, D.p6c);
+      }
+      const bool valid = w > 0.0f;
+      const uint64_t m = __ballot(valid);
+      const uint32_t pos = nw + (uint32_t)__builtin_popcountll(m & ((1ull << lane) - 1ull));
+      if (valid && pos < GS_ITEMS) {
+        const float d2 = xj.x * xj.x + xj.y * xj.y;
+        const float use = (d2 <= 1e-12f) ? 0.0f : 1.0f;
+        P.gbj[gb * GS_ITEMS + pos] = j;
+        P.gbr[gb * GS_ITEMS + pos] = make_float4(w, xj.x, xj.y, use);
+        sa[pos] = w;
+        sd[pos] = use;
+      }
+      nw += (uint32_t)__builtin_popcountll(m);
+    }
+    if (nw > GS_ITEMS) {
+      over = true;
+      nw = GS_ITEMS;
+    }
+    wave_sync();
+    float acc = 0.0f;  // local density; weight of the items off the axis
+    if (lane < 2)
+      for (uint32_t i = 0; i < nw; i++)
+        if (lane == 0 || sd[i] != 0.0f) acc += sa[i];
+    const float od = rdl(acc, 1);
+    if (lane == 0) P.gbs[gb] = make_float2(acc, od);
+  }
+  if (lane == 0) P.gbn[gb] = nw;
+  if (over && lane == 0) atomicOr(&P.st[tl.x].err, FE_ITEMS);
+}
+
+// acc + v[0] + v[1] + ... + v[n-1], added in that order (eight LDS values per two 16-byte reads).
+__device__ __forceinline__ float ordered_sum(const float* __restrict__ v, uint32_t n, float acc) {
+  const float4* v4 = reinterpret_cast<const float4*>(v);
+  uint32_t i = 0;
+  for (; i + 8 <= n; i += 8) {
+    const float4 a = v4[i >> 2], b = v4[(i >> 2) + 1];
+    acc += a.x;
+    acc += a.y;
+    acc += a.z;
+    acc += a.w;
+    acc += b.x;
+    acc += b.y;
+    acc += b.z;
+    acc += b.w;
+  }
+  for (; i < n; i++) acc += v[i];
+  return acc;
+}
+
+// The sweep over the balls in index order (one wave per drum); V is the drum's velocity array
+// (LDS when it fits, else global memory). A ball's items and its velocity-free sums are loaded one
+// ball ahead (they do not depend on the ball before it), so only the velocity gathers, the three
+// velocity sums and the scalar update stay on the sweep's critical path.
+struct GsBall {
+  uint32_t n;
+  float4 st;  // drag: m_ent, i_ent, s, s_scale; buoyancy: rho_l, od, -, -
+  float2 rb, bv, imp;
+  float bom, ang;
+  uint32_t ij[4];
+  float4 ir[4];
+};
+__device__ inline void gs_load(const FlPtrs& P, uint32_t b0, uint32_t nb, uint32_t b, bool drag, GsBall& B) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const size_t gb = (size_t)b0 + b;
+  const size_t gs = b < nb ? gb : (size_t)b0;  // past the last ball: any valid address, count 0
+  const uint32_t n = drag ? P.gdn[gs] : P.gbn[gs];
+  B.n = b < nb ? n : 0u;
+  if (drag) {
+    B.st = P.gds[gs];
+    B.rb = P.gdb[gs];
+    B.bom = P.bom[gs];
+    B.ang = P.ang[gs];
+  } else {
+    const float2 q = P.gbs[gs];
+    B.st = make_float4(q.x, q.y, 0.0f, 0.0f);
+    B.rb = P.bx[gs];
+  }
+  B.bv = P.bv[gs];
+  B.imp = P.imp[gs];
+  // every slot of the ball's item block is loaded (the block is always allocated), so the loads do
+  // not wait for the count; slots past it are never used
+  const size_t gq = b < nb ? gb : (size_t)b0;
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    const uint32_t k = (uint32_t)s * 64u + lane;
+    B.ij[s] = 0;
+    B.ir[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (k < GS_ITEMS) {
+      B.ij[s] = drag ? P.gdj[gq * GS_ITEMS + k] : P.gbj[gq * GS_ITEMS + k];
+      B.ir[s] = drag ? P.gdr[gq * GS_ITEMS + k] : P.gbr[gq * GS_ITEMS + k];
+    }
+  }
+}
+
+template <bool LDSV>
+__device__ inline void gs_sweep(const FlPtrs& P, const FlDrum& D, float2* V, float* s_t0, float* s_t1, float* s_t2) {
+  const uint32_t lane = threadIdx.x & 63u;
+  float2* imp = P.imp + D.b0;
+  float* ang = P.ang + D.b0;
+  // the drum's constants in registers: the loop's stores would otherwise make the compiler reload
+  // them, and every reload waits for the prefetched loads
+  const float ib = D.binertia, rest = D.rest, btrue = D.btrue, mass_true = D.mass_true, beta_b = D.beta_b;
+  const float dt = D.dt, bmass = D.bmass, hc = D.hc, rot_cap = D.rot_cap;
+  const uint32_t nb = D.nb, b0 = D.b0;
+  GsBall cur, nxt;
+  gs_load(P, b0, nb, 0, true, cur);
+  for (uint32_t b = 0; b < nb; b++) {
+    gs_load(P, b0, nb, b + 1, true, nxt);
+    const uint32_t nit = cur.n;
+    const float m_ent = cur.st.x, i_ent = cur.st.y;
+    if (nit != 0 && m_ent > 1e-12f) {
+#pragma unroll
+      for (int s = 0; s < 4; s++) {
+        const uint32_t k = (uint32_t)s * 64u + lane;
+        if (k < nit) {
+          const float2 vj = V[cur.ij[s]];
+          const float w = cur.ir[s].w;
+          s_t0[k] = w * vj.x;
+          s_t1[k] = w * vj.y;
+          s_t2[k] = w * (cur.ir[s].x * vj.y - cur.ir[s].y * vj.x);
+        }
+      }
+      wave_sync();
+      float acc = 0.0f;
+      if (lane < 3) acc = ordered_sum(lane == 0 ? s_t0 : lane == 1 ? s_t1 : s_t2, nit, 0.0f);
+      const float vsx = rdl(acc, 0), vsy = rdl(acc, 1), l_sum = rdl(acc, 2);
+      const float rbx = cur.rb.x, rby = cur.rb.y, vbx = vsx / m_ent, vby = vsy / m_ent;
+      const float vrx = vbx - cur.bv.x, vry = vby - cur.bv.y;
       const float sp = sqrtf(vrx * vrx + vry * vry);
-      const float itf =
-          (sp > 1e-6f && D.rest > 0.0f) ? 0.5f * D.rest * 1.0f * (2.0f * D.btrue) * sp / D.mass_true : 0.0f;
-      const float bt = D.beta_b * (1.0f - mgm::ref_expf(-D.dt * (0.0f + itf)));
-      const float a_lin = bt * m_ent / (D.bmass + m_ent);
+      const float itf = (sp > 1e-6f && rest > 0.0f) ? 0.5f * rest * 1.0f * (2.0f * btrue) * sp / mass_true : 0.0f;
+      const float bt = beta_b * (1.0f - mgm::ref_expf(-dt * (0.0f + itf)));
+      const float a_lin = bt * m_ent / (bmass + m_ent);
       const float dvbx = a_lin * vrx, dvby = a_lin * vry;
       float dwb = 0.0f, c_rot = 0.0f, pbx = 0.0f, pby = 0.0f;
-      const float ib = D.binertia;
       if (i_ent > 1e-12f) {
         const float om_bar = l_sum / i_ent;
         const float a_rot = bt * i_ent / (ib + i_ent);
-        const float cand = a_rot * (om_bar - P.bom[D.b0 + b]);
+        const float cand = a_rot * (om_bar - cur.bom);
         pbx = -rby;
         pby = rbx;
-        float s = 0.0f, ss = 0.0f;
-        for (uint32_t c0 = 0; c0 < nit; c0 += 64) {
-          const uint32_t k = c0 + lane;
-          float ts = 0.0f, tss = 0.0f;
-          if (k < nit) {
-            const float ax = s_rx[k], ay = s_ry[k], ph = s_ph[k];
-            const float phi2 = ph * ph;
-            ts = D.mass * phi2 * (ax * (ax - pby) - ay * (-ay - pbx));
-            tss = D.mass * phi2 * (ax * ax + ay * ay);
-          }
-          const uint32_t lim = min(64u, nit - c0);
-          for (uint32_t L = 0; L < lim; L++) {
-            s += rdl(ts, (int)L);
-            ss += rdl(tss, (int)L);
-          }
-        }
+        const float s = cur.st.z, ss = cur.st.w;
         if (fabsf(s) > 0.1f * ss) {
           const float raw = -ib * cand / s;
-          const float ml = 2.0f * D.hc;
-          const float cap = ml > 1e-9f ? D.rot_cap / ml : 0.0f;
+          const float ml = 2.0f * hc;
+          const float cap = ml > 1e-9f ? rot_cap / ml : 0.0f;
           c_rot = fclampf(raw, -cap, cap);
           dwb = -c_rot * s / ib;
         }
       }
-      const float f = -(D.bmass / m_ent);
-      for (uint32_t k = lane; k < nit; k += 64) {
-        const uint32_t j = s_j[k];
-        const float ph = s_ph[k];
-        float dx = f * dvbx * ph, dy = f * dvby * ph;
-        if (c_rot != 0.0f) {
-          const float ax = s_rx[k], ay = s_ry[k];
-          const float cp = c_rot * ph;
-          dx += cp * (-ay - pbx);
-          dy += cp * (ax - pby);
+      const float f = -(bmass / m_ent);
+#pragma unroll
+      for (int s = 0; s < 4; s++) {
+        const uint32_t k = (uint32_t)s * 64u + lane;
+        if (k < nit) {
+          const float p = cur.ir[s].z;
+          float dx = f * dvbx * p, dy = f * dvby * p;
+          if (c_rot != 0.0f) {
+            const float cp = c_rot * p;
+            dx += cp * (-cur.ir[s].y - pbx);
+            dy += cp * (cur.ir[s].x - pby);
+          }
+          const float2 vj = V[cur.ij[s]];
+          V[cur.ij[s]] = make_float2(vj.x + dx, vj.y + dy);
         }
-        const float2 vj = v[j];
-        v[j] = make_float2(vj.x + dx, vj.y + dy);
       }
       if (lane == 0) {
-        const float2 i0 = imp[b];
-        imp[b] = make_float2(i0.x + D.bmass * dvbx, i0.y + D.bmass * dvby);
-        ang[b] += ib * dwb + D.bmass * (rbx * dvby - rby * dvbx);
+        imp[b] = make_float2(cur.imp.x + bmass * dvbx, cur.imp.y + bmass * dvby);
+        ang[b] = cur.ang + (ib * dwb + bmass * (rbx * dvby - rby * dvbx));
       }
-      __syncthreads();
+      if (!LDSV) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+      wave_sync();
     }
+    cur = nxt;
   }
-  {
-    double s = 0.0;
-    for (uint32_t i = lane; i < D.n; i += 64) {
-      const float2 q = v[i];
-      s += (double)(q.x * q.x + q.y * q.y);
-    }
-    s = wave_sum_d(s);
-    if (lane == 0) S.ke_drag = s;
-  }
-  if (D.buoy_on) {
-    const GView g = gview(P.grids + 5 * (size_t)d, P.gstart[0] + D.c0[0], P.gidx[0] + D.f0);
-    const float om_abs = fabsf(D.omega);
-    for (uint32_t b = 0; b < D.nb; b++) {
-      const float2 c = bx[b];
-      if (!fin2(c)) continue;
-      Near q;
-      near_cells(g, c, q);
-      if (q.total == 0) continue;
-      float rho_l = 0.0f;
-      uint32_t nw = 0;
-      for (uint32_t c0 = 0; c0 < q.total; c0 += 64) {
-        const uint32_t cc = c0 + lane;
-        uint32_t j = 0;
-        float w = 0.0f;
-        if (cc < q.total) {
-          j = near_at(g, q, cc);
-          const float2 xj = x[j];
-          const float dx = c.x - xj.x, dy = c.y - xj.y;
-          w = D.mass * poly6(dx * dx + dy * dy, D.h2, D.p6c);
-        }
-        uint64_t m = __ballot(w > 0.0f);
-        while (m) {
-          const int L = first_lane(m);
-          m &= m - 1;
-          const float ww = rdl(w, L);
-          rho_l += ww;
-          if (nw < FL_ITEMS) {
-            if (lane == 0) {
-              s_j[nw] = rdlu(j, L);
-              s_ph[nw] = ww;
-            }
-          } else {
-            over = true;
-          }
-          nw++;
-        }
-      }
-      __syncthreads();
-      nw = min(nw, (uint32_t)FL_ITEMS);
-      if (rho_l <= 0.0f) continue;
-      const float rho_eff = fminf(rho_l, D.rest);
-      float on = 0.0f, od = 0.0f;
-      for (uint32_t c0 = 0; c0 < nw; c0 += 64) {
-        const uint32_t k = c0 + lane;
-        float term = 0.0f, wk = 0.0f;
-        bool use = false;
+}
+
+template <bool LDSV>
+__device__ inline void gs_sweep_buoy(const FlPtrs& P, const FlDrum& D, float2* V, float* s_t0) {
+  const uint32_t lane = threadIdx.x & 63u;
+  float2* imp = P.imp + D.b0;
+  const float om_abs = fabsf(D.omega), rest = D.rest, area = D.area, dt = D.dt, mass = D.mass;
+  const uint32_t nb = D.nb, b0 = D.b0;
+  GsBall cur, nxt;
+  gs_load(P, b0, nb, 0, false, cur);
+  for (uint32_t b = 0; b < nb; b++) {
+    gs_load(P, b0, nb, b + 1, false, nxt);
+    const uint32_t nw = cur.n;
+    const float rho_l = cur.st.x, od = cur.st.y;
+    if (nw != 0 && rho_l > 0.0f) {
+#pragma unroll
+      for (int s = 0; s < 4; s++) {
+        const uint32_t k = (uint32_t)s * 64u + lane;
         if (k < nw) {
-          const uint32_t j = s_j[k];
-          wk = s_ph[k];
-          const float2 dj = x[j];
-          const float d2 = dj.x * dj.x + dj.y * dj.y;
-          if (!(d2 <= 1e-12f)) {
-            const float2 vj = v[j];
-            term = wk * (dj.x * vj.y - dj.y * vj.x) / d2;
-            use = true;
+          float term = 0.0f;
+          if (cur.ir[s].w != 0.0f) {
+            const float2 vj = V[cur.ij[s]];
+            const float dx = cur.ir[s].y, dy = cur.ir[s].z;
+            term = cur.ir[s].x * (dx * vj.y - dy * vj.x) / (dx * dx + dy * dy);
           }
-        }
-        uint64_t m = __ballot(use);
-        while (m) {
-          const int L = first_lane(m);
-          m &= m - 1;
-          on += rdl(term, L);
-          od += rdl(wk, L);
+          s_t0[k] = term;
         }
       }
+      wave_sync();
+      float acc = 0.0f;
+      if (lane == 0) acc = ordered_sum(s_t0, nw, 0.0f);
+      // an item on the axis adds nothing to `on` in the reference (it is skipped); its +0.0 term
+      // leaves the running sum unchanged unless the sum is -0.0, which a sum starting at +0.0 never is
+      const float on = rdl(acc, 0);
+      const float2 c = cur.rb;
+      const float rho_eff = fminf(rho_l, rest);
       const float om_l = od > 0.0f ? fclampf(on / od, -om_abs, om_abs) : 0.0f;
       const float o2 = om_l * om_l;
       const float gex = 0.0f + o2 * c.x, gey = FL_G + o2 * c.y;
-      const float sc = (-rho_eff) * D.area;
-      const float ipx = sc * gex * D.dt, ipy = sc * gey * D.dt;
-      for (uint32_t k = lane; k < nw; k += 64) {
-        const uint32_t j = s_j[k];
-        const float frac = s_ph[k] / rho_l;
-        const float2 vj = v[j];
-        v[j] = make_float2(vj.x + -(ipx * frac) / D.mass, vj.y + -(ipy * frac) / D.mass);
+      const float sc = (-rho_eff) * area;
+      const float ipx = sc * gex * dt, ipy = sc * gey * dt;
+#pragma unroll
+      for (int s = 0; s < 4; s++) {
+        const uint32_t k = (uint32_t)s * 64u + lane;
+        if (k < nw) {
+          const float frac = cur.ir[s].x / rho_l;
+          const float2 vj = V[cur.ij[s]];
+          V[cur.ij[s]] = make_float2(vj.x + -(ipx * frac) / mass, vj.y + -(ipy * frac) / mass);
+        }
       }
-      if (lane == 0) {
-        const float2 i0 = imp[b];
-        imp[b] = make_float2(i0.x + ipx, i0.y + ipy);
-      }
-      __syncthreads();
+      if (lane == 0) imp[b] = make_float2(cur.imp.x + ipx, cur.imp.y + ipy);
+      if (!LDSV) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+      wave_sync();
     }
+    cur = nxt;
   }
-  {
-    double s = 0.0;
-    for (uint32_t i = lane; i < D.n; i += 64) {
-      const float2 q = v[i];
-      s += (double)(q.x * q.x + q.y * q.y);
-    }
-    s = wave_sum_d(s);
-    if (lane == 0) S.ke_buoy = s;
+}
+
+__device__ inline double gs_ke(const float2* V, uint32_t n, double* s_sum) {
+  const uint32_t t = threadIdx.x, lane = t & 63u, wv = t >> 6;
+  double s = 0.0;
+  for (uint32_t i = t; i < n; i += FL_T) {
+    const float2 q = V[i];
+    s += (double)(q.x * q.x + q.y * q.y);
   }
-  if (over && lane == 0) atomicOr(&S.err, FE_ITEMS);
+  s = wave_sum_d(s);
+  if (lane == 0) s_sum[wv] = s;
+  __syncthreads();
+  double a = 0.0;
+  for (uint32_t q = 0; q < FL_T / 64; q++) a += s_sum[q];
+  __syncthreads();
+  return a;
+}
+
+// The sweep: wave 0 walks the balls; the drum's velocities in dynamic LDS when lds != 0.
+extern "C" __global__ void __launch_bounds__(FL_T) fl_gs(FlPtrs P, uint32_t lds_cap) {
+  extern __shared__ float2 s_v[];
+  __shared__ __align__(16) float s_t0[GS_ITEMS];
+  __shared__ __align__(16) float s_t1[GS_ITEMS];
+  __shared__ __align__(16) float s_t2[GS_ITEMS];
+  __shared__ double s_sum[FL_T / 64];
+  const uint32_t d = blockIdx.x, t = threadIdx.x;
+  const FlDrum& D = P.drums[d];
+  FlState& S = P.st[d];
+  if (D.n == 0) return;
+  float2* v = P.v + D.f0;
+  const bool lds = D.n <= lds_cap;
+  if (lds) {
+    for (uint32_t i = t; i < D.n; i += FL_T) s_v[i] = v[i];
+    __syncthreads();
+  }
+  if (t < 64 && D.nb > 0) {
+    if (lds) gs_sweep<true>(P, D, s_v, s_t0, s_t1, s_t2);
+    else gs_sweep<false>(P, D, v, s_t0, s_t1, s_t2);
+  }
+  __syncthreads();
+  const double k1 = gs_ke(lds ? s_v : v, D.n, s_sum);
+  if (t < 64 && D.nb > 0 && D.buoy_on) {
+    if (lds) gs_sweep_buoy<true>(P, D, s_v, s_t0);
+    else gs_sweep_buoy<false>(P, D, v, s_t0);
+  }
+  __syncthreads();
+  const double k2 = gs_ke(lds ? s_v : v, D.n, s_sum);
+  if (lds)
+    for (uint32_t i = t; i < D.n; i += FL_T) v[i] = s_v[i];
+  if (t == 0) {
+    S.ke_drag = k1;
+    S.ke_buoy = k2;
+  }
 }
 
 // 7a density at the final positions (pbf.rs:2136-2144).
@@ -2276,6 +2437,8 @@ int mg_fluid_step(mg_fluid* h, const mg_fluid_drum* drums, float* d_x, float* d_
     const bool active = m.n_fluid > 0 && m.dt > 0.0f;  // pbf.rs:1196-1198
     D.n = active ? m.n_fluid : 0u;
     D.nb = m.n_balls;
+    if (m.n_balls > 2048)
+      return fl_fail(h, err, errlen, MG_E_ARG, "drum " + std::to_string(d) + ": more than 2048 balls");
     const float hh = m.h, rest = m.rest_density, mass = m.particle_mass;
     D.h = hh;
     D.rest = rest;
@@ -2358,7 +2521,7 @@ int mg_fluid_step(mg_fluid* h, const mg_fluid_drum* drums, float* d_x, float* d_
     D.ecap = D.n ? 48u * D.n + 64u : 0u;
     const float cells[5] = {hh, D.bgc, std::fmax(D.hc, 1e-6f), hh, D.g5c};
     const bool need[5] = {D.n > 0, D.n > 0 && m.n_balls > 0, D.n > 0 && D.drag_on, D.n > 0 && D.grp_on,
-                          D.n > 0 && D.links_on};
+                          D.n > 0 && (D.links_on || D.drag_on || D.buoy_on)};
     for (int q = 0; q < 5; q++) {
       D.c0[q] = (uint32_t)TC[q];
       D.ccap[q] = need[q] ? cell_cap(m.radius_m, cells[q]) : 0u;
@@ -2453,6 +2616,15 @@ int mg_fluid_step(mg_fluid* h, const mg_fluid_drum* drums, float* d_x, float* d_
   take((void**)&P.gc, TB * GC * 16);
   take((void**)&P.fgoff, (TF + nd) * 4);
   take((void**)&P.fgids, TB * GC * 4);
+  take((void**)&P.gdj, TB * GS_ITEMS * 4);
+  take((void**)&P.gdr, TB * GS_ITEMS * 16);
+  take((void**)&P.gbj, TB * GS_ITEMS * 4);
+  take((void**)&P.gbr, TB * GS_ITEMS * 16);
+  take((void**)&P.gdn, TB * 4);
+  take((void**)&P.gbn, TB * 4);
+  take((void**)&P.gds, TB * 16);
+  take((void**)&P.gdb, TB * 8);
+  take((void**)&P.gbs, TB * 8);
   take((void**)&P.lcnt, TB * 4);
   take((void**)&P.lrec, TL * 16);
   take((void**)&P.locnt, TB * 4);
@@ -2523,7 +2695,18 @@ int mg_fluid_step(mg_fluid* h, const mg_fluid_drum* drums, float* d_x, float* d_
     hipLaunchKernelGGL(fl_vel, dim3(nft), T, 0, nullptr, P);
   }
   hipLaunchKernelGGL(fl_grid_b, G, T, 0, nullptr, P);
-  hipLaunchKernelGGL(fl_gs, G, dim3(64), 0, nullptr, P);
+  if (nwt) hipLaunchKernelGGL(fl_gs_pre, dim3(nwt), T, 0, nullptr, P, (const uint2*)h->d_wt);
+  {
+    // the drum's fluid velocities stay in LDS during the sweep when they fit (160 KiB per workgroup
+    // on gfx950, less the kernel's static LDS)
+    uint32_t maxn = 0;
+    for (uint32_t d = 0; d < nd; d++) maxn = std::max(maxn, hd[d].n);
+    const size_t cap_bytes = 160u * 1024u - 4096u;
+    const uint32_t lds_cap = (uint32_t)std::min<size_t>(cap_bytes / 8, (size_t)maxn);
+    const size_t bytes = (size_t)lds_cap * 8;
+    if (bytes > 65536) (void)hipFuncSetAttribute((const void*)fl_gs, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    hipLaunchKernelGGL(fl_gs, G, T, bytes, nullptr, P, lds_cap);
+  }
   if (nft) {
     hipLaunchKernelGGL(fl_dens2, dim3(nft), T, 0, nullptr, P);
     hipLaunchKernelGGL(fl_mu, dim3(nft), T, 0, nullptr, P);
