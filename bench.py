@@ -1,36 +1,32 @@
 #!/usr/bin/env python3
 """Benchmark driver: prints one JSON line.
 
-Headline metric (drum-substeps/s, batched FP32 vs plain): the drum substep engine is switched off
-in this build (an earlier form faulted on the GPU and the cause has not been shown), so the
-headline value is reported as null with the reason, never estimated.
+Headline (drum-substeps/s, batched FP32 vs plain): the C4 batched ensemble, 256 drums drawn from a
+seeded stream (2 mm balls, N_b U{100..2000} with the drum diameter that seeds them, fluid
+resolution U{25..75}, media fill U[0.2, 0.4], 50-90 % of critical speed, friction U[0.2, 0.5]),
+every drum advanced one substep per step (the fluid step, the DEM step with the fluid's coupling
+impulses, the power/torque/energy accumulators and the stress update), state resident in HBM, f32
+state with f64 CG dot products and energy sums. value = drums x substeps / wall time of the timed
+substeps; ms_per_step = one substep of all drums. Beside it: the plain single-drum path (the C2
+2000-disc drum, the C1 realtime preset), the power draw's error against the FP64 oracle after 480
+realtime substeps, the canonical algorithmic bytes of a substep (N_f (286 + 32 n_d + 84 n_cg) +
+N_b (48 (8 + n_dem) + 144)) and the kernel table from HIP event pairs over five more substeps.
 
-Timed on the GPU (config.workload, ms_per_step, roofline and cpu_baseline all describe it): the
-kernel-isolation workload of the same fluid hot path, a periodic box of
-side x side PBF particles (default 1024^2 = 1,048,576): per step the grid build + cell reorder,
-four density passes, velocity update, density recompute and the implicit viscosity CG to 1e-6
-relative residual, FP32 state with FP64 CG reductions, state resident in HBM. Reported as
-particle-steps/s and as algorithmic HBM GB/s from the canonical per-particle byte counts
-(B = 226 + 32 n_d + 84 n_cg bytes per particle per step).
+roofline: the dominant kernel of the batched substep by total time, its algorithmic bytes per launch
+(bench.kernel_bytes: the arrays it streams at FP32 widths) over its average launch time from a HIP
+event pair per launch on the stream it runs on; `traffic` is left null (HBM bytes per launch need a
+rocprofv3 --pmc run of their own).
+cpu_baseline: the C++ restatement of the reference (oracle/, f32 reference policy, 1 core) on a
+bounded sample of the same ensemble (drums in order, two substeps each, about --cpu-seconds).
+p50/p99 latency: reported by the served form; null while it is not built.
 
-roofline: the dominant kernel by total time, its algorithmic bytes per launch divided by its
-average launch time from HIP events on the box's stream (density passes: the timed region's last
-launch relaunched back to back inside one event pair; CG kernels: one event pair per launch over
-K more steps); `traffic` (HBM bytes per launch) and `valu_busy` are not measured in this process
-(hardware counters need a rocprofv3 --pmc run of their own): they are read from the committed
-summaries named in `traffic_source` / `valu_source` (the density passes are VALU-bound: their
-HBM fraction is low by nature).
-cpu_baseline: the C++ restatement of the reference (oracle/, 1 core) on a bounded sample of the
-same box workload; cpu_baseline_drum: the same restatement on C2 drum substeps (the headline unit,
-context only while drums do not step on the GPU). kernel_isolation.grid: the reference-layout
-uniform grid build (grid.rs:101-153) over the box's 1,048,576 initial points;
-kernel_isolation.mixing_index: the Lacey mixing index (metrics.rs:506-576) of 1024 synthetic
-C2-sized drum states in one launch; kernel_isolation.drum_metrics: metrics::compute of 256 C2 drum
-states in one call; kernel_isolation.stress: StressAccum::update of 256 drums per call;
-kernel_isolation.dem: DemState::step_recorded of 256 dry 2000-ball populations per call;
-kernel_isolation.surface: extract_surface of 256 C2 drum states per call; each with the
+kernel_isolation (skipped with --no-box): the periodic PBF box of side x side particles (default
+1024^2), per step the grid build + cell reorder, four density passes, velocity update, density
+recompute and the implicit viscosity CG to 1e-6 relative residual, with its own roofline and CPU
+baseline, FP64 and Bingham lines, and the standalone operators (uniform grid, mixing index, drum
+metrics, stress accumulators, DEM steps of 256 populations, free surfaces), each with the
 restatement's rate beside it where one exists.
---part grid / --part mixing time only that operator (for a profiler run).
+--part drums|grid|mixing|... times only that part (for a profiler run) and prints its JSON.
 """
 from __future__ import annotations
 
@@ -530,53 +526,179 @@ def precision_vs_fp64(cfg, steps) -> dict:
     return out
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--side", type=int, default=1024)
-    ap.add_argument("--mu", type=float, default=1.0e-3)
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-state box measurement")
-    ap.add_argument("--no-bingham", action="store_true", help="skip the Bingham-fluid box measurement")
-    ap.add_argument("--copy-mib", type=int, default=0,
-                    help="also run one plain device copy of this size (calibrates PMC byte counters)")
-    ap.add_argument("--part", choices=("all", "grid", "mixing", "drum_metrics", "stress", "dem", "surface"), default="all",
-                    help="grid / mixing: time only that operator (for a profiler run) and print its JSON")
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(drums and boxes shard as independent units, one process per GPU)", file=sys.stderr)
-        return 2
+
+# ---- drums: the headline ------------------------------------------------------------------------
+C1_PARAMS = {"simulation": {"max_balls": 300, "resolution": 25}}
+
+
+def c4_params(n=256, seed=2026):
+    """The batched ensemble: drum k draws N_b ~ U{100..2000}, resolution ~ U{25..75}, media fill
+    J ~ U[0.20, 0.40], speed ~ U[50, 90] % of critical and ball friction ~ U[0.2, 0.5] (ball-ball and
+    ball-wall, static = kinetic) from one seeded stream; the drum diameter d sqrt(N_b / (0.82 J)) makes
+    the seeded population N_b balls of 2 mm; seed k + 1."""
+    import math
 
     import numpy as np
-    import torch
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        nb = int(rng.integers(100, 2001))
+        res = int(rng.integers(25, 76))
+        fill = float(rng.uniform(0.20, 0.40))
+        nc = float(rng.uniform(50.0, 90.0))
+        mu = float(rng.uniform(0.2, 0.5))
+        dia = 0.002 * math.sqrt(nb / (0.82 * fill))
+        out.append({"mill": {"diameter_m": dia, "speed_mode": "percent_critical", "speed_value": nc},
+                    "media": {"fill_fraction": fill, "friction_ball_ball": mu, "friction_ball_wall": mu,
+                              "friction_ball_ball_static": mu, "friction_ball_wall_static": mu},
+                    "simulation": {"resolution": res, "max_balls": nb, "seed": k + 1}})
+    return out
 
-    from milldynamics3_amd import drum_engine_enabled
+
+def kernel_bytes(name, n_f, n_b, n_d, n_dem):
+    """Algorithmic bytes of one launch of a drum-step kernel over n_f fluid particles and n_b balls
+    (FP32 storage widths: Vec2 8 B, scalar and index 4 B; each array read or written once per element;
+    neighbour gathers, per-edge tables and merged item lists not counted). Density passes and CG
+    kernels follow the canonical per-particle table (pass A 12, pass B 20; CG apply 28, update 56);
+    the others count the arrays they stream."""
+    table = {
+        "fl_predict": 40 * n_f,                     # R x, v; W x, v, x0
+        "fl_nbr": 16 * n_f,                         # R x; W count / entries index
+        "fl_dens_a": 12 * n_f, "fl_dens_b": 20 * n_f,
+        "fl_dens_c": 24 * n_f,                      # R x, dp; W x
+        "fl_push": 40 * n_f,                        # R x, v; W x, excess; records
+        "fl_cohesion": 16 * n_f, "fl_wall": 16 * n_f + 4 * n_f,
+        "fl_vel": 56 * n_f,                         # R x, x0, excess, coh, adh; W v
+        "fl_gs": 16 * n_f + 36 * n_b,               # R, W v; per ball R x, v, omega, RW impulse
+        "fl_gs_pre": 8 * n_f + 24 * n_b,
+        "fl_dens2": 12 * n_f, "fl_mu": 12 * n_f, "fl_weights": 24 * n_f,
+        "fl_cg_apply": 28 * n_f, "fl_cg_upd": 56 * n_f, "fl_cg_dir": 24 * n_f, "fl_cg_setup": 52 * n_f,
+        "fl_shear": 28 * n_f, "fl_clamp": 16 * n_f, "fl_cg_out": 24 * n_f,
+        "dem_project": 48 * n_dem * n_b, "dem_predict": 64 * n_b, "dem_friction": 48 * n_b,
+        "dem_restitution": 48 * n_b, "dem_finish": 40 * n_b, "dem_sort": 24 * n_b, "dem_grid": 16 * n_b,
+    }
+    return table.get(name)
+
+
+def bench_drums(args, cpu=True) -> dict:
+    """The batched ensemble (headline), the plain single drums, the power-draw error of the FP32
+    engine against the FP64 oracle, the dominant kernel's roofline and the restatement's rate."""
+    import numpy as np
+
+    from milldynamics3_amd.sim import Batch, Simulation
+    plist = c4_params(args.drums)
+    b = Batch(plist)
+    nf = np.array([d.n_fluid for d in b.drums])
+    nb = np.array([d.n_balls for d in b.drums])
+    b.step(max(args.warmup, 1))
+    b.sync()
+    sync()
+    t0 = time.perf_counter()
+    b.step(args.steps)
+    b.sync()
+    wall = time.perf_counter() - t0
+    rate = len(plist) * args.steps / wall
+    st = [d.stats() for d in b.drums]
+    n_d = np.array([s["density_iterations"] for s in st])
+    n_cg = np.array([s["viscosity_iterations"] for s in st])
+    n_dem = 2
+    step_bytes = float(np.sum(nf * (286.0 + 32.0 * n_d + 84.0 * n_cg) + nb * (48.0 * (8 + n_dem) + 144.0)))
+    # per-kernel HIP-event timing over K more substeps: the dominant kernel's average launch
+    k2 = max(1, min(args.steps, 5))
+    b.set_timing(True)
+    b.step(k2)
+    b.sync()
+    kt = b.kernel_times()
+    b.set_timing(False)
+    dom = max(kt, key=lambda k: kt[k][0])
+    avg_ms = kt[dom][0] / max(kt[dom][1], 1)
+    bpl = kernel_bytes(dom, int(nf.sum()), int(nb.sum()), float(n_d.mean()), n_dem)
+    achieved = None if bpl is None else bpl / (avg_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": None if achieved is None else round(achieved, 2),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": None if achieved is None else round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": None, "traffic_note": "HBM bytes per launch need a rocprofv3 --pmc run of their own",
+                "bytes_per_launch": bpl, "avg_launch_us": round(1000.0 * avg_ms, 2), "launches": kt[dom][1],
+                "timing": f"HIP event pair per launch on the null stream over {k2} batched substeps",
+                "workload": f"C4 batched ensemble, {len(plist)} drums"}
+    total_ms = sum(v[0] for v in kt.values())
+    kernels = {k: {"total_ms": round(v[0], 3), "launches": v[1], "avg_us": round(1000.0 * v[0] / max(v[1], 1), 2),
+                   "share": round(v[0] / total_ms, 4)}
+               for k, v in sorted(kt.items(), key=lambda kv: -kv[1][0])[:16]}
+    b.close()
+    # the plain single-drum path
+    plain = {}
+    for label, p, k in (("C2", C2_PARAMS, max(4, args.steps // 2)), ("C1", C1_PARAMS, args.steps)):
+        s = Simulation(p)
+        s.step_fixed(2)
+        t0 = time.perf_counter()
+        s.step_fixed(k)
+        dt = time.perf_counter() - t0
+        plain[label] = {"drum_substeps_per_s": round(k / dt, 2), "ms_per_substep": round(1000.0 * dt / k, 3),
+                        "balls": s.n_balls, "fluid": s.n_fluid}
+        s.close()
+    out = {"drum_substeps_per_s": round(rate, 1), "ms_per_substep": round(1000.0 * wall / args.steps, 3),
+           "drums": len(plist), "fluid_particles": int(nf.sum()), "balls": int(nb.sum()),
+           "steps": args.steps, "canonical_bytes_per_substep": step_bytes,
+           "hbm_gbs_canonical": round(step_bytes * args.steps / wall / 1e9, 2),
+           "density_passes_mean": float(n_d.mean()), "cg_iterations_mean": float(n_cg.mean()),
+           "plain": plain, "batched_vs_plain_c2": round(rate / plain["C2"]["drum_substeps_per_s"], 1),
+           "roofline": roofline, "kernels": kernels}
+    out["precision"] = power_error(args.precision_substeps)
+    if cpu:
+        out["cpu_baseline"] = cpu_baseline_c4(plist, args.cpu_seconds)
+    return out
+
+
+def power_error(substeps) -> dict:
+    """The FP32 engine (f32 state, f64 CG dot products and energy sums) against the FP64 oracle on
+    the realtime preset from the same initial state: power, torque and collision-rate EMAs after
+    `substeps` substeps (the reference's 1 s time constant)."""
+    from milldynamics3_amd.sim import Simulation
+    from oracle import oracle as orc
+    g = Simulation(C1_PARAMS)
+    o = orc.OracleSim(C1_PARAMS, "fp64")
+    g.step_fixed(substeps)
+    o.step_fixed(substeps)
+    gs, os_ = g.stats(), o.scalars()
+    g.close()
+    out = {"workload": f"realtime preset (407 balls, 794 fluid), {substeps} substeps",
+           "reference": "FP64 oracle (f64 state and sums)"}
+    for k in ("power_draw_w", "torque_nm", "dissipated_power_w"):
+        ref = os_[k]
+        out[k] = {"fp32": gs[k], "fp64": ref, "rel_err": abs(gs[k] - ref) / max(abs(ref), 1e-30)}
+    out["power_within_1pct"] = out["power_draw_w"]["rel_err"] <= 0.01
+    return out
+
+
+def cpu_baseline_c4(plist, seconds) -> dict:
+    """The C++ restatement (oracle, f32 'ref' policy, 1 core) on a bounded sample of the same
+    ensemble: drums in order, two substeps each, until `seconds` have passed."""
+    from oracle import oracle as orc
+    start = time.perf_counter()
+    stepping, done, drums = 0.0, 0, 0
+    for p in plist:
+        o = orc.OracleSim(p, "ref")
+        t1 = time.perf_counter()
+        o.step_fixed(2)
+        stepping += time.perf_counter() - t1
+        done += 2
+        drums += 1
+        if time.perf_counter() - start > seconds:
+            break
+    return {"value": round(done / stepping, 2), "unit": "drum-substeps/s", "cores": 1, "kind": "port",
+            "sample": f"{done} substeps: the first {drums} drums of the ensemble, 2 substeps each, on the C++ "
+                      "restatement (oracle/, f32 reference policy), 1 core; stepping time only (each drum's "
+                      "seeding excluded)"}
+
+
+def bench_box(args) -> dict:
+    """The kernel-isolation workload (periodic PBF box): the box kernels' table and roofline, FP64 and
+    Bingham lines, and the standalone operators (grid, mixing index, drum metrics, stress, DEM,
+    free surface)."""
+    import numpy as np
+
     from milldynamics3_amd.box import Box
-
-    if not torch.cuda.is_available():
-        print("bench.py needs a GPU", file=sys.stderr)
-        return 1
-    torch.cuda.set_device(0)
-
-    if args.copy_mib > 0:
-        src = torch.ones(args.copy_mib * 262144, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
-    if args.part != "all":
-        part = {"grid": lambda: bench_grid(args, cpu=not args.no_cpu),
-                "mixing": lambda: bench_mixing(args, cpu=not args.no_cpu),
-                "drum_metrics": lambda: bench_drum_metrics(args, cpu=not args.no_cpu),
-                "stress": lambda: bench_stress(args),
-                "dem": lambda: bench_dem(args, cpu=not args.no_cpu),
-                "surface": lambda: bench_surface(args, cpu=not args.no_cpu)}[args.part]()
-        print(json.dumps({"part": args.part, **part}), flush=True)
-        return 0
     cfg = {"side": args.side, "mu": args.mu}
     box = Box(cfg)
     n = box.n
@@ -668,34 +790,100 @@ def main() -> int:
         cores = min(16, os.cpu_count() or 1)  # a one-GPU box's share of the host
         if cores > 1:
             iso["cpu_baseline_all_cores"] = cpu_baseline_box_cores(cfg, min(args.cpu_seconds, 5.0), cores)
+    iso["roofline"] = roofline
+    iso["cpu_baseline"] = box_cpu
+    iso["cpu_baseline_drum_c2"] = drum_cpu
+    box.close()
+    return iso
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--side", type=int, default=1024)
+    ap.add_argument("--mu", type=float, default=1.0e-3)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-state box measurement")
+    ap.add_argument("--no-bingham", action="store_true", help="skip the Bingham-fluid box measurement")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="also run one plain device copy of this size (calibrates PMC byte counters)")
+    ap.add_argument("--drums", type=int, default=256, help="drums of the batched ensemble (headline)")
+    ap.add_argument("--precision-substeps", type=int, default=480,
+                    help="substeps of the FP32-vs-FP64 power comparison (realtime preset)")
+    ap.add_argument("--no-box", action="store_true", help="skip the periodic-box kernel-isolation measurements")
+    ap.add_argument("--part", choices=("all", "drums", "grid", "mixing", "drum_metrics", "stress", "dem", "surface"),
+                    default="all",
+                    help="grid / mixing: time only that operator (for a profiler run) and print its JSON")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(drums and boxes shard as independent units, one process per GPU)", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    from milldynamics3_amd import drum_engine_enabled
+    from milldynamics3_amd.box import Box
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        return 1
+    torch.cuda.set_device(0)
+
+    if args.copy_mib > 0:
+        src = torch.ones(args.copy_mib * 262144, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    if args.part != "all":
+        part = {"grid": lambda: bench_grid(args, cpu=not args.no_cpu),
+                "mixing": lambda: bench_mixing(args, cpu=not args.no_cpu),
+                "drum_metrics": lambda: bench_drum_metrics(args, cpu=not args.no_cpu),
+                "stress": lambda: bench_stress(args),
+                "dem": lambda: bench_dem(args, cpu=not args.no_cpu),
+                "surface": lambda: bench_surface(args, cpu=not args.no_cpu),
+                "drums": lambda: bench_drums(args, cpu=not args.no_cpu)}[args.part]()
+        print(json.dumps({"part": args.part, **part}), flush=True)
+        return 0
+    drums = bench_drums(args, cpu=not args.no_cpu)
+    iso = None if args.no_box else bench_box(args)
     line = {
         "metric": METRIC,
-        "value": None,
+        "value": drums["drum_substeps_per_s"],
         "unit": "drum-substeps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": drums["ms_per_substep"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"kernel isolation: periodic PBF box {args.side}x{args.side} ({n} particles), "
-                               f"4 density passes + viscosity CG to 1e-6 |b|, mu {args.mu} Pa s, f32 state with "
-                               "f64 CG reductions, inputs resident in HBM",
-                   "timed": f"{args.steps} box steps in one call; ms_per_step is one box step",
-                   "headline_status": "value is null: the headline unit is drum substeps and the drum substep "
-                                      "engine is switched off in this build (an earlier form faulted on the GPU; "
-                                      "cause not shown)",
-                   "drum_engine_enabled": bool(drum_engine_enabled())},
-        "roofline": roofline,
-        "cpu_baseline": box_cpu,
-        "cpu_baseline_drum": drum_cpu,
+        "config": {"workload": f"C4 batched ensemble: {drums['drums']} drums ({drums['balls']} balls, "
+                               f"{drums['fluid_particles']} fluid particles) seeded from Rng 2026 (N_b U{{100..2000}}, "
+                               "resolution U{25..75}, media fill U[0.2, 0.4], 50-90 % critical speed, friction "
+                               "U[0.2, 0.5]); one substep of every drum per step (fluid step, DEM step with the "
+                               "coupling impulses, accumulators, stress update), state resident in HBM",
+                   "precision": "f32 state, f64 CG dot products and energy/power sums",
+                   "timed": f"{args.steps} batched substeps after {max(args.warmup, 1)} warm-up substeps; "
+                            "ms_per_step is one substep of all drums",
+                   "plain_c2_drum_substeps_per_s": drums["plain"]["C2"]["drum_substeps_per_s"],
+                   "batched_vs_plain_c2": drums["batched_vs_plain_c2"],
+                   "power_rel_err_vs_fp64": drums["precision"]["power_draw_w"]["rel_err"],
+                   "p50_p99_latency": None,
+                   "p50_p99_status": "the served form (request loop) is not built in this revision"},
+        "roofline": drums["roofline"],
+        "cpu_baseline": drums.get("cpu_baseline"),
+        "drums": drums,
         "kernel_isolation": iso,
     }
     print(json.dumps(line), flush=True)
-    box.close()
     return 0
 
 

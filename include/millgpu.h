@@ -123,6 +123,10 @@ mg_sim* mg_batch_drum(mg_batch* batch, uint32_t index);
 int mg_batch_sync(mg_batch* batch, char* err, size_t errlen);
 /* Host wall-clock time (ms) of the last mg_batch_step (it waits for the device). */
 float mg_batch_last_step_ms(const mg_batch* batch);
+/* Per-kernel timing of the batch's following substeps (the fluid and DEM operators' kernels), as
+ * mg_fluid_set_timing / mg_fluid_kernel_times. No reference counterpart: measurement. */
+int mg_batch_set_timing(mg_batch* batch, int on);
+int mg_batch_kernel_times(mg_batch* batch, char* json, size_t cap);
 
 const char* mg_version(void);
 
@@ -423,6 +427,9 @@ void mg_dem_destroy(mg_dem* dem);
 int mg_dem_step(mg_dem* dem, const mg_dem_drum* drums, float* d_x, float* d_v, float* d_theta, float* d_omega,
                 const float* d_ext_impulse, const float* d_ext_angular, mg_dem_stats* stats, char* err,
                 size_t errlen);
+/* Per-kernel timing of the DEM steps, as mg_fluid_set_timing / mg_fluid_kernel_times. */
+int mg_dem_set_timing(mg_dem* dem, int on);
+int mg_dem_kernel_times(mg_dem* dem, char* json, size_t cap);
 /* The last step's impact records of one population, in the reference's order (ImpactRecord,
  * dem.rs:152-169); *n receives the count. */
 int mg_dem_impacts(mg_dem* dem, uint32_t drum, mg_impact_record* out, uint32_t cap, uint32_t* n);
@@ -468,6 +475,12 @@ typedef struct mg_fluid mg_fluid;
 /* Workspace for n_drums drums (at most 65535); its device memory grows with the batch it steps. */
 int mg_fluid_create(uint32_t n_drums, mg_fluid** out, char* err, size_t errlen);
 void mg_fluid_destroy(mg_fluid* fluid);
+/* Per-kernel timing of the following steps (on = 1 starts it and clears the totals): each launch is
+ * bracketed by HIP events on its stream. mg_*_kernel_times writes {"kernel": [total_ms, launches],
+ * ...} as JSON into json (returns its length, or minus the size needed). No reference counterpart:
+ * measurement. */
+int mg_fluid_set_timing(mg_fluid* fluid, int on);
+int mg_fluid_kernel_times(mg_fluid* fluid, char* json, size_t cap);
 /* Fixed density-pass and CG-iteration counts for every drum (on = 1), as the reference's tests
  * force them; on = 0 restores the adaptive passes and the CG tolerance. No reference counterpart. */
 int mg_fluid_force_iterations(mg_fluid* fluid, int on, uint32_t density_passes, uint32_t cg_iterations);

@@ -37,6 +37,7 @@
 #include "mg_drumgeo.h"
 #include "mg_expf.h"
 #include "mg_internal.h"
+#include "mg_ktimer.h"
 
 #define DE_BLOCK 256
 #define DE_MAX_BALLS 2048
@@ -1159,6 +1160,7 @@ struct mg_dem {
   mg_impact_record* rec = nullptr;
   std::vector<DeOut> h_out;
   std::string msg;
+  mgk::Timer tm;
 };
 
 namespace {
@@ -1372,11 +1374,11 @@ int mg_dem_step(mg_dem* h, const mg_dem_drum* drums, float* d_x, float* d_v, flo
   auto* v = reinterpret_cast<float2*>(d_v);
   const uint32_t S = h->max_balls, P = h->pcap;
   const dim3 G(nd), B(DE_BLOCK);
-  hipLaunchKernelGGL(dem_predict, G, B, 0, nullptr, h->drums, x, v, d_theta, d_omega,
+  MG_TIMED_LAUNCH(h->tm, dem_predict, G, B, 0, h->drums, x, v, d_theta, d_omega,
                      reinterpret_cast<const float2*>(d_ext_impulse), d_ext_angular, h->x0, h->vpre, h->th0, h->ompre, S,
                      h->out);
-  hipLaunchKernelGGL(dem_grid, G, B, 0, nullptr, h->drums, x, h->pairs, h->lam, P, h->out);
-  hipLaunchKernelGGL(dem_ccd_flag, G, B, 0, nullptr, h->drums, h->lift, x, h->flagged, S, h->out);
+  MG_TIMED_LAUNCH(h->tm, dem_grid, G, B, 0, h->drums, x, h->pairs, h->lam, P, h->out);
+  MG_TIMED_LAUNCH(h->tm, dem_ccd_flag, G, B, 0, h->drums, h->lift, x, h->flagged, S, h->out);
   ok = hipGetLastError() == hipSuccess &&
        hipMemcpy(h->h_out.data(), h->out, nd * sizeof(DeOut), hipMemcpyDeviceToHost) == hipSuccess;
   if (!ok) return de_fail(h, err, errlen, MG_E_DEVICE, "HIP error in the DEM predict / grid kernels");
@@ -1433,19 +1435,20 @@ int mg_dem_step(mg_dem* h, const mg_dem_drum* drums, float* d_x, float* d_v, flo
     if (dw) (void)hipFree(dw);
     if (!ok) return de_fail(h, err, errlen, MG_E_DEVICE, "HIP error applying the swept clamp");
   }
-  hipLaunchKernelGGL(dem_project, G, B, 0, nullptr, h->drums, h->lift, x, v, d_theta, d_omega, h->x0, h->th0, h->pairs,
+  MG_TIMED_LAUNCH(h->tm, dem_project, G, B, 0, h->drums, h->lift, x, v, d_theta, d_omega, h->x0, h->th0, h->pairs,
                      h->lam, h->lam_bw, h->lev, h->lpk, h->lk, h->lam_s, S, P, h->out);
-  hipLaunchKernelGGL(dem_sort, G, B, 0, nullptr, h->drums, h->pairs, h->lam, h->sorted, h->spk, h->slam, h->lev, h->lpk,
+  MG_TIMED_LAUNCH(h->tm, dem_sort, G, B, 0, h->drums, h->pairs, h->lam, h->sorted, h->spk, h->slam, h->lev, h->lpk,
                      h->lk, h->clam, h->coff, P, h->out);
-  hipLaunchKernelGGL(dem_friction, G, B, 0, nullptr, h->drums, h->lift, x, v, d_theta, d_omega, h->x0, h->th0, h->spk,
+  MG_TIMED_LAUNCH(h->tm, dem_friction, G, B, 0, h->drums, h->lift, x, v, d_theta, d_omega, h->x0, h->th0, h->spk,
                      h->slam, h->lpk, h->clam, h->coff, h->lam_bw, S, P, h->out);
-  hipLaunchKernelGGL(dem_restitution, G, B, 0, nullptr, h->drums, h->lift, x, v, h->vpre, h->ompre, h->spk, h->slam,
+  MG_TIMED_LAUNCH(h->tm, dem_restitution, G, B, 0, h->drums, h->lift, x, v, h->vpre, h->ompre, h->spk, h->slam,
                      h->lpk, h->clam, h->coff, h->lam_bw, h->ebin, h->rec, h->rcap, S, P, h->out);
-  hipLaunchKernelGGL(dem_finish, G, B, 0, nullptr, h->drums, x, v, d_omega, h->spk, h->slam, h->lpk, h->clam, h->coff,
+  MG_TIMED_LAUNCH(h->tm, dem_finish, G, B, 0, h->drums, x, v, d_omega, h->spk, h->slam, h->lpk, h->clam, h->coff,
                      h->lam_bw, S, P, h->out);
   ok = hipGetLastError() == hipSuccess &&
        hipMemcpy(h->h_out.data(), h->out, nd * sizeof(DeOut), hipMemcpyDeviceToHost) == hipSuccess;
   if (!ok) return de_fail(h, err, errlen, MG_E_DEVICE, "HIP error in the DEM kernels");
+  h->tm.collect();
   int rc = MG_OK;
   for (uint32_t d = 0; d < nd; d++) {
     const DeOut& o = h->h_out[d];
@@ -1468,6 +1471,20 @@ int mg_dem_step(mg_dem* h, const mg_dem_drum* drums, float* d_x, float* d_v, flo
                    "the records beyond max_impacts are dropped)");
   }
   return rc;
+}
+
+int mg_dem_set_timing(mg_dem* h, int on) {
+  if (!h) return MG_E_ARG;
+  h->tm.set(on != 0);
+  return MG_OK;
+}
+
+int mg_dem_kernel_times(mg_dem* h, char* json, size_t cap) {
+  if (!h || !json || cap == 0) return MG_E_ARG;
+  const std::string s = h->tm.json();
+  if (s.size() + 1 > cap) return -(int)(s.size() + 1);
+  std::memcpy(json, s.c_str(), s.size() + 1);
+  return (int)s.size();
 }
 
 const mg_impact_record* mg_dem_records_device(const mg_dem* h, uint32_t* stride) {

@@ -38,6 +38,7 @@
 #include "mg_atan2f.h"
 #include "mg_drumgeo.h"
 #include "mg_expf.h"
+#include "mg_ktimer.h"
 
 #define FL_T 256       // particles (or CG rows) per tile, lanes per drum kernel
 #define FL_PK 8        // ball-push records per particle
@@ -2307,6 +2308,7 @@ struct mg_fluid {
   std::vector<FlState> h_st;
   std::vector<FlOut> h_out;
   std::string msg;
+  mgk::Timer tm;
 };
 
 namespace {
@@ -2673,29 +2675,29 @@ int mg_fluid_step(mg_fluid* h, const mg_fluid_drum* drums, float* d_x, float* d_
   const dim3 T(FL_T), G(nd);
   const uint32_t nft = (uint32_t)ft.size(), nrt = (uint32_t)rt.size(), nwt = (uint32_t)wt.size(),
                  nlt = (uint32_t)lt.size();
-  if (nft) hipLaunchKernelGGL(fl_predict, dim3(nft), T, 0, nullptr, P);
-  hipLaunchKernelGGL(fl_grid_a, G, T, 0, nullptr, P);
+  if (nft) MG_TIMED_LAUNCH(h->tm, fl_predict, dim3(nft), T, 0, P);
+  MG_TIMED_LAUNCH(h->tm, fl_grid_a, G, T, 0, P);
   if (nft) {
-    hipLaunchKernelGGL(fl_nbr, dim3(nft), T, 0, nullptr, P, 0);
-    hipLaunchKernelGGL(fl_nbr_scan, G, T, 0, nullptr, P);
-    hipLaunchKernelGGL(fl_nbr, dim3(nft), T, 0, nullptr, P, 1);
+    MG_TIMED_LAUNCH(h->tm, fl_nbr, dim3(nft), T, 0, P, 0);
+    MG_TIMED_LAUNCH(h->tm, fl_nbr_scan, G, T, 0, P);
+    MG_TIMED_LAUNCH(h->tm, fl_nbr, dim3(nft), T, 0, P, 1);
     for (uint32_t r = 0; r < max_rounds; r++) {
-      hipLaunchKernelGGL(fl_dens_a, dim3(nft), T, 0, nullptr, P, r);
-      hipLaunchKernelGGL(fl_dens_b, dim3(nft), T, 0, nullptr, P, r);
-      hipLaunchKernelGGL(fl_dens_c, dim3(nft), T, 0, nullptr, P, r);
+      MG_TIMED_LAUNCH(h->tm, fl_dens_a, dim3(nft), T, 0, P, r);
+      MG_TIMED_LAUNCH(h->tm, fl_dens_b, dim3(nft), T, 0, P, r);
+      MG_TIMED_LAUNCH(h->tm, fl_dens_c, dim3(nft), T, 0, P, r);
     }
-    hipLaunchKernelGGL(fl_push, dim3(nft), T, 0, nullptr, P);
+    MG_TIMED_LAUNCH(h->tm, fl_push, dim3(nft), T, 0, P);
   }
-  hipLaunchKernelGGL(fl_push_sum, G, T, 0, nullptr, P);
-  if (nft) hipLaunchKernelGGL(fl_cohesion, dim3(nft), T, 0, nullptr, P);
-  if (nwt) hipLaunchKernelGGL(fl_adhesion, dim3(nwt), T, 0, nullptr, P, (const uint2*)h->d_wt);
-  hipLaunchKernelGGL(fl_adh_sum, G, T, 0, nullptr, P);
+  MG_TIMED_LAUNCH(h->tm, fl_push_sum, G, T, 0, P);
+  if (nft) MG_TIMED_LAUNCH(h->tm, fl_cohesion, dim3(nft), T, 0, P);
+  if (nwt) MG_TIMED_LAUNCH(h->tm, fl_adhesion, dim3(nwt), T, 0, P, (const uint2*)h->d_wt);
+  MG_TIMED_LAUNCH(h->tm, fl_adh_sum, G, T, 0, P);
   if (nft) {
-    hipLaunchKernelGGL(fl_wall, dim3(nft), T, 0, nullptr, P);
-    hipLaunchKernelGGL(fl_vel, dim3(nft), T, 0, nullptr, P);
+    MG_TIMED_LAUNCH(h->tm, fl_wall, dim3(nft), T, 0, P);
+    MG_TIMED_LAUNCH(h->tm, fl_vel, dim3(nft), T, 0, P);
   }
-  hipLaunchKernelGGL(fl_grid_b, G, T, 0, nullptr, P);
-  if (nwt) hipLaunchKernelGGL(fl_gs_pre, dim3(nwt), T, 0, nullptr, P, (const uint2*)h->d_wt);
+  MG_TIMED_LAUNCH(h->tm, fl_grid_b, G, T, 0, P);
+  if (nwt) MG_TIMED_LAUNCH(h->tm, fl_gs_pre, dim3(nwt), T, 0, P, (const uint2*)h->d_wt);
   {
     // the drum's fluid velocities stay in LDS during the sweep when they fit (160 KiB per workgroup
     // on gfx950, less the kernel's static LDS)
@@ -2705,29 +2707,29 @@ int mg_fluid_step(mg_fluid* h, const mg_fluid_drum* drums, float* d_x, float* d_
     const uint32_t lds_cap = (uint32_t)std::min<size_t>(cap_bytes / 8, (size_t)maxn);
     const size_t bytes = (size_t)lds_cap * 8;
     if (bytes > 65536) (void)hipFuncSetAttribute((const void*)fl_gs, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    hipLaunchKernelGGL(fl_gs, G, T, bytes, nullptr, P, lds_cap);
+    MG_TIMED_LAUNCH(h->tm, fl_gs, G, T, bytes, P, lds_cap);
   }
   if (nft) {
-    hipLaunchKernelGGL(fl_dens2, dim3(nft), T, 0, nullptr, P);
-    hipLaunchKernelGGL(fl_mu, dim3(nft), T, 0, nullptr, P);
-    hipLaunchKernelGGL(fl_weights, dim3(nft), T, 0, nullptr, P);
+    MG_TIMED_LAUNCH(h->tm, fl_dens2, dim3(nft), T, 0, P);
+    MG_TIMED_LAUNCH(h->tm, fl_mu, dim3(nft), T, 0, P);
+    MG_TIMED_LAUNCH(h->tm, fl_weights, dim3(nft), T, 0, P);
   }
-  if (nwt) hipLaunchKernelGGL(fl_groups, dim3(nwt), T, 0, nullptr, P, (const uint2*)h->d_wt);
-  if (nlt) hipLaunchKernelGGL(fl_links, dim3(nlt), T, 0, nullptr, P, (const uint2*)h->d_lt);
-  hipLaunchKernelGGL(fl_cg_prep, G, T, 0, nullptr, P);
+  if (nwt) MG_TIMED_LAUNCH(h->tm, fl_groups, dim3(nwt), T, 0, P, (const uint2*)h->d_wt);
+  if (nlt) MG_TIMED_LAUNCH(h->tm, fl_links, dim3(nlt), T, 0, P, (const uint2*)h->d_lt);
+  MG_TIMED_LAUNCH(h->tm, fl_cg_prep, G, T, 0, P);
   if (nrt) {
-    hipLaunchKernelGGL(fl_cg_setup, dim3(nrt), T, 0, nullptr, P);
-    hipLaunchKernelGGL(fl_cg_apply, dim3(nrt), T, 0, nullptr, P, 0);
-    hipLaunchKernelGGL(fl_cg_init, dim3(nrt), T, 0, nullptr, P);
-    hipLaunchKernelGGL(fl_cg_start, G, dim3(64), 0, nullptr, P);
+    MG_TIMED_LAUNCH(h->tm, fl_cg_setup, dim3(nrt), T, 0, P);
+    MG_TIMED_LAUNCH(h->tm, fl_cg_apply, dim3(nrt), T, 0, P, 0);
+    MG_TIMED_LAUNCH(h->tm, fl_cg_init, dim3(nrt), T, 0, P);
+    MG_TIMED_LAUNCH(h->tm, fl_cg_start, G, dim3(64), 0, P);
     const dim3 FG((nd + 63) / 64);
     for (uint32_t done_it = 0; done_it < 200;) {
       const uint32_t k = std::min(h->cg_chunk, 200 - done_it);
       for (uint32_t q = 0; q < k; q++) {
-        hipLaunchKernelGGL(fl_cg_apply, dim3(nrt), T, 0, nullptr, P, 1);
-        hipLaunchKernelGGL(fl_cg_upd, dim3(nrt), T, 0, nullptr, P);
-        hipLaunchKernelGGL(fl_cg_fin, FG, dim3(64), 0, nullptr, P, nd);
-        hipLaunchKernelGGL(fl_cg_dir, dim3(nrt), T, 0, nullptr, P);
+        MG_TIMED_LAUNCH(h->tm, fl_cg_apply, dim3(nrt), T, 0, P, 1);
+        MG_TIMED_LAUNCH(h->tm, fl_cg_upd, dim3(nrt), T, 0, P);
+        MG_TIMED_LAUNCH(h->tm, fl_cg_fin, FG, dim3(64), 0, P, nd);
+        MG_TIMED_LAUNCH(h->tm, fl_cg_dir, dim3(nrt), T, 0, P);
       }
       done_it += k;
       if (hipMemcpy(h->h_st.data(), P.st, nd * sizeof(FlState), hipMemcpyDeviceToHost) != hipSuccess)
@@ -2738,16 +2740,17 @@ int mg_fluid_step(mg_fluid* h, const mg_fluid_drum* drums, float* d_x, float* d_
     }
   }
   if (nft) {
-    hipLaunchKernelGGL(fl_cg_out, dim3(nft), T, 0, nullptr, P);
-    hipLaunchKernelGGL(fl_shear, dim3(nft), T, 0, nullptr, P);
-    hipLaunchKernelGGL(fl_clamp, dim3(nft), T, 0, nullptr, P);
+    MG_TIMED_LAUNCH(h->tm, fl_cg_out, dim3(nft), T, 0, P);
+    MG_TIMED_LAUNCH(h->tm, fl_shear, dim3(nft), T, 0, P);
+    MG_TIMED_LAUNCH(h->tm, fl_clamp, dim3(nft), T, 0, P);
   }
-  if (nlt) hipLaunchKernelGGL(fl_ball_out, dim3(nlt), T, 0, nullptr, P, (const uint2*)h->d_lt);
-  hipLaunchKernelGGL(fl_stats, G, dim3(64), 0, nullptr, P, h->d_out);
+  if (nlt) MG_TIMED_LAUNCH(h->tm, fl_ball_out, dim3(nlt), T, 0, P, (const uint2*)h->d_lt);
+  MG_TIMED_LAUNCH(h->tm, fl_stats, G, dim3(64), 0, P, h->d_out);
   ok = hipGetLastError() == hipSuccess &&
        hipMemcpy(h->h_st.data(), P.st, nd * sizeof(FlState), hipMemcpyDeviceToHost) == hipSuccess &&
        hipMemcpy(h->h_out.data(), h->d_out, nd * sizeof(FlOut), hipMemcpyDeviceToHost) == hipSuccess;
   if (!ok) return fl_fail(h, err, errlen, MG_E_DEVICE, "HIP error in the fluid step kernels");
+  h->tm.collect();
   int rc = MG_OK;
   for (uint32_t d = 0; d < nd; d++) {
     const FlDrum& D = hd[d];
@@ -2796,6 +2799,20 @@ int mg_fluid_step(mg_fluid* h, const mg_fluid_drum* drums, float* d_x, float* d_
     s.links_in_range = S.in_range;
   }
   return rc;
+}
+
+int mg_fluid_set_timing(mg_fluid* h, int on) {
+  if (!h) return MG_E_ARG;
+  h->tm.set(on != 0);
+  return MG_OK;
+}
+
+int mg_fluid_kernel_times(mg_fluid* h, char* json, size_t cap) {
+  if (!h || !json || cap == 0) return MG_E_ARG;
+  const std::string s = h->tm.json();
+  if (s.size() + 1 > cap) return -(int)(s.size() + 1);
+  std::memcpy(json, s.c_str(), s.size() + 1);
+  return (int)s.size();
 }
 
 }  // extern "C"

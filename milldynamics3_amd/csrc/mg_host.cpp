@@ -1064,5 +1064,23 @@ int mg_batch_sync(mg_batch* b, char* err, size_t errlen) {
   return b->b->last_rc;
 }
 float mg_batch_last_step_ms(const mg_batch* b) { return b ? b->b->last_ms : 0.0f; }
+int mg_batch_set_timing(mg_batch* b, int on) {
+  if (!b) return MG_E_ARG;
+  const int r1 = mg_fluid_set_timing(b->b->fl, on), r2 = mg_dem_set_timing(b->b->dem, on);
+  return r1 != MG_OK ? r1 : r2;
+}
+int mg_batch_kernel_times(mg_batch* b, char* json, size_t cap) {
+  if (!b || !json || cap == 0) return MG_E_ARG;
+  std::vector<char> f(1 << 16), d(1 << 16);
+  if (mg_fluid_kernel_times(b->b->fl, f.data(), f.size()) < 0 || mg_dem_kernel_times(b->b->dem, d.data(), d.size()) < 0)
+    return MG_E_ARG;
+  std::string fs(f.data()), ds(d.data());
+  std::string s = fs.substr(0, fs.size() - 1);
+  if (ds.size() > 2) s += (s.size() > 1 ? "," : "") + ds.substr(1);
+  else s += "}";
+  if (s.size() + 1 > cap) return -(int)(s.size() + 1);
+  std::memcpy(json, s.c_str(), s.size() + 1);
+  return (int)s.size();
+}
 
 }  // extern "C"

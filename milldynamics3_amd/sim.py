@@ -123,6 +123,8 @@ def lib():
     L.mg_batch_sync.argtypes = [P, ch, sz]
     L.mg_batch_last_step_ms.argtypes = [P]
     L.mg_batch_last_step_ms.restype = C.c_float
+    L.mg_batch_set_timing.argtypes = [P, C.c_int]
+    L.mg_batch_kernel_times.argtypes = [P, C.c_char_p, C.c_size_t]
     L.mg_version.restype = ch
     _lib = L
     return L
@@ -381,6 +383,18 @@ class Batch:
 
     def last_step_ms(self) -> float:
         return float(self._L.mg_batch_last_step_ms(self._h))
+
+    def set_timing(self, on: bool):
+        """Per-kernel HIP-event timing of the following substeps (on clears the totals)."""
+        _check(self._L.mg_batch_set_timing(self._h, 1 if on else 0), what="mg_batch_set_timing")
+
+    def kernel_times(self) -> dict:
+        """{kernel: (total_ms, launches)} since timing was switched on."""
+        buf = C.create_string_buffer(1 << 16)
+        n = self._L.mg_batch_kernel_times(self._h, buf, 1 << 16)
+        if n < 0:
+            raise RuntimeError("kernel time buffer too small")
+        return {k: (float(v[0]), int(v[1])) for k, v in json.loads(buf.value.decode()).items()}
 
     def close(self):
         if getattr(self, "_h", None):
