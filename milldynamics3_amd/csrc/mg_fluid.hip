@@ -276,20 +276,23 @@ __device__ inline uint32_t wg_exscan(uint32_t v, uint32_t* lds, uint32_t* tot) {
 }
 
 // Exclusive scan of cnt[0..n) into off[0..n] (off[n] = total) by one workgroup; cnt and off may
-// alias. Returns the total.
+// be the same array. Each lane owns a contiguous run of ceil(n / FL_T) entries: the runs' sums
+// are scanned once, then each lane writes its run's offsets. Returns the total.
 __device__ inline uint32_t wg_scan_array(const uint32_t* cnt, uint32_t* off, uint32_t n, uint32_t* lds) {
-  uint32_t carry = 0;
-  for (uint32_t b = 0; b < n; b += FL_T) {
-    const uint32_t i = b + threadIdx.x;
-    const uint32_t v = i < n ? cnt[i] : 0u;
-    uint32_t tot;
-    const uint32_t ex = wg_exscan(v, lds, &tot);
-    if (i < n) off[i] = carry + ex;
-    carry += tot;
+  const uint32_t k = (n + FL_T - 1) / FL_T;
+  const uint32_t a = min(n, threadIdx.x * k), e = min(n, a + k);
+  uint32_t s = 0;
+  for (uint32_t i = a; i < e; i++) s += cnt[i];
+  uint32_t tot;
+  uint32_t run = wg_exscan(s, lds, &tot);
+  for (uint32_t i = a; i < e; i++) {
+    const uint32_t c = cnt[i];
+    off[i] = run;
+    run += c;
   }
-  if (threadIdx.x == 0) off[n] = carry;
+  if (threadIdx.x == 0) off[n] = tot;
   __syncthreads();
-  return carry;
+  return tot;
 }
 
 // UniformGrid::build (grid.rs:101-197) of points p[0..n) by one workgroup: finite points only,
@@ -540,6 +543,7 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_predict(FlPtrs P) {
     q = make_float2(0.0f, 0.0f);
   }
   P.x0[g] = p;
+  P.fcnt[g] = 0;  // adhesion kicks per particle, counted by fl_adhesion
   q.y += FL_G * D.dt;
   p.x += q.x * D.dt;
   p.y += q.y * D.dt;
@@ -568,6 +572,7 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_grid_a(FlPtrs P) {
   }
   __syncthreads();
   if (D.n == 0) return;
+  for (uint32_t b = threadIdx.x; b < D.nb; b += FL_T) P.bcnt[D.b0 + b] = 0;  // push records per ball
   FlGrid* G = P.grids + 5 * (size_t)d;
   wg_grid_build(P.x + D.f0, D.n, D.h, D.ccap[0], P.gstart[0] + D.c0[0], P.gcur[0] + D.c0[0], P.gidx[0] + D.f0, G,
                 &S.err, si, su);
@@ -737,8 +742,12 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_push(FlPtrs P) {
       exy += s * nh.y;
       const float rx = -im * nh.x, ry = -im * nh.y;
       const float an = lx * ry - ly * rx;
-      if (cnt < FL_PK) P.prec[g * FL_PK + cnt] = make_float4(rx, ry, an, __uint_as_float(b));
-      else atomicOr(&P.st[tl.x].err, FE_PUSH);
+      if (cnt < FL_PK) {
+        P.prec[g * FL_PK + cnt] = make_float4(rx, ry, an, __uint_as_float(b));
+        atomicAdd(&P.bcnt[D.b0 + b], 1u);
+      } else {
+        atomicOr(&P.st[tl.x].err, FE_PUSH);
+      }
       cnt++;
     }
   }
@@ -747,67 +756,67 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_push(FlPtrs P) {
   P.pcnt[g] = min(cnt, (uint32_t)FL_PK);
 }
 
-// The balls' push impulses: each ball's records summed in particle order (pbf.rs:1432-1440).
-extern "C" __global__ void __launch_bounds__(FL_T) fl_push_sum(FlPtrs P) {
+// The balls' push impulses (pbf.rs:1432-1440): each ball's records summed in particle order,
+// gathered by a counting sort — fl_push counts the records per ball, fl_push_scan forms each
+// ball's range, fl_push_scatter lists the records, fl_push_fin orders and sums them.
+extern "C" __global__ void __launch_bounds__(FL_T) fl_push_scan(FlPtrs P) {
   __shared__ uint32_t su[FL_T];
   const uint32_t d = blockIdx.x, t = threadIdx.x;
   const FlDrum& D = P.drums[d];
   const uint32_t nb = D.nb;
   if (nb == 0) return;
-  float2* imp = P.imp + D.b0;
-  float* ang = P.ang + D.b0;
   if (D.n == 0) {
     for (uint32_t b = t; b < nb; b += FL_T) {
-      imp[b] = make_float2(0.0f, 0.0f);
-      ang[b] = 0.0f;
+      P.imp[D.b0 + b] = make_float2(0.0f, 0.0f);
+      P.ang[D.b0 + b] = 0.0f;
     }
     return;
   }
-  uint32_t* bcnt = P.bcnt + D.b0;
   uint32_t* boff = P.boff + D.b0 + d;
-  uint32_t* bcur = P.bcur + D.b0;
+  wg_scan_array(P.bcnt + D.b0, boff, nb, su);
+  for (uint32_t b = t; b < nb; b += FL_T) P.bcur[D.b0 + b] = boff[b];
+}
+
+extern "C" __global__ void __launch_bounds__(FL_T) fl_push_scatter(FlPtrs P) {
+  const uint2 tl = P.ftiles[blockIdx.x];
+  const FlDrum& D = P.drums[tl.x];
+  const uint32_t i = tl.y + threadIdx.x;
+  if (i >= D.n) return;
+  const size_t g = (size_t)D.f0 + i;
+  const uint32_t c = P.pcnt[g];
   uint32_t* ids = P.pids + (size_t)D.f0 * FL_PK;
-  for (uint32_t b = t; b < nb; b += FL_T) bcnt[b] = 0;
-  __syncthreads();
-  for (uint32_t i = t; i < D.n; i += FL_T) {
-    const size_t g = (size_t)D.f0 + i;
-    const uint32_t c = P.pcnt[g];
-    for (uint32_t s = 0; s < c; s++) atomicAdd(&bcnt[__float_as_uint(P.prec[g * FL_PK + s].w)], 1u);
+  for (uint32_t s = 0; s < c; s++) {
+    const uint32_t b = __float_as_uint(P.prec[g * FL_PK + s].w);
+    ids[atomicAdd(&P.bcur[D.b0 + b], 1u)] = i * FL_PK + s;
   }
-  __syncthreads();
-  wg_scan_array(bcnt, boff, nb, su);
-  for (uint32_t b = t; b < nb; b += FL_T) bcur[b] = boff[b];
-  __syncthreads();
-  for (uint32_t i = t; i < D.n; i += FL_T) {
-    const size_t g = (size_t)D.f0 + i;
-    const uint32_t c = P.pcnt[g];
-    for (uint32_t s = 0; s < c; s++) {
-      const uint32_t b = __float_as_uint(P.prec[g * FL_PK + s].w);
-      ids[atomicAdd(&bcur[b], 1u)] = i * FL_PK + s;
+}
+
+extern "C" __global__ void __launch_bounds__(FL_T) fl_push_fin(FlPtrs P, const uint2* __restrict__ ltiles) {
+  const uint2 tl = ltiles[blockIdx.x];
+  const FlDrum& D = P.drums[tl.x];
+  const uint32_t b = tl.y + threadIdx.x;
+  if (b >= D.nb) return;
+  uint32_t* ids = P.pids + (size_t)D.f0 * FL_PK;
+  const uint32_t* boff = P.boff + D.b0 + tl.x;
+  const uint32_t s0 = boff[b], s1 = boff[b + 1];
+  for (uint32_t a = s0 + 1; a < s1; a++) {  // particle order
+    const uint32_t v = ids[a];
+    uint32_t c = a;
+    while (c > s0 && ids[c - 1] > v) {
+      ids[c] = ids[c - 1];
+      c--;
     }
+    ids[c] = v;
   }
-  __syncthreads();
-  for (uint32_t b = t; b < nb; b += FL_T) {
-    const uint32_t s0 = boff[b], s1 = boff[b + 1];
-    for (uint32_t a = s0 + 1; a < s1; a++) {  // particle order
-      const uint32_t v = ids[a];
-      uint32_t c = a;
-      while (c > s0 && ids[c - 1] > v) {
-        ids[c] = ids[c - 1];
-        c--;
-      }
-      ids[c] = v;
-    }
-    float ix = 0.0f, iy = 0.0f, an = 0.0f;
-    for (uint32_t a = s0; a < s1; a++) {
-      const float4 r = P.prec[(size_t)D.f0 * FL_PK + ids[a]];
-      ix += r.x;
-      iy += r.y;
-      an += r.z;
-    }
-    imp[b] = make_float2(ix, iy);
-    ang[b] = an;
+  float ix = 0.0f, iy = 0.0f, an = 0.0f;
+  for (uint32_t a = s0; a < s1; a++) {
+    const float4 r = P.prec[(size_t)D.f0 * FL_PK + ids[a]];
+    ix += r.x;
+    iy += r.y;
+    an += r.z;
   }
+  P.imp[D.b0 + b] = make_float2(ix, iy);
+  P.ang[D.b0 + b] = an;
 }
 
 // 3.6a cohesion (pbf.rs:1464-1491) over the neighbour lists at the pushed positions.
@@ -933,43 +942,48 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_adhesion(FlPtrs P, const u
       const float f = -(D.bmass * frac / D.mass);
       P.aj[base + k] = it.j[s];
       P.adv[base + k] = make_float2(f * dvx, f * dvy);
+      atomicAdd(&P.fcnt[D.f0 + it.j[s]], 1u);
     }
   }
 }
 
-// The fluid's adhesion kicks: each particle's items summed in ball order (pbf.rs:1620-1626).
-extern "C" __global__ void __launch_bounds__(FL_T) fl_adh_sum(FlPtrs P) {
+// The fluid's adhesion kicks (pbf.rs:1620-1626): each particle's items summed in ball order,
+// gathered like the push impulses — fl_adhesion counts the items per particle, fl_adh_scan forms
+// the ranges, fl_adh_scatter lists the items, fl_adh_fin orders and sums them.
+extern "C" __global__ void __launch_bounds__(FL_T) fl_adh_scan(FlPtrs P) {
   __shared__ uint32_t su[FL_T];
   const uint32_t d = blockIdx.x, t = threadIdx.x;
   const FlDrum& D = P.drums[d];
-  if (D.n == 0) return;
-  float2* adh = P.adh + D.f0;
-  if (!D.adh_on) {
-    for (uint32_t i = t; i < D.n; i += FL_T) adh[i] = make_float2(0.0f, 0.0f);
-    return;
-  }
-  uint32_t* cnt = P.fcnt + D.f0;
+  if (D.n == 0 || !D.adh_on) return;
   uint32_t* off = P.foff + D.f0 + d;
-  uint32_t* cur = P.fcur + D.f0;
+  wg_scan_array(P.fcnt + D.f0, off, D.n, su);
+  for (uint32_t i = t; i < D.n; i += FL_T) P.fcur[D.f0 + i] = off[i];
+}
+
+extern "C" __global__ void __launch_bounds__(FL_T) fl_adh_scatter(FlPtrs P, const uint2* __restrict__ btiles) {
+  const uint2 tl = btiles[blockIdx.x];
+  const FlDrum& D = P.drums[tl.x];
+  const uint32_t lane = threadIdx.x & 63u, b = tl.y + (threadIdx.x >> 6);
+  if (b >= D.nb || !D.adh_on) return;
+  const uint32_t na = P.acnt[D.b0 + b];
   uint32_t* ids = P.fids + (size_t)D.b0 * P.acap;
-  const uint32_t* acnt = P.acnt + D.b0;
   const uint32_t* aj = P.aj + (size_t)D.b0 * P.acap;
-  const float2* adv = P.adv + (size_t)D.b0 * P.acap;
-  for (uint32_t i = t; i < D.n; i += FL_T) cnt[i] = 0;
-  __syncthreads();
-  for (uint32_t b = t; b < D.nb; b += FL_T)
-    for (uint32_t k = 0; k < acnt[b]; k++) atomicAdd(&cnt[aj[(size_t)b * P.acap + k]], 1u);
-  __syncthreads();
-  wg_scan_array(cnt, off, D.n, su);
-  for (uint32_t i = t; i < D.n; i += FL_T) cur[i] = off[i];
-  __syncthreads();
-  for (uint32_t b = t; b < D.nb; b += FL_T)
-    for (uint32_t k = 0; k < acnt[b]; k++) {
-      const uint32_t id = b * P.acap + k;
-      ids[atomicAdd(&cur[aj[id]], 1u)] = id;
-    }
-  __syncthreads();
-  for (uint32_t i = t; i < D.n; i += FL_T) {
+  for (uint32_t k = lane; k < na; k += 64) {
+    const uint32_t id = b * P.acap + k;
+    ids[atomicAdd(&P.fcur[D.f0 + aj[id]], 1u)] = id;
+  }
+}
+
+extern "C" __global__ void __launch_bounds__(FL_T) fl_adh_fin(FlPtrs P) {
+  const uint2 tl = P.ftiles[blockIdx.x];
+  const FlDrum& D = P.drums[tl.x];
+  const uint32_t i = tl.y + threadIdx.x;
+  if (i >= D.n) return;
+  float sx = 0.0f, sy = 0.0f;
+  if (D.adh_on) {
+    uint32_t* ids = P.fids + (size_t)D.b0 * P.acap;
+    const float2* adv = P.adv + (size_t)D.b0 * P.acap;
+    const uint32_t* off = P.foff + D.f0 + tl.x;
     const uint32_t s0 = off[i], s1 = off[i + 1];
     for (uint32_t a = s0 + 1; a < s1; a++) {  // ball order
       const uint32_t v = ids[a];
@@ -980,14 +994,13 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_adh_sum(FlPtrs P) {
       }
       ids[c] = v;
     }
-    float sx = 0.0f, sy = 0.0f;
     for (uint32_t a = s0; a < s1; a++) {
       const float2 q = adv[ids[a]];
       sx += q.x;
       sy += q.y;
     }
-    adh[i] = make_float2(sx, sy);
   }
+  P.adh[D.f0 + i] = make_float2(sx, sy);
 }
 
 // 4 wall projection (pbf.rs:1635-1653): push inside the wall at angle_next, the f64 wall impulse
@@ -1580,6 +1593,7 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_weights(FlPtrs P) {
   const FlDrum& D = P.drums[tl.x];
   const uint32_t i = tl.y + threadIdx.x;
   if (i >= D.n || !(D.mu > 0.0f)) return;
+  P.fcnt[D.f0 + i] = 0;  // boundary groups per particle, counted by fl_groups
   const float2* x = P.x + D.f0;
   const float* dens = P.dens + D.f0;
   const float* mue = P.mu_eff + D.f0;
@@ -1614,6 +1628,7 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_groups(FlPtrs P, const uin
   const FlDrum& D = P.drums[tl.x];
   const uint32_t lane = threadIdx.x & 63u, b = tl.y + (threadIdx.x >> 6);
   if (b >= D.nb) return;
+  if (lane == 0) P.locnt[D.b0 + b] = 0;  // squeeze links naming the ball as partner, counted by fl_links
   uint32_t* gcnt = P.gcnt + D.b0;
   const float2 bxb = P.bx[D.b0 + b];
   if (!D.grp_on || D.n == 0 || !fin2(bxb)) {
@@ -1671,6 +1686,7 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_groups(FlPtrs P, const uin
     if (k < nit) {
       P.gj[base + k] = it.j[s];
       P.gc[base + k] = make_float4(it.a[s], it.b[s], it.c[s], it.e[s]);
+      atomicAdd(&P.fcnt[D.f0 + it.j[s]], 1u);
     }
   }
   if (lane == 0) gcnt[b] = nit;
@@ -1731,8 +1747,12 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_links(FlPtrs P, const uint
     const double wgt = 1.0 - resolved_ratio((double)gap / D.dxd);
     const double hh = fmax((double)gap, D.a2);
     const double cf = wgt * film_coef(D.sqp, hh);
-    if (cnt < D.lcap) out[cnt] = make_float4(nx, ny, fminf(1.0f * (float)cf, D.c_max), __uint_as_float(j));
-    else over = true;
+    if (cnt < D.lcap) {
+      out[cnt] = make_float4(nx, ny, fminf(1.0f * (float)cf, D.c_max), __uint_as_float(j));
+      atomicAdd(&P.locnt[D.b0 + j], 1u);
+    } else {
+      over = true;
+    }
     cnt++;
   }
   {
@@ -1757,8 +1777,11 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_links(FlPtrs P, const uint
 }
 
 // The viscosity solve's form (pbf.rs:2244-2299): none (mu = 0), the fluid alone when no boundary
-// group and no ball pair or wall is within the squeeze reach, else jointly with the balls; for the
-// joint form the groups listed per fluid particle and the links per partner ball, in their order.
+// group and no ball pair or wall is within the squeeze reach, else jointly with the balls. For the
+// joint form each fluid row lists the groups that hold its particle (ball, then slot order) and
+// each ball the links that name it as partner (ball, then slot order): fl_groups and fl_links
+// count them, fl_cg_prep forms the ranges, fl_cg_scatter lists them and fl_cg_setup orders each
+// row's list.
 extern "C" __global__ void __launch_bounds__(FL_T) fl_cg_prep(FlPtrs P) {
   __shared__ uint32_t su[FL_T];
   const uint32_t d = blockIdx.x, t = threadIdx.x;
@@ -1785,58 +1808,44 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_cg_prep(FlPtrs P) {
     S.maxit = D.force ? D.forced_cg : (mode == 1 ? 50u : 200u);
   }
   if (mode == 1) return;
-  // groups per fluid particle, in group order (ball, then slot)
-  uint32_t* cnt = P.fcnt + D.f0;
   uint32_t* off = P.fgoff + D.f0 + d;
-  uint32_t* cur = P.fcur + D.f0;
-  uint32_t* ids = P.fgids + (size_t)D.b0 * P.gcap;
-  const uint32_t* gcnt = P.gcnt + D.b0;
-  const uint32_t* gj = P.gj + (size_t)D.b0 * P.gcap;
-  for (uint32_t i = t; i < D.n; i += FL_T) cnt[i] = 0;
-  uint32_t* lc = P.locnt + D.b0;
+  wg_scan_array(P.fcnt + D.f0, off, D.n, su);
+  for (uint32_t i = t; i < D.n; i += FL_T) P.fcur[D.f0 + i] = off[i];
   uint32_t* lo = P.looff + D.b0 + d;
-  uint32_t* lr = P.locur + D.b0;
+  wg_scan_array(P.locnt + D.b0, lo, nb, su);
+  for (uint32_t b = t; b < nb; b += FL_T) P.locur[D.b0 + b] = lo[b];
+}
+
+extern "C" __global__ void __launch_bounds__(FL_T) fl_cg_scatter(FlPtrs P, const uint2* __restrict__ btiles) {
+  const uint2 tl = btiles[blockIdx.x];
+  const FlDrum& D = P.drums[tl.x];
+  const uint32_t lane = threadIdx.x & 63u, b = tl.y + (threadIdx.x >> 6);
+  if (b >= D.nb || !(D.mu > 0.0f) || P.st[tl.x].mode != 2) return;
+  const uint32_t ng = P.gcnt[D.b0 + b];
+  uint32_t* ids = P.fgids + (size_t)D.b0 * P.gcap;
+  const uint32_t* gj = P.gj + (size_t)D.b0 * P.gcap;
+  for (uint32_t k = lane; k < ng; k += 64) {
+    const uint32_t id = b * P.gcap + k;
+    ids[atomicAdd(&P.fcur[D.f0 + gj[id]], 1u)] = id;
+  }
+  const uint32_t nl = P.lcnt[D.b0 + b];
   uint32_t* lids = P.loids + D.l0;
-  for (uint32_t b = t; b < nb; b += FL_T) lc[b] = 0;
-  __syncthreads();
-  for (uint32_t b = t; b < nb; b += FL_T) {
-    for (uint32_t k = 0; k < gcnt[b]; k++) atomicAdd(&cnt[gj[(size_t)b * P.gcap + k]], 1u);
-    const uint32_t nl = P.lcnt[D.b0 + b];
-    for (uint32_t k = 0; k < nl; k++) {
-      const uint32_t o = __float_as_uint(P.lrec[D.l0 + (size_t)b * D.lcap + k].w);
-      if (o != FL_NO) atomicAdd(&lc[o], 1u);
-    }
+  for (uint32_t k = lane; k < nl; k += 64) {
+    const uint32_t o = __float_as_uint(P.lrec[D.l0 + (size_t)b * D.lcap + k].w);
+    if (o != FL_NO) lids[atomicAdd(&P.locur[D.b0 + o], 1u)] = b * D.lcap + k;
   }
-  __syncthreads();
-  wg_scan_array(cnt, off, D.n, su);
-  wg_scan_array(lc, lo, nb, su);
-  for (uint32_t i = t; i < D.n; i += FL_T) cur[i] = off[i];
-  for (uint32_t b = t; b < nb; b += FL_T) lr[b] = lo[b];
-  __syncthreads();
-  for (uint32_t b = t; b < nb; b += FL_T) {
-    for (uint32_t k = 0; k < gcnt[b]; k++) {
-      const uint32_t id = b * P.gcap + k;
-      ids[atomicAdd(&cur[gj[id]], 1u)] = id;
+}
+
+// Ascending order of v[s0..s1) (one lane's short list).
+__device__ inline void sort_range(uint32_t* v, uint32_t s0, uint32_t s1) {
+  for (uint32_t a = s0 + 1; a < s1; a++) {
+    const uint32_t x = v[a];
+    uint32_t c = a;
+    while (c > s0 && v[c - 1] > x) {
+      v[c] = v[c - 1];
+      c--;
     }
-    const uint32_t nl = P.lcnt[D.b0 + b];
-    for (uint32_t k = 0; k < nl; k++) {
-      const uint32_t o = __float_as_uint(P.lrec[D.l0 + (size_t)b * D.lcap + k].w);
-      if (o != FL_NO) lids[atomicAdd(&lr[o], 1u)] = b * D.lcap + k;
-    }
-  }
-  __syncthreads();
-  for (uint32_t i = t; i < D.n + nb; i += FL_T) {
-    uint32_t* v = i < D.n ? ids : lids;
-    const uint32_t s0 = i < D.n ? off[i] : lo[i - D.n], s1 = i < D.n ? off[i + 1] : lo[i - D.n + 1];
-    for (uint32_t a = s0 + 1; a < s1; a++) {
-      const uint32_t x = v[a];
-      uint32_t c = a;
-      while (c > s0 && v[c - 1] > x) {
-        v[c] = v[c - 1];
-        c--;
-      }
-      v[c] = x;
-    }
+    v[c] = x;
   }
 }
 
@@ -1861,9 +1870,11 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_cg_setup(FlPtrs P) {
     }
     float dg = P.diagf[D.f0 + r];
     const uint32_t* off = P.fgoff + D.f0 + tl.x;
-    const uint32_t* ids = P.fgids + (size_t)D.b0 * P.gcap;
+    uint32_t* ids = P.fgids + (size_t)D.b0 * P.gcap;
     const float4* gc = P.gc + (size_t)D.b0 * P.gcap;
-    for (uint32_t a = off[r]; a < off[r + 1]; a++) dg += kd * gc[ids[a]].x;
+    const uint32_t s0 = off[r], s1 = off[r + 1];
+    sort_range(ids, s0, s1);
+    for (uint32_t a = s0; a < s1; a++) dg += kd * gc[ids[a]].x;
     P.inv[ro] = 1.0f / fmaxf(dg, 1e-20f);
     float2 x0 = vi;
     if (D.has_guess) {
@@ -1882,8 +1893,10 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_cg_setup(FlPtrs P) {
     float dg = D.bmass;
     for (uint32_t k = 0; k < ng; k++) dg += kd * P.gc[gb + k].x;
     const uint32_t* lo = P.looff + D.b0 + tl.x;
-    const uint32_t* lids = P.loids + D.l0;
-    for (uint32_t a = lo[b]; a < lo[b + 1]; a++) dg += D.dt * P.lrec[D.l0 + lids[a]].z;
+    uint32_t* lids = P.loids + D.l0;
+    const uint32_t s0 = lo[b], s1 = lo[b + 1];
+    sort_range(lids, s0, s1);
+    for (uint32_t a = s0; a < s1; a++) dg += D.dt * P.lrec[D.l0 + lids[a]].z;
     const uint32_t nl = P.lcnt[D.b0 + b];
     for (uint32_t k = 0; k < nl; k++) dg += D.dt * P.lrec[D.l0 + (size_t)b * D.lcap + k].z;
     P.inv[ro] = 1.0f / fmaxf(dg, 1e-20f);
@@ -2688,10 +2701,16 @@ int mg_fluid_step(mg_fluid* h, const mg_fluid_drum* drums, float* d_x, float* d_
     }
     MG_TIMED_LAUNCH(h->tm, fl_push, dim3(nft), T, 0, P);
   }
-  MG_TIMED_LAUNCH(h->tm, fl_push_sum, G, T, 0, P);
+  MG_TIMED_LAUNCH(h->tm, fl_push_scan, G, T, 0, P);
+  if (nft) MG_TIMED_LAUNCH(h->tm, fl_push_scatter, dim3(nft), T, 0, P);
+  if (nlt) MG_TIMED_LAUNCH(h->tm, fl_push_fin, dim3(nlt), T, 0, P, (const uint2*)h->d_lt);
   if (nft) MG_TIMED_LAUNCH(h->tm, fl_cohesion, dim3(nft), T, 0, P);
-  if (nwt) MG_TIMED_LAUNCH(h->tm, fl_adhesion, dim3(nwt), T, 0, P, (const uint2*)h->d_wt);
-  MG_TIMED_LAUNCH(h->tm, fl_adh_sum, G, T, 0, P);
+  if (nwt) {
+    MG_TIMED_LAUNCH(h->tm, fl_adhesion, dim3(nwt), T, 0, P, (const uint2*)h->d_wt);
+    MG_TIMED_LAUNCH(h->tm, fl_adh_scan, G, T, 0, P);
+    MG_TIMED_LAUNCH(h->tm, fl_adh_scatter, dim3(nwt), T, 0, P, (const uint2*)h->d_wt);
+  }
+  if (nft) MG_TIMED_LAUNCH(h->tm, fl_adh_fin, dim3(nft), T, 0, P);
   if (nft) {
     MG_TIMED_LAUNCH(h->tm, fl_wall, dim3(nft), T, 0, P);
     MG_TIMED_LAUNCH(h->tm, fl_vel, dim3(nft), T, 0, P);
@@ -2717,6 +2736,7 @@ int mg_fluid_step(mg_fluid* h, const mg_fluid_drum* drums, float* d_x, float* d_
   if (nwt) MG_TIMED_LAUNCH(h->tm, fl_groups, dim3(nwt), T, 0, P, (const uint2*)h->d_wt);
   if (nlt) MG_TIMED_LAUNCH(h->tm, fl_links, dim3(nlt), T, 0, P, (const uint2*)h->d_lt);
   MG_TIMED_LAUNCH(h->tm, fl_cg_prep, G, T, 0, P);
+  if (nwt && nrt) MG_TIMED_LAUNCH(h->tm, fl_cg_scatter, dim3(nwt), T, 0, P, (const uint2*)h->d_wt);
   if (nrt) {
     MG_TIMED_LAUNCH(h->tm, fl_cg_setup, dim3(nrt), T, 0, P);
     MG_TIMED_LAUNCH(h->tm, fl_cg_apply, dim3(nrt), T, 0, P, 0);
