@@ -523,6 +523,85 @@ __device__ inline double wave_sum_d(double v) {
   return __shfl(v, 0, 64);
 }
 
+// Wave-local memory ordering for LDS written by some lanes and read by others of the same wave.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Ordering of the gathered lists. A counting sort's scatter lists each target's entries in no
+// particular order; every list holds distinct ids and is put in ascending id order, which is the
+// reference's order.
+#define FL_WCAP 256  // entries a wave orders through LDS
+
+// By one lane, insertion in memory (the fallback for long lists).
+__device__ inline void sort_insert(uint32_t* v, uint32_t s0, uint32_t s1) {
+  for (uint32_t a = s0 + 1; a < s1; a++) {
+    const uint32_t x = v[a];
+    uint32_t c = a;
+    while (c > s0 && v[c - 1] > x) {
+      v[c] = v[c - 1];
+      c--;
+    }
+    v[c] = x;
+  }
+}
+// By one lane, m <= N entries: all loaded at once, each entry's rank counted in registers, each
+// stored at its rank (no chain of dependent memory accesses).
+template <int N>
+__device__ inline void sort_ranks(uint32_t* v, uint32_t s0, uint32_t m) {
+  uint32_t a[N];
+#pragma unroll
+  for (int q = 0; q < N; q++) a[q] = (uint32_t)q < m ? v[s0 + q] : FL_NO;
+#pragma unroll
+  for (int q = 0; q < N; q++) {
+    uint32_t r = 0;
+#pragma unroll
+    for (int u = 0; u < N; u++) r += a[u] < a[q] ? 1u : 0u;
+    if ((uint32_t)q < m) v[s0 + r] = a[q];
+  }
+}
+__device__ inline void sort_lane(uint32_t* v, uint32_t s0, uint32_t s1) {
+  const uint32_t m = s1 - s0;
+  if (m <= 1) return;
+  if (m <= 8) sort_ranks<8>(v, s0, m);
+  else if (m <= 32) sort_ranks<32>(v, s0, m);
+  else sort_insert(v, s0, s1);
+}
+// Ranks of a wave's list of m <= FL_WCAP distinct ids: lane L holds entries L + 64 q in mine[q]
+// (FL_NO past the end); sb is the wave's LDS buffer. Every lane of the wave calls it.
+__device__ inline void wave_ranks(const uint32_t* v, uint32_t s0, uint32_t m, uint32_t* sb, uint32_t lane,
+                                  uint32_t (&mine)[FL_WCAP / 64], uint32_t (&rank)[FL_WCAP / 64]) {
+#pragma unroll
+  for (int q = 0; q < FL_WCAP / 64; q++) {
+    const uint32_t k = lane + 64u * q;
+    mine[q] = k < m ? v[s0 + k] : FL_NO;
+    if (k < m) sb[k] = mine[q];
+    rank[q] = 0;
+  }
+  wave_sync();
+  for (uint32_t u = 0; u < m; u++) {
+    const uint32_t o = sb[u];
+#pragma unroll
+    for (int q = 0; q < FL_WCAP / 64; q++) rank[q] += o < mine[q] ? 1u : 0u;
+  }
+}
+// By one wave (uniform arguments): up to FL_WCAP entries by their ranks, longer lists by lane 0.
+__device__ inline void wave_sort(uint32_t* v, uint32_t s0, uint32_t s1, uint32_t* sb, uint32_t lane) {
+  const uint32_t m = s1 - s0;
+  if (m <= 1) return;
+  if (m > FL_WCAP) {
+    if (lane == 0) sort_insert(v, s0, s1);
+    return;
+  }
+  uint32_t mine[FL_WCAP / 64], rank[FL_WCAP / 64];
+  wave_ranks(v, s0, m, sb, lane, mine, rank);
+#pragma unroll
+  for (int q = 0; q < FL_WCAP / 64; q++)
+    if (lane + 64u * q < m) v[s0 + rank[q]] = mine[q];
+}
+
 }  // namespace
 
 // ================================================================================================
@@ -791,32 +870,45 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_push_scatter(FlPtrs P) {
   }
 }
 
-extern "C" __global__ void __launch_bounds__(FL_T) fl_push_fin(FlPtrs P, const uint2* __restrict__ ltiles) {
-  const uint2 tl = ltiles[blockIdx.x];
+extern "C" __global__ void __launch_bounds__(FL_T) fl_push_fin(FlPtrs P, const uint2* __restrict__ btiles) {
+  __shared__ uint32_t sid[4][FL_WCAP];
+  __shared__ float4 srec[4][FL_WCAP];
+  const uint2 tl = btiles[blockIdx.x];
   const FlDrum& D = P.drums[tl.x];
-  const uint32_t b = tl.y + threadIdx.x;
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6, b = tl.y + w;
   if (b >= D.nb) return;
   uint32_t* ids = P.pids + (size_t)D.f0 * FL_PK;
+  const float4* rec = P.prec + (size_t)D.f0 * FL_PK;
   const uint32_t* boff = P.boff + D.b0 + tl.x;
-  const uint32_t s0 = boff[b], s1 = boff[b + 1];
-  for (uint32_t a = s0 + 1; a < s1; a++) {  // particle order
-    const uint32_t v = ids[a];
-    uint32_t c = a;
-    while (c > s0 && ids[c - 1] > v) {
-      ids[c] = ids[c - 1];
-      c--;
-    }
-    ids[c] = v;
-  }
+  const uint32_t s0 = boff[b], s1 = boff[b + 1], m = s1 - s0;
   float ix = 0.0f, iy = 0.0f, an = 0.0f;
-  for (uint32_t a = s0; a < s1; a++) {
-    const float4 r = P.prec[(size_t)D.f0 * FL_PK + ids[a]];
-    ix += r.x;
-    iy += r.y;
-    an += r.z;
+  if (m <= FL_WCAP) {  // the records placed in LDS at their ranks, then summed in particle order
+    uint32_t mine[FL_WCAP / 64], rank[FL_WCAP / 64];
+    wave_ranks(ids, s0, m, sid[w], lane, mine, rank);
+#pragma unroll
+    for (int q = 0; q < FL_WCAP / 64; q++)
+      if (lane + 64u * q < m) srec[w][rank[q]] = rec[mine[q]];
+    wave_sync();
+    if (lane == 0)
+      for (uint32_t a = 0; a < m; a++) {
+        const float4 r = srec[w][a];
+        ix += r.x;
+        iy += r.y;
+        an += r.z;
+      }
+  } else if (lane == 0) {
+    sort_insert(ids, s0, s1);
+    for (uint32_t a = s0; a < s1; a++) {
+      const float4 r = rec[ids[a]];
+      ix += r.x;
+      iy += r.y;
+      an += r.z;
+    }
   }
-  P.imp[D.b0 + b] = make_float2(ix, iy);
-  P.ang[D.b0 + b] = an;
+  if (lane == 0) {
+    P.imp[D.b0 + b] = make_float2(ix, iy);
+    P.ang[D.b0 + b] = an;
+  }
 }
 
 // 3.6a cohesion (pbf.rs:1464-1491) over the neighbour lists at the pushed positions.
@@ -985,15 +1077,7 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_adh_fin(FlPtrs P) {
     const float2* adv = P.adv + (size_t)D.b0 * P.acap;
     const uint32_t* off = P.foff + D.f0 + tl.x;
     const uint32_t s0 = off[i], s1 = off[i + 1];
-    for (uint32_t a = s0 + 1; a < s1; a++) {  // ball order
-      const uint32_t v = ids[a];
-      uint32_t c = a;
-      while (c > s0 && ids[c - 1] > v) {
-        ids[c] = ids[c - 1];
-        c--;
-      }
-      ids[c] = v;
-    }
+    sort_lane(ids, s0, s1);  // ball order
     for (uint32_t a = s0; a < s1; a++) {
       const float2 q = adv[ids[a]];
       sx += q.x;
@@ -1139,13 +1223,6 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_grid_b(FlPtrs P) {
 // held in LDS when they fit) then walks the balls in order: gather the items' velocities, the
 // three velocity sums in item order (one per lane), the ball's scalar update, the kicks.
 #define GS_ITEMS 224
-
-// Wave-local memory ordering for LDS written by some lanes and read by others of the same wave.
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // Per-ball items and velocity-independent sums, one wave per ball.
 extern "C" __global__ void __launch_bounds__(FL_T) fl_gs_pre(FlPtrs P, const uint2* __restrict__ btiles) {
@@ -1707,18 +1784,19 @@ __device__ inline double film_coef(const double* f, double h) {
   return f[3] / (h * sqrt(h)) * (1.0 + f[1] * eps + f[2] * eps * eps);
 }
 
-// 7d squeeze links (pbf.rs:544-625), one lane per ball: approaching ball pairs within the reach
-// (the partner after the ball) and the wall, with the velocities after the earlier impulses.
-extern "C" __global__ void __launch_bounds__(FL_T) fl_links(FlPtrs P, const uint2* __restrict__ ltiles) {
-  const uint2 tl = ltiles[blockIdx.x];
+// 7d squeeze links (pbf.rs:544-625), one wave per ball: approaching ball pairs within the reach
+// (the partner after the ball; the lanes take 64 candidates at a time and the links keep the
+// candidates' order) and the wall, with the velocities after the earlier impulses.
+extern "C" __global__ void __launch_bounds__(FL_T) fl_links(FlPtrs P, const uint2* __restrict__ btiles) {
+  const uint2 tl = btiles[blockIdx.x];
   const FlDrum& D = P.drums[tl.x];
-  const uint32_t i = tl.y + threadIdx.x;
+  const uint32_t lane = threadIdx.x & 63u, i = tl.y + (threadIdx.x >> 6);
   if (i >= D.nb) return;
   uint32_t* lcnt = P.lcnt + D.b0;
   const float2* bx = P.bx + D.b0;
   const float2 xi = bx[i];
   if (!D.links_on || D.n == 0 || !fin2(xi)) {
-    lcnt[i] = 0;
+    if (lane == 0) lcnt[i] = 0;
     return;
   }
   const float2* bv = P.bv + D.b0;
@@ -1727,51 +1805,63 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_links(FlPtrs P, const uint
   const float2 vi0 = make_float2(bv[i].x + imp[i].x / D.bmass, bv[i].y + imp[i].y / D.bmass);
   float4* out = P.lrec + D.l0 + (size_t)i * D.lcap;
   uint32_t cnt = 0, inr = 0;
-  bool over = false;
   Near q;
   near_cells(g, xi, q);
-  for (uint32_t c = 0; c < q.total; c++) {
-    const uint32_t j = near_at(g, q, c);
-    if (j <= i) continue;
-    const float2 xj = bx[j];
-    const float ex = xj.x - xi.x, ey = xj.y - xi.y;
-    const float e2 = ex * ex + ey * ey;
-    if (e2 >= D.rng2 || !(__builtin_isfinite(ex) && __builtin_isfinite(ey))) continue;
-    const float dist = sqrtf(e2);
-    const float gap = dist - 2.0f * D.br;
-    if (gap >= D.reach || dist < 1e-9f) continue;
-    inr++;
-    const float nx = ex / dist, ny = ey / dist;
-    const float2 vj0 = make_float2(bv[j].x + imp[j].x / D.bmass, bv[j].y + imp[j].y / D.bmass);
-    if (nx * (vi0.x - vj0.x) + ny * (vi0.y - vj0.y) <= 0.0f) continue;
-    const double wgt = 1.0 - resolved_ratio((double)gap / D.dxd);
-    const double hh = fmax((double)gap, D.a2);
-    const double cf = wgt * film_coef(D.sqp, hh);
-    if (cnt < D.lcap) {
-      out[cnt] = make_float4(nx, ny, fminf(1.0f * (float)cf, D.c_max), __uint_as_float(j));
-      atomicAdd(&P.locnt[D.b0 + j], 1u);
-    } else {
-      over = true;
-    }
-    cnt++;
-  }
-  {
-    const float dist = sqrtf(xi.x * xi.x + xi.y * xi.y);
-    const float gap = D.radius - D.br - dist;
-    if (gap < D.reach && dist > 1e-9f) {
-      inr++;
-      const float nx = xi.x / dist, ny = xi.y / dist;
-      if (!(nx * vi0.x + ny * vi0.y <= 0.0f)) {
-        const double wgt = 1.0 - resolved_ratio((double)gap / D.dxd);
-        const double hh = fmax((double)gap, D.a2);
-        const double cf = wgt * film_coef(D.sqw, hh);
-        if (cnt < D.lcap) out[cnt] = make_float4(nx, ny, fminf(1.0f * (float)cf, D.c_max), __uint_as_float(FL_NO));
-        else over = true;
-        cnt++;
+  for (uint32_t c0 = 0; c0 < q.total; c0 += 64) {
+    const uint32_t c = c0 + lane;
+    bool in = false, emit = false;
+    uint32_t j = 0;
+    float4 rec = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (c < q.total) {
+      j = near_at(g, q, c);
+      if (j > i) {
+        const float2 xj = bx[j];
+        const float ex = xj.x - xi.x, ey = xj.y - xi.y;
+        const float e2 = ex * ex + ey * ey;
+        if (!(e2 >= D.rng2 || !(__builtin_isfinite(ex) && __builtin_isfinite(ey)))) {
+          const float dist = sqrtf(e2);
+          const float gap = dist - 2.0f * D.br;
+          if (!(gap >= D.reach || dist < 1e-9f)) {
+            in = true;
+            const float nx = ex / dist, ny = ey / dist;
+            const float2 vj0 = make_float2(bv[j].x + imp[j].x / D.bmass, bv[j].y + imp[j].y / D.bmass);
+            if (!(nx * (vi0.x - vj0.x) + ny * (vi0.y - vj0.y) <= 0.0f)) {
+              emit = true;
+              const double wgt = 1.0 - resolved_ratio((double)gap / D.dxd);
+              const double hh = fmax((double)gap, D.a2);
+              const double cf = wgt * film_coef(D.sqp, hh);
+              rec = make_float4(nx, ny, fminf(1.0f * (float)cf, D.c_max), __uint_as_float(j));
+            }
+          }
+        }
       }
     }
+    const uint64_t mi = __ballot(in), me = __ballot(emit);
+    inr += (uint32_t)__popcll(mi);
+    if (emit) {
+      const uint32_t pos = cnt + (uint32_t)__popcll(me & ((1ull << lane) - 1ull));
+      if (pos < D.lcap) {
+        out[pos] = rec;
+        atomicAdd(&P.locnt[D.b0 + j], 1u);
+      }
+    }
+    cnt += (uint32_t)__popcll(me);
   }
-  if (over) atomicOr(&P.st[tl.x].err, FE_LINKS);
+  if (lane != 0) return;
+  const float dist = sqrtf(xi.x * xi.x + xi.y * xi.y);
+  const float gap = D.radius - D.br - dist;
+  if (gap < D.reach && dist > 1e-9f) {
+    inr++;
+    const float nx = xi.x / dist, ny = xi.y / dist;
+    if (!(nx * vi0.x + ny * vi0.y <= 0.0f)) {
+      const double wgt = 1.0 - resolved_ratio((double)gap / D.dxd);
+      const double hh = fmax((double)gap, D.a2);
+      const double cf = wgt * film_coef(D.sqw, hh);
+      if (cnt < D.lcap) out[cnt] = make_float4(nx, ny, fminf(1.0f * (float)cf, D.c_max), __uint_as_float(FL_NO));
+      cnt++;
+    }
+  }
+  if (cnt > D.lcap) atomicOr(&P.st[tl.x].err, FE_LINKS);
   lcnt[i] = min(cnt, D.lcap);
   if (inr) atomicAdd(&P.st[tl.x].in_range, inr);
 }
@@ -1836,17 +1926,16 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_cg_scatter(FlPtrs P, const
   }
 }
 
-// Ascending order of v[s0..s1) (one lane's short list).
-__device__ inline void sort_range(uint32_t* v, uint32_t s0, uint32_t s1) {
-  for (uint32_t a = s0 + 1; a < s1; a++) {
-    const uint32_t x = v[a];
-    uint32_t c = a;
-    while (c > s0 && v[c - 1] > x) {
-      v[c] = v[c - 1];
-      c--;
-    }
-    v[c] = x;
-  }
+// The joint solve's ball rows: each ball's list of links naming it as partner in link order, one
+// wave per ball.
+extern "C" __global__ void __launch_bounds__(FL_T) fl_cg_order(FlPtrs P, const uint2* __restrict__ btiles) {
+  __shared__ uint32_t sid[4][FL_WCAP];
+  const uint2 tl = btiles[blockIdx.x];
+  const FlDrum& D = P.drums[tl.x];
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6, b = tl.y + w;
+  if (b >= D.nb || !(D.mu > 0.0f) || P.st[tl.x].mode != 2) return;
+  const uint32_t* lo = P.looff + D.b0 + tl.x;
+  wave_sort(P.loids + D.l0, lo[b], lo[b + 1], sid[w], lane);
 }
 
 // Rows of a drum's viscosity system: fluid i < n, ball linear n + b, ball angular n + nb + b.
@@ -1873,7 +1962,7 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_cg_setup(FlPtrs P) {
     uint32_t* ids = P.fgids + (size_t)D.b0 * P.gcap;
     const float4* gc = P.gc + (size_t)D.b0 * P.gcap;
     const uint32_t s0 = off[r], s1 = off[r + 1];
-    sort_range(ids, s0, s1);
+    sort_lane(ids, s0, s1);
     for (uint32_t a = s0; a < s1; a++) dg += kd * gc[ids[a]].x;
     P.inv[ro] = 1.0f / fmaxf(dg, 1e-20f);
     float2 x0 = vi;
@@ -1885,32 +1974,129 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_cg_setup(FlPtrs P) {
     P.B[ro] = make_float2(D.mass * vi.x, D.mass * vi.y);
     return;
   }
-  const bool lin = r < D.n + D.nb;
-  const uint32_t b = lin ? r - D.n : r - D.n - D.nb;
+  // ball rows: fl_cg_balls
+}
+
+// The joint solve's ball rows, one wave per ball (the rows' sums run over the ball's boundary
+// group items and squeeze links; the lanes form 64 terms at a time and the sums take them in
+// order). what = 0: initial guess, right-hand side and Jacobi diagonal (pbf.rs:651-699); 1, 2:
+// A times X (before the iterations) or P (pbf.rs:700-744) into AP.
+extern "C" __global__ void __launch_bounds__(FL_T) fl_cg_balls(FlPtrs P, const uint2* __restrict__ btiles, int what) {
+  const uint2 tl = btiles[blockIdx.x];
+  const FlDrum& D = P.drums[tl.x];
+  const FlState& S = P.st[tl.x];
+  const uint32_t lane = threadIdx.x & 63u, b = tl.y + (threadIdx.x >> 6);
+  if (b >= D.nb || S.mode != 2 || S.done) return;
+  const uint32_t n = D.n, nb = D.nb;
+  const size_t rb = (size_t)D.f0 + 2 * (size_t)D.b0;
   const size_t gb = ((size_t)D.b0 + b) * P.gcap;
   const uint32_t ng = P.gcnt[D.b0 + b];
-  if (lin) {
-    float dg = D.bmass;
-    for (uint32_t k = 0; k < ng; k++) dg += kd * P.gc[gb + k].x;
-    const uint32_t* lo = P.looff + D.b0 + tl.x;
-    uint32_t* lids = P.loids + D.l0;
-    const uint32_t s0 = lo[b], s1 = lo[b + 1];
-    sort_range(lids, s0, s1);
-    for (uint32_t a = s0; a < s1; a++) dg += D.dt * P.lrec[D.l0 + lids[a]].z;
-    const uint32_t nl = P.lcnt[D.b0 + b];
-    for (uint32_t k = 0; k < nl; k++) dg += D.dt * P.lrec[D.l0 + (size_t)b * D.lcap + k].z;
-    P.inv[ro] = 1.0f / fmaxf(dg, 1e-20f);
-    const float2 bvb = P.bv[D.b0 + b], ib = P.imp[D.b0 + b];
-    const float2 v0 = make_float2(bvb.x + ib.x / D.bmass, bvb.y + ib.y / D.bmass);
-    P.X[ro] = v0;
-    P.B[ro] = make_float2(D.bmass * v0.x, D.bmass * v0.y);
-  } else {
-    float dg = D.binertia;
-    for (uint32_t k = 0; k < ng; k++) dg += kd * P.gc[gb + k].w;
-    P.inv[ro] = 1.0f / fmaxf(dg, 1e-20f);
-    const float w0 = P.bom[D.b0 + b] + P.ang[D.b0 + b] / D.binertia;
-    P.X[ro] = make_float2(w0, 0.0f);
-    P.B[ro] = make_float2(D.binertia * w0, 0.0f);
+  const uint32_t* lo = P.looff + D.b0 + tl.x;
+  const uint32_t* lids = P.loids + D.l0;
+  const uint32_t i0 = lo[b], i1 = lo[b + 1];
+  const uint32_t nl = P.lcnt[D.b0 + b];
+  const float4* lrec = P.lrec + D.l0;
+  const float kd = D.dt * D.mass;
+  if (what == 0) {
+    float dl = D.bmass, da = D.binertia;
+    for (uint32_t c0 = 0; c0 < ng; c0 += 64) {
+      float tg = 0.0f, ta = 0.0f;
+      if (c0 + lane < ng) {
+        const float4 q = P.gc[gb + c0 + lane];
+        tg = kd * q.x;
+        ta = kd * q.w;
+      }
+      const uint32_t lim = min(64u, ng - c0);
+      for (uint32_t L = 0; L < lim; L++) {
+        dl += rdl(tg, (int)L);
+        da += rdl(ta, (int)L);
+      }
+    }
+    for (uint32_t c0 = i0; c0 < i1; c0 += 64) {  // links of earlier balls to this one
+      const float t = c0 + lane < i1 ? D.dt * lrec[lids[c0 + lane]].z : 0.0f;
+      const uint32_t lim = min(64u, i1 - c0);
+      for (uint32_t L = 0; L < lim; L++) dl += rdl(t, (int)L);
+    }
+    for (uint32_t c0 = 0; c0 < nl; c0 += 64) {
+      const float t = c0 + lane < nl ? D.dt * lrec[(size_t)b * D.lcap + c0 + lane].z : 0.0f;
+      const uint32_t lim = min(64u, nl - c0);
+      for (uint32_t L = 0; L < lim; L++) dl += rdl(t, (int)L);
+    }
+    if (lane == 0) {
+      P.inv[rb + n + b] = 1.0f / fmaxf(dl, 1e-20f);
+      const float2 bvb = P.bv[D.b0 + b], ib = P.imp[D.b0 + b];
+      const float2 v0 = make_float2(bvb.x + ib.x / D.bmass, bvb.y + ib.y / D.bmass);
+      P.X[rb + n + b] = v0;
+      P.B[rb + n + b] = make_float2(D.bmass * v0.x, D.bmass * v0.y);
+      P.inv[rb + n + nb + b] = 1.0f / fmaxf(da, 1e-20f);
+      const float w0 = P.bom[D.b0 + b] + P.ang[D.b0 + b] / D.binertia;
+      P.X[rb + n + nb + b] = make_float2(w0, 0.0f);
+      P.B[rb + n + nb + b] = make_float2(D.binertia * w0, 0.0f);
+    }
+    return;
+  }
+  const float2* V = (what == 2 ? P.P : P.X) + rb;
+  const float2 vb = V[n + b];
+  const float w = V[n + nb + b].x;
+  float lx = D.bmass * vb.x, ly = D.bmass * vb.y, ax = D.binertia * w;
+  for (uint32_t c0 = 0; c0 < ng; c0 += 64) {
+    float tx = 0.0f, ty = 0.0f, ta = 0.0f;
+    if (c0 + lane < ng) {
+      const float4 q = P.gc[gb + c0 + lane];
+      const float2 vf = V[P.gj[gb + c0 + lane]];
+      const float rx = vf.x - vb.x, ry = vf.y - vb.y;
+      tx = kd * (q.x * rx - w * -q.z);
+      ty = kd * (q.x * ry - w * q.y);
+      ta = kd * ((q.y * ry - q.z * rx) - w * q.w);
+    }
+    const uint32_t lim = min(64u, ng - c0);
+    for (uint32_t L = 0; L < lim; L++) {
+      lx -= rdl(tx, (int)L);
+      ly -= rdl(ty, (int)L);
+      ax -= rdl(ta, (int)L);
+    }
+  }
+  for (uint32_t c0 = i0; c0 < i1; c0 += 64) {  // links of earlier balls to this one
+    float tx = 0.0f, ty = 0.0f;
+    if (c0 + lane < i1) {
+      const uint32_t id = lids[c0 + lane];
+      const float4 l = lrec[id];
+      const float2 va = V[n + id / D.lcap];
+      const float rx = va.x - vb.x, ry = va.y - vb.y;
+      const float f = D.dt * l.z * (l.x * rx + l.y * ry);
+      tx = f * l.x;
+      ty = f * l.y;
+    }
+    const uint32_t lim = min(64u, i1 - c0);
+    for (uint32_t L = 0; L < lim; L++) {
+      lx -= rdl(tx, (int)L);
+      ly -= rdl(ty, (int)L);
+    }
+  }
+  for (uint32_t c0 = 0; c0 < nl; c0 += 64) {
+    float tx = 0.0f, ty = 0.0f;
+    if (c0 + lane < nl) {
+      const float4 l = lrec[(size_t)b * D.lcap + c0 + lane];
+      const uint32_t o = __float_as_uint(l.w);
+      float rx = vb.x, ry = vb.y;
+      if (o != FL_NO) {
+        const float2 vo = V[n + o];
+        rx = rx - vo.x;
+        ry = ry - vo.y;
+      }
+      const float f = D.dt * l.z * (l.x * rx + l.y * ry);
+      tx = f * l.x;
+      ty = f * l.y;
+    }
+    const uint32_t lim = min(64u, nl - c0);
+    for (uint32_t L = 0; L < lim; L++) {
+      lx += rdl(tx, (int)L);
+      ly += rdl(ty, (int)L);
+    }
+  }
+  if (lane == 0) {
+    P.AP[rb + n + b] = make_float2(lx, ly);
+    P.AP[rb + n + nb + b] = make_float2(ax, 0.0f);
   }
 }
 
@@ -1962,62 +2148,12 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_cg_apply(FlPtrs P, int ite
           oy += kd * (q.x * ry - w * q.y);
         }
       }
-    } else if (r < n + nb) {
-      const uint32_t b = r - n;
-      const float2 vb = V[r];
-      const float w = V[n + nb + b].x;
-      ox = D.bmass * vb.x;
-      oy = D.bmass * vb.y;
-      const size_t gb = ((size_t)D.b0 + b) * P.gcap;
-      const uint32_t ng = P.gcnt[D.b0 + b];
-      for (uint32_t k = 0; k < ng; k++) {
-        const float4 q = P.gc[gb + k];
-        const float2 vf = V[P.gj[gb + k]];
-        const float rx = vf.x - vb.x, ry = vf.y - vb.y;
-        ox -= kd * (q.x * rx - w * -q.z);
-        oy -= kd * (q.x * ry - w * q.y);
-      }
-      const uint32_t* lo = P.looff + D.b0 + tl.x;
-      const uint32_t* lids = P.loids + D.l0;
-      for (uint32_t a = lo[b]; a < lo[b + 1]; a++) {  // links of earlier balls to this one
-        const uint32_t id = lids[a];
-        const float4 l = P.lrec[D.l0 + id];
-        const float2 va = V[n + id / D.lcap];
-        const float rx = va.x - vb.x, ry = va.y - vb.y;
-        const float f = D.dt * l.z * (l.x * rx + l.y * ry);
-        ox -= f * l.x;
-        oy -= f * l.y;
-      }
-      const uint32_t nl = P.lcnt[D.b0 + b];
-      for (uint32_t k = 0; k < nl; k++) {
-        const float4 l = P.lrec[D.l0 + (size_t)b * D.lcap + k];
-        const uint32_t o = __float_as_uint(l.w);
-        float rx = vb.x, ry = vb.y;
-        if (o != FL_NO) {
-          const float2 vo = V[n + o];
-          rx = rx - vo.x;
-          ry = ry - vo.y;
-        }
-        const float f = D.dt * l.z * (l.x * rx + l.y * ry);
-        ox += f * l.x;
-        oy += f * l.y;
-      }
-    } else {
-      const uint32_t b = r - n - nb;
-      const float2 vb = V[n + b];
-      const float w = V[r].x;
-      ox = D.binertia * w;
-      oy = 0.0f;
-      const size_t gb = ((size_t)D.b0 + b) * P.gcap;
-      const uint32_t ng = P.gcnt[D.b0 + b];
-      for (uint32_t k = 0; k < ng; k++) {
-        const float4 q = P.gc[gb + k];
-        const float2 vf = V[P.gj[gb + k]];
-        const float rx = vf.x - vb.x, ry = vf.y - vb.y;
-        ox -= kd * ((q.y * ry - q.z * rx) - w * q.w);
-      }
+    } else {  // ball rows, formed by fl_cg_balls
+      const float2 o = P.AP[rb + r];
+      ox = o.x;
+      oy = o.y;
     }
-    P.AP[rb + r] = make_float2(ox, oy);
+    if (r < n) P.AP[rb + r] = make_float2(ox, oy);
     if (iter) {
       const float2 p = V[r];
       pa = (double)p.x * (double)ox + (double)p.y * (double)oy;
@@ -2703,7 +2839,7 @@ int mg_fluid_step(mg_fluid* h, const mg_fluid_drum* drums, float* d_x, float* d_
   }
   MG_TIMED_LAUNCH(h->tm, fl_push_scan, G, T, 0, P);
   if (nft) MG_TIMED_LAUNCH(h->tm, fl_push_scatter, dim3(nft), T, 0, P);
-  if (nlt) MG_TIMED_LAUNCH(h->tm, fl_push_fin, dim3(nlt), T, 0, P, (const uint2*)h->d_lt);
+  if (nwt) MG_TIMED_LAUNCH(h->tm, fl_push_fin, dim3(nwt), T, 0, P, (const uint2*)h->d_wt);
   if (nft) MG_TIMED_LAUNCH(h->tm, fl_cohesion, dim3(nft), T, 0, P);
   if (nwt) {
     MG_TIMED_LAUNCH(h->tm, fl_adhesion, dim3(nwt), T, 0, P, (const uint2*)h->d_wt);
@@ -2734,11 +2870,20 @@ int mg_fluid_step(mg_fluid* h, const mg_fluid_drum* drums, float* d_x, float* d_
     MG_TIMED_LAUNCH(h->tm, fl_weights, dim3(nft), T, 0, P);
   }
   if (nwt) MG_TIMED_LAUNCH(h->tm, fl_groups, dim3(nwt), T, 0, P, (const uint2*)h->d_wt);
-  if (nlt) MG_TIMED_LAUNCH(h->tm, fl_links, dim3(nlt), T, 0, P, (const uint2*)h->d_lt);
+  if (nwt) MG_TIMED_LAUNCH(h->tm, fl_links, dim3(nwt), T, 0, P, (const uint2*)h->d_wt);
   MG_TIMED_LAUNCH(h->tm, fl_cg_prep, G, T, 0, P);
-  if (nwt && nrt) MG_TIMED_LAUNCH(h->tm, fl_cg_scatter, dim3(nwt), T, 0, P, (const uint2*)h->d_wt);
+  if (nwt && nrt) {
+    MG_TIMED_LAUNCH(h->tm, fl_cg_scatter, dim3(nwt), T, 0, P, (const uint2*)h->d_wt);
+    MG_TIMED_LAUNCH(h->tm, fl_cg_order, dim3(nwt), T, 0, P, (const uint2*)h->d_wt);
+  }
   if (nrt) {
-    MG_TIMED_LAUNCH(h->tm, fl_cg_setup, dim3(nrt), T, 0, P);
+    if (nwt) {
+      MG_TIMED_LAUNCH(h->tm, fl_cg_balls, dim3(nwt), T, 0, P, (const uint2*)h->d_wt, 0);
+      MG_TIMED_LAUNCH(h->tm, fl_cg_setup, dim3(nrt), T, 0, P);
+      MG_TIMED_LAUNCH(h->tm, fl_cg_balls, dim3(nwt), T, 0, P, (const uint2*)h->d_wt, 1);
+    } else {
+      MG_TIMED_LAUNCH(h->tm, fl_cg_setup, dim3(nrt), T, 0, P);
+    }
     MG_TIMED_LAUNCH(h->tm, fl_cg_apply, dim3(nrt), T, 0, P, 0);
     MG_TIMED_LAUNCH(h->tm, fl_cg_init, dim3(nrt), T, 0, P);
     MG_TIMED_LAUNCH(h->tm, fl_cg_start, G, dim3(64), 0, P);
@@ -2746,6 +2891,7 @@ int mg_fluid_step(mg_fluid* h, const mg_fluid_drum* drums, float* d_x, float* d_
     for (uint32_t done_it = 0; done_it < 200;) {
       const uint32_t k = std::min(h->cg_chunk, 200 - done_it);
       for (uint32_t q = 0; q < k; q++) {
+        if (nwt) MG_TIMED_LAUNCH(h->tm, fl_cg_balls, dim3(nwt), T, 0, P, (const uint2*)h->d_wt, 2);
         MG_TIMED_LAUNCH(h->tm, fl_cg_apply, dim3(nrt), T, 0, P, 1);
         MG_TIMED_LAUNCH(h->tm, fl_cg_upd, dim3(nrt), T, 0, P);
         MG_TIMED_LAUNCH(h->tm, fl_cg_fin, FG, dim3(64), 0, P, nd);
