@@ -26,7 +26,8 @@ recompute and the implicit viscosity CG to 1e-6 relative residual, with its own 
 baseline, FP64 and Bingham lines, and the standalone operators (uniform grid, mixing index, drum
 metrics, stress accumulators, DEM steps of 256 populations, free surfaces), each with the
 restatement's rate beside it where one exists.
---part drums|grid|mixing|... times only that part (for a profiler run) and prints its JSON.
+--part drums|ensemble|grid|mixing|... times only that part (for a profiler run) and prints its JSON;
+ensemble is the batched drums' timed substeps alone.
 """
 from __future__ import annotations
 
@@ -580,9 +581,10 @@ def kernel_bytes(name, n_f, n_b, n_d, n_dem):
     return table.get(name)
 
 
-def bench_drums(args, cpu=True) -> dict:
+def bench_drums(args, cpu=True, batch_only=False) -> dict:
     """The batched ensemble (headline), the plain single drums, the power-draw error of the FP32
-    engine against the FP64 oracle, the dominant kernel's roofline and the restatement's rate."""
+    engine against the FP64 oracle, the dominant kernel's roofline and the restatement's rate.
+    batch_only: the ensemble's timed substeps alone (no kernel timers, for a profiler run)."""
     import numpy as np
 
     from milldynamics3_amd.sim import Batch, Simulation
@@ -598,6 +600,11 @@ def bench_drums(args, cpu=True) -> dict:
     b.sync()
     wall = time.perf_counter() - t0
     rate = len(plist) * args.steps / wall
+    if batch_only:
+        out = {"drum_substeps_per_s": round(rate, 1), "ms_per_substep": round(1000.0 * wall / args.steps, 3),
+               "drums": len(plist), "fluid_particles": int(nf.sum()), "balls": int(nb.sum()), "steps": args.steps}
+        b.close()
+        return out
     st = [d.stats() for d in b.drums]
     n_d = np.array([s["density_iterations"] for s in st])
     n_cg = np.array([s["viscosity_iterations"] for s in st])
@@ -814,7 +821,8 @@ def main() -> int:
     ap.add_argument("--precision-substeps", type=int, default=480,
                     help="substeps of the FP32-vs-FP64 power comparison (realtime preset)")
     ap.add_argument("--no-box", action="store_true", help="skip the periodic-box kernel-isolation measurements")
-    ap.add_argument("--part", choices=("all", "drums", "grid", "mixing", "drum_metrics", "stress", "dem", "surface"),
+    ap.add_argument("--part", choices=("all", "drums", "ensemble", "grid", "mixing", "drum_metrics", "stress", "dem",
+                                       "surface"),
                     default="all",
                     help="grid / mixing: time only that operator (for a profiler run) and print its JSON")
     args = ap.parse_args()
@@ -847,7 +855,8 @@ def main() -> int:
                 "stress": lambda: bench_stress(args),
                 "dem": lambda: bench_dem(args, cpu=not args.no_cpu),
                 "surface": lambda: bench_surface(args, cpu=not args.no_cpu),
-                "drums": lambda: bench_drums(args, cpu=not args.no_cpu)}[args.part]()
+                "drums": lambda: bench_drums(args, cpu=not args.no_cpu),
+                "ensemble": lambda: bench_drums(args, cpu=False, batch_only=True)}[args.part]()
         print(json.dumps({"part": args.part, **part}), flush=True)
         return 0
     drums = bench_drums(args, cpu=not args.no_cpu)

@@ -128,6 +128,54 @@ float mg_batch_last_step_ms(const mg_batch* batch);
 int mg_batch_set_timing(mg_batch* batch, int on);
 int mg_batch_kernel_times(mg_batch* batch, char* json, size_t cap);
 
+/* ---- served sweeps: continuous batching of independent drum runs --------------------------------
+ * A request is one drum (Params JSON, validated as mg_sim_create does) run from its seeded initial
+ * state for `revolutions` drum revolutions: ceil(revolutions * 2 pi / |omega| / dt) substeps of its
+ * fixed sub-step dt (lib.rs:426-438). Each tick of the server admits queued requests in arrival
+ * order while a slot is free and the running drums' HBM footprints stay within the budget (the
+ * queue's head waits; no request overtakes it), runs one substep of every running drum as one batch
+ * (the batch form's operators), records each drum's per-revolution results (power and torque EMAs,
+ * collision rate, dissipated power, mixing index, the revolution's impact-energy histogram) and
+ * retires the finished runs. A run's results equal the same drum stepped alone with mg_sim_step_fixed
+ * (drums never exchange data). The reference has no server; its sweep analog runs seeds one after
+ * another (batch.rs:40-106). Times are seconds on the server's clock (a steady clock from create). */
+typedef struct mg_server mg_server;
+typedef struct mg_server_config {
+  uint32_t max_drums;        /* slots: drums stepped together at most (1..4096) */
+  uint32_t max_balls;        /* balls per drum at most (1..2048) */
+  uint64_t hbm_budget_bytes; /* the slots' fixed workspaces plus the running drums' footprints */
+} mg_server_config;
+typedef struct mg_server_status {
+  uint32_t queued, running, done;
+  uint64_t ticks, drum_substeps, used_bytes;
+  double now_s;
+} mg_server_status;
+typedef struct mg_request_result {
+  int32_t state; /* 0 queued, 1 running, 2 done */
+  uint32_t n_balls, n_fluid, revolutions_done;
+  uint64_t substeps_total, substeps_done, footprint_bytes;
+  double arrival_s, start_s, done_s;
+  float power_draw_w, torque_nm, collision_rate_per_s, dissipated_power_w; /* at the last substep run */
+} mg_request_result;
+/* MG_E_CAPACITY if the budget does not hold the slots' fixed workspaces. */
+int mg_server_create(const mg_server_config* config, mg_server** out, char* err, size_t errlen);
+void mg_server_destroy(mg_server* server);
+double mg_server_now(const mg_server* server);
+/* Queues a request; arrival_s < 0 means now (a driver replaying an arrival schedule passes the
+ * scheduled time, so the wait before the call counts in the latency). MG_E_PARAMS for invalid
+ * Params, a drum that does not turn or a non-positive revolution count; MG_E_CAPACITY if the drum
+ * has more balls than max_balls or its footprint alone exceeds the budget. */
+int mg_server_submit(mg_server* server, const char* params_json, double revolutions, double arrival_s, uint64_t* id,
+                     char* err, size_t errlen);
+/* Runs `ticks` ticks (fewer if nothing is queued or running); status may be NULL. */
+int mg_server_poll(mg_server* server, uint32_t ticks, mg_server_status* status, char* err, size_t errlen);
+int mg_server_result(mg_server* server, uint64_t id, mg_request_result* out);
+/* The request's per-revolution records as a JSON array; returns the length, or -(bytes needed)
+ * when cap is too small. */
+int mg_server_stream_json(mg_server* server, uint64_t id, char* buf, size_t cap);
+/* Forgets a finished request (its result and stream). */
+int mg_server_release(mg_server* server, uint64_t id);
+
 const char* mg_version(void);
 
 

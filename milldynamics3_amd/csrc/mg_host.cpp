@@ -17,6 +17,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <deque>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -179,7 +181,14 @@ struct Batch {
   std::vector<mg_fluid_stats> fs;
   std::vector<mg_dem_stats> ds;
   std::vector<mg_stress_step> ss;
-  std::vector<uint32_t> hend;
+  std::vector<uint32_t> hbeg, hend;
+  // The operators' handles may hold more drum slots than the batch has drums (a server's batch
+  // changes between substeps): slots past the drums step empty drums, and drum d's stress
+  // accumulator is slot sslot[d] of the stress handle (its records start at hbeg[sslot[d]]).
+  std::vector<uint32_t> sslot;
+  mg_fluid_drum fd_empty{};
+  mg_dem_drum dd_empty{};
+  bool ranges_dirty = false;
   ~Batch();
 };
 
@@ -341,7 +350,10 @@ Batch* make_batch(std::vector<std::unique_ptr<Drum>>&& drums, std::string& err) 
   B->fs.resize(nd);
   B->ds.resize(nd);
   B->ss.resize(nd);
+  B->hbeg = beg;
   B->hend.resize(nd);
+  B->sslot.resize(nd);
+  for (uint32_t d = 0; d < nd; d++) B->sslot[d] = d;
   Batch* out = B.release();
   for (uint32_t i = 0; i < nd; i++) out->views.push_back(new ::mg_sim{out, i, false});
   return out;
@@ -351,6 +363,15 @@ Batch* make_batch(std::vector<std::unique_ptr<Drum>>&& drums, std::string& err) 
 int substep(Batch& B, const std::vector<float>& dt) {
   const uint32_t nd = (uint32_t)B.drums.size();
   char e[512] = {0};
+  for (size_t d = nd; d < B.fd.size(); d++) {  // empty slots of a server's operators
+    B.fd[d] = B.fd_empty;
+    B.dd[d] = B.dd_empty;
+  }
+  for (size_t q = 0; q < B.ss.size(); q++) {  // stress slots without a drum: no records, no decay
+    B.ss[q] = mg_stress_step{};
+    B.ss[q].sub_dt = 1.0;
+    B.hend[q] = B.hbeg[q];
+  }
   const Drum* forced = nullptr;
   for (uint32_t d = 0; d < nd; d++) {
     const Drum& D = *B.drums[d];
@@ -450,7 +471,8 @@ int substep(Batch& B, const std::vector<float>& dt) {
     const float tinst = std::fabs(omega) > 1e-6f ? pinst / omega : 0.0f;
     D.torque += alpha * (tinst - D.torque);
     const double rr = (double)D.br;
-    mg_stress_step& s = B.ss[d];
+    const uint32_t slot = B.sslot[d];
+    mg_stress_step& s = B.ss[slot];
     s.m3 = (double)D.bmass * 4.0 / 3.0 * rr;
     s.d = 2.0 * rr;
     s.scale_j = s.m3 * 9.81 * s.d;
@@ -458,7 +480,7 @@ int substep(Batch& B, const std::vector<float>& dt) {
     s.sub_dt = (double)sd;
     s.alpha = (double)alpha;
     s.wall_work_j = (double)ww;
-    B.hend[d] = d * B.rstride + std::min(ds.n_impacts, B.rstride);
+    B.hend[slot] = B.hbeg[slot] + std::min(ds.n_impacts, B.rstride);
     const float dpi = std::fmax(ds.dissipated_energy_j, 0.0f) / sd;
     D.diss_power += alpha * (dpi - D.diss_power);
     D.max_disp = std::fmax(D.max_disp, ds.max_substep_displacement_over_diameter);
@@ -466,10 +488,13 @@ int substep(Batch& B, const std::vector<float>& dt) {
     D.drum_angle = a < 0.0f ? a + std::fabs(TAU_F) : a;
     D.sim_time += (double)sd;
   }
-  if (hipMemcpy(B.rend, B.hend.data(), nd * 4, hipMemcpyHostToDevice) != hipSuccess) {
+  if ((B.ranges_dirty &&
+       hipMemcpy(B.rbeg, B.hbeg.data(), B.hbeg.size() * 4, hipMemcpyHostToDevice) != hipSuccess) ||
+      hipMemcpy(B.rend, B.hend.data(), B.hend.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
     B.msg = "HIP copy error";
     return MG_E_DEVICE;
   }
+  B.ranges_dirty = false;
   uint32_t stride = 0;
   const mg_impact_record* recs = mg_dem_records_device(B.dem, &stride);
   rc = mg_stress_update_ranges(B.st, recs, B.rbeg, B.rend, B.ss.data(), e, sizeof e);
@@ -491,6 +516,62 @@ int run_substeps(Batch& B, uint32_t k, float dt_each) {
   B.last_rc = rc;
   B.last_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return rc;
+}
+
+// metrics::mixing_index (metrics.rs:506-576): the Lacey index of the dye on a 16 x 16 grid over
+// the drum's square, from host copies of the fluid positions (fx: 2 per particle).
+void mixing_index_host(const Drum& Dr, const float* fx, float* out, int* okp) {
+  const Drum* D = &Dr;  // the drum's sizes, radius and dye
+  *okp = 0;
+  *out = 0.0f;
+  const size_t n = (size_t)D->nf;
+  const float r = D->P.radius_m();
+  if (n == 0 || r <= 0.0f) return;
+  const int G = 16;
+  const float cell = 2.0f * r / (float)G;
+  std::vector<float> dsum(G * G, 0.0f);
+  std::vector<uint32_t> cnt(G * G, 0);
+  float tot = 0.0f;
+  for (size_t i = 0; i < n; i++) {
+    const int gi = std::min(std::max(as_i32(std::floor((fx[2 * i] + r) / cell)), 0), G - 1);
+    const int gj = std::min(std::max(as_i32(std::floor((fx[2 * i + 1] + r) / cell)), 0), G - 1);
+    const size_t idx = (size_t)gj * G + gi;
+    dsum[idx] += D->dye[i];
+    cnt[idx]++;
+    tot += D->dye[i];
+  }
+  const float pm = tot / (float)n;
+  std::vector<float> means;
+  std::vector<uint32_t> cts;
+  for (size_t q = 0; q < dsum.size(); q++)
+    if (cnt[q]) {
+      means.push_back(dsum[q] / (float)cnt[q]);
+      cts.push_back(cnt[q]);
+    }
+  if (means.size() < 2) return;
+  uint32_t wsu = 0;
+  for (auto c : cts) wsu += c;
+  const float ws = (float)wsu;
+  float ov = 0.0f;
+  for (size_t q = 0; q < means.size(); q++) {
+    const float d = means[q] - pm;
+    ov += (float)cts[q] * (d * d);
+  }
+  ov = ov / ws;
+  const float s0 = pm * (1.0f - pm);
+  float wa = 0.0f;
+  for (auto c : cts) wa += (float)c * (float)c;
+  wa = wa / ws;
+  const float sr = s0 / std::fmax(wa, 1.0f);
+  if (s0 < 1e-9f) return;
+  const float den = s0 - sr;
+  *okp = 1;
+  if (std::fabs(den) < 1e-9f) {
+    *out = 1.0f;
+    return;
+  }
+  const float mm = (s0 - ov) / den;
+  *out = mm < 0.0f ? 0.0f : (mm > 1.0f ? 1.0f : mm);
 }
 
 bool download(const Batch& B, const Drum& D, std::vector<float>& bx, std::vector<float>& bv, std::vector<float>& bom,
@@ -739,7 +820,14 @@ int mg_sim_stats(mg_sim* sim, double* o, size_t cap) {
                       (double)s.collision_count, (double)f.n_groups, (double)f.n_links, (double)f.links_in_range,
                       (double)s.n_pairs, f.ke_delta_noslip_j, f.ke_delta_drag_j, f.ke_delta_buoyancy_j,
                       f.ke_delta_viscosity_j, f.ke_delta_clamp_j, f.wall_impulse_x, f.wall_impulse_y,
-                      f.wall_angular_impulse, (double)f.error_bits, (double)s.n_impacts};
+                      f.wall_angular_impulse, (double)f.error_bits, (double)s.n_impacts,
+                      // DemStepStats::impact_energy_histogram (dem.rs:99-109)
+                      (double)s.impact_energy_histogram[0], (double)s.impact_energy_histogram[1],
+                      (double)s.impact_energy_histogram[2], (double)s.impact_energy_histogram[3],
+                      (double)s.impact_energy_histogram[4], (double)s.impact_energy_histogram[5],
+                      (double)s.impact_energy_histogram[6], (double)s.impact_energy_histogram[7],
+                      (double)s.impact_energy_histogram[8], (double)s.impact_energy_histogram[9],
+                      (double)s.impact_energy_histogram[10], (double)s.impact_energy_histogram[11]};
   const size_t n = sizeof(v) / sizeof(v[0]);
   std::memcpy(o, v, std::min(n, cap) * sizeof(double));
   return (int)n;
@@ -774,55 +862,10 @@ int mg_sim_mixing_index(mg_sim* sim, float* out, int* okp) {
   *okp = 0;
   *out = 0.0f;
   const size_t n = (size_t)D->nf;
-  const float r = D->P.radius_m();
-  if (n == 0 || r <= 0.0f) return MG_OK;
+  if (n == 0 || D->P.radius_m() <= 0.0f) return MG_OK;
   std::vector<float> fx(2 * n);
-  mg_sim_copy_fluid(sim, fx.data(), nullptr, nullptr);
-  const int G = 16;
-  const float cell = 2.0f * r / (float)G;
-  std::vector<float> dsum(G * G, 0.0f);
-  std::vector<uint32_t> cnt(G * G, 0);
-  float tot = 0.0f;
-  for (size_t i = 0; i < n; i++) {
-    const int gi = std::min(std::max(as_i32(std::floor((fx[2 * i] + r) / cell)), 0), G - 1);
-    const int gj = std::min(std::max(as_i32(std::floor((fx[2 * i + 1] + r) / cell)), 0), G - 1);
-    const size_t idx = (size_t)gj * G + gi;
-    dsum[idx] += D->dye[i];
-    cnt[idx]++;
-    tot += D->dye[i];
-  }
-  const float pm = tot / (float)n;
-  std::vector<float> means;
-  std::vector<uint32_t> cts;
-  for (size_t q = 0; q < dsum.size(); q++)
-    if (cnt[q]) {
-      means.push_back(dsum[q] / (float)cnt[q]);
-      cts.push_back(cnt[q]);
-    }
-  if (means.size() < 2) return MG_OK;
-  uint32_t wsu = 0;
-  for (auto c : cts) wsu += c;
-  const float ws = (float)wsu;
-  float ov = 0.0f;
-  for (size_t q = 0; q < means.size(); q++) {
-    const float d = means[q] - pm;
-    ov += (float)cts[q] * (d * d);
-  }
-  ov = ov / ws;
-  const float s0 = pm * (1.0f - pm);
-  float wa = 0.0f;
-  for (auto c : cts) wa += (float)c * (float)c;
-  wa = wa / ws;
-  const float sr = s0 / std::fmax(wa, 1.0f);
-  if (s0 < 1e-9f) return MG_OK;
-  const float den = s0 - sr;
-  *okp = 1;
-  if (std::fabs(den) < 1e-9f) {
-    *out = 1.0f;
-    return MG_OK;
-  }
-  const float mm = (s0 - ov) / den;
-  *out = mm < 0.0f ? 0.0f : (mm > 1.0f ? 1.0f : mm);
+  if (mg_sim_copy_fluid(sim, fx.data(), nullptr, nullptr) != MG_OK) return MG_E_DEVICE;
+  mixing_index_host(*D, fx.data(), out, okp);
   return MG_OK;
 }
 
@@ -1081,6 +1124,520 @@ int mg_batch_kernel_times(mg_batch* b, char* json, size_t cap) {
   if (s.size() + 1 > cap) return -(int)(s.size() + 1);
   std::memcpy(json, s.c_str(), s.size() + 1);
   return (int)s.size();
+}
+
+}  // extern "C"
+
+// ---- served sweeps (continuous batching) --------------------------------------------------------
+namespace mg {
+namespace {
+
+struct RevRec {
+  uint32_t rev;
+  double sim_time;
+  float power, torque, coll, diss, mix;
+  int mix_ok;
+  uint64_t hist[12];
+};
+
+struct Request {
+  uint64_t id = 0;
+  int state = 0;                // 0 queued, 1 running, 2 done
+  std::unique_ptr<Drum> owned;  // until admitted (then the batch owns the drum)
+  uint64_t total = 0, steps = 0, footprint = 0;
+  double arrival = 0.0, start = 0.0, done = 0.0, period = 0.0, next_rev = 0.0;
+  float dt = 0.0f;
+  uint32_t slot = 0, revs = 0, nb = 0, nf = 0;
+  uint64_t hist[12] = {};
+  float power = 0.0f, torque = 0.0f, coll = 0.0f, diss = 0.0f;
+  std::vector<RevRec> stream;
+};
+
+// The device bytes a running drum takes: its state and its share of the fluid operator's
+// workspace, which mg_fluid_step lays out per step from the drums' sizes (per particle: the
+// neighbour lists at their 48-entry capacity with weights and directions, the push records, the
+// gather lists, the CG rows; per ball: the adhesion, boundary-group and form-drag item lists, the
+// squeeze links at their capacity, the CG rows; per drum: the cells of its five grids). An upper
+// estimate of the layout, not a measurement.
+uint64_t drum_footprint(const Drum& D) {
+  const double nf = D.nf, nb = D.nb, br = D.br, h = D.h, R = D.P.radius_m();
+  const double lr = br > 0.0 ? (2.0 * br + 4.0 * h) / br : 0.0;
+  const double lcap = std::ceil(0.95 * lr * lr) + 4.0;
+  const double cell = std::max(std::min(h, br > 0.0 ? 2.0 * br : h), 1e-6);
+  const double side = std::floor(3.0 * R / cell) + 3.0;
+  return (uint64_t)(nf * 1100.0 + nb * (19500.0 + 20.0 * lcap) + 5.0 * 8.0 * side * side + 65536.0);
+}
+
+struct PoolView {
+  float *fx, *fv, *fg, *bx, *bv, *bth, *bom, *imp, *ang;
+};
+uint64_t up256(uint64_t b) { return ((b + 255) / 256) * 256; }
+uint64_t pool_bytes(uint64_t F, uint64_t Bn) {
+  return 3 * up256(F * 8) + 3 * up256(Bn * 8) + 3 * up256(Bn * 4);
+}
+PoolView pool_view(char* p, uint64_t F, uint64_t Bn) {
+  PoolView v;
+  auto take = [&](uint64_t b) {
+    char* q = p;
+    p += up256(b);
+    return reinterpret_cast<float*>(q);
+  };
+  v.fx = take(F * 8);
+  v.fv = take(F * 8);
+  v.fg = take(F * 8);
+  v.bx = take(Bn * 8);
+  v.bv = take(Bn * 8);
+  v.imp = take(Bn * 8);
+  v.bth = take(Bn * 4);
+  v.bom = take(Bn * 4);
+  v.ang = take(Bn * 4);
+  return v;
+}
+
+}  // namespace
+
+// A server's batch holds the running drums packed in admission order; two state pools alternate:
+// a change of the running set copies the drums that keep running (runs of consecutive drums, one
+// device copy per run and array) and uploads the admitted drums' initial states into the other
+// pool.
+struct Server {
+  mg_server_config cfg{};
+  std::unique_ptr<Batch> B;
+  std::vector<Request*> running;  // in the batch's drum order
+  std::deque<Request*> queue;
+  std::map<uint64_t, std::unique_ptr<Request>> reqs;
+  std::vector<uint32_t> free_slots;
+  char* pool[2] = {nullptr, nullptr};
+  uint64_t fcap[2] = {0, 0}, bcap[2] = {0, 0};
+  int cur = 0;
+  uint64_t fixed = 0, used = 0, next_id = 1, ticks = 0, drum_substeps = 0, n_done = 0;
+  bool retired = false;
+  std::chrono::steady_clock::time_point t0;
+  std::string msg;
+  double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+  ~Server() {
+    B.reset();
+    for (char* p : pool)
+      if (p) (void)hipFree(p);
+  }
+};
+
+namespace {
+
+int server_fail(Server& S, const std::string& m, int rc) {
+  S.msg = m;
+  return rc;
+}
+
+// Lays out the drums that keep running followed by the admitted ones in the other pool.
+int rebuild(Server& S, std::vector<Request*>& incoming) {
+  Batch& B = *S.B;
+  std::vector<Request*> keep;
+  std::vector<std::unique_ptr<Drum>> drums;
+  std::vector<uint32_t> of, ob;  // the kept drums' old offsets
+  for (size_t i = 0; i < S.running.size(); i++) {
+    if (S.running[i]->state != 1) continue;  // finished: its drum is dropped here
+    keep.push_back(S.running[i]);
+    of.push_back(B.drums[i]->f0);
+    ob.push_back(B.drums[i]->b0);
+    drums.push_back(std::move(B.drums[i]));
+  }
+  uint64_t F = 0, Bn = 0;
+  for (auto& d : drums) {
+    F += d->nf;
+    Bn += d->nb;
+  }
+  for (auto* r : incoming) {
+    F += r->owned->nf;
+    Bn += r->owned->nb;
+  }
+  const int dst = 1 - S.cur;
+  if (!S.pool[dst] || F > S.fcap[dst] || Bn > S.bcap[dst]) {
+    const uint64_t nf = std::max<uint64_t>({F, 2 * S.fcap[dst], 1u << 16});
+    const uint64_t nbc = std::max<uint64_t>({Bn, 2 * S.bcap[dst], 1u << 12});
+    if (S.pool[dst]) (void)hipFree(S.pool[dst]);
+    S.pool[dst] = nullptr;
+    S.fcap[dst] = S.bcap[dst] = 0;
+    if (hipMalloc(&S.pool[dst], pool_bytes(nf, nbc)) != hipSuccess)
+      return server_fail(S, "hipMalloc failed for the server's state pool", MG_E_DEVICE);
+    S.fcap[dst] = nf;
+    S.bcap[dst] = nbc;
+  }
+  const PoolView dv = pool_view(S.pool[dst], S.fcap[dst], S.bcap[dst]);
+  bool ok = true;
+  uint64_t f0 = 0, b0 = 0;
+  if (!drums.empty()) {
+    const PoolView sv = pool_view(S.pool[S.cur], S.fcap[S.cur], S.bcap[S.cur]);
+    const hipMemcpyKind dd = hipMemcpyDeviceToDevice;
+    for (size_t k = 0; k < drums.size();) {
+      size_t m = k + 1;  // drums k..m-1 are consecutive in the old layout
+      while (m < drums.size() && of[m] == of[m - 1] + drums[m - 1]->nf && ob[m] == ob[m - 1] + drums[m - 1]->nb) m++;
+      const uint64_t fn = of[m - 1] + drums[m - 1]->nf - of[k], bn = ob[m - 1] + drums[m - 1]->nb - ob[k];
+      if (fn) {
+        ok &= hipMemcpyAsync(dv.fx + 2 * f0, sv.fx + 2 * (uint64_t)of[k], fn * 8, dd, nullptr) == hipSuccess;
+        ok &= hipMemcpyAsync(dv.fv + 2 * f0, sv.fv + 2 * (uint64_t)of[k], fn * 8, dd, nullptr) == hipSuccess;
+        ok &= hipMemcpyAsync(dv.fg + 2 * f0, sv.fg + 2 * (uint64_t)of[k], fn * 8, dd, nullptr) == hipSuccess;
+      }
+      if (bn) {
+        ok &= hipMemcpyAsync(dv.bx + 2 * b0, sv.bx + 2 * (uint64_t)ob[k], bn * 8, dd, nullptr) == hipSuccess;
+        ok &= hipMemcpyAsync(dv.bv + 2 * b0, sv.bv + 2 * (uint64_t)ob[k], bn * 8, dd, nullptr) == hipSuccess;
+        ok &= hipMemcpyAsync(dv.bth + b0, sv.bth + ob[k], bn * 4, dd, nullptr) == hipSuccess;
+        ok &= hipMemcpyAsync(dv.bom + b0, sv.bom + ob[k], bn * 4, dd, nullptr) == hipSuccess;
+      }
+      for (size_t j = k; j < m; j++) {
+        drums[j]->f0 = (uint32_t)(f0 + (of[j] - of[k]));
+        drums[j]->b0 = (uint32_t)(b0 + (ob[j] - ob[k]));
+      }
+      f0 += fn;
+      b0 += bn;
+      k = m;
+    }
+  }
+  const hipMemcpyKind hd = hipMemcpyHostToDevice;
+  for (auto* r : incoming) {
+    Drum& D = *r->owned;
+    D.f0 = (uint32_t)f0;
+    D.b0 = (uint32_t)b0;
+    if (D.nf) {
+      ok &= hipMemcpy(dv.fx + 2 * f0, D.init_fx.data(), D.nf * 8, hd) == hipSuccess;
+      ok &= hipMemcpy(dv.fv + 2 * f0, D.init_fv.data(), D.nf * 8, hd) == hipSuccess;
+      ok &= hipMemset(dv.fg + 2 * f0, 0, D.nf * 8) == hipSuccess;
+    }
+    if (D.nb) {
+      ok &= hipMemcpy(dv.bx + 2 * b0, D.init_bx.data(), D.nb * 8, hd) == hipSuccess;
+      ok &= hipMemcpy(dv.bv + 2 * b0, D.init_bv.data(), D.nb * 8, hd) == hipSuccess;
+      ok &= hipMemcpy(dv.bom + b0, D.init_bom.data(), D.nb * 4, hd) == hipSuccess;
+      ok &= hipMemset(dv.bth + b0, 0, D.nb * 4) == hipSuccess;
+    }
+    std::vector<float>().swap(D.init_bx);
+    std::vector<float>().swap(D.init_bv);
+    std::vector<float>().swap(D.init_bom);
+    std::vector<float>().swap(D.init_fx);
+    std::vector<float>().swap(D.init_fv);
+    f0 += D.nf;
+    b0 += D.nb;
+    drums.push_back(std::move(r->owned));
+    keep.push_back(r);
+  }
+  ok &= hipDeviceSynchronize() == hipSuccess;
+  if (!ok) return server_fail(S, "HIP error laying out the running drums", MG_E_DEVICE);
+  S.cur = dst;
+  B.fx = dv.fx;
+  B.fv = dv.fv;
+  B.fg = dv.fg;
+  B.bx = dv.bx;
+  B.bv = dv.bv;
+  B.bth = dv.bth;
+  B.bom = dv.bom;
+  B.imp = dv.imp;
+  B.ang = dv.ang;
+  B.TF = F;
+  B.TB = Bn;
+  B.drums = std::move(drums);
+  S.running = std::move(keep);
+  B.sslot.resize(S.running.size());
+  for (size_t d = 0; d < S.running.size(); d++) {
+    B.sslot[d] = S.running[d]->slot;
+    B.hbeg[S.running[d]->slot] = (uint32_t)d * B.rstride;
+  }
+  B.ranges_dirty = true;
+  return MG_OK;
+}
+
+// One tick: admissions, one substep of every running drum, per-revolution records, finished runs.
+int tick(Server& S) {
+  Batch& B = *S.B;
+  std::vector<Request*> incoming;
+  while (!S.queue.empty() && !S.free_slots.empty()) {
+    Request* r = S.queue.front();
+    if (S.fixed + S.used + r->footprint > S.cfg.hbm_budget_bytes) break;  // the head waits
+    S.queue.pop_front();
+    r->slot = S.free_slots.back();
+    S.free_slots.pop_back();
+    if (mg_stress_reset(B.st, r->slot) != MG_OK) return server_fail(S, "HIP error resetting a stress slot", MG_E_DEVICE);
+    r->state = 1;
+    r->start = S.now();
+    S.used += r->footprint;
+    incoming.push_back(r);
+  }
+  if (!incoming.empty() || S.retired) {
+    const int rc = rebuild(S, incoming);
+    if (rc != MG_OK) return rc;
+    S.retired = false;
+  }
+  const uint32_t nd = (uint32_t)S.running.size();
+  if (nd == 0) return MG_OK;
+  std::vector<float> dt(nd);
+  for (uint32_t d = 0; d < nd; d++) dt[d] = S.running[d]->dt;
+  const int rc = substep(B, dt);
+  if (rc != MG_OK) return server_fail(S, B.msg, rc);
+  S.ticks++;
+  S.drum_substeps += nd;
+  const double t = S.now();
+  for (uint32_t d = 0; d < nd; d++) {
+    Request* r = S.running[d];
+    const Drum& D = *B.drums[d];
+    r->steps++;
+    for (int k = 0; k < 12; k++) r->hist[k] += D.ds.impact_energy_histogram[k];
+    r->power = D.power;
+    r->torque = D.torque;
+    r->diss = D.diss_power;
+    const bool fin = r->steps >= r->total, rev = D.sim_time >= r->next_rev;
+    if (fin || rev) {  // StressAccum::total_rate of the drum's accumulator (stress.rs:176-182)
+      double st[49];
+      if (mg_stress_state(B.st, r->slot, st) != MG_OK) return server_fail(S, "HIP copy error", MG_E_DEVICE);
+      r->coll = (float)(st[0] + st[1] + st[2] + st[3]);
+    }
+    if (rev) {
+      RevRec q{};
+      q.rev = ++r->revs;
+      q.sim_time = D.sim_time;
+      q.power = r->power;
+      q.torque = r->torque;
+      q.coll = r->coll;
+      q.diss = r->diss;
+      std::vector<float> fx(2 * (size_t)D.nf);
+      if (D.nf && hipMemcpy(fx.data(), B.fx + 2 * (size_t)D.f0, D.nf * 8, hipMemcpyDeviceToHost) != hipSuccess)
+        return server_fail(S, "HIP copy error", MG_E_DEVICE);
+      mixing_index_host(D, fx.data(), &q.mix, &q.mix_ok);
+      std::memcpy(q.hist, r->hist, sizeof q.hist);
+      std::memset(r->hist, 0, sizeof r->hist);
+      r->stream.push_back(q);
+      r->next_rev += r->period;
+    }
+    if (fin) {
+      r->state = 2;
+      r->done = t;
+      S.used -= r->footprint;
+      S.n_done++;
+      S.retired = true;
+      if (mg_stress_reset(B.st, r->slot) != MG_OK) return server_fail(S, "HIP error resetting a stress slot", MG_E_DEVICE);
+      S.free_slots.push_back(r->slot);
+    }
+  }
+  return MG_OK;
+}
+
+}  // namespace
+}  // namespace mg
+
+struct mg_server {
+  mg::Server s;
+};
+
+extern "C" {
+
+int mg_server_create(const mg_server_config* cfg, mg_server** out, char* err, size_t errlen) {
+  if (!cfg || !out) return MG_E_ARG;
+  *out = nullptr;
+  if (cfg->max_drums < 1 || cfg->max_drums > 4096 || cfg->max_balls < 1 || cfg->max_balls > 2048) {
+    set_err(err, errlen, "max_drums must be in 1..4096 and max_balls in 1..2048");
+    return MG_E_ARG;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    set_err(err, errlen, "no HIP device");
+    return MG_E_DEVICE;
+  }
+  auto h = std::make_unique<mg_server>();
+  Server& S = h->s;
+  S.cfg = *cfg;
+  if (S.cfg.hbm_budget_bytes == 0) S.cfg.hbm_budget_bytes = 16ull << 30;
+  const uint32_t cap = cfg->max_drums, mb = cfg->max_balls;
+  // the slots' fixed workspaces: the DEM's per-population arrays and the stress accumulators
+  uint64_t tbl = 64;
+  while (tbl < 4ull * mb + 1024) tbl <<= 1;
+  S.fixed = (uint64_t)cap * ((uint64_t)mb * (24 * 32 + 96) + (4ull * mb + 64) * 32 + 49 * 8 + 16 * tbl);
+  if (S.fixed > S.cfg.hbm_budget_bytes) {
+    set_err(err, errlen, "the budget does not hold the slots' fixed workspaces (" + std::to_string(S.fixed >> 20) +
+                             " MiB)");
+    return MG_E_CAPACITY;
+  }
+  S.B = std::make_unique<Batch>();
+  Batch& B = *S.B;
+  char e[512] = {0};
+  if (mg_fluid_create(cap, &B.fl, e, sizeof e) != MG_OK || mg_dem_create(cap, mb, 24, 4 * mb + 64, &B.dem, e, sizeof e) != MG_OK ||
+      mg_stress_create(cap, 4 * mb + 1024, &B.st, e, sizeof e) != MG_OK) {
+    set_err(err, errlen, e);
+    return MG_E_DEVICE;
+  }
+  const size_t rb = up256((uint64_t)cap * 4);
+  if (hipMalloc(&B.dev, 2 * rb) != hipSuccess || hipMemset(B.dev, 0, 2 * rb) != hipSuccess) {
+    set_err(err, errlen, "hipMalloc failed for the server");
+    return MG_E_DEVICE;
+  }
+  B.rbeg = reinterpret_cast<uint32_t*>(B.dev);
+  B.rend = reinterpret_cast<uint32_t*>(B.dev + rb);
+  mg_dem_records_device(B.dem, &B.rstride);
+  B.fd.resize(cap);
+  B.dd.resize(cap);
+  B.fs.resize(cap);
+  B.ds.resize(cap);
+  B.ss.resize(cap);
+  B.hbeg.assign(cap, 0);
+  B.hend.assign(cap, 0);
+  mg_fluid_drum& f = B.fd_empty;  // a slot without a drum: no particles, no balls
+  f.radius_m = 0.1f;
+  f.h = 1e-3f;
+  f.rest_density = 1000.0f;
+  f.particle_mass = 1e-3f;
+  f.ball_radius = 1e-3f;
+  f.ball_mass = 1e-3f;
+  f.ball_inertia = 1e-9f;
+  f.ball_true_radius = 1e-3f;
+  f.iterations = 3;
+  f.dt = 1e-3f;
+  mg_dem_drum& m = B.dd_empty;
+  m.radius_m = 0.1f;
+  m.ball_radius = 1e-3f;
+  m.ball_mass = 1e-3f;
+  m.ball_inertia = 1e-9f;
+  m.iterations = 2;
+  m.dt = 1e-3f;
+  m.has_external = 1;
+  m.energy_f64 = 1;
+  for (uint32_t q = cap; q-- > 0;) S.free_slots.push_back(q);
+  S.t0 = std::chrono::steady_clock::now();
+  *out = h.release();
+  return MG_OK;
+}
+
+void mg_server_destroy(mg_server* h) { delete h; }
+
+double mg_server_now(const mg_server* h) { return h ? h->s.now() : 0.0; }
+
+int mg_server_submit(mg_server* h, const char* params_json, double revolutions, double arrival_s, uint64_t* id,
+                     char* err, size_t errlen) {
+  if (!h || !id) return MG_E_ARG;
+  Server& S = h->s;
+  Params P;
+  std::string e;
+  if (params_json && !Params::from_json(params_json, P, e)) {
+    set_err(err, errlen, e);
+    return MG_E_PARAMS;
+  }
+  if (!P.validate(e)) {
+    set_err(err, errlen, e);
+    return MG_E_PARAMS;
+  }
+  const double w = std::fabs((double)P.omega());
+  if (!(w > 0.0) || !std::isfinite(w)) {
+    set_err(err, errlen, "a drum that does not turn has no revolutions");
+    return MG_E_PARAMS;
+  }
+  if (!(revolutions > 0.0) || !std::isfinite(revolutions)) {
+    set_err(err, errlen, "revolutions must be positive and finite");
+    return MG_E_PARAMS;
+  }
+  auto r = std::make_unique<Request>();
+  r->owned = make_drum(P, nullptr, 0);
+  const Drum& D = *r->owned;
+  if (D.nb > S.cfg.max_balls) {
+    set_err(err, errlen, std::to_string(D.nb) + " balls: more than the server's max_balls");
+    return MG_E_CAPACITY;
+  }
+  r->footprint = drum_footprint(D);
+  if (S.fixed + r->footprint > S.cfg.hbm_budget_bytes) {
+    set_err(err, errlen, "the drum's footprint alone exceeds the budget");
+    return MG_E_CAPACITY;
+  }
+  r->dt = 1.0f / (60.0f * (float)std::max(P.effective_substeps(), 1u));
+  r->period = 6.283185307179586 / w;
+  r->total = std::max<uint64_t>(1, (uint64_t)std::ceil(revolutions * r->period / (double)r->dt));
+  r->next_rev = r->period;
+  r->nb = D.nb;
+  r->nf = D.nf;
+  r->id = S.next_id++;
+  r->arrival = arrival_s < 0.0 ? S.now() : arrival_s;
+  *id = r->id;
+  S.queue.push_back(r.get());
+  S.reqs[r->id] = std::move(r);
+  return MG_OK;
+}
+
+int mg_server_poll(mg_server* h, uint32_t ticks, mg_server_status* st, char* err, size_t errlen) {
+  if (!h) return MG_E_ARG;
+  Server& S = h->s;
+  for (uint32_t t = 0; t < ticks; t++) {
+    if (S.queue.empty() && S.running.empty()) break;
+    const int rc = tick(S);
+    if (rc != MG_OK) {
+      set_err(err, errlen, S.msg);
+      return rc;
+    }
+  }
+  if (st) {
+    st->queued = (uint32_t)S.queue.size();
+    uint32_t run = 0;
+    for (auto* r : S.running) run += r->state == 1 ? 1u : 0u;
+    st->running = run;
+    st->done = (uint32_t)S.n_done;
+    st->ticks = S.ticks;
+    st->drum_substeps = S.drum_substeps;
+    st->used_bytes = S.fixed + S.used;
+    st->now_s = S.now();
+  }
+  return MG_OK;
+}
+
+int mg_server_result(mg_server* h, uint64_t id, mg_request_result* o) {
+  if (!h || !o) return MG_E_ARG;
+  auto it = h->s.reqs.find(id);
+  if (it == h->s.reqs.end()) return MG_E_ARG;
+  const Request& r = *it->second;
+  std::memset(o, 0, sizeof *o);
+  o->state = r.state;
+  o->n_balls = r.nb;
+  o->n_fluid = r.nf;
+  o->revolutions_done = r.revs;
+  o->substeps_total = r.total;
+  o->substeps_done = r.steps;
+  o->footprint_bytes = r.footprint;
+  o->arrival_s = r.arrival;
+  o->start_s = r.start;
+  o->done_s = r.done;
+  o->power_draw_w = r.power;
+  o->torque_nm = r.torque;
+  o->collision_rate_per_s = r.coll;
+  o->dissipated_power_w = r.diss;
+  return MG_OK;
+}
+
+int mg_server_stream_json(mg_server* h, uint64_t id, char* buf, size_t cap) {
+  if (!h) return MG_E_ARG;
+  auto it = h->s.reqs.find(id);
+  if (it == h->s.reqs.end()) return MG_E_ARG;
+  std::string s = "[";
+  for (const RevRec& q : it->second->stream) {
+    char b[1024];
+    char mix[32];
+    if (q.mix_ok) std::snprintf(mix, sizeof mix, "%.9g", (double)q.mix);
+    else std::snprintf(mix, sizeof mix, "null");
+    int n = std::snprintf(b, sizeof b,
+                          "%s{\"rev\":%u,\"sim_time_s\":%.17g,\"power_draw_w\":%.9g,\"torque_nm\":%.9g,"
+                          "\"collision_rate_per_s\":%.9g,\"dissipated_power_w\":%.9g,\"mixing_index\":%s,"
+                          "\"impact_energy_histogram\":[",
+                          s.size() > 1 ? "," : "", q.rev, q.sim_time, (double)q.power, (double)q.torque,
+                          (double)q.coll, (double)q.diss, mix);
+    s.append(b, (size_t)n);
+    for (int k = 0; k < 12; k++) {
+      n = std::snprintf(b, sizeof b, "%s%llu", k ? "," : "", (unsigned long long)q.hist[k]);
+      s.append(b, (size_t)n);
+    }
+    s += "]}";
+  }
+  s += "]";
+  if (!buf || s.size() + 1 > cap) return -(int)(s.size() + 1);
+  std::memcpy(buf, s.c_str(), s.size() + 1);
+  return (int)s.size();
+}
+
+int mg_server_release(mg_server* h, uint64_t id) {
+  if (!h) return MG_E_ARG;
+  auto it = h->s.reqs.find(id);
+  if (it == h->s.reqs.end() || it->second->state != 2) return MG_E_ARG;
+  h->s.reqs.erase(it);
+  return MG_OK;
 }
 
 }  // extern "C"

@@ -60,3 +60,10 @@ def test_params_validation_messages_without_device(L):
     rc = L.mg_sim_create(bad, C.byref(h), err, 512)
     assert rc == -1
     assert err.value
+
+
+def test_server_config_is_checked_before_the_device(L):
+    from milldynamics3_amd import server
+    for bad in ((0, 2048), (4097, 2048), (4, 0), (4, 2049)):
+        with pytest.raises(RuntimeError):
+            server.Server(max_drums=bad[0], max_balls=bad[1])
