@@ -18,7 +18,10 @@ event pair per launch on the stream it runs on; `traffic` is left null (HBM byte
 rocprofv3 --pmc run of their own).
 cpu_baseline: the C++ restatement of the reference (oracle/, f32 reference policy, 1 core) on a
 bounded sample of the same ensemble (drums in order, two substeps each, about --cpu-seconds).
-p50/p99 latency: reported by the served form; null while it is not built.
+p50/p99 latency: the served sweep (--no-served skips it): requests drawn as the ensemble's drums
+with Poisson arrivals at 50/s, submitted to mg_server_* as the wall clock reaches their arrival
+times; latency = completion - scheduled arrival. Their durations are U[2, 20] revolutions scaled by
+--served-scale (default 0.02, so a run of the sweep finishes in seconds; 1.0 is the full spec).
 
 kernel_isolation (skipped with --no-box): the periodic PBF box of side x side particles (default
 1024^2), per step the grid build + cell reorder, four density passes, velocity update, density
@@ -678,6 +681,78 @@ def power_error(substeps) -> dict:
     return out
 
 
+def c5_requests(n, scale, seed=7):
+    """The served sweep's requests: Poisson arrivals at 50/s (exponential inter-arrival times) from
+    one seeded stream, each drawing its drum as the batched ensemble does (same ranges) and a
+    duration of U[2, 20] revolutions times `scale`; seed k + 1."""
+    import math
+
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    for k in range(n):
+        t += float(rng.exponential(1.0 / 50.0))
+        nb = int(rng.integers(100, 2001))
+        res = int(rng.integers(25, 76))
+        fill = float(rng.uniform(0.20, 0.40))
+        nc = float(rng.uniform(50.0, 90.0))
+        mu = float(rng.uniform(0.2, 0.5))
+        revs = float(rng.uniform(2.0, 20.0)) * scale
+        dia = 0.002 * math.sqrt(nb / (0.82 * fill))
+        p = {"mill": {"diameter_m": dia, "speed_mode": "percent_critical", "speed_value": nc},
+             "media": {"fill_fraction": fill, "friction_ball_ball": mu, "friction_ball_wall": mu,
+                       "friction_ball_ball_static": mu, "friction_ball_wall_static": mu},
+             "simulation": {"resolution": res, "max_balls": nb, "seed": k + 1}}
+        out.append((t, p, revs))
+    return out
+
+
+def bench_served(args) -> dict:
+    """The served sweep (continuous batching through mg_server_*): requests submitted as the wall
+    clock reaches their arrival times, the server ticked in between; latency = completion minus
+    scheduled arrival. Aggregate rate = drum-substeps over the wall time from the first arrival to
+    the last completion."""
+    import numpy as np
+
+    from milldynamics3_amd.server import Server
+    reqs = c5_requests(args.served_requests, args.served_scale)
+    s = Server(max_drums=256, max_balls=2048, hbm_budget_bytes=64 << 30)
+    warm = s.submit(reqs[0][1], 0.01)  # the operators' first workspaces, outside the timed run
+    while s.poll(8)["running"]:
+        pass
+    s.release(warm)
+    t0 = s.now() + 0.05
+    ids, i, st = [], 0, s.poll(0)
+    base_steps = st["drum_substeps"]
+    peak_running = 0
+    while i < len(reqs) or st["running"] or st["queued"]:
+        now = s.now()
+        while i < len(reqs) and t0 + reqs[i][0] <= now:
+            ids.append(s.submit(reqs[i][1], reqs[i][2], t0 + reqs[i][0]))
+            i += 1
+        st = s.poll(1)
+        peak_running = max(peak_running, st["running"])
+        if not st["running"] and not st["queued"] and i < len(reqs):
+            time.sleep(max(0.0, t0 + reqs[i][0] - s.now()))
+    res = [s.result(r) for r in ids]
+    end = max(r["done_s"] for r in res)
+    lat = np.array([r["done_s"] - r["arrival_s"] for r in res])
+    wait = np.array([r["start_s"] - r["arrival_s"] for r in res])
+    steps = st["drum_substeps"] - base_steps
+    out = {"workload": f"{len(reqs)} requests, Poisson arrivals at 50/s (seed 7), drums drawn as the batched "
+                       f"ensemble, durations U[2, 20] revolutions x {args.served_scale} "
+                       f"({min(r['substeps_total'] for r in res)}-{max(r['substeps_total'] for r in res)} substeps)",
+           "latency_p50_s": round(float(np.percentile(lat, 50)), 4),
+           "latency_p99_s": round(float(np.percentile(lat, 99)), 4),
+           "queue_wait_p99_s": round(float(np.percentile(wait, 99)), 4),
+           "drum_substeps_per_s": round(steps / (end - t0), 1), "drum_substeps": int(steps),
+           "ticks": int(st["ticks"]), "peak_running": int(peak_running),
+           "hbm_budget_bytes": 64 << 30,
+           "revolution_records": int(sum(r["revolutions_done"] for r in res))}
+    s.close()
+    return out
+
+
 def cpu_baseline_c4(plist, seconds) -> dict:
     """The C++ restatement (oracle, f32 'ref' policy, 1 core) on a bounded sample of the same
     ensemble: drums in order, two substeps each, until `seconds` have passed."""
@@ -821,8 +896,12 @@ def main() -> int:
     ap.add_argument("--precision-substeps", type=int, default=480,
                     help="substeps of the FP32-vs-FP64 power comparison (realtime preset)")
     ap.add_argument("--no-box", action="store_true", help="skip the periodic-box kernel-isolation measurements")
-    ap.add_argument("--part", choices=("all", "drums", "ensemble", "grid", "mixing", "drum_metrics", "stress", "dem",
-                                       "surface"),
+    ap.add_argument("--served-requests", type=int, default=250, help="requests of the served sweep")
+    ap.add_argument("--served-scale", type=float, default=0.02,
+                    help="the served sweep's durations: U[2, 20] revolutions times this factor")
+    ap.add_argument("--no-served", action="store_true", help="skip the served sweep")
+    ap.add_argument("--part", choices=("all", "drums", "ensemble", "served", "grid", "mixing", "drum_metrics", "stress",
+                                       "dem", "surface"),
                     default="all",
                     help="grid / mixing: time only that operator (for a profiler run) and print its JSON")
     args = ap.parse_args()
@@ -856,10 +935,12 @@ def main() -> int:
                 "dem": lambda: bench_dem(args, cpu=not args.no_cpu),
                 "surface": lambda: bench_surface(args, cpu=not args.no_cpu),
                 "drums": lambda: bench_drums(args, cpu=not args.no_cpu),
-                "ensemble": lambda: bench_drums(args, cpu=False, batch_only=True)}[args.part]()
+                "ensemble": lambda: bench_drums(args, cpu=False, batch_only=True),
+                "served": lambda: bench_served(args)}[args.part]()
         print(json.dumps({"part": args.part, **part}), flush=True)
         return 0
     drums = bench_drums(args, cpu=not args.no_cpu)
+    served = None if args.no_served else bench_served(args)
     iso = None if args.no_box else bench_box(args)
     line = {
         "metric": METRIC,
@@ -885,11 +966,13 @@ def main() -> int:
                    "plain_c2_drum_substeps_per_s": drums["plain"]["C2"]["drum_substeps_per_s"],
                    "batched_vs_plain_c2": drums["batched_vs_plain_c2"],
                    "power_rel_err_vs_fp64": drums["precision"]["power_draw_w"]["rel_err"],
-                   "p50_p99_latency": None,
-                   "p50_p99_status": "the served form (request loop) is not built in this revision"},
+                   "p50_p99_latency": None if served is None else [served["latency_p50_s"],
+                                                                     served["latency_p99_s"]],
+                   "p50_p99_workload": None if served is None else served["workload"]},
         "roofline": drums["roofline"],
         "cpu_baseline": drums.get("cpu_baseline"),
         "drums": drums,
+        "served": served,
         "kernel_isolation": iso,
     }
     print(json.dumps(line), flush=True)
