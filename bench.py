@@ -14,8 +14,10 @@ N_b (48 (8 + n_dem) + 144)) and the kernel table from HIP event pairs over five 
 
 roofline: the dominant kernel of the batched substep by total time, its algorithmic bytes per launch
 (bench.kernel_bytes: the arrays it streams at FP32 widths) over its average launch time from a HIP
-event pair per launch on the stream it runs on; `traffic` is left null (HBM bytes per launch need a
-rocprofv3 --pmc run of their own).
+event pair per launch on the stream it runs on; `traffic` (HBM bytes per launch) is read from the
+committed rocprofv3 summary profiles/drums_c4x256_kernel_trace.md (two --pmc runs, FETCH_SIZE and
+WRITE_SIZE, scaled by a plain copy's counters) and names that source; null for another kernel or
+batch size.
 cpu_baseline: the C++ restatement of the reference (oracle/, f32 reference policy, 1 core) on a
 bounded sample of the same ensemble (drums in order, two substeps each, about --cpu-seconds).
 p50/p99 latency: the served sweep (--no-served skips it): requests drawn as the ensemble's drums
@@ -75,6 +77,24 @@ def measured_traffic(kernel, side):
         if len(cells) == 5 and cells[0] == kernel:
             try:
                 return float(cells[4]) * 1024 * 1024
+            except ValueError:
+                return None
+    return None
+
+
+DRUM_PROFILE = os.path.join(ROOT, "profiles", "drums_c4x256_kernel_trace.md")
+
+
+def measured_drum_traffic(kernel):
+    """HBM bytes per launch (fetch + write) of a batched-substep kernel from the committed
+    rocprofv3 summary of the 256-drum ensemble (calibrated against a plain copy), or None."""
+    if kernel is None or not os.path.exists(DRUM_PROFILE):
+        return None
+    for line in open(DRUM_PROFILE):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 7 and cells[0] == f"`{kernel}`":
+            try:
+                return (float(cells[5]) + float(cells[6])) * 1024 * 1024
             except ValueError:
                 return None
     return None
@@ -624,10 +644,13 @@ def bench_drums(args, cpu=True, batch_only=False) -> dict:
     avg_ms = kt[dom][0] / max(kt[dom][1], 1)
     bpl = kernel_bytes(dom, int(nf.sum()), int(nb.sum()), float(n_d.mean()), n_dem)
     achieved = None if bpl is None else bpl / (avg_ms * 1e-3) / 1e9
+    traffic = measured_drum_traffic(dom)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": None if achieved is None else round(achieved, 2),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": None if achieved is None else round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": None, "traffic_note": "HBM bytes per launch need a rocprofv3 --pmc run of their own",
+                "traffic": None if traffic is None or args.drums != 256 else round(traffic),
+                "traffic_source": ("profiles/drums_c4x256_kernel_trace.md (rocprofv3 FETCH_SIZE + WRITE_SIZE per "
+                                   "dispatch, copy-calibrated; not measured in this run)"),
                 "bytes_per_launch": bpl, "avg_launch_us": round(1000.0 * avg_ms, 2), "launches": kt[dom][1],
                 "timing": f"HIP event pair per launch on the null stream over {k2} batched substeps",
                 "workload": f"C4 batched ensemble, {len(plist)} drums"}
