@@ -2454,6 +2454,9 @@ struct mg_fluid {
   bool force = false;
   uint32_t forced_d = 0, forced_cg = 0;
   uint32_t cg_chunk = 8;
+  // side streams (non-blocking, default priority) for stages that run beside the Gauss-Seidel sweep
+  hipStream_t side[2] = {nullptr, nullptr};
+  hipEvent_t fork = nullptr, join[2] = {nullptr, nullptr};
   std::vector<FlState> h_st;
   std::vector<FlOut> h_out;
   std::string msg;
@@ -2529,6 +2532,11 @@ int mg_fluid_create(uint32_t n_drums, mg_fluid** out, char* err, size_t errlen) 
 void mg_fluid_destroy(mg_fluid* h) {
   if (!h) return;
   if (h->dev) (void)hipFree(h->dev);
+  for (int k = 0; k < 2; k++) {
+    if (h->side[k]) (void)hipStreamDestroy(h->side[k]);
+    if (h->join[k]) (void)hipEventDestroy(h->join[k]);
+  }
+  if (h->fork) (void)hipEventDestroy(h->fork);
   delete h;
 }
 
@@ -2854,6 +2862,30 @@ int mg_fluid_step(mg_fluid* h, const mg_fluid_drum* drums, float* d_x, float* d_
   MG_TIMED_LAUNCH(h->tm, fl_grid_b, G, T, 0, P);
   if (nwt) MG_TIMED_LAUNCH(h->tm, fl_gs_pre, dim3(nwt), T, 0, P, (const uint2*)h->d_wt);
   {
+    // The sweep (one wave per drum walking its balls in order) leaves most of the GPU idle. The
+    // density at the final positions, the Morris weights and the boundary groups read neither the
+    // velocities nor the impulses it writes when no drum is Bingham (a Bingham drum's viscosity
+    // reads the swept velocities), so they then run beside it on a second stream.
+    bool bingham = false;
+    for (uint32_t d = 0; d < nd; d++) bingham |= hd[d].bingham != 0 && hd[d].n > 0 && hd[d].mu > 0.0f;
+    const bool overlap = !bingham && nft > 0;
+    if (overlap && !h->side[0]) {
+      bool ok2 = true;
+      for (int k = 0; k < 2; k++) {
+        ok2 &= hipStreamCreateWithFlags(&h->side[k], hipStreamNonBlocking) == hipSuccess;
+        ok2 &= hipEventCreateWithFlags(&h->join[k], hipEventDisableTiming) == hipSuccess;
+      }
+      ok2 &= hipEventCreateWithFlags(&h->fork, hipEventDisableTiming) == hipSuccess;
+      if (!ok2) return fl_fail(h, err, errlen, MG_E_DEVICE, "HIP error creating the side streams");
+    }
+    hipStream_t sg = nullptr, sw = nullptr;
+    if (overlap) {
+      sg = h->side[0];
+      sw = h->side[1];
+      (void)hipEventRecord(h->fork, nullptr);
+      (void)hipStreamWaitEvent(sg, h->fork, 0);
+      (void)hipStreamWaitEvent(sw, h->fork, 0);
+    }
     // the drum's fluid velocities stay in LDS during the sweep when they fit (160 KiB per workgroup
     // on gfx950, less the kernel's static LDS)
     uint32_t maxn = 0;
@@ -2862,14 +2894,20 @@ int mg_fluid_step(mg_fluid* h, const mg_fluid_drum* drums, float* d_x, float* d_
     const uint32_t lds_cap = (uint32_t)std::min<size_t>(cap_bytes / 8, (size_t)maxn);
     const size_t bytes = (size_t)lds_cap * 8;
     if (bytes > 65536) (void)hipFuncSetAttribute((const void*)fl_gs, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    MG_TIMED_LAUNCH(h->tm, fl_gs, G, T, bytes, P, lds_cap);
+    MG_TIMED_LAUNCH_S(h->tm, sg, fl_gs, G, T, bytes, P, lds_cap);
+    if (nft) {
+      MG_TIMED_LAUNCH_S(h->tm, sw, fl_dens2, dim3(nft), T, 0, P);
+      MG_TIMED_LAUNCH_S(h->tm, sw, fl_mu, dim3(nft), T, 0, P);
+      MG_TIMED_LAUNCH_S(h->tm, sw, fl_weights, dim3(nft), T, 0, P);
+    }
+    if (nwt) MG_TIMED_LAUNCH_S(h->tm, sw, fl_groups, dim3(nwt), T, 0, P, (const uint2*)h->d_wt);
+    if (overlap) {
+      (void)hipEventRecord(h->join[0], sg);
+      (void)hipEventRecord(h->join[1], sw);
+      (void)hipStreamWaitEvent(nullptr, h->join[0], 0);
+      (void)hipStreamWaitEvent(nullptr, h->join[1], 0);
+    }
   }
-  if (nft) {
-    MG_TIMED_LAUNCH(h->tm, fl_dens2, dim3(nft), T, 0, P);
-    MG_TIMED_LAUNCH(h->tm, fl_mu, dim3(nft), T, 0, P);
-    MG_TIMED_LAUNCH(h->tm, fl_weights, dim3(nft), T, 0, P);
-  }
-  if (nwt) MG_TIMED_LAUNCH(h->tm, fl_groups, dim3(nwt), T, 0, P, (const uint2*)h->d_wt);
   if (nwt) MG_TIMED_LAUNCH(h->tm, fl_links, dim3(nwt), T, 0, P, (const uint2*)h->d_wt);
   MG_TIMED_LAUNCH(h->tm, fl_cg_prep, G, T, 0, P);
   if (nwt && nrt) {

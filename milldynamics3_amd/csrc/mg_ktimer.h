@@ -76,18 +76,22 @@ struct Timer {
 
 }  // namespace mgk
 
-// hipLaunchKernelGGL on the null stream with the launch timed when the timer is on.
-#define MG_TIMED_LAUNCH(timer, kernel, grid, block, shm, ...)                     \
+// hipLaunchKernelGGL on `stream` with the launch timed when the timer is on.
+#define MG_TIMED_LAUNCH_S(timer, stream, kernel, grid, block, shm, ...)           \
   do {                                                                            \
     hipEvent_t mg_e0_ = nullptr;                                                  \
     if ((timer).on) {                                                             \
       mg_e0_ = (timer).take();                                                    \
-      (void)hipEventRecord(mg_e0_, nullptr);                                      \
+      (void)hipEventRecord(mg_e0_, (stream));                                     \
     }                                                                             \
-    hipLaunchKernelGGL(kernel, grid, block, shm, nullptr, __VA_ARGS__);           \
+    hipLaunchKernelGGL(kernel, grid, block, shm, (stream), __VA_ARGS__);          \
     if ((timer).on) {                                                             \
       hipEvent_t mg_e1_ = (timer).take();                                         \
-      (void)hipEventRecord(mg_e1_, nullptr);                                      \
+      (void)hipEventRecord(mg_e1_, (stream));                                     \
       (timer).pend.push_back({#kernel, mg_e0_, mg_e1_});                          \
     }                                                                             \
   } while (0)
+
+// The same on the null stream.
+#define MG_TIMED_LAUNCH(timer, kernel, grid, block, shm, ...) \
+  MG_TIMED_LAUNCH_S(timer, nullptr, kernel, grid, block, shm, __VA_ARGS__)
