@@ -125,6 +125,7 @@ struct FlState {
   double rs, target;
   float alpha, beta;
   double ke_drag, ke_buoy;  // sum |v|^2 after form drag / buoyancy
+  uint32_t gs_levels;       // levels of the form-drag and buoyancy schedule
 };
 
 // ---- small helpers ----------------------------------------------------------------------------
@@ -444,7 +445,7 @@ struct FlPtrs {
   uint32_t *fgoff, *fgids;
   // squeeze links
   // form drag and buoyancy items per ball (GS_ITEMS slots each) and their velocity-free sums
-  uint32_t *gdj, *gdn, *gbj, *gbn;
+  uint32_t *gdj, *gdn, *gbj, *gbn, *glev;
   float4 *gdr, *gds, *gbr;
   float2 *gdb, *gbs;
   uint32_t *lcnt;
@@ -1303,7 +1304,10 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_gs_pre(FlPtrs P, const uin
     }
     wave_sync();
   }
-  if (lane == 0) P.gdn[gb] = nit;
+  if (lane == 0) {
+    P.gdn[gb] = nit;
+    if (nit < GS_ITEMS) P.gdj[gb * GS_ITEMS + nit] = FL_NO;  // the list's end for the schedule
+  }
   uint32_t nw = 0;
   if (D.buoy_on && fin2(c)) {
     const GView g = gview(P.grids + 5 * (size_t)tl.x, P.gstart[0] + D.c0[0], P.gidx[0] + D.f0);
@@ -1345,7 +1349,10 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_gs_pre(FlPtrs P, const uin
     const float od = rdl(acc, 1);
     if (lane == 0) P.gbs[gb] = make_float2(acc, od);
   }
-  if (lane == 0) P.gbn[gb] = nw;
+  if (lane == 0) {
+    P.gbn[gb] = nw;
+    if (nw < GS_ITEMS) P.gbj[gb * GS_ITEMS + nw] = FL_NO;
+  }
   if (over && lane == 0) atomicOr(&P.st[tl.x].err, FE_ITEMS);
 }
 
@@ -1368,24 +1375,136 @@ __device__ __forceinline__ float ordered_sum(const float* __restrict__ v, uint32
   return acc;
 }
 
+// The sweeps' schedule. Ball b of the reference's in-order walks reads and writes the velocities of
+// its items (the particles of its drag and buoyancy lists); two balls sharing no item commute. A
+// ball's level is one more than the largest level of the earlier balls it shares an item with
+// (the sequential order restricted to dependent balls), so every ball of a level is independent
+// of the others and a particle's updates happen in ball order: walking the levels in order, the
+// balls of a level side by side, reproduces the in-order walks exactly. One wave per drum finds
+// the levels in ball order, keeping the last level of every particle in LDS (16 bits); a drum
+// with more particles than that table holds keeps the plain order (level b + 1). A ball's item
+// lists end at the first FL_NO (fl_gs_pre writes one after the last item of a short list); they
+// are loaded eight balls ahead.
+struct LvItems {
+  uint32_t jd[4], jb[4];
+};
+__device__ __forceinline__ void lv_load(const FlPtrs& P, size_t g0, uint32_t nb, uint32_t b, LvItems& I) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const size_t g = g0 + (b < nb ? b : nb - 1);
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    const uint32_t k = (uint32_t)s * 64u + lane;
+    I.jd[s] = k < GS_ITEMS ? P.gdj[g * GS_ITEMS + k] : FL_NO;
+    I.jb[s] = k < GS_ITEMS ? P.gbj[g * GS_ITEMS + k] : FL_NO;
+  }
+}
+// The length of a list held four slots per lane: the position of its first FL_NO.
+__device__ __forceinline__ uint32_t lv_len(const uint32_t (&j)[4]) {
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    const uint64_t m = __ballot(j[s] == FL_NO);
+    if (m) return (uint32_t)s * 64u + (uint32_t)__builtin_ctzll(m);
+  }
+  return 256u;
+}
+__device__ __forceinline__ uint32_t lv_step(uint16_t* s_last, const LvItems& I, uint32_t* lev, uint32_t b, uint32_t nb) {
+  if (b >= nb) return 0;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t cd = lv_len(I.jd), cb = lv_len(I.jb);
+  uint32_t m = 0;
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    const uint32_t k = (uint32_t)s * 64u + lane;
+    const uint32_t a = s_last[k < cd ? I.jd[s] : 0u], c = s_last[k < cb ? I.jb[s] : 0u];
+    if (k < cd) m = max(m, a);
+    if (k < cb) m = max(m, c);
+  }
+  // the wave's largest level (levels are below 2^12): its bits from the top, one ballot each
+  uint32_t top = 0;
+#pragma unroll
+  for (int bit = 11; bit >= 0; bit--) {
+    const uint32_t cand = top | (1u << bit);
+    if (__ballot(m >= cand)) top = cand;
+  }
+  const uint32_t l = top + 1;
+  wave_sync();
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    const uint32_t k = (uint32_t)s * 64u + lane;
+    if (k < cd) s_last[I.jd[s]] = (uint16_t)l;
+    if (k < cb) s_last[I.jb[s]] = (uint16_t)l;
+  }
+  if (lane == 0) lev[b] = l;
+  wave_sync();
+  return l;
+}
+extern "C" __global__ void __launch_bounds__(64) fl_gs_level(FlPtrs P, uint32_t cap) {
+  extern __shared__ uint16_t s_last[];
+  const uint32_t d = blockIdx.x, lane = threadIdx.x;
+  const FlDrum& D = P.drums[d];
+  FlState& S = P.st[d];
+  const uint32_t n = D.n, nb = D.nb;
+  if (n == 0 || nb == 0) return;
+  uint32_t* lev = P.glev + D.b0;
+  if (n > cap) {
+    for (uint32_t b = lane; b < nb; b += 64) lev[b] = b + 1;
+    if (lane == 0) S.gs_levels = nb;
+    return;
+  }
+  for (uint32_t i = lane; i < n; i += 64) s_last[i] = 0;
+  wave_sync();
+  const size_t g0 = (size_t)D.b0;
+  LvItems R0, R1, R2, R3, R4, R5, R6, R7;
+  lv_load(P, g0, nb, 0, R0);
+  lv_load(P, g0, nb, 1, R1);
+  lv_load(P, g0, nb, 2, R2);
+  lv_load(P, g0, nb, 3, R3);
+  lv_load(P, g0, nb, 4, R4);
+  lv_load(P, g0, nb, 5, R5);
+  lv_load(P, g0, nb, 6, R6);
+  lv_load(P, g0, nb, 7, R7);
+  uint32_t top = 0;
+  for (uint32_t b = 0; b < nb; b += 8) {
+    top = max(top, lv_step(s_last, R0, lev, b, nb));
+    lv_load(P, g0, nb, b + 8, R0);
+    top = max(top, lv_step(s_last, R1, lev, b + 1, nb));
+    lv_load(P, g0, nb, b + 9, R1);
+    top = max(top, lv_step(s_last, R2, lev, b + 2, nb));
+    lv_load(P, g0, nb, b + 10, R2);
+    top = max(top, lv_step(s_last, R3, lev, b + 3, nb));
+    lv_load(P, g0, nb, b + 11, R3);
+    top = max(top, lv_step(s_last, R4, lev, b + 4, nb));
+    lv_load(P, g0, nb, b + 12, R4);
+    top = max(top, lv_step(s_last, R5, lev, b + 5, nb));
+    lv_load(P, g0, nb, b + 13, R5);
+    top = max(top, lv_step(s_last, R6, lev, b + 6, nb));
+    lv_load(P, g0, nb, b + 14, R6);
+    top = max(top, lv_step(s_last, R7, lev, b + 7, nb));
+    lv_load(P, g0, nb, b + 15, R7);
+  }
+  if (lane == 0) S.gs_levels = top;
+}
+
 // The sweep over the balls in index order (one wave per drum); V is the drum's velocity array
-// (LDS when it fits, else global memory). A ball's items and its velocity-free sums are loaded one
-// ball ahead (they do not depend on the ball before it), so only the velocity gathers, the three
-// velocity sums and the scalar update stay on the sweep's critical path.
+// (LDS when it fits, else global memory). The balls' item counts are copied to LDS before a sweep;
+// a ball's items and velocity-free sums do not depend on the balls before it, so they are loaded
+// two balls ahead (three rotating buffers) and only the velocity gathers, the three velocity sums,
+// the scalar update and the kicks stay on the sweep's critical path. A kick adds to the velocity
+// the gather read: no other ball writes the drum's velocities in between, and a ball's items are
+// distinct particles.
+#define GS_MAXB 2048  // balls per drum (the fluid operator's limit)
+#define GS_WAVES 8    // waves of the sweep kernel
+#define GS_THREADS (64 * GS_WAVES)
 struct GsBall {
-  uint32_t n;
   float4 st;  // drag: m_ent, i_ent, s, s_scale; buoyancy: rho_l, od, -, -
   float2 rb, bv, imp;
   float bom, ang;
   uint32_t ij[4];
   float4 ir[4];
 };
-__device__ inline void gs_load(const FlPtrs& P, uint32_t b0, uint32_t nb, uint32_t b, bool drag, GsBall& B) {
+__device__ __forceinline__ void gs_load(const FlPtrs& P, uint32_t b0, uint32_t nb, uint32_t b, bool drag, GsBall& B) {
   const uint32_t lane = threadIdx.x & 63u;
-  const size_t gb = (size_t)b0 + b;
-  const size_t gs = b < nb ? gb : (size_t)b0;  // past the last ball: any valid address, count 0
-  const uint32_t n = drag ? P.gdn[gs] : P.gbn[gs];
-  B.n = b < nb ? n : 0u;
+  const size_t gs = b < nb ? (size_t)b0 + b : (size_t)b0;  // past the last ball: any valid address
   if (drag) {
     B.st = P.gds[gs];
     B.rb = P.gdb[gs];
@@ -1395,166 +1514,194 @@ __device__ inline void gs_load(const FlPtrs& P, uint32_t b0, uint32_t nb, uint32
     const float2 q = P.gbs[gs];
     B.st = make_float4(q.x, q.y, 0.0f, 0.0f);
     B.rb = P.bx[gs];
+    B.bom = 0.0f;
+    B.ang = 0.0f;
   }
   B.bv = P.bv[gs];
   B.imp = P.imp[gs];
   // every slot of the ball's item block is loaded (the block is always allocated), so the loads do
   // not wait for the count; slots past it are never used
-  const size_t gq = b < nb ? gb : (size_t)b0;
 #pragma unroll
   for (int s = 0; s < 4; s++) {
     const uint32_t k = (uint32_t)s * 64u + lane;
     B.ij[s] = 0;
     B.ir[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (k < GS_ITEMS) {
-      B.ij[s] = drag ? P.gdj[gq * GS_ITEMS + k] : P.gbj[gq * GS_ITEMS + k];
-      B.ir[s] = drag ? P.gdr[gq * GS_ITEMS + k] : P.gbr[gq * GS_ITEMS + k];
+      B.ij[s] = drag ? P.gdj[gs * GS_ITEMS + k] : P.gbj[gs * GS_ITEMS + k];
+      B.ir[s] = drag ? P.gdr[gs * GS_ITEMS + k] : P.gbr[gs * GS_ITEMS + k];
     }
   }
 }
 
+// The drum's constants of the sweeps, in registers (the loop's stores would otherwise make the
+// compiler reload them, and every reload waits for the prefetched loads).
+struct GsConst {
+  float ib, rest, btrue, mass_true, beta_b, dt, bmass, hc, rot_cap, om_abs, area, mass;
+  float2* imp;
+  float* ang;
+};
+
+// Form drag of ball b (pbf.rs:1824-2011).
 template <bool LDSV>
-__device__ inline void gs_sweep(const FlPtrs& P, const FlDrum& D, float2* V, float* s_t0, float* s_t1, float* s_t2) {
+__device__ __forceinline__ void gs_drag_ball(const GsConst& C, float2* V, float* s_t0, float* s_t1, float* s_t2,
+                                             uint32_t nit, uint32_t b, const GsBall& G) {
   const uint32_t lane = threadIdx.x & 63u;
-  float2* imp = P.imp + D.b0;
-  float* ang = P.ang + D.b0;
-  // the drum's constants in registers: the loop's stores would otherwise make the compiler reload
-  // them, and every reload waits for the prefetched loads
-  const float ib = D.binertia, rest = D.rest, btrue = D.btrue, mass_true = D.mass_true, beta_b = D.beta_b;
-  const float dt = D.dt, bmass = D.bmass, hc = D.hc, rot_cap = D.rot_cap;
-  const uint32_t nb = D.nb, b0 = D.b0;
-  GsBall cur, nxt;
-  gs_load(P, b0, nb, 0, true, cur);
-  for (uint32_t b = 0; b < nb; b++) {
-    gs_load(P, b0, nb, b + 1, true, nxt);
-    const uint32_t nit = cur.n;
-    const float m_ent = cur.st.x, i_ent = cur.st.y;
-    if (nit != 0 && m_ent > 1e-12f) {
+  const float m_ent = G.st.x, i_ent = G.st.y;
+  if (nit == 0 || !(m_ent > 1e-12f)) return;
+  float2 vj[4];
 #pragma unroll
-      for (int s = 0; s < 4; s++) {
-        const uint32_t k = (uint32_t)s * 64u + lane;
-        if (k < nit) {
-          const float2 vj = V[cur.ij[s]];
-          const float w = cur.ir[s].w;
-          s_t0[k] = w * vj.x;
-          s_t1[k] = w * vj.y;
-          s_t2[k] = w * (cur.ir[s].x * vj.y - cur.ir[s].y * vj.x);
-        }
-      }
-      wave_sync();
-      float acc = 0.0f;
-      if (lane < 3) acc = ordered_sum(lane == 0 ? s_t0 : lane == 1 ? s_t1 : s_t2, nit, 0.0f);
-      const float vsx = rdl(acc, 0), vsy = rdl(acc, 1), l_sum = rdl(acc, 2);
-      const float rbx = cur.rb.x, rby = cur.rb.y, vbx = vsx / m_ent, vby = vsy / m_ent;
-      const float vrx = vbx - cur.bv.x, vry = vby - cur.bv.y;
-      const float sp = sqrtf(vrx * vrx + vry * vry);
-      const float itf = (sp > 1e-6f && rest > 0.0f) ? 0.5f * rest * 1.0f * (2.0f * btrue) * sp / mass_true : 0.0f;
-      const float bt = beta_b * (1.0f - mgm::ref_expf(-dt * (0.0f + itf)));
-      const float a_lin = bt * m_ent / (bmass + m_ent);
-      const float dvbx = a_lin * vrx, dvby = a_lin * vry;
-      float dwb = 0.0f, c_rot = 0.0f, pbx = 0.0f, pby = 0.0f;
-      if (i_ent > 1e-12f) {
-        const float om_bar = l_sum / i_ent;
-        const float a_rot = bt * i_ent / (ib + i_ent);
-        const float cand = a_rot * (om_bar - cur.bom);
-        pbx = -rby;
-        pby = rbx;
-        const float s = cur.st.z, ss = cur.st.w;
-        if (fabsf(s) > 0.1f * ss) {
-          const float raw = -ib * cand / s;
-          const float ml = 2.0f * hc;
-          const float cap = ml > 1e-9f ? rot_cap / ml : 0.0f;
-          c_rot = fclampf(raw, -cap, cap);
-          dwb = -c_rot * s / ib;
-        }
-      }
-      const float f = -(bmass / m_ent);
+  for (int s = 0; s < 4; s++) vj[s] = V[(uint32_t)s * 64u + lane < nit ? G.ij[s] : 0u];
 #pragma unroll
-      for (int s = 0; s < 4; s++) {
-        const uint32_t k = (uint32_t)s * 64u + lane;
-        if (k < nit) {
-          const float p = cur.ir[s].z;
-          float dx = f * dvbx * p, dy = f * dvby * p;
-          if (c_rot != 0.0f) {
-            const float cp = c_rot * p;
-            dx += cp * (-cur.ir[s].y - pbx);
-            dy += cp * (cur.ir[s].x - pby);
-          }
-          const float2 vj = V[cur.ij[s]];
-          V[cur.ij[s]] = make_float2(vj.x + dx, vj.y + dy);
-        }
-      }
-      if (lane == 0) {
-        imp[b] = make_float2(cur.imp.x + bmass * dvbx, cur.imp.y + bmass * dvby);
-        ang[b] = cur.ang + (ib * dwb + bmass * (rbx * dvby - rby * dvbx));
-      }
-      if (!LDSV) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-      wave_sync();
+  for (int s = 0; s < 4; s++) {
+    const uint32_t k = (uint32_t)s * 64u + lane;
+    if (k < nit) {
+      const float w = G.ir[s].w;
+      s_t0[k] = w * vj[s].x;
+      s_t1[k] = w * vj[s].y;
+      s_t2[k] = w * (G.ir[s].x * vj[s].y - G.ir[s].y * vj[s].x);
     }
-    cur = nxt;
   }
+  wave_sync();
+  float acc = 0.0f;
+  if (lane < 3) acc = ordered_sum(lane == 0 ? s_t0 : lane == 1 ? s_t1 : s_t2, nit, 0.0f);
+  const float vsx = rdl(acc, 0), vsy = rdl(acc, 1), l_sum = rdl(acc, 2);
+  const float rbx = G.rb.x, rby = G.rb.y, vbx = vsx / m_ent, vby = vsy / m_ent;
+  const float vrx = vbx - G.bv.x, vry = vby - G.bv.y;
+  const float sp = sqrtf(vrx * vrx + vry * vry);
+  const float itf = (sp > 1e-6f && C.rest > 0.0f) ? 0.5f * C.rest * 1.0f * (2.0f * C.btrue) * sp / C.mass_true : 0.0f;
+  const float bt = C.beta_b * (1.0f - mgm::ref_expf(-C.dt * (0.0f + itf)));
+  const float a_lin = bt * m_ent / (C.bmass + m_ent);
+  const float dvbx = a_lin * vrx, dvby = a_lin * vry;
+  float dwb = 0.0f, c_rot = 0.0f, pbx = 0.0f, pby = 0.0f;
+  if (i_ent > 1e-12f) {
+    const float om_bar = l_sum / i_ent;
+    const float a_rot = bt * i_ent / (C.ib + i_ent);
+    const float cand = a_rot * (om_bar - G.bom);
+    pbx = -rby;
+    pby = rbx;
+    const float s = G.st.z, ss = G.st.w;
+    if (fabsf(s) > 0.1f * ss) {
+      const float raw = -C.ib * cand / s;
+      const float ml = 2.0f * C.hc;
+      const float cap = ml > 1e-9f ? C.rot_cap / ml : 0.0f;
+      c_rot = fclampf(raw, -cap, cap);
+      dwb = -c_rot * s / C.ib;
+    }
+  }
+  const float f = -(C.bmass / m_ent);
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    const uint32_t k = (uint32_t)s * 64u + lane;
+    if (k < nit) {
+      const float p = G.ir[s].z;
+      float dx = f * dvbx * p, dy = f * dvby * p;
+      if (c_rot != 0.0f) {
+        const float cp = c_rot * p;
+        dx += cp * (-G.ir[s].y - pbx);
+        dy += cp * (G.ir[s].x - pby);
+      }
+      V[G.ij[s]] = make_float2(vj[s].x + dx, vj[s].y + dy);
+    }
+  }
+  if (lane == 0) {
+    C.imp[b] = make_float2(G.imp.x + C.bmass * dvbx, G.imp.y + C.bmass * dvby);
+    C.ang[b] = G.ang + (C.ib * dwb + C.bmass * (rbx * dvby - rby * dvbx));
+  }
+  if (!LDSV) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  wave_sync();
 }
 
+// Buoyancy of ball b (pbf.rs:2035-2122).
 template <bool LDSV>
-__device__ inline void gs_sweep_buoy(const FlPtrs& P, const FlDrum& D, float2* V, float* s_t0) {
+__device__ __forceinline__ void gs_buoy_ball(const GsConst& C, float2* V, float* s_t0, uint32_t nw, uint32_t b,
+                                             const GsBall& G) {
   const uint32_t lane = threadIdx.x & 63u;
-  float2* imp = P.imp + D.b0;
-  const float om_abs = fabsf(D.omega), rest = D.rest, area = D.area, dt = D.dt, mass = D.mass;
-  const uint32_t nb = D.nb, b0 = D.b0;
-  GsBall cur, nxt;
-  gs_load(P, b0, nb, 0, false, cur);
-  for (uint32_t b = 0; b < nb; b++) {
-    gs_load(P, b0, nb, b + 1, false, nxt);
-    const uint32_t nw = cur.n;
-    const float rho_l = cur.st.x, od = cur.st.y;
-    if (nw != 0 && rho_l > 0.0f) {
+  const float rho_l = G.st.x, od = G.st.y;
+  if (nw == 0 || !(rho_l > 0.0f)) return;
+  float2 vj[4];
 #pragma unroll
-      for (int s = 0; s < 4; s++) {
-        const uint32_t k = (uint32_t)s * 64u + lane;
-        if (k < nw) {
-          float term = 0.0f;
-          if (cur.ir[s].w != 0.0f) {
-            const float2 vj = V[cur.ij[s]];
-            const float dx = cur.ir[s].y, dy = cur.ir[s].z;
-            term = cur.ir[s].x * (dx * vj.y - dy * vj.x) / (dx * dx + dy * dy);
-          }
-          s_t0[k] = term;
-        }
-      }
-      wave_sync();
-      float acc = 0.0f;
-      if (lane == 0) acc = ordered_sum(s_t0, nw, 0.0f);
-      // an item on the axis adds nothing to `on` in the reference (it is skipped); its +0.0 term
-      // leaves the running sum unchanged unless the sum is -0.0, which a sum starting at +0.0 never is
-      const float on = rdl(acc, 0);
-      const float2 c = cur.rb;
-      const float rho_eff = fminf(rho_l, rest);
-      const float om_l = od > 0.0f ? fclampf(on / od, -om_abs, om_abs) : 0.0f;
-      const float o2 = om_l * om_l;
-      const float gex = 0.0f + o2 * c.x, gey = FL_G + o2 * c.y;
-      const float sc = (-rho_eff) * area;
-      const float ipx = sc * gex * dt, ipy = sc * gey * dt;
+  for (int s = 0; s < 4; s++) vj[s] = V[(uint32_t)s * 64u + lane < nw ? G.ij[s] : 0u];
 #pragma unroll
-      for (int s = 0; s < 4; s++) {
-        const uint32_t k = (uint32_t)s * 64u + lane;
-        if (k < nw) {
-          const float frac = cur.ir[s].x / rho_l;
-          const float2 vj = V[cur.ij[s]];
-          V[cur.ij[s]] = make_float2(vj.x + -(ipx * frac) / mass, vj.y + -(ipy * frac) / mass);
-        }
+  for (int s = 0; s < 4; s++) {
+    const uint32_t k = (uint32_t)s * 64u + lane;
+    if (k < nw) {
+      float term = 0.0f;
+      if (G.ir[s].w != 0.0f) {
+        const float dx = G.ir[s].y, dy = G.ir[s].z;
+        term = G.ir[s].x * (dx * vj[s].y - dy * vj[s].x) / (dx * dx + dy * dy);
       }
-      if (lane == 0) imp[b] = make_float2(cur.imp.x + ipx, cur.imp.y + ipy);
-      if (!LDSV) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-      wave_sync();
+      s_t0[k] = term;
     }
-    cur = nxt;
+  }
+  wave_sync();
+  float acc = 0.0f;
+  if (lane == 0) acc = ordered_sum(s_t0, nw, 0.0f);
+  // an item on the axis adds nothing to `on` in the reference (it is skipped); its +0.0 term
+  // leaves the running sum unchanged unless the sum is -0.0, which a sum starting at +0.0 never is
+  const float on = rdl(acc, 0);
+  const float2 c = G.rb;
+  const float rho_eff = fminf(rho_l, C.rest);
+  const float om_l = od > 0.0f ? fclampf(on / od, -C.om_abs, C.om_abs) : 0.0f;
+  const float o2 = om_l * om_l;
+  const float gex = 0.0f + o2 * c.x, gey = FL_G + o2 * c.y;
+  const float sc = (-rho_eff) * C.area;
+  const float ipx = sc * gex * C.dt, ipy = sc * gey * C.dt;
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    const uint32_t k = (uint32_t)s * 64u + lane;
+    if (k < nw) {
+      const float frac = G.ir[s].x / rho_l;
+      V[G.ij[s]] = make_float2(vj[s].x + -(ipx * frac) / C.mass, vj[s].y + -(ipy * frac) / C.mass);
+    }
+  }
+  if (lane == 0) C.imp[b] = make_float2(G.imp.x + ipx, G.imp.y + ipy);
+  if (!LDSV) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  wave_sync();
+}
+
+template <bool LDSV, bool DRAG>
+__device__ inline void gs_sweep(const FlPtrs& P, const FlDrum& D, float2* V, float* s_t0, float* s_t1, float* s_t2,
+                                const uint32_t* s_cnt, const uint32_t* s_list, const uint32_t* s_end, uint32_t L) {
+  GsConst C;
+  C.ib = D.binertia;
+  C.rest = D.rest;
+  C.btrue = D.btrue;
+  C.mass_true = D.mass_true;
+  C.beta_b = D.beta_b;
+  C.dt = D.dt;
+  C.bmass = D.bmass;
+  C.hc = D.hc;
+  C.rot_cap = D.rot_cap;
+  C.om_abs = fabsf(D.omega);
+  C.area = D.area;
+  C.mass = D.mass;
+  C.imp = P.imp + D.b0;
+  C.ang = P.ang + D.b0;
+  const uint32_t nb = D.nb, b0 = D.b0, w = threadIdx.x >> 6;
+  // wave w takes entries w, w + GS_WAVES, ... of the level-ordered list; its next ball is loaded
+  // while the current one runs (also across a level's end)
+  uint32_t e = w;
+  GsBall cur, nxt;
+  gs_load(P, b0, nb, e < nb ? s_list[e] : nb, DRAG, cur);
+  for (uint32_t l = 1; l <= L; l++) {
+    const uint32_t end = s_end[l];
+    while (e < end) {
+      const uint32_t en = e + GS_WAVES;
+      gs_load(P, b0, nb, en < nb ? s_list[en] : nb, DRAG, nxt);
+      const uint32_t b = s_list[e];
+      if (DRAG) gs_drag_ball<LDSV>(C, V, s_t0, s_t1, s_t2, s_cnt[b], b, cur);
+      else gs_buoy_ball<LDSV>(C, V, s_t0, s_cnt[b], b, cur);
+      cur = nxt;
+      e = en;
+    }
+    __syncthreads();  // the level's velocity updates before the next level reads them
   }
 }
 
 __device__ inline double gs_ke(const float2* V, uint32_t n, double* s_sum) {
   const uint32_t t = threadIdx.x, lane = t & 63u, wv = t >> 6;
   double s = 0.0;
-  for (uint32_t i = t; i < n; i += FL_T) {
+  for (uint32_t i = t; i < n; i += GS_THREADS) {
     const float2 q = V[i];
     s += (double)(q.x * q.x + q.y * q.y);
   }
@@ -1562,42 +1709,74 @@ __device__ inline double gs_ke(const float2* V, uint32_t n, double* s_sum) {
   if (lane == 0) s_sum[wv] = s;
   __syncthreads();
   double a = 0.0;
-  for (uint32_t q = 0; q < FL_T / 64; q++) a += s_sum[q];
+  for (uint32_t q = 0; q < GS_WAVES; q++) a += s_sum[q];
   __syncthreads();
   return a;
 }
 
-// The sweep: wave 0 walks the balls; the drum's velocities in dynamic LDS when lds != 0.
-extern "C" __global__ void __launch_bounds__(FL_T) fl_gs(FlPtrs P, uint32_t lds_cap) {
+// The sweeps, level by level (GS_WAVES waves); the drum's velocities in dynamic LDS when they fit.
+extern "C" __global__ void __launch_bounds__(GS_THREADS) fl_gs(FlPtrs P, uint32_t lds_cap) {
   extern __shared__ float2 s_v[];
-  __shared__ __align__(16) float s_t0[GS_ITEMS];
-  __shared__ __align__(16) float s_t1[GS_ITEMS];
-  __shared__ __align__(16) float s_t2[GS_ITEMS];
-  __shared__ double s_sum[FL_T / 64];
-  const uint32_t d = blockIdx.x, t = threadIdx.x;
+  __shared__ __align__(16) float s_t[GS_WAVES][3][GS_ITEMS];
+  __shared__ uint32_t s_cnt[GS_MAXB];        // the current sweep's item count of every ball
+  __shared__ uint32_t s_list[GS_MAXB];       // the balls in level order
+  __shared__ uint32_t s_end[GS_MAXB + 1];    // s_end[l]: one past the last entry of level l
+  __shared__ double s_sum[GS_WAVES];
+  const uint32_t d = blockIdx.x, t = threadIdx.x, w = t >> 6, lane = t & 63u;
   const FlDrum& D = P.drums[d];
   FlState& S = P.st[d];
   if (D.n == 0) return;
+  const uint32_t nb = D.nb, L = nb ? min(S.gs_levels, nb) : 0u;
   float2* v = P.v + D.f0;
   const bool lds = D.n <= lds_cap;
-  if (lds) {
-    for (uint32_t i = t; i < D.n; i += FL_T) s_v[i] = v[i];
-    __syncthreads();
+  if (lds)
+    for (uint32_t i = t; i < D.n; i += GS_THREADS) s_v[i] = v[i];
+  // the balls bucketed by level: counts, an exclusive scan (one wave, a run per lane), cursors
+  for (uint32_t l = t; l <= L; l += GS_THREADS) s_end[l] = 0;
+  for (uint32_t b = t; b < nb; b += GS_THREADS) s_cnt[b] = P.gdn[D.b0 + b];
+  __syncthreads();
+  const uint32_t* lev = P.glev + D.b0;
+  for (uint32_t b = t; b < nb; b += GS_THREADS) atomicAdd(&s_end[min(lev[b], L)], 1u);
+  __syncthreads();
+  if (w == 0) {
+    const uint32_t k = (L + 1 + 63) / 64, a = min(L + 1, lane * k), z = min(L + 1, a + k);
+    uint32_t sum = 0;
+    for (uint32_t i = a; i < z; i++) sum += s_end[i];
+    uint32_t x = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o, 64);
+      if (lane >= (uint32_t)o) x += y;
+    }
+    uint32_t run = x - sum;
+    for (uint32_t i = a; i < z; i++) {
+      const uint32_t c = s_end[i];
+      s_end[i] = run;
+      run += c;
+    }
   }
-  if (t < 64 && D.nb > 0) {
-    if (lds) gs_sweep<true>(P, D, s_v, s_t0, s_t1, s_t2);
-    else gs_sweep<false>(P, D, v, s_t0, s_t1, s_t2);
+  __syncthreads();
+  for (uint32_t b = t; b < nb; b += GS_THREADS) s_list[atomicAdd(&s_end[min(lev[b], L)], 1u)] = b;
+  __syncthreads();
+  float* st0 = s_t[w][0];
+  float* st1 = s_t[w][1];
+  float* st2 = s_t[w][2];
+  if (nb > 0) {
+    if (lds) gs_sweep<true, true>(P, D, s_v, st0, st1, st2, s_cnt, s_list, s_end, L);
+    else gs_sweep<false, true>(P, D, v, st0, st1, st2, s_cnt, s_list, s_end, L);
   }
   __syncthreads();
   const double k1 = gs_ke(lds ? s_v : v, D.n, s_sum);
-  if (t < 64 && D.nb > 0 && D.buoy_on) {
-    if (lds) gs_sweep_buoy<true>(P, D, s_v, s_t0);
-    else gs_sweep_buoy<false>(P, D, v, s_t0);
+  for (uint32_t b = t; b < nb; b += GS_THREADS) s_cnt[b] = P.gbn[D.b0 + b];
+  __syncthreads();
+  if (nb > 0 && D.buoy_on) {
+    if (lds) gs_sweep<true, false>(P, D, s_v, st0, st1, st2, s_cnt, s_list, s_end, L);
+    else gs_sweep<false, false>(P, D, v, st0, st1, st2, s_cnt, s_list, s_end, L);
   }
   __syncthreads();
   const double k2 = gs_ke(lds ? s_v : v, D.n, s_sum);
   if (lds)
-    for (uint32_t i = t; i < D.n; i += FL_T) v[i] = s_v[i];
+    for (uint32_t i = t; i < D.n; i += GS_THREADS) v[i] = s_v[i];
   if (t == 0) {
     S.ke_drag = k1;
     S.ke_buoy = k2;
@@ -2784,6 +2963,7 @@ int mg_fluid_step(mg_fluid* h, const mg_fluid_drum* drums, float* d_x, float* d_
   take((void**)&P.gds, TB * 16);
   take((void**)&P.gdb, TB * 8);
   take((void**)&P.gbs, TB * 8);
+  take((void**)&P.glev, TB * 4);
   take((void**)&P.lcnt, TB * 4);
   take((void**)&P.lrec, TL * 16);
   take((void**)&P.locnt, TB * 4);
@@ -2862,10 +3042,10 @@ int mg_fluid_step(mg_fluid* h, const mg_fluid_drum* drums, float* d_x, float* d_
   MG_TIMED_LAUNCH(h->tm, fl_grid_b, G, T, 0, P);
   if (nwt) MG_TIMED_LAUNCH(h->tm, fl_gs_pre, dim3(nwt), T, 0, P, (const uint2*)h->d_wt);
   {
-    // The sweep (one wave per drum walking its balls in order) leaves most of the GPU idle. The
+    // The sweeps' schedule and the sweeps (a few waves per drum) leave most of the GPU idle. The
     // density at the final positions, the Morris weights and the boundary groups read neither the
-    // velocities nor the impulses it writes when no drum is Bingham (a Bingham drum's viscosity
-    // reads the swept velocities), so they then run beside it on a second stream.
+    // velocities nor the impulses the sweeps write when no drum is Bingham (a Bingham drum's
+    // viscosity reads the swept velocities), so they then run beside them on a second stream.
     bool bingham = false;
     for (uint32_t d = 0; d < nd; d++) bingham |= hd[d].bingham != 0 && hd[d].n > 0 && hd[d].mu > 0.0f;
     const bool overlap = !bingham && nft > 0;
@@ -2886,15 +3066,23 @@ int mg_fluid_step(mg_fluid* h, const mg_fluid_drum* drums, float* d_x, float* d_
       (void)hipStreamWaitEvent(sg, h->fork, 0);
       (void)hipStreamWaitEvent(sw, h->fork, 0);
     }
-    // the drum's fluid velocities stay in LDS during the sweep when they fit (160 KiB per workgroup
-    // on gfx950, less the kernel's static LDS)
     uint32_t maxn = 0;
     for (uint32_t d = 0; d < nd; d++) maxn = std::max(maxn, hd[d].n);
-    const size_t cap_bytes = 160u * 1024u - 4096u;
+    {  // the sweeps' levels: a 16-bit last level per particle in LDS (up to 160 KiB)
+      const uint32_t cap = (uint32_t)std::min<size_t>((160u * 1024u) / 2, (size_t)std::max(maxn, 1u));
+      const size_t bytes = (size_t)cap * 2;
+      if (bytes > 65536)
+        (void)hipFuncSetAttribute((const void*)fl_gs_level, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+      MG_TIMED_LAUNCH_S(h->tm, sg, fl_gs_level, G, dim3(64), bytes, P, cap);
+    }
+    // the drum's fluid velocities stay in LDS during the sweep when they fit (160 KiB per workgroup
+    // on gfx950, less the kernel's static LDS)
+    const size_t static_lds = 4u * (GS_WAVES * 3u * GS_ITEMS + 3u * GS_MAXB + 1u) + 8u * GS_WAVES;
+    const size_t cap_bytes = 160u * 1024u - static_lds - 1024u;
     const uint32_t lds_cap = (uint32_t)std::min<size_t>(cap_bytes / 8, (size_t)maxn);
     const size_t bytes = (size_t)lds_cap * 8;
     if (bytes > 65536) (void)hipFuncSetAttribute((const void*)fl_gs, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    MG_TIMED_LAUNCH_S(h->tm, sg, fl_gs, G, T, bytes, P, lds_cap);
+    MG_TIMED_LAUNCH_S(h->tm, sg, fl_gs, G, dim3(GS_THREADS), bytes, P, lds_cap);
     if (nft) {
       MG_TIMED_LAUNCH_S(h->tm, sw, fl_dens2, dim3(nft), T, 0, P);
       MG_TIMED_LAUNCH_S(h->tm, sw, fl_mu, dim3(nft), T, 0, P);
