@@ -455,6 +455,8 @@ typedef struct mg_dem_stats {
   uint32_t collision_count, impact_energy_histogram[12];
   float dissipated_energy_j, max_substep_displacement_over_diameter, coupling_work_j;
   uint32_t n_pairs, n_impacts, n_swept;
+  uint32_t projection_restarts; /* 0: the projection walked the near pairs only; 1: their travel
+                                   check failed and a wider set passed it; 2: it walked every pair */
 } mg_dem_stats;
 typedef struct mg_dem mg_dem;
 /* Workspace for n_drums ball populations of at most max_balls balls each (at most 2048: a

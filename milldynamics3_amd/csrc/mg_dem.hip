@@ -76,6 +76,7 @@ struct DeOut {
   uint32_t depth;   // levels of the projection sweep's schedule (DE_LCAP + 1: sequential)
   uint32_t nact;    // active contacts (normal impulse > 0) of the contact book
   uint32_t cdepth;  // levels of the contact sweeps' schedule (DE_LCAP + 1: sequential)
+  uint32_t restart; // the projection walked every candidate pair after the near pairs' travel check failed
 };
 
 struct DeItem {  // a sorted contact: (i, j) and its accumulated normal impulse
@@ -521,10 +522,13 @@ __device__ uint32_t de_levels(ItemFn item, uint32_t m, uint32_t n, int32_t* s_la
   __syncthreads();
   if (t < 64) {
     int32_t depth = 0;
+    uint32_t ni = 0, nj = 0;  // the next chunk's pair, loaded while this chunk runs
+    bool nact = lane < m && item(lane, ni, nj);
     for (uint32_t base = 0; base < m; base += 64) {
       const uint32_t k = base + lane;
-      uint32_t i = 0, j = 0;
-      const bool act = k < m && item(k, i, j);
+      uint32_t i = ni, j = nj;
+      const bool act = nact;
+      nact = k + 64 < m && item(k + 64, ni, nj);
       int32_t l = act ? 1 + max(s_last[i], s_last[j]) : -1;
       uint64_t am = __ballot(act);
       while (am) {
@@ -593,13 +597,23 @@ __device__ uint32_t de_levels(ItemFn item, uint32_t m, uint32_t n, int32_t* s_la
 
 // ---- 4. XPBD projection (dem.rs:710-750) and the velocity rebuild (:752-756) ---------------------
 // Pairs in emission order, then walls, per iteration.
-// One workgroup per population, the positions in LDS. The pair sweep runs by levels (de_levels over
-// the emission order): per level the lanes update distinct pairs side by side, with the same f32
-// operations as the sequential sweep, and each pair's accumulated impulse is kept in level order
-// (lam_s) and written back to emission order at the end. A schedule deeper than DE_LCAP levels
-// falls back to lane 0 walking the pairs in order.
-__device__ __forceinline__ void de_project_pair(float2* s_x, uint32_t pk, float& lam, float w, float ws, float cap_c,
-                                                float two_r) {
+// One workgroup per population, the positions in LDS. A candidate pair changes nothing unless its
+// balls overlap at its turn, and most candidates (every pair of balls in neighbouring grid cells)
+// never do, so the sweep first takes only the pairs closer than 2r + DE_GAP r at the start (found
+// by all lanes, each over a contiguous run of the emission order, and compacted in that order): by
+// levels of that subset's emission order (de_levels: per level the lanes update distinct pairs side
+// by side, with the same f32 operations as the sequential sweep; each pair's accumulated impulse
+// kept in level order and written back to emission order at the end), every move's length added to
+// its balls' travel. A pair left out is then exact to have skipped if its start distance less both
+// balls' travel still exceeds 2r (with margins over the rounding): it could not have overlapped at
+// any of its turns. If any left-out pair fails that test, the projection starts again from the
+// start positions with the pairs that the first pass's travel (doubled) could have brought within
+// the reach, checked the same way, and failing that with every candidate pair. A schedule deeper
+// than DE_LCAP levels falls back to lane 0 walking all pairs in order.
+#define DE_GAP 0.05f
+#define DE_PCHUNK 512  // pairs staged in LDS per chunk of the projection
+__device__ __forceinline__ void de_project_pair(float2* s_x, float* s_d, uint32_t pk, float& lam, float w, float ws,
+                                                float cap_c, float two_r) {
   const uint32_t i = pk & 0xFFFFu, j = pk >> 16;
   const float dx = s_x[i].x - s_x[j].x, dy = s_x[i].y - s_x[j].y;
   const float dist = sqrtf(dx * dx + dy * dy);
@@ -619,6 +633,10 @@ __device__ __forceinline__ void de_project_pair(float2* s_x, uint32_t pk, float&
   s_x[j].x -= s * nx;
   s_x[j].y -= s * ny;
   lam += dl;
+  if (s_d) {
+    s_d[i] += fabsf(s);
+    s_d[j] += fabsf(s);
+  }
 }
 
 extern "C" __global__ void __launch_bounds__(DE_BLOCK)
@@ -626,88 +644,170 @@ dem_project(const DeDrum* __restrict__ drums, const float2* __restrict__ lift, f
             float2* __restrict__ v, const float* __restrict__ th, float* __restrict__ om,
             const float2* __restrict__ x0, const float* __restrict__ th0, const uint2* __restrict__ pairs,
             float* __restrict__ lam, float* __restrict__ lam_bw, uint32_t* __restrict__ lev_g,
-            uint32_t* __restrict__ pk_g, uint32_t* __restrict__ k_g, float* __restrict__ lam_s, uint32_t stride,
+            uint32_t* __restrict__ pk_g, uint32_t* __restrict__ k_g, float* __restrict__ lam_s,
+            uint32_t* __restrict__ cpk_g, uint32_t* __restrict__ ck_g, float* __restrict__ slack_g, uint32_t stride,
             uint32_t pcap, DeOut* __restrict__ out) {
   __shared__ float2 s_x[DE_MAX_BALLS];
   __shared__ float s_lbw[DE_MAX_BALLS];
-  __shared__ int32_t s_last[DE_MAX_BALLS];
+  __shared__ int32_t s_last[DE_MAX_BALLS];  // the schedule's scratch, then each ball's travel
+  __shared__ float s_d0[DE_MAX_BALLS];      // each ball's travel in the near pass
   __shared__ uint32_t s_off[DE_LCAP + 1];
-  __shared__ uint32_t s_pk[DE_CHUNK];
-  __shared__ float s_l[DE_CHUNK];
-  __shared__ uint32_t s_w[DE_BLOCK / 64], s_depth;
+  __shared__ uint32_t s_pk[DE_PCHUNK];
+  __shared__ float s_l[DE_PCHUNK];
+  __shared__ uint32_t s_w[DE_BLOCK / 64], s_depth, s_fail;
   const DeDrum D = drums[blockIdx.x];
   const mgg::Geo g = de_geo(D, lift);
   const uint32_t t = threadIdx.x, n = D.n;
   const size_t sb = (size_t)blockIdx.x * stride, pb = (size_t)blockIdx.x * pcap;
   const uint32_t np = min(out[blockIdx.x].n_pairs, pcap);
   const uint2* P = pairs + pb;
-  auto item = [P](uint32_t k, uint32_t& i, uint32_t& j) {
-    const uint2 p = P[k];
-    i = p.x;
-    j = p.y;
-    return true;
-  };
-  const uint32_t depth = de_levels(item, np, n, s_last, s_off, s_w, &s_depth, lev_g + pb, pk_g + pb, k_g + pb);
-  const bool leveled = depth <= DE_LCAP;
-  if (t == 0) out[blockIdx.x].depth = depth;
-  for (uint32_t i = t; i < n; i += DE_BLOCK) {
-    s_x[i] = x[D.b0 + i];
-    s_lbw[i] = 0.0f;
-  }
-  if (leveled)
-    for (uint32_t p = t; p < np; p += DE_BLOCK) lam_s[pb + p] = 0.0f;
-  __syncthreads();
+  const float2* X = x + D.b0;  // the start positions (written back only at the end)
   const float w = D.w, ws = D.w + D.w, cap_c = -0.2f * 2.0f * D.r, two_r = 2.0f * D.r;
-  for (uint32_t it = 0; it < max(D.iters, 1u); it++) {
-    uint32_t L = 0;
-    for (uint32_t c0 = 0; c0 < np; c0 += DE_CHUNK) {
-      const uint32_t m = min(np - c0, (uint32_t)DE_CHUNK), c1 = c0 + m;
-      for (uint32_t q = t; q < m; q += DE_BLOCK) {
-        if (leveled) {
-          s_pk[q] = pk_g[pb + c0 + q];
-          s_l[q] = lam_s[pb + c0 + q];
-        } else {
-          const uint2 p = P[c0 + q];
-          s_pk[q] = p.x | (p.y << 16);
-          s_l[q] = lam[pb + c0 + q];
+  const float reach = two_r + DE_GAP * D.r, near2 = reach * reach, sure = two_r * 1.0001f;
+  float* s_d = reinterpret_cast<float*>(s_last);
+  uint32_t* CPK = cpk_g + pb;  // the walked pairs (i | j << 16) in emission order
+  uint32_t* CK = ck_g + pb;    // their emission indices
+  float* SL = slack_g + pb;    // per emission index: start distance less 2r (with a margin) of a left-out pair
+  for (uint32_t i = t; i < n; i += DE_BLOCK) s_x[i] = X[i];
+  __syncthreads();
+  uint32_t depth = 0, na = 0;
+  for (int pass = 0; pass < 3; pass++) {
+    // the pairs this pass walks, in emission order: each lane a contiguous run, counted, scanned,
+    // written (the test evaluated twice, the same way)
+    auto walks = [&](uint32_t k, uint32_t& pk, float& slack) {
+      const uint2 p = P[k];
+      pk = p.x | (p.y << 16);
+      slack = __int_as_float(0x7f800000);
+      if (pass == 2) return true;
+      const float dx = s_x[p.x].x - s_x[p.y].x, dy = s_x[p.x].y - s_x[p.y].y;
+      const float d2 = dx * dx + dy * dy;
+      if (pass == 0) {
+        if (!(d2 >= near2)) return true;  // near, or a non-finite distance
+      } else if (!(sqrtf(d2) - 2.0f * (s_d0[p.x] + s_d0[p.y]) >= reach)) {
+        return true;  // the first pass's travel (doubled) could have brought it near
+      }
+      slack = sqrtf(d2) - sure;
+      return false;
+    };
+    {
+      const uint32_t run = (np + DE_BLOCK - 1) / DE_BLOCK, a = min(np, t * run), e = min(np, a + run);
+      uint32_t c = 0;
+      for (uint32_t k = a; k < e; k++) {
+        uint32_t pk;
+        float sl;
+        c += walks(k, pk, sl) ? 1u : 0u;
+      }
+      uint32_t tot;
+      uint32_t o = de_block_scan(c, s_w, &tot);
+      na = tot;
+      for (uint32_t k = a; k < e; k++) {
+        uint32_t pk;
+        float sl;
+        const bool wk = walks(k, pk, sl);
+        SL[k] = sl;
+        if (wk) {
+          CPK[o] = pk;
+          CK[o] = k;
+          o++;
         }
       }
-      __syncthreads();
-      if (leveled) {
-        while (L < depth && s_off[L] < c1) {  // the levels (or their parts) inside this chunk
-          const uint32_t a = max(s_off[L], c0), e = min(s_off[L + 1], c1);
-          for (uint32_t p = a + t; p < e; p += DE_BLOCK) de_project_pair(s_x, s_pk[p - c0], s_l[p - c0], w, ws, cap_c, two_r);
-          __syncthreads();
-          if (s_off[L + 1] > c1) break;  // the level goes on in the next chunk
-          L++;
-        }
-      } else if (t == 0) {
-        for (uint32_t q = 0; q < m; q++) de_project_pair(s_x, s_pk[q], s_l[q], w, ws, cap_c, two_r);
-      }
-      __syncthreads();
-      for (uint32_t q = t; q < m; q += DE_BLOCK) {
-        if (leveled) lam_s[pb + c0 + q] = s_l[q];
-        else lam[pb + c0 + q] = s_l[q];
-      }
-      __syncthreads();
-    }
-    for (uint32_t i = t; i < n; i += DE_BLOCK) {  // walls: each ball on its own
-      float nx, ny;
-      const float d = mgg::sdf_world(s_x[i].x, s_x[i].y, D.rc1, D.rs1, D.wc1, D.ws1, g, nx, ny);
-      const float c = d - D.r;
-      if (c >= 0.0f) continue;
-      if (w <= 0.0f) continue;
-      const float ce = fmaxf(c, cap_c);
-      const float dl = -ce / w;
-      const float s = w * dl;
-      s_x[i].x += s * nx;
-      s_x[i].y += s * ny;
-      s_lbw[i] += dl;
     }
     __syncthreads();
+    auto item = [CPK](uint32_t q, uint32_t& i, uint32_t& j) {
+      const uint32_t pk = CPK[q];
+      i = pk & 0xFFFFu;
+      j = pk >> 16;
+      return true;
+    };
+    depth = de_levels(item, na, n, s_last, s_off, s_w, &s_depth, lev_g + pb, pk_g + pb, k_g + pb);
+    const bool leveled = depth <= DE_LCAP;
+    const bool track = pass < 2;  // a pass that leaves pairs out checks them afterwards
+    if (t == 0) out[blockIdx.x].depth = depth;
+    for (uint32_t i = t; i < n; i += DE_BLOCK) {
+      s_x[i] = X[i];
+      s_lbw[i] = 0.0f;
+      s_d[i] = 0.0f;
+    }
+    for (uint32_t p = t; p < na; p += DE_BLOCK) lam_s[pb + p] = 0.0f;
+    __syncthreads();
+    for (uint32_t it = 0; it < max(D.iters, 1u); it++) {
+      uint32_t L = 0;
+      for (uint32_t c0 = 0; c0 < na; c0 += DE_PCHUNK) {
+        const uint32_t m = min(na - c0, (uint32_t)DE_PCHUNK), c1 = c0 + m;
+        for (uint32_t q = t; q < m; q += DE_BLOCK) {
+          s_pk[q] = leveled ? pk_g[pb + c0 + q] : CPK[c0 + q];
+          s_l[q] = lam_s[pb + c0 + q];
+        }
+        __syncthreads();
+        if (leveled) {
+          while (L < depth && s_off[L] < c1) {  // the levels (or their parts) inside this chunk
+            const uint32_t a = max(s_off[L], c0), e = min(s_off[L + 1], c1);
+            for (uint32_t p = a + t; p < e; p += DE_BLOCK)
+              de_project_pair(s_x, track ? s_d : nullptr, s_pk[p - c0], s_l[p - c0], w, ws, cap_c, two_r);
+            __syncthreads();
+            if (s_off[L + 1] > c1) break;  // the level goes on in the next chunk
+            L++;
+          }
+        } else if (t == 0) {
+          for (uint32_t q = 0; q < m; q++)
+            de_project_pair(s_x, track ? s_d : nullptr, s_pk[q], s_l[q], w, ws, cap_c, two_r);
+        }
+        __syncthreads();
+        for (uint32_t q = t; q < m; q += DE_BLOCK) lam_s[pb + c0 + q] = s_l[q];
+        __syncthreads();
+      }
+      for (uint32_t i = t; i < n; i += DE_BLOCK) {  // walls: each ball on its own
+        float nx, ny;
+        const float d = mgg::sdf_world(s_x[i].x, s_x[i].y, D.rc1, D.rs1, D.wc1, D.ws1, g, nx, ny);
+        const float c = d - D.r;
+        if (c >= 0.0f) continue;
+        if (w <= 0.0f) continue;
+        const float ce = fmaxf(c, cap_c);
+        const float dl = -ce / w;
+        const float s = w * dl;
+        s_x[i].x += s * nx;
+        s_x[i].y += s * ny;
+        s_lbw[i] += dl;
+        s_d[i] += fabsf(s);
+      }
+      __syncthreads();
+    }
+    if (!track) break;  // every candidate pair was walked
+    // the left-out pairs: start distance less both balls' travel above 2r, with margins
+    if (t == 0) s_fail = 0;
+    __syncthreads();
+    uint32_t bad = 0;
+    for (uint32_t k0 = 0; k0 < np; k0 += 8 * DE_BLOCK) {  // eight pairs per lane in flight
+      float sl[8];
+      uint2 pp[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {  // unconditional loads (a clamped index), so none waits for another
+        const uint32_t k = min(k0 + (uint32_t)u * DE_BLOCK + t, np - 1);
+        sl[u] = SL[k];
+        pp[u] = P[k];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+        if (sl[u] != __int_as_float(0x7f800000) && !(sl[u] - 1.001f * (s_d[pp[u].x] + s_d[pp[u].y]) > 0.0f)) bad = 1;
+    }
+    if (bad) s_fail = 1;
+    __syncthreads();
+    if (!s_fail) break;
+    // restart from the start positions: with the pairs the travel could have brought near, then
+    // with every pair
+    if (t == 0) out[blockIdx.x].restart = pass + 1;
+    if (pass == 0)
+      for (uint32_t i = t; i < n; i += DE_BLOCK) s_d0[i] = s_d[i];
+    for (uint32_t i = t; i < n; i += DE_BLOCK) s_x[i] = X[i];
+    __syncthreads();
   }
-  if (leveled)
-    for (uint32_t p = t; p < np; p += DE_BLOCK) lam[pb + k_g[pb + p]] = lam_s[pb + p];
+  // each walked pair's impulse back to emission order; the others are zero
+  for (uint32_t k = t; k < np; k += DE_BLOCK) lam[pb + k] = 0.0f;
+  __syncthreads();
+  if (depth <= DE_LCAP)
+    for (uint32_t p = t; p < na; p += DE_BLOCK) lam[pb + CK[k_g[pb + p]]] = lam_s[pb + p];
+  else
+    for (uint32_t q = t; q < na; q += DE_BLOCK) lam[pb + CK[q]] = lam_s[pb + q];
   for (uint32_t i = t; i < n; i += DE_BLOCK) {
     const float2 p = s_x[i], p0 = x0[sb + i];
     x[D.b0 + i] = p;
@@ -1436,7 +1536,8 @@ int mg_dem_step(mg_dem* h, const mg_dem_drum* drums, float* d_x, float* d_v, flo
     if (!ok) return de_fail(h, err, errlen, MG_E_DEVICE, "HIP error applying the swept clamp");
   }
   MG_TIMED_LAUNCH(h->tm, dem_project, G, B, 0, h->drums, h->lift, x, v, d_theta, d_omega, h->x0, h->th0, h->pairs,
-                     h->lam, h->lam_bw, h->lev, h->lpk, h->lk, h->lam_s, S, P, h->out);
+                     h->lam, h->lam_bw, h->lev, h->lpk, h->lk, h->lam_s, h->spk, reinterpret_cast<uint32_t*>(h->clam),
+                     h->slam, S, P, h->out);
   MG_TIMED_LAUNCH(h->tm, dem_sort, G, B, 0, h->drums, h->pairs, h->lam, h->sorted, h->spk, h->slam, h->lev, h->lpk,
                      h->lk, h->clam, h->coff, P, h->out);
   MG_TIMED_LAUNCH(h->tm, dem_friction, G, B, 0, h->drums, h->lift, x, v, d_theta, d_omega, h->x0, h->th0, h->spk,
@@ -1465,6 +1566,7 @@ int mg_dem_step(mg_dem* h, const mg_dem_drum* drums, float* d_x, float* d_v, flo
     s.n_pairs = o.n_pairs;
     s.n_impacts = o.n_imp;
     s.n_swept = swept[d];
+    s.projection_restarts = o.restart;
     if (o.n_imp > h->rcap && rc == MG_OK)
       rc = de_fail(h, err, errlen, MG_E_CAPACITY,
                    "drum " + std::to_string(d) + ": more impact records than the handle holds (the count is exact, "

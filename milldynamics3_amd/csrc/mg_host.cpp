@@ -827,7 +827,8 @@ int mg_sim_stats(mg_sim* sim, double* o, size_t cap) {
                       (double)s.impact_energy_histogram[4], (double)s.impact_energy_histogram[5],
                       (double)s.impact_energy_histogram[6], (double)s.impact_energy_histogram[7],
                       (double)s.impact_energy_histogram[8], (double)s.impact_energy_histogram[9],
-                      (double)s.impact_energy_histogram[10], (double)s.impact_energy_histogram[11]};
+                      (double)s.impact_energy_histogram[10], (double)s.impact_energy_histogram[11],
+                      (double)s.projection_restarts};
   const size_t n = sizeof(v) / sizeof(v[0]);
   std::memcpy(o, v, std::min(n, cap) * sizeof(double));
   return (int)n;

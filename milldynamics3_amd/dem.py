@@ -36,7 +36,8 @@ class _Stats(C.Structure):
     _fields_ = [("wall_work_j", C.c_float), ("collision_count", C.c_uint32),
                 ("impact_energy_histogram", C.c_uint32 * 12), ("dissipated_energy_j", C.c_float),
                 ("max_substep_displacement_over_diameter", C.c_float), ("coupling_work_j", C.c_float),
-                ("n_pairs", C.c_uint32), ("n_impacts", C.c_uint32), ("n_swept", C.c_uint32)]
+                ("n_pairs", C.c_uint32), ("n_impacts", C.c_uint32), ("n_swept", C.c_uint32),
+                ("projection_restarts", C.c_uint32)]
 
 
 def _lib():

@@ -173,7 +173,7 @@ STAT_KEYS = ["drum_angle", "sim_time", "power_draw_w", "torque_nm", "collision_r
              "collision_count", "n_groups", "n_links", "links_in_range", "n_ball_pairs",
              "ke_delta_noslip_j", "ke_delta_drag_j", "ke_delta_buoyancy_j", "ke_delta_viscosity_j",
              "ke_delta_clamp_j", "wall_impulse_x", "wall_impulse_y", "wall_angular_impulse",
-             "error_bits", "n_impacts"] + [f"impact_bin_{k}" for k in range(12)]
+             "error_bits", "n_impacts"] + [f"impact_bin_{k}" for k in range(12)] + ["projection_restarts"]
 ENERGY_KEYS = ["elapsed_s", "shaft_work_j", "dem_wall_work_j", "fluid_wall_work_j",
                "ball_contact_dissipation_j", "fluid_wall_slip_j", "fluid_viscous_j",
                "fluid_clamp_removed_j", "interface_created_j", "delta_mechanical_j", "unattributed_j",

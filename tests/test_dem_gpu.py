@@ -8,7 +8,8 @@ collisions, impact histogram, largest displacement, candidate pairs) and the imp
 realtime preset's 407 balls, the 2000-disc drum at 75 % critical speed, a clockwise lifter drum
 with static friction above kinetic (the exp blend) and coupling impulses, balls fired through
 the wall (the swept time-of-impact clamp), and a dozen balls spread over a large drum (the Sparse
-grid layout).
+grid layout); a cluster of heavily overlapping balls (the projection's restart with every
+candidate pair).
 """
 import math
 
@@ -119,7 +120,9 @@ def test_populations_in_one_batch_match_oracle(dm, orc):
                            "media": {"friction_ball_ball_static": 0.4, "friction_ball_wall_static": 0.6},
                            "simulation": {"max_balls": 400}}, ext_seed=3)
     cases.append(("lifters, static friction, impulses", o, p, rng))
-    _run(dm, cases, 12)
+    first = _run(dm, cases, 12)
+    print("projection restarts at the first step: " + ", ".join(
+        f"{label} {g['projection_restarts']}" for (label, _, _, _), g in zip(cases, first)), flush=True)
 
 
 def test_balls_through_the_wall_use_the_swept_clamp(dm, orc):
@@ -157,6 +160,26 @@ def test_sparse_layout_of_a_few_spread_balls(dm, orc):
     o, p, _ = _pop(orc, {"mill": {"diameter_m": 2.0, "speed_value": 5.0}, "media": {"ball_diameter_m": 0.002},
                          "simulation": {"coarse_graining_mode": "off"}}, initial=xy)
     _run(dm, [("sparse", o, p, None)], 8)
+
+
+def test_overlapping_cluster_restarts_the_projection(dm, orc):
+    """The projection walks only the pairs closer than 2.5 r at the start and checks that every
+    other candidate pair's start distance less both balls' travel stays above 2 r. A hexagonal
+    cluster at spacing 1.4 r (every neighbour overlapping by 0.6 r, each correction moving a ball
+    0.2 r, six neighbours, two iterations) travels far more than the 0.5 r margin, so the check fails
+    and the projection restarts (with a wider set of pairs, or every candidate pair); the results
+    stay the reference's."""
+    r = 0.001
+    pts = []
+    for a in range(-4, 5):
+        for b in range(-4, 5):
+            if abs(a + b) <= 4:
+                pts.append(((a + 0.5 * b) * 1.4 * r, b * 1.4 * r * math.sqrt(3.0) / 2.0 - 0.012))
+    xy = np.asarray(pts, np.float32)
+    o, p, _ = _pop(orc, {"media": {"ball_diameter_m": 2 * r}, "simulation": {"coarse_graining_mode": "off"}},
+                   initial=xy)
+    first = _run(dm, [("cluster", o, p, None)], 4)
+    assert first[0]["projection_restarts"] >= 1
 
 
 def test_f64_energy_sums_match_the_mixed_policy(dm, orc):
