@@ -524,6 +524,19 @@ __device__ inline double wave_sum_d(double v) {
   return __shfl(v, 0, 64);
 }
 
+// The largest of the wave's 64 values (unsigned), in every lane: prefix maxima inside each row of
+// 16 lanes by DPP row shifts (lanes shifted in from outside the row read 0), the rows combined by
+// row broadcasts, lane 63 read out.
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true));  // row_shr:1
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true));  // row_shr:2
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true));  // row_shr:4
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true));  // row_shr:8
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false));  // row_bcast:15
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false));  // row_bcast:31
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
 // Wave-local memory ordering for LDS written by some lanes and read by others of the same wave.
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1419,14 +1432,7 @@ __device__ __forceinline__ uint32_t lv_step(uint16_t* s_last, const LvItems& I, 
     if (k < cd) m = max(m, a);
     if (k < cb) m = max(m, c);
   }
-  // the wave's largest level (levels are below 2^12): its bits from the top, one ballot each
-  uint32_t top = 0;
-#pragma unroll
-  for (int bit = 11; bit >= 0; bit--) {
-    const uint32_t cand = top | (1u << bit);
-    if (__ballot(m >= cand)) top = cand;
-  }
-  const uint32_t l = top + 1;
+  const uint32_t l = wave_max_u32(m) + 1;
   wave_sync();
 #pragma unroll
   for (int s = 0; s < 4; s++) {
