@@ -2163,15 +2163,21 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_cg_setup(FlPtrs P) {
 }
 
 // The joint solve's ball rows, one wave per ball (the rows' sums run over the ball's boundary
-// group items and squeeze links; the lanes form 64 terms at a time and the sums take them in
-// order). what = 0: initial guess, right-hand side and Jacobi diagonal (pbf.rs:651-699); 1, 2:
-// A times X (before the iterations) or P (pbf.rs:700-744) into AP.
+// group items and squeeze links). The lanes form 64 terms at a time into LDS; lanes 0, 1, 2 then
+// add them to the row sums in order (a subtracted term stored negated: a - t is a + (-t)).
+// what = 0: initial guess, right-hand side and Jacobi diagonal (pbf.rs:651-699); 1, 2: A times X
+// (before the iterations) or P (pbf.rs:700-744) into AP.
 extern "C" __global__ void __launch_bounds__(FL_T) fl_cg_balls(FlPtrs P, const uint2* __restrict__ btiles, int what) {
+  __shared__ __align__(16) float s_term[FL_T / 64][3][64];
   const uint2 tl = btiles[blockIdx.x];
   const FlDrum& D = P.drums[tl.x];
   const FlState& S = P.st[tl.x];
-  const uint32_t lane = threadIdx.x & 63u, b = tl.y + (threadIdx.x >> 6);
+  const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6, b = tl.y + wv;
   if (b >= D.nb || S.mode != 2 || S.done) return;
+  float* t0 = s_term[wv][0];
+  float* t1 = s_term[wv][1];
+  float* t2 = s_term[wv][2];
+  float* mine = s_term[wv][lane < 3 ? lane : 0];
   const uint32_t n = D.n, nb = D.nb;
   const size_t rb = (size_t)D.f0 + 2 * (size_t)D.b0;
   const size_t gb = ((size_t)D.b0 + b) * P.gcap;
@@ -2182,8 +2188,14 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_cg_balls(FlPtrs P, const u
   const uint32_t nl = P.lcnt[D.b0 + b];
   const float4* lrec = P.lrec + D.l0;
   const float kd = D.dt * D.mass;
+  // lane q < 3 adds its quantity's terms of the chunk (lim of them; quantities >= nq take none)
+  auto add_chunk = [&](float& acc, uint32_t lim, uint32_t nq) {
+    wave_sync();
+    if (lane < nq) acc = ordered_sum(mine, lim, acc);
+    wave_sync();
+  };
   if (what == 0) {
-    float dl = D.bmass, da = D.binertia;
+    float acc = lane == 0 ? D.bmass : D.binertia;  // lane 0: the linear diagonal, lane 1: the angular
     for (uint32_t c0 = 0; c0 < ng; c0 += 64) {
       float tg = 0.0f, ta = 0.0f;
       if (c0 + lane < ng) {
@@ -2191,22 +2203,19 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_cg_balls(FlPtrs P, const u
         tg = kd * q.x;
         ta = kd * q.w;
       }
-      const uint32_t lim = min(64u, ng - c0);
-      for (uint32_t L = 0; L < lim; L++) {
-        dl += rdl(tg, (int)L);
-        da += rdl(ta, (int)L);
-      }
+      t0[lane] = tg;
+      t1[lane] = ta;
+      add_chunk(acc, min(64u, ng - c0), 2);
     }
     for (uint32_t c0 = i0; c0 < i1; c0 += 64) {  // links of earlier balls to this one
-      const float t = c0 + lane < i1 ? D.dt * lrec[lids[c0 + lane]].z : 0.0f;
-      const uint32_t lim = min(64u, i1 - c0);
-      for (uint32_t L = 0; L < lim; L++) dl += rdl(t, (int)L);
+      t0[lane] = c0 + lane < i1 ? D.dt * lrec[lids[c0 + lane]].z : 0.0f;
+      add_chunk(acc, min(64u, i1 - c0), 1);
     }
     for (uint32_t c0 = 0; c0 < nl; c0 += 64) {
-      const float t = c0 + lane < nl ? D.dt * lrec[(size_t)b * D.lcap + c0 + lane].z : 0.0f;
-      const uint32_t lim = min(64u, nl - c0);
-      for (uint32_t L = 0; L < lim; L++) dl += rdl(t, (int)L);
+      t0[lane] = c0 + lane < nl ? D.dt * lrec[(size_t)b * D.lcap + c0 + lane].z : 0.0f;
+      add_chunk(acc, min(64u, nl - c0), 1);
     }
+    const float dl = rdl(acc, 0), da = rdl(acc, 1);
     if (lane == 0) {
       P.inv[rb + n + b] = 1.0f / fmaxf(dl, 1e-20f);
       const float2 bvb = P.bv[D.b0 + b], ib = P.imp[D.b0 + b];
@@ -2223,7 +2232,8 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_cg_balls(FlPtrs P, const u
   const float2* V = (what == 2 ? P.P : P.X) + rb;
   const float2 vb = V[n + b];
   const float w = V[n + nb + b].x;
-  float lx = D.bmass * vb.x, ly = D.bmass * vb.y, ax = D.binertia * w;
+  // lane 0: the linear row's x, lane 1: its y, lane 2: the angular row
+  float acc = lane == 0 ? D.bmass * vb.x : lane == 1 ? D.bmass * vb.y : D.binertia * w;
   for (uint32_t c0 = 0; c0 < ng; c0 += 64) {
     float tx = 0.0f, ty = 0.0f, ta = 0.0f;
     if (c0 + lane < ng) {
@@ -2234,12 +2244,10 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_cg_balls(FlPtrs P, const u
       ty = kd * (q.x * ry - w * q.y);
       ta = kd * ((q.y * ry - q.z * rx) - w * q.w);
     }
-    const uint32_t lim = min(64u, ng - c0);
-    for (uint32_t L = 0; L < lim; L++) {
-      lx -= rdl(tx, (int)L);
-      ly -= rdl(ty, (int)L);
-      ax -= rdl(ta, (int)L);
-    }
+    t0[lane] = -tx;
+    t1[lane] = -ty;
+    t2[lane] = -ta;
+    add_chunk(acc, min(64u, ng - c0), 3);
   }
   for (uint32_t c0 = i0; c0 < i1; c0 += 64) {  // links of earlier balls to this one
     float tx = 0.0f, ty = 0.0f;
@@ -2252,11 +2260,9 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_cg_balls(FlPtrs P, const u
       tx = f * l.x;
       ty = f * l.y;
     }
-    const uint32_t lim = min(64u, i1 - c0);
-    for (uint32_t L = 0; L < lim; L++) {
-      lx -= rdl(tx, (int)L);
-      ly -= rdl(ty, (int)L);
-    }
+    t0[lane] = -tx;
+    t1[lane] = -ty;
+    add_chunk(acc, min(64u, i1 - c0), 2);
   }
   for (uint32_t c0 = 0; c0 < nl; c0 += 64) {
     float tx = 0.0f, ty = 0.0f;
@@ -2273,12 +2279,11 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_cg_balls(FlPtrs P, const u
       tx = f * l.x;
       ty = f * l.y;
     }
-    const uint32_t lim = min(64u, nl - c0);
-    for (uint32_t L = 0; L < lim; L++) {
-      lx += rdl(tx, (int)L);
-      ly += rdl(ty, (int)L);
-    }
+    t0[lane] = tx;
+    t1[lane] = ty;
+    add_chunk(acc, min(64u, nl - c0), 2);
   }
+  const float lx = rdl(acc, 0), ly = rdl(acc, 1), ax = rdl(acc, 2);
   if (lane == 0) {
     P.AP[rb + n + b] = make_float2(lx, ly);
     P.AP[rb + n + nb + b] = make_float2(ax, 0.0f);
