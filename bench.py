@@ -657,7 +657,7 @@ def bench_drums(args, cpu=True, batch_only=False) -> dict:
     total_ms = sum(v[0] for v in kt.values())
     kernels = {k: {"total_ms": round(v[0], 3), "launches": v[1], "avg_us": round(1000.0 * v[0] / max(v[1], 1), 2),
                    "share": round(v[0] / total_ms, 4)}
-               for k, v in sorted(kt.items(), key=lambda kv: -kv[1][0])[:16]}
+               for k, v in sorted(kt.items(), key=lambda kv: -kv[1][0])[:48]}
     b.close()
     # the plain single-drum path
     plain = {}

@@ -23,7 +23,10 @@ static __device__ __constant__ uint64_t kExp2Tab[32] = {
     0x3feeace5422aa0dbull, 0x3feeb737b0cdc5e5ull, 0x3feec49182a3f090ull, 0x3feed503b23e255dull,
     0x3feee89f995ad3adull, 0x3feeff76f2fb5e47ull, 0x3fef199bdd85529cull, 0x3fef3720dcef9069ull,
     0x3fef5818dcfba487ull, 0x3fef7c97337b9b5full, 0x3fefa4afa2a490daull, 0x3fefd0765b6e4540ull};
-__device__ inline float ref_expf(float x) {
+// tab: the 32 entries of kExp2Tab, wherever the caller keeps them (a kernel whose loop keeps vector
+// loads in flight passes an LDS copy: waiting for a table load from global memory waits for every
+// load issued before it).
+__device__ inline float ref_expf_t(float x, const uint64_t* tab) {
   const uint32_t ux = __float_as_uint(x);
   const uint32_t abstop = (ux >> 20) & 0x7ffu;
   if (abstop >= 0x42bu) {  // |x| >= 88, infinities, NaN
@@ -40,7 +43,7 @@ __device__ inline float ref_expf(float x) {
   const uint64_t ki = (uint64_t)__double_as_longlong(kd);
   kd -= shift;
   const double r = z - kd;
-  const uint64_t t = kExp2Tab[ki % 32] + (ki << 47);
+  const uint64_t t = tab[ki % 32] + (ki << 47);
   const double s = __longlong_as_double((long long)t);
   z = c0 * r + c1;
   const double r2 = r * r;
@@ -49,5 +52,6 @@ __device__ inline float ref_expf(float x) {
   y = y * s;
   return (float)y;
 }
+__device__ inline float ref_expf(float x) { return ref_expf_t(x, kExp2Tab); }
 
 }  // namespace mgm

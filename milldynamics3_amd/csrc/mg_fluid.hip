@@ -460,26 +460,31 @@ struct FlPtrs {
 };
 
 // Item merge used by adhesion and the boundary groups: items keyed by a particle index, item k
-// held by lane k % 64 in slot k / 64, created in order of first appearance and accumulated in
-// candidate order (pbf.rs:1553-1574, :2204-2226).
+// held by lane k % 64 in slot k / 64 (NS slots), created in order of first appearance and
+// accumulated in candidate order (pbf.rs:1553-1574, :2204-2226). Adhesion merges with one slot
+// first and merges a ball again with FL_SLOTS slots when its items outgrow 64.
+template <int NS>
 struct Items {
-  uint32_t j[FL_SLOTS];
-  float a[FL_SLOTS], b[FL_SLOTS], c[FL_SLOTS], e[FL_SLOTS];
+  uint32_t j[NS];
+  float a[NS], b[NS], c[NS], e[NS];
 };
-__device__ inline void items_clear(Items& it) {
+template <int NS>
+__device__ inline void items_clear(Items<NS>& it) {
 #pragma unroll
-  for (int s = 0; s < FL_SLOTS; s++) {
+  for (int s = 0; s < NS; s++) {
     it.j[s] = FL_NO;
     it.a[s] = it.b[s] = it.c[s] = it.e[s] = 0.0f;
   }
 }
-// Adds (va, vb, vc, ve) to item jj (uniform values), creating it after the nit existing ones.
-__device__ inline void items_add(Items& it, uint32_t& nit, bool& over, uint32_t lane, uint32_t jj, float va, float vb,
-                                 float vc, float ve) {
+// Adds (va, vb, vc, ve) to item jj (uniform values), creating it after the nit existing ones
+// (past 64 NS items only nit and `over` change).
+template <int NS>
+__device__ inline void items_add(Items<NS>& it, uint32_t& nit, bool& over, uint32_t lane, uint32_t jj, float va,
+                                 float vb, float vc, float ve) {
   // updates are value selects (no branch per slot), so the slots stay in registers
   bool mine = false;
 #pragma unroll
-  for (int s = 0; s < FL_SLOTS; s++) {
+  for (int s = 0; s < NS; s++) {
     const bool m = it.j[s] == jj;
     mine |= m;
     it.a[s] = m ? it.a[s] + va : it.a[s];
@@ -488,14 +493,14 @@ __device__ inline void items_add(Items& it, uint32_t& nit, bool& over, uint32_t 
     it.e[s] = m ? it.e[s] + ve : it.e[s];
   }
   if (__ballot(mine)) return;
-  if (nit >= 64u * FL_SLOTS) {
+  if (nit >= 64u * NS) {
     over = true;
     nit++;
     return;
   }
   const uint32_t ol = nit & 63u, os = nit >> 6;
 #pragma unroll
-  for (int s = 0; s < FL_SLOTS; s++) {
+  for (int s = 0; s < NS; s++) {
     const bool w = lane == ol && (uint32_t)s == os;
     it.j[s] = w ? jj : it.j[s];
     it.a[s] = w ? va : it.a[s];
@@ -952,24 +957,15 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_cohesion(FlPtrs P) {
   P.coh[D.f0 + i] = make_float2(px, py);
 }
 
-// 3.6b adhesion (pbf.rs:1515-1631), one wave per ball: the boundary points' candidates (the grid
-// over the predicted positions, current positions) merged into items in order of first
-// appearance, one bounded pull on the ball and its reaction spread over the items.
-extern "C" __global__ void __launch_bounds__(FL_T) fl_adhesion(FlPtrs P, const uint2* __restrict__ btiles) {
-  const uint2 tl = btiles[blockIdx.x];
-  const FlDrum& D = P.drums[tl.x];
-  const uint32_t lane = threadIdx.x & 63u, b = tl.y + (threadIdx.x >> 6);
-  if (b >= D.nb) return;
-  uint32_t* acnt = P.acnt + D.b0;
-  const float2 bxb = P.bx[D.b0 + b];
-  if (!D.adh_on || D.n == 0 || !fin2(bxb)) {
-    if (lane == 0) acnt[b] = 0;
-    return;
-  }
+// One ball's adhesion with NS item slots a lane; false (and nothing written) when NS is 1 and
+// the ball's items outgrow them.
+template <int NS>
+__device__ __forceinline__ bool adh_ball(const FlPtrs& P, const FlDrum& D, uint2 tl, uint32_t lane, uint32_t b,
+                                         uint32_t* acnt, float2 bxb) {
   const GView g = gview(P.grids + 5 * (size_t)tl.x, P.gstart[0] + D.c0[0], P.gidx[0] + D.f0);
   const float2* x = P.x + D.f0;
-  Items it;
-  items_clear(it);
+  Items<NS> it;
+  items_clear<NS>(it);
   uint32_t nit = 0;
   bool over = false;
   for (uint32_t k = 0; k < D.cpb; k++) {
@@ -1008,11 +1004,12 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_adhesion(FlPtrs P, const u
       }
     }
   }
+  if (NS == 1 && nit > 64u) return false;  // merged again with FL_SLOTS slots
   if (over && lane == 0) atomicOr(&P.st[tl.x].err, FE_ADH);
-  nit = min(nit, 64u * FL_SLOTS);
+  nit = min(nit, 64u * NS);
   float wsum = 0.0f, pdx = 0.0f, pdy = 0.0f;
 #pragma unroll
-  for (int s = 0; s < FL_SLOTS; s++) {  // items in order: slot-major, then lane
+  for (int s = 0; s < NS; s++) {  // items in order: slot-major, then lane
     const uint32_t lim = nit > 64u * s ? min(64u, nit - 64u * s) : 0u;
     for (uint32_t L = 0; L < lim; L++) {
       pdx += rdl(it.a[s], (int)L);
@@ -1023,7 +1020,7 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_adhesion(FlPtrs P, const u
   const float pl = sqrtf(pdx * pdx + pdy * pdy);
   if (nit == 0 || wsum <= 1e-9f || pl <= 1e-9f) {
     if (lane == 0) acnt[b] = 0;
-    return;
+    return true;
   }
   const float bdx = -pdx / pl, bdy = -pdy / pl;
   const float cov = fminf(wsum, 1.0f);
@@ -1041,7 +1038,7 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_adhesion(FlPtrs P, const u
   }
   const size_t base = ((size_t)D.b0 + b) * P.acap;
 #pragma unroll
-  for (int s = 0; s < FL_SLOTS; s++) {
+  for (int s = 0; s < NS; s++) {
     const uint32_t k = (uint32_t)s * 64u + lane;
     if (k < nit) {
       const float frac = it.c[s] / wsum;
@@ -1051,6 +1048,24 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_adhesion(FlPtrs P, const u
       atomicAdd(&P.fcnt[D.f0 + it.j[s]], 1u);
     }
   }
+  return true;
+}
+
+// 3.6b adhesion (pbf.rs:1515-1631), one wave per ball: the boundary points' candidates (the grid
+// over the predicted positions, current positions) merged into items in order of first
+// appearance, one bounded pull on the ball and its reaction spread over the items.
+extern "C" __global__ void __launch_bounds__(FL_T) fl_adhesion(FlPtrs P, const uint2* __restrict__ btiles) {
+  const uint2 tl = btiles[blockIdx.x];
+  const FlDrum& D = P.drums[tl.x];
+  const uint32_t lane = threadIdx.x & 63u, b = tl.y + (threadIdx.x >> 6);
+  if (b >= D.nb) return;
+  uint32_t* acnt = P.acnt + D.b0;
+  const float2 bxb = P.bx[D.b0 + b];
+  if (!D.adh_on || D.n == 0 || !fin2(bxb)) {
+    if (lane == 0) acnt[b] = 0;
+    return;
+  }
+  if (!adh_ball<1>(P, D, tl, lane, b, acnt, bxb)) adh_ball<FL_SLOTS>(P, D, tl, lane, b, acnt, bxb);
 }
 
 // The fluid's adhesion kicks (pbf.rs:1620-1626): each particle's items summed in ball order,
@@ -1237,6 +1252,7 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_grid_b(FlPtrs P) {
 // held in LDS when they fit) then walks the balls in order: gather the items' velocities, the
 // three velocity sums in item order (one per lane), the ball's scalar update, the kicks.
 #define GS_ITEMS 224
+#define GS_RING 6  // balls' records in flight per wave in a sweep of short lists
 
 // Per-ball items and velocity-independent sums, one wave per ball.
 extern "C" __global__ void __launch_bounds__(FL_T) fl_gs_pre(FlPtrs P, const uint2* __restrict__ btiles) {
@@ -1394,55 +1410,79 @@ __device__ __forceinline__ float ordered_sum(const float* __restrict__ v, uint32
 // (the sequential order restricted to dependent balls), so every ball of a level is independent
 // of the others and a particle's updates happen in ball order: walking the levels in order, the
 // balls of a level side by side, reproduces the in-order walks exactly. One wave per drum finds
-// the levels in ball order, keeping the last level of every particle in LDS (16 bits); a drum
-// with more particles than that table holds keeps the plain order (level b + 1). A ball's item
-// lists end at the first FL_NO (fl_gs_pre writes one after the last item of a short list); they
-// are loaded eight balls ahead.
+// the levels in ball order, keeping the last level of every particle in LDS (16 bits; entry n reads
+// as 0 and entry n + 1 takes the writes of idle lanes); a drum with more particles than that table
+// holds keeps the plain order (level b + 1). A ball's lists are one item per lane (lists of up to
+// 64 items; a longer list takes the wide step, which reads the lists as it goes); the short lists
+// and the counts are loaded eight balls ahead, and the levels leave in runs of 64.
 struct LvItems {
-  uint32_t jd[4], jb[4];
+  uint32_t jd, jb, nd, nw;
 };
 __device__ __forceinline__ void lv_load(const FlPtrs& P, size_t g0, uint32_t nb, uint32_t b, LvItems& I) {
   const uint32_t lane = threadIdx.x & 63u;
   const size_t g = g0 + (b < nb ? b : nb - 1);
-#pragma unroll
-  for (int s = 0; s < 4; s++) {
-    const uint32_t k = (uint32_t)s * 64u + lane;
-    I.jd[s] = k < GS_ITEMS ? P.gdj[g * GS_ITEMS + k] : FL_NO;
-    I.jb[s] = k < GS_ITEMS ? P.gbj[g * GS_ITEMS + k] : FL_NO;
-  }
+  I.nd = P.gdn[g];
+  I.nw = P.gbn[g];
+  I.jd = P.gdj[g * GS_ITEMS + lane];
+  I.jb = P.gbj[g * GS_ITEMS + lane];
 }
-// The length of a list held four slots per lane: the position of its first FL_NO.
-__device__ __forceinline__ uint32_t lv_len(const uint32_t (&j)[4]) {
-#pragma unroll
-  for (int s = 0; s < 4; s++) {
-    const uint64_t m = __ballot(j[s] == FL_NO);
-    if (m) return (uint32_t)s * 64u + (uint32_t)__builtin_ctzll(m);
-  }
-  return 256u;
-}
-__device__ __forceinline__ uint32_t lv_step(uint16_t* s_last, const LvItems& I, uint32_t* lev, uint32_t b, uint32_t nb) {
-  if (b >= nb) return 0;
+// The level of a ball with a list longer than 64 items: the first 64 items of each list from the
+// registers, the rest read here, 64 at a time.
+__device__ __forceinline__ uint32_t lv_wide(const FlPtrs& P, uint16_t* s_last, const LvItems& I, size_t g, uint32_t nd,
+                                            uint32_t nw, uint32_t n) {
   const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t cd = lv_len(I.jd), cb = lv_len(I.jb);
-  uint32_t m = 0;
-#pragma unroll
-  for (int s = 0; s < 4; s++) {
-    const uint32_t k = (uint32_t)s * 64u + lane;
-    const uint32_t a = s_last[k < cd ? I.jd[s] : 0u], c = s_last[k < cb ? I.jb[s] : 0u];
-    if (k < cd) m = max(m, a);
-    if (k < cb) m = max(m, c);
-  }
+  uint32_t m = max(s_last[lane < nd ? I.jd : n], s_last[lane < nw ? I.jb : n]);
+  for (uint32_t k = lane + 64; k < nd; k += 64) m = max(m, (uint32_t)s_last[P.gdj[g * GS_ITEMS + k]]);
+  for (uint32_t k = lane + 64; k < nw; k += 64) m = max(m, (uint32_t)s_last[P.gbj[g * GS_ITEMS + k]]);
   const uint32_t l = wave_max_u32(m) + 1;
   wave_sync();
-#pragma unroll
-  for (int s = 0; s < 4; s++) {
-    const uint32_t k = (uint32_t)s * 64u + lane;
-    if (k < cd) s_last[I.jd[s]] = (uint16_t)l;
-    if (k < cb) s_last[I.jb[s]] = (uint16_t)l;
-  }
-  if (lane == 0) lev[b] = l;
+  s_last[lane < nd ? I.jd : n + 1] = (uint16_t)l;
+  s_last[lane < nw ? I.jb : n + 1] = (uint16_t)l;
+  for (uint32_t k = lane + 64; k < nd; k += 64) s_last[P.gdj[g * GS_ITEMS + k]] = (uint16_t)l;
+  for (uint32_t k = lane + 64; k < nw; k += 64) s_last[P.gbj[g * GS_ITEMS + k]] = (uint16_t)l;
   wave_sync();
   return l;
+}
+template <bool WIDE>
+__device__ __forceinline__ uint32_t lv_step(const FlPtrs& P, uint16_t* s_last, const LvItems& I, size_t g0, uint32_t b,
+                                            uint32_t nb, uint32_t n) {
+  if (b >= nb) return 0;
+  const uint32_t nd = __builtin_amdgcn_readfirstlane(I.nd), nw = __builtin_amdgcn_readfirstlane(I.nw);
+  if (WIDE && (nd > 64u || nw > 64u)) return lv_wide(P, s_last, I, g0 + b, nd, nw, n);
+  const uint32_t lane = threadIdx.x & 63u;
+  const bool vd = lane < nd, vb = lane < nw;
+  const uint32_t a = s_last[vd ? I.jd : n], c = s_last[vb ? I.jb : n];
+  const uint32_t l = wave_max_u32(max(a, c)) + 1;
+  wave_sync();
+  s_last[vd ? I.jd : n + 1] = (uint16_t)l;
+  s_last[vb ? I.jb : n + 1] = (uint16_t)l;
+  wave_sync();
+  return l;
+}
+// The levels of a drum's balls in ball order; WIDE: some list is longer than 64 items (a separate
+// loop, so that the short steps' waits for the loads eight balls ahead stay exact).
+template <bool WIDE>
+__device__ __forceinline__ uint32_t lv_walk(const FlPtrs& P, uint16_t* s_last, uint32_t* lev, size_t g0, uint32_t nb,
+                                            uint32_t n) {
+  const uint32_t lane = threadIdx.x & 63u;
+  LvItems R[8];
+#pragma unroll
+  for (int r = 0; r < 8; r++) lv_load(P, g0, nb, (uint32_t)r, R[r]);
+  uint32_t top = 0, run = 0;  // run: lane q holds the level of ball 64 k + q of the current run
+  for (uint32_t b = 0; b < nb; b += 8) {
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+      const uint32_t l = lv_step<WIDE>(P, s_last, R[r], g0, b + r, nb, n);
+      lv_load(P, g0, nb, b + r + 8, R[r]);
+      top = max(top, l);
+      if (lane == ((b + r) & 63u)) run = l;
+    }
+    if (((b + 8) & 63u) == 0 || b + 8 >= nb) {
+      const uint32_t q = (b & ~63u) + lane;
+      if (q < nb) lev[q] = run;
+    }
+  }
+  return top;
 }
 extern "C" __global__ void __launch_bounds__(64) fl_gs_level(FlPtrs P, uint32_t cap) {
   extern __shared__ uint16_t s_last[];
@@ -1452,42 +1492,18 @@ extern "C" __global__ void __launch_bounds__(64) fl_gs_level(FlPtrs P, uint32_t 
   const uint32_t n = D.n, nb = D.nb;
   if (n == 0 || nb == 0) return;
   uint32_t* lev = P.glev + D.b0;
-  if (n > cap) {
+  if (n + 2 > cap) {
     for (uint32_t b = lane; b < nb; b += 64) lev[b] = b + 1;
     if (lane == 0) S.gs_levels = nb;
     return;
   }
-  for (uint32_t i = lane; i < n; i += 64) s_last[i] = 0;
-  wave_sync();
   const size_t g0 = (size_t)D.b0;
-  LvItems R0, R1, R2, R3, R4, R5, R6, R7;
-  lv_load(P, g0, nb, 0, R0);
-  lv_load(P, g0, nb, 1, R1);
-  lv_load(P, g0, nb, 2, R2);
-  lv_load(P, g0, nb, 3, R3);
-  lv_load(P, g0, nb, 4, R4);
-  lv_load(P, g0, nb, 5, R5);
-  lv_load(P, g0, nb, 6, R6);
-  lv_load(P, g0, nb, 7, R7);
-  uint32_t top = 0;
-  for (uint32_t b = 0; b < nb; b += 8) {
-    top = max(top, lv_step(s_last, R0, lev, b, nb));
-    lv_load(P, g0, nb, b + 8, R0);
-    top = max(top, lv_step(s_last, R1, lev, b + 1, nb));
-    lv_load(P, g0, nb, b + 9, R1);
-    top = max(top, lv_step(s_last, R2, lev, b + 2, nb));
-    lv_load(P, g0, nb, b + 10, R2);
-    top = max(top, lv_step(s_last, R3, lev, b + 3, nb));
-    lv_load(P, g0, nb, b + 11, R3);
-    top = max(top, lv_step(s_last, R4, lev, b + 4, nb));
-    lv_load(P, g0, nb, b + 12, R4);
-    top = max(top, lv_step(s_last, R5, lev, b + 5, nb));
-    lv_load(P, g0, nb, b + 13, R5);
-    top = max(top, lv_step(s_last, R6, lev, b + 6, nb));
-    lv_load(P, g0, nb, b + 14, R6);
-    top = max(top, lv_step(s_last, R7, lev, b + 7, nb));
-    lv_load(P, g0, nb, b + 15, R7);
-  }
+  uint32_t mx = 0;
+  for (uint32_t b = lane; b < nb; b += 64) mx = max(mx, max(P.gdn[g0 + b], P.gbn[g0 + b]));
+  for (uint32_t i = lane; i < n + 2; i += 64) s_last[i] = 0;
+  const bool wide = wave_max_u32(mx) > 64u;
+  wave_sync();
+  const uint32_t top = wide ? lv_walk<true>(P, s_last, lev, g0, nb, n) : lv_walk<false>(P, s_last, lev, g0, nb, n);
   if (lane == 0) S.gs_levels = top;
 }
 
@@ -1501,34 +1517,36 @@ extern "C" __global__ void __launch_bounds__(64) fl_gs_level(FlPtrs P, uint32_t 
 #define GS_MAXB 2048  // balls per drum (the fluid operator's limit)
 #define GS_WAVES 8    // waves of the sweep kernel
 #define GS_THREADS (64 * GS_WAVES)
+// A ball's record for a sweep; its items NS slots of 64 (one slot when every list of the drum's
+// sweep holds at most 64 items, else four).
+template <int NS>
 struct GsBall {
   float4 st;  // drag: m_ent, i_ent, s, s_scale; buoyancy: rho_l, od, -, -
-  float2 rb, bv, imp;
-  float bom, ang;
-  uint32_t ij[4];
-  float4 ir[4];
+  float2 rb, bv;
+  float bom;
+  uint32_t ij[NS];
+  float4 ir[NS];
 };
-__device__ __forceinline__ void gs_load(const FlPtrs& P, uint32_t b0, uint32_t nb, uint32_t b, bool drag, GsBall& B) {
+template <int NS>
+__device__ __forceinline__ void gs_load(const FlPtrs& P, uint32_t b0, uint32_t nb, uint32_t b, bool drag,
+                                        GsBall<NS>& B) {
   const uint32_t lane = threadIdx.x & 63u;
   const size_t gs = b < nb ? (size_t)b0 + b : (size_t)b0;  // past the last ball: any valid address
   if (drag) {
     B.st = P.gds[gs];
     B.rb = P.gdb[gs];
     B.bom = P.bom[gs];
-    B.ang = P.ang[gs];
   } else {
     const float2 q = P.gbs[gs];
     B.st = make_float4(q.x, q.y, 0.0f, 0.0f);
     B.rb = P.bx[gs];
     B.bom = 0.0f;
-    B.ang = 0.0f;
   }
   B.bv = P.bv[gs];
-  B.imp = P.imp[gs];
   // every slot of the ball's item block is loaded (the block is always allocated), so the loads do
   // not wait for the count; slots past it are never used
 #pragma unroll
-  for (int s = 0; s < 4; s++) {
+  for (int s = 0; s < NS; s++) {
     const uint32_t k = (uint32_t)s * 64u + lane;
     B.ij[s] = 0;
     B.ir[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -1540,42 +1558,47 @@ __device__ __forceinline__ void gs_load(const FlPtrs& P, uint32_t b0, uint32_t n
 }
 
 // The drum's constants of the sweeps, in registers (the loop's stores would otherwise make the
-// compiler reload them, and every reload waits for the prefetched loads).
+// compiler reload them, and every reload waits for the prefetched loads). The balls' impulses and
+// angular impulses are LDS copies during the sweeps: a global store in the loop would make a later
+// load into its data registers wait for every load in flight.
 struct GsConst {
   float ib, rest, btrue, mass_true, beta_b, dt, bmass, hc, rot_cap, om_abs, area, mass;
   float2* imp;
   float* ang;
+  const uint64_t* exp_tab;  // an LDS copy of the expf table
 };
 
-// Form drag of ball b (pbf.rs:1824-2011).
-template <bool LDSV>
+// Form drag of ball b (pbf.rs:1824-2011). The three item sums run over chunks of 64 terms in LDS,
+// added in item order by lanes 0, 1, 2.
+template <bool LDSV, int NS>
 __device__ __forceinline__ void gs_drag_ball(const GsConst& C, float2* V, float* s_t0, float* s_t1, float* s_t2,
-                                             uint32_t nit, uint32_t b, const GsBall& G) {
+                                             uint32_t nit, uint32_t b, const GsBall<NS>& G) {
   const uint32_t lane = threadIdx.x & 63u;
   const float m_ent = G.st.x, i_ent = G.st.y;
   if (nit == 0 || !(m_ent > 1e-12f)) return;
-  float2 vj[4];
+  float2 vj[NS];
 #pragma unroll
-  for (int s = 0; s < 4; s++) vj[s] = V[(uint32_t)s * 64u + lane < nit ? G.ij[s] : 0u];
+  for (int s = 0; s < NS; s++) vj[s] = V[(uint32_t)s * 64u + lane < nit ? G.ij[s] : 0u];
+  float acc = 0.0f;
+  const float* mine = lane == 0 ? s_t0 : lane == 1 ? s_t1 : s_t2;
 #pragma unroll
-  for (int s = 0; s < 4; s++) {
-    const uint32_t k = (uint32_t)s * 64u + lane;
-    if (k < nit) {
+  for (int s = 0; s < NS; s++) {
+    if ((uint32_t)s * 64u < nit) {
       const float w = G.ir[s].w;
-      s_t0[k] = w * vj[s].x;
-      s_t1[k] = w * vj[s].y;
-      s_t2[k] = w * (G.ir[s].x * vj[s].y - G.ir[s].y * vj[s].x);
+      s_t0[lane] = w * vj[s].x;
+      s_t1[lane] = w * vj[s].y;
+      s_t2[lane] = w * (G.ir[s].x * vj[s].y - G.ir[s].y * vj[s].x);
+      wave_sync();
+      if (lane < 3) acc = ordered_sum(mine, min(64u, nit - (uint32_t)s * 64u), acc);
+      wave_sync();
     }
   }
-  wave_sync();
-  float acc = 0.0f;
-  if (lane < 3) acc = ordered_sum(lane == 0 ? s_t0 : lane == 1 ? s_t1 : s_t2, nit, 0.0f);
   const float vsx = rdl(acc, 0), vsy = rdl(acc, 1), l_sum = rdl(acc, 2);
   const float rbx = G.rb.x, rby = G.rb.y, vbx = vsx / m_ent, vby = vsy / m_ent;
   const float vrx = vbx - G.bv.x, vry = vby - G.bv.y;
   const float sp = sqrtf(vrx * vrx + vry * vry);
   const float itf = (sp > 1e-6f && C.rest > 0.0f) ? 0.5f * C.rest * 1.0f * (2.0f * C.btrue) * sp / C.mass_true : 0.0f;
-  const float bt = C.beta_b * (1.0f - mgm::ref_expf(-C.dt * (0.0f + itf)));
+  const float bt = C.beta_b * (1.0f - mgm::ref_expf_t(-C.dt * (0.0f + itf), C.exp_tab));
   const float a_lin = bt * m_ent / (C.bmass + m_ent);
   const float dvbx = a_lin * vrx, dvby = a_lin * vry;
   float dwb = 0.0f, c_rot = 0.0f, pbx = 0.0f, pby = 0.0f;
@@ -1596,7 +1619,7 @@ __device__ __forceinline__ void gs_drag_ball(const GsConst& C, float2* V, float*
   }
   const float f = -(C.bmass / m_ent);
 #pragma unroll
-  for (int s = 0; s < 4; s++) {
+  for (int s = 0; s < NS; s++) {
     const uint32_t k = (uint32_t)s * 64u + lane;
     if (k < nit) {
       const float p = G.ir[s].z;
@@ -1610,38 +1633,40 @@ __device__ __forceinline__ void gs_drag_ball(const GsConst& C, float2* V, float*
     }
   }
   if (lane == 0) {
-    C.imp[b] = make_float2(G.imp.x + C.bmass * dvbx, G.imp.y + C.bmass * dvby);
-    C.ang[b] = G.ang + (C.ib * dwb + C.bmass * (rbx * dvby - rby * dvbx));
+    const float2 ib = C.imp[b];
+    C.imp[b] = make_float2(ib.x + C.bmass * dvbx, ib.y + C.bmass * dvby);
+    C.ang[b] = C.ang[b] + (C.ib * dwb + C.bmass * (rbx * dvby - rby * dvbx));
   }
   if (!LDSV) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
   wave_sync();
 }
 
-// Buoyancy of ball b (pbf.rs:2035-2122).
-template <bool LDSV>
+// Buoyancy of ball b (pbf.rs:2035-2122); the item sum runs over chunks of 64 terms in LDS, added
+// in item order by lane 0.
+template <bool LDSV, int NS>
 __device__ __forceinline__ void gs_buoy_ball(const GsConst& C, float2* V, float* s_t0, uint32_t nw, uint32_t b,
-                                             const GsBall& G) {
+                                             const GsBall<NS>& G) {
   const uint32_t lane = threadIdx.x & 63u;
   const float rho_l = G.st.x, od = G.st.y;
   if (nw == 0 || !(rho_l > 0.0f)) return;
-  float2 vj[4];
+  float2 vj[NS];
 #pragma unroll
-  for (int s = 0; s < 4; s++) vj[s] = V[(uint32_t)s * 64u + lane < nw ? G.ij[s] : 0u];
+  for (int s = 0; s < NS; s++) vj[s] = V[(uint32_t)s * 64u + lane < nw ? G.ij[s] : 0u];
+  float acc = 0.0f;
 #pragma unroll
-  for (int s = 0; s < 4; s++) {
-    const uint32_t k = (uint32_t)s * 64u + lane;
-    if (k < nw) {
+  for (int s = 0; s < NS; s++) {
+    if ((uint32_t)s * 64u < nw) {
       float term = 0.0f;
       if (G.ir[s].w != 0.0f) {
         const float dx = G.ir[s].y, dy = G.ir[s].z;
         term = G.ir[s].x * (dx * vj[s].y - dy * vj[s].x) / (dx * dx + dy * dy);
       }
-      s_t0[k] = term;
+      s_t0[lane] = term;
+      wave_sync();
+      if (lane == 0) acc = ordered_sum(s_t0, min(64u, nw - (uint32_t)s * 64u), acc);
+      wave_sync();
     }
   }
-  wave_sync();
-  float acc = 0.0f;
-  if (lane == 0) acc = ordered_sum(s_t0, nw, 0.0f);
   // an item on the axis adds nothing to `on` in the reference (it is skipped); its +0.0 term
   // leaves the running sum unchanged unless the sum is -0.0, which a sum starting at +0.0 never is
   const float on = rdl(acc, 0);
@@ -1653,22 +1678,32 @@ __device__ __forceinline__ void gs_buoy_ball(const GsConst& C, float2* V, float*
   const float sc = (-rho_eff) * C.area;
   const float ipx = sc * gex * C.dt, ipy = sc * gey * C.dt;
 #pragma unroll
-  for (int s = 0; s < 4; s++) {
+  for (int s = 0; s < NS; s++) {
     const uint32_t k = (uint32_t)s * 64u + lane;
     if (k < nw) {
       const float frac = G.ir[s].x / rho_l;
       V[G.ij[s]] = make_float2(vj[s].x + -(ipx * frac) / C.mass, vj[s].y + -(ipy * frac) / C.mass);
     }
   }
-  if (lane == 0) C.imp[b] = make_float2(G.imp.x + ipx, G.imp.y + ipy);
+  if (lane == 0) {
+    const float2 ib = C.imp[b];
+    C.imp[b] = make_float2(ib.x + ipx, ib.y + ipy);
+  }
   if (!LDSV) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
   wave_sync();
 }
 
-template <bool LDSV, bool DRAG>
+// One sweep, level by level: wave w takes entries w, w + GS_WAVES, ... of the level-ordered list
+// and meets the other waves at a barrier after every level (L barriers in all, whichever levels
+// hold its entries). Its balls' records are loaded K entries ahead (a register ring, rotated by
+// the unrolled loop), also across levels: a record holds nothing the sweep writes before the
+// ball's own turn.
+template <bool LDSV, bool DRAG, int NS, int K>
 __device__ inline void gs_sweep(const FlPtrs& P, const FlDrum& D, float2* V, float* s_t0, float* s_t1, float* s_t2,
-                                const uint32_t* s_cnt, const uint32_t* s_list, const uint32_t* s_end, uint32_t L) {
+                                const uint32_t* s_cnt, const uint32_t* s_list, const uint32_t* s_end, uint32_t L,
+                                float2* s_imp, float* s_ang, const uint64_t* s_exp) {
   GsConst C;
+  C.exp_tab = s_exp;
   C.ib = D.binertia;
   C.rest = D.rest;
   C.btrue = D.btrue;
@@ -1681,27 +1716,36 @@ __device__ inline void gs_sweep(const FlPtrs& P, const FlDrum& D, float2* V, flo
   C.om_abs = fabsf(D.omega);
   C.area = D.area;
   C.mass = D.mass;
-  C.imp = P.imp + D.b0;
-  C.ang = P.ang + D.b0;
+  C.imp = s_imp;
+  C.ang = s_ang;
   const uint32_t nb = D.nb, b0 = D.b0, w = threadIdx.x >> 6;
-  // wave w takes entries w, w + GS_WAVES, ... of the level-ordered list; its next ball is loaded
-  // while the current one runs (also across a level's end)
-  uint32_t e = w;
-  GsBall cur, nxt;
-  gs_load(P, b0, nb, e < nb ? s_list[e] : nb, DRAG, cur);
-  for (uint32_t l = 1; l <= L; l++) {
-    const uint32_t end = s_end[l];
-    while (e < end) {
-      const uint32_t en = e + GS_WAVES;
-      gs_load(P, b0, nb, en < nb ? s_list[en] : nb, DRAG, nxt);
-      const uint32_t b = s_list[e];
-      if (DRAG) gs_drag_ball<LDSV>(C, V, s_t0, s_t1, s_t2, s_cnt[b], b, cur);
-      else gs_buoy_ball<LDSV>(C, V, s_t0, s_cnt[b], b, cur);
-      cur = nxt;
-      e = en;
-    }
-    __syncthreads();  // the level's velocity updates before the next level reads them
+  GsBall<NS> R[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const uint32_t e = w + (uint32_t)k * GS_WAVES;
+    gs_load<NS>(P, b0, nb, e < nb ? s_list[e] : nb, DRAG, R[k]);
   }
+  uint32_t l = 1;
+  for (uint32_t e0 = w; e0 < nb; e0 += K * GS_WAVES) {
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const uint32_t e = e0 + (uint32_t)k * GS_WAVES;
+      if (e < nb) {
+        while (e >= s_end[l]) {
+          __syncthreads();  // the level's velocity updates before the next level reads them
+          l++;
+        }
+        const uint32_t b = s_list[e];
+        if (DRAG) gs_drag_ball<LDSV, NS>(C, V, s_t0, s_t1, s_t2, s_cnt[b], b, R[k]);
+        else gs_buoy_ball<LDSV, NS>(C, V, s_t0, s_cnt[b], b, R[k]);
+      }
+      // every slot reloads on every pass (past the end: any valid record), so that the number of
+      // loads issued after each slot's is the same on every path
+      const uint32_t en = e + K * GS_WAVES;
+      gs_load<NS>(P, b0, nb, en < nb ? s_list[en] : nb, DRAG, R[k]);
+    }
+  }
+  for (; l <= L; l++) __syncthreads();
 }
 
 __device__ inline double gs_ke(const float2* V, uint32_t n, double* s_sum) {
@@ -1720,14 +1764,17 @@ __device__ inline double gs_ke(const float2* V, uint32_t n, double* s_sum) {
   return a;
 }
 
-// The sweeps, level by level (GS_WAVES waves); the drum's velocities in dynamic LDS when they fit.
-extern "C" __global__ void __launch_bounds__(GS_THREADS) fl_gs(FlPtrs P, uint32_t lds_cap) {
+// The sweeps, level by level (GS_WAVES waves). Dynamic LDS: the drum's velocities (lds_cap
+// entries, used when the drum's fit) and the balls' impulses and angular impulses (bcap entries).
+extern "C" __global__ void __launch_bounds__(GS_THREADS) fl_gs(FlPtrs P, uint32_t lds_cap, uint32_t bcap) {
   extern __shared__ float2 s_v[];
-  __shared__ __align__(16) float s_t[GS_WAVES][3][GS_ITEMS];
+  __shared__ __align__(16) float s_t[GS_WAVES][3][64];
   __shared__ uint32_t s_cnt[GS_MAXB];        // the current sweep's item count of every ball
   __shared__ uint32_t s_list[GS_MAXB];       // the balls in level order
   __shared__ uint32_t s_end[GS_MAXB + 1];    // s_end[l]: one past the last entry of level l
   __shared__ double s_sum[GS_WAVES];
+  __shared__ uint32_t s_wide[2];             // the longest drag and buoyancy lists
+  __shared__ uint64_t s_exp[32];             // the expf table
   const uint32_t d = blockIdx.x, t = threadIdx.x, w = t >> 6, lane = t & 63u;
   const FlDrum& D = P.drums[d];
   FlState& S = P.st[d];
@@ -1735,11 +1782,34 @@ extern "C" __global__ void __launch_bounds__(GS_THREADS) fl_gs(FlPtrs P, uint32_
   const uint32_t nb = D.nb, L = nb ? min(S.gs_levels, nb) : 0u;
   float2* v = P.v + D.f0;
   const bool lds = D.n <= lds_cap;
+  float2* s_imp = s_v + lds_cap;
+  float* s_ang = reinterpret_cast<float*>(s_imp + bcap);
   if (lds)
     for (uint32_t i = t; i < D.n; i += GS_THREADS) s_v[i] = v[i];
+  for (uint32_t b = t; b < nb; b += GS_THREADS) {
+    s_imp[b] = P.imp[D.b0 + b];
+    s_ang[b] = P.ang[D.b0 + b];
+  }
   // the balls bucketed by level: counts, an exclusive scan (one wave, a run per lane), cursors
   for (uint32_t l = t; l <= L; l += GS_THREADS) s_end[l] = 0;
-  for (uint32_t b = t; b < nb; b += GS_THREADS) s_cnt[b] = P.gdn[D.b0 + b];
+  if (t < 2) s_wide[t] = 0;
+  if (t < 32) s_exp[t] = mgm::kExp2Tab[t];
+  __syncthreads();
+  {
+    uint32_t md = 0, mw = 0;
+    for (uint32_t b = t; b < nb; b += GS_THREADS) {
+      const uint32_t c = P.gdn[D.b0 + b];
+      s_cnt[b] = c;
+      md = max(md, c);
+      mw = max(mw, P.gbn[D.b0 + b]);
+    }
+    md = wave_max_u32(md);
+    mw = wave_max_u32(mw);
+    if (lane == 0) {
+      atomicMax(&s_wide[0], md);
+      atomicMax(&s_wide[1], mw);
+    }
+  }
   __syncthreads();
   const uint32_t* lev = P.glev + D.b0;
   for (uint32_t b = t; b < nb; b += GS_THREADS) atomicAdd(&s_end[min(lev[b], L)], 1u);
@@ -1768,21 +1838,35 @@ extern "C" __global__ void __launch_bounds__(GS_THREADS) fl_gs(FlPtrs P, uint32_
   float* st1 = s_t[w][1];
   float* st2 = s_t[w][2];
   if (nb > 0) {
-    if (lds) gs_sweep<true, true>(P, D, s_v, st0, st1, st2, s_cnt, s_list, s_end, L);
-    else gs_sweep<false, true>(P, D, v, st0, st1, st2, s_cnt, s_list, s_end, L);
+    if (s_wide[0] > 64u) {
+      if (lds) gs_sweep<true, true, 4, 1>(P, D, s_v, st0, st1, st2, s_cnt, s_list, s_end, L, s_imp, s_ang, s_exp);
+      else gs_sweep<false, true, 4, 1>(P, D, v, st0, st1, st2, s_cnt, s_list, s_end, L, s_imp, s_ang, s_exp);
+    } else {
+      if (lds) gs_sweep<true, true, 1, GS_RING>(P, D, s_v, st0, st1, st2, s_cnt, s_list, s_end, L, s_imp, s_ang, s_exp);
+      else gs_sweep<false, true, 1, GS_RING>(P, D, v, st0, st1, st2, s_cnt, s_list, s_end, L, s_imp, s_ang, s_exp);
+    }
   }
   __syncthreads();
   const double k1 = gs_ke(lds ? s_v : v, D.n, s_sum);
   for (uint32_t b = t; b < nb; b += GS_THREADS) s_cnt[b] = P.gbn[D.b0 + b];
   __syncthreads();
   if (nb > 0 && D.buoy_on) {
-    if (lds) gs_sweep<true, false>(P, D, s_v, st0, st1, st2, s_cnt, s_list, s_end, L);
-    else gs_sweep<false, false>(P, D, v, st0, st1, st2, s_cnt, s_list, s_end, L);
+    if (s_wide[1] > 64u) {
+      if (lds) gs_sweep<true, false, 4, 1>(P, D, s_v, st0, st1, st2, s_cnt, s_list, s_end, L, s_imp, s_ang, s_exp);
+      else gs_sweep<false, false, 4, 1>(P, D, v, st0, st1, st2, s_cnt, s_list, s_end, L, s_imp, s_ang, s_exp);
+    } else {
+      if (lds) gs_sweep<true, false, 1, GS_RING>(P, D, s_v, st0, st1, st2, s_cnt, s_list, s_end, L, s_imp, s_ang, s_exp);
+      else gs_sweep<false, false, 1, GS_RING>(P, D, v, st0, st1, st2, s_cnt, s_list, s_end, L, s_imp, s_ang, s_exp);
+    }
   }
   __syncthreads();
   const double k2 = gs_ke(lds ? s_v : v, D.n, s_sum);
   if (lds)
     for (uint32_t i = t; i < D.n; i += GS_THREADS) v[i] = s_v[i];
+  for (uint32_t b = t; b < nb; b += GS_THREADS) {
+    P.imp[D.b0 + b] = s_imp[b];
+    P.ang[D.b0 + b] = s_ang[b];
+  }
   if (t == 0) {
     S.ke_drag = k1;
     S.ke_buoy = k2;
@@ -1900,8 +1984,8 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_groups(FlPtrs P, const uin
   const GView g = gview(P.grids + 5 * (size_t)tl.x + 3, P.gstart[3] + D.c0[3], P.gidx[3] + D.f0);
   const float2* x = P.x + D.f0;
   const float* ff = P.ff + D.f0;
-  Items it;
-  items_clear(it);
+  Items<FL_SLOTS> it;
+  items_clear<FL_SLOTS>(it);
   uint32_t nit = 0;
   bool over = false;
   for (uint32_t k = 0; k < D.cpb; k++) {
@@ -3080,7 +3164,7 @@ int mg_fluid_step(mg_fluid* h, const mg_fluid_drum* drums, float* d_x, float* d_
     uint32_t maxn = 0;
     for (uint32_t d = 0; d < nd; d++) maxn = std::max(maxn, hd[d].n);
     {  // the sweeps' levels: a 16-bit last level per particle in LDS (up to 160 KiB)
-      const uint32_t cap = (uint32_t)std::min<size_t>((160u * 1024u) / 2, (size_t)std::max(maxn, 1u));
+      const uint32_t cap = (uint32_t)std::min<size_t>((160u * 1024u) / 2, (size_t)maxn + 2);
       const size_t bytes = (size_t)cap * 2;
       if (bytes > 65536)
         (void)hipFuncSetAttribute((const void*)fl_gs_level, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
@@ -3088,12 +3172,15 @@ int mg_fluid_step(mg_fluid* h, const mg_fluid_drum* drums, float* d_x, float* d_
     }
     // the drum's fluid velocities stay in LDS during the sweep when they fit (160 KiB per workgroup
     // on gfx950, less the kernel's static LDS)
-    const size_t static_lds = 4u * (GS_WAVES * 3u * GS_ITEMS + 3u * GS_MAXB + 1u) + 8u * GS_WAVES;
-    const size_t cap_bytes = 160u * 1024u - static_lds - 1024u;
+    // (after the balls' impulse copies, 12 bytes a ball)
+    uint32_t bcap = 1;
+    for (uint32_t d = 0; d < nd; d++) bcap = std::max(bcap, hd[d].nb);
+    const size_t static_lds = 4u * (GS_WAVES * 3u * 64u + 3u * GS_MAXB + 1u) + 8u * GS_WAVES + 8u + 256u;
+    const size_t cap_bytes = 160u * 1024u - static_lds - 1024u - 12u * (size_t)bcap;
     const uint32_t lds_cap = (uint32_t)std::min<size_t>(cap_bytes / 8, (size_t)maxn);
-    const size_t bytes = (size_t)lds_cap * 8;
+    const size_t bytes = (size_t)lds_cap * 8 + 12u * (size_t)bcap;
     if (bytes > 65536) (void)hipFuncSetAttribute((const void*)fl_gs, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    MG_TIMED_LAUNCH_S(h->tm, sg, fl_gs, G, dim3(GS_THREADS), bytes, P, lds_cap);
+    MG_TIMED_LAUNCH_S(h->tm, sg, fl_gs, G, dim3(GS_THREADS), bytes, P, lds_cap, bcap);
     if (nft) {
       MG_TIMED_LAUNCH_S(h->tm, sw, fl_dens2, dim3(nft), T, 0, P);
       MG_TIMED_LAUNCH_S(h->tm, sw, fl_mu, dim3(nft), T, 0, P);
