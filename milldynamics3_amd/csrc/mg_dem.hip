@@ -609,8 +609,11 @@ __device__ uint32_t de_levels(ItemFn item, uint32_t m, uint32_t n, int32_t* s_la
 // any of its turns. If any left-out pair fails that test, the projection starts again from the
 // start positions with the pairs that the first pass's travel (doubled) could have brought within
 // the reach, checked the same way, and failing that with every candidate pair. A schedule deeper
-// than DE_LCAP levels falls back to lane 0 walking all pairs in order.
-#define DE_GAP 0.05f
+// than DE_LCAP levels falls back to lane 0 walking all pairs in order. The gap is kept per drum
+// slot between substeps, from the travel seen (any gap gives the same results; it trades near
+// pairs against checks and restarts).
+#define DE_GAP 0.05f      // the smallest gap (in r) of the near pairs
+#define DE_GAP_MAX 0.8f   // the largest
 #define DE_PCHUNK 512  // pairs staged in LDS per chunk of the projection
 __device__ __forceinline__ void de_project_pair(float2* s_x, float* s_d, uint32_t pk, float& lam, float w, float ws,
                                                 float cap_c, float two_r) {
@@ -645,8 +648,8 @@ dem_project(const DeDrum* __restrict__ drums, const float2* __restrict__ lift, f
             const float2* __restrict__ x0, const float* __restrict__ th0, const uint2* __restrict__ pairs,
             float* __restrict__ lam, float* __restrict__ lam_bw, uint32_t* __restrict__ lev_g,
             uint32_t* __restrict__ pk_g, uint32_t* __restrict__ k_g, float* __restrict__ lam_s,
-            uint32_t* __restrict__ cpk_g, uint32_t* __restrict__ ck_g, float* __restrict__ slack_g, uint32_t stride,
-            uint32_t pcap, DeOut* __restrict__ out) {
+            uint32_t* __restrict__ cpk_g, uint32_t* __restrict__ ck_g, float* __restrict__ slack_g,
+            float* __restrict__ gap_g, uint32_t stride, uint32_t pcap, DeOut* __restrict__ out) {
   __shared__ float2 s_x[DE_MAX_BALLS];
   __shared__ float s_lbw[DE_MAX_BALLS];
   __shared__ int32_t s_last[DE_MAX_BALLS];  // the schedule's scratch, then each ball's travel
@@ -654,7 +657,7 @@ dem_project(const DeDrum* __restrict__ drums, const float2* __restrict__ lift, f
   __shared__ uint32_t s_off[DE_LCAP + 1];
   __shared__ uint32_t s_pk[DE_PCHUNK];
   __shared__ float s_l[DE_PCHUNK];
-  __shared__ uint32_t s_w[DE_BLOCK / 64], s_depth, s_fail;
+  __shared__ uint32_t s_w[DE_BLOCK / 64], s_depth, s_fail, s_maxd;
   const DeDrum D = drums[blockIdx.x];
   const mgg::Geo g = de_geo(D, lift);
   const uint32_t t = threadIdx.x, n = D.n;
@@ -663,7 +666,10 @@ dem_project(const DeDrum* __restrict__ drums, const float2* __restrict__ lift, f
   const uint2* P = pairs + pb;
   const float2* X = x + D.b0;  // the start positions (written back only at the end)
   const float w = D.w, ws = D.w + D.w, cap_c = -0.2f * 2.0f * D.r, two_r = 2.0f * D.r;
-  const float reach = two_r + DE_GAP * D.r, near2 = reach * reach, sure = two_r * 1.0001f;
+  const float gap = fminf(fmaxf(gap_g[blockIdx.x], DE_GAP), DE_GAP_MAX);
+  const float reach = two_r + gap * D.r, near2 = reach * reach, sure = two_r * 1.0001f;
+  bool restarted = false;
+  float maxd = 0.0f;  // the largest travel of the last checked pass
   float* s_d = reinterpret_cast<float*>(s_last);
   uint32_t* CPK = cpk_g + pb;  // the walked pairs (i | j << 16) in emission order
   uint32_t* CK = ck_g + pb;    // their emission indices
@@ -773,9 +779,23 @@ dem_project(const DeDrum* __restrict__ drums, const float2* __restrict__ lift, f
       __syncthreads();
     }
     if (!track) break;  // every candidate pair was walked
-    // the left-out pairs: start distance less both balls' travel above 2r, with margins
-    if (t == 0) s_fail = 0;
+    // the left-out pairs: start distance less both balls' travel above 2r, with margins. A left-out
+    // pair starts at least the gap less the margin beyond 2r, so when twice the largest travel
+    // (with the same margin factor) stays below that, none can fail and the pairs are not read.
+    if (t == 0) {
+      s_fail = 0;
+      s_maxd = 0u;
+    }
     __syncthreads();
+    {
+      float md = 0.0f;
+      for (uint32_t i = t; i < n; i += DE_BLOCK) md = fmaxf(md, s_d[i]);
+      if (!(md >= 0.0f && md <= 3.0e38f)) md = __int_as_float(0x7f800000);  // a non-finite travel: check
+      atomicMax(&s_maxd, __float_as_uint(md));  // non-negative floats order as their bits
+    }
+    __syncthreads();
+    maxd = __uint_as_float(s_maxd);
+    if (2.01f * maxd < (gap - 0.001f) * D.r) break;
     uint32_t bad = 0;
     for (uint32_t k0 = 0; k0 < np; k0 += 8 * DE_BLOCK) {  // eight pairs per lane in flight
       float sl[8];
@@ -796,6 +816,7 @@ dem_project(const DeDrum* __restrict__ drums, const float2* __restrict__ lift, f
     // restart from the start positions: with the pairs the travel could have brought near, then
     // with every pair
     if (t == 0) out[blockIdx.x].restart = pass + 1;
+    restarted = true;
     if (pass == 0)
       for (uint32_t i = t; i < n; i += DE_BLOCK) s_d0[i] = s_d[i];
     for (uint32_t i = t; i < n; i += DE_BLOCK) s_x[i] = X[i];
@@ -814,6 +835,12 @@ dem_project(const DeDrum* __restrict__ drums, const float2* __restrict__ lift, f
     v[D.b0 + i] = make_float2((p.x - p0.x) / D.dt, (p.y - p0.y) / D.dt);
     om[D.b0 + i] = de_angle_diff(th[D.b0 + i], th0[sb + i]) / D.dt;
     lam_bw[sb + i] = s_lbw[i];
+  }
+  if (t == 0) {
+    // the next substep's gap: three times this one's largest travel (so that its check is
+    // usually skipped), shrinking by at most 10% a substep; doubled after a restart
+    const float next = restarted ? 2.0f * gap : fmaxf(3.0f * maxd / D.r, 0.9f * gap);
+    gap_g[blockIdx.x] = fmaxf(fminf(next, DE_GAP_MAX), DE_GAP);
   }
 }
 
@@ -1254,6 +1281,7 @@ struct mg_dem {
   uint32_t *lev = nullptr, *lpk = nullptr, *lk = nullptr;  // a level schedule (projection, then contacts)
   uint32_t *spk = nullptr, *coff = nullptr;  // active contacts in book order; contact level starts
   float *slam = nullptr, *clam = nullptr;
+  float* gap = nullptr;  // the projection's near-pair gap per drum slot (a schedule choice, not state)
   uint32_t* flagged = nullptr;
   uint2* pairs = nullptr;
   DeItem* sorted = nullptr;
@@ -1360,6 +1388,7 @@ int mg_dem_create(uint32_t n_drums, uint32_t max_balls, uint32_t pairs_per_ball,
   const size_t o_spk = take(np * 4), o_slam = take(np * 4), o_clam = take(np * 4);
   const size_t o_coff = take((size_t)n_drums * (DE_LCAP + 1) * 4);
   const size_t o_rec = take((size_t)n_drums * h->rcap * sizeof(mg_impact_record));
+  const size_t o_gap = take((size_t)n_drums * 4);
   if (hipMalloc(&h->dev, off) != hipSuccess) {
     delete h;
     return de_fail(nullptr, err, errlen, MG_E_DEVICE, "hipMalloc failed for the DEM workspace");
@@ -1386,9 +1415,12 @@ int mg_dem_create(uint32_t n_drums, uint32_t max_balls, uint32_t pairs_per_ball,
   h->coff = reinterpret_cast<uint32_t*>(h->dev + o_coff);
   h->sorted = reinterpret_cast<DeItem*>(h->dev + o_sorted);
   h->rec = reinterpret_cast<mg_impact_record*>(h->dev + o_rec);
+  h->gap = reinterpret_cast<float*>(h->dev + o_gap);
   float thr[11];
   ebin_thresholds(thr);
-  if (hipMemcpy(h->ebin, thr, sizeof thr, hipMemcpyHostToDevice) != hipSuccess) {
+  const std::vector<float> gap0(n_drums, DE_GAP);
+  if (hipMemcpy(h->ebin, thr, sizeof thr, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->gap, gap0.data(), gap0.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
     (void)hipFree(h->dev);
     delete h;
     return de_fail(nullptr, err, errlen, MG_E_DEVICE, "HIP copy error");
@@ -1537,7 +1569,7 @@ int mg_dem_step(mg_dem* h, const mg_dem_drum* drums, float* d_x, float* d_v, flo
   }
   MG_TIMED_LAUNCH(h->tm, dem_project, G, B, 0, h->drums, h->lift, x, v, d_theta, d_omega, h->x0, h->th0, h->pairs,
                      h->lam, h->lam_bw, h->lev, h->lpk, h->lk, h->lam_s, h->spk, reinterpret_cast<uint32_t*>(h->clam),
-                     h->slam, S, P, h->out);
+                     h->slam, h->gap, S, P, h->out);
   MG_TIMED_LAUNCH(h->tm, dem_sort, G, B, 0, h->drums, h->pairs, h->lam, h->sorted, h->spk, h->slam, h->lev, h->lpk,
                      h->lk, h->clam, h->coff, P, h->out);
   MG_TIMED_LAUNCH(h->tm, dem_friction, G, B, 0, h->drums, h->lift, x, v, d_theta, d_omega, h->x0, h->th0, h->spk,
