@@ -240,6 +240,36 @@ __device__ inline void near_cells(const GView& g, float2 p, Near& q) {
     }
   }
 }
+// near_cells for a point that is the same on every lane of the wave (every lane active): lane r < 9
+// finds cell r's range, all nine in one round of loads, and every lane reads the nine.
+__device__ inline void near_cells_wave(const GView& g, float2 p, Near& q) {
+  const uint32_t lane = threadIdx.x & 63u;
+  q.total = 0;
+#pragma unroll
+  for (int r = 0; r < 9; r++) q.s[r] = q.e[r] = 0;
+  if (!fin2(p) || g.W == 0) return;
+  int32_t kx, ky;
+  gkey(p, g.cell, kx, ky);
+  uint32_t s = 0, e = 0;
+  if (lane < 9) {
+    const int32_t r = (int32_t)lane;
+    const int32_t ax = sadd(kx, r % 3 - 1), ay = sadd(ky, r / 3 - 1);
+    bool dup = false;
+#pragma unroll
+    for (int32_t u = 0; u < 8; u++) dup |= u < r && sadd(kx, u % 3 - 1) == ax && sadd(ky, u / 3 - 1) == ay;
+    uint32_t cs = 0, ce = 0;
+    if (!dup && grange(g, ax, ay, cs, ce)) {
+      s = cs;
+      e = ce;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 9; r++) {
+    q.s[r] = rdlu(s, r);
+    q.e[r] = rdlu(e, r);
+    q.total += q.e[r] - q.s[r];
+  }
+}
 // Candidate c (0-based, c < q.total) of a near query: its point index.
 __device__ inline uint32_t near_at(const GView& g, const Near& q, uint32_t c) {
   uint32_t j = FL_NO;
@@ -972,7 +1002,7 @@ __device__ __forceinline__ bool adh_ball(const FlPtrs& P, const FlDrum& D, uint2
     const float2 o = P.offs[D.off0 + k];
     const float2 bp = make_float2(bxb.x + o.x, bxb.y + o.y);
     Near q;
-    near_cells(g, bp, q);
+    near_cells_wave(g, bp, q);
     for (uint32_t c0 = 0; c0 < q.total; c0 += 64) {
       const uint32_t c = c0 + lane;
       bool valid = false;
@@ -1270,7 +1300,7 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_gs_pre(FlPtrs P, const uin
   if (D.drag_on && fin2(c)) {
     const GView g = gview(P.grids + 5 * (size_t)tl.x + 2, P.gstart[2] + D.c0[2], P.gidx[2] + D.f0);
     Near q;
-    near_cells(g, c, q);
+    near_cells_wave(g, c, q);
     for (uint32_t c0 = 0; c0 < q.total; c0 += 64) {
       const uint32_t cc = c0 + lane;
       bool valid = false;
@@ -1341,7 +1371,7 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_gs_pre(FlPtrs P, const uin
   if (D.buoy_on && fin2(c)) {
     const GView g = gview(P.grids + 5 * (size_t)tl.x, P.gstart[0] + D.c0[0], P.gidx[0] + D.f0);
     Near q;
-    near_cells(g, c, q);
+    near_cells_wave(g, c, q);
     for (uint32_t c0 = 0; c0 < q.total; c0 += 64) {
       const uint32_t cc = c0 + lane;
       uint32_t j = 0;
@@ -1994,7 +2024,7 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_groups(FlPtrs P, const uin
     const float lx = bp.x - bxb.x, ly = bp.y - bxb.y;
     const float ll = lx * lx + ly * ly;
     Near q;
-    near_cells(g, bp, q);
+    near_cells_wave(g, bp, q);
     for (uint32_t c0 = 0; c0 < q.total; c0 += 64) {
       const uint32_t c = c0 + lane;
       bool valid = false;
@@ -2075,7 +2105,7 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_links(FlPtrs P, const uint
   float4* out = P.lrec + D.l0 + (size_t)i * D.lcap;
   uint32_t cnt = 0, inr = 0;
   Near q;
-  near_cells(g, xi, q);
+  near_cells_wave(g, xi, q);
   for (uint32_t c0 = 0; c0 < q.total; c0 += 64) {
     const uint32_t c = c0 + lane;
     bool in = false, emit = false;
