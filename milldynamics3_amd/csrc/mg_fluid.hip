@@ -211,6 +211,16 @@ __device__ inline bool grange(const GView& g, int32_t cx, int32_t cy, uint32_t& 
   e = g.start[k + 1];
   return s != e;
 }
+// grange with its loads issued whatever the cell (an absent cell reads cell 0 and gives the empty
+// range), so that the loads of several cells are in flight together.
+__device__ inline void grange_any(const GView& g, int32_t cx, int32_t cy, uint32_t& s, uint32_t& e) {
+  const int64_t dx = (int64_t)cx - g.mnx, dy = (int64_t)cy - g.mny;
+  const bool in = dx >= 0 && dy >= 0 && dx < (int64_t)g.W && dy < (int64_t)g.H;
+  const uint32_t k = in ? (uint32_t)(dx * (int64_t)g.H + dy) : 0u;
+  const uint32_t a = g.start[k], b = g.start[k + 1];
+  s = in ? a : 0u;
+  e = in ? b : 0u;
+}
 // The cells for_each_near visits around p, in its order (rows dy = -1..1, dx = -1..1, repeats
 // skipped): up to 9 ranges.
 struct Near {
@@ -233,11 +243,10 @@ __device__ inline void near_cells(const GView& g, float2 p, Near& q) {
 #pragma unroll
     for (int u = 0; u < r; u++) dup |= ax[u] == ax[r] && ay[u] == ay[r];
     uint32_t s = 0, e = 0;
-    if (!dup && grange(g, ax[r], ay[r], s, e)) {
-      q.s[r] = s;
-      q.e[r] = e;
-      q.total += e - s;
-    }
+    grange_any(g, ax[r], ay[r], s, e);
+    q.s[r] = dup ? 0u : s;
+    q.e[r] = dup ? 0u : e;
+    q.total += q.e[r] - q.s[r];
   }
 }
 // near_cells for a point that is the same on every lane of the wave (every lane active): lane r < 9
@@ -731,10 +740,12 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_nbr(FlPtrs P, int fill) {
     gkey(p, g.cell, kx, ky);
     const int OX[9] = {-1, -1, 0, 0, 1, 1, 0, -1, 1};
     const int OY[9] = {-1, 0, -1, 0, 0, 1, 1, 1, -1};
+    uint32_t cs[9], ce[9];  // the nine cells' ranges, their loads in flight together
+#pragma unroll
+    for (int q = 0; q < 9; q++) grange_any(g, sadd(kx, OX[q]), sadd(ky, OY[q]), cs[q], ce[q]);
 #pragma unroll
     for (int q = 0; q < 9; q++) {
-      uint32_t s, e;
-      if (!grange(g, sadd(kx, OX[q]), sadd(ky, OY[q]), s, e)) continue;
+      const uint32_t s = cs[q], e = ce[q];
       for (uint32_t a = s; a < e; a++) {
         const uint32_t j = g.idx[a];
         if (j == i) continue;
