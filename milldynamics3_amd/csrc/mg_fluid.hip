@@ -1260,22 +1260,23 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_vel(FlPtrs P) {
 }
 
 // Grids over the final positions (form drag's at h + r, the boundary groups' at h) and over the
-// balls at the squeeze-link reach (pbf.rs:1790, :2176, :557).
+// balls at the squeeze-link reach (pbf.rs:1790, :2176, :557); workgroup (d, k) builds drum d's
+// grid 2 + k (the three are independent).
 extern "C" __global__ void __launch_bounds__(FL_T) fl_grid_b(FlPtrs P) {
   __shared__ int32_t si[4 * FL_T];
   __shared__ uint32_t su[FL_T];
-  const uint32_t d = blockIdx.x;
+  const uint32_t d = blockIdx.x, k = blockIdx.y;
   const FlDrum& D = P.drums[d];
   if (D.n == 0) return;
   FlGrid* G = P.grids + 5 * (size_t)d;
   FlState& S = P.st[d];
-  if (D.drag_on)
+  if (k == 0 && D.drag_on)
     wg_grid_build(P.x + D.f0, D.n, fmaxf(D.hc, 1e-6f), D.ccap[2], P.gstart[2] + D.c0[2], P.gcur[2] + D.c0[2],
                   P.gidx[2] + D.f0, G + 2, &S.err, si, su);
-  if (D.grp_on)
+  if (k == 1 && D.grp_on)
     wg_grid_build(P.x + D.f0, D.n, D.h, D.ccap[3], P.gstart[3] + D.c0[3], P.gcur[3] + D.c0[3], P.gidx[3] + D.f0,
                   G + 3, &S.err, si, su);
-  if (D.links_on || D.drag_on || D.buoy_on)
+  if (k == 2 && (D.links_on || D.drag_on || D.buoy_on))
     wg_grid_build(P.bx + D.b0, D.nb, D.g5c, D.ccap[4], P.gstart[4] + D.c0[4], P.gcur[4] + D.c0[4], P.gidx[4] + D.b0,
                   G + 4, &S.err, si, su);
 }
@@ -3175,7 +3176,7 @@ int mg_fluid_step(mg_fluid* h, const mg_fluid_drum* drums, float* d_x, float* d_
     MG_TIMED_LAUNCH(h->tm, fl_wall, dim3(nft), T, 0, P);
     MG_TIMED_LAUNCH(h->tm, fl_vel, dim3(nft), T, 0, P);
   }
-  MG_TIMED_LAUNCH(h->tm, fl_grid_b, G, T, 0, P);
+  MG_TIMED_LAUNCH(h->tm, fl_grid_b, dim3(G.x, 3), T, 0, P);
   if (nwt) MG_TIMED_LAUNCH(h->tm, fl_gs_pre, dim3(nwt), T, 0, P, (const uint2*)h->d_wt);
   {
     // The sweeps' schedule and the sweeps (a few waves per drum) leave most of the GPU idle. The
