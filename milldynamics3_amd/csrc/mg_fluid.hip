@@ -3195,6 +3195,16 @@ int mg_fluid_step(mg_fluid* h, const mg_fluid_drum* drums, float* d_x, float* d_
       ok2 &= hipEventCreateWithFlags(&h->fork, hipEventDisableTiming) == hipSuccess;
       if (!ok2) return fl_fail(h, err, errlen, MG_E_DEVICE, "HIP error creating the side streams");
     }
+    uint32_t maxn = 0;
+    for (uint32_t d = 0; d < nd; d++) maxn = std::max(maxn, hd[d].n);
+    {  // the sweeps' levels (before the fork: one wave per drum, slowed most by the side stream's
+       // kernels): a 16-bit last level per particle in LDS (up to 160 KiB)
+      const uint32_t cap = (uint32_t)std::min<size_t>((160u * 1024u) / 2, (size_t)maxn + 2);
+      const size_t bytes = (size_t)cap * 2;
+      if (bytes > 65536)
+        (void)hipFuncSetAttribute((const void*)fl_gs_level, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+      MG_TIMED_LAUNCH(h->tm, fl_gs_level, G, dim3(64), bytes, P, cap);
+    }
     hipStream_t sg = nullptr, sw = nullptr;
     if (overlap) {
       sg = h->side[0];
@@ -3202,15 +3212,6 @@ int mg_fluid_step(mg_fluid* h, const mg_fluid_drum* drums, float* d_x, float* d_
       (void)hipEventRecord(h->fork, nullptr);
       (void)hipStreamWaitEvent(sg, h->fork, 0);
       (void)hipStreamWaitEvent(sw, h->fork, 0);
-    }
-    uint32_t maxn = 0;
-    for (uint32_t d = 0; d < nd; d++) maxn = std::max(maxn, hd[d].n);
-    {  // the sweeps' levels: a 16-bit last level per particle in LDS (up to 160 KiB)
-      const uint32_t cap = (uint32_t)std::min<size_t>((160u * 1024u) / 2, (size_t)maxn + 2);
-      const size_t bytes = (size_t)cap * 2;
-      if (bytes > 65536)
-        (void)hipFuncSetAttribute((const void*)fl_gs_level, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-      MG_TIMED_LAUNCH_S(h->tm, sg, fl_gs_level, G, dim3(64), bytes, P, cap);
     }
     // the drum's fluid velocities stay in LDS during the sweep when they fit (160 KiB per workgroup
     // on gfx950, less the kernel's static LDS)
