@@ -1304,6 +1304,7 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_gs_pre(FlPtrs P, const uin
   const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6, b = tl.y + wv;
   if (b >= D.nb) return;
   const size_t gb = (size_t)D.b0 + b;
+  if (lane == 0) P.locnt[gb] = 0;  // squeeze links naming the ball as partner, counted by fl_links
   const float2 c = P.bx[gb];
   const float2* x = P.x + D.f0;
   float *sa = s_a[wv], *sb = s_b[wv], *sc = s_c[wv], *sd = s_d[wv];
@@ -2016,7 +2017,6 @@ extern "C" __global__ void __launch_bounds__(FL_T) fl_groups(FlPtrs P, const uin
   const FlDrum& D = P.drums[tl.x];
   const uint32_t lane = threadIdx.x & 63u, b = tl.y + (threadIdx.x >> 6);
   if (b >= D.nb) return;
-  if (lane == 0) P.locnt[D.b0 + b] = 0;  // squeeze links naming the ball as partner, counted by fl_links
   uint32_t* gcnt = P.gcnt + D.b0;
   const float2 bxb = P.bx[D.b0 + b];
   if (!D.grp_on || D.n == 0 || !fin2(bxb)) {
@@ -3179,9 +3179,9 @@ int mg_fluid_step(mg_fluid* h, const mg_fluid_drum* drums, float* d_x, float* d_
   MG_TIMED_LAUNCH(h->tm, fl_grid_b, dim3(G.x, 3), T, 0, P);
   if (nwt) MG_TIMED_LAUNCH(h->tm, fl_gs_pre, dim3(nwt), T, 0, P, (const uint2*)h->d_wt);
   {
-    // The sweeps' schedule and the sweeps (a few waves per drum) leave most of the GPU idle. The
-    // density at the final positions, the Morris weights and the boundary groups read neither the
-    // velocities nor the impulses the sweeps write when no drum is Bingham (a Bingham drum's
+    // The sweeps (a few waves per drum) leave most of the GPU idle. The density at the final
+    // positions, the Morris weights and the boundary groups read neither the velocities nor the
+    // impulses the sweeps and the squeeze links write when no drum is Bingham (a Bingham drum's
     // viscosity reads the swept velocities), so they then run beside them on a second stream.
     bool bingham = false;
     for (uint32_t d = 0; d < nd; d++) bingham |= hd[d].bingham != 0 && hd[d].n > 0 && hd[d].mu > 0.0f;
@@ -3224,6 +3224,9 @@ int mg_fluid_step(mg_fluid* h, const mg_fluid_drum* drums, float* d_x, float* d_
     const size_t bytes = (size_t)lds_cap * 8 + 12u * (size_t)bcap;
     if (bytes > 65536) (void)hipFuncSetAttribute((const void*)fl_gs, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     MG_TIMED_LAUNCH_S(h->tm, sg, fl_gs, G, dim3(GS_THREADS), bytes, P, lds_cap, bcap);
+    // the squeeze links read the swept impulses and the ball grid only (their partner counts are
+    // cleared by fl_gs_pre), so they follow the sweeps on the sweeps' stream
+    if (nwt) MG_TIMED_LAUNCH_S(h->tm, sg, fl_links, dim3(nwt), T, 0, P, (const uint2*)h->d_wt);
     if (nft) {
       MG_TIMED_LAUNCH_S(h->tm, sw, fl_dens2, dim3(nft), T, 0, P);
       MG_TIMED_LAUNCH_S(h->tm, sw, fl_mu, dim3(nft), T, 0, P);
@@ -3237,7 +3240,6 @@ int mg_fluid_step(mg_fluid* h, const mg_fluid_drum* drums, float* d_x, float* d_
       (void)hipStreamWaitEvent(nullptr, h->join[1], 0);
     }
   }
-  if (nwt) MG_TIMED_LAUNCH(h->tm, fl_links, dim3(nwt), T, 0, P, (const uint2*)h->d_wt);
   MG_TIMED_LAUNCH(h->tm, fl_cg_prep, G, T, 0, P);
   if (nwt && nrt) {
     MG_TIMED_LAUNCH(h->tm, fl_cg_scatter, dim3(nwt), T, 0, P, (const uint2*)h->d_wt);
